@@ -1,0 +1,321 @@
+/*
+ * yafgpu.h -- C ABI of libyafgpu.so, the MI355X (gfx950) ray-intersection and
+ * global-illumination engine that drops in for YafRay 0.0.9's per-ray CPU calls.
+ *
+ * Conventions (mirroring src/interface/interface_impl.cc:141-148 and
+ * src/yafraycore/scene.cc:69-78 of the reference):
+ *   - every entry point returns YG_OK (0) or a negative status; no exceptions
+ *     cross this boundary and nothing calls exit();
+ *   - yg_last_error() returns the text of the last failure on the calling thread;
+ *   - the caller owns host buffers, a scene handle owns its device buffers;
+ *   - buffers named d_* are device (HBM) pointers, all device work is ordered
+ *     on the HIP stream passed in (NULL = the null stream);
+ *   - a handle must not be used from two threads at once without external locking.
+ *
+ * Arithmetic is FP32 like the reference (PFLOAT = float, configio.py:135-140)
+ * except where the reference itself computes in double (Halton state,
+ * mcqmc.h:35-49; sample-direction trigonometry, pathtools.cc:32-43).
+ */
+#ifndef YAFGPU_H
+#define YAFGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YG_OK 0
+#define YG_ERR_ARG (-1)
+#define YG_ERR_HIP (-2)
+#define YG_ERR_STATE (-3)
+#define YG_ERR_UNSUPPORTED (-4)
+#define YG_ERR_NOMEM (-5)
+
+#define YG_ABI_VERSION 1
+
+/* ---------------------------------------------------------------- rays -- */
+
+/* One traversal query, 32 B.  Closest-hit accepts tmin <= t < tmax
+ * (kdTree_t::Intersect, kdtree.cc:690-694 with tmin = 0, tmax = dist);
+ * any-hit accepts tmin < t < tmax (kdTree_t::IntersectS, kdtree.cc:835-841). */
+typedef struct yg_ray {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+} yg_ray;
+
+/* Closest-hit result, 16 B.  prim = index of the triangle in mesh-add order
+ * (-1 on miss); t = plane distance (triangle_t::intersect, triangle.h:83-86);
+ * u, v = barycentric weights of vertices b and c from the dominant-axis
+ * projection of triangle.cc:74-136 (fb, fc). */
+typedef struct yg_hit {
+    int32_t prim;
+    float t;
+    float u;
+    float v;
+} yg_hit;
+
+/* ---------------------------------------------------------- descriptors -- */
+
+/* Perspective pinhole camera; camera_t::camera_t (camera.cc:27-87). */
+typedef struct yg_camera {
+    float pos[3];
+    float look[3];
+    float up[3];     /* a point, like the reference ("up" - pos is the up vector) */
+    int32_t resx, resy;
+    float aspect;    /* render param "aspect_ratio" (1 by default) */
+    float focal;     /* render param "focal" (df) */
+} yg_camera;
+
+enum {
+    YG_NOISE_NEWPERLIN = 0, /* newPerlin_t, noise.cc:161-183 */
+    YG_NOISE_STDPERLIN = 1, /* stdPerlin_t (not implemented: YG_ERR_UNSUPPORTED) */
+    YG_NOISE_BLENDER = 2    /* blenderNoise_t (not implemented: YG_ERR_UNSUPPORTED) */
+};
+
+enum {
+    YG_TEX_CLOUDS = 1,   /* textureClouds_t, basictex.cc:70-115 */
+    YG_TEX_MARBLE = 2,   /* textureMarble_t, basictex.cc:121-177 */
+    YG_TEX_WOOD = 3,     /* textureWood_t, basictex.cc:183-230 */
+    YG_TEX_MUSGRAVE = 4  /* textureMusgrave_t with fBm_t, basictex.cc:657-709 */
+};
+
+enum { YG_SHAPE_SIN = 0, YG_SHAPE_SAW = 1, YG_SHAPE_TRI = 2 };
+
+/* Procedural texture; fields not used by a type are ignored. */
+typedef struct yg_texture {
+    int32_t type;
+    int32_t noise;       /* YG_NOISE_* */
+    float color1[3];
+    float color2[3];
+    int32_t depth;       /* clouds "depth", marble/wood "depth" (octaves) */
+    float size;          /* turbulence "size" (clouds/marble/wood), Musgrave "size" */
+    int32_t hard;
+    int32_t bias;        /* clouds: 0 none, 1 positive, 2 negative */
+    float turb;          /* marble/wood "turbulence" */
+    float sharpness;     /* marble: 1/shp if shp > 1 else 1 (basictex.cc:125-126) */
+    int32_t shape;       /* YG_SHAPE_* */
+    int32_t rings;       /* wood "type" == rings */
+    float H, lacunarity, octaves, iscale; /* Musgrave fBm parameters */
+} yg_texture;
+
+enum { YG_MOD_MIX = 0, YG_MOD_MUL = 1, YG_MOD_ADD = 2, YG_MOD_SUB = 3 };
+
+/* modulator_t on a procedural texture with global coordinates and
+ * projection x,y,z: texpt = size * (P + ofs) (texture.cc:311-317), colour
+ * blend by mode and factor "color" (texture.cc:499-534). */
+typedef struct yg_modulator {
+    int32_t tex;         /* index into the texture table */
+    int32_t mode;        /* YG_MOD_* */
+    float color;         /* blend factor ("color" parameter) */
+    float size[3];       /* sizex, sizey, sizez (each times "size") */
+    float ofs[3];
+} yg_modulator;
+
+/* genericShader_t in its diffuse form (basicshaders.cc:30-70,191-204):
+ * scolor = albedo, edif = 1, ks = 0, no reflection/refraction. */
+typedef struct yg_material {
+    float color[3];
+    int32_t mod_first;
+    int32_t mod_count;
+} yg_material;
+
+enum {
+    YG_LIGHT_SUN = 1,    /* sunLight_t, sunlight.cc:27-57 */
+    YG_LIGHT_AREA = 2,   /* areaLight_t, arealight.cc:31-174 */
+    YG_LIGHT_PATH = 3,   /* pathLight_t full-GI, non-cached (pathlight.cc:107-403) */
+    YG_LIGHT_PHOTON = 4  /* photonLight_t, diffuse mode (photonlight.cc:33-501) */
+};
+
+/* One light; lights are summed in the order given, which must be the
+ * reference's order (alphabetical by light name, interface_impl.cc:571-576). */
+typedef struct yg_light {
+    int32_t type;
+    int32_t use_in_render;   /* light_t::useInRender */
+    int32_t use_in_indirect; /* light_t::useInIndirect (pathlight sets false) */
+    float color[3];
+    float power;
+    /* sun */
+    float dir[3];            /* "from", interpreted as a direction */
+    int32_t cast_shadows;
+    /* area */
+    float a[3], b[3], c[3], d[3];
+    int32_t samples;         /* area "samples", path "samples" */
+    int32_t psamples;        /* area "psamples" (only 0 supported) */
+    /* path */
+    int32_t depth;           /* path "depth" / photon "depth" */
+    int32_t use_qmc;         /* path/photon "use_QMC" */
+    float threshold;         /* path "threshold" */
+    /* photon */
+    float from[3], to[3];
+    float angle;
+    int32_t photons;
+    int32_t search;          /* K */
+    int32_t mindepth;
+    float bias;
+    float fixedradius;
+    float cluster;
+} yg_light;
+
+/* Render parameters (interface_impl.cc:480-644 / scene.cc:50-60). */
+typedef struct yg_render_opts {
+    float min_raydis;        /* MIN_RAYDIST, 5e-5 (configio.py:95-100) */
+    float self_bias;         /* render param "bias" */
+    int32_t raydepth;        /* maxraylevel */
+    float background[3];     /* constBackground_t colour */
+    int32_t has_background;
+    int32_t block;           /* render area edge, 64 (scene.cc:541) */
+    uint32_t rand_seed;      /* srand() seed before the block shuffle (glibc default 1) */
+    uint32_t lcg_seed;       /* ourRandom() state at render start (vector3d.cc:160: 123212) */
+    int32_t flags;           /* YG_RENDER_* */
+} yg_render_opts;
+
+#define YG_RENDER_TIMING 1   /* fill yg_render_stats.ms with device time */
+
+typedef struct yg_render_stats {
+    uint64_t rays_camera;
+    uint64_t rays_extension;
+    uint64_t rays_shadow;
+    uint64_t adaptive_extra;  /* paths whose pathlight asked for extra repetitions */
+    uint64_t lcg_draws;       /* ourRandom() draws consumed (area lights) */
+    float ms;
+    float ms_trace;
+} yg_render_stats;
+
+typedef struct yg_build_opts {
+    int32_t max_leaf;      /* triangles per leaf, 1..8 */
+    int32_t sah_bins;      /* 8..64 */
+    int32_t threads;       /* host build threads, 0 = all */
+} yg_build_opts;
+
+typedef struct yg_scene yg_scene;
+
+/* ------------------------------------------------------------ lifecycle -- */
+
+int yg_abi_version(void);
+int yg_last_error(char* buf, size_t len);
+int yg_device_count(int* n);
+
+/* interface_impl.cc:51 (interfaceImpl_t ctor) */
+int yg_scene_create(yg_scene** out);
+/* interface_impl.cc:58-91 (~interfaceImpl_t / clear) */
+int yg_scene_destroy(yg_scene* s);
+
+/* addObject_trimesh (interface_impl.cc:193-265): world-space float3 vertices,
+ * uint32 index triples, optional per-face material ids (NULL: all 0).
+ * Primitive ids continue across calls in add order; *obj_id receives the
+ * object index (objects are numbered in add order, which must be the
+ * reference's alphabetical object order, render.cc:1158-1164).
+ * flags: bit0 = cast shadows, bit1 = receive radiosity, bit2 = use for radiosity
+ * (object3d_t flags; 7 = reference defaults). */
+int yg_mesh_add(yg_scene* s, const float* xyz, uint32_t nv, const uint32_t* idx,
+                uint32_t nt, const int32_t* mat, int32_t flags, int32_t* obj_id);
+
+/* kdTree_t::kdTree_t (kdtree.cc:57-138) + buildObjectTree (object3d.cc:30-53):
+ * builds the SAH BVH over all meshes and uploads it. */
+int yg_build(yg_scene* s, const yg_build_opts* opts);
+
+/* Node / triangle counts and device bytes of the built tree. */
+int yg_scene_info(yg_scene* s, uint64_t* n_tris, uint64_t* n_nodes, uint64_t* dev_bytes,
+                  int32_t* max_depth);
+
+/* -------------------------------------------------------- ray queries -- */
+
+/* kdTree_t::Intersect over every object (kdtree.cc:591-727): closest hit. */
+int yg_trace_closest(yg_scene* s, const yg_ray* d_rays, uint32_t n, yg_hit* d_hits,
+                     void* stream);
+/* kdTree_t::IntersectS (kdtree.cc:730-866): any hit; d_occluded gets 0/1 per ray. */
+int yg_trace_any(yg_scene* s, const yg_ray* d_rays, uint32_t n, uint8_t* d_occluded,
+                 void* stream);
+
+/* scene_t::firstHit (scene.cc:688-721): origin from + ray*min_raydis, closest
+ * with Z > 0.  d_from, d_dir: n float3.  Writes prim/t/u/v per query. */
+int yg_first_hit(yg_scene* s, const float* d_from, const float* d_dir, uint32_t n,
+                 float min_raydis, yg_hit* d_hits, void* stream);
+
+/* scene_t::isShadowed (scene.cc:107-146 point form when to_point != 0,
+ * scene.cc:148-185 direction form otherwise): d_p = shading points,
+ * d_l = light point or direction, d_self_obj = object of the shading point
+ * (or -1).  Writes 0/1. */
+int yg_is_shadowed(yg_scene* s, const float* d_p, const float* d_l,
+                   const int32_t* d_self_obj, uint32_t n, int32_t to_point,
+                   float min_raydis, float self_bias, uint8_t* d_out, void* stream);
+
+/* ------------------------------------------------------------- noise -- */
+
+enum {
+    YG_EVAL_NOISE = 0,      /* newPerlin_t::operator() in [0,1] */
+    YG_EVAL_FBM = 1,        /* fBm_t::operator() (noise.cc:638-650) */
+    YG_EVAL_TURBULENCE = 2, /* turbulence() (noise.cc:808-819) */
+    YG_EVAL_TEX_FLOAT = 3,  /* texture getFloat */
+    YG_EVAL_TEX_COLOR = 4   /* texture getColor (3 floats per point) */
+};
+
+/* Evaluate noise/texture functions at n float3 points on the device.
+ * For NOISE/FBM/TURBULENCE the parameters come from tex (H, lacunarity,
+ * octaves for FBM; depth, size, hard for TURBULENCE). */
+int yg_noise_eval(int32_t what, const yg_texture* tex, const float* d_pts, uint32_t n,
+                  float* d_out, void* stream);
+
+/* Heightfield vertices on the device: nx*ny vertices over [0,ex]x[0,ey],
+ * z = amp * fBm(freq * (x, y, 0)) with fBm(H, lacunarity, octaves) over
+ * newPerlin_t.  d_xyz receives nx*ny float3, row-major in y then x. */
+int yg_gen_heightfield(int32_t nx, int32_t ny, float ex, float ey, float amp, float freq,
+                       float H, float lacunarity, float octaves, float* d_xyz,
+                       void* stream);
+
+/* ------------------------------------------------------- QMC / RNG -- */
+
+/* Halton radical inverse of index i (first getNext() of a fresh Halton(base)
+ * returns index 1; mcqmc.h:35-49) for n consecutive indices starting at i0. */
+int yg_halton(int32_t base, uint32_t i0, uint32_t n, float* d_out, void* stream);
+
+/* -------------------------------------------------------- rendering -- */
+
+int yg_set_camera(yg_scene* s, const yg_camera* cam);
+int yg_set_shading(yg_scene* s, const yg_material* mats, uint32_t nmat,
+                   const yg_modulator* mods, uint32_t nmod,
+                   const yg_texture* texs, uint32_t ntex);
+int yg_set_lights(yg_scene* s, const yg_light* lights, uint32_t n);
+
+/* scene_t::render(colorOutput_t&) in mono mode (scene.cc:538-600) with
+ * AA_passes = 0: d_rgb receives resx*resy*3 floats (row-major, top row first).
+ * Lights supported: sun, area (psamples 0), path (use_QMC on, cache off),
+ * photon (diffuse mode; yg_photon_build must have run). */
+int yg_render(yg_scene* s, const yg_render_opts* opts, float* d_rgb,
+              yg_render_stats* stats, void* stream);
+
+/* ----------------------------------------------------------- photons -- */
+
+typedef struct yg_photon_stats {
+    uint64_t emitted;
+    uint64_t stored;
+    uint64_t clustered;   /* photons kept after preGathering */
+    float ms;
+} yg_photon_stats;
+
+/* photonLight_t::init (photonlight.cc:453-501) for light index li of the
+ * light table: emission, tracing, cell clustering, pre-gathering and the
+ * gather index.  Results stay on the device for yg_render / yg_photon_gather. */
+int yg_photon_build(yg_scene* s, uint32_t li, const yg_render_opts* opts,
+                    yg_photon_stats* stats, void* stream);
+
+/* Copy the clustered photons (photonMark_t: dir, pos, colour = 9 floats each,
+ * in the reference's std::map cell order) to d_out (capacity in photons). */
+int yg_photon_read(yg_scene* s, float* d_out, uint64_t capacity, uint64_t* count,
+                   void* stream);
+
+/* photonLight_t::illuminate (photonlight.cc:297-395) at n shading points:
+ * d_p = position, d_n = shading normal (already FACE_FORWARDed), d_ng =
+ * geometric normal, d_eye = eye vector, d_mat = material id; d_rgb gets the
+ * radiance estimate. */
+int yg_photon_gather(yg_scene* s, const float* d_p, const float* d_n, const float* d_ng,
+                     const float* d_eye, const int32_t* d_mat, uint32_t n, float* d_rgb,
+                     void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* YAFGPU_H */

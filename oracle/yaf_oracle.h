@@ -1,0 +1,117 @@
+/*
+ * yaf_oracle.h -- CPU restatement of YafRay 0.0.9's ray-intersection / GI path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for libyafgpu.so:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * it.  The product path never calls it.
+ *
+ * Every function restates a reference function (file:line cited at the
+ * definition) in plain C99, FP32 where the reference is FP32 and double where
+ * the reference computes in double.  Built with -ffp-contract=off and without
+ * -ffast-math so each expression is evaluated exactly as written, left to right.
+ */
+#ifndef YAF_ORACLE_H
+#define YAF_ORACLE_H
+
+#include <stdint.h>
+#include "../include/yafgpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { float x, y, z; } yo_v3;
+
+/* ------------------------------------------------------------ triangle -- */
+void yo_tri_normal(const float* a, const float* b, const float* c, float* n);
+int yo_tri_hit(const float* a, const float* b, const float* c, const float* n,
+               const float* from, const float* ray);
+float yo_tri_intersect(const float* a, const float* n, const float* from, const float* ray);
+void yo_tri_bary(const float* a, const float* b, const float* c, const float* n,
+                 const float* h, float* fb, float* fc);
+
+/* --------------------------------------------------------------- scene -- */
+typedef struct yo_scene yo_scene;
+
+/* verts: nv*3, idx: nt*3, tri_obj/tri_mat: nt (may be NULL -> 0). */
+yo_scene* yo_scene_new(const float* verts, int64_t nv, const uint32_t* idx, int64_t nt,
+                       const int32_t* tri_obj, const int32_t* tri_mat);
+void yo_scene_free(yo_scene* s);
+/* canonical BVH2 node count (binned SAH, 32 bins, leaf <= 4, FP64 boxes) */
+int64_t yo_scene_nodes(yo_scene* s);
+
+/* brute force over every triangle; mode 0 closest (tmin <= t < tmax),
+ * 1 any (tmin < t < tmax).  rays: n*8 floats in yg_ray layout. */
+void yo_trace_brute(yo_scene* s, const float* rays, int64_t n, int mode,
+                    int32_t* prim, float* t, float* u, float* v);
+/* same results through the canonical BVH2; counts node visits / tri tests */
+void yo_trace_bvh(yo_scene* s, const float* rays, int64_t n, int mode, int32_t* prim,
+                  float* t, float* u, float* v, uint64_t* nodes, uint64_t* tris,
+                  int nthreads);
+
+/* scene_t::firstHit (closest, Z > 0, origin from+ray*min_raydis) */
+void yo_first_hit(yo_scene* s, const float* from, const float* dir, int64_t n,
+                  float min_raydis, int32_t* prim, float* t);
+/* scene_t::isShadowed, point form (to_point=1) or direction form */
+void yo_is_shadowed(yo_scene* s, const float* p, const float* l, const int32_t* self_obj,
+                    int64_t n, int to_point, float min_raydis, float self_bias,
+                    uint8_t* out);
+
+/* ------------------------------------------------------------------ QMC -- */
+/* n successive Halton::getNext() values of a fresh Halton(base) (mcqmc.h:35-49) */
+void yo_halton_seq(int base, int64_t n, float* out);
+/* index-addressed value via Halton::setStart's digit loop (mcqmc.h:25-33) */
+float yo_halton_at(int base, uint32_t i);
+float yo_ri_vdc(uint32_t bits, uint32_t r);
+float yo_ri_s(uint32_t i, uint32_t r);
+float yo_ri_lp(uint32_t i, uint32_t r);
+int yo_next_prime(int p);
+/* ourRandom(): advances *seed, returns the float (vector3d.h:288-297) */
+float yo_our_random(int32_t* seed);
+/* seed after n steps via a^n mod m */
+int32_t yo_lcg_jump(int32_t seed, uint64_t n);
+/* glibc TYPE_3 rand(): n outputs after srand(seed) */
+void yo_glibc_rand(uint32_t seed, int64_t n, int32_t* out);
+
+/* ---------------------------------------------------------------- noise -- */
+void yo_perm(unsigned char* out512);
+float yo_newperlin(float x, float y, float z);
+float yo_fbm(float x, float y, float z, float H, float lac, float oct);
+float yo_turbulence(float x, float y, float z, int oct, float size, int hard);
+/* what: YG_EVAL_* ; out gets 1 float (3 for TEX_COLOR) per point */
+int yo_noise_eval(int what, const yg_texture* tex, const float* pts, int64_t n, float* out);
+void yo_gen_heightfield(int nx, int ny, float ex, float ey, float amp, float freq, float H,
+                        float lac, float oct, float* xyz);
+
+/* ------------------------------------------------------------ rendering -- */
+/* scene_t::render(colorOutput_t&) mono mode, AA_passes 0 (scene.cc:538-600,
+ * 724-775).  rgb: resx*resy*3.  stats: [camera, extension, shadow, extra,
+ * lcg_draws] ray counts.  Photon lights need yo_photon_build first. */
+int yo_render(yo_scene* s, const yg_camera* cam, const yg_material* mats, int nmat,
+              const yg_modulator* mods, int nmod, const yg_texture* texs, int ntex,
+              const yg_light* lights, int nl, const int32_t* obj_flags, int nobj,
+              const yg_render_opts* opts, float* rgb, uint64_t* stats);
+
+/* same, areas spread over nthreads (only without area lights) */
+int yo_render_mt(yo_scene* s, const yg_camera* cam, const yg_material* mats, int nmat,
+                 const yg_modulator* mods, int nmod, const yg_texture* texs, int ntex,
+                 const yg_light* lights, int nl, const int32_t* obj_flags, int nobj,
+                 const yg_render_opts* opts, float* rgb, uint64_t* stats, int nthreads);
+
+/* photonLight_t::init for light li; keeps the clustered photons in s */
+int yo_photon_build(yo_scene* s, const yg_material* mats, int nmat, const yg_modulator* mods,
+                    int nmod, const yg_texture* texs, int ntex, const yg_light* lights,
+                    int nl, int li, const int32_t* obj_flags, int nobj,
+                    const yg_render_opts* opts, uint64_t* stats);
+/* copy clustered photons (9 floats each: dir, pos, colour) */
+int64_t yo_photon_read(yo_scene* s, float* out, int64_t cap);
+/* photonLight_t::illuminate at explicit shading points (see yg_photon_gather) */
+void yo_photon_gather(yo_scene* s, const yg_material* mats, const yg_modulator* mods,
+                      const yg_texture* texs, const yg_light* light, const float* p,
+                      const float* n, const float* ng, const float* eye, const int32_t* mat,
+                      int64_t cnt, float* rgb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

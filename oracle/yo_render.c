@@ -1,0 +1,944 @@
+/*
+ * CPU oracle: camera, generic diffuse shader, sun / area / path / photon
+ * lights and the mono render loop.  TEST INFRASTRUCTURE ONLY (see yaf_oracle.h).
+ */
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "yo_internal.h"
+
+yo_col yo_modulate(const yg_modulator* m, const yg_texture* texs, yo_col C, yo_v3 P);
+int yo_first_hit_one(const yo_scene* s, yo_v3 from, yo_v3 ray, float min_raydis, int32_t* prim,
+                     float* t);
+int yo_is_shadowed_one(const yo_scene* s, yo_v3 p, yo_v3 l, int self_obj, int to_point,
+                       float min_raydis, float self_bias);
+
+static inline float dcos(float x) { return (float)cos((double)x); }
+static inline float dsin(float x) { return (float)sin((double)x); }
+
+/* surfacePoint_t subset (surface.h:64-90) */
+typedef struct {
+    yo_v3 P, N, Ng, NU, NV;
+    float Z;
+    int32_t prim, obj, mat;
+} yo_sp;
+
+/* area light precomputation, arealight.cc:31-91 */
+typedef struct {
+    int samples;
+    yo_v3* points;
+    yo_v3* jit_h;
+    yo_v3* jit_v;
+    yo_v3 direction, from, corner, toX, toY;
+    yo_col color;
+    float pow;
+} yo_area;
+
+typedef struct {
+    const yo_scene* s;
+    const yg_material* mats;
+    int nmat;
+    const yg_modulator* mods;
+    const yg_texture* texs;
+    const yg_light* lights;
+    int nl;
+    const int32_t* obj_flags;
+    int nobj;
+    yg_render_opts opts;
+    yo_area* areas;        /* per light (only for area lights) */
+    yo_v3* sun_dir;        /* per light */
+    yo_col* sun_col;
+    yo_halton** hseq;      /* per light: path sampler state of the current render area */
+    int32_t lcg;           /* global myseed */
+    uint64_t stats[5];
+} rctx;
+
+static int obj_flag(const rctx* R, int obj, int bit)
+{
+    if (!R->obj_flags || obj < 0 || obj >= R->nobj) return 1;
+    return (R->obj_flags[obj] >> bit) & 1;
+}
+
+/* meshObject_t::shoot + triangle_t::getSurface fast path (mesh.cc:517-577,
+ * triangle.cc:138-141): P = from + Z*ray, N = Ng = face normal, NU/NV from createCS. */
+static void make_sp(const yo_scene* s, int32_t prim, yo_v3 f, yo_v3 ray, float Z, yo_sp* sp)
+{
+    sp->prim = prim;
+    sp->Z = Z;
+    sp->P = v3_add(f, v3_mul(ray, Z));
+    sp->Ng = v3_ld(s->tri_n + 3 * (int64_t)prim);
+    sp->N = sp->Ng;
+    create_cs(sp->N, &sp->NU, &sp->NV);
+    sp->obj = s->tri_obj[prim];
+    sp->mat = s->tri_mat[prim];
+}
+
+/* scene_t::firstHit (scene.cc:688-721) */
+static int first_hit(rctx* R, yo_v3 from, yo_v3 ray, yo_sp* sp)
+{
+    int32_t p; float t;
+    if (!yo_first_hit_one(R->s, from, ray, R->opts.min_raydis, &p, &t)) return 0;
+    yo_v3 f = v3_add(from, v3_mul(ray, R->opts.min_raydis));
+    make_sp(R->s, p, f, ray, t, sp);
+    return 1;
+}
+
+/* genericShader_t::getDiffuse, basicshaders.cc:191-204 */
+static yo_col get_diffuse(const rctx* R, const yo_sp* sp)
+{
+    const yg_material* m = &R->mats[sp->mat];
+    yo_col c = col(m->color[0], m->color[1], m->color[2]);
+    for (int k = 0; k < m->mod_count; k++)
+        c = yo_modulate(&R->mods[m->mod_first + k], R->texs, c, sp->P);
+    return c;
+}
+
+/* genericShader_t::fromLight, basicshaders.cc:46-70, diffuse only (ks = 0, edif = 1) */
+static yo_col from_light(const rctx* R, const yo_sp* sp, yo_v3 edir_dir, yo_col ecol, yo_v3 eye)
+{
+    yo_v3 edir = v3_norm(eye);
+    yo_v3 N = face_forward(sp->Ng, sp->N, edir);
+    float inte = v3_dot(N, edir_dir);
+    if (inte < 0) return col(0, 0, 0);
+    yo_col lkc = get_diffuse(R, sp);
+    return col_mul(col_scale(ecol, inte), lkc);
+}
+
+/* genericShader_t::fromRadiosity, basicshaders.cc:30-44 */
+static yo_col from_radiosity(const rctx* R, const yo_sp* sp, yo_v3 edir_dir, yo_col ecol, yo_v3 eye)
+{
+    if (v3_dot(face_forward(sp->Ng, sp->N, eye), edir_dir) < 0) return col(0, 0, 0);
+    yo_col lkc = get_diffuse(R, sp);
+    return col_mul(ecol, lkc);
+}
+
+static int is_shadowed(rctx* R, const yo_sp* sp, yo_v3 l, int to_point)
+{
+    R->stats[2]++;
+    return yo_is_shadowed_one(R->s, sp->P, l, sp->obj, to_point, R->opts.min_raydis,
+                              R->opts.self_bias);
+}
+
+/* sunLight_t::illuminate, sunlight.cc:35-57 */
+static yo_col sun_illuminate(rctx* R, int li, const yo_sp* sp, yo_v3 eye)
+{
+    const yg_light* L = &R->lights[li];
+    if (L->cast_shadows) {
+        if (!is_shadowed(R, sp, R->sun_dir[li], 0))
+            return from_light(R, sp, R->sun_dir[li], R->sun_col[li], eye);
+        return col(0, 0, 0);
+    }
+    return from_light(R, sp, R->sun_dir[li], R->sun_col[li], eye);
+}
+
+/* areaLight_t::fillQuad, arealight.cc:31-73 */
+static void area_setup(const yg_light* L, yo_area* A)
+{
+    yo_v3 a = v3_ld(L->a), b = v3_ld(L->b), c = v3_ld(L->c), d = v3_ld(L->d);
+    int samp = L->samples;
+    A->points = (yo_v3*)calloc((size_t)(samp > 0 ? samp : 1), sizeof(yo_v3));
+    A->jit_h = (yo_v3*)calloc((size_t)(samp > 0 ? samp : 1), sizeof(yo_v3));
+    A->jit_v = (yo_v3*)calloc((size_t)(samp > 0 ? samp : 1), sizeof(yo_v3));
+    float ab_len = v3_len(v3_sub(b, a)), bc_len = v3_len(v3_sub(c, b));
+    float cd_len = v3_len(v3_sub(d, c)), da_len = v3_len(v3_sub(a, d));
+    float max_v = (bc_len > da_len) ? bc_len : da_len;
+    float max_h = (ab_len > cd_len) ? ab_len : cd_len;
+    int v_samp = (int)((double)(max_v / (max_v + max_h)) * 2.0 * (double)sqrtf((float)samp));
+    int h_samp = (int)((double)(max_h / (max_v + max_h)) * 2.0 * (double)sqrtf((float)samp));
+    int index = 0;
+    if (samp == 1) index = 1;
+    else {
+        yo_v3 inc_v_a = v3_div(v3_sub(d, a), (float)v_samp);
+        yo_v3 inc_v_b = v3_div(v3_sub(c, b), (float)v_samp);
+        yo_v3 line_a = v3_add(a, v3_mul(inc_v_a, 0.5f));
+        yo_v3 line_b = v3_add(b, v3_mul(inc_v_b, 0.5f));
+        for (int i = 0; i < v_samp; ++i) {
+            yo_v3 inc_h = v3_div(v3_sub(line_b, line_a), (float)h_samp);
+            yo_v3 point = v3_add(line_a, v3_mul(inc_h, 0.5f));
+            for (int j = 0; j < h_samp; ++j) {
+                float vf = ((float)j) / ((float)h_samp);
+                yo_v3 inc_v = v3_add(v3_mul(inc_v_a, (float)(1.0 - (double)vf)), v3_mul(inc_v_b, vf));
+                if (index < samp) {
+                    A->points[index] = point;
+                    A->jit_h[index] = inc_h;
+                    A->jit_v[index] = inc_v;
+                }
+                index++;
+                point = v3_add(point, inc_h);
+            }
+            line_a = v3_add(line_a, inc_v_a);
+            line_b = v3_add(line_b, inc_v_b);
+        }
+    }
+    A->samples = index;
+    A->direction = v3_norm(v3_cross(v3_sub(b, a), v3_sub(d, a)));
+    A->color = col(L->color[0], L->color[1], L->color[2]);
+    A->pow = L->power;
+    A->from = v3_mul(v3_add(v3_add(v3_add(a, b), c), d), 0.25f);
+    A->corner = a;
+    A->toX = v3_sub(b, a);
+    A->toY = v3_sub(d, a);
+}
+
+/* areaLight_t::illuminate, arealight.cc:93-174, psamples = 0 (always PENUMBRA) */
+static yo_col area_illuminate(rctx* R, int li, const yo_sp* sp, yo_v3 eye)
+{
+    const yo_area* A = &R->areas[li];
+    yo_col resul = col(0, 0, 0);
+    yo_v3 edir = v3_norm(eye);
+    yo_v3 N = face_forward(sp->Ng, sp->N, edir);
+    float plane_at = v3_dot(v3_sub(sp->P, A->from), A->direction);
+    if (plane_at <= 0) return col(0, 0, 0); /* fromLight of a zero colour */
+    yo_col pc = col_scale(A->color, A->pow);
+    if (A->samples == 1) {
+        float r1 = our_random(&R->lcg);
+        float r2 = our_random(&R->lcg);
+        R->stats[4] += 2;
+        yo_v3 sampleP = v3_add(v3_add(A->corner, v3_mul(A->toX, r1)), v3_mul(A->toY, r2));
+        yo_v3 L = v3_sub(sampleP, sp->P);
+        if (v3_dot(L, N) < 0) return col(0, 0, 0);
+        if (!is_shadowed(R, sp, sampleP, 1)) {
+            yo_v3 dir = L;
+            float LD2 = v3_normlensqr(&dir);
+            LD2 = (LD2 != 0) ? (float)(1.0 / (double)LD2) : 1.0f;
+            resul = col_add(resul, from_light(R, sp, dir, col_scale(pc, LD2), eye));
+        }
+        return col_scale(resul, plane_at);
+    }
+    for (int i = 0; i < A->samples; ++i) {
+        float dish = (float)((double)our_random(&R->lcg) - 0.5);
+        float disv = (float)((double)our_random(&R->lcg) - 0.5);
+        R->stats[4] += 2;
+        yo_v3 sampleP = v3_add(v3_add(A->points[i], v3_mul(A->jit_h[i], dish)), v3_mul(A->jit_v[i], disv));
+        yo_v3 L = v3_sub(sampleP, sp->P);
+        if (v3_dot(L, N) < 0) continue;
+        if (!is_shadowed(R, sp, sampleP, 1)) {
+            yo_v3 dir = L;
+            float LD2 = v3_normlensqr(&dir);
+            LD2 = (LD2 != 0) ? (float)(1.0 / (double)LD2) : 1.0f;
+            resul = col_add(resul, from_light(R, sp, dir, col_scale(pc, LD2), eye));
+        }
+    }
+    yo_col r = col_scale(resul, plane_at);
+    float fs = (float)A->samples;
+    return col(r.r / fs, r.g / fs, r.b / fs);
+}
+
+static yo_col photon_illuminate(rctx* R, int li, const yo_sp* sp, yo_v3 eye);
+static yo_col path_illuminate(rctx* R, int li, const yo_sp* sp, yo_v3 eye);
+
+/* scene_t::light, scene.cc:260-277 (fromWorld of a diffuse shader is 0) */
+static yo_col scene_light(rctx* R, const yo_sp* sp, yo_v3 from, int indirect)
+{
+    yo_col fl = col(0, 0, 0);
+    yo_v3 eye = v3_sub(from, sp->P);
+    for (int li = 0; li < R->nl; li++) {
+        const yg_light* L = &R->lights[li];
+        if (!indirect && !L->use_in_render) continue;
+        if (indirect && !L->use_in_indirect) continue;
+        switch (L->type) {
+        case YG_LIGHT_SUN: fl = col_add(fl, sun_illuminate(R, li, sp, eye)); break;
+        case YG_LIGHT_AREA: fl = col_add(fl, area_illuminate(R, li, sp, eye)); break;
+        case YG_LIGHT_PATH: fl = col_add(fl, path_illuminate(R, li, sp, eye)); break;
+        case YG_LIGHT_PHOTON: fl = col_add(fl, photon_illuminate(R, li, sp, eye)); break;
+        }
+    }
+    return fl;
+}
+
+/* haltonSampler_t::nextDirection, pathtools.cc:32-43 */
+static yo_v3 next_direction(yo_halton* hs, int* taken, yo_v3 N, yo_v3 Ru, yo_v3 Rv, int cursam,
+                            int curlev)
+{
+    if (cursam > *taken) *taken = cursam;
+    curlev <<= 1;
+    float z1 = halton_next(&hs[curlev]);
+    float z2 = halton_next(&hs[curlev + 1]);
+    if (z1 > 1.0f) z1 = 1.0f;
+    z2 = (float)((double)z2 * (2.0 * M_PI));
+    yo_v3 a = v3_add(v3_mul(Ru, dcos(z2)), v3_mul(Rv, dsin(z2)));
+    return v3_add(v3_mul(a, (float)sqrt(1.0 - (double)z1)), v3_mul(N, sqrtf(z1)));
+}
+
+static yo_col background(const rctx* R)
+{
+    if (!R->opts.has_background) return col(0, 0, 0);
+    return col(R->opts.background[0], R->opts.background[1], R->opts.background[2]);
+}
+
+/* pathLight_t::takeSample full-GI branch, pathlight.cc:223-387 (no cache, no
+ * caustics, direct = false) */
+static yo_col path_take_sample(rctx* R, int li, yo_v3 N, const yo_sp* sp)
+{
+    const yg_light* L = &R->lights[li];
+    yo_halton* hs = R->hseq[li];
+    int samples = L->samples, maxdepth = L->depth;
+    yo_col total = col(0, 0, 0), subtotal[4];
+    for (int k = 0; k < 4; k++) subtotal[k] = col(0, 0, 0);
+    int taken = 0, first = 1;
+    float repetitions = 0.0f;
+    for (int ite = 1; ite > 0; --ite, repetitions += 1.0f) {
+        taken = 0;
+        for (int i = 0; i < samples; ++i) {
+            yo_col raycolor = col(1, 1, 1);
+            yo_v3 ray = next_direction(hs, &taken, N, sp->NU, sp->NV, i, 0);
+            yo_v3 startray = ray;
+            yo_v3 where = sp->P;
+            for (int j = 0; j < maxdepth; ++j) {
+                if (col_energy(raycolor) < 0.05f) break;
+                yo_sp t;
+                R->stats[1]++;
+                if (!first_hit(R, where, ray, &t)) {
+                    yo_col contri = col_mul(col_scale(raycolor, v3_dot(startray, N)), background(R));
+                    total = col_add(total, contri);
+                    if (first) subtotal[i % 4] = col_add(subtotal[i % 4], contri);
+                    break;
+                }
+                /* getLight, pathlight.cc:186-191: sc.light(indirect) at P + eye */
+                yo_v3 eye = v3_neg(ray);
+                yo_col light = scene_light(R, &t, v3_add(t.P, eye), 1);
+                yo_col contri = col_mul(col_scale(raycolor, v3_dot(startray, N)), light);
+                total = col_add(total, contri);
+                if (first) subtotal[i % 4] = col_add(subtotal[i % 4], contri);
+                yo_v3 HN = face_forward(t.Ng, t.N, v3_neg(ray));
+                raycolor = col_mul(raycolor, get_diffuse(R, &t));
+                ray = next_direction(hs, &taken, HN, t.NU, t.NV, i, j + 1);
+                where = t.P;
+            }
+        }
+        if (first) {
+            first = 0;
+            float mult = (float)(1.0 / (double)(float)(taken + 1));
+            float minR = 1000, minG = 1000, minB = 1000, maxR = 0, maxG = 0, maxB = 0;
+            for (int i = 0; i < 4; ++i) {
+                subtotal[i] = col_scale(subtotal[i], mult);
+                if (subtotal[i].r > maxR) maxR = subtotal[i].r;
+                if (subtotal[i].g > maxG) maxG = subtotal[i].g;
+                if (subtotal[i].b > maxB) maxB = subtotal[i].b;
+                if (subtotal[i].r < minR) minR = subtotal[i].r;
+                if (subtotal[i].g < minG) minG = subtotal[i].g;
+                if (subtotal[i].b < minB) minB = subtotal[i].b;
+            }
+            float ene = fmaxf(fabsf(minR), fmaxf(fabsf(minG), fabsf(minB)));
+            float dmax = fmaxf(fabsf(maxR - minR), fmaxf(fabsf(maxG - minG), fabsf(maxB - minB)));
+            float diff = (float)((double)dmax / ((double)((ene < 1.0f) ? 1.0f : ene) * 2.0 * 2.0));
+            diff /= L->threshold;
+            int extra = (int)(diff * diff) - 1;
+            if (extra > 3) extra = 3;
+            if (extra < 0) extra = 0;
+            if (extra > 0) R->stats[3]++;
+            ite += extra;
+        }
+    }
+    float mult = (float)(1.0 / (double)(float)(taken + 1));
+    return col_scale(total, mult / repetitions);
+}
+
+/* pathLight_t::normalSample, pathlight.cc:389-403 */
+static yo_col path_illuminate(rctx* R, int li, const yo_sp* sp, yo_v3 eye)
+{
+    const yg_light* L = &R->lights[li];
+    yo_v3 N = face_forward(sp->Ng, sp->N, eye);
+    yo_col dif = get_diffuse(R, sp);
+    if (col_energy(dif) < 0.05f) return col(0, 0, 0);
+    yo_col total = path_take_sample(R, li, N, sp);
+    return col_scale(col_mul(dif, total), L->power);
+}
+
+/* ------------------------------------------------------------- photons -- */
+
+/* randomVectorCone, vector3d.cc:162-168 */
+static yo_v3 random_vector_cone(yo_v3 D, yo_v3 U, yo_v3 V, float cosang, float z1, float z2)
+{
+    float t1 = (float)(2.0 * M_PI * (double)z1);
+    float t2 = (float)(1.0 - (1.0 - (double)cosang) * (double)z2);
+    yo_v3 a = v3_add(v3_mul(U, dcos(t1)), v3_mul(V, dsin(t1)));
+    return v3_add(v3_mul(a, (float)sqrt(1.0 - (double)(t2 * t2))), v3_mul(D, t2));
+}
+
+typedef struct {
+    int32_t cx, cy, cz;
+    int64_t seq;
+    yo_v3 dir, pos;
+    yo_col c;
+} yo_mark;
+
+typedef struct {
+    yo_mark* v;
+    int64_t n, cap;
+} markvec;
+
+static void mark_push(markvec* m, yo_mark x)
+{
+    if (m->n == m->cap) {
+        m->cap = m->cap ? 2 * m->cap : 1024;
+        m->v = (yo_mark*)realloc(m->v, sizeof(yo_mark) * (size_t)m->cap);
+    }
+    m->v[m->n++] = x;
+}
+
+/* hash3d_t::getBox (hash3d.h:218-228): truncation toward zero, minus one below 0 */
+static void hash_cell(float cell, yo_v3 p, int32_t* nx, int32_t* ny, int32_t* nz)
+{
+    *nx = (int32_t)(p.x / cell);
+    *ny = (int32_t)(p.y / cell);
+    *nz = (int32_t)(p.z / cell);
+    if (p.x < 0.0f) (*nx)--;
+    if (p.y < 0.0f) (*ny)--;
+    if (p.z < 0.0f) (*nz)--;
+}
+
+/* findBox = findCreateBox(getBox(p)) (hash3d.h:41-42): the cell of the cell centre */
+static void find_box(float cell, yo_v3 p, int32_t* bx, int32_t* by, int32_t* bz)
+{
+    int32_t nx, ny, nz;
+    hash_cell(cell, p, &nx, &ny, &nz);
+    yo_v3 c = v3((float)((double)(cell * (float)nx) + 0.5 * (double)cell),
+                 (float)((double)(cell * (float)ny) + 0.5 * (double)cell),
+                 (float)((double)(cell * (float)nz) + 0.5 * (double)cell));
+    hash_cell(cell, c, bx, by, bz);
+}
+
+typedef struct {
+    rctx* R;
+    const yg_light* L;
+    yo_halton* hs;
+    int depth;
+    markvec* marks;
+    int64_t photon_idx;
+    uint64_t stored;
+} pctx;
+
+typedef struct { yo_v3 pos, lastpos; yo_col c; } yo_photon;
+
+/* photonLight_t::shoot_photon_diffuse, photonlight.cc:203-244 */
+static void shoot_diffuse(pctx* P, yo_photon* ph, yo_v3 dir)
+{
+    rctx* R = P->R;
+    P->depth++;
+    yo_sp sp;
+    R->stats[1]++;
+    if (!first_hit(R, ph->pos, dir, &sp)) { P->depth--; return; }
+    /* photon_t::position, photonlight.cc:39-47 */
+    float ad = v3_len(v3_sub(ph->pos, sp.P));
+    if (ad > P->L->bias) ph->lastpos = ph->pos;
+    ph->pos = sp.P;
+    yo_v3 edir = v3_norm(v3_sub(ph->lastpos, ph->pos));
+    yo_v3 Ng = face_forward(sp.Ng, sp.Ng, edir);
+    int canreceive = (P->depth > P->L->mindepth) && obj_flag(R, sp.obj, 1);
+    if (canreceive) {
+        yo_mark mk;
+        mk.pos = ph->pos;
+        mk.c = ph->c;
+        mk.dir = v3_norm(v3_sub(ph->lastpos, ph->pos));
+        find_box(P->L->cluster, mk.pos, &mk.cx, &mk.cy, &mk.cz);
+        mk.seq = (int64_t)P->marks->n;
+        mark_push(P->marks, mk);
+        P->stored++;
+    }
+    if (obj_flag(R, sp.obj, 2) && P->depth <= P->L->depth) {
+        edir = v3_norm(edir);
+        int d2 = P->depth << 1;
+        float r1 = halton_next(&P->hs[d2]);
+        float r2 = halton_next(&P->hs[d2 + 1]);
+        yo_v3 refDir = random_vector_cone(Ng, sp.NU, sp.NV, 0.05f, r1, r2);
+        ph->c = from_radiosity(R, &sp, edir, ph->c, refDir);
+        shoot_diffuse(P, ph, refDir);
+    }
+    P->depth--;
+}
+
+static int mark_cmp(const void* a, const void* b)
+{
+    const yo_mark* x = (const yo_mark*)a;
+    const yo_mark* y = (const yo_mark*)b;
+    if (x->cx != y->cx) return x->cx < y->cx ? -1 : 1;
+    if (x->cy != y->cy) return x->cy < y->cy ? -1 : 1;
+    if (x->cz != y->cz) return x->cz < y->cz ? -1 : 1;
+    return x->seq < y->seq ? -1 : (x->seq > y->seq);
+}
+
+static int photon_np(const yg_light* L)
+{
+    int np = L->photons / (L->depth + 1 - L->mindepth);
+    int r = (int)sqrtf((float)np);
+    return r * r;
+}
+
+static void rctx_init(rctx* R, yo_scene* s, const yg_material* mats, int nmat,
+                      const yg_modulator* mods, const yg_texture* texs, const yg_light* lights,
+                      int nl, const int32_t* obj_flags, int nobj, const yg_render_opts* opts)
+{
+    memset(R, 0, sizeof(*R));
+    R->s = s; R->mats = mats; R->nmat = nmat; R->mods = mods; R->texs = texs;
+    R->lights = lights; R->nl = nl; R->obj_flags = obj_flags; R->nobj = nobj;
+    R->opts = *opts;
+    R->lcg = (int32_t)opts->lcg_seed;
+    R->areas = (yo_area*)calloc((size_t)(nl > 0 ? nl : 1), sizeof(yo_area));
+    R->sun_dir = (yo_v3*)calloc((size_t)(nl > 0 ? nl : 1), sizeof(yo_v3));
+    R->sun_col = (yo_col*)calloc((size_t)(nl > 0 ? nl : 1), sizeof(yo_col));
+    R->hseq = (yo_halton**)calloc((size_t)(nl > 0 ? nl : 1), sizeof(yo_halton*));
+    for (int li = 0; li < nl; li++) {
+        const yg_light* L = &lights[li];
+        if (L->type == YG_LIGHT_SUN) {
+            /* sunLight_t ctor, sunlight.cc:27-33 */
+            R->sun_dir[li] = v3_norm(v3_ld(L->dir));
+            R->sun_col[li] = col_scale(col(L->color[0], L->color[1], L->color[2]), L->power);
+        } else if (L->type == YG_LIGHT_AREA) {
+            area_setup(L, &R->areas[li]);
+        } else if (L->type == YG_LIGHT_PATH) {
+            int nd = (L->depth + 1) * 2;
+            R->hseq[li] = (yo_halton*)calloc((size_t)nd, sizeof(yo_halton));
+        }
+    }
+}
+
+static void rctx_free(rctx* R)
+{
+    for (int li = 0; li < R->nl; li++) {
+        free(R->areas[li].points); free(R->areas[li].jit_h); free(R->areas[li].jit_v);
+        free(R->hseq[li]);
+    }
+    free(R->areas); free(R->sun_dir); free(R->sun_col); free(R->hseq);
+}
+
+static void grid_build(yo_photonmap* pm, float radius)
+{
+    free(pm->gstart); free(pm->gitems);
+    pm->gstart = NULL; pm->gitems = NULL;
+    if (pm->n == 0) return;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = 0; i < pm->n; i++)
+        for (int k = 0; k < 3; k++) {
+            float v = pm->data[9 * i + 3 + k];
+            if (v < lo[k]) lo[k] = v;
+            if (v > hi[k]) hi[k] = v;
+        }
+    pm->gcell = radius > 0 ? radius : 1.0f;
+    int64_t nc = 1;
+    for (int k = 0; k < 3; k++) {
+        pm->gmin[k] = lo[k];
+        int64_t d = (int64_t)((hi[k] - lo[k]) / pm->gcell) + 1;
+        if (d > 1024) d = 1024;
+        pm->gdim[k] = (int)d;
+        nc *= d;
+    }
+    pm->gstart = (int64_t*)calloc((size_t)nc + 1, sizeof(int64_t));
+    pm->gitems = (int64_t*)malloc(sizeof(int64_t) * (size_t)pm->n);
+    int64_t* cell = (int64_t*)malloc(sizeof(int64_t) * (size_t)pm->n);
+    for (int64_t i = 0; i < pm->n; i++) {
+        int64_t c = 0;
+        for (int k = 2; k >= 0; k--) {
+            int64_t g = (int64_t)((pm->data[9 * i + 3 + k] - pm->gmin[k]) / pm->gcell);
+            if (g < 0) g = 0;
+            if (g >= pm->gdim[k]) g = pm->gdim[k] - 1;
+            c = c * pm->gdim[k] + g;
+        }
+        cell[i] = c;
+        pm->gstart[c + 1]++;
+    }
+    for (int64_t c = 0; c < nc; c++) pm->gstart[c + 1] += pm->gstart[c];
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)nc);
+    memcpy(fill, pm->gstart, sizeof(int64_t) * (size_t)nc);
+    for (int64_t i = 0; i < pm->n; i++) pm->gitems[fill[cell[i]]++] = i;
+    free(fill); free(cell);
+}
+
+/* photonLight_t::init, photonlight.cc:453-501 (+ preGathering :397-411) */
+int yo_photon_build(yo_scene* s, const yg_material* mats, int nmat, const yg_modulator* mods,
+                    int nmod, const yg_texture* texs, int ntex, const yg_light* lights, int nl,
+                    int li, const int32_t* obj_flags, int nobj, const yg_render_opts* opts,
+                    uint64_t* stats)
+{
+    (void)nmod; (void)ntex;
+    if (li < 0 || li >= nl || lights[li].type != YG_LIGHT_PHOTON) return -1;
+    rctx R;
+    rctx_init(&R, s, mats, nmat, mods, texs, lights, nl, obj_flags, nobj, opts);
+    const yg_light* L = &lights[li];
+    int Np = photon_np(L);
+    float angle_cos = (float)cos(((double)L->angle / 180.0) * M_PI);
+    float pw = (float)((double)L->power * 2.0 * M_PI * (1.0 - (double)angle_cos));
+    yo_col pcol = col_scale(col(L->color[0], L->color[1], L->color[2]), pw);
+    int nd = (L->depth + 1) * 2;
+    yo_halton* hs = (yo_halton*)calloc((size_t)nd, sizeof(yo_halton));
+    int base = 2;
+    for (int i = 0; i < nd; i++) { halton_init(&hs[i], base); base = yo_next_prime(base); }
+    yo_v3 from = v3_ld(L->from), to = v3_ld(L->to);
+    yo_v3 light_dir = v3_norm(v3_sub(to, from));
+    yo_v3 LU, LV;
+    create_cs(light_dir, &LU, &LV);
+    markvec marks = {0};
+    pctx P;
+    P.R = &R; P.L = L; P.hs = hs; P.depth = 0; P.marks = &marks; P.stored = 0;
+    int64_t emitted = 0;
+    for (int64_t i = 0; emitted < Np; ++i) {
+        yo_photon ph;
+        ph.c = pcol; ph.pos = from; ph.lastpos = v3(0, 0, 0);
+        float r1 = halton_next(&hs[0]);
+        float r2 = halton_next(&hs[1]);
+        yo_v3 dir = random_vector_cone(light_dir, LU, LV, angle_cos, r1, r2);
+        if (v3_null(dir)) continue;
+        P.photon_idx = emitted;
+        shoot_diffuse(&P, &ph, dir);
+        emitted++;
+    }
+    /* hash3d_t cells iterate in (x, y, z) order; sums follow insertion order */
+    qsort(marks.v, (size_t)marks.n, sizeof(yo_mark), mark_cmp);
+    yo_photonmap* pm = &s->pm;
+    free(pm->data);
+    pm->data = (float*)malloc(sizeof(float) * 9 * (size_t)(marks.n > 0 ? marks.n : 1));
+    pm->n = 0;
+    pm->cell = L->cluster;
+    pm->np_emit = Np;
+    for (int64_t a = 0; a < marks.n;) {
+        int64_t b = a;
+        yo_v3 dsum = v3(0, 0, 0), psum = v3(0, 0, 0);
+        yo_col csum = col(0, 0, 0);
+        float count = 0;
+        while (b < marks.n && marks.v[b].cx == marks.v[a].cx && marks.v[b].cy == marks.v[a].cy &&
+               marks.v[b].cz == marks.v[a].cz) {
+            dsum = v3_add(dsum, marks.v[b].dir);
+            psum = v3_add(psum, marks.v[b].pos);
+            csum = col_add(csum, marks.v[b].c);
+            count += 1;
+            b++;
+        }
+        if (!v3_null(dsum)) {
+            yo_v3 d = v3_norm(dsum);
+            yo_v3 p = v3_div(psum, count);
+            float* o = pm->data + 9 * pm->n;
+            o[0] = d.x; o[1] = d.y; o[2] = d.z; o[3] = p.x; o[4] = p.y; o[5] = p.z;
+            o[6] = csum.r; o[7] = csum.g; o[8] = csum.b;
+            pm->n++;
+        }
+        a = b;
+    }
+    grid_build(pm, L->fixedradius);
+    if (stats) {
+        stats[0] = (uint64_t)emitted; stats[1] = P.stored; stats[2] = (uint64_t)pm->n;
+        stats[3] = R.stats[1];
+    }
+    free(marks.v);
+    free(hs);
+    rctx_free(&R);
+    return 0;
+}
+
+int64_t yo_photon_read(yo_scene* s, float* out, int64_t cap)
+{
+    int64_t n = s->pm.n < cap ? s->pm.n : cap;
+    if (out && n > 0) memcpy(out, s->pm.data, sizeof(float) * 9 * (size_t)n);
+    return s->pm.n;
+}
+
+typedef struct { float dis; int64_t idx; } yo_found;
+
+static int found_cmp(const void* a, const void* b)
+{
+    const yo_found* x = (const yo_found*)a;
+    const yo_found* y = (const yo_found*)b;
+    if (x->dis != y->dis) return x->dis < y->dis ? -1 : 1;
+    return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+/* photonLight_t::illuminate, photonlight.cc:297-395.  The K nearest photons
+ * (dir.N > 0, dis <= fixedRadius) are summed in increasing (dis, index) order;
+ * the reference sums in heap order, which changes only float rounding. */
+static yo_col photon_estimate(const rctx* R, const yg_light* L, const yo_sp* sp, yo_v3 eye)
+{
+    const yo_photonmap* pm = &R->s->pm;
+    if (!obj_flag(R, sp->obj, 1)) return col(0, 0, 0);
+    yo_v3 N = face_forward(sp->Ng, sp->N, eye);
+    int K = L->search;
+    yo_found* cand = NULL;
+    int64_t nc = 0, cap = 0;
+    if (pm->n > 0 && pm->gstart) {
+        int g0[3], g1[3];
+        float pp[3] = {sp->P.x, sp->P.y, sp->P.z};
+        for (int k = 0; k < 3; k++) {
+            int64_t a = (int64_t)floorf((pp[k] - L->fixedradius - pm->gmin[k]) / pm->gcell);
+            int64_t b = (int64_t)floorf((pp[k] + L->fixedradius - pm->gmin[k]) / pm->gcell);
+            if (a < 0) a = 0;
+            if (b >= pm->gdim[k]) b = pm->gdim[k] - 1;
+            g0[k] = (int)a; g1[k] = (int)b;
+        }
+        for (int z = g0[2]; z <= g1[2]; z++)
+            for (int y = g0[1]; y <= g1[1]; y++)
+                for (int x = g0[0]; x <= g1[0]; x++) {
+                    int64_t c = ((int64_t)z * pm->gdim[1] + y) * pm->gdim[0] + x;
+                    for (int64_t q = pm->gstart[c]; q < pm->gstart[c + 1]; q++) {
+                        int64_t i = pm->gitems[q];
+                        const float* ph = pm->data + 9 * i;
+                        if (v3_dot(v3(ph[0], ph[1], ph[2]), N) <= 0.0f) continue;
+                        float dis = v3_len(v3_sub(sp->P, v3(ph[3], ph[4], ph[5])));
+                        if (dis <= L->fixedradius) {
+                            if (nc == cap) {
+                                cap = cap ? 2 * cap : 256;
+                                cand = (yo_found*)realloc(cand, sizeof(yo_found) * (size_t)cap);
+                            }
+                            cand[nc].dis = dis; cand[nc].idx = i; nc++;
+                        }
+                    }
+                }
+    }
+    qsort(cand, (size_t)nc, sizeof(yo_found), found_cmp);
+    if (nc > K) nc = K;
+    if (nc < 2) { free(cand); return col(0, 0, 0); }
+    const float radius = cand[nc - 1].dis;
+    yo_col total = col(0, 0, 0);
+    yo_v3 VX, VY;
+    create_cs(N, &VX, &VY);
+    float avgX = 0, avgY = 0, den = 0;
+    float* sx = (float*)malloc(sizeof(float) * (size_t)nc);
+    float* sy = (float*)malloc(sizeof(float) * (size_t)nc);
+    float* sw = (float*)malloc(sizeof(float) * (size_t)nc);
+    for (int64_t k = 0; k < nc; k++) {
+        const float* ph = pm->data + 9 * cand[k].idx;
+        const float dis = cand[k].dis;
+        /* filterCone, photonlight.h:205-210 */
+        float filter = (radius == 0) ? 1.0f : (float)((double)(float)(1.0 - (double)(dis / radius)) * 2.0);
+        yo_v3 pdir = v3(ph[0], ph[1], ph[2]);
+        total = col_add(total, from_light(R, sp, pdir, col_scale(col(ph[6], ph[7], ph[8]), filter), eye));
+        yo_v3 sub = v3_div(v3_sub(v3(ph[3], ph[4], ph[5]), sp->P), radius);
+        sx[k] = v3_dot(sub, VX);
+        sy[k] = v3_dot(sub, VY);
+        sw[k] = (float)((double)(v3_dot(pdir, N) * filter) * 0.5);
+    }
+    float area = (float)(M_PI * (double)radius * (double)radius);
+    /* fillArea, photonlight.cc:259-290 */
+    float fa = 1.0f;
+    {
+        const float SQRT2 = (float)sqrt(2.0);
+        const float offset = (float)1.414213562373095;
+        for (int64_t k = 0; k < nc; k++) { avgX += sx[k] * sw[k]; avgY += sy[k] * sw[k]; den += sw[k]; }
+        if (den != 0.0f) {
+            float avgR = 0;
+            avgX /= den; avgY /= den;
+            for (int64_t k = 0; k < nc; k++) {
+                float dx = sx[k] - avgX, dy = sy[k] - avgY;
+                avgR += sqrtf(dx * dx + dy * dy) * sw[k];
+            }
+            avgR = (avgR / den) * SQRT2 * offset;
+            avgR *= avgR;
+            if (avgR < 0.25f) avgR = 0.25f;
+            fa = avgR;
+        }
+    }
+    area *= fa;
+    const float minarea = (float)(4.0 * M_PI / (double)(float)pm->np_emit);
+    if (area < minarea) area = minarea;
+    const float factor = (float)(4.0 * M_PI / (double)((float)pm->np_emit * area));
+    free(sx); free(sy); free(sw); free(cand);
+    return col_scale(total, factor);
+}
+
+static yo_col photon_illuminate(rctx* R, int li, const yo_sp* sp, yo_v3 eye)
+{
+    return photon_estimate(R, &R->lights[li], sp, eye);
+}
+
+void yo_photon_gather(yo_scene* s, const yg_material* mats, const yg_modulator* mods,
+                      const yg_texture* texs, const yg_light* light, const float* p,
+                      const float* n, const float* ng, const float* eye, const int32_t* mat,
+                      int64_t cnt, float* rgb)
+{
+    rctx R;
+    yg_render_opts o;
+    memset(&o, 0, sizeof(o));
+    rctx_init(&R, s, mats, 1 << 30, mods, texs, light, 1, NULL, 0, &o);
+    for (int64_t i = 0; i < cnt; i++) {
+        yo_sp sp;
+        sp.P = v3_ld(p + 3 * i);
+        sp.N = v3_ld(n + 3 * i);
+        sp.Ng = v3_ld(ng + 3 * i);
+        create_cs(sp.N, &sp.NU, &sp.NV);
+        sp.Z = 0; sp.prim = -1; sp.obj = -1; sp.mat = mat[i];
+        yo_col c = photon_estimate(&R, light, &sp, v3_ld(eye + 3 * i));
+        rgb[3 * i] = c.r; rgb[3 * i + 1] = c.g; rgb[3 * i + 2] = c.b;
+    }
+    rctx_free(&R);
+}
+
+/* ------------------------------------------------------------- render -- */
+
+typedef struct { yo_v3 pos, vright, vup, vto; } yo_cam;
+
+/* camera_t::camera_t perspective part, camera.cc:27-87 */
+static void cam_setup(const yg_camera* c, yo_cam* k)
+{
+    yo_v3 pos = v3_ld(c->pos), look = v3_ld(c->look), up = v3_ld(c->up);
+    yo_v3 vup = v3_sub(up, pos), vto = v3_sub(look, pos);
+    yo_v3 vright = v3_cross(vup, vto);
+    vup = v3_cross(vright, vto);
+    vup = v3_norm(vup);
+    vright = v3_norm(vright);
+    vright = v3_mul(vright, -1.0f);
+    (void)v3_normlen(&vto);
+    vup = v3_mul(vup, c->aspect * (float)c->resy / (float)c->resx);
+    vto = v3_sub(v3_mul(vto, c->focal), v3_mul(v3_add(vup, vright), 0.5f));
+    vup = v3_div(vup, (float)c->resy);
+    vright = v3_div(vright, (float)c->resx);
+    k->pos = pos; k->vright = vright; k->vup = vup; k->vto = vto;
+}
+
+/* camera_t::shootRay perspective, camera.cc:184-189 */
+static yo_v3 cam_ray(const yo_cam* k, float px, float py)
+{
+    return v3_norm(v3_add(v3_add(v3_mul(k->vright, px), v3_mul(k->vup, py)), k->vto));
+}
+
+/* scene_t::raytrace, scene.cc:200-258 (raylevel 0, no fog, no displacement) */
+static yo_col raytrace(rctx* R, yo_v3 from, yo_v3 ray)
+{
+    yo_sp sp;
+    R->stats[0]++;
+    if (first_hit(R, from, ray, &sp)) return scene_light(R, &sp, from, 0);
+    return background(R);
+}
+
+typedef struct { int x, y, w, h, rx, ry, rw, rh; } yo_region;
+
+/* blockSpliter_t::blockSpliter_t, renderblock.cc:71-101 */
+int yo_block_regions(int width, int height, int b, uint32_t seed, yo_region** out)
+{
+    int bw = width / b, bh = height / b;
+    if (width % b) bw++;
+    if (height % b) bh++;
+    int n = bw * bh;
+    int* scram = (int*)malloc(sizeof(int) * (size_t)n);
+    int32_t* rnd = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    yo_glibc_rand(seed, n, rnd);
+    for (int i = 0; i < n; ++i) scram[i] = i;
+    for (int i = 0; i < n; ++i) {
+        int j = rnd[i] % n;
+        int t = scram[i]; scram[i] = scram[j]; scram[j] = t;
+    }
+    yo_region* regions = (yo_region*)calloc((size_t)n, sizeof(yo_region));
+    int sec = 0;
+    for (int i = 0; i < bh; ++i)
+        for (int j = 0; j < bw; ++j) {
+            yo_region r;
+            r.x = r.rx = j * b;
+            r.y = r.ry = i * b;
+            r.w = r.rw = width - r.rx;
+            r.h = r.rh = height - r.ry;
+            if (r.w > b) r.w = r.rw = b;
+            if (r.h > b) r.h = r.rh = b;
+            if (r.x > 0) { r.x--; r.w++; }
+            if (r.y > 0) { r.y--; r.h++; }
+            if ((r.x + r.w) < (width - 1)) r.w++;
+            if ((r.y + r.h) < (height - 1)) r.h++;
+            regions[scram[sec]] = r;
+            sec++;
+        }
+    free(scram); free(rnd);
+    *out = regions;
+    return n;
+}
+
+typedef struct {
+    yo_scene* s;
+    const yg_camera* cam;
+    const yg_material* mats; int nmat;
+    const yg_modulator* mods; const yg_texture* texs;
+    const yg_light* lights; int nl;
+    const int32_t* obj_flags; int nobj;
+    const yg_render_opts* opts;
+    const yo_cam* k;
+    const yo_region* regions;
+    int nreg;
+    float* rgb;
+    volatile int next;
+    pthread_mutex_t mu;
+    uint64_t stats[5];
+    rctx* shared_R;   /* sequential mode only */
+} render_job;
+
+static void render_area(rctx* R, const render_job* J, int ri)
+{
+    const yo_region* a = &J->regions[ri];
+    /* a fresh renderState_t (and sampler context) per area, scene.cc:726 */
+    for (int li = 0; li < J->nl; li++)
+        if (J->lights[li].type == YG_LIGHT_PATH) {
+            int nd = (J->lights[li].depth + 1) * 2, base = 2;
+            for (int d = 0; d < nd; d++) { halton_init(&R->hseq[li][d], base); base = yo_next_prime(base); }
+        }
+    for (int i = a->y; i < a->y + a->h; ++i)
+        for (int j = a->x; j < a->x + a->w; ++j) {
+            yo_v3 ray = cam_ray(J->k, (float)j + 0.5f, (float)i + 0.5f);
+            yo_col c = raytrace(R, J->k->pos, ray);
+            if (j >= a->rx && j < a->rx + a->rw && i >= a->ry && i < a->ry + a->rh) {
+                int64_t px = (int64_t)i * J->cam->resx + j;
+                J->rgb[3 * px] = c.r; J->rgb[3 * px + 1] = c.g; J->rgb[3 * px + 2] = c.b;
+            }
+        }
+}
+
+static void* render_worker(void* arg)
+{
+    render_job* J = (render_job*)arg;
+    rctx R;
+    rctx_init(&R, J->s, J->mats, J->nmat, J->mods, J->texs, J->lights, J->nl, J->obj_flags, J->nobj, J->opts);
+    for (;;) {
+        pthread_mutex_lock(&J->mu);
+        int q = J->next++;
+        pthread_mutex_unlock(&J->mu);
+        if (q >= J->nreg) break;
+        render_area(&R, J, J->nreg - 1 - q);
+    }
+    pthread_mutex_lock(&J->mu);
+    for (int k = 0; k < 5; k++) J->stats[k] += R.stats[k];
+    pthread_mutex_unlock(&J->mu);
+    rctx_free(&R);
+    return NULL;
+}
+
+/* scene_t::render(colorOutput_t&) mono loop, scene.cc:538-600; per area
+ * scene_t::render(renderArea_t&) first pass, scene.cc:724-775.  Areas are
+ * independent unless an area light draws from the global ourRandom() stream,
+ * so without area lights they may be rendered by several threads. */
+int yo_render_mt(yo_scene* s, const yg_camera* cam, const yg_material* mats, int nmat,
+                 const yg_modulator* mods, int nmod, const yg_texture* texs, int ntex,
+                 const yg_light* lights, int nl, const int32_t* obj_flags, int nobj,
+                 const yg_render_opts* opts, float* rgb, uint64_t* stats, int nthreads)
+{
+    (void)nmod; (void)ntex;
+    for (int o = 0; o < nobj; o++)
+        if (obj_flags && !(obj_flags[o] & 1)) return -4; /* non-shadow-casting objects */
+    int sequential = 0;
+    for (int li = 0; li < nl; li++)
+        if (lights[li].type == YG_LIGHT_AREA) sequential = 1;
+    if (nthreads < 1 || sequential) nthreads = 1;
+    if (nthreads > 64) nthreads = 64;
+    yo_cam k;
+    cam_setup(cam, &k);
+    yo_region* regions = NULL;
+    int nreg = yo_block_regions(cam->resx, cam->resy, opts->block > 0 ? opts->block : 64,
+                                opts->rand_seed, &regions);
+    render_job J;
+    memset(&J, 0, sizeof(J));
+    J.s = s; J.cam = cam; J.mats = mats; J.nmat = nmat; J.mods = mods; J.texs = texs;
+    J.lights = lights; J.nl = nl; J.obj_flags = obj_flags; J.nobj = nobj; J.opts = opts;
+    J.k = &k; J.regions = regions; J.nreg = nreg; J.rgb = rgb; J.next = 0;
+    pthread_mutex_init(&J.mu, NULL);
+    pthread_t th[64];
+    if (nthreads == 1) render_worker(&J);
+    else {
+        for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, render_worker, &J);
+        for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    }
+    pthread_mutex_destroy(&J.mu);
+    if (stats) for (int q = 0; q < 5; q++) stats[q] = J.stats[q];
+    free(regions);
+    return 0;
+}
+
+int yo_render(yo_scene* s, const yg_camera* cam, const yg_material* mats, int nmat,
+              const yg_modulator* mods, int nmod, const yg_texture* texs, int ntex,
+              const yg_light* lights, int nl, const int32_t* obj_flags, int nobj,
+              const yg_render_opts* opts, float* rgb, uint64_t* stats)
+{
+    return yo_render_mt(s, cam, mats, nmat, mods, nmod, texs, ntex, lights, nl, obj_flags, nobj,
+                        opts, rgb, stats, 1);
+}

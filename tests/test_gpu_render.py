@@ -1,0 +1,172 @@
+"""GPU renders (yg_render through the C ABI) against the CPU oracle's
+restatement of scene_t::render: same Halton / ourRandom() sample streams, so
+images agree pixel for pixel up to float rounding of transcendentals."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from yafray_amd import descriptors as D
+from yafray_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _Y():
+    from yafray_amd import lib as Y
+    return Y
+
+
+def gpu_render(desc, torch, build_kw=None):
+    Y = _Y()
+    sc = Y.Scene.from_desc(desc, **(build_kw or {}))
+    out = torch.zeros(desc.cam.resx * desc.cam.resy * 3, dtype=torch.float32, device="cuda")
+    st = sc.render(out)
+    torch.cuda.synchronize()
+    return out.cpu().numpy().reshape(desc.cam.resy, desc.cam.resx, 3), st, sc
+
+
+def rel_rmse(a, b):
+    return float(np.sqrt(np.mean((a.astype(np.float64) - b) ** 2)) / max(np.sqrt(np.mean(b.astype(np.float64) ** 2)), 1e-30))
+
+
+def small_terrain(nx=192, ny=96, resx=160, resy=90, samples=4, depth=3, nmat=64, mods=None):
+    return S.terrain_gi(po.heightfield, nx=nx, ny=ny, resx=resx, resy=resy, samples=samples,
+                        depth=depth, nmat=nmat)
+
+
+def test_gi_terrain_small_matches_oracle(torch_cuda):
+    torch = torch_cuda
+    d = small_terrain()
+    img, st, _ = gpu_render(d, torch)
+    ref, rs = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat).render(d, threads=8)
+    assert np.all(np.isfinite(img))
+    assert rel_rmse(img, ref) <= 1e-3
+    assert np.mean(np.all(img == ref, axis=2)) > 0.99
+    # identical control flow: same number of camera, extension and shadow rays
+    assert st.rays_camera == rs[0]
+    assert st.rays_extension == rs[1]
+    assert st.rays_shadow == rs[2]
+    assert st.adaptive_extra == 0
+
+
+def test_direct_room_area_light_matches_oracle(torch_cuda):
+    torch = torch_cuda
+    d = S.direct_room(160, 120, subdiv=4)
+    img, st, _ = gpu_render(d, torch)
+    ref, rs = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat).render(d)
+    assert rel_rmse(img, ref) <= 1e-3
+    assert np.mean(np.all(img == ref, axis=2)) > 0.99
+    assert st.rays_camera == rs[0] and st.rays_shadow == rs[2]
+
+
+def test_plane_gi_closed_form(torch_cuda):
+    """An infinite-looking plane under an empty sky: every path sample escapes,
+    so each pixel is rho*[(1/S) sum_i (w_i.N) B] + rho (N.s) E with w_i the
+    cosine-weighted Halton direction of sample i (pathlight.cc:305-347,
+    pathtools.cc:32-43, sunlight.cc:35-57).  (w_i.N) = sqrt(z1) with z1 the
+    base-2 Halton value of the pixel's per-area call index."""
+    torch = torch_cuda
+    E = 50.0
+    v = np.array([[-E, -E, 0], [E, -E, 0], [E, E, 0], [-E, E, 0]], np.float32)
+    f = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+    rho = (0.6, 0.5, 0.3)
+    S_ = 4
+    sun = np.array([1.0, 0.5, 1.5])
+    sun /= np.linalg.norm(sun)
+    lights = [D.pathlight(samples=S_, depth=2, use_qmc=True, threshold=1e3), D.sunlight(sun, power=2.0)]
+    resx, resy = 96, 80
+    cam = D.camera((0, 0, 3), (0, 0.01, 0), (0, 1, 4), resx, resy, 1.0, 1.0)
+    opts = D.render_opts(bias=1e-3, background=(0.4, 0.5, 0.7))
+    d = D.SceneDesc(v, f, [0, 0], [0, 0], [7], cam, [D.material(rho)], [], [], lights, opts)
+    img, st, _ = gpu_render(d, torch)
+    # expected value from the block layout of renderblock.cc:71-101
+    B = np.array([0.4, 0.5, 0.7])
+    rho_ = np.array(rho, np.float64)
+    ndots = float(np.dot([0, 0, 1], sun))
+    exp = np.zeros((resy, resx, 3))
+    blk = 64
+    for by in range((resy + blk - 1) // blk):
+        for bx in range((resx + blk - 1) // blk):
+            rx, ry = bx * blk, by * blk
+            rw, rh = min(blk, resx - rx), min(blk, resy - ry)
+            x, y, w, h = rx, ry, rw, rh
+            if x > 0:
+                x -= 1; w += 1
+            if y > 0:
+                y -= 1; h += 1
+            if x + w < resx - 1:
+                w += 1
+            if y + h < resy - 1:
+                h += 1
+            for i in range(ry, ry + rh):
+                for j in range(rx, rx + rw):
+                    k = (i - y) * w + (j - x)
+                    z1 = [float(radinv2(k * S_ + s + 1)) for s in range(S_)]
+                    path = rho_ * (np.sum(np.sqrt(z1)) / S_) * B
+                    exp[i, j] = path + rho_ * ndots * 2.0
+    assert np.max(np.abs(img - exp) / exp) < 2e-5
+
+
+def radinv2(i):
+    r, f = 0.0, 0.5
+    while i:
+        r += (i & 1) * f
+        i >>= 1
+        f *= 0.5
+    return r
+
+
+def test_gi_textured_musgrave_matches_oracle(torch_cuda):
+    """Albedo modulated by a Musgrave fBm texture (mode mul) on the terrain."""
+    torch = torch_cuda
+    d = small_terrain(nx=96, ny=48, resx=96, resy=64, samples=4, depth=3, nmat=8)
+    texs = [D.musgrave(color1=(0.2, 0.2, 0.2), color2=(1, 1, 1), octaves=4.0, size=1.0)]
+    mods = [D.modulator(0, D.YG_MOD_MUL, color=0.7, size=3.0)]
+    mats = [D.material(tuple(m.color), 0, 1) for m in d.mats]
+    d = D.SceneDesc(d.verts, d.idx, d.tri_obj, d.tri_mat, d.obj_flags, d.cam, mats, mods, texs,
+                    list(d.lights), d.opts)
+    img, st, _ = gpu_render(d, torch)
+    ref, rs = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat).render(d, threads=8)
+    assert rel_rmse(img, ref) <= 1e-3
+    assert st.rays_extension == rs[1]
+
+
+def test_gi_linear_in_sun_power(torch_cuda):
+    """The estimator is linear in the sun's emission (every path vertex's light
+    is the sun term): doubling the power doubles the sun-lit part exactly."""
+    torch = torch_cuda
+    d = small_terrain(nx=64, ny=32, resx=64, resy=48)
+    d.opts.has_background = 0
+    a, _, _ = gpu_render(d, torch)
+    L = list(d.lights)
+    L[1].power *= 2.0
+    d2 = D.SceneDesc(d.verts, d.idx, d.tri_obj, d.tri_mat, d.obj_flags, d.cam, list(d.mats), [], [], L, d.opts)
+    b, _, _ = gpu_render(d2, torch)
+    assert np.allclose(b, 2 * a, rtol=1e-5, atol=1e-7)
+
+
+def test_photon_small_matches_oracle(torch_cuda):
+    torch = torch_cuda
+    Y = _Y()
+    d = S.photon_field(n_side=4, subdiv=2, extent=10.0, resx=96, resy=64, photons=2 * 150 * 150,
+                       search=20, nmat=4)
+    sc = Y.Scene.from_desc(d)
+    ps = sc.photon_build(0)
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    os_ = oc.photon_build(d, 0)
+    assert ps.emitted == os_[0] == 150 * 150
+    assert ps.stored == os_[1]
+    assert ps.clustered == os_[2]
+    n = ps.clustered
+    buf = torch.zeros(n * 9, dtype=torch.float32, device="cuda")
+    assert sc.photon_read(buf) == n
+    g = buf.cpu().numpy().reshape(n, 9)
+    c = oc.photon_read()
+    assert np.mean(np.all(g == c, axis=1)) > 0.99
+    assert np.allclose(g, c, rtol=1e-5, atol=1e-6)
+    out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+    sc.render(out)
+    torch.cuda.synchronize()
+    img = out.cpu().numpy().reshape(d.cam.resy, d.cam.resx, 3)
+    ref, _ = oc.render(d)
+    assert rel_rmse(img, ref) <= 1e-3

@@ -1,0 +1,216 @@
+// C ABI of libyafgpu: errors, scene lifecycle, geometry ingestion, BVH build,
+// shading descriptors, QMC tables.
+#include <chrono>
+#include <cstring>
+#include <mutex>
+
+#include "qmc.h"
+#include "yg_internal.h"
+
+static thread_local std::string g_err;
+
+void yg_set_error(const std::string& msg) { g_err = msg; }
+
+int yg_fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+extern "C" int yg_abi_version(void) { return YG_ABI_VERSION; }
+
+extern "C" int yg_last_error(char* buf, size_t len)
+{
+    if (!buf || len == 0) return YG_ERR_ARG;
+    size_t n = std::min(len - 1, g_err.size());
+    memcpy(buf, g_err.data(), n);
+    buf[n] = 0;
+    return YG_OK;
+}
+
+extern "C" int yg_device_count(int* n)
+{
+    if (!n) return yg_fail(YG_ERR_ARG, "null");
+    YG_HIP(hipGetDeviceCount(n));
+    return YG_OK;
+}
+
+extern "C" int yg_scene_create(yg_scene** out)
+{
+    if (!out) return yg_fail(YG_ERR_ARG, "null out");
+    *out = new (std::nothrow) yg_scene();
+    if (!*out) return yg_fail(YG_ERR_NOMEM, "scene allocation");
+    return YG_OK;
+}
+
+static void free_dev(yg_scene* s)
+{
+    void* ptrs[] = {s->d_nodes, s->d_tris, s->d_tprim, s->d_tobj, s->d_tmat, s->d_obj_flags,
+                    s->d_mats, s->d_mods, s->d_texs, s->pm.d_photons, s->pm.d_gstart,
+                    s->pm.d_gitems};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    s->d_nodes = nullptr; s->d_tris = nullptr; s->d_tprim = nullptr; s->d_tobj = nullptr;
+    s->d_tmat = nullptr; s->d_obj_flags = nullptr; s->d_mats = nullptr; s->d_mods = nullptr;
+    s->d_texs = nullptr; s->pm = PhotonMapDev();
+}
+
+extern "C" int yg_scene_destroy(yg_scene* s)
+{
+    if (!s) return YG_OK;
+    free_dev(s);
+    delete s;
+    return YG_OK;
+}
+
+extern "C" int yg_mesh_add(yg_scene* s, const float* xyz, uint32_t nv, const uint32_t* idx,
+                           uint32_t nt, const int32_t* mat, int32_t flags, int32_t* obj_id)
+{
+    if (!s) return yg_fail(YG_ERR_ARG, "null scene");
+    if ((nv && !xyz) || (nt && !idx)) return yg_fail(YG_ERR_ARG, "null geometry");
+    for (uint64_t i = 0; i < 3ull * nt; i++)
+        if (idx[i] >= nv) return yg_fail(YG_ERR_ARG, "vertex index out of range");
+    const uint32_t vbase = (uint32_t)(s->verts.size() / 3);
+    if ((uint64_t)vbase + nv > 0xffffffffull) return yg_fail(YG_ERR_ARG, "too many vertices");
+    if (s->tri_mat.size() + nt > (size_t)(0x7fffffff / YG_LEAF_MAX))
+        return yg_fail(YG_ERR_ARG, "too many triangles");
+    s->verts.insert(s->verts.end(), xyz, xyz + 3ull * nv);
+    for (uint64_t i = 0; i < 3ull * nt; i++) s->idx.push_back(idx[i] + vbase);
+    const int32_t oid = s->nobj++;
+    s->tri_obj.insert(s->tri_obj.end(), nt, oid);
+    if (mat) s->tri_mat.insert(s->tri_mat.end(), mat, mat + nt);
+    else s->tri_mat.insert(s->tri_mat.end(), nt, 0);
+    s->obj_flags.push_back(flags);
+    s->built = false;
+    if (obj_id) *obj_id = oid;
+    return YG_OK;
+}
+
+extern "C" int yg_build(yg_scene* s, const yg_build_opts* opts)
+{
+    if (!s) return yg_fail(YG_ERR_ARG, "null scene");
+    int max_leaf = opts ? opts->max_leaf : 4;
+    int bins = opts ? opts->sah_bins : 32;
+    int threads = opts ? opts->threads : 0;
+    HostBVH h;
+    const int64_t nt = (int64_t)s->tri_mat.size();
+    build_bvh(s->verts.data(), s->idx.data(), nt, s->tri_obj.data(), s->tri_mat.data(), max_leaf,
+              bins, threads, h);
+    if (h.max_depth > YG_MAX_DEPTH) return yg_fail(YG_ERR_STATE, "BVH deeper than the traversal stack");
+    // the device copy replaces the previous one
+    void* old[] = {s->d_nodes, s->d_tris, s->d_tprim, s->d_tobj, s->d_tmat, s->d_obj_flags};
+    for (void* p : old)
+        if (p) (void)hipFree(p);
+    s->d_nodes = nullptr; s->d_tris = nullptr; s->d_tprim = nullptr; s->d_tobj = nullptr;
+    s->d_tmat = nullptr; s->d_obj_flags = nullptr;
+    s->dev_bytes = 0;
+    int rc;
+    if ((rc = dev_upload(&s->d_nodes, h.nodes.data(), h.nodes.size(), &s->dev_bytes))) return rc;
+    if ((rc = dev_upload(&s->d_tris, h.tris.data(), h.tris.size(), &s->dev_bytes))) return rc;
+    if ((rc = dev_upload(&s->d_tprim, h.tprim.data(), h.tprim.size(), &s->dev_bytes))) return rc;
+    if ((rc = dev_upload(&s->d_tobj, h.tobj.data(), h.tobj.size(), &s->dev_bytes))) return rc;
+    if ((rc = dev_upload(&s->d_tmat, h.tmat.data(), h.tmat.size(), &s->dev_bytes))) return rc;
+    if ((rc = dev_upload(&s->d_obj_flags, s->obj_flags.data(), s->obj_flags.size(), &s->dev_bytes))) return rc;
+    if (nt == 0) {
+        // keep valid (dummy) triangle arrays so kernels never see null pointers
+        float4 z[3] = {};
+        int32_t zi = 0;
+        if ((rc = dev_upload(&s->d_tris, z, 3))) return rc;
+        if ((rc = dev_upload(&s->d_tprim, &zi, 1))) return rc;
+        if ((rc = dev_upload(&s->d_tobj, &zi, 1))) return rc;
+        if ((rc = dev_upload(&s->d_tmat, &zi, 1))) return rc;
+    }
+    s->n_nodes = h.nodes.size();
+    s->max_depth = h.max_depth;
+    s->built = true;
+    return YG_OK;
+}
+
+extern "C" int yg_scene_info(yg_scene* s, uint64_t* n_tris, uint64_t* n_nodes, uint64_t* dev_bytes,
+                             int32_t* max_depth)
+{
+    if (!s) return yg_fail(YG_ERR_ARG, "null scene");
+    if (n_tris) *n_tris = s->tri_mat.size();
+    if (n_nodes) *n_nodes = s->n_nodes;
+    if (dev_bytes) *dev_bytes = s->dev_bytes;
+    if (max_depth) *max_depth = s->max_depth;
+    return YG_OK;
+}
+
+// ------------------------------------------------------------- shading --
+
+extern "C" int yg_set_camera(yg_scene* s, const yg_camera* cam)
+{
+    if (!s || !cam) return yg_fail(YG_ERR_ARG, "null argument");
+    if (cam->resx <= 0 || cam->resy <= 0) return yg_fail(YG_ERR_ARG, "bad resolution");
+    s->cam = *cam;
+    s->has_cam = true;
+    return YG_OK;
+}
+
+extern "C" int yg_set_shading(yg_scene* s, const yg_material* mats, uint32_t nmat,
+                              const yg_modulator* mods, uint32_t nmod, const yg_texture* texs,
+                              uint32_t ntex)
+{
+    if (!s) return yg_fail(YG_ERR_ARG, "null scene");
+    if (nmat == 0 || !mats) return yg_fail(YG_ERR_ARG, "at least one material is required");
+    for (uint32_t m = 0; m < nmat; m++)
+        if (mats[m].mod_count < 0 || mats[m].mod_first < 0 ||
+            (uint64_t)mats[m].mod_first + mats[m].mod_count > nmod)
+            return yg_fail(YG_ERR_ARG, "material modulator range out of bounds");
+    for (uint32_t k = 0; k < nmod; k++)
+        if (mods[k].tex < 0 || (uint32_t)mods[k].tex >= ntex)
+            return yg_fail(YG_ERR_ARG, "modulator texture index out of bounds");
+    for (uint32_t k = 0; k < ntex; k++) {
+        if (texs[k].noise != YG_NOISE_NEWPERLIN)
+            return yg_fail(YG_ERR_UNSUPPORTED, "only newPerlin_t noise is implemented");
+        if (texs[k].type < YG_TEX_CLOUDS || texs[k].type > YG_TEX_MUSGRAVE)
+            return yg_fail(YG_ERR_ARG, "bad texture type");
+    }
+    s->mats.assign(mats, mats + nmat);
+    s->mods.assign(mods, mods + nmod);
+    s->texs.assign(texs, texs + ntex);
+    int rc;
+    if ((rc = dev_upload(&s->d_mats, mats, nmat))) return rc;
+    yg_modulator dm{};
+    yg_texture dt{};
+    if ((rc = dev_upload(&s->d_mods, nmod ? mods : &dm, nmod ? nmod : 1))) return rc;
+    if ((rc = dev_upload(&s->d_texs, ntex ? texs : &dt, ntex ? ntex : 1))) return rc;
+    return YG_OK;
+}
+
+extern "C" int yg_set_lights(yg_scene* s, const yg_light* lights, uint32_t n)
+{
+    if (!s) return yg_fail(YG_ERR_ARG, "null scene");
+    for (uint32_t i = 0; i < n; i++) {
+        const yg_light& L = lights[i];
+        if (L.type < YG_LIGHT_SUN || L.type > YG_LIGHT_PHOTON) return yg_fail(YG_ERR_ARG, "bad light type");
+        if (L.type == YG_LIGHT_AREA && L.psamples != 0)
+            return yg_fail(YG_ERR_UNSUPPORTED, "area light psamples > 0 (guessLight's rand()) is not implemented");
+        if (L.type == YG_LIGHT_PATH && !L.use_qmc)
+            return yg_fail(YG_ERR_UNSUPPORTED, "pathlight needs use_QMC=on (the LCG sampler is not implemented)");
+        if (L.type == YG_LIGHT_PHOTON && !L.use_qmc)
+            return yg_fail(YG_ERR_UNSUPPORTED, "photonlight needs use_QMC=on");
+        if ((L.type == YG_LIGHT_PATH || L.type == YG_LIGHT_AREA) && L.samples < 1)
+            return yg_fail(YG_ERR_ARG, "samples must be >= 1");
+        if (L.type == YG_LIGHT_PATH && (L.depth < 0 || L.depth > 8))
+            return yg_fail(YG_ERR_UNSUPPORTED, "pathlight depth must be in 0..8");
+    }
+    s->lights.assign(lights, lights + n);
+    s->pm.light = -1;
+    return YG_OK;
+}
+
+// ----------------------------------------------------------------- QMC --
+
+extern "C" int yg_halton(int32_t base, uint32_t i0, uint32_t n, float* d_out, void* stream)
+{
+    if (base < 2 || (!d_out && n)) return yg_fail(YG_ERR_ARG, "bad halton arguments");
+    if (n == 0) return YG_OK;
+    std::vector<float> h;
+    halton_incremental(base, (uint64_t)i0 + n, h);
+    YG_HIP(hipMemcpyAsync(d_out, h.data() + i0, sizeof(float) * n, hipMemcpyHostToDevice,
+                          (hipStream_t)stream));
+    YG_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return YG_OK;
+}

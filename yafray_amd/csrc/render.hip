@@ -1,0 +1,869 @@
+// Wavefront renderer: scene_t::render(colorOutput_t&) in mono mode
+// (scene.cc:538-600, 724-775) for sun, area, path and photon lights.
+//
+// Work items are the pixels of the render areas (renderArea_t, 64x64 plus a
+// one-pixel overlap) laid out area by area in the reference's processing
+// order.  Sequential per-area state of the reference -- the Halton samplers a
+// pathlight keeps per area (pathlight.cc:725-738) and the global ourRandom()
+// stream the area light draws from (arealight.cc:139) -- is turned into
+// indices by exclusive prefix sums over the work items and path segments, so
+// every path reads exactly the sample values the CPU renderer would.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+
+#include "photon_dev.h"
+#include "qmc.h"
+#include "render_dev.h"
+
+#define RB 128
+
+__constant__ uint64_t YG_LCG_POW[64];
+
+// ------------------------------------------------------------- helpers --
+
+__device__ __forceinline__ int find_area(const Area* areas, int nareas, uint32_t item)
+{
+    int lo = 0, hi = nareas - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (areas[mid].item_start <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// camera_t::shootRay perspective (camera.cc:184-189)
+__device__ __forceinline__ f3 cam_ray(const RenderScene& R, float px, float py)
+{
+    return norm3((R.vright * px + R.vup * py) + R.vto);
+}
+
+__device__ __forceinline__ void item_pixel(const Area& a, uint32_t item, int& i, int& j)
+{
+    const uint32_t li = item - a.item_start;
+    i = a.y + (int)(li / (uint32_t)a.w);
+    j = a.x + (int)(li % (uint32_t)a.w);
+}
+
+struct Counters {
+    unsigned long long ext, shadow, extra;
+};
+
+__device__ __forceinline__ void block_add(unsigned long long* dst, uint32_t v)
+{
+    // one atomic per wavefront
+    uint64_t s = v;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(dst, (unsigned long long)s);
+}
+
+// ------------------------------------------------------------- kernels --
+
+// Camera rays and their closest hits (scene_t::raytrace, scene.cc:200-235).
+__global__ __launch_bounds__(RB) void k_primary(const RenderScene* __restrict__ scenes,
+                                                const Area* __restrict__ areas, int nareas,
+                                                uint32_t nitems, int32_t* __restrict__ item_ti,
+                                                float* __restrict__ item_t)
+{
+    __shared__ int s_stk[YG_STACK_LDS * RB];
+    const uint32_t item = blockIdx.x * RB + threadIdx.x;
+    if (item >= nitems) return;
+    LaneStack st;
+    st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = RB; st.sp = 0;
+    const Area a = areas[find_area(areas, nareas, item)];
+    const RenderScene& R = scenes[a.scene];
+    int i, j;
+    item_pixel(a, item, i, j);
+    const f3 ray = cam_ray(R, (float)j + 0.5f, (float)i + 0.5f);
+    float t = INFINITY;
+    int ti = -1;
+    if (R.raydepth > 0) ti = first_hit(R.bvh, R.cam_pos, ray, R.min_raydis, t, st);
+    item_ti[item] = ti;
+    item_t[item] = t;
+}
+
+__device__ __forceinline__ Surf primary_surf(const RenderScene& R, const Area& a, uint32_t item, int ti,
+                                             float t, f3& eye)
+{
+    int i, j;
+    item_pixel(a, item, i, j);
+    const f3 ray = cam_ray(R, (float)j + 0.5f, (float)i + 0.5f);
+    const f3 f = R.cam_pos + ray * R.min_raydis;
+    Surf sp = make_surf(R.bvh, ti, f, ray, t);
+    eye = R.cam_pos - sp.P;
+    return sp;
+}
+
+// Per item: ourRandom() draws of the area lights and whether the pathlight
+// samples at all (pathlight.cc:398: diffuse energy >= 0.05).
+__global__ __launch_bounds__(RB) void k_prep(const RenderScene* __restrict__ scenes,
+                                             const Area* __restrict__ areas, int nareas,
+                                             uint32_t nitems, const int32_t* __restrict__ item_ti,
+                                             const float* __restrict__ item_t,
+                                             uint32_t* __restrict__ draws, uint32_t* __restrict__ qual)
+{
+    const uint32_t item = blockIdx.x * RB + threadIdx.x;
+    if (item >= nitems) return;
+    const int ti = item_ti[item];
+    uint32_t dr = 0, q = 0;
+    if (ti >= 0) {
+        const Area a = areas[find_area(areas, nareas, item)];
+        const RenderScene& R = scenes[a.scene];
+        f3 eye;
+        const Surf sp = primary_surf(R, a, item, ti, item_t[item], eye);
+        for (int li = 0; li < R.nl; li++) {
+            const DevLight& L = R.lights[li];
+            if (!L.use_in_render || L.type != YG_LIGHT_AREA) continue;
+            if (dot(sp.P - L.from, L.dir) > 0) dr += 2u * (uint32_t)L.samples;
+        }
+        if (R.path_light >= 0 && R.lights[R.path_light].use_in_render) {
+            const c3 dif = get_diffuse(R.mats, R.mods, R.texs, sp.mat, sp.P);
+            q = (cenergy(dif) < 0.05f) ? 0u : 1u;
+        }
+    }
+    draws[item] = dr;
+    qual[item] = q;
+}
+
+// Sun and area lights at the primary hit, one colour per light.
+__global__ __launch_bounds__(RB) void k_direct(const RenderScene* __restrict__ scenes,
+                                               const Area* __restrict__ areas, int nareas,
+                                               uint32_t nitems, const int32_t* __restrict__ item_ti,
+                                               const float* __restrict__ item_t,
+                                               const uint32_t* __restrict__ lcg_off,
+                                               c3* __restrict__ lightc, int nl_max,
+                                               Counters* __restrict__ cnt)
+{
+    __shared__ int s_stk[YG_STACK_LDS * RB];
+    const uint32_t item = blockIdx.x * RB + threadIdx.x;
+    uint32_t nshadow = 0;
+    if (item < nitems) {
+        const int ti = item_ti[item];
+        if (ti >= 0) {
+            LaneStack st;
+            st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = RB; st.sp = 0;
+            const Area a = areas[find_area(areas, nareas, item)];
+            const RenderScene& R = scenes[a.scene];
+            f3 eye;
+            const Surf sp = primary_surf(R, a, item, ti, item_t[item], eye);
+            int32_t seed = 0;
+            bool seeded = false;
+            for (int li = 0; li < R.nl; li++) {
+                const DevLight& L = R.lights[li];
+                c3 c = mkc(0, 0, 0);
+                if (L.use_in_render) {
+                    if (L.type == YG_LIGHT_SUN) c = sun_illuminate(R, L, sp, eye, st, nshadow);
+                    else if (L.type == YG_LIGHT_AREA) {
+                        if (!seeded) {
+                            seed = lcg_jump(R.lcg_seed, (uint64_t)(lcg_off[item] - lcg_off[R.item_base]), YG_LCG_POW);
+                            seeded = true;
+                        }
+                        c = area_illuminate(R, L, sp, eye, seed, st, nshadow);
+                    }
+                }
+                lightc[(size_t)item * nl_max + li] = c;
+            }
+        }
+    }
+    block_add(&cnt->shadow, nshadow);
+}
+
+// One bounce of the pathlight's full-GI loop (pathlight.cc:305-347) for the
+// active path segments.  Path p = item*S + sample.
+__global__ __launch_bounds__(RB) void k_bounce(const RenderScene* __restrict__ scenes,
+                                               const Area* __restrict__ areas, int nareas, int j,
+                                               const uint32_t* __restrict__ active, uint32_t nactive,
+                                               int S, int D, const int32_t* __restrict__ item_ti,
+                                               const float* __restrict__ item_t,
+                                               const uint32_t* __restrict__ qoff,
+                                               const uint32_t* __restrict__ hidx,
+                                               f3* __restrict__ st_where, f3* __restrict__ st_hn,
+                                               int32_t* __restrict__ st_ti, c3* __restrict__ st_col,
+                                               float* __restrict__ st_s, c3* __restrict__ contri,
+                                               uint8_t* __restrict__ nseg, uint32_t* __restrict__ flag,
+                                               Counters* __restrict__ cnt)
+{
+    __shared__ int s_stk[YG_STACK_LDS * RB];
+    const uint32_t idx = blockIdx.x * RB + threadIdx.x;
+    uint32_t next = 0, nshadow = 0;
+    if (idx < nactive) {
+        LaneStack st;
+        st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = RB; st.sp = 0;
+        const uint32_t p = active[idx];
+        const uint32_t item = p / (uint32_t)S;
+        const int smp = (int)(p % (uint32_t)S);
+        const Area a = areas[find_area(areas, nareas, item)];
+        const RenderScene& R = scenes[a.scene];
+        f3 where, N, Ru, Rv;
+        c3 raycolor;
+        uint32_t k;
+        if (j == 0) {
+            f3 eye;
+            const Surf sp = primary_surf(R, a, item, item_ti[item], item_t[item], eye);
+            where = sp.P;
+            N = face_forward(sp.Ng, sp.N, eye);
+            Ru = sp.NU;
+            Rv = sp.NV;
+            raycolor = mkc(1, 1, 1);
+            k = (qoff[item] - qoff[a.item_start]) * (uint32_t)S + (uint32_t)smp;
+        } else {
+            where = st_where[p];
+            N = st_hn[p];
+            const int pti = st_ti[p];
+            const float4 A = R.bvh.tris[3 * pti], B = R.bvh.tris[3 * pti + 1], C = R.bvh.tris[3 * pti + 2];
+            create_cs(mk3(A.w, B.w, C.w), Ru, Rv);
+            raycolor = st_col[p];
+            k = hidx[p];
+        }
+        const float z1 = R.halton[(size_t)(2 * j) * R.hstride + k];
+        const float z2 = R.halton[(size_t)(2 * j + 1) * R.hstride + k];
+        const f3 ray = hemi_direction(z1, z2, N, Ru, Rv);
+        float s;
+        if (j == 0) {
+            s = dot(ray, N);
+            st_s[p] = s;
+        } else s = st_s[p];
+        if (cenergy(raycolor) < 0.05f) {
+            nseg[p] = (uint8_t)j;
+        } else {
+            next = 1;
+            float t;
+            const int ti = first_hit(R.bvh, where, ray, R.min_raydis, t, st);
+            if (ti < 0) {
+                contri[(size_t)p * D + j] = cmul(cscale(raycolor, s), R.bg);
+                nseg[p] = (uint8_t)(j + 1);
+            } else {
+                const Surf sp = make_surf(R.bvh, ti, where + ray * R.min_raydis, ray, t);
+                const c3 light = indirect_light(R, sp, ray, st, nshadow);
+                contri[(size_t)p * D + j] = cmul(cscale(raycolor, s), light);
+                st_hn[p] = face_forward(sp.Ng, sp.N, -ray);
+                st_col[p] = cmul(raycolor, get_diffuse(R.mats, R.mods, R.texs, sp.mat, sp.P));
+                st_where[p] = sp.P;
+                st_ti[p] = ti;
+                flag[p] = 1;
+                nseg[p] = (uint8_t)(j + 1);
+            }
+        }
+    }
+    block_add(&cnt->ext, next);
+    block_add(&cnt->shadow, nshadow);
+}
+
+// Scatter flagged paths into the next active list and give each its Halton
+// call index within its render area.
+__global__ void k_compact(const Area* __restrict__ areas, int nareas, uint32_t npaths, int S,
+                          const uint32_t* __restrict__ flag, const uint32_t* __restrict__ ex,
+                          uint32_t* __restrict__ active, uint32_t* __restrict__ hidx)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npaths || !flag[p]) return;
+    const uint32_t item = p / (uint32_t)S;
+    const Area& a = areas[find_area(areas, nareas, item)];
+    const uint32_t e = ex[p];
+    active[e] = p;
+    hidx[p] = e - ex[(size_t)a.item_start * S];
+}
+
+// level-0 active list: every sample of every qualifying item
+__global__ void k_path_init(uint32_t nitems, int S, const uint32_t* __restrict__ qual,
+                            const uint32_t* __restrict__ qoff, uint32_t* __restrict__ active)
+{
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= nitems || !qual[item]) return;
+    for (int i = 0; i < S; i++) active[(size_t)qoff[item] * S + i] = item * (uint32_t)S + i;
+}
+
+// Ordered light sum, pathlight estimate (pathlight.cc:349-403) and pixel output.
+__global__ __launch_bounds__(RB) void k_finalize(const RenderScene* __restrict__ scenes,
+                                                 const Area* __restrict__ areas, int nareas,
+                                                 uint32_t nitems, int S, int D,
+                                                 const int32_t* __restrict__ item_ti,
+                                                 const float* __restrict__ item_t,
+                                                 const uint32_t* __restrict__ qual,
+                                                 const c3* __restrict__ lightc, int nl_max,
+                                                 const c3* __restrict__ contri,
+                                                 const uint8_t* __restrict__ nseg,
+                                                 Counters* __restrict__ cnt)
+{
+    const uint32_t item = blockIdx.x * RB + threadIdx.x;
+    uint32_t extra_n = 0;
+    if (item < nitems) {
+        const Area a = areas[find_area(areas, nareas, item)];
+        const RenderScene& R = scenes[a.scene];
+        int i, j;
+        item_pixel(a, item, i, j);
+        const int ti = item_ti[item];
+        c3 col;
+        if (ti < 0) col = R.bg;
+        else {
+            c3 pathc = mkc(0, 0, 0);
+            if (R.path_light >= 0 && qual[item]) {
+                c3 total = mkc(0, 0, 0), sub[4];
+                for (int q = 0; q < 4; q++) sub[q] = mkc(0, 0, 0);
+                for (int smp = 0; smp < S; smp++) {
+                    const size_t p = (size_t)item * S + smp;
+                    const int ns = nseg[p];
+                    for (int b = 0; b < ns; b++) {
+                        const c3 c = contri[p * D + b];
+                        total = cadd(total, c);
+                        sub[smp % 4] = cadd(sub[smp % 4], c);
+                    }
+                }
+                const float mult = (float)(1.0 / (double)(float)S);
+                // adaptive repetition test (pathlight.cc:349-374): only counted
+                float minR = 1000, minG = 1000, minB = 1000, maxR = 0, maxG = 0, maxB = 0;
+                for (int q = 0; q < 4; q++) {
+                    const c3 v = cscale(sub[q], mult);
+                    maxR = fmaxf(maxR, v.r); maxG = fmaxf(maxG, v.g); maxB = fmaxf(maxB, v.b);
+                    minR = fminf(minR, v.r); minG = fminf(minG, v.g); minB = fminf(minB, v.b);
+                }
+                const float ene = fmaxf(fabsf(minR), fmaxf(fabsf(minG), fabsf(minB)));
+                const float dmax = fmaxf(fabsf(maxR - minR), fmaxf(fabsf(maxG - minG), fabsf(maxB - minB)));
+                float diff = (float)((double)dmax / ((double)((ene < 1.0f) ? 1.0f : ene) * 2.0 * 2.0));
+                diff /= R.threshold;
+                if ((int)(diff * diff) - 1 > 0) extra_n = 1;
+                total = cscale(total, mult / 1.0f);
+                f3 eye;
+                const Surf sp = primary_surf(R, a, item, ti, item_t[item], eye);
+                const c3 dif = get_diffuse(R.mats, R.mods, R.texs, sp.mat, sp.P);
+                pathc = cscale(cmul(dif, total), R.path_power);
+            }
+            col = mkc(0, 0, 0);
+            for (int li = 0; li < R.nl; li++) {
+                if (!R.lights[li].use_in_render) continue;
+                col = cadd(col, (li == R.path_light) ? pathc : lightc[(size_t)item * nl_max + li]);
+            }
+        }
+        if (j >= a.rx && j < a.rx + a.rw && i >= a.ry && i < a.ry + a.rh) {
+            float* o = R.out_rgb + 3 * ((size_t)i * R.resx + j);
+            o[0] = col.r; o[1] = col.g; o[2] = col.b;
+        }
+    }
+    block_add(&cnt->extra, extra_n);
+}
+
+// ------------------------------------------------------------------ host --
+
+struct HostCam {
+    f3 pos, vright, vup, vto;
+};
+
+// camera_t::camera_t perspective part (camera.cc:27-87)
+static HostCam camera_setup(const yg_camera& c)
+{
+    const f3 pos = ld3(c.pos), look = ld3(c.look), up = ld3(c.up);
+    f3 vup = up - pos, vto = look - pos;
+    f3 vright = cross(vup, vto);
+    vup = cross(vright, vto);
+    vup = norm3(vup);
+    vright = norm3(vright);
+    vright = vright * -1.0f;
+    (void)normlen3(vto);
+    vup = vup * (c.aspect * (float)c.resy / (float)c.resx);
+    vto = vto * c.focal - (vup + vright) * 0.5f;
+    vup = vup / (float)c.resy;
+    vright = vright / (float)c.resx;
+    HostCam h;
+    h.pos = pos; h.vright = vright; h.vup = vup; h.vto = vto;
+    return h;
+}
+
+// blockSpliter_t (renderblock.cc:71-101): areas in processing order (the
+// mono loop takes regions.back() first, scene.cc:585-588)
+static void block_regions(int width, int height, int b, uint32_t seed, std::vector<Area>& out)
+{
+    int bw = width / b, bh = height / b;
+    if (width % b) bw++;
+    if (height % b) bh++;
+    const int n = bw * bh;
+    std::vector<int> scram(n);
+    std::vector<int32_t> rnd;
+    glibc_rand(seed, n, rnd);
+    for (int i = 0; i < n; ++i) scram[i] = i;
+    for (int i = 0; i < n; ++i) std::swap(scram[i], scram[rnd[i] % n]);
+    std::vector<Area> regions(n);
+    int sec = 0;
+    for (int i = 0; i < bh; ++i)
+        for (int j = 0; j < bw; ++j) {
+            Area r{};
+            r.x = r.rx = j * b;
+            r.y = r.ry = i * b;
+            r.w = r.rw = width - r.rx;
+            r.h = r.rh = height - r.ry;
+            if (r.w > b) r.w = r.rw = b;
+            if (r.h > b) r.h = r.rh = b;
+            if (r.x > 0) { r.x--; r.w++; }
+            if (r.y > 0) { r.y--; r.h++; }
+            if ((r.x + r.w) < (width - 1)) r.w++;
+            if ((r.y + r.h) < (height - 1)) r.h++;
+            regions[scram[sec]] = r;
+            sec++;
+        }
+    out.clear();
+    for (int k = n - 1; k >= 0; k--) out.push_back(regions[k]);
+}
+
+// areaLight_t::fillQuad (arealight.cc:31-73) and constructor (:75-91)
+static void area_setup(const yg_light& Lg, DevLight& L, std::vector<f3>& pts, std::vector<f3>& jh,
+                       std::vector<f3>& jv)
+{
+    const f3 a = ld3(Lg.a), b = ld3(Lg.b), c = ld3(Lg.c), d = ld3(Lg.d);
+    const int samp = Lg.samples;
+    const float ab_len = len3(b - a), bc_len = len3(c - b), cd_len = len3(d - c), da_len = len3(a - d);
+    const float max_v = (bc_len > da_len) ? bc_len : da_len;
+    const float max_h = (ab_len > cd_len) ? ab_len : cd_len;
+    const int v_samp = (int)((double)(max_v / (max_v + max_h)) * 2.0 * (double)sqrtf((float)samp));
+    const int h_samp = (int)((double)(max_h / (max_v + max_h)) * 2.0 * (double)sqrtf((float)samp));
+    L.pts = (int)pts.size();
+    std::vector<f3> P(samp > 0 ? samp : 1, mk3(0, 0, 0)), H(P), V(P);
+    int index = 0;
+    if (samp == 1) index = 1;
+    else {
+        const f3 inc_v_a = (d - a) / (float)v_samp;
+        const f3 inc_v_b = (c - b) / (float)v_samp;
+        f3 line_a = a + inc_v_a * 0.5f;
+        f3 line_b = b + inc_v_b * 0.5f;
+        for (int i = 0; i < v_samp; ++i) {
+            const f3 inc_h = (line_b - line_a) / (float)h_samp;
+            f3 point = line_a + inc_h * 0.5f;
+            for (int j = 0; j < h_samp; ++j) {
+                const float vf = ((float)j) / ((float)h_samp);
+                const f3 inc_v = inc_v_a * (float)(1.0 - (double)vf) + inc_v_b * vf;
+                if (index < samp) { P[index] = point; H[index] = inc_h; V[index] = inc_v; }
+                index++;
+                point = point + inc_h;
+            }
+            line_a = line_a + inc_v_a;
+            line_b = line_b + inc_v_b;
+        }
+    }
+    L.samples = index;
+    pts.insert(pts.end(), P.begin(), P.end());
+    jh.insert(jh.end(), H.begin(), H.end());
+    jv.insert(jv.end(), V.begin(), V.end());
+    L.dir = norm3(cross(b - a, d - a));
+    L.col = cscale(mkc(Lg.color[0], Lg.color[1], Lg.color[2]), Lg.power);
+    L.from = (((a + b) + c) + d) * 0.25f;
+}
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t n)
+    {
+        if (n <= cap && p) return YG_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(n, 1);
+        hipError_t e = hipMalloc((void**)&p, sizeof(T) * want);
+        if (e != hipSuccess) return yg_fail(YG_ERR_NOMEM, std::string("render buffer: ") + hipGetErrorString(e));
+        cap = want;
+        return YG_OK;
+    }
+};
+
+// Per-process render workspace (grow-only, reused across calls).
+struct RenderWS {
+    DevBuf<RenderScene> scenes;
+    DevBuf<Area> areas;
+    DevBuf<DevLight> lights;
+    DevBuf<f3> apts, ajh, ajv;
+    DevBuf<float> halton;
+    DevBuf<int32_t> item_ti;
+    DevBuf<float> item_t;
+    DevBuf<uint32_t> draws, lcg_off, qual, qoff;
+    DevBuf<c3> lightc;
+    DevBuf<uint32_t> active0, active1, flag, ex, hidx;
+    DevBuf<f3> st_where, st_hn;
+    DevBuf<int32_t> st_ti;
+    DevBuf<c3> st_col;
+    DevBuf<float> st_s;
+    DevBuf<c3> contri;
+    DevBuf<uint8_t> nseg;
+    DevBuf<Counters> cnt;
+    DevBuf<uint8_t> scan_tmp;
+    bool pow_ready = false;
+};
+
+static RenderWS g_ws;
+
+static int exclusive_scan(const uint32_t* in, uint32_t* out, size_t n, hipStream_t st)
+{
+    size_t bytes = 0;
+    YG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, st));
+    int rc = g_ws.scan_tmp.ensure(bytes);
+    if (rc) return rc;
+    YG_HIP(hipcub::DeviceScan::ExclusiveSum(g_ws.scan_tmp.p, bytes, in, out, (int)n, st));
+    return YG_OK;
+}
+
+
+// Prepared host description of one scene for a render launch.
+struct ScenePlan {
+    yg_scene* s;
+    RenderScene R;
+    std::vector<Area> areas;
+    std::vector<DevLight> lights;
+    std::vector<f3> apts, ajh, ajv;
+    std::vector<float> halton;
+    uint32_t nitems = 0;
+    int max_area_items = 0;
+};
+
+static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, ScenePlan& P)
+{
+    if (!s) return yg_fail(YG_ERR_ARG, "null scene");
+    if (!s->built) return yg_fail(YG_ERR_STATE, "scene not built");
+    if (!s->has_cam) return yg_fail(YG_ERR_STATE, "no camera (yg_set_camera)");
+    if (s->mats.empty()) return yg_fail(YG_ERR_STATE, "no materials (yg_set_shading)");
+    if (!o || !d_rgb) return yg_fail(YG_ERR_ARG, "null argument");
+    for (size_t t = 0; t < s->tri_mat.size(); t++)
+        if (s->tri_mat[t] < 0 || (size_t)s->tri_mat[t] >= s->mats.size())
+            return yg_fail(YG_ERR_ARG, "triangle material id out of range");
+    for (int32_t f : s->obj_flags)
+        if (!(f & 1)) return yg_fail(YG_ERR_UNSUPPORTED, "objects that cast no shadows are not implemented");
+    P.s = s;
+    RenderScene& R = P.R;
+    memset(&R, 0, sizeof(R));
+    R.bvh = s->view();
+    R.mats = s->d_mats;
+    R.mods = s->d_mods;
+    R.texs = s->d_texs;
+    R.obj_flags = s->d_obj_flags;
+    const HostCam hc = camera_setup(s->cam);
+    R.cam_pos = hc.pos; R.vright = hc.vright; R.vup = hc.vup; R.vto = hc.vto;
+    R.resx = s->cam.resx;
+    R.resy = s->cam.resy;
+    R.min_raydis = o->min_raydis;
+    R.self_bias = o->self_bias;
+    R.raydepth = o->raydepth;
+    R.bg = o->has_background ? mkc(o->background[0], o->background[1], o->background[2]) : mkc(0, 0, 0);
+    R.lcg_seed = o->lcg_seed;
+    R.out_rgb = d_rgb;
+    R.path_light = -1;
+    R.photon_light = -1;
+    R.nl = (int)s->lights.size();
+    for (int li = 0; li < R.nl; li++) {
+        const yg_light& Lg = s->lights[li];
+        DevLight L;
+        memset(&L, 0, sizeof(L));
+        L.type = Lg.type;
+        L.use_in_render = Lg.use_in_render;
+        L.use_in_indirect = Lg.use_in_indirect;
+        L.cast_shadows = Lg.cast_shadows;
+        if (Lg.type == YG_LIGHT_SUN) {
+            // sunLight_t ctor (sunlight.cc:27-33)
+            L.dir = norm3(ld3(Lg.dir));
+            L.col = cscale(mkc(Lg.color[0], Lg.color[1], Lg.color[2]), Lg.power);
+        } else if (Lg.type == YG_LIGHT_AREA) {
+            area_setup(Lg, L, P.apts, P.ajh, P.ajv);
+        } else if (Lg.type == YG_LIGHT_PATH) {
+            if (R.path_light >= 0) return yg_fail(YG_ERR_UNSUPPORTED, "more than one pathlight");
+            R.path_light = li;
+            R.samples = Lg.samples;
+            R.depth = Lg.depth;
+            R.path_power = Lg.power;
+            R.threshold = Lg.threshold;
+        } else if (Lg.type == YG_LIGHT_PHOTON) {
+            if (R.photon_light >= 0) return yg_fail(YG_ERR_UNSUPPORTED, "more than one photonlight");
+            if (s->pm.light != li) return yg_fail(YG_ERR_STATE, "photon map not built (yg_photon_build)");
+            if (Lg.search < 1 || Lg.search > YG_KNN_MAX) return yg_fail(YG_ERR_UNSUPPORTED, "photon search must be 1..64");
+            R.photon_light = li;
+            L.K = Lg.search;
+            L.fixedradius = Lg.fixedradius;
+        }
+        P.lights.push_back(L);
+    }
+    if (R.path_light >= 0) {
+        for (int li = 0; li < R.nl; li++) {
+            const yg_light& Lg = s->lights[li];
+            if (Lg.use_in_indirect && (Lg.type == YG_LIGHT_AREA))
+                return yg_fail(YG_ERR_UNSUPPORTED, "area lights inside pathlight GI paths are not implemented");
+        }
+        if (R.depth > 255) return yg_fail(YG_ERR_UNSUPPORTED, "pathlight depth");
+    }
+    block_regions(R.resx, R.resy, o->block > 0 ? o->block : 64, o->rand_seed, P.areas);
+    uint32_t acc = 0;
+    for (auto& a : P.areas) {
+        a.item_start = acc;
+        acc += (uint32_t)(a.w * a.h);
+        P.max_area_items = std::max(P.max_area_items, a.w * a.h);
+    }
+    P.nitems = acc;
+    if (R.path_light >= 0) {
+        const int ndim = 2 * (R.depth + 1);
+        R.hstride = P.max_area_items * R.samples + 1;
+        P.halton.resize((size_t)ndim * R.hstride);
+        int base = 2;
+        std::vector<float> tab;
+        for (int d = 0; d < ndim; d++) {
+            halton_incremental(base, (uint64_t)R.hstride, tab);
+            memcpy(&P.halton[(size_t)d * R.hstride], tab.data(), sizeof(float) * R.hstride);
+            base = next_prime(base);
+        }
+    }
+    if (R.photon_light >= 0) {
+        R.pm.photons = s->pm.d_photons;
+        R.pm.gstart = s->pm.d_gstart;
+        R.pm.gitems = s->pm.d_gitems;
+        for (int k = 0; k < 3; k++) { R.pm.gmin[k] = s->pm.gmin[k]; R.pm.gdim[k] = s->pm.gdim[k]; }
+        R.pm.gcell = s->pm.gcell;
+        R.pm.np_emit = s->pm.np_emit;
+        R.pm.n = s->pm.n;
+    }
+    return YG_OK;
+}
+
+__global__ __launch_bounds__(64) void k_photon_direct(const RenderScene* __restrict__ scenes,
+                                                      const Area* __restrict__ areas, int nareas,
+                                                      uint32_t nitems, const int32_t* __restrict__ item_ti,
+                                                      const float* __restrict__ item_t,
+                                                      c3* __restrict__ lightc, int nl_max);
+
+// Launch the whole pipeline over a set of planned scenes (one launch per stage).
+static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render_stats* stats,
+                        bool timing)
+{
+    int rc;
+    if ((rc = perm_upload())) return rc;
+    if (!g_ws.pow_ready) {
+        uint64_t pw[64];
+        lcg_pow_table(pw);
+        YG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(YG_LCG_POW), pw, sizeof(pw)));
+        g_ws.pow_ready = true;
+    }
+    // concatenate scenes
+    std::vector<RenderScene> scenes;
+    std::vector<Area> areas;
+    std::vector<DevLight> lights;
+    std::vector<f3> apts, ajh, ajv;
+    std::vector<float> halton;
+    uint32_t nitems = 0;
+    int S = 0, D = 0, nl_max = 1;
+    bool any_path = false, any_photon = false, any_area = false;
+    for (auto& P : plans) {
+        RenderScene R = P.R;
+        R.item_base = nitems;
+        R.area_base = (uint32_t)areas.size();
+        for (auto L : P.lights) {
+            if (L.type == YG_LIGHT_AREA) { L.pts += (int)apts.size(); any_area = true; }
+            lights.push_back(L);
+        }
+        R.lights = (const DevLight*)(uintptr_t)(lights.size() - P.lights.size());  // offset, fixed below
+        R.apts = nullptr;
+        apts.insert(apts.end(), P.apts.begin(), P.apts.end());
+        ajh.insert(ajh.end(), P.ajh.begin(), P.ajh.end());
+        ajv.insert(ajv.end(), P.ajv.begin(), P.ajv.end());
+        R.halton = (const float*)(uintptr_t)halton.size();
+        halton.insert(halton.end(), P.halton.begin(), P.halton.end());
+        for (auto a : P.areas) {
+            a.scene = (int)scenes.size();
+            a.item_start += nitems;
+            areas.push_back(a);
+        }
+        nitems += P.nitems;
+        nl_max = std::max(nl_max, R.nl);
+        if (R.path_light >= 0) {
+            if (any_path && (R.samples != S || R.depth != D))
+                return yg_fail(YG_ERR_UNSUPPORTED, "batched pathlights need equal samples and depth");
+            any_path = true;
+            S = R.samples;
+            D = R.depth;
+        }
+        if (R.photon_light >= 0) any_photon = true;
+        scenes.push_back(R);
+    }
+    if (!any_path) { S = 1; D = 1; }
+    if (D == 0) D = 1;
+    RenderWS& W = g_ws;
+    if ((rc = W.lights.ensure(lights.size()))) return rc;
+    if ((rc = W.apts.ensure(apts.size()))) return rc;
+    if ((rc = W.ajh.ensure(ajh.size()))) return rc;
+    if ((rc = W.ajv.ensure(ajv.size()))) return rc;
+    if ((rc = W.halton.ensure(halton.size()))) return rc;
+    for (auto& R : scenes) {
+        R.lights = W.lights.p + (uintptr_t)R.lights;
+        R.apts = W.apts.p;
+        R.ajh = W.ajh.p;
+        R.ajv = W.ajv.p;
+        R.halton = W.halton.p + (uintptr_t)R.halton;
+    }
+    const uint64_t npaths64 = (uint64_t)nitems * S;
+    if (npaths64 >= 0xffffffffull) return yg_fail(YG_ERR_UNSUPPORTED, "too many path samples in one launch");
+    const uint32_t npaths = (uint32_t)npaths64;
+    if ((rc = W.scenes.ensure(scenes.size()))) return rc;
+    if ((rc = W.areas.ensure(areas.size()))) return rc;
+    if ((rc = W.item_ti.ensure(nitems))) return rc;
+    if ((rc = W.item_t.ensure(nitems))) return rc;
+    if ((rc = W.draws.ensure(nitems))) return rc;
+    if ((rc = W.lcg_off.ensure(nitems))) return rc;
+    if ((rc = W.qual.ensure(nitems))) return rc;
+    if ((rc = W.qoff.ensure(nitems))) return rc;
+    if ((rc = W.lightc.ensure((size_t)nitems * nl_max))) return rc;
+    if ((rc = W.cnt.ensure(1))) return rc;
+    if (any_path) {
+        if ((rc = W.active0.ensure(npaths))) return rc;
+        if ((rc = W.active1.ensure(npaths))) return rc;
+        if ((rc = W.flag.ensure(npaths))) return rc;
+        if ((rc = W.ex.ensure(npaths))) return rc;
+        if ((rc = W.hidx.ensure(npaths))) return rc;
+        if ((rc = W.st_where.ensure(npaths))) return rc;
+        if ((rc = W.st_hn.ensure(npaths))) return rc;
+        if ((rc = W.st_ti.ensure(npaths))) return rc;
+        if ((rc = W.st_col.ensure(npaths))) return rc;
+        if ((rc = W.st_s.ensure(npaths))) return rc;
+        if ((rc = W.contri.ensure((size_t)npaths * D))) return rc;
+        if ((rc = W.nseg.ensure(npaths))) return rc;
+    }
+    YG_HIP(hipMemcpyAsync(W.scenes.p, scenes.data(), sizeof(RenderScene) * scenes.size(), hipMemcpyHostToDevice, st));
+    YG_HIP(hipMemcpyAsync(W.areas.p, areas.data(), sizeof(Area) * areas.size(), hipMemcpyHostToDevice, st));
+    if (!lights.empty())
+        YG_HIP(hipMemcpyAsync(W.lights.p, lights.data(), sizeof(DevLight) * lights.size(), hipMemcpyHostToDevice, st));
+    if (!apts.empty()) {
+        YG_HIP(hipMemcpyAsync(W.apts.p, apts.data(), sizeof(f3) * apts.size(), hipMemcpyHostToDevice, st));
+        YG_HIP(hipMemcpyAsync(W.ajh.p, ajh.data(), sizeof(f3) * ajh.size(), hipMemcpyHostToDevice, st));
+        YG_HIP(hipMemcpyAsync(W.ajv.p, ajv.data(), sizeof(f3) * ajv.size(), hipMemcpyHostToDevice, st));
+    }
+    if (!halton.empty())
+        YG_HIP(hipMemcpyAsync(W.halton.p, halton.data(), sizeof(float) * halton.size(), hipMemcpyHostToDevice, st));
+    YG_HIP(hipMemsetAsync(W.cnt.p, 0, sizeof(Counters), st));
+
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    if (timing) {
+        YG_HIP(hipEventCreate(&e0));
+        YG_HIP(hipEventCreate(&e1));
+        YG_HIP(hipEventCreate(&e2));
+        YG_HIP(hipEventRecord(e0, st));
+    }
+    const int na = (int)areas.size();
+    hipLaunchKernelGGL(k_primary, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
+                       nitems, W.item_ti.p, W.item_t.p);
+    YG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_prep, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
+                       nitems, W.item_ti.p, W.item_t.p, W.draws.p, W.qual.p);
+    YG_HIP(hipGetLastError());
+    if (any_area && (rc = exclusive_scan(W.draws.p, W.lcg_off.p, nitems, st))) return rc;
+    hipLaunchKernelGGL(k_direct, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
+                       nitems, W.item_ti.p, W.item_t.p, W.lcg_off.p, W.lightc.p, nl_max, W.cnt.p);
+    YG_HIP(hipGetLastError());
+    if (any_photon) {
+        hipLaunchKernelGGL(k_photon_direct, dim3(grid_for(nitems, 64)), dim3(64), 0, st, W.scenes.p,
+                           W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lightc.p, nl_max);
+        YG_HIP(hipGetLastError());
+    }
+    if (any_path) {
+        if ((rc = exclusive_scan(W.qual.p, W.qoff.p, nitems, st))) return rc;
+        uint32_t last_q = 0, last_off = 0;
+        YG_HIP(hipMemcpyAsync(&last_q, W.qual.p + nitems - 1, 4, hipMemcpyDeviceToHost, st));
+        YG_HIP(hipMemcpyAsync(&last_off, W.qoff.p + nitems - 1, 4, hipMemcpyDeviceToHost, st));
+        YG_HIP(hipStreamSynchronize(st));
+        uint32_t nactive = (last_q + last_off) * (uint32_t)S;
+        hipLaunchKernelGGL(k_path_init, dim3(grid_for(nitems, 256)), dim3(256), 0, st, nitems, S, W.qual.p,
+                           W.qoff.p, W.active0.p);
+        YG_HIP(hipGetLastError());
+        uint32_t* act = W.active0.p;
+        uint32_t* nxt = W.active1.p;
+        for (int j = 0; j < D && nactive > 0; j++) {
+            YG_HIP(hipMemsetAsync(W.flag.p, 0, sizeof(uint32_t) * npaths, st));
+            hipLaunchKernelGGL(k_bounce, dim3(grid_for(nactive, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p,
+                               na, j, act, nactive, S, D, W.item_ti.p, W.item_t.p, W.qoff.p, W.hidx.p,
+                               W.st_where.p, W.st_hn.p, W.st_ti.p, W.st_col.p, W.st_s.p, W.contri.p,
+                               W.nseg.p, W.flag.p, W.cnt.p);
+            YG_HIP(hipGetLastError());
+            if (j + 1 >= D) break;
+            if ((rc = exclusive_scan(W.flag.p, W.ex.p, npaths, st))) return rc;
+            hipLaunchKernelGGL(k_compact, dim3(grid_for(npaths, 256)), dim3(256), 0, st, W.areas.p, na, npaths,
+                               S, W.flag.p, W.ex.p, nxt, W.hidx.p);
+            YG_HIP(hipGetLastError());
+            uint32_t lf = 0, le = 0;
+            YG_HIP(hipMemcpyAsync(&lf, W.flag.p + npaths - 1, 4, hipMemcpyDeviceToHost, st));
+            YG_HIP(hipMemcpyAsync(&le, W.ex.p + npaths - 1, 4, hipMemcpyDeviceToHost, st));
+            YG_HIP(hipStreamSynchronize(st));
+            nactive = lf + le;
+            std::swap(act, nxt);
+        }
+    }
+    if (timing) YG_HIP(hipEventRecord(e1, st));
+    hipLaunchKernelGGL(k_finalize, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
+                       nitems, S, D, W.item_ti.p, W.item_t.p, W.qual.p, W.lightc.p, nl_max, W.contri.p,
+                       W.nseg.p, W.cnt.p);
+    YG_HIP(hipGetLastError());
+    if (stats) {
+        Counters c;
+        YG_HIP(hipMemcpyAsync(&c, W.cnt.p, sizeof(c), hipMemcpyDeviceToHost, st));
+        if (timing) YG_HIP(hipEventRecord(e2, st));
+        YG_HIP(hipStreamSynchronize(st));
+        memset(stats, 0, sizeof(*stats));
+        stats->rays_camera = nitems;
+        stats->rays_extension = c.ext;
+        stats->rays_shadow = c.shadow;
+        stats->adaptive_extra = c.extra;
+        if (timing) {
+            float ms = 0, ms2 = 0;
+            YG_HIP(hipEventElapsedTime(&ms, e0, e2));
+            YG_HIP(hipEventElapsedTime(&ms2, e0, e1));
+            stats->ms = ms;
+            stats->ms_trace = ms2;
+        }
+    }
+    if (timing) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipEventDestroy(e2);
+    }
+    return YG_OK;
+}
+
+extern "C" int yg_render(yg_scene* s, const yg_render_opts* opts, float* d_rgb, yg_render_stats* stats,
+                         void* stream)
+{
+    std::vector<ScenePlan> plans(1);
+    int rc = plan_scene(s, opts, d_rgb, plans[0]);
+    if (rc) return rc;
+    return render_plans(plans, (hipStream_t)stream, stats, opts && (opts->flags & YG_RENDER_TIMING));
+}
+
+// Batched render: several scenes in one set of launches (yg_render per scene
+// otherwise).  d_rgb[k] receives scene k's image.
+extern "C" int yg_render_batch(yg_scene** scenes, uint32_t n, const yg_render_opts* opts, float** d_rgb,
+                               yg_render_stats* stats, void* stream)
+{
+    if (!scenes || !opts || !d_rgb || n == 0) return yg_fail(YG_ERR_ARG, "null argument");
+    std::vector<ScenePlan> plans(n);
+    for (uint32_t k = 0; k < n; k++) {
+        int rc = plan_scene(scenes[k], &opts[k], d_rgb[k], plans[k]);
+        if (rc) return rc;
+    }
+    bool timing = false;
+    for (uint32_t k = 0; k < n; k++) timing = timing || (opts[k].flags & YG_RENDER_TIMING);
+    return render_plans(plans, (hipStream_t)stream, stats, timing);
+}
+
+// photonLight_t::illuminate at every primary hit (photonlight.cc:297-395)
+__global__ __launch_bounds__(64) void k_photon_direct(const RenderScene* __restrict__ scenes,
+                                                      const Area* __restrict__ areas, int nareas,
+                                                      uint32_t nitems, const int32_t* __restrict__ item_ti,
+                                                      const float* __restrict__ item_t,
+                                                      c3* __restrict__ lightc, int nl_max)
+{
+    __shared__ float hd[YG_KNN_MAX * 64];
+    __shared__ int hi[YG_KNN_MAX * 64];
+    const uint32_t item = blockIdx.x * 64 + threadIdx.x;
+    if (item >= nitems) return;
+    const int ti = item_ti[item];
+    if (ti < 0) return;
+    const Area a = areas[find_area(areas, nareas, item)];
+    const RenderScene& R = scenes[a.scene];
+    if (R.photon_light < 0) return;
+    const DevLight& L = R.lights[R.photon_light];
+    if (!L.use_in_render) return;
+    f3 eye;
+    const Surf sp = primary_surf(R, a, item, ti, item_t[item], eye);
+    const bool receive = (R.obj_flags[sp.obj] & 2) != 0;
+    lightc[(size_t)item * nl_max + R.photon_light] =
+        photon_estimate(R, R.pm, L.K, L.fixedradius, sp, eye, receive, hd + threadIdx.x, hi + threadIdx.x, 64);
+}

@@ -1,0 +1,190 @@
+// Device-side shading for the render pipeline: surface points, the generic
+// diffuse shader, sun and area lights, and the pathlight hemisphere sampler.
+#pragma once
+
+#include "noise_dev.h"
+#include "traverse.h"
+
+// One light of the render's light table (see yg_light); derived quantities
+// are precomputed on the host with the reference's float operations.
+struct DevLight {
+    int type, use_in_render, use_in_indirect, cast_shadows;
+    f3 dir;       // sun: normalised direction; area: plane normal (b-a)^(d-a)
+    c3 col;       // sun: colour*power; area: pow*colour
+    f3 from;      // area centre
+    int samples;  // area sample count after fillQuad
+    int pts;      // area: first entry of the sample point / jitter arrays
+    int K;        // photon: search count
+    float fixedradius;
+};
+
+// Photon map view for the K-nearest gather.
+struct PhotonView {
+    const float* photons;   // 9 floats per photon: dir, pos, colour
+    const int32_t* gstart;  // uniform grid cell ranges
+    const int32_t* gitems;
+    float gmin[3];
+    float gcell;
+    int gdim[3];
+    int np_emit;
+    int64_t n;
+};
+
+// Everything a render kernel needs about one scene.
+struct RenderScene {
+    BVHView bvh;
+    const yg_material* mats;
+    const yg_modulator* mods;
+    const yg_texture* texs;
+    const DevLight* lights;
+    const f3* apts;          // area-light sample points
+    const f3* ajh;           // area-light horizontal jitter vectors
+    const f3* ajv;           // area-light vertical jitter vectors
+    const int32_t* obj_flags;
+    int nl;
+    f3 cam_pos, vright, vup, vto;
+    int resx, resy;
+    float min_raydis, self_bias;
+    int raydepth;
+    c3 bg;
+    int path_light;          // index of the pathlight or -1
+    int samples, depth;
+    float path_power, threshold;
+    const float* halton;     // [2*(depth+1)][hstride] getNext() tables
+    int hstride;
+    uint32_t lcg_seed;
+    int photon_light;        // index of the photon light or -1
+    PhotonView pm;
+    float* out_rgb;
+    uint32_t item_base;      // first item of this scene
+    uint32_t area_base;      // first area of this scene
+};
+
+// A render area (renderArea_t): the extended rectangle x,y,w,h is rendered,
+// the real rectangle rx,ry,rw,rh is written out (renderblock.cc:8-20, 71-101).
+struct Area {
+    int scene;
+    int x, y, w, h, rx, ry, rw, rh;
+    uint32_t item_start;
+};
+
+// surfacePoint_t subset for a hit (mesh.cc:517-577, triangle.cc:138-141):
+// P = from + Z*ray; N = Ng = face normal; NU, NV = createCS(N).
+struct Surf {
+    f3 P, N, Ng, NU, NV;
+    int obj, mat, ti;
+};
+
+__device__ __forceinline__ Surf make_surf(const BVHView& bvh, int ti, f3 f, f3 ray, float Z)
+{
+    Surf s;
+    s.ti = ti;
+    s.P = f + ray * Z;
+    const float4 A = bvh.tris[3 * ti], B = bvh.tris[3 * ti + 1], C = bvh.tris[3 * ti + 2];
+    s.Ng = mk3(A.w, B.w, C.w);
+    s.N = s.Ng;
+    create_cs(s.N, s.NU, s.NV);
+    s.obj = bvh.tobj[ti];
+    s.mat = bvh.tmat[ti];
+    return s;
+}
+
+// genericShader_t::fromLight, diffuse part (basicshaders.cc:46-70)
+__device__ __forceinline__ c3 from_light(const RenderScene& R, const Surf& sp, f3 ldir, c3 lcol, f3 eye)
+{
+    const f3 edir = norm3(eye);
+    const f3 N = face_forward(sp.Ng, sp.N, edir);
+    const float inte = dot(N, ldir);
+    if (inte < 0) return mkc(0, 0, 0);
+    const c3 lkc = get_diffuse(R.mats, R.mods, R.texs, sp.mat, sp.P);
+    return cmul(cscale(lcol, inte), lkc);
+}
+
+// sunLight_t::illuminate (sunlight.cc:35-57)
+__device__ __forceinline__ c3 sun_illuminate(const RenderScene& R, const DevLight& L, const Surf& sp,
+                                             f3 eye, LaneStack& st, uint32_t& nshadow)
+{
+    if (L.cast_shadows) {
+        nshadow++;
+        if (is_shadowed(R.bvh, sp.P, L.dir, sp.obj, false, R.min_raydis, R.self_bias, st))
+            return mkc(0, 0, 0);
+    }
+    return from_light(R, sp, L.dir, L.col, eye);
+}
+
+// ourRandom() (vector3d.h:288-297)
+__device__ __forceinline__ float our_random(int32_t& seed)
+{
+    const int a = 16807, m = 2147483647, q = m / a, r = m % a;
+    int32_t s = seed;
+    s = a * (s % q) - r * (s / q);
+    if (s < 0) s += m;
+    seed = s;
+    return (float)s / (float)m;
+}
+
+// state of ourRandom() after n further draws: seed * 16807^n mod (2^31-1)
+__device__ __forceinline__ int32_t lcg_jump(uint32_t seed, uint64_t n, const uint64_t* pow2)
+{
+    const uint64_t m = 2147483647u;
+    uint64_t acc = seed;
+    for (int k = 0; n; k++, n >>= 1)
+        if (n & 1) acc = (acc * pow2[k]) % m;
+    return (int32_t)acc;
+}
+
+// areaLight_t::illuminate, PENUMBRA branch (arealight.cc:93-174), draws taken
+// from the ourRandom() stream starting at *seed
+__device__ __forceinline__ c3 area_illuminate(const RenderScene& R, const DevLight& L, const Surf& sp,
+                                              f3 eye, int32_t& seed, LaneStack& st, uint32_t& nshadow)
+{
+    c3 resul = mkc(0, 0, 0);
+    const f3 edir = norm3(eye);
+    const f3 N = face_forward(sp.Ng, sp.N, edir);
+    const float plane_at = dot(sp.P - L.from, L.dir);
+    if (plane_at <= 0) return mkc(0, 0, 0);
+    for (int i = 0; i < L.samples; ++i) {
+        const float dish = (float)((double)our_random(seed) - 0.5);
+        const float disv = (float)((double)our_random(seed) - 0.5);
+        const f3 sampleP = (R.apts[L.pts + i] + R.ajh[L.pts + i] * dish) + R.ajv[L.pts + i] * disv;
+        const f3 Lv = sampleP - sp.P;
+        if (dot(Lv, N) < 0) continue;
+        nshadow++;
+        if (!is_shadowed(R.bvh, sp.P, sampleP, sp.obj, true, R.min_raydis, R.self_bias, st)) {
+            f3 dir = Lv;
+            float LD2 = normlensqr3(dir);
+            LD2 = (LD2 != 0) ? (1.0f / LD2) : 1.0f;
+            resul = cadd(resul, from_light(R, sp, dir, cscale(L.col, LD2), eye));
+        }
+    }
+    const c3 r = cscale(resul, plane_at);
+    const float fs = (float)L.samples;
+    return mkc(r.r / fs, r.g / fs, r.b / fs);
+}
+
+// haltonSampler_t::nextDirection (pathtools.cc:32-43) with the two Halton
+// values already fetched
+__device__ __forceinline__ f3 hemi_direction(float z1, float z2, f3 N, f3 Ru, f3 Rv)
+{
+    if (z1 > 1.0f) z1 = 1.0f;
+    z2 = (float)((double)z2 * (2.0 * M_PI));
+    const f3 a = Ru * dcos(z2) + Rv * dsin(z2);
+    return a * (float)sqrt(1.0 - (double)z1) + N * sqrtf(z1);
+}
+
+// scene_t::light with indirect = true restricted to sun lights (the only
+// lights the GPU path supports inside GI paths): the getLight of
+// pathlight.cc:186-191 at a path vertex, eye = (P + (-ray)) - P.
+__device__ __forceinline__ c3 indirect_light(const RenderScene& R, const Surf& sp, f3 ray, LaneStack& st,
+                                             uint32_t& nshadow)
+{
+    const f3 from = sp.P + (-ray);
+    const f3 eye = from - sp.P;
+    c3 fl = mkc(0, 0, 0);
+    for (int li = 0; li < R.nl; li++) {
+        const DevLight& L = R.lights[li];
+        if (!L.use_in_indirect) continue;
+        if (L.type == YG_LIGHT_SUN) fl = cadd(fl, sun_illuminate(R, L, sp, eye, st, nshadow));
+    }
+    return fl;
+}

@@ -1,0 +1,232 @@
+// Ray-query entry points of libyafgpu: closest / any hit over ray batches,
+// scene_t::firstHit / isShadowed semantics, noise evaluation and heightfield
+// generation on the device.
+#include "noise_dev.h"
+#include "traverse.h"
+
+#define TB 128
+
+// --------------------------------------------------------------- kernels --
+
+template <bool COUNT>
+__global__ __launch_bounds__(TB) void k_trace_closest(BVHView bvh, const yg_ray* __restrict__ rays,
+                                                       uint32_t n, yg_hit* __restrict__ hits,
+                                                       unsigned long long* counters)
+{
+    __shared__ int s_stk[YG_STACK_LDS * TB];
+    const uint32_t i = blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    LaneStack st;
+    st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = TB; st.sp = 0;
+    const yg_ray r = rays[i];
+    const f3 o = mk3(r.ox, r.oy, r.oz), d = mk3(r.dx, r.dy, r.dz);
+    float t = r.tmax;
+    TraceStats cs = {0, 0};
+    const int ti = trace_closest<COUNT>(bvh, o, d, r.tmin, t, st, &cs);
+    yg_hit h;
+    if (ti >= 0) {
+        f3 a, b, c, nn;
+        load_tri(bvh.tris, ti, a, b, c, nn);
+        const f3 hp = o + d * t;  // mesh.cc:570
+        tri_bary(nn, a, b, c, hp, h.u, h.v);
+        h.prim = bvh.tprim[ti];
+        h.t = t;
+    } else {
+        h.prim = -1; h.t = r.tmax; h.u = 0; h.v = 0;
+    }
+    hits[i] = h;
+    if (COUNT) {
+        atomicAdd(&counters[0], (unsigned long long)cs.nodes);
+        atomicAdd(&counters[1], (unsigned long long)cs.tris);
+    }
+}
+
+__global__ __launch_bounds__(TB) void k_trace_any(BVHView bvh, const yg_ray* __restrict__ rays,
+                                                  uint32_t n, uint8_t* __restrict__ occ)
+{
+    __shared__ int s_stk[YG_STACK_LDS * TB];
+    const uint32_t i = blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    LaneStack st;
+    st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = TB; st.sp = 0;
+    const yg_ray r = rays[i];
+    const f3 o = mk3(r.ox, r.oy, r.oz), d = mk3(r.dx, r.dy, r.dz);
+    const float blo = r.tmin - 1e-4f * fabsf(r.tmin) - 1e-30f;
+    occ[i] = trace_any<false>(bvh, o, d, r.tmin, r.tmax, -1, o, blo, r.tmax, st) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(TB) void k_first_hit(BVHView bvh, const float* __restrict__ from,
+                                                  const float* __restrict__ dir, uint32_t n,
+                                                  float min_raydis, yg_hit* __restrict__ hits)
+{
+    __shared__ int s_stk[YG_STACK_LDS * TB];
+    const uint32_t i = blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    LaneStack st;
+    st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = TB; st.sp = 0;
+    const f3 fr = ld3(from + 3 * (size_t)i), d = ld3(dir + 3 * (size_t)i);
+    float t;
+    const int ti = first_hit(bvh, fr, d, min_raydis, t, st);
+    yg_hit h;
+    h.prim = (ti >= 0) ? bvh.tprim[ti] : -1;
+    h.t = t;
+    h.u = 0; h.v = 0;
+    hits[i] = h;
+}
+
+__global__ __launch_bounds__(TB) void k_is_shadowed(BVHView bvh, const float* __restrict__ p,
+                                                    const float* __restrict__ l,
+                                                    const int32_t* __restrict__ self_obj,
+                                                    uint32_t n, int to_point, float min_raydis,
+                                                    float self_bias, uint8_t* __restrict__ out)
+{
+    __shared__ int s_stk[YG_STACK_LDS * TB];
+    const uint32_t i = blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    LaneStack st;
+    st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = TB; st.sp = 0;
+    const int so = self_obj ? self_obj[i] : -1;
+    out[i] = is_shadowed(bvh, ld3(p + 3 * (size_t)i), ld3(l + 3 * (size_t)i), so, to_point != 0,
+                         min_raydis, self_bias, st) ? 1 : 0;
+}
+
+__global__ void k_noise_eval(int what, yg_texture tex, const float* __restrict__ pts, uint32_t n,
+                             float* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const f3 p = ld3(pts + 3 * (size_t)i);
+    switch (what) {
+    case YG_EVAL_NOISE: out[i] = new_perlin(p.x, p.y, p.z); break;
+    case YG_EVAL_FBM: out[i] = fbm(p.x, p.y, p.z, tex.H, tex.lacunarity, tex.octaves); break;
+    case YG_EVAL_TURBULENCE: out[i] = turbulence(p.x, p.y, p.z, tex.depth, tex.size, tex.hard); break;
+    case YG_EVAL_TEX_FLOAT: out[i] = tex_float(tex, p); break;
+    case YG_EVAL_TEX_COLOR: {
+        const c3 c = tex_color(tex, p);
+        out[3 * (size_t)i] = c.r; out[3 * (size_t)i + 1] = c.g; out[3 * (size_t)i + 2] = c.b;
+        break;
+    }
+    }
+}
+
+__global__ void k_heightfield(int nx, int ny, float ex, float ey, float amp, float freq, float H,
+                              float lac, float oct, float* __restrict__ xyz)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)nx * ny) return;
+    const int i = (int)(k % nx), j = (int)(k / nx);
+    const float x = ex * (float)i / (float)(nx - 1);
+    const float y = ey * (float)j / (float)(ny - 1);
+    const float z = amp * fbm(freq * x, freq * y, 0.0f, H, lac, oct);
+    xyz[3 * k] = x; xyz[3 * k + 1] = y; xyz[3 * k + 2] = z;
+}
+
+// ------------------------------------------------------------- C entry --
+
+static int check_scene(yg_scene* s)
+{
+    if (!s) return yg_fail(YG_ERR_ARG, "null scene");
+    if (!s->built) return yg_fail(YG_ERR_STATE, "scene not built (call yg_build)");
+    return YG_OK;
+}
+
+extern "C" int yg_trace_closest(yg_scene* s, const yg_ray* d_rays, uint32_t n, yg_hit* d_hits,
+                                void* stream)
+{
+    int rc = check_scene(s);
+    if (rc) return rc;
+    if (n == 0) return YG_OK;
+    if (!d_rays || !d_hits) return yg_fail(YG_ERR_ARG, "null buffer");
+    hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid_for(n, TB)), dim3(TB), 0,
+                       (hipStream_t)stream, s->view(), d_rays, n, d_hits, nullptr);
+    YG_HIP(hipGetLastError());
+    return YG_OK;
+}
+
+// Closest-hit with per-query node-visit and triangle-test totals (counters[2]).
+extern "C" int yg_trace_closest_counted(yg_scene* s, const yg_ray* d_rays, uint32_t n,
+                                        yg_hit* d_hits, unsigned long long* d_counters,
+                                        void* stream)
+{
+    int rc = check_scene(s);
+    if (rc) return rc;
+    if (n == 0) return YG_OK;
+    hipLaunchKernelGGL(k_trace_closest<true>, dim3(grid_for(n, TB)), dim3(TB), 0,
+                       (hipStream_t)stream, s->view(), d_rays, n, d_hits, d_counters);
+    YG_HIP(hipGetLastError());
+    return YG_OK;
+}
+
+extern "C" int yg_trace_any(yg_scene* s, const yg_ray* d_rays, uint32_t n, uint8_t* d_occluded,
+                            void* stream)
+{
+    int rc = check_scene(s);
+    if (rc) return rc;
+    if (n == 0) return YG_OK;
+    if (!d_rays || !d_occluded) return yg_fail(YG_ERR_ARG, "null buffer");
+    hipLaunchKernelGGL(k_trace_any, dim3(grid_for(n, TB)), dim3(TB), 0, (hipStream_t)stream,
+                       s->view(), d_rays, n, d_occluded);
+    YG_HIP(hipGetLastError());
+    return YG_OK;
+}
+
+extern "C" int yg_first_hit(yg_scene* s, const float* d_from, const float* d_dir, uint32_t n,
+                            float min_raydis, yg_hit* d_hits, void* stream)
+{
+    int rc = check_scene(s);
+    if (rc) return rc;
+    if (n == 0) return YG_OK;
+    if (!d_from || !d_dir || !d_hits) return yg_fail(YG_ERR_ARG, "null buffer");
+    hipLaunchKernelGGL(k_first_hit, dim3(grid_for(n, TB)), dim3(TB), 0, (hipStream_t)stream,
+                       s->view(), d_from, d_dir, n, min_raydis, d_hits);
+    YG_HIP(hipGetLastError());
+    return YG_OK;
+}
+
+extern "C" int yg_is_shadowed(yg_scene* s, const float* d_p, const float* d_l,
+                              const int32_t* d_self_obj, uint32_t n, int32_t to_point,
+                              float min_raydis, float self_bias, uint8_t* d_out, void* stream)
+{
+    int rc = check_scene(s);
+    if (rc) return rc;
+    if (n == 0) return YG_OK;
+    if (!d_p || !d_l || !d_out) return yg_fail(YG_ERR_ARG, "null buffer");
+    hipLaunchKernelGGL(k_is_shadowed, dim3(grid_for(n, TB)), dim3(TB), 0, (hipStream_t)stream,
+                       s->view(), d_p, d_l, d_self_obj, n, to_point, min_raydis, self_bias, d_out);
+    YG_HIP(hipGetLastError());
+    return YG_OK;
+}
+
+extern "C" int yg_noise_eval(int32_t what, const yg_texture* tex, const float* d_pts, uint32_t n,
+                             float* d_out, void* stream)
+{
+    if (what < YG_EVAL_NOISE || what > YG_EVAL_TEX_COLOR) return yg_fail(YG_ERR_ARG, "bad eval kind");
+    yg_texture t{};
+    if (tex) t = *tex;
+    if ((what >= YG_EVAL_TEX_FLOAT) && !tex) return yg_fail(YG_ERR_ARG, "texture required");
+    if (tex && t.noise != YG_NOISE_NEWPERLIN)
+        return yg_fail(YG_ERR_UNSUPPORTED, "only newPerlin_t noise is implemented");
+    if (what >= YG_EVAL_TEX_FLOAT && (t.type < YG_TEX_CLOUDS || t.type > YG_TEX_MUSGRAVE))
+        return yg_fail(YG_ERR_ARG, "bad texture type");
+    int rc = perm_upload();
+    if (rc) return rc;
+    if (n == 0) return YG_OK;
+    hipLaunchKernelGGL(k_noise_eval, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                       what, t, d_pts, n, d_out);
+    YG_HIP(hipGetLastError());
+    return YG_OK;
+}
+
+extern "C" int yg_gen_heightfield(int32_t nx, int32_t ny, float ex, float ey, float amp,
+                                  float freq, float H, float lacunarity, float octaves,
+                                  float* d_xyz, void* stream)
+{
+    if (nx < 2 || ny < 2 || !d_xyz) return yg_fail(YG_ERR_ARG, "bad heightfield arguments");
+    int rc = perm_upload();
+    if (rc) return rc;
+    const uint64_t n = (uint64_t)nx * ny;
+    hipLaunchKernelGGL(k_heightfield, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                       nx, ny, ex, ey, amp, freq, H, lacunarity, octaves, d_xyz);
+    YG_HIP(hipGetLastError());
+    return YG_OK;
+}

@@ -1,0 +1,250 @@
+// Device-side ray queries over the BVH: the reference triangle test and the
+// closest / any-hit acceptance rules of kdTree_t::Intersect / IntersectS.
+#pragma once
+
+#include "yg_internal.h"
+
+// triangle_t::hit (triangle.h:48-58): flip the ray to the normal's side, then
+// the three edge triple products must be >= 0.  Shared edges give exactly
+// opposite products, so no ray slips between two triangles.
+__device__ __forceinline__ bool tri_hit(f3 a, f3 b, f3 c, f3 n, f3 from, f3 ray)
+{
+    const f3 va = a - from, vb = b - from, vc = c - from;
+    f3 r = (dot(ray, n) < 0) ? -ray : ray;
+    if (dot(cross(va, vb), r) < 0) return false;
+    if (dot(cross(vb, vc), r) < 0) return false;
+    if (dot(cross(vc, va), r) < 0) return false;
+    return true;
+}
+
+// triangle_t::intersect (triangle.h:83-86)
+__device__ __forceinline__ float tri_t(f3 a, f3 n, f3 from, f3 ray) { return dot(n, a - from) / dot(n, ray); }
+
+// getInterpolation (triangle.cc:74-136): fb, fc of the dominant-axis projection
+__device__ __forceinline__ void tri_bary(f3 n, f3 a, f3 b, f3 c, f3 h, float& fb, float& fc)
+{
+    float uu0, vv0, uu1, vv1, uu2, vv2;
+    float nx = fabsf(n.x), ny = fabsf(n.y), nz = fabsf(n.z);
+    if (nx >= ny && nx >= nz) {
+        uu0 = h.y - a.y; vv0 = h.z - a.z; uu1 = b.y - a.y; vv1 = b.z - a.z; uu2 = c.y - a.y; vv2 = c.z - a.z;
+    } else if (ny >= nx && ny >= nz) {
+        uu0 = h.x - a.x; vv0 = h.z - a.z; uu1 = b.x - a.x; vv1 = b.z - a.z; uu2 = c.x - a.x; vv2 = c.z - a.z;
+    } else {
+        uu0 = h.x - a.x; vv0 = h.y - a.y; uu1 = b.x - a.x; vv1 = b.y - a.y; uu2 = c.x - a.x; vv2 = c.y - a.y;
+    }
+    float alpha_numer = uu0 * vv2 - vv0 * uu2;
+    float beta_numer = uu1 * vv0 - vv1 * uu0;
+    float denum = uu1 * vv2 - vv1 * uu2;
+    fb = alpha_numer / denum;
+    fc = beta_numer / denum;
+}
+
+__device__ __forceinline__ void load_tri(const float4* tris, int ti, f3& a, f3& b, f3& c, f3& n)
+{
+    const float4 A = tris[3 * ti], B = tris[3 * ti + 1], C = tris[3 * ti + 2];
+    a = mk3(A.x, A.y, A.z);
+    b = mk3(B.x, B.y, B.z);
+    c = mk3(C.x, C.y, C.z);
+    n = mk3(A.w, B.w, C.w);
+}
+
+// Per-lane traversal stack: the top YG_STACK_LDS entries live in LDS
+// (entry k of lane t at lds[k*nlanes + t], conflict free), deeper ones spill
+// to a private array that only very deep trees touch.
+struct LaneStack {
+    int* lds;
+    int lane, nlanes, sp;
+    int spill[YG_STACK_SPILL];
+    __device__ void push(int v)
+    {
+        if (sp < YG_STACK_LDS) lds[sp * nlanes + lane] = v;
+        else spill[sp - YG_STACK_LDS] = v;
+        sp++;
+    }
+    __device__ int pop()
+    {
+        sp--;
+        return (sp < YG_STACK_LDS) ? lds[sp * nlanes + lane] : spill[sp - YG_STACK_LDS];
+    }
+};
+
+struct RayBox {
+    float ix, iy, iz, ox, oy, oz;  // 1/d and o/d for the slab test
+};
+
+__device__ __forceinline__ RayBox make_raybox(f3 o, f3 d)
+{
+    RayBox r;
+    // a zero component gets a tiny same-signed stand-in: that slab then
+    // accepts everything on the origin's side, like bound_t::cross skipping the axis
+    float dx = (fabsf(d.x) > 1e-30f) ? d.x : copysignf(1e-30f, d.x);
+    float dy = (fabsf(d.y) > 1e-30f) ? d.y : copysignf(1e-30f, d.y);
+    float dz = (fabsf(d.z) > 1e-30f) ? d.z : copysignf(1e-30f, d.z);
+    r.ix = 1.0f / dx; r.iy = 1.0f / dy; r.iz = 1.0f / dz;
+    r.ox = o.x * r.ix; r.oy = o.y * r.iy; r.oz = o.z * r.iz;
+    return r;
+}
+
+// Slab test of both children of a node.  Boxes are conservatively inflated at
+// build time, so fused multiply-adds here cannot drop a hit.
+__device__ __forceinline__ void node_test(const GNode& n, const RayBox& r, float lo, float hi,
+                                          float& t0, float& t1, bool& h0, bool& h1)
+{
+    float a0 = __builtin_fmaf(n.a.x, r.ix, -r.ox), a1 = __builtin_fmaf(n.a.y, r.ix, -r.ox);
+    float a2 = __builtin_fmaf(n.a.z, r.iy, -r.oy), a3 = __builtin_fmaf(n.a.w, r.iy, -r.oy);
+    float a4 = __builtin_fmaf(n.c.x, r.iz, -r.oz), a5 = __builtin_fmaf(n.c.y, r.iz, -r.oz);
+    float b0 = __builtin_fmaf(n.b.x, r.ix, -r.ox), b1 = __builtin_fmaf(n.b.y, r.ix, -r.ox);
+    float b2 = __builtin_fmaf(n.b.z, r.iy, -r.oy), b3 = __builtin_fmaf(n.b.w, r.iy, -r.oy);
+    float b4 = __builtin_fmaf(n.c.z, r.iz, -r.oz), b5 = __builtin_fmaf(n.c.w, r.iz, -r.oz);
+    float n0 = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), lo));
+    float f0 = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), hi));
+    float n1 = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), lo));
+    float f1 = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), hi));
+    h0 = n0 <= f0;
+    h1 = n1 <= f1;
+    t0 = n0;
+    t1 = n1;
+}
+
+struct TraceStats {
+    uint32_t nodes, tris;
+};
+
+// Closest hit: accepts tmin <= t < tbest (kdtree.cc:690-694); exact ties go to
+// the lowest primitive id so the result does not depend on the tree.
+// Returns the leaf-ordered triangle index or -1; tbest holds the distance.
+template <bool COUNT>
+__device__ int trace_closest(const BVHView& bvh, f3 o, f3 d, float tmin, float& tbest,
+                             LaneStack& st, TraceStats* cs = nullptr)
+{
+    const RayBox rb = make_raybox(o, d);
+    const float lo = tmin - 1e-4f * fabsf(tmin) - 1e-30f;
+    int best = -1;
+    int bestprim = 0x7fffffff;
+    int node = 0;
+    st.sp = 0;
+    for (;;) {
+        if (node >= 0) {
+            const GNode n = bvh.nodes[node];
+            if (COUNT) cs->nodes++;
+            float t0, t1;
+            bool h0, h1;
+            node_test(n, rb, lo, tbest, t0, t1, h0, h1);
+            if (h0 && h1) {
+                int nearc = (t0 <= t1) ? n.d.x : n.d.y;
+                int farc = (t0 <= t1) ? n.d.y : n.d.x;
+                st.push(farc);
+                node = nearc;
+                continue;
+            }
+            if (h0) { node = n.d.x; continue; }
+            if (h1) { node = n.d.y; continue; }
+        } else {
+            const int code = ~node;
+            const int first = code >> 4, cnt = (code & 15) + 1;
+            for (int k = 0; k < cnt; k++) {
+                const int ti = first + k;
+                f3 a, b, c, nn;
+                load_tri(bvh.tris, ti, a, b, c, nn);
+                if (COUNT) cs->tris++;
+                if (!tri_hit(a, b, c, nn, o, d)) continue;
+                const float t = tri_t(a, nn, o, d);
+                if (!(t >= tmin)) continue;
+                if (t < tbest) {
+                    tbest = t;
+                    best = ti;
+                    bestprim = bvh.tprim[ti];
+                } else if (t == tbest && best >= 0) {
+                    const int p = bvh.tprim[ti];
+                    if (p < bestprim) { best = ti; bestprim = p; }
+                }
+            }
+        }
+        if (st.sp == 0) break;
+        node = st.pop();
+    }
+    return best;
+}
+
+// Any hit: accepts tmin < t < tmax (kdtree.cc:835-841).  Triangles of object
+// self_obj are tested from oself instead of o (scene_t::isShadowed's
+// self_bias rule, scene.cc:115-141); [blo, bhi] is the box interval in o's
+// parameter and must cover both origins' segments.
+template <bool COUNT>
+__device__ bool trace_any(const BVHView& bvh, f3 o, f3 d, float tmin, float tmax, int self_obj,
+                          f3 oself, float blo, float bhi, LaneStack& st, TraceStats* cs = nullptr)
+{
+    const RayBox rb = make_raybox(o, d);
+    int node = 0;
+    st.sp = 0;
+    for (;;) {
+        if (node >= 0) {
+            const GNode n = bvh.nodes[node];
+            if (COUNT) cs->nodes++;
+            float t0, t1;
+            bool h0, h1;
+            node_test(n, rb, blo, bhi, t0, t1, h0, h1);
+            if (h0 && h1) {
+                int nearc = (t0 <= t1) ? n.d.x : n.d.y;
+                int farc = (t0 <= t1) ? n.d.y : n.d.x;
+                st.push(farc);
+                node = nearc;
+                continue;
+            }
+            if (h0) { node = n.d.x; continue; }
+            if (h1) { node = n.d.y; continue; }
+        } else {
+            const int code = ~node;
+            const int first = code >> 4, cnt = (code & 15) + 1;
+            for (int k = 0; k < cnt; k++) {
+                const int ti = first + k;
+                f3 a, b, c, nn;
+                load_tri(bvh.tris, ti, a, b, c, nn);
+                if (COUNT) cs->tris++;
+                const f3 org = (self_obj >= 0 && bvh.tobj[ti] == self_obj) ? oself : o;
+                if (!tri_hit(a, b, c, nn, org, d)) continue;
+                const float t = tri_t(a, nn, org, d);
+                if (t > tmin && t < tmax) return true;
+            }
+        }
+        if (st.sp == 0) break;
+        node = st.pop();
+    }
+    return false;
+}
+
+// scene_t::firstHit (scene.cc:688-721): origin from + ray*min_raydis, closest
+// hit with Z > 0 (a zero-distance closest hit is rejected by the scene).
+__device__ __forceinline__ int first_hit(const BVHView& bvh, f3 from, f3 ray, float min_raydis,
+                                         float& t, LaneStack& st)
+{
+    const f3 f = from + ray * min_raydis;
+    t = INFINITY;
+    int ti = trace_closest<false>(bvh, f, ray, 0.0f, t, st);
+    if (ti >= 0 && !(t > 0.0f)) ti = -1;
+    return ti;
+}
+
+// scene_t::isShadowed, point form (scene.cc:107-146) or direction form (:148-185)
+__device__ __forceinline__ bool is_shadowed(const BVHView& bvh, f3 p, f3 l, int self_obj,
+                                            bool to_point, float min_raydis, float self_bias,
+                                            LaneStack& st)
+{
+    f3 ray;
+    float dist;
+    if (to_point) {
+        ray = l - p;
+        dist = len3(ray);
+    } else {
+        ray = l;
+        dist = INFINITY;
+    }
+    ray = norm3(ray);
+    const f3 self = p + ray * self_bias;
+    const f3 q = p + ray * min_raydis;
+    const float delta = self_bias - min_raydis;
+    const float slack = 1e-4f * (fabsf(self_bias) + fabsf(min_raydis)) + 1e-30f;
+    float blo = fminf(0.0f, delta) - slack;
+    float bhi = dist + fmaxf(0.0f, delta) + slack;
+    return trace_any<false>(bvh, q, ray, 0.0f, dist, self_obj, self, blo, bhi, st);
+}

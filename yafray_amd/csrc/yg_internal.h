@@ -1,0 +1,131 @@
+// Internal types of libyafgpu: scene handle, BVH layout, error plumbing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/yafgpu.h"
+#include "dev_math.h"
+
+// ----------------------------------------------------------------- BVH --
+// Two-wide BVH, one 64-byte node per inner node holding both child boxes
+// (one cache line per visit).  Children >= 0 are inner nodes; a leaf child is
+// ~(first*16 + count-1): `count` (1..16) triangles from position `first` of
+// the leaf-ordered triangle array.
+struct alignas(64) GNode {
+    float4 a;  // c0.lo.x c0.hi.x c0.lo.y c0.hi.y
+    float4 b;  // c1.lo.x c1.hi.x c1.lo.y c1.hi.y
+    float4 c;  // c0.lo.z c0.hi.z c1.lo.z c1.hi.z
+    int4 d;    // child0 child1 - -
+};
+
+#define YG_LEAF_MAX 16
+#define YG_STACK_LDS 24
+#define YG_STACK_SPILL 72
+#define YG_MAX_DEPTH (YG_STACK_LDS + YG_STACK_SPILL - 2)
+
+// Leaf-ordered triangle record, 48 B: (a.xyz, n.x) (b.xyz, n.y) (c.xyz, n.z),
+// n = the unit face normal of triangle_t::recNormal (triangle.cc:45-49).
+struct BVHView {
+    const GNode* nodes;
+    const float4* tris;     // 3 per triangle
+    const int32_t* tprim;   // original primitive id per leaf-ordered triangle
+    const int32_t* tobj;    // object id per leaf-ordered triangle
+    const int32_t* tmat;    // material id per leaf-ordered triangle
+    int32_t ntris;
+};
+
+struct HostBVH {
+    std::vector<GNode> nodes;
+    std::vector<float4> tris;
+    std::vector<int32_t> tprim, tobj, tmat;
+    int32_t max_depth = 0;
+};
+
+// Builds the BVH over world-space triangles (binned SAH).  prim ids are
+// positions in idx/3.
+void build_bvh(const float* verts, const uint32_t* idx, int64_t nt, const int32_t* tri_obj,
+               const int32_t* tri_mat, int max_leaf, int bins, int threads, HostBVH& out);
+
+// Face normal exactly as triangle_t::recNormal computes it.
+YG_HD f3 tri_normal(f3 a, f3 b, f3 c) { return norm3(cross(b - a, c - a)); }
+
+// ---------------------------------------------------------- photon map --
+struct PhotonMapDev {
+    float* d_photons = nullptr;     // 9 floats each (dir, pos, colour) in cell order
+    int64_t n = 0;
+    int32_t* d_gstart = nullptr;    // uniform grid over photons: ncells+1
+    int32_t* d_gitems = nullptr;
+    float gmin[3] = {0, 0, 0};
+    float gcell = 1;
+    int gdim[3] = {1, 1, 1};
+    int np_emit = 0;                // photonLight_t::Np
+    int light = -1;
+};
+
+// --------------------------------------------------------- scene handle --
+struct yg_scene {
+    std::vector<float> verts;
+    std::vector<uint32_t> idx;
+    std::vector<int32_t> tri_obj, tri_mat, obj_flags;
+    int32_t nobj = 0;
+
+    bool built = false;
+    int32_t max_depth = 0;
+    uint64_t n_nodes = 0;
+    GNode* d_nodes = nullptr;
+    float4* d_tris = nullptr;
+    int32_t* d_tprim = nullptr;
+    int32_t* d_tobj = nullptr;
+    int32_t* d_tmat = nullptr;
+    int32_t* d_obj_flags = nullptr;
+    uint64_t dev_bytes = 0;
+
+    bool has_cam = false;
+    yg_camera cam{};
+    std::vector<yg_material> mats;
+    std::vector<yg_modulator> mods;
+    std::vector<yg_texture> texs;
+    std::vector<yg_light> lights;
+    yg_material* d_mats = nullptr;
+    yg_modulator* d_mods = nullptr;
+    yg_texture* d_texs = nullptr;
+
+    PhotonMapDev pm;
+
+    BVHView view() const
+    {
+        BVHView v;
+        v.nodes = d_nodes; v.tris = d_tris; v.tprim = d_tprim; v.tobj = d_tobj; v.tmat = d_tmat;
+        v.ntris = (int32_t)tri_mat.size();
+        return v;
+    }
+};
+
+// ---------------------------------------------------------------- errors --
+void yg_set_error(const std::string& msg);
+int yg_fail(int code, const std::string& msg);
+
+#define YG_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return yg_fail(YG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+static inline int dev_upload(T** dst, const T* src, size_t n, uint64_t* acct = nullptr)
+{
+    if (*dst) { (void)hipFree(*dst); *dst = nullptr; }
+    if (n == 0) return YG_OK;
+    hipError_t e = hipMalloc((void**)dst, sizeof(T) * n);
+    if (e != hipSuccess) return yg_fail(YG_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    e = hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return yg_fail(YG_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    if (acct) *acct += sizeof(T) * n;
+    return YG_OK;
+}
+
+static inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }

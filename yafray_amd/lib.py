@@ -1,0 +1,243 @@
+"""ctypes binding of libyafgpu.so (the C ABI declared in include/yafgpu.h).
+
+Device buffers are torch tensors on the current HIP device; their data
+pointers cross the ABI.  There is no fallback: if the library is missing,
+import fails loudly.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import descriptors as D
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libyafgpu.so")
+_LIB = None
+
+vp = C.c_void_p
+u32 = C.c_uint32
+i32 = C.c_int32
+f32 = C.c_float
+
+
+class YgError(RuntimeError):
+    pass
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libyafgpu.so not built: run `python -m yafray_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    sigs = {
+        "yg_abi_version": (C.c_int, []),
+        "yg_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
+        "yg_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+        "yg_scene_create": (C.c_int, [C.POINTER(vp)]),
+        "yg_scene_destroy": (C.c_int, [vp]),
+        "yg_mesh_add": (C.c_int, [vp, vp, u32, vp, u32, vp, i32, C.POINTER(i32)]),
+        "yg_build": (C.c_int, [vp, vp]),
+        "yg_scene_info": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                    C.POINTER(C.c_uint64), C.POINTER(i32)]),
+        "yg_trace_closest": (C.c_int, [vp, vp, u32, vp, vp]),
+        "yg_trace_closest_counted": (C.c_int, [vp, vp, u32, vp, vp, vp]),
+        "yg_trace_any": (C.c_int, [vp, vp, u32, vp, vp]),
+        "yg_first_hit": (C.c_int, [vp, vp, vp, u32, f32, vp, vp]),
+        "yg_is_shadowed": (C.c_int, [vp, vp, vp, vp, u32, i32, f32, f32, vp, vp]),
+        "yg_noise_eval": (C.c_int, [i32, vp, vp, u32, vp, vp]),
+        "yg_gen_heightfield": (C.c_int, [i32, i32] + [f32] * 7 + [vp, vp]),
+        "yg_halton": (C.c_int, [i32, u32, u32, vp, vp]),
+        "yg_set_camera": (C.c_int, [vp, vp]),
+        "yg_set_shading": (C.c_int, [vp, vp, u32, vp, u32, vp, u32]),
+        "yg_set_lights": (C.c_int, [vp, vp, u32]),
+        "yg_render": (C.c_int, [vp, vp, vp, vp, vp]),
+        "yg_photon_build": (C.c_int, [vp, u32, vp, vp, vp]),
+        "yg_photon_read": (C.c_int, [vp, vp, C.c_uint64, C.POINTER(C.c_uint64), vp]),
+        "yg_photon_gather": (C.c_int, [vp, vp, vp, vp, vp, vp, u32, vp, vp]),
+    }
+    optional = {
+        "yg_render_batch": (C.c_int, [C.POINTER(vp), u32, vp, C.POINTER(vp), vp, vp]),
+        "yg_render_adjoint": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    for name, (res, args) in optional.items():
+        if hasattr(L, name):
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+    _LIB = L
+    return L
+
+
+def exported_symbols():
+    return lib()
+
+
+def last_error():
+    buf = C.create_string_buffer(1024)
+    lib().yg_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc, what=""):
+    if rc != 0:
+        raise YgError("%s failed (%d): %s" % (what, rc, last_error()))
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return vp(t.data_ptr())
+
+
+def _stream():
+    import torch
+    return vp(torch.cuda.current_stream().cuda_stream)
+
+
+class Scene:
+    """A yg_scene handle: meshes (addObject_trimesh), BVH, shading, lights."""
+
+    def __init__(self):
+        self.h = vp()
+        check(lib().yg_scene_create(C.byref(self.h)), "yg_scene_create")
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            lib().yg_scene_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_mesh(self, verts, idx, mat=None, flags=7):
+        v = np.ascontiguousarray(verts, np.float32).reshape(-1)
+        f = np.ascontiguousarray(idx, np.uint32).reshape(-1)
+        m = None if mat is None else np.ascontiguousarray(mat, np.int32)
+        oid = i32()
+        check(lib().yg_mesh_add(self.h, v.ctypes.data_as(vp), len(v) // 3, f.ctypes.data_as(vp),
+                                len(f) // 3, None if m is None else m.ctypes.data_as(vp), flags,
+                                C.byref(oid)), "yg_mesh_add")
+        return oid.value
+
+    def build(self, max_leaf=4, bins=32, threads=0):
+        o = D.yg_build_opts(max_leaf, bins, threads)
+        check(lib().yg_build(self.h, C.byref(o)), "yg_build")
+
+    def info(self):
+        nt, nn, nb = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        md = i32()
+        check(lib().yg_scene_info(self.h, C.byref(nt), C.byref(nn), C.byref(nb), C.byref(md)),
+              "yg_scene_info")
+        return {"tris": nt.value, "nodes": nn.value, "dev_bytes": nb.value, "max_depth": md.value}
+
+    @classmethod
+    def from_desc(cls, desc, build=True, **kw):
+        """Ingest a SceneDesc object by object (object order = id order)."""
+        s = cls()
+        for o in range(len(desc.obj_flags)):
+            sel = np.nonzero(desc.tri_obj == o)[0]
+            tri = desc.idx[sel]
+            used, inv = np.unique(tri.reshape(-1), return_inverse=True)
+            s.add_mesh(desc.verts[used], inv.reshape(-1, 3).astype(np.uint32), desc.tri_mat[sel],
+                       int(desc.obj_flags[o]))
+        if build:
+            s.build(**kw)
+        s.set_desc(desc)
+        return s
+
+    def set_desc(self, desc):
+        L = lib()
+        check(L.yg_set_camera(self.h, C.byref(desc.cam)), "yg_set_camera")
+        check(L.yg_set_shading(self.h, C.cast(desc.mats, vp), len(desc.mats),
+                               C.cast(desc.mods, vp) if len(desc.mods) else None, len(desc.mods),
+                               C.cast(desc.texs, vp) if len(desc.texs) else None, len(desc.texs)),
+              "yg_set_shading")
+        check(L.yg_set_lights(self.h, C.cast(desc.lights, vp) if len(desc.lights) else None,
+                              len(desc.lights)), "yg_set_lights")
+        self.desc = desc
+
+    # --- queries on device tensors ---
+    def trace_closest(self, rays, hits, counters=None):
+        n = rays.shape[0]
+        if counters is None:
+            check(lib().yg_trace_closest(self.h, _ptr(rays), n, _ptr(hits), _stream()),
+                  "yg_trace_closest")
+        else:
+            check(lib().yg_trace_closest_counted(self.h, _ptr(rays), n, _ptr(hits),
+                                                 _ptr(counters), _stream()),
+                  "yg_trace_closest_counted")
+
+    def trace_any(self, rays, occ):
+        check(lib().yg_trace_any(self.h, _ptr(rays), rays.shape[0], _ptr(occ), _stream()),
+              "yg_trace_any")
+
+    def first_hit(self, frm, d, hits, min_raydis=D.MIN_RAYDIST):
+        check(lib().yg_first_hit(self.h, _ptr(frm), _ptr(d), frm.shape[0], min_raydis, _ptr(hits),
+                                 _stream()), "yg_first_hit")
+
+    def is_shadowed(self, p, l, self_obj, out, to_point, min_raydis=D.MIN_RAYDIST, self_bias=0.1):
+        check(lib().yg_is_shadowed(self.h, _ptr(p), _ptr(l), _ptr(self_obj), p.shape[0],
+                                   int(to_point), min_raydis, self_bias, _ptr(out), _stream()),
+              "yg_is_shadowed")
+
+    def render(self, out_rgb, opts=None, timing=False):
+        o = opts if opts is not None else self.desc.opts
+        if timing:
+            o = D.yg_render_opts.from_buffer_copy(o)
+            o.flags |= 1
+        st = D.yg_render_stats()
+        check(lib().yg_render(self.h, C.byref(o), _ptr(out_rgb), C.byref(st), _stream()),
+              "yg_render")
+        return st
+
+    def photon_build(self, li, opts=None):
+        o = opts if opts is not None else self.desc.opts
+        st = D.yg_photon_stats()
+        check(lib().yg_photon_build(self.h, li, C.byref(o), C.byref(st), _stream()),
+              "yg_photon_build")
+        return st
+
+    def photon_read(self, out):
+        cnt = C.c_uint64()
+        check(lib().yg_photon_read(self.h, _ptr(out), out.numel() // 9 if out is not None else 0,
+                                   C.byref(cnt), _stream()), "yg_photon_read")
+        return cnt.value
+
+    def photon_gather(self, p, n, ng, eye, mat, out):
+        check(lib().yg_photon_gather(self.h, _ptr(p), _ptr(n), _ptr(ng), _ptr(eye), _ptr(mat),
+                                     mat.shape[0], _ptr(out), _stream()), "yg_photon_gather")
+
+
+def noise_eval(what, tex, pts, out):
+    check(lib().yg_noise_eval(what, None if tex is None else C.byref(tex), _ptr(pts),
+                              pts.shape[0], _ptr(out), _stream()), "yg_noise_eval")
+
+
+def gen_heightfield(nx, ny, ex, ey, amp, freq, H, lac, octv, out):
+    check(lib().yg_gen_heightfield(nx, ny, ex, ey, amp, freq, H, lac, octv, _ptr(out), _stream()),
+          "yg_gen_heightfield")
+
+
+def halton(base, i0, n, out):
+    check(lib().yg_halton(base, i0, n, _ptr(out), _stream()), "yg_halton")
+
+
+def device_heights(nx, ny, ex, ey, amp, freq, H, lac, octv):
+    """Heightfield vertices computed on the device, returned as numpy."""
+    import torch
+    out = torch.empty(nx * ny * 3, dtype=torch.float32, device="cuda")
+    gen_heightfield(nx, ny, ex, ey, amp, freq, H, lac, octv, out)
+    torch.cuda.synchronize()
+    return out.cpu().numpy().reshape(-1, 3)
