@@ -1,0 +1,164 @@
+"""Benchmark: path-traced GI on the 1,045,506-triangle fBm terrain.
+
+One step = one full yg_render of the terrain at 1280x720, 4 samples per
+pixel, 3 bounces (pathlight use_QMC on, sun + constant sky), with the scene,
+BVH and buffers already resident in HBM.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def gpu_leg(args):
+    import numpy as np
+    import torch
+
+    from yafray_amd import descriptors as D
+    from yafray_amd import lib as Y
+    from yafray_amd import scenes as S
+
+    torch.cuda.set_device(0)
+    t0 = time.time()
+    d = S.terrain_gi(Y.device_heights)
+    sc = Y.Scene.from_desc(d)
+    info = sc.info()
+    setup_s = time.time() - t0
+    out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+    opts = D.yg_render_opts.from_buffer_copy(d.opts)
+    for _ in range(args.warmup):
+        sc.render(out, opts)
+    torch.cuda.synchronize()
+    rays = 0
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        st = sc.render(out, opts)
+        rays += st.rays_camera + st.rays_extension + st.rays_shadow
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t_start
+    # kernel-level timing of the GI bounce launches (HIP events on the render stream)
+    kt = D.yg_render_opts.from_buffer_copy(d.opts)
+    kt.flags = 3
+    ks = sc.render(out, kt)
+    torch.cuda.synchronize()
+    img = out.cpu().numpy().reshape(d.cam.resy, d.cam.resx, 3)
+    return {
+        "dt": dt, "rays": rays, "stats": ks, "info": info, "setup_s": setup_s,
+        "finite": bool(np.all(np.isfinite(img))), "mean": float(img.mean()),
+    }
+
+
+def parity_leg(cpu_threads):
+    """Bounded CPU sample: the same terrain and camera at 320x180, rendered by
+    the oracle (timed on the host cores) and by the GPU (for the RMSE)."""
+    import numpy as np
+    import torch
+
+    from oracle import pyoracle as po
+    from yafray_amd import lib as Y
+    from yafray_amd import scenes as S
+
+    d = S.terrain_gi(Y.device_heights, resx=320, resy=180)
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    t0 = time.perf_counter()
+    ref, rs = oc.render(d, threads=cpu_threads)
+    cpu_dt = time.perf_counter() - t0
+    cpu_rays = int(rs[0] + rs[1] + rs[2])
+    sc = Y.Scene.from_desc(d)
+    out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+    sc.render(out)
+    torch.cuda.synchronize()
+    img = out.cpu().numpy().reshape(ref.shape).astype(np.float64)
+    rel = float(np.sqrt(np.mean((img - ref) ** 2)) / np.sqrt(np.mean(ref.astype(np.float64) ** 2)))
+    exact = float(np.mean(np.all(img == ref, axis=2)))
+    return {"cpu_dt": cpu_dt, "cpu_rays": cpu_rays, "rel_rmse": rel, "exact_px": exact}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(rendering would shard by image tiles; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    g = gpu_leg(args)
+    ms = 1000.0 * g["dt"] / args.steps
+    value = g["rays"] / g["dt"] / 1e6
+    st = g["stats"]
+    bm = json.load(open(os.path.join(ROOT, "tests", "golden", "byte_model.json")))
+    # algorithmic bytes of the bounce launches: extension and shadow rays at
+    # the frozen canonical-BVH2 visit counts
+    ext_b = bm["extension"]["B_ray"]
+    sh_b = bm["shadow"]["B_ray"]
+    bshadow = st.rays_bounce - st.rays_extension
+    alg_bytes = st.rays_extension * ext_b + bshadow * sh_b
+    achieved = alg_bytes / (st.ms_bounce * 1e-3) / 1e9 if st.ms_bounce > 0 else 0.0
+    line = {
+        "metric": "GI Mrays/s",
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "path-traced GI: 1024x512-vertex fBm terrain (1,045,506 tris), 1280x720, "
+                        "4 spp, 3 bounces, sun + constant sky, 64 height-band materials",
+            "triangles": int(g["info"]["tris"]),
+            "resolution": "1280x720",
+            "spp": 4,
+            "depth": 3,
+            "rays_per_step": int(g["rays"] / args.steps),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_bounce",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "launches": int(st.bounce_launches),
+            "ms_per_launch": round(st.ms_bounce / max(st.bounce_launches, 1), 4),
+            "alg_bytes_per_launch": round(alg_bytes / max(st.bounce_launches, 1)),
+        },
+        "bvh": {"nodes": int(g["info"]["nodes"]), "max_depth": int(g["info"]["max_depth"]),
+                "dev_bytes": int(g["info"]["dev_bytes"])},
+        "image_finite": g["finite"],
+    }
+    if not args.no_cpu:
+        cores = min(os.cpu_count() or 1, 16)
+        p = parity_leg(cores)
+        line["cpu_baseline"] = {
+            "value": round(p["cpu_rays"] / p["cpu_dt"] / 1e6, 4),
+            "unit": "Mrays/s",
+            "cores": cores,
+            "kind": "port",
+            "sample": "oracle render of the same terrain and camera at 320x180 (4 spp, 3 bounces), "
+                      "%d rays in %.1f s" % (p["cpu_rays"], p["cpu_dt"]),
+        }
+        line["rmse_vs_oracle"] = {"rel_rmse": p["rel_rmse"], "bit_exact_pixels": p["exact_px"],
+                                  "sample": "320x180 render of the same scene"}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

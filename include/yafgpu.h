@@ -172,15 +172,21 @@ typedef struct yg_render_opts {
 } yg_render_opts;
 
 #define YG_RENDER_TIMING 1   /* fill yg_render_stats.ms with device time */
+#define YG_RENDER_KTIMING 2  /* also time each GI bounce launch (ms_bounce) */
+#define YG_RENDER_REUSE 4    /* yg_render_adjoint: replay the last yg_render of this scene
+                                if nothing changed since, instead of rendering again */
 
 typedef struct yg_render_stats {
     uint64_t rays_camera;
     uint64_t rays_extension;
     uint64_t rays_shadow;
-    uint64_t adaptive_extra;  /* paths whose pathlight asked for extra repetitions */
+    uint64_t adaptive_extra;  /* pixels whose pathlight asked for extra repetitions */
     uint64_t lcg_draws;       /* ourRandom() draws consumed (area lights) */
-    float ms;
-    float ms_trace;
+    float ms;                 /* whole render, device time */
+    float ms_trace;           /* up to the final pixel pass */
+    float ms_bounce;          /* sum of the GI bounce launches (YG_RENDER_KTIMING) */
+    uint32_t bounce_launches;
+    uint64_t rays_bounce;     /* extension + shadow rays traced inside the bounce launches */
 } yg_render_stats;
 
 typedef struct yg_build_opts {
@@ -285,6 +291,28 @@ int yg_set_lights(yg_scene* s, const yg_light* lights, uint32_t n);
  * photon (diffuse mode; yg_photon_build must have run). */
 int yg_render(yg_scene* s, const yg_render_opts* opts, float* d_rgb,
               yg_render_stats* stats, void* stream);
+
+/* Several scenes rendered by one set of launches (batched mode): the work
+ * items of all scenes share the kernels, so small scenes no longer leave the
+ * GPU idle.  d_rgb[k] receives scene k's image; opts[k] are per scene.
+ * Every scene's pathlight must use the same samples and depth.  No reference
+ * counterpart: the reference renders one scene per process. */
+int yg_render_batch(yg_scene** scenes, uint32_t n, const yg_render_opts* opts, float** d_rgb,
+                    yg_render_stats* stats, void* stream);
+
+/* Gradient of a loss through the image (no reference counterpart: the
+ * reference has no derivatives).  Renders the image into d_rgb, then replays
+ * every pathlight path with the same Halton samples and visibility held
+ * fixed, applying the chain rule to the estimator of pathlight.cc:305-403 and
+ * sunlight.cc:35-57 with d_grad_rgb = dLoss/dpixel (resx*resy*3).  Outputs
+ * (overwritten, FP64): d_grad_mat[nmat*3] = dLoss/d albedo, d_grad_mod[nmod*2]
+ * = dLoss/d modulator colour and dLoss/d lambda where every modulator size is
+ * scaled by lambda (at lambda = 1), d_grad_light[nl] = dLoss/d power.
+ * Scenes: one pathlight + at most one sun light, at most one modulator per
+ * material (Musgrave fBm or clouds texture). */
+int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const float* d_grad_rgb, float* d_rgb,
+                      double* d_grad_mat, double* d_grad_mod, double* d_grad_light,
+                      yg_render_stats* stats, void* stream);
 
 /* ----------------------------------------------------------- photons -- */
 

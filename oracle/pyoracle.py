@@ -70,6 +70,9 @@ def lib():
         L.yo_render_mt.restype = C.c_int
         L.yo_render_mt.argtypes = [vp, vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int,
                                    vp, C.c_int, vp, F32P, U64P, C.c_int]
+        L.yo_render_adjoint.restype = C.c_int
+        L.yo_render_adjoint.argtypes = [vp, vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int,
+                                        vp, C.c_int, vp, F32P, F32P, vp, vp, vp]
         L.yo_photon_build.restype = C.c_int
         L.yo_photon_build.argtypes = [vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int,
                                       C.c_int, vp, C.c_int, vp, U64P]
@@ -158,6 +161,24 @@ class OracleScene:
         if rc != 0:
             raise RuntimeError("yo_render failed: %d" % rc)
         return rgb.reshape(desc.cam.resy, desc.cam.resx, 3), stats
+
+    def render_adjoint(self, desc, grad_rgb):
+        """FP64 reference gradient: returns (image, gmat[nmat,3], gmod[nmod,2], glight[nl])."""
+        L = lib()
+        rgb = np.zeros(desc.cam.resx * desc.cam.resy * 3, np.float32)
+        g = np.ascontiguousarray(grad_rgb, np.float32).reshape(-1)
+        gmat = np.zeros(len(desc.mats) * 3, np.float64)
+        gmod = np.zeros(max(len(desc.mods), 1) * 2, np.float64)
+        glight = np.zeros(len(desc.lights), np.float64)
+        rc = L.yo_render_adjoint(self.h, C.addressof(desc.cam), _addr(desc.mats), len(desc.mats),
+                                 _addr(desc.mods), len(desc.mods), _addr(desc.texs), len(desc.texs),
+                                 _addr(desc.lights), len(desc.lights), _ptr(desc.obj_flags),
+                                 len(desc.obj_flags), C.addressof(desc.opts), g, rgb, _ptr(gmat),
+                                 _ptr(gmod), _ptr(glight))
+        if rc != 0:
+            raise RuntimeError("yo_render_adjoint failed: %d" % rc)
+        return (rgb.reshape(desc.cam.resy, desc.cam.resx, 3), gmat.reshape(-1, 3),
+                gmod[: 2 * len(desc.mods)].reshape(-1, 2), glight)
 
     def photon_build(self, desc, li):
         L = lib()

@@ -98,6 +98,14 @@ int yo_render_mt(yo_scene* s, const yg_camera* cam, const yg_material* mats, int
                  const yg_light* lights, int nl, const int32_t* obj_flags, int nobj,
                  const yg_render_opts* opts, float* rgb, uint64_t* stats, int nthreads);
 
+/* FP64 reference gradient of a loss through the image (single thread):
+ * gmat nmat*3, gmod nmod*2 (colour, size scale), glight nl (power) */
+int yo_render_adjoint(yo_scene* s, const yg_camera* cam, const yg_material* mats, int nmat,
+                      const yg_modulator* mods, int nmod, const yg_texture* texs, int ntex,
+                      const yg_light* lights, int nl, const int32_t* obj_flags, int nobj,
+                      const yg_render_opts* opts, const float* grad_rgb, float* rgb, double* gmat,
+                      double* gmod, double* glight);
+
 /* photonLight_t::init for light li; keeps the clustered photons in s */
 int yo_photon_build(yo_scene* s, const yg_material* mats, int nmat, const yg_modulator* mods,
                     int nmod, const yg_texture* texs, int ntex, const yg_light* lights,

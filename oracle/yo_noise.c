@@ -220,3 +220,98 @@ void yo_gen_heightfield(int nx, int ny, float ex, float ey, float amp, float fre
             xyz[3 * k] = x; xyz[3 * k + 1] = y; xyz[3 * k + 2] = z;
         }
 }
+
+/* ---------------------------------------------- FP64 texture derivatives -- */
+
+static double fade_d(double t) { return 30.0 * t * t * (t - 1.0) * (t - 1.0); }
+static double fade_dd(double t) { return t * t * t * (t * (t * 6 - 15) + 10); }
+
+static void grad_vec(int hash, double g[3])
+{
+    int h = hash & 15;
+    double su = (h & 1) ? -1.0 : 1.0, sv = (h & 2) ? -1.0 : 1.0;
+    g[0] = g[1] = g[2] = 0;
+    if (h < 8) g[0] += su; else g[1] += su;
+    if (h < 4) g[1] += sv;
+    else if (h == 12 || h == 14) g[0] += sv;
+    else g[2] += sv;
+}
+
+/* d newPerlin_t / d(x,y,z) in double: the noise is a trilinear blend (fade
+ * weights) of the 8 corner gradient ramps, differentiated analytically */
+static void perlin_grad_d(double x, double y, double z, double out[3])
+{
+    hash_init();
+    double fx = floor(x), fy = floor(y), fz = floor(z);
+    int X = ((int)fx) & 255, Y = ((int)fy) & 255, Z = ((int)fz) & 255;
+    x -= fx; y -= fy; z -= fz;
+    double u = fade_dd(x), v = fade_dd(y), w = fade_dd(z);
+    double du = fade_d(x), dv = fade_d(y), dw = fade_d(z);
+    int A = HASH[X] + Y, AA = HASH[A] + Z, AB = HASH[A + 1] + Z;
+    int B = HASH[X + 1] + Y, BA = HASH[B] + Z, BB = HASH[B + 1] + Z;
+    int hc[8] = {HASH[AA], HASH[BA], HASH[AB], HASH[BB], HASH[AA + 1], HASH[BA + 1], HASH[AB + 1], HASH[BB + 1]};
+    double g[8], G[8][3];
+    for (int k = 0; k < 8; k++) {
+        double c[3] = {x - (k & 1), y - ((k >> 1) & 1), z - ((k >> 2) & 1)};
+        grad_vec(hc[k], G[k]);
+        g[k] = G[k][0] * c[0] + G[k][1] * c[1] + G[k][2] * c[2];
+    }
+    double k1 = g[1] - g[0], k2 = g[2] - g[0], k3 = g[4] - g[0];
+    double k4 = g[0] - g[1] - g[2] + g[3], k5 = g[0] - g[2] - g[4] + g[6], k6 = g[0] - g[1] - g[4] + g[5];
+    double k7 = -g[0] + g[1] + g[2] - g[3] + g[4] - g[5] - g[6] + g[7];
+    double d[3] = {du * (k1 + k4 * v + k6 * w + k7 * v * w), dv * (k2 + k4 * u + k5 * w + k7 * u * w),
+                   dw * (k3 + k5 * v + k6 * u + k7 * u * v)};
+    for (int k = 0; k < 8; k++) {
+        double wu = (k & 1) ? u : 1 - u, wv = ((k >> 1) & 1) ? v : 1 - v, ww = ((k >> 2) & 1) ? w : 1 - w;
+        for (int a = 0; a < 3; a++) d[a] += G[k][a] * wu * wv * ww;
+    }
+    for (int a = 0; a < 3; a++) out[a] = 0.5 * d[a];
+}
+
+/* d getFloat(p) / d lambda for p scaled by lambda at 1 (= grad . p), for
+ * Musgrave fBm and clouds turbulence */
+double yo_tex_dF_dlambda(const yg_texture* t, const float* pp)
+{
+    double p[3] = {pp[0], pp[1], pp[2]}, g[3] = {0, 0, 0}, q[3];
+    if (t->type == YG_TEX_MUSGRAVE) {
+        double s = t->size, pwr = 1, scale = 1, pwHL = pow((double)t->lacunarity, -(double)t->H);
+        for (int i = 0; i < (int)t->octaves; i++) {
+            perlin_grad_d(p[0] * s * scale, p[1] * s * scale, p[2] * s * scale, q);
+            for (int a = 0; a < 3; a++) g[a] += 2.0 * pwr * scale * q[a];
+            pwr *= pwHL;
+            scale *= t->lacunarity;
+        }
+        double rmd = t->octaves - floor(t->octaves);
+        if (rmd != 0) {
+            perlin_grad_d(p[0] * s * scale, p[1] * s * scale, p[2] * s * scale, q);
+            for (int a = 0; a < 3; a++) g[a] += 2.0 * rmd * pwr * scale * q[a];
+        }
+        for (int a = 0; a < 3; a++) g[a] *= t->iscale * s;
+    } else if (t->type == YG_TEX_CLOUDS) {
+        double s = t->size, amp = 1, scale = 1, v = 0;
+        for (int i = 0; i <= t->depth; i++, amp *= 0.5, scale *= 2) {
+            double x = p[0] * s * scale, y = p[1] * s * scale, z = p[2] * s * scale;
+            double n = yo_newperlin((float)x, (float)y, (float)z);
+            perlin_grad_d(x, y, z, q);
+            double f = scale;
+            double val = n;
+            if (t->hard) {
+                val = fabs(2 * n - 1);
+                f *= (2 * n - 1) >= 0 ? 2.0 : -2.0;
+            }
+            v += amp * val;
+            for (int a = 0; a < 3; a++) g[a] += amp * f * q[a];
+        }
+        double norm = (double)(1 << t->depth) / (double)((1 << (t->depth + 1)) - 1);
+        v *= norm;
+        for (int a = 0; a < 3; a++) g[a] *= norm * s;
+        if (t->bias) for (int a = 0; a < 3; a++) g[a] *= (t->bias == 1) ? -2.0 * v : 2.0 * v;
+    }
+    return g[0] * p[0] + g[1] * p[1] + g[2] * p[2];
+}
+
+void yo_tex_color_f(const yg_texture* t, const float* p, float* out)
+{
+    yo_col c = tex_color(t, v3_ld(p));
+    out[0] = c.r; out[1] = c.g; out[2] = c.b;
+}

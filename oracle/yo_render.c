@@ -52,7 +52,21 @@ typedef struct {
     yo_halton** hseq;      /* per light: path sampler state of the current render area */
     int32_t lcg;           /* global myseed */
     uint64_t stats[5];
+    /* adjoint (FP64 reference gradient) */
+    int adj;
+    double gpx[3];         /* dLoss/dpixel of the pixel being rendered, or 0 */
+    int gpx_on;
+    int primary;           /* inside the camera ray's scene_t::light */
+    double* gmat;          /* nmat*3 */
+    double* gmod;          /* nmod*2 */
+    double* glight;        /* nl */
 } rctx;
+
+static void adj_path_sample_begin(rctx* R);
+static void adj_path_vertex(rctx* R, int hit, const yo_sp* sp, yo_col lf, yo_col lfp);
+static void adj_path_sample_end(rctx* R, float s);
+static void adj_path_pixel(rctx* R, int li, const yo_sp* sp0, float mult);
+static void adj_direct_sun(rctx* R, int li, const yo_sp* sp, float inte);
 
 static int obj_flag(const rctx* R, int obj, int bit)
 {
@@ -124,10 +138,10 @@ static int is_shadowed(rctx* R, const yo_sp* sp, yo_v3 l, int to_point)
 static yo_col sun_illuminate(rctx* R, int li, const yo_sp* sp, yo_v3 eye)
 {
     const yg_light* L = &R->lights[li];
-    if (L->cast_shadows) {
-        if (!is_shadowed(R, sp, R->sun_dir[li], 0))
-            return from_light(R, sp, R->sun_dir[li], R->sun_col[li], eye);
-        return col(0, 0, 0);
+    if (L->cast_shadows && is_shadowed(R, sp, R->sun_dir[li], 0)) return col(0, 0, 0);
+    if (R->adj && R->gpx_on && R->primary) {
+        const float inte = v3_dot(face_forward(sp->Ng, sp->N, v3_norm(eye)), R->sun_dir[li]);
+        if (inte >= 0) adj_direct_sun(R, li, sp, inte);
     }
     return from_light(R, sp, R->sun_dir[li], R->sun_col[li], eye);
 }
@@ -285,6 +299,8 @@ static yo_col path_take_sample(rctx* R, int li, yo_v3 N, const yo_sp* sp)
             yo_v3 ray = next_direction(hs, &taken, N, sp->NU, sp->NV, i, 0);
             yo_v3 startray = ray;
             yo_v3 where = sp->P;
+            const int rec = R->adj && R->gpx_on && first;
+            if (rec) adj_path_sample_begin(R);
             for (int j = 0; j < maxdepth; ++j) {
                 if (col_energy(raycolor) < 0.05f) break;
                 yo_sp t;
@@ -293,10 +309,29 @@ static yo_col path_take_sample(rctx* R, int li, yo_v3 N, const yo_sp* sp)
                     yo_col contri = col_mul(col_scale(raycolor, v3_dot(startray, N)), background(R));
                     total = col_add(total, contri);
                     if (first) subtotal[i % 4] = col_add(subtotal[i % 4], contri);
+                    if (rec) adj_path_vertex(R, 0, NULL, col(0, 0, 0), col(0, 0, 0));
                     break;
                 }
                 /* getLight, pathlight.cc:186-191: sc.light(indirect) at P + eye */
                 yo_v3 eye = v3_neg(ray);
+                if (rec) {
+                    /* sun factors of this vertex without the albedo (with / without power) */
+                    yo_col lf = col(0, 0, 0), lfp = col(0, 0, 0);
+                    yo_v3 e2 = v3_sub(v3_add(t.P, eye), t.P);
+                    for (int l2 = 0; l2 < R->nl; l2++) {
+                        const yg_light* Ls = &R->lights[l2];
+                        if (Ls->type != YG_LIGHT_SUN || !Ls->use_in_indirect) continue;
+                        if (Ls->cast_shadows &&
+                            yo_is_shadowed_one(R->s, t.P, R->sun_dir[l2], t.obj, 0, R->opts.min_raydis,
+                                               R->opts.self_bias))
+                            continue;
+                        float inte = v3_dot(face_forward(t.Ng, t.N, v3_norm(e2)), R->sun_dir[l2]);
+                        if (inte < 0) continue;
+                        lf = col_add(lf, col_scale(R->sun_col[l2], inte));
+                        lfp = col_add(lfp, col_scale(col(Ls->color[0], Ls->color[1], Ls->color[2]), inte));
+                    }
+                    adj_path_vertex(R, 1, &t, lf, lfp);
+                }
                 yo_col light = scene_light(R, &t, v3_add(t.P, eye), 1);
                 yo_col contri = col_mul(col_scale(raycolor, v3_dot(startray, N)), light);
                 total = col_add(total, contri);
@@ -306,6 +341,7 @@ static yo_col path_take_sample(rctx* R, int li, yo_v3 N, const yo_sp* sp)
                 ray = next_direction(hs, &taken, HN, t.NU, t.NV, i, j + 1);
                 where = t.P;
             }
+            if (rec) adj_path_sample_end(R, v3_dot(startray, N));
         }
         if (first) {
             first = 0;
@@ -342,8 +378,179 @@ static yo_col path_illuminate(rctx* R, int li, const yo_sp* sp, yo_v3 eye)
     yo_v3 N = face_forward(sp->Ng, sp->N, eye);
     yo_col dif = get_diffuse(R, sp);
     if (col_energy(dif) < 0.05f) return col(0, 0, 0);
+    const int prim = R->primary;
+    R->primary = 0;  /* lights evaluated along the paths are not the pixel's direct terms */
     yo_col total = path_take_sample(R, li, N, sp);
+    R->primary = prim;
+    if (R->adj && R->gpx_on) adj_path_pixel(R, li, sp, (float)(1.0 / (double)(float)L->samples));
     return col_scale(col_mul(dif, total), L->power);
+}
+
+/* --------------------------------------------- FP64 reference gradient -- */
+
+double yo_tex_dF_dlambda(const yg_texture* t, const float* p);
+void yo_tex_color_f(const yg_texture* t, const float* p, float* out);
+
+#define ADJ_MAXD 8
+typedef struct {
+    int n, hit[ADJ_MAXD], mat[ADJ_MAXD];
+    yo_v3 P[ADJ_MAXD];
+    yo_col lf[ADJ_MAXD], lfp[ADJ_MAXD];
+    float s;
+} adj_path;
+
+static adj_path* g_ap = NULL;     /* per-thread use only: adjoint renders run single-threaded */
+static int g_nap = 0, g_apcap = 0;
+
+/* albedo with its partial derivatives, in double (one modulator at most):
+ * rho, d rho/d albedo, d rho/d modulator colour, d rho/d lambda */
+static int diffuse_d(const rctx* R, int mat, yo_v3 P, double rho[3], double dalb[3], double damp[3],
+                     double dlam[3])
+{
+    const yg_material* m = &R->mats[mat];
+    double a[3] = {m->color[0], m->color[1], m->color[2]};
+    for (int c = 0; c < 3; c++) { rho[c] = a[c]; dalb[c] = 1; damp[c] = 0; dlam[c] = 0; }
+    if (m->mod_count == 0) return -1;
+    const yg_modulator* md = &R->mods[m->mod_first];
+    const yg_texture* t = &R->texs[md->tex];
+    float tp[3] = {md->size[0] * (P.x + md->ofs[0]), md->size[1] * (P.y + md->ofs[1]),
+                   md->size[2] * (P.z + md->ofs[2])};
+    float tcf[3];
+    yo_tex_color_f(t, tp, tcf);
+    yo_col rc = yo_modulate(md, R->texs, col(m->color[0], m->color[1], m->color[2]), P);
+    rho[0] = rc.r; rho[1] = rc.g; rho[2] = rc.b;
+    double A = md->color;
+    if (!(A > 0)) return m->mod_first;
+    double dF = yo_tex_dF_dlambda(t, tp);
+    double c21[3] = {t->color2[0] - t->color1[0], t->color2[1] - t->color1[1], t->color2[2] - t->color1[2]};
+    int inside = A <= 1.0;
+    for (int c = 0; c < 3; c++) {
+        double tc = tcf[c], dtex = c21[c] * dF;
+        switch (md->mode) {
+        case YG_MOD_MUL:
+            if (inside) { dalb[c] = tc * A + (1 - A); damp[c] = (tc - 1) * a[c]; dlam[c] = A * dtex * a[c]; }
+            else { dalb[c] = tc; dlam[c] = dtex * a[c]; }
+            break;
+        case YG_MOD_MIX:
+            if (inside) { dalb[c] = 1 - A; damp[c] = tc - a[c]; dlam[c] = A * dtex; }
+            else { dalb[c] = 0; dlam[c] = dtex; }
+            break;
+        case YG_MOD_ADD: damp[c] = tc; dlam[c] = A * dtex; break;
+        case YG_MOD_SUB: damp[c] = -tc; dlam[c] = -A * dtex; break;
+        }
+    }
+    return m->mod_first;
+}
+
+static void adj_vertex_add(rctx* R, int mat, yo_v3 P, const double gamma[3])
+{
+    double rho[3], dalb[3], damp[3], dlam[3];
+    int mod = diffuse_d(R, mat, P, rho, dalb, damp, dlam);
+    for (int c = 0; c < 3; c++) R->gmat[3 * mat + c] += gamma[c] * dalb[c];
+    if (mod >= 0) {
+        for (int c = 0; c < 3; c++) {
+            R->gmod[2 * mod] += gamma[c] * damp[c];
+            R->gmod[2 * mod + 1] += gamma[c] * dlam[c];
+        }
+    }
+}
+
+static void adj_path_sample_begin(rctx* R)
+{
+    (void)R;
+    if (g_nap == g_apcap) {
+        g_apcap = g_apcap ? 2 * g_apcap : 64;
+        g_ap = (adj_path*)realloc(g_ap, sizeof(adj_path) * (size_t)g_apcap);
+    }
+    g_ap[g_nap].n = 0;
+}
+
+static void adj_path_vertex(rctx* R, int hit, const yo_sp* sp, yo_col lf, yo_col lfp)
+{
+    (void)R;
+    adj_path* a = &g_ap[g_nap];
+    if (a->n >= ADJ_MAXD) return;
+    a->hit[a->n] = hit;
+    if (hit) { a->mat[a->n] = sp->mat; a->P[a->n] = sp->P; a->lf[a->n] = lf; a->lfp[a->n] = lfp; }
+    a->n++;
+}
+
+static void adj_path_sample_end(rctx* R, float s)
+{
+    (void)R;
+    g_ap[g_nap].s = s;
+    g_nap++;
+}
+
+/* product rule over the recorded samples of one pixel (see yg_render_adjoint) */
+static void adj_path_pixel(rctx* R, int li, const yo_sp* sp0, float mult)
+{
+    const yg_light* L = &R->lights[li];
+    double pm = (double)mult * (double)L->power;
+    double rho0[3], d0a[3], d0m[3], d0l[3];
+    diffuse_d(R, sp0->mat, sp0->P, rho0, d0a, d0m, d0l);
+    double bg[3] = {0, 0, 0};
+    if (R->opts.has_background) for (int c = 0; c < 3; c++) bg[c] = R->opts.background[c];
+    double T[3] = {0, 0, 0}, dpow[3] = {0, 0, 0};
+    double W[3];
+    for (int c = 0; c < 3; c++) W[c] = R->gpx[c] * rho0[c] * pm;
+    int sun = -1;
+    for (int l2 = 0; l2 < R->nl; l2++)
+        if (R->lights[l2].type == YG_LIGHT_SUN && R->lights[l2].use_in_indirect) { sun = l2; break; }
+    for (int q = 0; q < g_nap; q++) {
+        const adj_path* a = &g_ap[q];
+        double rv[ADJ_MAXD][3];
+        for (int b = 0; b < a->n; b++)
+            if (a->hit[b]) {
+                double da[3], dm[3], dl[3];
+                diffuse_d(R, a->mat[b], a->P[b], rv[b], da, dm, dl);
+            }
+        double Rb[3] = {1, 1, 1};
+        for (int b = 0; b < a->n; b++) {
+            for (int c = 0; c < 3; c++) {
+                double X = a->hit[b] ? (double)(&a->lf[b].r)[c] * rv[b][c] : bg[c];
+                T[c] += a->s * Rb[c] * X;
+                if (a->hit[b]) dpow[c] += a->s * Rb[c] * (double)(&a->lfp[b].r)[c] * rv[b][c];
+            }
+            if (a->hit[b]) for (int c = 0; c < 3; c++) Rb[c] *= rv[b][c];
+        }
+        for (int k = 0; k < a->n; k++) {
+            if (!a->hit[k]) continue;
+            double dk[3] = {0, 0, 0}, Rex[3] = {1, 1, 1};
+            for (int b = 0; b < a->n; b++) {
+                for (int c = 0; c < 3; c++) {
+                    if (b == k) dk[c] += a->s * Rex[c] * (double)(&a->lf[b].r)[c];
+                    else if (b > k) {
+                        double X = a->hit[b] ? (double)(&a->lf[b].r)[c] * rv[b][c] : bg[c];
+                        dk[c] += a->s * Rex[c] * X;
+                    }
+                }
+                if (a->hit[b] && b != k) for (int c = 0; c < 3; c++) Rex[c] *= rv[b][c];
+            }
+            double gam[3] = {W[0] * dk[0], W[1] * dk[1], W[2] * dk[2]};
+            adj_vertex_add(R, a->mat[k], a->P[k], gam);
+        }
+    }
+    double g0[3];
+    for (int c = 0; c < 3; c++) g0[c] = R->gpx[c] * T[c] * pm;
+    adj_vertex_add(R, sp0->mat, sp0->P, g0);
+    if (sun >= 0) R->glight[sun] += W[0] * dpow[0] + W[1] * dpow[1] + W[2] * dpow[2];
+    g_nap = 0;
+}
+
+static void adj_direct_sun(rctx* R, int li, const yo_sp* sp, float inte)
+{
+    double rho[3], da[3], dm[3], dl[3];
+    diffuse_d(R, sp->mat, sp->P, rho, da, dm, dl);
+    const yg_light* L = &R->lights[li];
+    double gam[3], dp = 0;
+    for (int c = 0; c < 3; c++) {
+        double colp = (double)(&R->sun_col[li].r)[c];
+        gam[c] = R->gpx[c] * (double)(inte * colp);
+        dp += R->gpx[c] * (double)inte * (double)L->color[c] * rho[c];
+    }
+    adj_vertex_add(R, sp->mat, sp->P, gam);
+    R->glight[li] += dp;
 }
 
 /* ------------------------------------------------------------- photons -- */
@@ -793,7 +1000,12 @@ static yo_col raytrace(rctx* R, yo_v3 from, yo_v3 ray)
 {
     yo_sp sp;
     R->stats[0]++;
-    if (first_hit(R, from, ray, &sp)) return scene_light(R, &sp, from, 0);
+    if (first_hit(R, from, ray, &sp)) {
+        R->primary = 1;
+        yo_col c = scene_light(R, &sp, from, 0);
+        R->primary = 0;
+        return c;
+    }
     return background(R);
 }
 
@@ -853,6 +1065,8 @@ typedef struct {
     pthread_mutex_t mu;
     uint64_t stats[5];
     rctx* shared_R;   /* sequential mode only */
+    const float* grad_rgb;  /* adjoint: dLoss/dpixel */
+    double *gmat, *gmod, *glight;
 } render_job;
 
 static void render_area(rctx* R, const render_job* J, int ri)
@@ -867,8 +1081,15 @@ static void render_area(rctx* R, const render_job* J, int ri)
     for (int i = a->y; i < a->y + a->h; ++i)
         for (int j = a->x; j < a->x + a->w; ++j) {
             yo_v3 ray = cam_ray(J->k, (float)j + 0.5f, (float)i + 0.5f);
+            const int out = j >= a->rx && j < a->rx + a->rw && i >= a->ry && i < a->ry + a->rh;
+            R->gpx_on = 0;
+            if (R->adj && out) {
+                const float* g = J->grad_rgb + 3 * ((int64_t)i * J->cam->resx + j);
+                R->gpx[0] = g[0]; R->gpx[1] = g[1]; R->gpx[2] = g[2];
+                R->gpx_on = (g[0] != 0 || g[1] != 0 || g[2] != 0);
+            }
             yo_col c = raytrace(R, J->k->pos, ray);
-            if (j >= a->rx && j < a->rx + a->rw && i >= a->ry && i < a->ry + a->rh) {
+            if (out) {
                 int64_t px = (int64_t)i * J->cam->resx + j;
                 J->rgb[3 * px] = c.r; J->rgb[3 * px + 1] = c.g; J->rgb[3 * px + 2] = c.b;
             }
@@ -880,6 +1101,12 @@ static void* render_worker(void* arg)
     render_job* J = (render_job*)arg;
     rctx R;
     rctx_init(&R, J->s, J->mats, J->nmat, J->mods, J->texs, J->lights, J->nl, J->obj_flags, J->nobj, J->opts);
+    if (J->grad_rgb) {
+        R.adj = 1;
+        R.gmat = J->gmat;
+        R.gmod = J->gmod;
+        R.glight = J->glight;
+    }
     for (;;) {
         pthread_mutex_lock(&J->mu);
         int q = J->next++;
@@ -941,4 +1168,36 @@ int yo_render(yo_scene* s, const yg_camera* cam, const yg_material* mats, int nm
 {
     return yo_render_mt(s, cam, mats, nmat, mods, nmod, texs, ntex, lights, nl, obj_flags, nobj,
                         opts, rgb, stats, 1);
+}
+
+/* FP64 reference gradient of a loss through the image (see yg_render_adjoint):
+ * renders like yo_render (single thread) and accumulates, per output pixel,
+ * dLoss/dparams = sum over the estimator's terms of grad_rgb * d(term)/d(param)
+ * in double precision.  gmat: nmat*3, gmod: nmod*2, glight: nl (zeroed here). */
+int yo_render_adjoint(yo_scene* s, const yg_camera* cam, const yg_material* mats, int nmat,
+                      const yg_modulator* mods, int nmod, const yg_texture* texs, int ntex,
+                      const yg_light* lights, int nl, const int32_t* obj_flags, int nobj,
+                      const yg_render_opts* opts, const float* grad_rgb, float* rgb, double* gmat,
+                      double* gmod, double* glight)
+{
+    (void)ntex;
+    memset(gmat, 0, sizeof(double) * 3 * (size_t)nmat);
+    if (gmod) memset(gmod, 0, sizeof(double) * 2 * (size_t)nmod);
+    memset(glight, 0, sizeof(double) * (size_t)nl);
+    yo_cam k;
+    cam_setup(cam, &k);
+    yo_region* regions = NULL;
+    int nreg = yo_block_regions(cam->resx, cam->resy, opts->block > 0 ? opts->block : 64,
+                                opts->rand_seed, &regions);
+    render_job J;
+    memset(&J, 0, sizeof(J));
+    J.s = s; J.cam = cam; J.mats = mats; J.nmat = nmat; J.mods = mods; J.texs = texs;
+    J.lights = lights; J.nl = nl; J.obj_flags = obj_flags; J.nobj = nobj; J.opts = opts;
+    J.k = &k; J.regions = regions; J.nreg = nreg; J.rgb = rgb; J.next = 0;
+    J.grad_rgb = grad_rgb; J.gmat = gmat; J.gmod = gmod; J.glight = glight;
+    pthread_mutex_init(&J.mu, NULL);
+    render_worker(&J);
+    pthread_mutex_destroy(&J.mu);
+    free(regions);
+    return 0;
 }

@@ -1,0 +1,101 @@
+"""Differentiable GI on the GPU: the path-replay adjoint (yg_render_adjoint)
+against the oracle's FP64 reference gradient and central finite differences
+(h = 1e-3) of the GPU forward render."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from yafray_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _render(Y, torch, sc, desc):
+    sc.set_desc(desc)
+    out = torch.zeros(desc.cam.resx * desc.cam.resy * 3, dtype=torch.float32, device="cuda")
+    sc.render(out)
+    torch.cuda.synchronize()
+    return out.cpu().numpy().astype(np.float64).reshape(desc.cam.resy, desc.cam.resx, 3)
+
+
+@pytest.fixture(scope="module")
+def setup(torch_cuda):
+    torch = torch_cuda
+    from yafray_amd import lib as Y
+    d = S.terrain_diff(po.heightfield, nx=96, ny=48, res=48, samples=4, depth=3, nmat=16)
+    sc = Y.Scene.from_desc(d)
+    alb = np.array([list(m.color) for m in d.mats], np.float64)
+    target = _render(Y, torch, sc, S.with_params(d, albedo=alb * 0.9 + 0.04, amp=0.4, size=2.7,
+                                                 sun_power=2.5))
+    img = _render(Y, torch, sc, d)
+    g = (2 * (img - target)).astype(np.float32)
+    gt = torch.from_numpy(g.reshape(-1)).cuda()
+    out = torch.zeros_like(gt)
+    gmat = torch.zeros(len(d.mats) * 3, dtype=torch.float64, device="cuda")
+    gmod = torch.zeros(2, dtype=torch.float64, device="cuda")
+    glight = torch.zeros(len(d.lights), dtype=torch.float64, device="cuda")
+    sc.render_adjoint(gt, out, gmat, gmod, glight)
+    torch.cuda.synchronize()
+    res = dict(gmat=gmat.cpu().numpy().reshape(-1, 3), gmod=gmod.cpu().numpy(),
+               glight=glight.cpu().numpy(), img_adj=out.cpu().numpy().reshape(img.shape))
+
+    def loss(desc):
+        im = _render(Y, torch, sc, desc)
+        return float(np.sum((im - target) ** 2))
+
+    return Y, torch, sc, d, alb, img, g, target, res, loss
+
+
+def test_adjoint_forward_image_matches_render(setup):
+    Y, torch, sc, d, alb, img, g, target, res, loss = setup
+    assert np.array_equal(res["img_adj"], img.astype(np.float32))
+
+
+def test_adjoint_matches_fp64_oracle(setup):
+    Y, torch, sc, d, alb, img, g, target, res, loss = setup
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    _, gm, gmod, gl = oc.render_adjoint(d, g)
+    for a, b in ((res["gmat"], gm), (res["gmod"], gmod.reshape(-1)), (res["glight"], gl)):
+        err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        assert err <= 1e-3, err
+
+
+H = 1e-3
+
+
+def test_adjoint_matches_finite_differences(setup):
+    Y, torch, sc, d, alb, img, g, target, res, loss = setup
+    gm = res["gmat"]
+    big = np.argsort(-np.abs(gm).reshape(-1))[:4]
+    for flat in big:
+        m, c = divmod(int(flat), 3)
+        a = alb.copy()
+        a[m, c] += H
+        lp = loss(S.with_params(d, albedo=a))
+        a[m, c] -= 2 * H
+        lm = loss(S.with_params(d, albedo=a))
+        f = (lp - lm) / (2 * H)
+        assert abs(gm[m, c] - f) <= 1e-3 * abs(f), (m, c, gm[m, c], f)
+    f = (loss(S.with_params(d, sun_power=3.0 + H)) - loss(S.with_params(d, sun_power=3.0 - H))) / (2 * H)
+    assert abs(res["glight"][1] - f) <= 1e-3 * abs(f)
+    f = (loss(S.with_params(d, amp=0.5 + H)) - loss(S.with_params(d, amp=0.5 - H))) / (2 * H)
+    assert abs(res["gmod"][0] - f) <= 1e-3 * abs(f)
+    f = (loss(S.with_params(d, size=3.0 + H)) - loss(S.with_params(d, size=3.0 - H))) / (2 * H)
+    assert abs(res["gmod"][1] / 3.0 - f) <= 2e-3 * abs(f)
+
+
+def test_autograd_function(setup):
+    Y, torch, sc, d, alb, img, g, target, res, loss = setup
+    from yafray_amd.autograd import render_gi
+    albedo = torch.tensor(alb, dtype=torch.float64, requires_grad=True)
+    amp = torch.tensor(0.5, requires_grad=True)
+    size = torch.tensor(3.0, requires_grad=True)
+    power = torch.tensor(3.0, requires_grad=True)
+    tgt = torch.from_numpy(target).float().cuda()
+    im = render_gi(sc, albedo, amp, size, power)
+    L = ((im - tgt) ** 2).sum()
+    L.backward()
+    assert np.allclose(albedo.grad.numpy(), res["gmat"], rtol=1e-4, atol=1e-6)
+    assert abs(float(power.grad) - res["glight"][1]) <= 1e-4 * abs(res["glight"][1])
+    assert abs(float(amp.grad) - res["gmod"][0]) <= 1e-4 * abs(res["gmod"][0])
+    assert abs(float(size.grad) - res["gmod"][1] / 3.0) <= 1e-4 * abs(res["gmod"][1] / 3.0)

@@ -1,0 +1,125 @@
+"""Pins of the oracle's render loop: closed-form scenes and invariants."""
+import numpy as np
+
+from oracle import pyoracle as po
+from yafray_amd import descriptors as D
+from yafray_amd import scenes as S
+
+
+def radinv2(i):
+    r, f = 0.0, 0.5
+    while i:
+        r += (i & 1) * f
+        i >>= 1
+        f *= 0.5
+    return r
+
+
+def plane_desc(S_=4, resx=96, resy=80, power=2.0):
+    E = 50.0
+    v = np.array([[-E, -E, 0], [E, -E, 0], [E, E, 0], [-E, E, 0]], np.float32)
+    f = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+    sun = np.array([1.0, 0.5, 1.5])
+    sun /= np.linalg.norm(sun)
+    lights = [D.pathlight(samples=S_, depth=2, use_qmc=True, threshold=1e3),
+              D.sunlight(sun, power=power)]
+    cam = D.camera((0, 0, 3), (0, 0.01, 0), (0, 1, 4), resx, resy, 1.0, 1.0)
+    opts = D.render_opts(bias=1e-3, background=(0.4, 0.5, 0.7))
+    return D.SceneDesc(v, f, [0, 0], [0, 0], [7], cam, [D.material((0.6, 0.5, 0.3))], [], [],
+                       lights, opts), sun
+
+
+def plane_expected(S_, resx, resy, sun, power):
+    B = np.array([0.4, 0.5, 0.7])
+    rho = np.array([0.6, 0.5, 0.3])
+    exp = np.zeros((resy, resx, 3))
+    blk = 64
+    for by in range((resy + blk - 1) // blk):
+        for bx in range((resx + blk - 1) // blk):
+            rx, ry = bx * blk, by * blk
+            rw, rh = min(blk, resx - rx), min(blk, resy - ry)
+            x, y, w, h = rx, ry, rw, rh
+            if x > 0:
+                x -= 1; w += 1
+            if y > 0:
+                y -= 1; h += 1
+            if x + w < resx - 1:
+                w += 1
+            if y + h < resy - 1:
+                h += 1
+            for i in range(ry, ry + rh):
+                for j in range(rx, rx + rw):
+                    k = (i - y) * w + (j - x)
+                    z1 = [radinv2(k * S_ + s + 1) for s in range(S_)]
+                    exp[i, j] = rho * (np.sum(np.sqrt(z1)) / S_) * B + rho * sun[2] * power
+    return exp
+
+
+def test_plane_gi_closed_form_oracle():
+    """Every path sample escapes to the sky: pixel = rho*[(1/S) sum_i sqrt(z1_i) B]
+    + rho (N.s) E, with z1_i the base-2 Halton value of the pixel's per-area
+    call index (pathlight.cc:305-347, pathtools.cc:32-43, scene.cc:726)."""
+    d, sun = plane_desc()
+    img, st = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat).render(d)
+    exp = plane_expected(4, 96, 80, sun, 2.0)
+    assert np.max(np.abs(img - exp) / exp) < 2e-5
+    # one extension ray per sample (all escape) and one sun shadow ray per pixel
+    assert st[1] == st[0] * 4 and st[2] == st[0]
+
+
+def test_blocked_sun_is_black():
+    """A quad between the sun and the floor: points under it get no direct
+    light (isShadowed, scene.cc:148-185) and only sky light through paths."""
+    v = np.array([[-50, -50, 0], [50, -50, 0], [50, 50, 0], [-50, 50, 0],
+                  [-50, -50, 1], [50, -50, 1], [50, 50, 1], [-50, 50, 1]], np.float32)
+    f = np.array([[0, 1, 2], [0, 2, 3], [4, 5, 6], [4, 6, 7]], np.uint32)
+    cam = D.camera((0, 0, 0.5), (0, 0.01, 0), (0, 1, 1.5), 32, 32, 1.0, 1.0)
+    opts = D.render_opts(bias=1e-3, background=(1, 1, 1))
+    d = D.SceneDesc(v, f, [0, 0, 1, 1], [0, 0, 0, 0], [7, 7], cam, [D.material((0.5, 0.5, 0.5))],
+                    [], [], [D.sunlight((0, 0, 1), power=5.0)], opts)
+    img, st = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat).render(d)
+    assert np.all(img == 0)
+
+
+def test_area_light_draw_accounting_and_linearity():
+    """psamples = 0: every lit shading point draws 2 ourRandom() values per
+    sample before the N.L test (arealight.cc:136-153); the result is linear in
+    the light's power."""
+    d = S.direct_room(48, 36, subdiv=2)
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    img, st = oc.render(d)
+    assert st[4] % (2 * 16) == 0 and st[4] > 0
+    assert st[4] // 32 <= st[0]
+    lights = list(d.lights)
+    lights[0].power *= 3.0
+    d3 = D.SceneDesc(d.verts, d.idx, d.tri_obj, d.tri_mat, d.obj_flags, d.cam, list(d.mats), [], [],
+                     lights, d.opts)
+    img3, _ = oc.render(d3)
+    assert np.allclose(img3, 3 * img, rtol=1e-5, atol=1e-6)
+
+
+def test_threaded_render_is_deterministic():
+    d = S.terrain_gi(po.heightfield, nx=64, ny=32, resx=80, resy=72)
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    a, sa = oc.render(d, threads=1)
+    b, sb = oc.render(d, threads=4)
+    assert np.array_equal(a, b) and np.array_equal(sa, sb)
+
+
+def test_photon_counts():
+    """Np = photons/(depth+1-mindepth) rounded down to a square
+    (photonlight.cc:77-81); every emitted photon is stored at most depth+1 times."""
+    d = S.photon_field(n_side=3, subdiv=1, extent=10.0, resx=32, resy=24, photons=2 * 101 * 101 + 57,
+                       search=10, nmat=2)
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    st = oc.photon_build(d, 0)
+    assert st[0] == 101 * 101
+    assert 0 < st[1] <= 2 * st[0]
+    ph = oc.photon_read()
+    assert len(ph) == st[2]
+    assert np.allclose(np.linalg.norm(ph[:, :3], axis=1), 1, atol=1e-5)
+    # cells come out in (x, y, z) order of the hash (hash3d.h:218-254)
+    cell = 2.5 / 4 if False else d.lights[0].cluster
+    c = np.floor(ph[:, 3:6] / cell)
+    keys = [tuple(r) for r in c.astype(np.int64)]
+    assert keys == sorted(keys)

@@ -82,6 +82,7 @@ extern "C" int yg_mesh_add(yg_scene* s, const float* xyz, uint32_t nv, const uin
     else s->tri_mat.insert(s->tri_mat.end(), nt, 0);
     s->obj_flags.push_back(flags);
     s->built = false;
+    s->gen++;
     if (obj_id) *obj_id = oid;
     return YG_OK;
 }
@@ -123,6 +124,7 @@ extern "C" int yg_build(yg_scene* s, const yg_build_opts* opts)
     s->n_nodes = h.nodes.size();
     s->max_depth = h.max_depth;
     s->built = true;
+    s->gen++;
     return YG_OK;
 }
 
@@ -145,6 +147,7 @@ extern "C" int yg_set_camera(yg_scene* s, const yg_camera* cam)
     if (cam->resx <= 0 || cam->resy <= 0) return yg_fail(YG_ERR_ARG, "bad resolution");
     s->cam = *cam;
     s->has_cam = true;
+    s->gen++;
     return YG_OK;
 }
 
@@ -170,6 +173,7 @@ extern "C" int yg_set_shading(yg_scene* s, const yg_material* mats, uint32_t nma
     s->mats.assign(mats, mats + nmat);
     s->mods.assign(mods, mods + nmod);
     s->texs.assign(texs, texs + ntex);
+    s->gen++;
     int rc;
     if ((rc = dev_upload(&s->d_mats, mats, nmat))) return rc;
     yg_modulator dm{};
@@ -198,6 +202,7 @@ extern "C" int yg_set_lights(yg_scene* s, const yg_light* lights, uint32_t n)
     }
     s->lights.assign(lights, lights + n);
     s->pm.light = -1;
+    s->gen++;
     return YG_OK;
 }
 

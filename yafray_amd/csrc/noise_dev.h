@@ -157,6 +157,162 @@ __device__ __forceinline__ c3 modulate(const yg_modulator& m, const yg_texture* 
     return C;
 }
 
+// ---- derivatives for the adjoint (gradient of the noise value in space) --
+
+// gradient vector of grad(hash, .) (noise.h:35-40): grad is linear in (x,y,z)
+__device__ __forceinline__ f3 pgrad_vec(int hash)
+{
+    const int h = hash & 15;
+    f3 g = mk3(0, 0, 0);
+    const float su = (h & 1) ? -1.0f : 1.0f, sv = (h & 2) ? -1.0f : 1.0f;
+    if (h < 8) g.x += su; else g.y += su;
+    if (h < 4) g.y += sv;
+    else if (h == 12 || h == 14) g.x += sv;
+    else g.z += sv;
+    return g;
+}
+
+__device__ __forceinline__ float pfade_d(float t) { return 30.0f * t * t * (t - 1.0f) * (t - 1.0f); }
+
+// d newPerlin_t / d(x,y,z)
+__device__ __forceinline__ f3 new_perlin_grad(float x, float y, float z)
+{
+    const unsigned char* H = YG_PERM;
+    float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+    int X = ((int)fx) & 255, Y = ((int)fy) & 255, Z = ((int)fz) & 255;
+    x -= fx; y -= fy; z -= fz;
+    const float u = pfade(x), v = pfade(y), w = pfade(z);
+    const float du = pfade_d(x), dv = pfade_d(y), dw = pfade_d(z);
+    int A = H[X] + Y, AA = H[A] + Z, AB = H[A + 1] + Z;
+    int B = H[X + 1] + Y, BA = H[B] + Z, BB = H[B + 1] + Z;
+    const int hc[8] = {H[AA], H[BA], H[AB], H[BB], H[AA + 1], H[BA + 1], H[AB + 1], H[BB + 1]};
+    float g[8];
+    f3 G[8];
+    for (int k = 0; k < 8; k++) {
+        const float cx = x - (float)(k & 1), cy = y - (float)((k >> 1) & 1), cz = z - (float)((k >> 2) & 1);
+        g[k] = pgrad(hc[k], cx, cy, cz);
+        G[k] = pgrad_vec(hc[k]);
+    }
+    const float k1 = g[1] - g[0], k2 = g[2] - g[0], k3 = g[4] - g[0];
+    const float k4 = g[0] - g[1] - g[2] + g[3], k5 = g[0] - g[2] - g[4] + g[6];
+    const float k6 = g[0] - g[1] - g[4] + g[5];
+    const float k7 = -g[0] + g[1] + g[2] - g[3] + g[4] - g[5] - g[6] + g[7];
+    f3 d = mk3(du * (k1 + k4 * v + k6 * w + k7 * v * w), dv * (k2 + k4 * u + k5 * w + k7 * u * w),
+               dw * (k3 + k5 * v + k6 * u + k7 * u * v));
+    for (int k = 0; k < 8; k++) {
+        const float wu = (k & 1) ? u : 1.0f - u, wv = ((k >> 1) & 1) ? v : 1.0f - v, ww = ((k >> 2) & 1) ? w : 1.0f - w;
+        d = d + G[k] * (wu * wv * ww);
+    }
+    return d * 0.5f;
+}
+
+// d getFloat / dp for the textures with a derivative (Musgrave fBm, clouds)
+__device__ __forceinline__ f3 tex_float_grad(const yg_texture& t, f3 p)
+{
+    if (t.type == YG_TEX_MUSGRAVE) {
+        const float s = t.size;
+        float x = p.x * s, y = p.y * s, z = p.z * s, pwr = 1.0f, scale = 1.0f;
+        const float pwHL = dpow(t.lacunarity, -t.H);
+        f3 g = mk3(0, 0, 0);
+        for (int i = 0; i < (int)t.octaves; i++) {
+            g = g + new_perlin_grad(x, y, z) * (2.0f * pwr * scale);
+            pwr *= pwHL;
+            x *= t.lacunarity; y *= t.lacunarity; z *= t.lacunarity;
+            scale *= t.lacunarity;
+        }
+        const float rmd = t.octaves - floorf(t.octaves);
+        if (rmd != 0.f) g = g + new_perlin_grad(x, y, z) * (2.0f * rmd * pwr * scale);
+        return g * (t.iscale * s);
+    }
+    if (t.type == YG_TEX_CLOUDS) {
+        float x = p.x * t.size, y = p.y * t.size, z = p.z * t.size, amp = 1.0f, scale = 1.0f;
+        f3 g = mk3(0, 0, 0);
+        float v = 0.0f;
+        for (int i = 0; i <= t.depth; i++, amp *= 0.5f, scale *= 2.0f, x *= 2.0f, y *= 2.0f, z *= 2.0f) {
+            const float n = new_perlin(x, y, z);
+            f3 dn = new_perlin_grad(x, y, z) * scale;
+            float val = n;
+            if (t.hard) {
+                val = fabsf(2.0f * n - 1.0f);
+                dn = dn * ((2.0f * n - 1.0f) >= 0.0f ? 2.0f : -2.0f);
+            }
+            v += amp * val;
+            g = g + dn * amp;
+        }
+        const float norm = (float)(1 << t.depth) / (float)((1 << (t.depth + 1)) - 1);
+        v *= norm;
+        g = g * (norm * t.size);
+        if (t.bias) g = g * (t.bias == 1 ? -2.0f * v : 2.0f * v);
+        return g;
+    }
+    return mk3(0, 0, 0);
+}
+
+// Albedo of a material with its partial derivatives: rho = modulated colour
+// (at most one modulator), d rho / d albedo, d rho / d(modulator colour),
+// d rho / d(lambda) where every modulator size is scaled by lambda (at 1).
+struct RhoD {
+    c3 rho, d_alb, d_amp, d_lam;
+    int mod;  // modulator index or -1
+};
+
+__device__ __forceinline__ RhoD diffuse_d(const yg_material* mats, const yg_modulator* mods, const yg_texture* texs,
+                                          int mat, f3 P)
+{
+    const yg_material& m = mats[mat];
+    RhoD r;
+    const c3 a = mkc(m.color[0], m.color[1], m.color[2]);
+    r.rho = a;
+    r.d_alb = mkc(1, 1, 1);
+    r.d_amp = mkc(0, 0, 0);
+    r.d_lam = mkc(0, 0, 0);
+    r.mod = -1;
+    if (m.mod_count == 0) return r;
+    const yg_modulator& md = mods[m.mod_first];
+    r.mod = m.mod_first;
+    r.rho = modulate(md, texs, a, P);
+    const float A = md.color;
+    if (!(A > 0)) return r;
+    const yg_texture& t = texs[md.tex];
+    const f3 tp = mk3(md.size[0] * (P.x + md.ofs[0]), md.size[1] * (P.y + md.ofs[1]), md.size[2] * (P.z + md.ofs[2]));
+    const c3 tc = tex_color(t, tp);
+    const c3 c21 = mkc(t.color2[0] - t.color1[0], t.color2[1] - t.color1[1], t.color2[2] - t.color1[2]);
+    const float dF = dot(tex_float_grad(t, tp), tp);  // d getFloat / d lambda
+    const c3 dtex = cscale(c21, dF);
+    const bool inside = A <= 1.0f;
+    switch (md.mode) {
+    case YG_MOD_MUL:
+        if (inside) {
+            r.d_alb = cadd(cscale(tc, A), mkc(1 - A, 1 - A, 1 - A));
+            r.d_amp = cmul(csub(tc, mkc(1, 1, 1)), a);
+            r.d_lam = cmul(cscale(dtex, A), a);
+        } else {
+            r.d_alb = tc;
+            r.d_lam = cmul(dtex, a);
+        }
+        break;
+    case YG_MOD_MIX:
+        if (inside) {
+            r.d_alb = mkc(1 - A, 1 - A, 1 - A);
+            r.d_amp = csub(tc, a);
+            r.d_lam = cscale(dtex, A);
+        } else {
+            r.d_alb = mkc(0, 0, 0);
+            r.d_lam = dtex;
+        }
+        break;
+    case YG_MOD_ADD:
+        r.d_amp = tc;
+        r.d_lam = cscale(dtex, A);
+        break;
+    case YG_MOD_SUB:
+        r.d_amp = cscale(tc, -1.0f);
+        r.d_lam = cscale(dtex, -A);
+        break;
+    }
+    return r;
+}
+
 // genericShader_t::getDiffuse (basicshaders.cc:191-204)
 __device__ __forceinline__ c3 get_diffuse(const yg_material* mats, const yg_modulator* mods,
                                           const yg_texture* texs, int mat, f3 P)

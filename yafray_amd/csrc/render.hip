@@ -48,7 +48,7 @@ __device__ __forceinline__ void item_pixel(const Area& a, uint32_t item, int& i,
 }
 
 struct Counters {
-    unsigned long long ext, shadow, extra;
+    unsigned long long ext, shadow, extra, bshadow, draws;
 };
 
 __device__ __forceinline__ void block_add(unsigned long long* dst, uint32_t v)
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(RB) void k_direct(const RenderScene* __restrict__ s
 {
     __shared__ int s_stk[YG_STACK_LDS * RB];
     const uint32_t item = blockIdx.x * RB + threadIdx.x;
-    uint32_t nshadow = 0;
+    uint32_t nshadow = 0, ndraws = 0;
     if (item < nitems) {
         const int ti = item_ti[item];
         if (ti >= 0) {
@@ -160,6 +160,7 @@ __global__ __launch_bounds__(RB) void k_direct(const RenderScene* __restrict__ s
                             seed = lcg_jump(R.lcg_seed, (uint64_t)(lcg_off[item] - lcg_off[R.item_base]), YG_LCG_POW);
                             seeded = true;
                         }
+                        if (dot(sp.P - L.from, L.dir) > 0) ndraws += 2u * (uint32_t)L.samples;
                         c = area_illuminate(R, L, sp, eye, seed, st, nshadow);
                     }
                 }
@@ -168,6 +169,7 @@ __global__ __launch_bounds__(RB) void k_direct(const RenderScene* __restrict__ s
         }
     }
     block_add(&cnt->shadow, nshadow);
+    block_add(&cnt->draws, ndraws);
 }
 
 // One bounce of the pathlight's full-GI loop (pathlight.cc:305-347) for the
@@ -175,6 +177,7 @@ __global__ __launch_bounds__(RB) void k_direct(const RenderScene* __restrict__ s
 __global__ __launch_bounds__(RB) void k_bounce(const RenderScene* __restrict__ scenes,
                                                const Area* __restrict__ areas, int nareas, int j,
                                                const uint32_t* __restrict__ active, uint32_t nactive,
+                                               uint32_t nactive_cap,
                                                int S, int D, const int32_t* __restrict__ item_ti,
                                                const float* __restrict__ item_t,
                                                const uint32_t* __restrict__ qoff,
@@ -215,7 +218,7 @@ __global__ __launch_bounds__(RB) void k_bounce(const RenderScene* __restrict__ s
             const float4 A = R.bvh.tris[3 * pti], B = R.bvh.tris[3 * pti + 1], C = R.bvh.tris[3 * pti + 2];
             create_cs(mk3(A.w, B.w, C.w), Ru, Rv);
             raycolor = st_col[p];
-            k = hidx[p];
+            k = hidx[(size_t)j * nactive_cap + p];
         }
         const float z1 = R.halton[(size_t)(2 * j) * R.hstride + k];
         const float z2 = R.halton[(size_t)(2 * j + 1) * R.hstride + k];
@@ -249,13 +252,14 @@ __global__ __launch_bounds__(RB) void k_bounce(const RenderScene* __restrict__ s
     }
     block_add(&cnt->ext, next);
     block_add(&cnt->shadow, nshadow);
+    block_add(&cnt->bshadow, nshadow);
 }
 
 // Scatter flagged paths into the next active list and give each its Halton
-// call index within its render area.
+// call index within its render area (hidx_level: npaths entries of this level).
 __global__ void k_compact(const Area* __restrict__ areas, int nareas, uint32_t npaths, int S,
                           const uint32_t* __restrict__ flag, const uint32_t* __restrict__ ex,
-                          uint32_t* __restrict__ active, uint32_t* __restrict__ hidx)
+                          uint32_t* __restrict__ active, uint32_t* __restrict__ hidx_level)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npaths || !flag[p]) return;
@@ -263,7 +267,7 @@ __global__ void k_compact(const Area* __restrict__ areas, int nareas, uint32_t n
     const Area& a = areas[find_area(areas, nareas, item)];
     const uint32_t e = ex[p];
     active[e] = p;
-    hidx[p] = e - ex[(size_t)a.item_start * S];
+    hidx_level[p] = e - ex[(size_t)a.item_start * S];
 }
 
 // level-0 active list: every sample of every qualifying item
@@ -439,7 +443,7 @@ static void area_setup(const yg_light& Lg, DevLight& L, std::vector<f3>& pts, st
             line_b = line_b + inc_v_b;
         }
     }
-    L.samples = index;
+    L.samples = std::min(index, samp);
     pts.insert(pts.end(), P.begin(), P.end());
     jh.insert(jh.end(), H.begin(), H.end());
     jv.insert(jv.end(), V.begin(), V.end());
@@ -487,7 +491,22 @@ struct RenderWS {
     DevBuf<Counters> cnt;
     DevBuf<uint8_t> scan_tmp;
     bool pow_ready = false;
+    // shape of the last launch (the adjoint replays it)
+    uint32_t nitems = 0, npaths = 0;
+    int S = 0, D = 0, na = 0, nl_max = 0;
+    bool any_path = false;
+    // the single-scene render the workspace holds (for YG_RENDER_REUSE)
+    const yg_scene* last_scene = nullptr;
+    uint64_t last_gen = 0;
+    yg_render_opts last_opts{};
 };
+
+static bool same_opts(const yg_render_opts& a, const yg_render_opts& b)
+{
+    yg_render_opts x = a, y = b;
+    x.flags = y.flags = 0;
+    return memcmp(&x, &y, sizeof(x)) == 0;
+}
 
 static RenderWS g_ws;
 
@@ -555,6 +574,8 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
         L.use_in_render = Lg.use_in_render;
         L.use_in_indirect = Lg.use_in_indirect;
         L.cast_shadows = Lg.cast_shadows;
+        L.power = Lg.power;
+        L.color = mkc(Lg.color[0], Lg.color[1], Lg.color[2]);
         if (Lg.type == YG_LIGHT_SUN) {
             // sunLight_t ctor (sunlight.cc:27-33)
             L.dir = norm3(ld3(Lg.dir));
@@ -626,9 +647,10 @@ __global__ __launch_bounds__(64) void k_photon_direct(const RenderScene* __restr
 
 // Launch the whole pipeline over a set of planned scenes (one launch per stage).
 static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render_stats* stats,
-                        bool timing)
+                        bool timing, bool ktiming)
 {
     int rc;
+    g_ws.last_scene = nullptr;  // the workspace is about to hold this launch
     if ((rc = perm_upload())) return rc;
     if (!g_ws.pow_ready) {
         uint64_t pw[64];
@@ -710,7 +732,7 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         if ((rc = W.active1.ensure(npaths))) return rc;
         if ((rc = W.flag.ensure(npaths))) return rc;
         if ((rc = W.ex.ensure(npaths))) return rc;
-        if ((rc = W.hidx.ensure(npaths))) return rc;
+        if ((rc = W.hidx.ensure((size_t)npaths * D))) return rc;
         if ((rc = W.st_where.ensure(npaths))) return rc;
         if ((rc = W.st_hn.ensure(npaths))) return rc;
         if ((rc = W.st_ti.ensure(npaths))) return rc;
@@ -732,7 +754,16 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         YG_HIP(hipMemcpyAsync(W.halton.p, halton.data(), sizeof(float) * halton.size(), hipMemcpyHostToDevice, st));
     YG_HIP(hipMemsetAsync(W.cnt.p, 0, sizeof(Counters), st));
 
+    W.nitems = nitems;
+    W.npaths = npaths;
+    W.S = S;
+    W.D = D;
+    W.na = (int)areas.size();
+    W.nl_max = nl_max;
+    W.any_path = any_path;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    std::vector<hipEvent_t> kev;
+    if (ktiming) timing = true;
     if (timing) {
         YG_HIP(hipEventCreate(&e0));
         YG_HIP(hipEventCreate(&e1));
@@ -769,15 +800,24 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         uint32_t* nxt = W.active1.p;
         for (int j = 0; j < D && nactive > 0; j++) {
             YG_HIP(hipMemsetAsync(W.flag.p, 0, sizeof(uint32_t) * npaths, st));
+            if (ktiming) {
+                hipEvent_t a, b;
+                YG_HIP(hipEventCreate(&a));
+                YG_HIP(hipEventCreate(&b));
+                kev.push_back(a);
+                kev.push_back(b);
+                YG_HIP(hipEventRecord(a, st));
+            }
             hipLaunchKernelGGL(k_bounce, dim3(grid_for(nactive, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p,
-                               na, j, act, nactive, S, D, W.item_ti.p, W.item_t.p, W.qoff.p, W.hidx.p,
+                               na, j, act, nactive, npaths, S, D, W.item_ti.p, W.item_t.p, W.qoff.p, W.hidx.p,
                                W.st_where.p, W.st_hn.p, W.st_ti.p, W.st_col.p, W.st_s.p, W.contri.p,
                                W.nseg.p, W.flag.p, W.cnt.p);
             YG_HIP(hipGetLastError());
+            if (ktiming) YG_HIP(hipEventRecord(kev.back(), st));
             if (j + 1 >= D) break;
             if ((rc = exclusive_scan(W.flag.p, W.ex.p, npaths, st))) return rc;
             hipLaunchKernelGGL(k_compact, dim3(grid_for(npaths, 256)), dim3(256), 0, st, W.areas.p, na, npaths,
-                               S, W.flag.p, W.ex.p, nxt, W.hidx.p);
+                               S, W.flag.p, W.ex.p, nxt, W.hidx.p + (size_t)(j + 1) * npaths);
             YG_HIP(hipGetLastError());
             uint32_t lf = 0, le = 0;
             YG_HIP(hipMemcpyAsync(&lf, W.flag.p + npaths - 1, 4, hipMemcpyDeviceToHost, st));
@@ -802,6 +842,8 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         stats->rays_extension = c.ext;
         stats->rays_shadow = c.shadow;
         stats->adaptive_extra = c.extra;
+        stats->lcg_draws = c.draws;
+        stats->rays_bounce = c.ext + c.bshadow;
         if (timing) {
             float ms = 0, ms2 = 0;
             YG_HIP(hipEventElapsedTime(&ms, e0, e2));
@@ -809,7 +851,14 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
             stats->ms = ms;
             stats->ms_trace = ms2;
         }
+        for (size_t q = 0; q + 1 < kev.size(); q += 2) {
+            float ms = 0;
+            YG_HIP(hipEventElapsedTime(&ms, kev[q], kev[q + 1]));
+            stats->ms_bounce += ms;
+            stats->bounce_launches++;
+        }
     }
+    for (auto e : kev) (void)hipEventDestroy(e);
     if (timing) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -824,7 +873,15 @@ extern "C" int yg_render(yg_scene* s, const yg_render_opts* opts, float* d_rgb, 
     std::vector<ScenePlan> plans(1);
     int rc = plan_scene(s, opts, d_rgb, plans[0]);
     if (rc) return rc;
-    return render_plans(plans, (hipStream_t)stream, stats, opts && (opts->flags & YG_RENDER_TIMING));
+    g_ws.last_scene = nullptr;
+    rc = render_plans(plans, (hipStream_t)stream, stats, opts && (opts->flags & YG_RENDER_TIMING),
+                      opts && (opts->flags & YG_RENDER_KTIMING));
+    if (rc == YG_OK) {
+        g_ws.last_scene = s;
+        g_ws.last_gen = s->gen;
+        g_ws.last_opts = *opts;
+    }
+    return rc;
 }
 
 // Batched render: several scenes in one set of launches (yg_render per scene
@@ -838,9 +895,12 @@ extern "C" int yg_render_batch(yg_scene** scenes, uint32_t n, const yg_render_op
         int rc = plan_scene(scenes[k], &opts[k], d_rgb[k], plans[k]);
         if (rc) return rc;
     }
-    bool timing = false;
-    for (uint32_t k = 0; k < n; k++) timing = timing || (opts[k].flags & YG_RENDER_TIMING);
-    return render_plans(plans, (hipStream_t)stream, stats, timing);
+    bool timing = false, ktiming = false;
+    for (uint32_t k = 0; k < n; k++) {
+        timing = timing || (opts[k].flags & YG_RENDER_TIMING);
+        ktiming = ktiming || (opts[k].flags & YG_RENDER_KTIMING);
+    }
+    return render_plans(plans, (hipStream_t)stream, stats, timing, ktiming);
 }
 
 // photonLight_t::illuminate at every primary hit (photonlight.cc:297-395)
@@ -866,4 +926,327 @@ __global__ __launch_bounds__(64) void k_photon_direct(const RenderScene* __restr
     const bool receive = (R.obj_flags[sp.obj] & 2) != 0;
     lightc[(size_t)item * nl_max + R.photon_light] =
         photon_estimate(R, R.pm, L.K, L.fixedradius, sp, eye, receive, hd + threadIdx.x, hi + threadIdx.x, 64);
+}
+
+// ------------------------------------------------------------ adjoint --
+//
+// Gradient of a scalar loss through the image, given dLoss/dpixel, with
+// visibility held fixed: path replay re-traces every pathlight sample with
+// the Halton indices of the forward pass (kept per level), recomputes the
+// vertex albedos with their partial derivatives and applies the product rule
+// to the estimator of pathlight.cc:305-387 (contri_b = s * R_b * X_b, pixel
+// term = rho0 * (mult * sum_b contri_b) * power).  Sun power enters every
+// light term linearly (sunlight.cc:35-57).  Supported scenes: one pathlight
+// plus sun lights, materials with at most one modulator.
+
+#define ADJ_MAXD 8
+#define ADJ_LDS_MAT 256
+#define ADJ_LDS_MOD 16
+#define ADJ_LDS_LIGHT 8
+
+struct AdjAcc {
+    double* mat;   // nmat*3
+    double* mod;   // nmod*2 (d/dcolour, d/dlambda)
+    double* light; // nl
+    int nmat, nmod, nl;
+};
+
+// block-private accumulators in LDS, flushed once per block
+struct AdjLds {
+    double mat[ADJ_LDS_MAT * 3];
+    double mod[ADJ_LDS_MOD * 2];
+    double light[ADJ_LDS_LIGHT];
+};
+
+__device__ __forceinline__ void adj_add_vertex(AdjLds& L, const AdjAcc& G, bool lds_ok, int mat, const RhoD& r,
+                                               c3 gamma)
+{
+    const c3 ga = cmul(gamma, r.d_alb);
+    double* dst = lds_ok ? L.mat : G.mat;
+    atomicAdd(&dst[3 * mat], (double)ga.r);
+    atomicAdd(&dst[3 * mat + 1], (double)ga.g);
+    atomicAdd(&dst[3 * mat + 2], (double)ga.b);
+    if (r.mod >= 0) {
+        double* dm = lds_ok ? L.mod : G.mod;
+        const c3 a = cmul(gamma, r.d_amp), l = cmul(gamma, r.d_lam);
+        atomicAdd(&dm[2 * r.mod], (double)(a.r + a.g + a.b));
+        atomicAdd(&dm[2 * r.mod + 1], (double)(l.r + l.g + l.b));
+    }
+}
+
+__device__ __forceinline__ void adj_lds_init(AdjLds& L)
+{
+    for (int k = threadIdx.x; k < ADJ_LDS_MAT * 3; k += blockDim.x) L.mat[k] = 0.0;
+    for (int k = threadIdx.x; k < ADJ_LDS_MOD * 2; k += blockDim.x) L.mod[k] = 0.0;
+    for (int k = threadIdx.x; k < ADJ_LDS_LIGHT; k += blockDim.x) L.light[k] = 0.0;
+    __syncthreads();
+}
+
+__device__ __forceinline__ void adj_lds_flush(AdjLds& L, const AdjAcc& G, bool lds_ok)
+{
+    __syncthreads();
+    if (!lds_ok) return;
+    for (int k = threadIdx.x; k < G.nmat * 3; k += blockDim.x)
+        if (L.mat[k] != 0.0) atomicAdd(&G.mat[k], L.mat[k]);
+    for (int k = threadIdx.x; k < G.nmod * 2; k += blockDim.x)
+        if (L.mod[k] != 0.0) atomicAdd(&G.mod[k], L.mod[k]);
+    for (int k = threadIdx.x; k < G.nl; k += blockDim.x)
+        if (L.light[k] != 0.0) atomicAdd(&G.light[k], L.light[k]);
+}
+
+__device__ __forceinline__ bool item_output(const Area& a, uint32_t item, int& i, int& j)
+{
+    item_pixel(a, item, i, j);
+    return j >= a.rx && j < a.rx + a.rw && i >= a.ry && i < a.ry + a.rh;
+}
+
+// Direct sun term at the primary hit: Dir = vis * inte * (colour*power) * rho0.
+__global__ __launch_bounds__(RB) void k_adjoint_direct(const RenderScene* __restrict__ scenes,
+                                                       const Area* __restrict__ areas, int nareas, uint32_t nitems,
+                                                       const int32_t* __restrict__ item_ti,
+                                                       const float* __restrict__ item_t,
+                                                       const float* __restrict__ grad_rgb, AdjAcc G)
+{
+    __shared__ int s_stk[YG_STACK_LDS * RB];
+    __shared__ AdjLds L;
+    const bool lds_ok = G.nmat <= ADJ_LDS_MAT && G.nmod <= ADJ_LDS_MOD && G.nl <= ADJ_LDS_LIGHT;
+    adj_lds_init(L);
+    const uint32_t item = blockIdx.x * RB + threadIdx.x;
+    if (item < nitems && item_ti[item] >= 0) {
+        const Area a = areas[find_area(areas, nareas, item)];
+        int i, j;
+        if (item_output(a, item, i, j)) {
+            const RenderScene& R = scenes[a.scene];
+            const float* gp = grad_rgb + 3 * ((size_t)i * R.resx + j);
+            const c3 g = mkc(gp[0], gp[1], gp[2]);
+            LaneStack st;
+            st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = RB; st.sp = 0;
+            f3 eye;
+            const Surf sp = primary_surf(R, a, item, item_ti[item], item_t[item], eye);
+            const RhoD r0 = diffuse_d(R.mats, R.mods, R.texs, sp.mat, sp.P);
+            for (int li = 0; li < R.nl; li++) {
+                const DevLight& Lt = R.lights[li];
+                if (!Lt.use_in_render || Lt.type != YG_LIGHT_SUN) continue;
+                if (Lt.cast_shadows && is_shadowed(R.bvh, sp.P, Lt.dir, sp.obj, false, R.min_raydis, R.self_bias, st))
+                    continue;
+                const f3 N = face_forward(sp.Ng, sp.N, norm3(eye));
+                const float inte = dot(N, Lt.dir);
+                if (inte < 0) continue;
+                const c3 fac = cscale(Lt.col, inte);  // d Dir / d rho0
+                adj_add_vertex(L, G, lds_ok, sp.mat, r0, cmul(g, fac));
+                const c3 dp = cmul(cscale(Lt.color, inte), r0.rho);  // d Dir / d power
+                atomicAdd(lds_ok ? &L.light[li] : &G.light[li], (double)(g.r * dp.r + g.g * dp.g + g.b * dp.b));
+            }
+        }
+    }
+    adj_lds_flush(L, G, lds_ok);
+}
+
+// Pathlight sample replay (one thread per qualifying path).
+__global__ __launch_bounds__(RB) void k_adjoint_path(const RenderScene* __restrict__ scenes,
+                                                     const Area* __restrict__ areas, int nareas,
+                                                     const uint32_t* __restrict__ active, uint32_t nactive,
+                                                     uint32_t npaths, int S, int D,
+                                                     const int32_t* __restrict__ item_ti,
+                                                     const float* __restrict__ item_t,
+                                                     const uint32_t* __restrict__ qoff,
+                                                     const uint32_t* __restrict__ hidx,
+                                                     const float* __restrict__ grad_rgb, AdjAcc G)
+{
+    __shared__ int s_stk[YG_STACK_LDS * RB];
+    __shared__ AdjLds L;
+    const bool lds_ok = G.nmat <= ADJ_LDS_MAT && G.nmod <= ADJ_LDS_MOD && G.nl <= ADJ_LDS_LIGHT;
+    adj_lds_init(L);
+    const uint32_t idx = blockIdx.x * RB + threadIdx.x;
+    if (idx < nactive) {
+        const uint32_t p = active[idx];
+        const uint32_t item = p / (uint32_t)S;
+        const int smp = (int)(p % (uint32_t)S);
+        const Area a = areas[find_area(areas, nareas, item)];
+        int pi, pj;
+        if (item_output(a, item, pi, pj)) {
+            const RenderScene& R = scenes[a.scene];
+            const float* gp = grad_rgb + 3 * ((size_t)pi * R.resx + pj);
+            const c3 g = mkc(gp[0], gp[1], gp[2]);
+            LaneStack st;
+            st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = RB; st.sp = 0;
+            f3 eye;
+            const Surf sp0 = primary_surf(R, a, item, item_ti[item], item_t[item], eye);
+            const RhoD r0 = diffuse_d(R.mats, R.mods, R.texs, sp0.mat, sp0.P);
+            f3 where = sp0.P, N = face_forward(sp0.Ng, sp0.N, eye), Ru = sp0.NU, Rv = sp0.NV;
+            // replayed path: per bounce the hit vertex (albedo + derivatives), the
+            // light factor of its sun term without / with power, or a miss
+            RhoD rv[ADJ_MAXD];
+            int mv[ADJ_MAXD];
+            c3 lf[ADJ_MAXD], lfp[ADJ_MAXD];
+            bool hit[ADJ_MAXD];
+            int nb = 0;
+            c3 raycolor = mkc(1, 1, 1);
+            float s = 0.0f;
+            for (int b = 0; b < D && b < ADJ_MAXD; b++) {
+                const uint32_t k = (b == 0) ? (qoff[item] - qoff[a.item_start]) * (uint32_t)S + (uint32_t)smp
+                                            : hidx[(size_t)b * npaths + p];
+                const float z1 = R.halton[(size_t)(2 * b) * R.hstride + k];
+                const float z2 = R.halton[(size_t)(2 * b + 1) * R.hstride + k];
+                const f3 ray = hemi_direction(z1, z2, N, Ru, Rv);
+                if (b == 0) s = dot(ray, N);
+                if (cenergy(raycolor) < 0.05f) break;
+                float t;
+                const int ti = first_hit(R.bvh, where, ray, R.min_raydis, t, st);
+                nb = b + 1;
+                if (ti < 0) {
+                    hit[b] = false;
+                    break;
+                }
+                hit[b] = true;
+                const Surf sp = make_surf(R.bvh, ti, where + ray * R.min_raydis, ray, t);
+                rv[b] = diffuse_d(R.mats, R.mods, R.texs, sp.mat, sp.P);
+                mv[b] = sp.mat;
+                // sun terms of getLight (scene_t::light indirect), without the albedo
+                c3 fac = mkc(0, 0, 0), facp = mkc(0, 0, 0);
+                const f3 from = sp.P + (-ray);
+                const f3 eye2 = from - sp.P;
+                for (int li = 0; li < R.nl; li++) {
+                    const DevLight& Lt = R.lights[li];
+                    if (!Lt.use_in_indirect || Lt.type != YG_LIGHT_SUN) continue;
+                    if (Lt.cast_shadows &&
+                        is_shadowed(R.bvh, sp.P, Lt.dir, sp.obj, false, R.min_raydis, R.self_bias, st))
+                        continue;
+                    const f3 Nl = face_forward(sp.Ng, sp.N, norm3(eye2));
+                    const float inte = dot(Nl, Lt.dir);
+                    if (inte < 0) continue;
+                    fac = cadd(fac, cscale(Lt.col, inte));
+                    facp = cadd(facp, cscale(Lt.color, inte));
+                }
+                lf[b] = fac;
+                lfp[b] = facp;
+                N = face_forward(sp.Ng, sp.N, -ray);
+                create_cs(sp.N, Ru, Rv);
+                raycolor = cmul(raycolor, rv[b].rho);
+                where = sp.P;
+            }
+            const float mult = (float)(1.0 / (double)(float)S);
+            const float pm = mult * R.path_power;
+            // T = sum_b s R_b X_b; d pixel / d rho0 = T*pm; d pixel / d T = rho0*pm
+            c3 T = mkc(0, 0, 0), Rb = mkc(1, 1, 1);
+            c3 dpow = mkc(0, 0, 0);
+            for (int b = 0; b < nb; b++) {
+                const c3 X = hit[b] ? cmul(lf[b], rv[b].rho) : R.bg;
+                T = cadd(T, cscale(cmul(Rb, X), s));
+                if (hit[b]) {
+                    dpow = cadd(dpow, cscale(cmul(Rb, cmul(lfp[b], rv[b].rho)), s));
+                    Rb = cmul(Rb, rv[b].rho);
+                }
+            }
+            adj_add_vertex(L, G, lds_ok, sp0.mat, r0, cmul(g, cscale(T, pm)));
+            const c3 W = cmul(g, cscale(r0.rho, pm));
+            // d T / d rho(H_k): bounces after k through R_b, bounce k through X_k
+            for (int k = 0; k < nb; k++) {
+                if (!hit[k]) continue;
+                c3 dk = mkc(0, 0, 0);
+                c3 Rex = mkc(1, 1, 1);  // product of rho(H_k') for k' < b, k' != k
+                for (int b = 0; b < nb; b++) {
+                    if (b == k) dk = cadd(dk, cscale(cmul(Rex, lf[b]), s));
+                    else if (b > k) {
+                        const c3 X = hit[b] ? cmul(lf[b], rv[b].rho) : R.bg;
+                        dk = cadd(dk, cscale(cmul(Rex, X), s));
+                    }
+                    if (hit[b] && b != k) Rex = cmul(Rex, rv[b].rho);
+                }
+                adj_add_vertex(L, G, lds_ok, mv[k], rv[k], cmul(W, dk));
+            }
+            // sun power: every hit term is linear in it (one sun light assumed)
+            for (int li = 0; li < R.nl; li++) {
+                const DevLight& Lt = R.lights[li];
+                if (Lt.use_in_indirect && Lt.type == YG_LIGHT_SUN) {
+                    const c3 q = cmul(W, dpow);
+                    atomicAdd(lds_ok ? &L.light[li] : &G.light[li], (double)(q.r + q.g + q.b));
+                    break;
+                }
+            }
+        }
+    }
+    adj_lds_flush(L, G, lds_ok);
+}
+
+// Forward render (image into d_rgb) followed by the replay.  Gradient
+// buffers are overwritten: d_grad_mat[nmat*3] (albedo), d_grad_mod[nmod*2]
+// (modulator colour, modulator size scale lambda), d_grad_light[nl] (power).
+extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const float* d_grad_rgb, float* d_rgb,
+                                 double* d_grad_mat, double* d_grad_mod, double* d_grad_light,
+                                 yg_render_stats* stats, void* stream)
+{
+    if (!s || !opts || !d_grad_rgb || !d_rgb || !d_grad_mat || !d_grad_light)
+        return yg_fail(YG_ERR_ARG, "null argument");
+    int nsun = 0, npath = 0;
+    for (const auto& L : s->lights) {
+        if (L.type == YG_LIGHT_SUN) nsun++;
+        else if (L.type == YG_LIGHT_PATH) npath++;
+        else return yg_fail(YG_ERR_UNSUPPORTED, "adjoint supports pathlight + sun lights only");
+    }
+    if (nsun > 1) return yg_fail(YG_ERR_UNSUPPORTED, "adjoint supports one sun light");
+    for (const auto& m : s->mats)
+        if (m.mod_count > 1) return yg_fail(YG_ERR_UNSUPPORTED, "adjoint supports one modulator per material");
+    if (!s->mods.empty() && !d_grad_mod) return yg_fail(YG_ERR_ARG, "d_grad_mod required");
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<ScenePlan> plans(1);
+    int rc = plan_scene(s, opts, d_rgb, plans[0]);
+    if (rc) return rc;
+    if (plans[0].R.depth > ADJ_MAXD) return yg_fail(YG_ERR_UNSUPPORTED, "adjoint supports depth <= 8");
+    const bool timing = opts->flags & YG_RENDER_TIMING;
+    RenderWS& W = g_ws;
+    const bool reuse = (opts->flags & YG_RENDER_REUSE) && W.last_scene == s && W.last_gen == s->gen &&
+                       same_opts(W.last_opts, *opts);
+    if (!reuse) {
+        W.last_scene = nullptr;
+        if ((rc = render_plans(plans, st, stats, timing, false))) return rc;
+        W.last_scene = s;
+        W.last_gen = s->gen;
+        W.last_opts = *opts;
+    }
+    AdjAcc G;
+    G.mat = d_grad_mat;
+    G.mod = d_grad_mod;
+    G.light = d_grad_light;
+    G.nmat = (int)s->mats.size();
+    G.nmod = (int)s->mods.size();
+    G.nl = (int)s->lights.size();
+    YG_HIP(hipMemsetAsync(d_grad_mat, 0, sizeof(double) * 3 * G.nmat, st));
+    if (G.nmod) YG_HIP(hipMemsetAsync(d_grad_mod, 0, sizeof(double) * 2 * G.nmod, st));
+    YG_HIP(hipMemsetAsync(d_grad_light, 0, sizeof(double) * G.nl, st));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timing) {
+        YG_HIP(hipEventCreate(&e0));
+        YG_HIP(hipEventCreate(&e1));
+        YG_HIP(hipEventRecord(e0, st));
+    }
+    hipLaunchKernelGGL(k_adjoint_direct, dim3(grid_for(W.nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, W.na,
+                       W.nitems, W.item_ti.p, W.item_t.p, d_grad_rgb, G);
+    YG_HIP(hipGetLastError());
+    if (W.any_path) {
+        uint32_t last_q = 0, last_off = 0;
+        YG_HIP(hipMemcpyAsync(&last_q, W.qual.p + W.nitems - 1, 4, hipMemcpyDeviceToHost, st));
+        YG_HIP(hipMemcpyAsync(&last_off, W.qoff.p + W.nitems - 1, 4, hipMemcpyDeviceToHost, st));
+        YG_HIP(hipStreamSynchronize(st));
+        const uint32_t n0 = (last_q + last_off) * (uint32_t)W.S;
+        hipLaunchKernelGGL(k_path_init, dim3(grid_for(W.nitems, 256)), dim3(256), 0, st, W.nitems, W.S, W.qual.p,
+                           W.qoff.p, W.active0.p);
+        YG_HIP(hipGetLastError());
+        if (n0 > 0) {
+            hipLaunchKernelGGL(k_adjoint_path, dim3(grid_for(n0, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, W.na,
+                               W.active0.p, n0, W.npaths, W.S, W.D, W.item_ti.p, W.item_t.p, W.qoff.p, W.hidx.p,
+                               d_grad_rgb, G);
+            YG_HIP(hipGetLastError());
+        }
+    }
+    if (timing) {
+        YG_HIP(hipEventRecord(e1, st));
+        YG_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        YG_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (stats) stats->ms_trace = ms;  // adjoint replay time
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    return YG_OK;
 }

@@ -16,6 +16,8 @@ struct DevLight {
     int pts;      // area: first entry of the sample point / jitter arrays
     int K;        // photon: search count
     float fixedradius;
+    float power;  // the light's "power" parameter
+    c3 color;     // the light's "color" parameter
 };
 
 // Photon map view for the K-nearest gather.

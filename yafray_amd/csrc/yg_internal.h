@@ -94,6 +94,7 @@ struct yg_scene {
     yg_texture* d_texs = nullptr;
 
     PhotonMapDev pm;
+    uint64_t gen = 0;  // bumped by every change of geometry, shading, lights or camera
 
     BVHView view() const
     {

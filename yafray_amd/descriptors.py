@@ -66,7 +66,9 @@ class yg_render_opts(C.Structure):
 class yg_render_stats(C.Structure):
     _fields_ = [("rays_camera", C.c_uint64), ("rays_extension", C.c_uint64),
                 ("rays_shadow", C.c_uint64), ("adaptive_extra", C.c_uint64),
-                ("lcg_draws", C.c_uint64), ("ms", C.c_float), ("ms_trace", C.c_float)]
+                ("lcg_draws", C.c_uint64), ("ms", C.c_float), ("ms_trace", C.c_float),
+                ("ms_bounce", C.c_float), ("bounce_launches", C.c_uint32),
+                ("rays_bounce", C.c_uint64)]
 
 
 class yg_build_opts(C.Structure):

@@ -58,19 +58,12 @@ def lib():
         "yg_photon_read": (C.c_int, [vp, vp, C.c_uint64, C.POINTER(C.c_uint64), vp]),
         "yg_photon_gather": (C.c_int, [vp, vp, vp, vp, vp, vp, u32, vp, vp]),
     }
-    optional = {
-        "yg_render_batch": (C.c_int, [C.POINTER(vp), u32, vp, C.POINTER(vp), vp, vp]),
-        "yg_render_adjoint": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
-    }
+    sigs["yg_render_batch"] = (C.c_int, [C.POINTER(vp), u32, vp, C.POINTER(vp), vp, vp])
+    sigs["yg_render_adjoint"] = (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp])
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
         fn.restype = res
         fn.argtypes = args
-    for name, (res, args) in optional.items():
-        if hasattr(L, name):
-            fn = getattr(L, name)
-            fn.restype = res
-            fn.argtypes = args
     _LIB = L
     return L
 
@@ -202,6 +195,15 @@ class Scene:
               "yg_render")
         return st
 
+    def render_adjoint(self, grad_rgb, out_rgb, gmat, gmod, glight, opts=None):
+        """Forward render into out_rgb and dLoss/dparams from dLoss/dpixel."""
+        o = opts if opts is not None else self.desc.opts
+        st = D.yg_render_stats()
+        check(lib().yg_render_adjoint(self.h, C.byref(o), _ptr(grad_rgb), _ptr(out_rgb), _ptr(gmat),
+                                      _ptr(gmod), _ptr(glight), C.byref(st), _stream()),
+              "yg_render_adjoint")
+        return st
+
     def photon_build(self, li, opts=None):
         o = opts if opts is not None else self.desc.opts
         st = D.yg_photon_stats()
@@ -218,6 +220,22 @@ class Scene:
     def photon_gather(self, p, n, ng, eye, mat, out):
         check(lib().yg_photon_gather(self.h, _ptr(p), _ptr(n), _ptr(ng), _ptr(eye), _ptr(mat),
                                      mat.shape[0], _ptr(out), _stream()), "yg_photon_gather")
+
+
+def render_batch(scenes, outs, opts=None, timing=False):
+    """One batched launch over several Scene objects; outs[k] gets scene k's image."""
+    n = len(scenes)
+    arr = (vp * n)(*[s.h for s in scenes])
+    optarr = (D.yg_render_opts * n)()
+    for k, s in enumerate(scenes):
+        o = D.yg_render_opts.from_buffer_copy(opts[k] if opts else s.desc.opts)
+        if timing:
+            o.flags |= 1
+        optarr[k] = o
+    outp = (vp * n)(*[vp(t.data_ptr()) for t in outs])
+    st = D.yg_render_stats()
+    check(lib().yg_render_batch(arr, n, optarr, outp, C.byref(st), _stream()), "yg_render_batch")
+    return st
 
 
 def noise_eval(what, tex, pts, out):

@@ -287,6 +287,52 @@ def photon_field(n_side=64, subdiv=4, extent=100.0, resx=1920, resy=1080,
     return D.SceneDesc(verts, idx, tri_obj, tri_mat, [7] * (n + 1), cam, mats, [], [], [light], opts)
 
 
+def terrain_diff(heights_fn, nx=1024, ny=512, res=512, samples=16, depth=3, nmat=256,
+                 amp=0.5, size=3.0, sun_power=3.0, albedo_lo=0.35, albedo_hi=0.9, seed_key=5):
+    """Differentiable GI: the terrain at res x res with nmat height-band
+    materials whose albedo is modulated (mode mul) by a Musgrave fBm texture;
+    amplitude = modulator colour, frequency = modulator size.  Albedos are
+    drawn in [albedo_lo, albedo_hi] so no path of depth 3 crosses the
+    pathlight's energy cut-off (pathlight.cc:315), which keeps the image
+    differentiable in every parameter."""
+    ex, ey = 2.0, 1.0
+    verts, idx, band = terrain_mesh(nx, ny, heights_fn, ex, ey, nmat)
+    rng = SplitMix64(sub_seed(seed_key))
+    alb = rng.uniform(3 * nmat, albedo_lo, albedo_hi).reshape(nmat, 3)
+    mats = [D.material(a, 0, 1) for a in alb]
+    texs = [D.musgrave(color1=(0.6, 0.6, 0.6), color2=(1.0, 1.0, 1.0), H=1.0, lacunarity=2.0,
+                       octaves=4.0, size=1.0, intensity=0.5)]
+    mods = [D.modulator(0, D.YG_MOD_MUL, color=amp, size=size)]
+    sun_dir = np.array([1.0, 0.5, 1.5])
+    sun_dir /= np.linalg.norm(sun_dir)
+    lights = [D.pathlight(samples=samples, power=1.0, depth=depth, use_qmc=True, threshold=1e3),
+              D.sunlight(sun_dir, power=sun_power)]
+    cam = terrain_camera(res, res, ex, ey)
+    opts = D.render_opts(bias=2e-3, raydepth=3, background=(0.4, 0.5, 0.7))
+    return D.SceneDesc(verts, idx, np.zeros(len(idx), np.int32), band, [7], cam, mats, mods, texs,
+                       lights, opts)
+
+
+def with_params(desc, albedo=None, amp=None, size=None, sun_power=None):
+    """Copy of a terrain_diff SceneDesc with some parameters replaced."""
+    mats = list(desc.mats)
+    if albedo is not None:
+        mats = [D.material(tuple(float(x) for x in a), m.mod_first, m.mod_count)
+                for a, m in zip(albedo, desc.mats)]
+    mods = [D.yg_modulator.from_buffer_copy(m) for m in desc.mods]
+    if amp is not None:
+        mods[0].color = float(amp)
+    if size is not None:
+        mods[0].size = D.F3(float(size), float(size), float(size))
+    lights = [D.yg_light.from_buffer_copy(L) for L in desc.lights]
+    if sun_power is not None:
+        for L in lights:
+            if L.type == D.YG_LIGHT_SUN:
+                L.power = float(sun_power)
+    return D.SceneDesc(desc.verts, desc.idx, desc.tri_obj, desc.tri_mat, desc.obj_flags, desc.cam,
+                       mats, mods, list(desc.texs), lights, desc.opts)
+
+
 BATCH_RES = [(256, 256), (640, 480), (1280, 720), (1920, 1080)]
 
 
