@@ -82,17 +82,174 @@ def parity_leg(cpu_threads):
     return {"cpu_dt": cpu_dt, "cpu_rays": cpu_rays, "rel_rmse": rel, "exact_px": exact}
 
 
+def _timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(steps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, out
+
+
+def bench_batched(args):
+    """64 heightfield scenes (1e3..2e6 triangles, four resolutions), 2 spp,
+    3 bounces: one batched launch vs 64 per-scene renders on the same GPU."""
+    import torch
+
+    from yafray_amd import lib as Y
+    from yafray_amd import scenes as S
+
+    specs = S.batch_specs(args.batch_n)
+    scenes, outs, tris, rays_px = [], [], 0, 0
+    for k, (nx, ny, rx, ry) in enumerate(specs):
+        d = S.terrain_gi(Y.device_heights, nx=nx, ny=ny, resx=rx, resy=ry, samples=2, depth=3,
+                         seed_key=100 + k)
+        scenes.append(Y.Scene.from_desc(d))
+        outs.append(torch.zeros(rx * ry * 3, dtype=torch.float32, device="cuda"))
+        tris += d.n_tris
+        rays_px += rx * ry
+    seq_dt, _ = _timed(lambda: [s.render(o) for s, o in zip(scenes, outs)], args.steps, args.warmup)
+    ref = [o.clone() for o in outs]
+    bat_dt, st = _timed(lambda: Y.render_batch(scenes, outs), args.steps, args.warmup)
+    same = all(torch.equal(a, b) for a, b in zip(ref, outs))
+    rays = st.rays_camera + st.rays_extension + st.rays_shadow
+    return {
+        "metric": "batched GI ms/iter", "value": round(bat_dt * 1e3, 3), "unit": "ms/iter",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(bat_dt * 1e3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "%d heightfield scenes, triangles log-uniform in [1e3, 2e6], "
+                               "resolutions from 256^2..1920x1080, 2 spp, 3 bounces" % len(specs),
+                   "triangles": int(tris), "pixels": int(rays_px)},
+        "sequential_ms_per_iter": round(seq_dt * 1e3, 3),
+        "speedup_vs_sequential": round(seq_dt / bat_dt, 3),
+        "batched_mrays_per_s": round(rays / bat_dt / 1e6, 2),
+        "images_identical_to_sequential": bool(same),
+    }
+
+
+def bench_adjoint(args):
+    """Differentiable GI: terrain at 512x512, 16 spp, 3 bounces, 256 materials,
+    Musgrave-modulated albedo; forward vs forward+adjoint (replay) time."""
+    import torch
+
+    from yafray_amd import lib as Y
+    from yafray_amd import scenes as S
+    from yafray_amd.autograd import render_gi
+
+    d = S.terrain_diff(Y.device_heights)
+    sc = Y.Scene.from_desc(d)
+    out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+    fwd_dt, _ = _timed(lambda: sc.render(out), args.steps, args.warmup)
+    import numpy as np
+    alb0 = np.array([list(m.color) for m in d.mats])
+    target = render_gi(sc, torch.tensor(alb0 * 0.9 + 0.04), torch.tensor(0.4), torch.tensor(2.7),
+                       torch.tensor(2.5)).detach().clone()
+    albedo = torch.tensor(alb0, requires_grad=True)
+    amp = torch.tensor(0.5, requires_grad=True)
+    size = torch.tensor(3.0, requires_grad=True)
+    power = torch.tensor(3.0, requires_grad=True)
+
+    def step():
+        for t in (albedo, amp, size, power):
+            t.grad = None
+        im = render_gi(sc, albedo, amp, size, power)
+        L = ((im - target) ** 2).sum()
+        L.backward()
+        return float(L)
+
+    both_dt, loss = _timed(step, args.steps, args.warmup)
+    return {
+        "metric": "fwd+adjoint ms/iter", "value": round(both_dt * 1e3, 3), "unit": "ms/iter",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(both_dt * 1e3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "differentiable GI: 1,045,506-tri terrain, 512x512, 16 spp, 3 bounces, "
+                               "256 materials, Musgrave albedo texture (mul)",
+                   "params": 256 * 3 + 3},
+        "forward_ms": round(fwd_dt * 1e3, 3),
+        "ratio_fwd_adj_over_fwd": round(both_dt / fwd_dt, 3),
+        "loss": loss,
+    }
+
+
+def bench_photon(args):
+    """Photon mapping: 2M photons from a 45-degree spot into 4,096 icospheres
+    (20,971,522 triangles), K = 50 gather at 1920x1080."""
+    import torch
+
+    from yafray_amd import lib as Y
+    from yafray_amd import scenes as S
+
+    t0 = time.time()
+    d = S.photon_field()
+    sc = Y.Scene.from_desc(d)
+    setup_s = time.time() - t0
+    pst = sc.photon_build(0)
+    build_dt, _ = _timed(lambda: sc.photon_build(0), max(1, args.steps // 2), 0)
+    out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+    rdt, st = _timed(lambda: sc.render(out), args.steps, args.warmup)
+    return {
+        "metric": "photon map ms/frame", "value": round((build_dt + rdt) * 1e3, 3), "unit": "ms/frame",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round((build_dt + rdt) * 1e3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "4096 icospheres (20,971,522 tris), 1,999,396 photons, K=50, 1920x1080",
+                   "triangles": int(d.n_tris)},
+        "photon_build_ms": round(build_dt * 1e3, 3), "render_ms": round(rdt * 1e3, 3),
+        "emitted": int(pst.emitted), "stored": int(pst.stored), "clustered": int(pst.clustered),
+        "scene_setup_s": round(setup_s, 1),
+    }
+
+
+def bench_direct(args):
+    """Plain direct lighting: box room + icosphere + two boxes, 1x1 area light
+    with 16 samples, 640x480."""
+    import torch
+
+    from yafray_amd import lib as Y
+    from yafray_amd import scenes as S
+
+    d = S.direct_room()
+    sc = Y.Scene.from_desc(d)
+    out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+    dt, st = _timed(lambda: sc.render(out), args.steps, args.warmup)
+    rays = st.rays_camera + st.rays_extension + st.rays_shadow
+    return {
+        "metric": "direct-lighting Mrays/s", "value": round(rays / dt / 1e6, 3), "unit": "Mrays/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "room + icosphere(20480) + 2 boxes, area light 16 samples, 640x480",
+                   "triangles": int(d.n_tris)},
+        "rays_per_frame": int(rays), "lcg_draws": int(st.lcg_draws),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="gi", choices=["gi", "batched", "adjoint", "photon", "direct"])
+    ap.add_argument("--batch-n", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(rendering would shard by image tiles; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "gi":
+        import torch
+        torch.cuda.set_device(0)
+        fn = {"batched": bench_batched, "adjoint": bench_adjoint, "photon": bench_photon,
+              "direct": bench_direct}[args.workload]
+        print(json.dumps(fn(args)))
+        return
 
     g = gpu_leg(args)
     ms = 1000.0 * g["dt"] / args.steps

@@ -1,0 +1,72 @@
+// Drives the C++ host adapter (factories by plugin name) without a GPU and
+// prints the descriptors it would hand to libyafgpu.so, one per line.
+#include <cstdio>
+
+#include "yafgpu_host.h"
+
+using yafgpu::paramMap_t;
+
+static void pl(const yg_light& L)
+{
+    printf("light %d %d %d %g %g %g %g %g %g %g %d %d %d %d %g %g %g %g %d %d %d %g %g %g\n", L.type,
+           L.use_in_render, L.use_in_indirect, L.color[0], L.color[1], L.color[2], L.power, L.dir[0], L.dir[1],
+           L.dir[2], L.cast_shadows, L.samples, L.depth, L.use_qmc, L.threshold, L.from[2], L.to[2], L.angle,
+           L.photons, L.search, L.mindepth, L.bias, L.fixedradius, L.cluster);
+}
+
+int main()
+{
+    yafgpu::gpuInterface_t I;
+    paramMap_t sun;
+    sun.setPoint("from", 1, 0.5f, 1.5f);
+    sun.setFloat("power", 3.0f);
+    paramMap_t path;
+    path.setInt("samples", 4);
+    path.setBool("use_QMC", true);
+    path.setFloat("threshold", 1000.0f);
+    paramMap_t ph;
+    ph.setBool("use_QMC", true);
+    ph.setString("mode", "diffuse");
+    ph.setFloat("fixedradius", 5.0f);
+    ph.setInt("photons", 3998792);
+    ph.setInt("depth", 1);
+    ph.setInt("mindepth", 0);
+    paramMap_t area;
+    area.setPoint("a", 0, 0, 1);
+    area.setInt("samples", 16);
+    area.setFloat("power", 10.0f);
+    area.setColor("color", 1, 1, 1);
+    // names out of alphabetical order on purpose: the interface sorts them
+    if (I.addLight("sun", "sunlight", sun) || I.addLight("path", "pathlight", path) ||
+        I.addLight("photons", "photonlight", ph) || I.addLight("area", "arealight", area))
+        return 1;
+    paramMap_t mus;
+    mus.setString("type", "fBm");
+    mus.setFloat("octaves", 4.0f);
+    mus.setFloat("intensity", 0.5f);
+    if (I.addTexture("mus", "musgrave", mus)) return 2;
+    paramMap_t sh;
+    sh.setColor("color", 0.5f, 0.4f, 0.3f);
+    paramMap_t mod;
+    mod.setString("texname", "mus");
+    mod.setString("mode", "mul");
+    mod.setString("texco", "global");
+    mod.setFloat("color", 0.6f);
+    mod.setFloat("size", 3.0f);
+    if (I.addShader("ground", "generic", sh, {mod})) return 3;
+    std::vector<yg_light> lights;
+    std::vector<yg_material> mats;
+    std::vector<yg_modulator> mods;
+    std::vector<yg_texture> texs;
+    if (I.describe(lights, mats, mods, texs)) return 4;
+    for (auto& L : lights) pl(L);
+    for (auto& m : mats) printf("mat %g %g %g %d %d\n", m.color[0], m.color[1], m.color[2], m.mod_first, m.mod_count);
+    for (auto& m : mods)
+        printf("mod %d %d %g %g %g %g\n", m.tex, m.mode, m.color, m.size[0], m.size[1], m.size[2]);
+    for (auto& t : texs) printf("tex %d %g %g %g %g\n", t.type, t.H, t.lacunarity, t.octaves, t.iscale);
+    // unsupported features are refused with a status, not an exit
+    paramMap_t cache;
+    cache.setBool("cache", true);
+    printf("refused %d\n", I.addLight("cached", "pathlight", cache));
+    return 0;
+}

@@ -127,7 +127,8 @@ def test_terrain_parity_full_size(torch_cuda):
     oc = po.OracleScene(verts, idx, None, band)
     pc, tc, uc, vc = oc.trace(rays[sub], threads=8)
     _check(pg[sub], tg[sub], ug[sub], vg[sub], pc, tc, uc, vc)
-    assert np.mean(pg >= 0) > 0.9
+    # a good part of the downward rays leave the [0,2]x[0,1] terrain before landing
+    assert np.mean(pg >= 0) > 0.3
 
 
 def test_first_hit_is_shadowed(torch_cuda):

@@ -1,0 +1,63 @@
+"""The C++ host adapter maps the reference plugin parameters to the same
+descriptors as yafray_amd.descriptors (no GPU needed)."""
+import os
+import subprocess
+
+import pytest
+
+from yafray_amd import descriptors as D
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def adapter_output(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("host") / "host_adapter_check")
+    lib = os.path.join(ROOT, "yafray_amd")
+    cmd = ["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "yafray_amd", "host"),
+           os.path.join(ROOT, "tests", "host_adapter_check.cpp"), "-L", lib, "-lyafgpu",
+           "-Wl,-rpath," + lib, "-Wl,-rpath,/opt/rocm/lib", "-o", exe]
+    subprocess.run(cmd, check=True, capture_output=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return [l.split() for l in r.stdout.strip().splitlines()]
+
+
+def _f(x):
+    return float(x)
+
+
+def test_lights_sorted_and_defaults(adapter_output):
+    lights = [l for l in adapter_output if l[0] == "light"]
+    # alphabetical: area, path, photons, sun
+    assert [int(l[1]) for l in lights] == [D.YG_LIGHT_AREA, D.YG_LIGHT_PATH, D.YG_LIGHT_PHOTON, D.YG_LIGHT_SUN]
+    area, path, ph, sun = lights
+    ref = D.pathlight(samples=4, use_qmc=True, threshold=1000.0)
+    assert int(path[12]) == ref.samples and int(path[13]) == ref.depth and _f(path[15]) == pytest.approx(1000.0)
+    assert int(path[3]) == 0  # pathlight is not used in indirect light
+    r = D.photonlight(photons=3998792, depth=1, mindepth=0, fixedradius=5.0, use_qmc=True)
+    assert int(ph[19]) == r.photons and int(ph[20]) == r.search and int(ph[21]) == r.mindepth
+    assert _f(ph[18]) == pytest.approx(r.angle) and _f(ph[22]) == pytest.approx(r.bias)
+    assert _f(ph[24]) == pytest.approx(r.cluster)  # cluster defaults to fixedradius / 4
+    s = D.sunlight((1, 0.5, 1.5), power=3.0)
+    assert _f(sun[7]) == pytest.approx(s.power) and int(sun[11]) == 1
+    assert int(area[12]) == 16 and _f(area[7]) == pytest.approx(10.0)
+
+
+def test_shader_modulator_texture(adapter_output):
+    mats = [l for l in adapter_output if l[0] == "mat"]
+    mods = [l for l in adapter_output if l[0] == "mod"]
+    texs = [l for l in adapter_output if l[0] == "tex"]
+    assert mats == [["mat", "0.5", "0.4", "0.3", "0", "1"]]
+    assert int(mods[0][2]) == D.YG_MOD_MUL and _f(mods[0][3]) == pytest.approx(0.6)
+    assert [_f(x) for x in mods[0][4:7]] == [3.0, 3.0, 3.0]
+    t = D.musgrave(octaves=4.0, intensity=0.5)
+    assert int(texs[0][1]) == D.YG_TEX_MUSGRAVE
+    assert [_f(x) for x in texs[0][2:6]] == [t.H, t.lacunarity, t.octaves, t.iscale]
+
+
+def test_unsupported_is_refused(adapter_output):
+    assert [l for l in adapter_output if l[0] == "refused"] == [["refused", str(D_ERR_UNSUPPORTED)]]
+
+
+D_ERR_UNSUPPORTED = -4

@@ -12,7 +12,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libyafgpu.so")
 
-SOURCES = ["capi.cpp", "qmc.cpp", "bvh_build.cpp", "trace.hip", "render.hip", "photon.hip"]
+SOURCES = ["capi.cpp", "qmc.cpp", "bvh_build.cpp", "trace.hip", "render.hip", "photon.hip",
+           "../host/yafgpu_host.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("YG_OFFLOAD_ARCH", "gfx950")
@@ -29,7 +30,7 @@ FLAGS = [
 
 
 def _compile(src):
-    out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    out = os.path.join(OBJ, os.path.splitext(os.path.basename(src))[0] + ".o")
     cmd = [HIPCC] + FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

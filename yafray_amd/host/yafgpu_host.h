@@ -1,0 +1,96 @@
+// Host-side mirror of the reference's plugin interface for the GPU path.
+//
+// The reference builds lights, shaders, textures and backgrounds through
+// factories registered by name that read a paramMap_t
+// (src/interface/external.h:112-117, params.h:54-73; e.g. pathLight_t::factory
+// pathlight.cc:753-807) and ingests meshes through addObject_trimesh
+// (interface_impl.cc:193-265).  This header offers the same names, parameter
+// keys, defaults and error behaviour (factories return an error instead of an
+// object, never exit), but each factory produces the POD descriptor of
+// include/yafgpu.h, and render() runs the frame through libyafgpu.so.
+#pragma once
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/yafgpu.h"
+
+namespace yafgpu {
+
+// paramMap_t (params.h:54-73): string keys to typed values.
+class paramMap_t {
+public:
+    void setFloat(const std::string& k, float v) { f_[k] = v; }
+    void setInt(const std::string& k, int v) { i_[k] = v; }
+    void setBool(const std::string& k, bool v) { b_[k] = v; }
+    void setString(const std::string& k, const std::string& v) { s_[k] = v; }
+    void setPoint(const std::string& k, float x, float y, float z) { p_[k] = {x, y, z}; }
+    void setColor(const std::string& k, float r, float g, float b) { p_[k] = {r, g, b}; }
+
+    // getParam: leaves `out` untouched and returns false when the key is absent
+    bool getParam(const std::string& k, float& out) const;
+    bool getParam(const std::string& k, int& out) const;
+    bool getParam(const std::string& k, bool& out) const;
+    bool getParam(const std::string& k, std::string& out) const;
+    bool getParam(const std::string& k, float out[3]) const;
+
+private:
+    std::map<std::string, float> f_;
+    std::map<std::string, int> i_;
+    std::map<std::string, bool> b_;
+    std::map<std::string, std::string> s_;
+    std::map<std::string, std::vector<float>> p_;
+};
+
+// Factories by plugin name, with the reference defaults.  Each returns YG_OK
+// and fills the descriptor, or a negative status (message in lastError()).
+int lightFactory(const std::string& type, const paramMap_t& p, yg_light& out);
+int textureFactory(const std::string& type, const paramMap_t& p, yg_texture& out);
+// genericShader_t::factory (basicshaders.cc:238-367): base params + one
+// paramMap_t per modulator; texture names resolve through `textures`.
+int shaderFactory(const std::string& type, const paramMap_t& p, const std::vector<paramMap_t>& mods,
+                  const std::map<std::string, int>& textures, yg_material& out,
+                  std::vector<yg_modulator>& mods_out);
+const std::string& lastError();
+
+// renderEnvironment / yafrayInterface_t subset for the GPU path.
+class gpuInterface_t {
+public:
+    gpuInterface_t();
+    ~gpuInterface_t();
+    int addTexture(const std::string& name, const std::string& type, const paramMap_t& p);
+    int addShader(const std::string& name, const std::string& type, const paramMap_t& p,
+                  const std::vector<paramMap_t>& mods);
+    int addLight(const std::string& name, const std::string& type, const paramMap_t& p);
+    int addBackground(const std::string& name, const paramMap_t& p);
+    int addCamera(const std::string& name, const paramMap_t& p);
+    // addObject_trimesh: world-space vertices, index triples, one shader name
+    // per face (or a single name for all faces)
+    int addObject_trimesh(const std::string& name, const std::vector<float>& verts,
+                          const std::vector<uint32_t>& faces, const std::vector<std::string>& shaders,
+                          bool cast_shadows = true, bool recv_rad = true, bool use_rad = true);
+    // render params: camera_name, background_name, bias, raydepth; rgb gets
+    // resx*resy*3 floats (HDR, no tone mapping).  Photon lights are built first.
+    int render(const paramMap_t& params, std::vector<float>& rgb, yg_render_stats* stats = nullptr);
+
+    // the descriptors the next render would use (object order, light order)
+    int describe(std::vector<yg_light>& lights, std::vector<yg_material>& mats,
+                 std::vector<yg_modulator>& mods, std::vector<yg_texture>& texs);
+
+private:
+    struct Obj {
+        std::vector<float> v;
+        std::vector<uint32_t> f;
+        std::vector<std::string> sh;
+        int flags;
+    };
+    std::map<std::string, yg_texture> textures_;
+    std::map<std::string, std::pair<paramMap_t, std::vector<paramMap_t>>> shaders_;
+    std::map<std::string, yg_light> lights_;
+    std::map<std::string, std::vector<float>> backgrounds_;
+    std::map<std::string, yg_camera> cameras_;
+    std::map<std::string, Obj> objects_;
+};
+
+}  // namespace yafgpu
