@@ -51,35 +51,44 @@ def gpu_leg(args):
     torch.cuda.synchronize()
     img = out.cpu().numpy().reshape(d.cam.resy, d.cam.resx, 3)
     return {
+        "desc": d, "img": img,
         "dt": dt, "rays": rays, "stats": ks, "info": info, "setup_s": setup_s,
         "finite": bool(np.all(np.isfinite(img))), "mean": float(img.mean()),
     }
 
 
-def parity_leg(cpu_threads):
-    """Bounded CPU sample: the same terrain and camera at 320x180, rendered by
-    the oracle (timed on the host cores) and by the GPU (for the RMSE)."""
+def parity_leg(d, img, cpu_threads):
+    """Bounded CPU sample: the same full 1280x720 frame rendered by the oracle
+    on the host cores (timed), compared pixel by pixel with the GPU image."""
     import numpy as np
-    import torch
 
     from oracle import pyoracle as po
-    from yafray_amd import lib as Y
-    from yafray_amd import scenes as S
 
-    d = S.terrain_gi(Y.device_heights, resx=320, resy=180)
     oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
     t0 = time.perf_counter()
     ref, rs = oc.render(d, threads=cpu_threads)
     cpu_dt = time.perf_counter() - t0
     cpu_rays = int(rs[0] + rs[1] + rs[2])
-    sc = Y.Scene.from_desc(d)
-    out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
-    sc.render(out)
-    torch.cuda.synchronize()
-    img = out.cpu().numpy().reshape(ref.shape).astype(np.float64)
-    rel = float(np.sqrt(np.mean((img - ref) ** 2)) / np.sqrt(np.mean(ref.astype(np.float64) ** 2)))
+    img = img.reshape(ref.shape).astype(np.float64)
+    ref = ref.astype(np.float64)
+    rel = float(np.sqrt(np.mean((img - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)))
     exact = float(np.mean(np.all(img == ref, axis=2)))
     return {"cpu_dt": cpu_dt, "cpu_rays": cpu_rays, "rel_rmse": rel, "exact_px": exact}
+
+
+def copy_probe():
+    """A plain 1 GiB device-to-device copy (torch's copy kernel), run after the
+    timed region so that a counter run of this script can scale the bounce
+    kernel's FETCH_SIZE / WRITE_SIZE by what the counters read for a known
+    number of bytes."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return {"bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 2}
 
 
 def _timed(fn, steps, warmup):
@@ -238,6 +247,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="gi", choices=["gi", "batched", "adjoint", "photon", "direct"])
     ap.add_argument("--batch-n", type=int, default=64)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a 1 GiB device copy after the timed region (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -301,19 +312,21 @@ def main():
                 "dev_bytes": int(g["info"]["dev_bytes"])},
         "image_finite": g["finite"],
     }
+    if args.copy_probe:
+        line["copy_probe"] = copy_probe()
     if not args.no_cpu:
         cores = min(os.cpu_count() or 1, 16)
-        p = parity_leg(cores)
+        p = parity_leg(g["desc"], g["img"], cores)
         line["cpu_baseline"] = {
             "value": round(p["cpu_rays"] / p["cpu_dt"] / 1e6, 4),
             "unit": "Mrays/s",
             "cores": cores,
             "kind": "port",
-            "sample": "oracle render of the same terrain and camera at 320x180 (4 spp, 3 bounces), "
+            "sample": "oracle render of the full 1280x720 frame (4 spp, 3 bounces), "
                       "%d rays in %.1f s" % (p["cpu_rays"], p["cpu_dt"]),
         }
         line["rmse_vs_oracle"] = {"rel_rmse": p["rel_rmse"], "bit_exact_pixels": p["exact_px"],
-                                  "sample": "320x180 render of the same scene"}
+                                  "sample": "full 1280x720 frame, GPU image vs oracle image"}
     print(json.dumps(line))
 
 

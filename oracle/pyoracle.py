@@ -28,6 +28,8 @@ def lib():
         vp = C.c_void_p
         L.yo_scene_new.restype = vp
         L.yo_scene_new.argtypes = [F32P, C.c_int64, U32P, C.c_int64, vp, vp]
+        L.yo_log_begin.argtypes = [vp, C.c_int64, vp, C.c_int64]
+        L.yo_log_end.argtypes = [C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.yo_scene_free.argtypes = [vp]
         L.yo_scene_nodes.restype = C.c_int64
         L.yo_scene_nodes.argtypes = [vp]
@@ -161,6 +163,24 @@ class OracleScene:
         if rc != 0:
             raise RuntimeError("yo_render failed: %d" % rc)
         return rgb.reshape(desc.cam.resy, desc.cam.resx, 3), stats
+
+    def render_logged(self, desc, threads=1, ext_cap=0, sh_cap=0):
+        """render() that also returns every first-hit and shadow query the
+        render made: (img, stats, ext[n,8], sh[n,8]) with the record layouts
+        of yo_log_begin (int fields are int32 bit patterns in the floats)."""
+        L = lib()
+        ext = np.zeros((max(ext_cap, 1), 8), np.float32)
+        sh = np.zeros((max(sh_cap, 1), 8), np.float32)
+        L.yo_log_begin(_ptr(ext), ext_cap, _ptr(sh), sh_cap)
+        try:
+            img, stats = self.render(desc, threads)
+        finally:
+            ne, ns = C.c_int64(), C.c_int64()
+            L.yo_log_end(C.byref(ne), C.byref(ns))
+        if ne.value > ext_cap or ns.value > sh_cap:
+            raise RuntimeError("query log overflow: %d/%d first-hit, %d/%d shadow"
+                               % (ne.value, ext_cap, ns.value, sh_cap))
+        return img, stats, ext[:ne.value], sh[:ns.value]
 
     def render_adjoint(self, desc, grad_rgb):
         """FP64 reference gradient: returns (image, gmat[nmat,3], gmod[nmod,2], glight[nl])."""

@@ -59,6 +59,10 @@ void yo_is_shadowed(yo_scene* s, const float* p, const float* l, const int32_t* 
 
 /* ------------------------------------------------------------------ QMC -- */
 /* n successive Halton::getNext() values of a fresh Halton(base) (mcqmc.h:35-49) */
+/* query log of yo_first_hit_one / yo_is_shadowed_one (8 floats per record) */
+void yo_log_begin(float* ext, int64_t ext_cap, float* sh, int64_t sh_cap);
+void yo_log_end(int64_t* ext_n, int64_t* sh_n);
+
 void yo_halton_seq(int base, int64_t n, float* out);
 /* index-addressed value via Halton::setStart's digit loop (mcqmc.h:25-33) */
 float yo_halton_at(int base, uint32_t i);

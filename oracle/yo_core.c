@@ -87,6 +87,8 @@ void yo_tri_bary(const float* a, const float* b, const float* c, const float* n,
 
 #define YO_BINS 32
 #define YO_LEAF 4
+#define YO_TEPS 6.103515625e-05  /* 2^-14 */
+#define YO_HEPS 7.8125e-03       /* 2^-7 */
 
 typedef struct { double lo[3], hi[3]; } box_d;
 
@@ -120,6 +122,7 @@ typedef struct {
     box_d* tb;         /* per-triangle boxes */
     double* cen;       /* per-triangle centroids */
     int64_t cap;
+    double G;          /* largest vertex coordinate magnitude */
 } build_ctx;
 
 static int32_t node_alloc(build_ctx* bc)
@@ -142,9 +145,15 @@ static void build_rec(build_ctx* bc, int32_t ni, int64_t start, int64_t count)
         box_grow(&nb, &bc->tb[ord[i]]);
         box_grow_pt(&cb, &bc->cen[3 * (int64_t)ord[i]]);
     }
-    /* inflate so float-rounded hit points stay inside */
+    /* inflate so that every triangle the float edge test accepts is reached:
+     * that test's slack scales with the ray origin's distance and the
+     * scene's magnitude G (far rays over small triangles pass it up to ~1e-5
+     * outside the triangle), not with this box's own coordinates, so boxes
+     * grow by 5% of their extent plus 2^-14 G (a kd-tree cell of the
+     * reference is likewise much larger than that slack) */
     for (int k = 0; k < 3; k++) {
-        double e = 1e-7 * (fabs(nb.lo[k]) + fabs(nb.hi[k]) + 1e-30);
+        double e = 0.05 * (nb.hi[k] - nb.lo[k]) + 1e-7 * (fabs(nb.lo[k]) + fabs(nb.hi[k]) + 1e-30) +
+                   bc->G * 6.103515625e-05;
         s->nodes[ni].lo[k] = nb.lo[k] - e;
         s->nodes[ni].hi[k] = nb.hi[k] + e;
     }
@@ -227,7 +236,7 @@ yo_scene* yo_scene_new(const float* verts, int64_t nv, const uint32_t* idx, int6
     s->tri_n = (float*)malloc(sizeof(float) * 3 * (size_t)(nt ? nt : 1));
     s->order = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nt ? nt : 1));
     build_ctx bc;
-    bc.s = s; bc.cap = 0;
+    bc.s = s; bc.cap = 0; bc.G = 0.0;
     bc.tb = (box_d*)malloc(sizeof(box_d) * (size_t)(nt ? nt : 1));
     bc.cen = (double*)malloc(sizeof(double) * 3 * (size_t)(nt ? nt : 1));
     for (int64_t i = 0; i < nt; i++) {
@@ -237,7 +246,11 @@ yo_scene* yo_scene_new(const float* verts, int64_t nv, const uint32_t* idx, int6
         box_empty(&bc.tb[i]);
         double pa[3] = {a.x, a.y, a.z}, pb[3] = {b.x, b.y, b.z}, pc[3] = {c.x, c.y, c.z};
         box_grow_pt(&bc.tb[i], pa); box_grow_pt(&bc.tb[i], pb); box_grow_pt(&bc.tb[i], pc);
-        for (int k = 0; k < 3; k++) bc.cen[3 * i + k] = 0.5 * (bc.tb[i].lo[k] + bc.tb[i].hi[k]);
+        for (int k = 0; k < 3; k++) {
+            bc.cen[3 * i + k] = 0.5 * (bc.tb[i].lo[k] + bc.tb[i].hi[k]);
+            if (fabs(bc.tb[i].lo[k]) > bc.G) bc.G = fabs(bc.tb[i].lo[k]);
+            if (fabs(bc.tb[i].hi[k]) > bc.G) bc.G = fabs(bc.tb[i].hi[k]);
+        }
         s->order[i] = (int32_t)i;
     }
     int32_t root = node_alloc(&bc);
@@ -277,8 +290,10 @@ static int box_hit(const yo_node* nd, const double* o, const double* d, double t
         if (ta > tb) { double x = ta; ta = tb; tb = x; }
         if (ta > t0) t0 = ta;
         if (tb < t1) t1 = tb;
-        if (t0 > t1) return 0;
     }
+    /* relative slack: along grazing rays the float plane distance of an
+     * accepted triangle can fall a little outside its box's interval */
+    if (t0 - YO_TEPS * fabs(t0) > t1 + YO_TEPS * fabs(t1)) return 0;
     *tnear = t0;
     return 1;
 }
@@ -301,7 +316,10 @@ int yo_query(const yo_scene* s, yo_v3 o, yo_v3 d, float tmin, float tmax, int mo
         const yo_node* nd = &s->nodes[stack[--sp]];
         double tn;
         double hi = (double)bt;
-        if (isinf(hi)) hi = DBL_MAX; else hi = hi + 1e-6 * (fabs(hi) + 1.0);
+        /* near a shared vertex a grazing ray can pass the float edge test of
+         * a triangle at a plane distance in front of its box: boxes are kept
+         * until they start 2^-7 (relative) beyond the bound */
+        if (isinf(hi)) hi = DBL_MAX; else hi = hi + YO_HEPS * fabs(hi) + 1e-6;
         if (cn) (*cn)++;
         if (!box_hit(nd, od, dd, span_lo, hi, &tn)) continue;
         if (nd->left < 0) {
@@ -426,6 +444,38 @@ void yo_trace_bvh(yo_scene* s, const float* rays, int64_t n, int mode, int32_t* 
     if (tris) *tris = st;
 }
 
+/* Query log: while enabled, every yo_first_hit_one / yo_is_shadowed_one call
+ * appends its inputs and its answer (8 floats per record), so that a full
+ * render's queries can be replayed elsewhere and compared one by one.
+ *   first hit: from xyz, ray xyz, prim (int32 bits), t
+ *   shadow:    p xyz, l xyz, self_obj (int32 bits), to_point*2 + result (int32 bits) */
+static struct {
+    float *ext, *sh;
+    int64_t ext_cap, sh_cap, ext_n, sh_n;
+    int on;
+} g_log;
+
+void yo_log_begin(float* ext, int64_t ext_cap, float* sh, int64_t sh_cap)
+{
+    g_log.ext = ext; g_log.ext_cap = ext ? ext_cap : 0;
+    g_log.sh = sh; g_log.sh_cap = sh ? sh_cap : 0;
+    g_log.ext_n = 0; g_log.sh_n = 0;
+    g_log.on = 1;
+}
+
+void yo_log_end(int64_t* ext_n, int64_t* sh_n)
+{
+    g_log.on = 0;
+    if (ext_n) *ext_n = g_log.ext_n;
+    if (sh_n) *sh_n = g_log.sh_n;
+}
+
+static void log_put(float* buf, int64_t cap, int64_t* n, const float* rec)
+{
+    int64_t k = __atomic_fetch_add(n, 1, __ATOMIC_RELAXED);
+    if (k < cap) memcpy(buf + 8 * k, rec, 8 * sizeof(float));
+}
+
 /* scene_t::firstHit, scene.cc:688-721: f = from + ray*min_raydis; closest hit
  * with Z > 0 (a zero-distance closest hit counts as a miss, as the per-object
  * kd result would be rejected by the Z > 0 test). */
@@ -438,6 +488,11 @@ int yo_first_hit_one(const yo_scene* s, yo_v3 from, yo_v3 ray, float min_raydis,
     if (p >= 0 && !(tt > 0.0f)) p = -1;
     *prim = p;
     *t = tt;
+    if (g_log.on) {
+        float rec[8] = {from.x, from.y, from.z, ray.x, ray.y, ray.z, 0, tt};
+        memcpy(&rec[6], &p, 4);
+        log_put(g_log.ext, g_log.ext_cap, &g_log.ext_n, rec);
+    }
     return p >= 0;
 }
 
@@ -452,8 +507,25 @@ void yo_first_hit(yo_scene* s, const float* from, const float* dir, int64_t n, f
  * direction is normalised; the shading point's own object is tested from
  * p + ray*self_bias, every other object from p + ray*min_raydis; any hit with
  * 0 < t < dist occludes (dist = |l-p| or infinity). */
+static int is_shadowed_q(const yo_scene* s, yo_v3 p, yo_v3 l, int self_obj, int to_point,
+                         float min_raydis, float self_bias);
+
 int yo_is_shadowed_one(const yo_scene* s, yo_v3 p, yo_v3 l, int self_obj, int to_point,
                        float min_raydis, float self_bias)
+{
+    const int r = is_shadowed_q(s, p, l, self_obj, to_point, min_raydis, self_bias);
+    if (g_log.on) {
+        float rec[8] = {p.x, p.y, p.z, l.x, l.y, l.z, 0, 0};
+        const int32_t code = to_point * 2 + r;
+        memcpy(&rec[6], &self_obj, 4);
+        memcpy(&rec[7], &code, 4);
+        log_put(g_log.sh, g_log.sh_cap, &g_log.sh_n, rec);
+    }
+    return r;
+}
+
+static int is_shadowed_q(const yo_scene* s, yo_v3 p, yo_v3 l, int self_obj, int to_point,
+                         float min_raydis, float self_bias)
 {
     yo_v3 ray;
     float dist;

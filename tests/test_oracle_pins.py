@@ -183,6 +183,38 @@ def test_bvh_equals_brute_force_room():
     assert np.array_equal(av >= 0, (pc >= 0) & (tc < 1.0) & (tc > 0))
 
 
+def test_bvh_equals_brute_force_grazing_terrain():
+    """Camera rays over the full 1,045,506-triangle fBm terrain: a ray that
+    crosses a terrain cell near z = 0 at a shallow angle is accepted by the
+    float edge test while passing a few 1e-8 outside the cell's exact box, so
+    a box inflation relative to the box's own coordinates alone loses it (the
+    oracle once answered "miss" here)."""
+    d = S.terrain_gi(po.heightfield)
+    sc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    h = [float.fromhex(x) for x in ("0x1p+0", "-0x1.cccccc0000000p-2", "0x1.19999a0000000p+0",
+                                    "0x1.516f800000000p-4", "0x1.65bef60000000p-1",
+                                    "-0x1.6bd8260000000p-1")]
+    v = np.array(h, np.float32)
+    p, t = sc.first_hit(v[:3], v[3:], d.opts.min_raydis)
+    f = v[:3] + v[3:] * np.float32(d.opts.min_raydis)
+    ray = np.array([[*f, 0, *v[3:], np.inf]], np.float32)
+    pb, tb, _, _ = sc.trace(ray, brute=True)
+    assert pb[0] == 700890 and p[0] == pb[0] and t[0] == tb[0]
+    # a sample of grazing camera rays toward the far terrain
+    rng = np.random.default_rng(5)
+    n = 400
+    dirs = np.stack([rng.uniform(-0.45, 0.45, n), np.full(n, 0.75), rng.uniform(-0.62, -0.5, n)], 1)
+    dirs = (dirs / np.linalg.norm(dirs, axis=1, keepdims=True)).astype(np.float32)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = v[:3]
+    rays[:, 4:7] = dirs
+    rays[:, 7] = np.inf
+    pb, tb, _, _ = sc.trace(rays, brute=True)
+    pv, tv, _, _ = sc.trace(rays)
+    assert np.mean(pb >= 0) > 0.5
+    assert np.array_equal(pb, pv) and np.array_equal(tb[pb >= 0], tv[pb >= 0])
+
+
 def test_barycentrics_reconstruct_hit_point():
     d = S.direct_room(32, 24, subdiv=2)
     sc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)

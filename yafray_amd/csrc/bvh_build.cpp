@@ -176,13 +176,16 @@ struct Builder {
     }
 };
 
-// Boxes are widened outward by a relative 2^-18 so float plane distances of
-// triangles on a box face are never culled by the slab test.
-static void inflate(const BBox& b, float lo[3], float hi[3])
+// Boxes are widened outward by 2^-18 relative to their own coordinates and to
+// the scene's largest coordinate G.  The float edge test of a triangle accepts
+// rays that pass up to a few ulps of the operands' magnitude outside it; the
+// G term keeps such rays inside the box of a triangle whose own coordinates
+// are small (a terrain cell near z = 0 has almost no relative slack in z).
+static void inflate(const BBox& b, float G, float lo[3], float hi[3])
 {
     for (int k = 0; k < 3; k++) {
         float ext = b.hi[k] - b.lo[k];
-        float e = (fabsf(b.lo[k]) + fabsf(b.hi[k]) + ext) * 3.814697265625e-06f + 1e-30f;
+        float e = (fabsf(b.lo[k]) + fabsf(b.hi[k]) + ext + G) * 3.814697265625e-06f + 1e-30f;
         lo[k] = b.lo[k] - e;
         hi[k] = b.hi[k] + e;
     }
@@ -259,6 +262,11 @@ void build_bvh(const float* verts, const uint32_t* idx, int64_t nt, const int32_
         out.tmat[i] = tri_mat ? tri_mat[p] : 0;
     }
 
+    // scene magnitude for the box inflation
+    float G = 0.0f;
+    if (nt > 0)
+        for (int k = 0; k < 3; k++) G = std::max(G, std::max(fabsf(bn[0].box.lo[k]), fabsf(bn[0].box.hi[k])));
+
     // inner nodes in depth-first order, both child boxes per node
     auto enc_leaf = [&](const BNode& n) -> int32_t {
         return ~(int32_t)(n.start * YG_LEAF_MAX + (n.count - 1));
@@ -270,7 +278,7 @@ void build_bvh(const float* verts, const uint32_t* idx, int64_t nt, const int32_
         if (c.count == 0 && c.left < 0) {
             lo[0] = lo[1] = lo[2] = 1.0f;
             hi[0] = hi[1] = hi[2] = -1.0f;
-        } else inflate(c.box, lo, hi);
+        } else inflate(c.box, G, lo, hi);
         if (which == 0) {
             g.a.x = lo[0]; g.a.y = hi[0]; g.a.z = lo[1]; g.a.w = hi[1];
             g.c.x = lo[2]; g.c.y = hi[2];

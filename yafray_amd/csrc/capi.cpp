@@ -1,5 +1,6 @@
 // C ABI of libyafgpu: errors, scene lifecycle, geometry ingestion, BVH build,
 // shading descriptors, QMC tables.
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <mutex>
@@ -40,6 +41,8 @@ extern "C" int yg_scene_create(yg_scene** out)
     if (!out) return yg_fail(YG_ERR_ARG, "null out");
     *out = new (std::nothrow) yg_scene();
     if (!*out) return yg_fail(YG_ERR_NOMEM, "scene allocation");
+    static std::atomic<uint64_t> next_uid{1};
+    (*out)->uid = next_uid++;
     return YG_OK;
 }
 
@@ -47,12 +50,13 @@ static void free_dev(yg_scene* s)
 {
     void* ptrs[] = {s->d_nodes, s->d_tris, s->d_tprim, s->d_tobj, s->d_tmat, s->d_obj_flags,
                     s->d_mats, s->d_mods, s->d_texs, s->pm.d_photons, s->pm.d_gstart,
-                    s->pm.d_gitems};
+                    s->pm.d_gitems, s->d_halton};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     s->d_nodes = nullptr; s->d_tris = nullptr; s->d_tprim = nullptr; s->d_tobj = nullptr;
     s->d_tmat = nullptr; s->d_obj_flags = nullptr; s->d_mats = nullptr; s->d_mods = nullptr;
     s->d_texs = nullptr; s->pm = PhotonMapDev();
+    s->d_halton = nullptr; s->halton_ndim = 0; s->halton_stride = 0;
 }
 
 extern "C" int yg_scene_destroy(yg_scene* s)

@@ -476,7 +476,6 @@ struct RenderWS {
     DevBuf<Area> areas;
     DevBuf<DevLight> lights;
     DevBuf<f3> apts, ajh, ajv;
-    DevBuf<float> halton;
     DevBuf<int32_t> item_ti;
     DevBuf<float> item_t;
     DevBuf<uint32_t> draws, lcg_off, qual, qoff;
@@ -490,13 +489,14 @@ struct RenderWS {
     DevBuf<uint8_t> nseg;
     DevBuf<Counters> cnt;
     DevBuf<uint8_t> scan_tmp;
+    DevBuf<uint8_t> akey0, akey1;
     bool pow_ready = false;
     // shape of the last launch (the adjoint replays it)
     uint32_t nitems = 0, npaths = 0;
     int S = 0, D = 0, na = 0, nl_max = 0;
     bool any_path = false;
     // the single-scene render the workspace holds (for YG_RENDER_REUSE)
-    const yg_scene* last_scene = nullptr;
+    uint64_t last_uid = 0;  // yg_scene::uid, 0 = none
     uint64_t last_gen = 0;
     yg_render_opts last_opts{};
 };
@@ -528,7 +528,6 @@ struct ScenePlan {
     std::vector<Area> areas;
     std::vector<DevLight> lights;
     std::vector<f3> apts, ajh, ajv;
-    std::vector<float> halton;
     uint32_t nitems = 0;
     int max_area_items = 0;
 };
@@ -540,11 +539,14 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
     if (!s->has_cam) return yg_fail(YG_ERR_STATE, "no camera (yg_set_camera)");
     if (s->mats.empty()) return yg_fail(YG_ERR_STATE, "no materials (yg_set_shading)");
     if (!o || !d_rgb) return yg_fail(YG_ERR_ARG, "null argument");
-    for (size_t t = 0; t < s->tri_mat.size(); t++)
-        if (s->tri_mat[t] < 0 || (size_t)s->tri_mat[t] >= s->mats.size())
-            return yg_fail(YG_ERR_ARG, "triangle material id out of range");
-    for (int32_t f : s->obj_flags)
-        if (!(f & 1)) return yg_fail(YG_ERR_UNSUPPORTED, "objects that cast no shadows are not implemented");
+    if (s->checked_gen != s->gen) {
+        for (size_t t = 0; t < s->tri_mat.size(); t++)
+            if (s->tri_mat[t] < 0 || (size_t)s->tri_mat[t] >= s->mats.size())
+                return yg_fail(YG_ERR_ARG, "triangle material id out of range");
+        for (int32_t f : s->obj_flags)
+            if (!(f & 1)) return yg_fail(YG_ERR_UNSUPPORTED, "objects that cast no shadows are not implemented");
+        s->checked_gen = s->gen;
+    }
     P.s = s;
     RenderScene& R = P.R;
     memset(&R, 0, sizeof(R));
@@ -618,14 +620,21 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
     if (R.path_light >= 0) {
         const int ndim = 2 * (R.depth + 1);
         R.hstride = P.max_area_items * R.samples + 1;
-        P.halton.resize((size_t)ndim * R.hstride);
-        int base = 2;
-        std::vector<float> tab;
-        for (int d = 0; d < ndim; d++) {
-            halton_incremental(base, (uint64_t)R.hstride, tab);
-            memcpy(&P.halton[(size_t)d * R.hstride], tab.data(), sizeof(float) * R.hstride);
-            base = next_prime(base);
+        if (!s->d_halton || s->halton_ndim != ndim || s->halton_stride != R.hstride) {
+            std::vector<float> all((size_t)ndim * R.hstride), tab;
+            int base = 2;
+            for (int d = 0; d < ndim; d++) {
+                halton_incremental(base, (uint64_t)R.hstride, tab);
+                memcpy(&all[(size_t)d * R.hstride], tab.data(), sizeof(float) * R.hstride);
+                base = next_prime(base);
+            }
+            s->halton_ndim = 0;
+            int rc = dev_upload(&s->d_halton, all.data(), all.size());
+            if (rc) return rc;
+            s->halton_ndim = ndim;
+            s->halton_stride = R.hstride;
         }
+        R.halton = s->d_halton;
     }
     if (R.photon_light >= 0) {
         R.pm.photons = s->pm.d_photons;
@@ -650,7 +659,7 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
                         bool timing, bool ktiming)
 {
     int rc;
-    g_ws.last_scene = nullptr;  // the workspace is about to hold this launch
+    g_ws.last_uid = 0;  // the workspace is about to hold this launch
     if ((rc = perm_upload())) return rc;
     if (!g_ws.pow_ready) {
         uint64_t pw[64];
@@ -663,7 +672,6 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     std::vector<Area> areas;
     std::vector<DevLight> lights;
     std::vector<f3> apts, ajh, ajv;
-    std::vector<float> halton;
     uint32_t nitems = 0;
     int S = 0, D = 0, nl_max = 1;
     bool any_path = false, any_photon = false, any_area = false;
@@ -680,8 +688,6 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         apts.insert(apts.end(), P.apts.begin(), P.apts.end());
         ajh.insert(ajh.end(), P.ajh.begin(), P.ajh.end());
         ajv.insert(ajv.end(), P.ajv.begin(), P.ajv.end());
-        R.halton = (const float*)(uintptr_t)halton.size();
-        halton.insert(halton.end(), P.halton.begin(), P.halton.end());
         for (auto a : P.areas) {
             a.scene = (int)scenes.size();
             a.item_start += nitems;
@@ -706,13 +712,11 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     if ((rc = W.apts.ensure(apts.size()))) return rc;
     if ((rc = W.ajh.ensure(ajh.size()))) return rc;
     if ((rc = W.ajv.ensure(ajv.size()))) return rc;
-    if ((rc = W.halton.ensure(halton.size()))) return rc;
     for (auto& R : scenes) {
         R.lights = W.lights.p + (uintptr_t)R.lights;
         R.apts = W.apts.p;
         R.ajh = W.ajh.p;
         R.ajv = W.ajv.p;
-        R.halton = W.halton.p + (uintptr_t)R.halton;
     }
     const uint64_t npaths64 = (uint64_t)nitems * S;
     if (npaths64 >= 0xffffffffull) return yg_fail(YG_ERR_UNSUPPORTED, "too many path samples in one launch");
@@ -750,8 +754,6 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         YG_HIP(hipMemcpyAsync(W.ajh.p, ajh.data(), sizeof(f3) * ajh.size(), hipMemcpyHostToDevice, st));
         YG_HIP(hipMemcpyAsync(W.ajv.p, ajv.data(), sizeof(f3) * ajv.size(), hipMemcpyHostToDevice, st));
     }
-    if (!halton.empty())
-        YG_HIP(hipMemcpyAsync(W.halton.p, halton.data(), sizeof(float) * halton.size(), hipMemcpyHostToDevice, st));
     YG_HIP(hipMemsetAsync(W.cnt.p, 0, sizeof(Counters), st));
 
     W.nitems = nitems;
@@ -873,11 +875,11 @@ extern "C" int yg_render(yg_scene* s, const yg_render_opts* opts, float* d_rgb, 
     std::vector<ScenePlan> plans(1);
     int rc = plan_scene(s, opts, d_rgb, plans[0]);
     if (rc) return rc;
-    g_ws.last_scene = nullptr;
+    g_ws.last_uid = 0;
     rc = render_plans(plans, (hipStream_t)stream, stats, opts && (opts->flags & YG_RENDER_TIMING),
                       opts && (opts->flags & YG_RENDER_KTIMING));
     if (rc == YG_OK) {
-        g_ws.last_scene = s;
+        g_ws.last_uid = s->uid;
         g_ws.last_gen = s->gen;
         g_ws.last_opts = *opts;
     }
@@ -1169,6 +1171,14 @@ __global__ __launch_bounds__(RB) void k_adjoint_path(const RenderScene* __restri
     adj_lds_flush(L, G, lds_ok);
 }
 
+// sort key of a replayed path: the number of segments the forward traced
+__global__ void k_adj_keys(const uint32_t* __restrict__ active, uint32_t n, const uint8_t* __restrict__ nseg,
+                           uint8_t* __restrict__ key)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) key[i] = nseg[active[i]];
+}
+
 // Forward render (image into d_rgb) followed by the replay.  Gradient
 // buffers are overwritten: d_grad_mat[nmat*3] (albedo), d_grad_mod[nmod*2]
 // (modulator colour, modulator size scale lambda), d_grad_light[nl] (power).
@@ -1195,12 +1205,12 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
     if (plans[0].R.depth > ADJ_MAXD) return yg_fail(YG_ERR_UNSUPPORTED, "adjoint supports depth <= 8");
     const bool timing = opts->flags & YG_RENDER_TIMING;
     RenderWS& W = g_ws;
-    const bool reuse = (opts->flags & YG_RENDER_REUSE) && W.last_scene == s && W.last_gen == s->gen &&
+    const bool reuse = (opts->flags & YG_RENDER_REUSE) && W.last_uid == s->uid && W.last_gen == s->gen &&
                        same_opts(W.last_opts, *opts);
     if (!reuse) {
-        W.last_scene = nullptr;
+        W.last_uid = 0;
         if ((rc = render_plans(plans, st, stats, timing, false))) return rc;
-        W.last_scene = s;
+        W.last_uid = s->uid;
         W.last_gen = s->gen;
         W.last_opts = *opts;
     }
@@ -1233,8 +1243,21 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
                            W.qoff.p, W.active0.p);
         YG_HIP(hipGetLastError());
         if (n0 > 0) {
+            // Replay order: longest paths first (stable, so pixel order is kept
+            // within a length), so that a wavefront's lanes run the same number
+            // of bounces instead of idling behind the longest path.
+            if ((rc = W.akey0.ensure(n0)) || (rc = W.akey1.ensure(n0))) return rc;
+            hipLaunchKernelGGL(k_adj_keys, dim3(grid_for(n0, 256)), dim3(256), 0, st, W.active0.p, n0, W.nseg.p,
+                               W.akey0.p);
+            YG_HIP(hipGetLastError());
+            size_t bytes = 0;
+            YG_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, bytes, W.akey0.p, W.akey1.p, W.active0.p,
+                                                                W.active1.p, (int)n0, 0, 4, st));
+            if ((rc = W.scan_tmp.ensure(bytes))) return rc;
+            YG_HIP(hipcub::DeviceRadixSort::SortPairsDescending(W.scan_tmp.p, bytes, W.akey0.p, W.akey1.p,
+                                                                W.active0.p, W.active1.p, (int)n0, 0, 4, st));
             hipLaunchKernelGGL(k_adjoint_path, dim3(grid_for(n0, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, W.na,
-                               W.active0.p, n0, W.npaths, W.S, W.D, W.item_ti.p, W.item_t.p, W.qoff.p, W.hidx.p,
+                               W.active1.p, n0, W.npaths, W.S, W.D, W.item_ti.p, W.item_t.p, W.qoff.p, W.hidx.p,
                                d_grad_rgb, G);
             YG_HIP(hipGetLastError());
         }

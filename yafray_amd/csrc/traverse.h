@@ -85,8 +85,20 @@ __device__ __forceinline__ RayBox make_raybox(f3 o, f3 d)
     return r;
 }
 
+// Relative slack of the slab interval: along grazing rays the float plane
+// distance of an accepted triangle can fall a little outside its box's
+// [near, far] interval, or just beyond the current closest distance.
+#define YG_TEPS 6.103515625e-05f  // 2^-14
+// Relative slack of the distance bound a box must start before (the current
+// closest hit, or a shadow ray's length): near a shared vertex a grazing ray
+// can pass the float edge test of a triangle at a plane distance in front of
+// where it enters that triangle's box.
+#define YG_HEPS 9.765625e-04f  // 2^-10
+__device__ __forceinline__ float cull_bound(float t) { return __builtin_fmaf(fabsf(t), YG_HEPS, t); }
+
 // Slab test of both children of a node.  Boxes are conservatively inflated at
-// build time, so fused multiply-adds here cannot drop a hit.
+// build time and the interval is widened by YG_TEPS, so fused multiply-adds
+// here cannot drop a hit.
 __device__ __forceinline__ void node_test(const GNode& n, const RayBox& r, float lo, float hi,
                                           float& t0, float& t1, bool& h0, bool& h1)
 {
@@ -100,8 +112,8 @@ __device__ __forceinline__ void node_test(const GNode& n, const RayBox& r, float
     float f0 = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), hi));
     float n1 = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), lo));
     float f1 = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), hi));
-    h0 = n0 <= f0;
-    h1 = n1 <= f1;
+    h0 = __builtin_fmaf(-fabsf(n0), YG_TEPS, n0) <= __builtin_fmaf(fabsf(f0), YG_TEPS, f0);
+    h1 = __builtin_fmaf(-fabsf(n1), YG_TEPS, n1) <= __builtin_fmaf(fabsf(f1), YG_TEPS, f1);
     t0 = n0;
     t1 = n1;
 }
@@ -121,6 +133,7 @@ __device__ int trace_closest(const BVHView& bvh, f3 o, f3 d, float tmin, float& 
     const float lo = tmin - 1e-4f * fabsf(tmin) - 1e-30f;
     int best = -1;
     int bestprim = 0x7fffffff;
+    float tcull = cull_bound(tbest);
     int node = 0;
     st.sp = 0;
     for (;;) {
@@ -129,7 +142,7 @@ __device__ int trace_closest(const BVHView& bvh, f3 o, f3 d, float tmin, float& 
             if (COUNT) cs->nodes++;
             float t0, t1;
             bool h0, h1;
-            node_test(n, rb, lo, tbest, t0, t1, h0, h1);
+            node_test(n, rb, lo, tcull, t0, t1, h0, h1);
             if (h0 && h1) {
                 int nearc = (t0 <= t1) ? n.d.x : n.d.y;
                 int farc = (t0 <= t1) ? n.d.y : n.d.x;
@@ -152,6 +165,7 @@ __device__ int trace_closest(const BVHView& bvh, f3 o, f3 d, float tmin, float& 
                 if (!(t >= tmin)) continue;
                 if (t < tbest) {
                     tbest = t;
+                    tcull = cull_bound(t);
                     best = ti;
                     bestprim = bvh.tprim[ti];
                 } else if (t == tbest && best >= 0) {
@@ -175,6 +189,7 @@ __device__ bool trace_any(const BVHView& bvh, f3 o, f3 d, float tmin, float tmax
                           f3 oself, float blo, float bhi, LaneStack& st, TraceStats* cs = nullptr)
 {
     const RayBox rb = make_raybox(o, d);
+    const float hi = cull_bound(bhi);
     int node = 0;
     st.sp = 0;
     for (;;) {
@@ -183,7 +198,7 @@ __device__ bool trace_any(const BVHView& bvh, f3 o, f3 d, float tmin, float tmax
             if (COUNT) cs->nodes++;
             float t0, t1;
             bool h0, h1;
-            node_test(n, rb, blo, bhi, t0, t1, h0, h1);
+            node_test(n, rb, blo, hi, t0, t1, h0, h1);
             if (h0 && h1) {
                 int nearc = (t0 <= t1) ? n.d.x : n.d.y;
                 int farc = (t0 <= t1) ? n.d.y : n.d.x;

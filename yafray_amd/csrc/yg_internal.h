@@ -95,6 +95,14 @@ struct yg_scene {
 
     PhotonMapDev pm;
     uint64_t gen = 0;  // bumped by every change of geometry, shading, lights or camera
+    uint64_t checked_gen = ~0ull;  // gen at which the render inputs were last validated
+    uint64_t uid = 0;              // process-unique scene number (never reused)
+
+    // per-area Halton tables of the last render (dimension-major, hstride
+    // entries per dimension), kept on the device while ndim/hstride repeat
+    float* d_halton = nullptr;
+    int halton_ndim = 0;
+    int halton_stride = 0;
 
     BVHView view() const
     {

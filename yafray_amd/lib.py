@@ -139,12 +139,19 @@ class Scene:
     def from_desc(cls, desc, build=True, **kw):
         """Ingest a SceneDesc object by object (object order = id order)."""
         s = cls()
+        order = np.argsort(desc.tri_obj, kind="stable")
+        bounds = np.searchsorted(desc.tri_obj[order], np.arange(len(desc.obj_flags) + 1))
+        contiguous = bool(np.all(order == np.arange(len(order))))
         for o in range(len(desc.obj_flags)):
-            sel = np.nonzero(desc.tri_obj == o)[0]
+            sel = order[bounds[o]:bounds[o + 1]]
             tri = desc.idx[sel]
-            used, inv = np.unique(tri.reshape(-1), return_inverse=True)
-            s.add_mesh(desc.verts[used], inv.reshape(-1, 3).astype(np.uint32), desc.tri_mat[sel],
-                       int(desc.obj_flags[o]))
+            if contiguous and len(tri):
+                lo, hi = int(tri.min()), int(tri.max()) + 1
+                v, f = desc.verts[lo:hi], (tri - lo).astype(np.uint32)
+            else:
+                used, inv = np.unique(tri.reshape(-1), return_inverse=True)
+                v, f = desc.verts[used], inv.reshape(-1, 3).astype(np.uint32)
+            s.add_mesh(v, f, desc.tri_mat[sel], int(desc.obj_flags[o]))
         if build:
             s.build(**kw)
         s.set_desc(desc)
