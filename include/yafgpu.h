@@ -33,7 +33,7 @@ extern "C" {
 #define YG_ERR_UNSUPPORTED (-4)
 #define YG_ERR_NOMEM (-5)
 
-#define YG_ABI_VERSION 1
+#define YG_ABI_VERSION 2
 
 /* ---------------------------------------------------------------- rays -- */
 
@@ -185,8 +185,10 @@ typedef struct yg_render_stats {
     float ms;                 /* whole render, device time */
     float ms_trace;           /* up to the final pixel pass */
     float ms_bounce;          /* sum of the GI bounce launches (YG_RENDER_KTIMING) */
-    uint32_t bounce_launches;
+    uint32_t bounce_launches; /* GI levels traced */
     uint64_t rays_bounce;     /* extension + shadow rays traced inside the bounce launches */
+    float ms_ext;             /* sum of the extension-ray (closest hit) launches */
+    float ms_shadow;          /* sum of the bounce shadow-ray (any hit) launches */
 } yg_render_stats;
 
 typedef struct yg_build_opts {

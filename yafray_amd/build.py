@@ -29,29 +29,32 @@ FLAGS = [
 ]
 
 
-def _compile(src):
-    out = os.path.join(OBJ, os.path.splitext(os.path.basename(src))[0] + ".o")
-    cmd = [HIPCC] + FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", out]
+def _compile(src, obj_dir=OBJ, defines=()):
+    out = os.path.join(obj_dir, os.path.splitext(os.path.basename(src))[0] + ".o")
+    cmd = [HIPCC] + FLAGS + ["-D" + d for d in defines] + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     return out
 
 
-def build(verbose=True):
-    os.makedirs(OBJ, exist_ok=True)
+def build(verbose=True, defines=(), lib=LIB, obj_dir=OBJ):
+    """Compile every source and link `lib`.  `defines` (NAME=VALUE strings)
+    override the tuning constants (stack depths, block size) for a variant
+    build; the default build uses none."""
+    os.makedirs(obj_dir, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda s: _compile(s, obj_dir, tuple(defines)), srcs))
+    tmp = lib + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", lib)
+    return lib
 
 
 if __name__ == "__main__":

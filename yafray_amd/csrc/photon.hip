@@ -78,7 +78,8 @@ __global__ void k_emit_pick(uint32_t natt, uint32_t np, const uint32_t* __restri
 }
 
 // One photon segment at depth d (shoot_photon_diffuse, photonlight.cc:203-244)
-__global__ __launch_bounds__(128) void k_photon_seg(PhotonSrc src, BVHView bvh, const yg_material* __restrict__ mats,
+#define PB 64
+__global__ __launch_bounds__(PB) void k_photon_seg(PhotonSrc src, BVHView bvh, const yg_material* __restrict__ mats,
                                                     const yg_modulator* __restrict__ mods,
                                                     const yg_texture* __restrict__ texs,
                                                     const int32_t* __restrict__ obj_flags, float min_raydis,
@@ -89,11 +90,11 @@ __global__ __launch_bounds__(128) void k_photon_seg(PhotonSrc src, BVHView bvh, 
                                                     uint32_t* __restrict__ mvalid, uint32_t* __restrict__ bflag,
                                                     unsigned long long* __restrict__ nrays)
 {
-    __shared__ int s_stk[YG_STACK_LDS * 128];
-    const uint32_t idx = blockIdx.x * 128 + threadIdx.x;
+    extern __shared__ int s_stk[];
+    const uint32_t idx = blockIdx.x * PB + threadIdx.x;
     if (idx >= nactive) return;
     LaneStack st;
-    st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = 128; st.sp = 0;
+    st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = PB; st.sp = 0;
     const uint32_t p = active[idx];
     f3 pos, lastpos, dir;
     c3 col;
@@ -397,7 +398,7 @@ extern "C" int yg_photon_build(yg_scene* s, uint32_t li, const yg_render_opts* o
         YG_HIP(hipMemsetAsync(bflag, 0, sizeof(uint32_t) * np, st));
         const float* h0 = (d == 1) ? hdim[0] : hdim[2 * (d - 1)];
         const float* h1 = (d == 1) ? hdim[1] : hdim[2 * (d - 1) + 1];
-        hipLaunchKernelGGL(k_photon_seg, dim3(grid_for(nactive, 128)), dim3(128), 0, st, src, s->view(), s->d_mats,
+        hipLaunchKernelGGL(k_photon_seg, dim3(grid_for(nactive, PB)), dim3(PB), stack_lds_bytes(s->max_depth, PB), st, src, s->view(), s->d_mats,
                            s->d_mods, s->d_texs, s->d_obj_flags, opts->min_raydis, d, act, nactive, att, h0, h1,
                            bidx, pst, slots, mvalid, bflag, nrays);
         YG_HIP(hipGetLastError());

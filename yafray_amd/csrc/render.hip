@@ -17,7 +17,9 @@
 #include "qmc.h"
 #include "render_dev.h"
 
-#define RB 128
+#ifndef RB
+#define RB 64
+#endif
 
 __constant__ uint64_t YG_LCG_POW[64];
 
@@ -67,7 +69,7 @@ __global__ __launch_bounds__(RB) void k_primary(const RenderScene* __restrict__ 
                                                 uint32_t nitems, int32_t* __restrict__ item_ti,
                                                 float* __restrict__ item_t)
 {
-    __shared__ int s_stk[YG_STACK_LDS * RB];
+    extern __shared__ int s_stk[];
     const uint32_t item = blockIdx.x * RB + threadIdx.x;
     if (item >= nitems) return;
     LaneStack st;
@@ -136,7 +138,7 @@ __global__ __launch_bounds__(RB) void k_direct(const RenderScene* __restrict__ s
                                                c3* __restrict__ lightc, int nl_max,
                                                Counters* __restrict__ cnt)
 {
-    __shared__ int s_stk[YG_STACK_LDS * RB];
+    extern __shared__ int s_stk[];
     const uint32_t item = blockIdx.x * RB + threadIdx.x;
     uint32_t nshadow = 0, ndraws = 0;
     if (item < nitems) {
@@ -173,28 +175,51 @@ __global__ __launch_bounds__(RB) void k_direct(const RenderScene* __restrict__ s
 }
 
 // One bounce of the pathlight's full-GI loop (pathlight.cc:305-347) for the
-// active path segments.  Path p = item*S + sample.
-__global__ __launch_bounds__(RB) void k_bounce(const RenderScene* __restrict__ scenes,
-                                               const Area* __restrict__ areas, int nareas, int j,
-                                               const uint32_t* __restrict__ active, uint32_t nactive,
-                                               uint32_t nactive_cap,
-                                               int S, int D, const int32_t* __restrict__ item_ti,
-                                               const float* __restrict__ item_t,
-                                               const uint32_t* __restrict__ qoff,
-                                               const uint32_t* __restrict__ hidx,
-                                               f3* __restrict__ st_where, f3* __restrict__ st_hn,
-                                               int32_t* __restrict__ st_ti, c3* __restrict__ st_col,
-                                               float* __restrict__ st_s, c3* __restrict__ contri,
-                                               uint8_t* __restrict__ nseg, uint32_t* __restrict__ flag,
-                                               Counters* __restrict__ cnt)
+// active path segments, in three launches so that each traversal kernel
+// carries only the registers it needs:
+//   k_ext     hemisphere direction from the Halton tables, closest hit;
+//             queues the paths whose ray hit something
+//   k_shadow  the sun shadow rays of the queued hit points (isShadowed)
+//   k_shade   scene_t::light at the hit from the visibility bits, the
+//             path's contribution and its next-vertex state
+// Path p = item*S + sample.  Every value passed between them is a float the
+// fused evaluation would hold in a register, so results are bit-identical.
+
+// Per-level path queue of hit paths (order irrelevant: paths are independent).
+__device__ __forceinline__ void queue_push(bool want, uint32_t p, uint32_t* qcount, uint32_t* queue)
 {
-    __shared__ int s_stk[YG_STACK_LDS * RB];
+    const uint64_t m = __ballot(want);
+    if (!m) return;
+    const int lane = threadIdx.x & 63;
+    uint32_t base = 0;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    if (lane == leader) base = atomicAdd(qcount, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (want) queue[base + __popcll(m & ((1ull << lane) - 1))] = p;
+}
+
+#ifndef YG_BOUNCE_WPE
+#define YG_BOUNCE_WPE 1
+#endif
+__global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WPE))) void k_ext(
+    const RenderScene* __restrict__ scenes, const Area* __restrict__ areas, int nareas, int j,
+    const uint32_t* __restrict__ active, uint32_t nactive, uint32_t nactive_cap, int S, int D,
+    const int32_t* __restrict__ item_ti, const float* __restrict__ item_t, const uint32_t* __restrict__ qoff,
+    const uint32_t* __restrict__ hidx, const f3* __restrict__ st_where, const f3* __restrict__ st_hn,
+    const int32_t* __restrict__ st_ti, const c3* __restrict__ st_col, float* __restrict__ st_s,
+    c3* __restrict__ contri, uint8_t* __restrict__ nseg, f3* __restrict__ h_p, f3* __restrict__ h_ray,
+    int32_t* __restrict__ h_ti, uint32_t* __restrict__ qcount, uint32_t* __restrict__ queue,
+    Counters* __restrict__ cnt)
+{
+    extern __shared__ int s_stk[];
     const uint32_t idx = blockIdx.x * RB + threadIdx.x;
-    uint32_t next = 0, nshadow = 0;
+    uint32_t next = 0;
+    bool hit = false;
+    uint32_t p = 0;
     if (idx < nactive) {
         LaneStack st;
         st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = RB; st.sp = 0;
-        const uint32_t p = active[idx];
+        p = active[idx];
         const uint32_t item = p / (uint32_t)S;
         const int smp = (int)(p % (uint32_t)S);
         const Area a = areas[find_area(areas, nareas, item)];
@@ -233,26 +258,100 @@ __global__ __launch_bounds__(RB) void k_bounce(const RenderScene* __restrict__ s
         } else {
             next = 1;
             float t;
+            const f3 f = where + ray * R.min_raydis;
             const int ti = first_hit(R.bvh, where, ray, R.min_raydis, t, st);
             if (ti < 0) {
                 contri[(size_t)p * D + j] = cmul(cscale(raycolor, s), R.bg);
                 nseg[p] = (uint8_t)(j + 1);
             } else {
-                const Surf sp = make_surf(R.bvh, ti, where + ray * R.min_raydis, ray, t);
-                const c3 light = indirect_light(R, sp, ray, st, nshadow);
-                contri[(size_t)p * D + j] = cmul(cscale(raycolor, s), light);
-                st_hn[p] = face_forward(sp.Ng, sp.N, -ray);
-                st_col[p] = cmul(raycolor, get_diffuse(R.mats, R.mods, R.texs, sp.mat, sp.P));
-                st_where[p] = sp.P;
-                st_ti[p] = ti;
-                flag[p] = 1;
-                nseg[p] = (uint8_t)(j + 1);
+                h_p[p] = f + ray * t;  // make_surf's P
+                h_ray[p] = ray;
+                h_ti[p] = ti;
+                hit = true;
             }
         }
     }
+    queue_push(hit, p, qcount, queue);
     block_add(&cnt->ext, next);
+}
+
+__global__ __launch_bounds__(RB) void k_shadow(const RenderScene* __restrict__ scenes,
+                                               const Area* __restrict__ areas, int nareas, int S,
+                                               const uint32_t* __restrict__ qcount,
+                                               const uint32_t* __restrict__ queue,
+                                               const f3* __restrict__ h_p, const int32_t* __restrict__ h_ti,
+                                               uint32_t* __restrict__ vis, Counters* __restrict__ cnt)
+{
+    extern __shared__ int s_stk[];
+    const uint32_t idx = blockIdx.x * RB + threadIdx.x;
+    uint32_t nshadow = 0;
+    if (idx < *qcount) {
+        LaneStack st;
+        st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = RB; st.sp = 0;
+        const uint32_t p = queue[idx];
+        const Area& a = areas[find_area(areas, nareas, p / (uint32_t)S)];
+        const RenderScene& R = scenes[a.scene];
+        const f3 P = h_p[p];
+        const int obj = R.bvh.tobj[h_ti[p]];
+        uint32_t mask = 0;
+        for (int li = 0; li < R.nl && li < 32; li++) {
+            const DevLight& L = R.lights[li];
+            if (!L.use_in_indirect || L.type != YG_LIGHT_SUN || !L.cast_shadows) continue;
+            nshadow++;
+            if (is_shadowed(R.bvh, P, L.dir, obj, false, R.min_raydis, R.self_bias, st)) mask |= 1u << li;
+        }
+        vis[p] = mask;
+    }
     block_add(&cnt->shadow, nshadow);
     block_add(&cnt->bshadow, nshadow);
+}
+
+__global__ __launch_bounds__(RB) void k_shade(const RenderScene* __restrict__ scenes,
+                                              const Area* __restrict__ areas, int nareas, int j, int S, int D,
+                                              const uint32_t* __restrict__ qcount,
+                                              const uint32_t* __restrict__ queue,
+                                              const f3* __restrict__ h_p, const f3* __restrict__ h_ray,
+                                              const int32_t* __restrict__ h_ti, const uint32_t* __restrict__ vis,
+                                              f3* __restrict__ st_where, f3* __restrict__ st_hn,
+                                              int32_t* __restrict__ st_ti, c3* __restrict__ st_col,
+                                              const float* __restrict__ st_s, c3* __restrict__ contri,
+                                              uint8_t* __restrict__ nseg, uint32_t* __restrict__ flag)
+{
+    const uint32_t idx = blockIdx.x * RB + threadIdx.x;
+    if (idx >= *qcount) return;
+    const uint32_t p = queue[idx];
+    const Area& a = areas[find_area(areas, nareas, p / (uint32_t)S)];
+    const RenderScene& R = scenes[a.scene];
+    const int ti = h_ti[p];
+    const f3 ray = h_ray[p];
+    Surf sp;
+    sp.ti = ti;
+    sp.P = h_p[p];
+    const float4 A = R.bvh.tris[3 * ti], B = R.bvh.tris[3 * ti + 1], C = R.bvh.tris[3 * ti + 2];
+    sp.Ng = mk3(A.w, B.w, C.w);
+    sp.N = sp.Ng;
+    create_cs(sp.N, sp.NU, sp.NV);
+    sp.obj = R.bvh.tobj[ti];
+    sp.mat = R.bvh.tmat[ti];
+    // scene_t::light with indirect = true (sun lights only inside GI paths)
+    const uint32_t mask = vis[p];
+    const f3 eye = (sp.P + (-ray)) - sp.P;
+    c3 fl = mkc(0, 0, 0);
+    for (int li = 0; li < R.nl; li++) {
+        const DevLight& L = R.lights[li];
+        if (!L.use_in_indirect || L.type != YG_LIGHT_SUN) continue;
+        if (L.cast_shadows && (mask >> li & 1u)) fl = cadd(fl, mkc(0, 0, 0));
+        else fl = cadd(fl, from_light(R, sp, L.dir, L.col, eye));
+    }
+    const c3 raycolor = (j == 0) ? mkc(1, 1, 1) : st_col[p];
+    const float s = st_s[p];
+    contri[(size_t)p * D + j] = cmul(cscale(raycolor, s), fl);
+    st_hn[p] = face_forward(sp.Ng, sp.N, -ray);
+    st_col[p] = cmul(raycolor, get_diffuse(R.mats, R.mods, R.texs, sp.mat, sp.P));
+    st_where[p] = sp.P;
+    st_ti[p] = ti;
+    flag[p] = 1;
+    nseg[p] = (uint8_t)(j + 1);
 }
 
 // Scatter flagged paths into the next active list and give each its Halton
@@ -487,6 +586,10 @@ struct RenderWS {
     DevBuf<float> st_s;
     DevBuf<c3> contri;
     DevBuf<uint8_t> nseg;
+    DevBuf<f3> h_p, h_ray;        // hit point and direction of a level's extension ray
+    DevBuf<int32_t> h_ti;
+    DevBuf<uint32_t> vis, queue;  // shadow visibility bits; queue of hit paths
+    DevBuf<uint32_t> qcnt;        // queue length per level
     DevBuf<Counters> cnt;
     DevBuf<uint8_t> scan_tmp;
     DevBuf<uint8_t> akey0, akey1;
@@ -495,6 +598,7 @@ struct RenderWS {
     uint32_t nitems = 0, npaths = 0;
     int S = 0, D = 0, na = 0, nl_max = 0;
     bool any_path = false;
+    size_t stk_bytes = 0;  // dynamic LDS of the traversal launches
     // the single-scene render the workspace holds (for YG_RENDER_REUSE)
     uint64_t last_uid = 0;  // yg_scene::uid, 0 = none
     uint64_t last_gen = 0;
@@ -608,6 +712,7 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
                 return yg_fail(YG_ERR_UNSUPPORTED, "area lights inside pathlight GI paths are not implemented");
         }
         if (R.depth > 255) return yg_fail(YG_ERR_UNSUPPORTED, "pathlight depth");
+        if (R.nl > 32) return yg_fail(YG_ERR_UNSUPPORTED, "more than 32 lights with a pathlight");
     }
     block_regions(R.resx, R.resy, o->block > 0 ? o->block : 64, o->rand_seed, P.areas);
     uint32_t acc = 0;
@@ -744,6 +849,12 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         if ((rc = W.st_s.ensure(npaths))) return rc;
         if ((rc = W.contri.ensure((size_t)npaths * D))) return rc;
         if ((rc = W.nseg.ensure(npaths))) return rc;
+        if ((rc = W.h_p.ensure(npaths))) return rc;
+        if ((rc = W.h_ray.ensure(npaths))) return rc;
+        if ((rc = W.h_ti.ensure(npaths))) return rc;
+        if ((rc = W.vis.ensure(npaths))) return rc;
+        if ((rc = W.queue.ensure(npaths))) return rc;
+        if ((rc = W.qcnt.ensure(D))) return rc;
     }
     YG_HIP(hipMemcpyAsync(W.scenes.p, scenes.data(), sizeof(RenderScene) * scenes.size(), hipMemcpyHostToDevice, st));
     YG_HIP(hipMemcpyAsync(W.areas.p, areas.data(), sizeof(Area) * areas.size(), hipMemcpyHostToDevice, st));
@@ -756,6 +867,9 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     }
     YG_HIP(hipMemsetAsync(W.cnt.p, 0, sizeof(Counters), st));
 
+    int max_depth = 1;
+    for (auto& P : plans) max_depth = std::max(max_depth, P.s->max_depth);
+    W.stk_bytes = stack_lds_bytes(max_depth, RB);
     W.nitems = nitems;
     W.npaths = npaths;
     W.S = S;
@@ -773,14 +887,14 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         YG_HIP(hipEventRecord(e0, st));
     }
     const int na = (int)areas.size();
-    hipLaunchKernelGGL(k_primary, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
+    hipLaunchKernelGGL(k_primary, dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na,
                        nitems, W.item_ti.p, W.item_t.p);
     YG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_prep, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
                        nitems, W.item_ti.p, W.item_t.p, W.draws.p, W.qual.p);
     YG_HIP(hipGetLastError());
     if (any_area && (rc = exclusive_scan(W.draws.p, W.lcg_off.p, nitems, st))) return rc;
-    hipLaunchKernelGGL(k_direct, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
+    hipLaunchKernelGGL(k_direct, dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na,
                        nitems, W.item_ti.p, W.item_t.p, W.lcg_off.p, W.lightc.p, nl_max, W.cnt.p);
     YG_HIP(hipGetLastError());
     if (any_photon) {
@@ -800,22 +914,34 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         YG_HIP(hipGetLastError());
         uint32_t* act = W.active0.p;
         uint32_t* nxt = W.active1.p;
+        YG_HIP(hipMemsetAsync(W.qcnt.p, 0, sizeof(uint32_t) * D, st));
         for (int j = 0; j < D && nactive > 0; j++) {
             YG_HIP(hipMemsetAsync(W.flag.p, 0, sizeof(uint32_t) * npaths, st));
+            hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
             if (ktiming) {
-                hipEvent_t a, b;
-                YG_HIP(hipEventCreate(&a));
-                YG_HIP(hipEventCreate(&b));
-                kev.push_back(a);
-                kev.push_back(b);
-                YG_HIP(hipEventRecord(a, st));
+                for (auto& e : ev) {
+                    YG_HIP(hipEventCreate(&e));
+                    kev.push_back(e);
+                }
+                YG_HIP(hipEventRecord(ev[0], st));
             }
-            hipLaunchKernelGGL(k_bounce, dim3(grid_for(nactive, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p,
-                               na, j, act, nactive, npaths, S, D, W.item_ti.p, W.item_t.p, W.qoff.p, W.hidx.p,
-                               W.st_where.p, W.st_hn.p, W.st_ti.p, W.st_col.p, W.st_s.p, W.contri.p,
-                               W.nseg.p, W.flag.p, W.cnt.p);
+            const unsigned grid = grid_for(nactive, RB);
+            uint32_t* qc = W.qcnt.p + j;
+            hipLaunchKernelGGL(k_ext, dim3(grid), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na, j, act,
+                               nactive, npaths, S, D, W.item_ti.p, W.item_t.p, W.qoff.p, W.hidx.p, W.st_where.p,
+                               W.st_hn.p, W.st_ti.p, W.st_col.p, W.st_s.p, W.contri.p, W.nseg.p, W.h_p.p,
+                               W.h_ray.p, W.h_ti.p, qc, W.queue.p, W.cnt.p);
             YG_HIP(hipGetLastError());
-            if (ktiming) YG_HIP(hipEventRecord(kev.back(), st));
+            if (ktiming) YG_HIP(hipEventRecord(ev[1], st));
+            hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na, S, qc,
+                               W.queue.p, W.h_p.p, W.h_ti.p, W.vis.p, W.cnt.p);
+            YG_HIP(hipGetLastError());
+            if (ktiming) YG_HIP(hipEventRecord(ev[2], st));
+            hipLaunchKernelGGL(k_shade, dim3(grid), dim3(RB), 0, st, W.scenes.p, W.areas.p, na, j, S, D, qc,
+                               W.queue.p, W.h_p.p, W.h_ray.p, W.h_ti.p, W.vis.p, W.st_where.p, W.st_hn.p,
+                               W.st_ti.p, W.st_col.p, W.st_s.p, W.contri.p, W.nseg.p, W.flag.p);
+            YG_HIP(hipGetLastError());
+            if (ktiming) YG_HIP(hipEventRecord(ev[3], st));
             if (j + 1 >= D) break;
             if ((rc = exclusive_scan(W.flag.p, W.ex.p, npaths, st))) return rc;
             hipLaunchKernelGGL(k_compact, dim3(grid_for(npaths, 256)), dim3(256), 0, st, W.areas.p, na, npaths,
@@ -853,10 +979,14 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
             stats->ms = ms;
             stats->ms_trace = ms2;
         }
-        for (size_t q = 0; q + 1 < kev.size(); q += 2) {
-            float ms = 0;
-            YG_HIP(hipEventElapsedTime(&ms, kev[q], kev[q + 1]));
-            stats->ms_bounce += ms;
+        for (size_t q = 0; q + 3 < kev.size(); q += 4) {
+            float m_ext = 0, m_sh = 0, m_all = 0;
+            YG_HIP(hipEventElapsedTime(&m_ext, kev[q], kev[q + 1]));
+            YG_HIP(hipEventElapsedTime(&m_sh, kev[q + 1], kev[q + 2]));
+            YG_HIP(hipEventElapsedTime(&m_all, kev[q], kev[q + 3]));
+            stats->ms_bounce += m_all;
+            stats->ms_ext += m_ext;
+            stats->ms_shadow += m_sh;
             stats->bounce_launches++;
         }
     }
@@ -1009,7 +1139,7 @@ __global__ __launch_bounds__(RB) void k_adjoint_direct(const RenderScene* __rest
                                                        const float* __restrict__ item_t,
                                                        const float* __restrict__ grad_rgb, AdjAcc G)
 {
-    __shared__ int s_stk[YG_STACK_LDS * RB];
+    extern __shared__ int s_stk[];
     __shared__ AdjLds L;
     const bool lds_ok = G.nmat <= ADJ_LDS_MAT && G.nmod <= ADJ_LDS_MOD && G.nl <= ADJ_LDS_LIGHT;
     adj_lds_init(L);
@@ -1055,7 +1185,7 @@ __global__ __launch_bounds__(RB) void k_adjoint_path(const RenderScene* __restri
                                                      const uint32_t* __restrict__ hidx,
                                                      const float* __restrict__ grad_rgb, AdjAcc G)
 {
-    __shared__ int s_stk[YG_STACK_LDS * RB];
+    extern __shared__ int s_stk[];
     __shared__ AdjLds L;
     const bool lds_ok = G.nmat <= ADJ_LDS_MAT && G.nmod <= ADJ_LDS_MOD && G.nl <= ADJ_LDS_LIGHT;
     adj_lds_init(L);
@@ -1230,7 +1360,7 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
         YG_HIP(hipEventCreate(&e1));
         YG_HIP(hipEventRecord(e0, st));
     }
-    hipLaunchKernelGGL(k_adjoint_direct, dim3(grid_for(W.nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, W.na,
+    hipLaunchKernelGGL(k_adjoint_direct, dim3(grid_for(W.nitems, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, W.na,
                        W.nitems, W.item_ti.p, W.item_t.p, d_grad_rgb, G);
     YG_HIP(hipGetLastError());
     if (W.any_path) {
@@ -1256,7 +1386,7 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
             if ((rc = W.scan_tmp.ensure(bytes))) return rc;
             YG_HIP(hipcub::DeviceRadixSort::SortPairsDescending(W.scan_tmp.p, bytes, W.akey0.p, W.akey1.p,
                                                                 W.active0.p, W.active1.p, (int)n0, 0, 4, st));
-            hipLaunchKernelGGL(k_adjoint_path, dim3(grid_for(n0, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, W.na,
+            hipLaunchKernelGGL(k_adjoint_path, dim3(grid_for(n0, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, W.na,
                                W.active1.p, n0, W.npaths, W.S, W.D, W.item_ti.p, W.item_t.p, W.qoff.p, W.hidx.p,
                                d_grad_rgb, G);
             YG_HIP(hipGetLastError());

@@ -4,7 +4,7 @@
 #include "noise_dev.h"
 #include "traverse.h"
 
-#define TB 128
+#define TB 64
 
 // --------------------------------------------------------------- kernels --
 
@@ -13,7 +13,7 @@ __global__ __launch_bounds__(TB) void k_trace_closest(BVHView bvh, const yg_ray*
                                                        uint32_t n, yg_hit* __restrict__ hits,
                                                        unsigned long long* counters)
 {
-    __shared__ int s_stk[YG_STACK_LDS * TB];
+    extern __shared__ int s_stk[];
     const uint32_t i = blockIdx.x * TB + threadIdx.x;
     if (i >= n) return;
     LaneStack st;
@@ -44,7 +44,7 @@ __global__ __launch_bounds__(TB) void k_trace_closest(BVHView bvh, const yg_ray*
 __global__ __launch_bounds__(TB) void k_trace_any(BVHView bvh, const yg_ray* __restrict__ rays,
                                                   uint32_t n, uint8_t* __restrict__ occ)
 {
-    __shared__ int s_stk[YG_STACK_LDS * TB];
+    extern __shared__ int s_stk[];
     const uint32_t i = blockIdx.x * TB + threadIdx.x;
     if (i >= n) return;
     LaneStack st;
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(TB) void k_first_hit(BVHView bvh, const float* __re
                                                   const float* __restrict__ dir, uint32_t n,
                                                   float min_raydis, yg_hit* __restrict__ hits)
 {
-    __shared__ int s_stk[YG_STACK_LDS * TB];
+    extern __shared__ int s_stk[];
     const uint32_t i = blockIdx.x * TB + threadIdx.x;
     if (i >= n) return;
     LaneStack st;
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(TB) void k_is_shadowed(BVHView bvh, const float* __
                                                     uint32_t n, int to_point, float min_raydis,
                                                     float self_bias, uint8_t* __restrict__ out)
 {
-    __shared__ int s_stk[YG_STACK_LDS * TB];
+    extern __shared__ int s_stk[];
     const uint32_t i = blockIdx.x * TB + threadIdx.x;
     if (i >= n) return;
     LaneStack st;
@@ -137,7 +137,7 @@ extern "C" int yg_trace_closest(yg_scene* s, const yg_ray* d_rays, uint32_t n, y
     if (rc) return rc;
     if (n == 0) return YG_OK;
     if (!d_rays || !d_hits) return yg_fail(YG_ERR_ARG, "null buffer");
-    hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid_for(n, TB)), dim3(TB), 0,
+    hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid_for(n, TB)), dim3(TB), stack_lds_bytes(s->max_depth, TB),
                        (hipStream_t)stream, s->view(), d_rays, n, d_hits, nullptr);
     YG_HIP(hipGetLastError());
     return YG_OK;
@@ -151,7 +151,7 @@ extern "C" int yg_trace_closest_counted(yg_scene* s, const yg_ray* d_rays, uint3
     int rc = check_scene(s);
     if (rc) return rc;
     if (n == 0) return YG_OK;
-    hipLaunchKernelGGL(k_trace_closest<true>, dim3(grid_for(n, TB)), dim3(TB), 0,
+    hipLaunchKernelGGL(k_trace_closest<true>, dim3(grid_for(n, TB)), dim3(TB), stack_lds_bytes(s->max_depth, TB),
                        (hipStream_t)stream, s->view(), d_rays, n, d_hits, d_counters);
     YG_HIP(hipGetLastError());
     return YG_OK;
@@ -164,7 +164,7 @@ extern "C" int yg_trace_any(yg_scene* s, const yg_ray* d_rays, uint32_t n, uint8
     if (rc) return rc;
     if (n == 0) return YG_OK;
     if (!d_rays || !d_occluded) return yg_fail(YG_ERR_ARG, "null buffer");
-    hipLaunchKernelGGL(k_trace_any, dim3(grid_for(n, TB)), dim3(TB), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(k_trace_any, dim3(grid_for(n, TB)), dim3(TB), stack_lds_bytes(s->max_depth, TB), (hipStream_t)stream,
                        s->view(), d_rays, n, d_occluded);
     YG_HIP(hipGetLastError());
     return YG_OK;
@@ -177,7 +177,7 @@ extern "C" int yg_first_hit(yg_scene* s, const float* d_from, const float* d_dir
     if (rc) return rc;
     if (n == 0) return YG_OK;
     if (!d_from || !d_dir || !d_hits) return yg_fail(YG_ERR_ARG, "null buffer");
-    hipLaunchKernelGGL(k_first_hit, dim3(grid_for(n, TB)), dim3(TB), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(k_first_hit, dim3(grid_for(n, TB)), dim3(TB), stack_lds_bytes(s->max_depth, TB), (hipStream_t)stream,
                        s->view(), d_from, d_dir, n, min_raydis, d_hits);
     YG_HIP(hipGetLastError());
     return YG_OK;
@@ -191,7 +191,7 @@ extern "C" int yg_is_shadowed(yg_scene* s, const float* d_p, const float* d_l,
     if (rc) return rc;
     if (n == 0) return YG_OK;
     if (!d_p || !d_l || !d_out) return yg_fail(YG_ERR_ARG, "null buffer");
-    hipLaunchKernelGGL(k_is_shadowed, dim3(grid_for(n, TB)), dim3(TB), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(k_is_shadowed, dim3(grid_for(n, TB)), dim3(TB), stack_lds_bytes(s->max_depth, TB), (hipStream_t)stream,
                        s->view(), d_p, d_l, d_self_obj, n, to_point, min_raydis, self_bias, d_out);
     YG_HIP(hipGetLastError());
     return YG_OK;

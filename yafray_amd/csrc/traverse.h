@@ -48,24 +48,14 @@ __device__ __forceinline__ void load_tri(const float4* tris, int ti, f3& a, f3& 
     n = mk3(A.w, B.w, C.w);
 }
 
-// Per-lane traversal stack: the top YG_STACK_LDS entries live in LDS
-// (entry k of lane t at lds[k*nlanes + t], conflict free), deeper ones spill
-// to a private array that only very deep trees touch.
+// Per-lane traversal stack in LDS: entry k of lane t at lds[k*nlanes + t]
+// (conflict free).  Launches size the dynamic LDS for the deepest tree they
+// traverse, so no entry ever goes to private (scratch) memory.
 struct LaneStack {
     int* lds;
     int lane, nlanes, sp;
-    int spill[YG_STACK_SPILL];
-    __device__ void push(int v)
-    {
-        if (sp < YG_STACK_LDS) lds[sp * nlanes + lane] = v;
-        else spill[sp - YG_STACK_LDS] = v;
-        sp++;
-    }
-    __device__ int pop()
-    {
-        sp--;
-        return (sp < YG_STACK_LDS) ? lds[sp * nlanes + lane] : spill[sp - YG_STACK_LDS];
-    }
+    __device__ void push(int v) { lds[sp * nlanes + lane] = v; sp++; }
+    __device__ int pop() { sp--; return lds[sp * nlanes + lane]; }
 };
 
 struct RayBox {

@@ -22,9 +22,17 @@ struct alignas(64) GNode {
 };
 
 #define YG_LEAF_MAX 16
-#define YG_STACK_LDS 24
-#define YG_STACK_SPILL 72
-#define YG_MAX_DEPTH (YG_STACK_LDS + YG_STACK_SPILL - 2)
+// Deepest tree the traversal accepts.  Each lane's stack lives in LDS and
+// holds at most one entry per tree level, so a launch reserves
+// (max_depth + 1) entries per lane (see stack_lds_bytes).
+#define YG_MAX_DEPTH 200
+
+// Dynamic LDS bytes of a traversal launch with `lanes` threads per block over
+// trees at most `max_depth` deep.
+static inline size_t stack_lds_bytes(int max_depth, int lanes)
+{
+    return (size_t)(max_depth + 1) * (size_t)lanes * sizeof(int);
+}
 
 // Leaf-ordered triangle record, 48 B: (a.xyz, n.x) (b.xyz, n.y) (c.xyz, n.z),
 // n = the unit face normal of triangle_t::recNormal (triangle.cc:45-49).
