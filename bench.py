@@ -172,6 +172,20 @@ def bench_adjoint(args):
         return float(L)
 
     both_dt, loss = _timed(step, args.steps, args.warmup)
+    # device time of the adjoint replay alone (REUSE of the forward just made)
+    from yafray_amd import descriptors as D
+    o = D.yg_render_opts.from_buffer_copy(sc.desc.opts)
+    sc.render(out, o)
+    o.flags |= 1 | 4
+    gt = torch.ones(out.numel(), dtype=torch.float32, device="cuda")
+    gm = torch.zeros(len(sc.desc.mats) * 3, dtype=torch.float64, device="cuda")
+    go = torch.zeros(max(len(sc.desc.mods), 1) * 2, dtype=torch.float64, device="cuda")
+    gl = torch.zeros(len(sc.desc.lights), dtype=torch.float64, device="cuda")
+    adj_ms = []
+    for _ in range(3):
+        st = sc.render_adjoint(gt, out, gm, go, gl, opts=o)
+        adj_ms.append(st.ms_trace)
+    fst = sc.render(out, timing=True)
     return {
         "metric": "fwd+adjoint ms/iter", "value": round(both_dt * 1e3, 3), "unit": "ms/iter",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(both_dt * 1e3, 3),
@@ -182,6 +196,8 @@ def bench_adjoint(args):
                    "params": 256 * 3 + 3},
         "forward_ms": round(fwd_dt * 1e3, 3),
         "ratio_fwd_adj_over_fwd": round(both_dt / fwd_dt, 3),
+        "adjoint_replay_device_ms": round(min(adj_ms), 3),
+        "forward_device_ms": round(fst.ms, 3),
         "loss": loss,
     }
 

@@ -99,3 +99,29 @@ def test_autograd_function(setup):
     assert abs(float(power.grad) - res["glight"][1]) <= 1e-4 * abs(res["glight"][1])
     assert abs(float(amp.grad) - res["gmod"][0]) <= 1e-4 * abs(res["gmod"][0])
     assert abs(float(size.grad) - res["gmod"][1] / 3.0) <= 1e-4 * abs(res["gmod"][1] / 3.0)
+
+
+def test_adjoint_zero_albedo_channel(setup):
+    """A vertex albedo with a zero channel: the replay cannot divide the
+    path's suffix radiance by it, so the per-path product-rule replay runs
+    instead; the gradient still matches the FP64 reference."""
+    Y, torch, sc, d, alb, img, g, target, res, loss = setup
+    a = alb.copy()
+    a[:, 1] = 0.0
+    d0 = S.with_params(d, albedo=a)
+    sc.set_desc(d0)
+    gt = torch.from_numpy(g.reshape(-1)).cuda()
+    out = torch.zeros_like(gt)
+    gmat = torch.zeros(len(d.mats) * 3, dtype=torch.float64, device="cuda")
+    gmod = torch.zeros(2, dtype=torch.float64, device="cuda")
+    glight = torch.zeros(len(d.lights), dtype=torch.float64, device="cuda")
+    sc.render_adjoint(gt, out, gmat, gmod, glight)
+    torch.cuda.synchronize()
+    sc.set_desc(d)
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    _, gm, gmo, gl = oc.render_adjoint(d0, g)
+    for x, y in ((gmat.cpu().numpy().reshape(-1, 3), gm), (gmod.cpu().numpy(), gmo.reshape(-1)),
+                 (glight.cpu().numpy(), gl)):
+        err = np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-30)
+        assert err <= 1e-3, err
+    assert np.any(gm[:, 1] != 0)

@@ -87,6 +87,7 @@ extern "C" int yg_mesh_add(yg_scene* s, const float* xyz, uint32_t nv, const uin
     s->obj_flags.push_back(flags);
     s->built = false;
     s->gen++;
+    s->geo_gen++;
     if (obj_id) *obj_id = oid;
     return YG_OK;
 }
@@ -174,16 +175,30 @@ extern "C" int yg_set_shading(yg_scene* s, const yg_material* mats, uint32_t nma
         if (texs[k].type < YG_TEX_CLOUDS || texs[k].type > YG_TEX_MUSGRAVE)
             return yg_fail(YG_ERR_ARG, "bad texture type");
     }
+    // same table sizes: overwrite the device copies in place (parameter
+    // updates of a differentiable render do this every step)
+    const bool same = s->d_mats && s->d_mods && s->d_texs && s->mats.size() == nmat &&
+                      s->mods.size() == nmod && s->texs.size() == ntex;
     s->mats.assign(mats, mats + nmat);
     s->mods.assign(mods, mods + nmod);
     s->texs.assign(texs, texs + ntex);
     s->gen++;
-    int rc;
-    if ((rc = dev_upload(&s->d_mats, mats, nmat))) return rc;
     yg_modulator dm{};
     yg_texture dt{};
-    if ((rc = dev_upload(&s->d_mods, nmod ? mods : &dm, nmod ? nmod : 1))) return rc;
-    if ((rc = dev_upload(&s->d_texs, ntex ? texs : &dt, ntex ? ntex : 1))) return rc;
+    const yg_modulator* mp = nmod ? mods : &dm;
+    const yg_texture* tp = ntex ? texs : &dt;
+    if (same) {
+        // renders still in flight on any stream may read the old tables
+        YG_HIP(hipDeviceSynchronize());
+        YG_HIP(hipMemcpy(s->d_mats, mats, sizeof(yg_material) * nmat, hipMemcpyHostToDevice));
+        YG_HIP(hipMemcpy(s->d_mods, mp, sizeof(yg_modulator) * (nmod ? nmod : 1), hipMemcpyHostToDevice));
+        YG_HIP(hipMemcpy(s->d_texs, tp, sizeof(yg_texture) * (ntex ? ntex : 1), hipMemcpyHostToDevice));
+        return YG_OK;
+    }
+    int rc;
+    if ((rc = dev_upload(&s->d_mats, mats, nmat))) return rc;
+    if ((rc = dev_upload(&s->d_mods, mp, nmod ? nmod : 1))) return rc;
+    if ((rc = dev_upload(&s->d_texs, tp, ntex ? ntex : 1))) return rc;
     return YG_OK;
 }
 

@@ -174,6 +174,72 @@ __global__ __launch_bounds__(RB) void k_direct(const RenderScene* __restrict__ s
     block_add(&cnt->draws, ndraws);
 }
 
+// ------------------------------------------------ adjoint accumulators --
+// Gradient buffers (double) and their block-private LDS copies; used by the
+// GI levels' shading kernel in replay mode and by the per-pixel adjoint.
+
+#define ADJ_MAXD 8
+#define ADJ_LDS_MAT 256
+#define ADJ_LDS_MOD 16
+#define ADJ_LDS_LIGHT 8
+#define ADJ_GRID 1024u  // blocks of a gradient-accumulating launch (about 4 per CU)
+
+struct AdjAcc {
+    double* mat;   // nmat*3
+    double* mod;   // nmod*2 (d/dcolour, d/dlambda)
+    double* light; // nl
+    int nmat, nmod, nl;
+};
+
+// block-private accumulators in LDS, flushed once per block
+struct AdjLds {
+    double mat[ADJ_LDS_MAT * 3];
+    double mod[ADJ_LDS_MOD * 2];
+    double light[ADJ_LDS_LIGHT];
+};
+
+__device__ __forceinline__ void adj_add_vertex(AdjLds& L, const AdjAcc& G, bool lds_ok, int mat, const RhoD& r,
+                                               c3 gamma)
+{
+    const c3 ga = cmul(gamma, r.d_alb);
+    double* dst = lds_ok ? L.mat : G.mat;
+    atomicAdd(&dst[3 * mat], (double)ga.r);
+    atomicAdd(&dst[3 * mat + 1], (double)ga.g);
+    atomicAdd(&dst[3 * mat + 2], (double)ga.b);
+    if (r.mod >= 0) {
+        double* dm = lds_ok ? L.mod : G.mod;
+        const c3 a = cmul(gamma, r.d_amp), l = cmul(gamma, r.d_lam);
+        atomicAdd(&dm[2 * r.mod], (double)(a.r + a.g + a.b));
+        atomicAdd(&dm[2 * r.mod + 1], (double)(l.r + l.g + l.b));
+    }
+}
+
+__device__ __forceinline__ void adj_lds_init(AdjLds& L)
+{
+    for (int k = threadIdx.x; k < ADJ_LDS_MAT * 3; k += blockDim.x) L.mat[k] = 0.0;
+    for (int k = threadIdx.x; k < ADJ_LDS_MOD * 2; k += blockDim.x) L.mod[k] = 0.0;
+    for (int k = threadIdx.x; k < ADJ_LDS_LIGHT; k += blockDim.x) L.light[k] = 0.0;
+    __syncthreads();
+}
+
+__device__ __forceinline__ void adj_lds_flush(AdjLds& L, const AdjAcc& G, bool lds_ok)
+{
+    __syncthreads();
+    if (!lds_ok) return;
+    for (int k = threadIdx.x; k < G.nmat * 3; k += blockDim.x)
+        if (L.mat[k] != 0.0) atomicAdd(&G.mat[k], L.mat[k]);
+    for (int k = threadIdx.x; k < G.nmod * 2; k += blockDim.x)
+        if (L.mod[k] != 0.0) atomicAdd(&G.mod[k], L.mod[k]);
+    for (int k = threadIdx.x; k < G.nl; k += blockDim.x)
+        if (L.light[k] != 0.0) atomicAdd(&G.light[k], L.light[k]);
+}
+
+__device__ __forceinline__ bool item_output(const Area& a, uint32_t item, int& i, int& j)
+{
+    item_pixel(a, item, i, j);
+    return j >= a.rx && j < a.rx + a.rw && i >= a.ry && i < a.ry + a.rh;
+}
+
 // One bounce of the pathlight's full-GI loop (pathlight.cc:305-347) for the
 // active path segments, in three launches so that each traversal kernel
 // carries only the registers it needs:
@@ -201,6 +267,7 @@ __device__ __forceinline__ void queue_push(bool want, uint32_t p, uint32_t* qcou
 #ifndef YG_BOUNCE_WPE
 #define YG_BOUNCE_WPE 1
 #endif
+template <bool ADJ>  // ADJ: replay for the adjoint (contri/nseg of the forward are kept)
 __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WPE))) void k_ext(
     const RenderScene* __restrict__ scenes, const Area* __restrict__ areas, int nareas, int j,
     const uint32_t* __restrict__ active, uint32_t nactive, uint32_t nactive_cap, int S, int D,
@@ -240,13 +307,14 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WP
             where = st_where[p];
             N = st_hn[p];
             const int pti = st_ti[p];
-            const float4 A = R.bvh.tris[3 * pti], B = R.bvh.tris[3 * pti + 1], C = R.bvh.tris[3 * pti + 2];
+            const float4 A = gload(R.bvh.tris + 3 * pti), B = gload(R.bvh.tris + 3 * pti + 1),
+                         C = gload(R.bvh.tris + 3 * pti + 2);
             create_cs(mk3(A.w, B.w, C.w), Ru, Rv);
             raycolor = st_col[p];
             k = hidx[(size_t)j * nactive_cap + p];
         }
-        const float z1 = R.halton[(size_t)(2 * j) * R.hstride + k];
-        const float z2 = R.halton[(size_t)(2 * j + 1) * R.hstride + k];
+        const float z1 = *(const YG_GLOBAL float*)(R.halton + (size_t)(2 * j) * R.hstride + k);
+        const float z2 = *(const YG_GLOBAL float*)(R.halton + (size_t)(2 * j + 1) * R.hstride + k);
         const f3 ray = hemi_direction(z1, z2, N, Ru, Rv);
         float s;
         if (j == 0) {
@@ -254,15 +322,17 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WP
             st_s[p] = s;
         } else s = st_s[p];
         if (cenergy(raycolor) < 0.05f) {
-            nseg[p] = (uint8_t)j;
+            if (!ADJ) nseg[p] = (uint8_t)j;
         } else {
             next = 1;
             float t;
             const f3 f = where + ray * R.min_raydis;
             const int ti = first_hit(R.bvh, where, ray, R.min_raydis, t, st);
             if (ti < 0) {
-                contri[(size_t)p * D + j] = cmul(cscale(raycolor, s), R.bg);
-                nseg[p] = (uint8_t)(j + 1);
+                if (!ADJ) {
+                    contri[(size_t)p * D + j] = cmul(cscale(raycolor, s), R.bg);
+                    nseg[p] = (uint8_t)(j + 1);
+                }
             } else {
                 h_p[p] = f + ray * t;  // make_surf's P
                 h_ray[p] = ray;
@@ -272,9 +342,10 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WP
         }
     }
     queue_push(hit, p, qcount, queue);
-    block_add(&cnt->ext, next);
+    if (!ADJ) block_add(&cnt->ext, next);
 }
 
+template <bool ADJ>
 __global__ __launch_bounds__(RB) void k_shadow(const RenderScene* __restrict__ scenes,
                                                const Area* __restrict__ areas, int nareas, int S,
                                                const uint32_t* __restrict__ qcount,
@@ -292,7 +363,7 @@ __global__ __launch_bounds__(RB) void k_shadow(const RenderScene* __restrict__ s
         const Area& a = areas[find_area(areas, nareas, p / (uint32_t)S)];
         const RenderScene& R = scenes[a.scene];
         const f3 P = h_p[p];
-        const int obj = R.bvh.tobj[h_ti[p]];
+        const int obj = gload(R.bvh.tobj + h_ti[p]);
         uint32_t mask = 0;
         for (int li = 0; li < R.nl && li < 32; li++) {
             const DevLight& L = R.lights[li];
@@ -302,11 +373,26 @@ __global__ __launch_bounds__(RB) void k_shadow(const RenderScene* __restrict__ s
         }
         vis[p] = mask;
     }
-    block_add(&cnt->shadow, nshadow);
-    block_add(&cnt->bshadow, nshadow);
+    if (!ADJ) {
+        block_add(&cnt->shadow, nshadow);
+        block_add(&cnt->bshadow, nshadow);
+    }
 }
 
-__global__ __launch_bounds__(RB) void k_shade(const RenderScene* __restrict__ scenes,
+#define SB 256  // shading block size (no traversal stack)
+
+// Forward: the level's light sum, contribution and next-vertex state.
+// ADJ (replay): the same state update, plus the gradient terms of this path
+// vertex.  With W = dLoss/dpixel * rho0 * mult * power (per item) and the
+// estimator's pixel term rho0 * mult * power * sum_b contri_b,
+// contri_b = s * R_b * X_b (X_b = lf_b * rho_b at a hit, bg at a miss),
+//   d pixel / d rho_j = W * (s * R_j * lf_j + (sum_{b>j} contri_b) / rho_j)
+//   d pixel / d power = W * sum_j s * R_j * lfp_j * rho_j
+// where the suffix sum comes from the forward's contri of this path.  A
+// vertex whose albedo has a zero channel (the suffix cannot be divided) is
+// counted in *degenerate and left to the per-path replay.
+template <bool ADJ>
+__global__ __launch_bounds__(SB) void k_shade(const RenderScene* __restrict__ scenes,
                                               const Area* __restrict__ areas, int nareas, int j, int S, int D,
                                               const uint32_t* __restrict__ qcount,
                                               const uint32_t* __restrict__ queue,
@@ -315,43 +401,96 @@ __global__ __launch_bounds__(RB) void k_shade(const RenderScene* __restrict__ sc
                                               f3* __restrict__ st_where, f3* __restrict__ st_hn,
                                               int32_t* __restrict__ st_ti, c3* __restrict__ st_col,
                                               const float* __restrict__ st_s, c3* __restrict__ contri,
-                                              uint8_t* __restrict__ nseg, uint32_t* __restrict__ flag)
+                                              uint8_t* __restrict__ nseg, uint32_t* __restrict__ flag,
+                                              AdjAcc G, const c3* __restrict__ w_item,
+                                              unsigned long long* __restrict__ degenerate)
 {
-    const uint32_t idx = blockIdx.x * RB + threadIdx.x;
-    if (idx >= *qcount) return;
-    const uint32_t p = queue[idx];
-    const Area& a = areas[find_area(areas, nareas, p / (uint32_t)S)];
-    const RenderScene& R = scenes[a.scene];
-    const int ti = h_ti[p];
-    const f3 ray = h_ray[p];
-    Surf sp;
-    sp.ti = ti;
-    sp.P = h_p[p];
-    const float4 A = R.bvh.tris[3 * ti], B = R.bvh.tris[3 * ti + 1], C = R.bvh.tris[3 * ti + 2];
-    sp.Ng = mk3(A.w, B.w, C.w);
-    sp.N = sp.Ng;
-    create_cs(sp.N, sp.NU, sp.NV);
-    sp.obj = R.bvh.tobj[ti];
-    sp.mat = R.bvh.tmat[ti];
-    // scene_t::light with indirect = true (sun lights only inside GI paths)
-    const uint32_t mask = vis[p];
-    const f3 eye = (sp.P + (-ray)) - sp.P;
-    c3 fl = mkc(0, 0, 0);
-    for (int li = 0; li < R.nl; li++) {
-        const DevLight& L = R.lights[li];
-        if (!L.use_in_indirect || L.type != YG_LIGHT_SUN) continue;
-        if (L.cast_shadows && (mask >> li & 1u)) fl = cadd(fl, mkc(0, 0, 0));
-        else fl = cadd(fl, from_light(R, sp, L.dir, L.col, eye));
+    extern __shared__ double s_adj[];
+    AdjLds& Ls = *reinterpret_cast<AdjLds*>(s_adj);
+    const bool lds_ok = ADJ && G.nmat <= ADJ_LDS_MAT && G.nmod <= ADJ_LDS_MOD && G.nl <= ADJ_LDS_LIGHT;
+    if (ADJ) adj_lds_init(Ls);
+    uint32_t ndeg = 0;
+    const uint32_t nq = *qcount;
+    // grid-stride: the replay launches a bounded grid so that each block's
+    // LDS accumulators gather many vertices before their one flush
+    for (uint32_t idx = blockIdx.x * SB + threadIdx.x; idx < nq; idx += gridDim.x * SB) {
+        const uint32_t p = queue[idx];
+        const Area& a = areas[find_area(areas, nareas, p / (uint32_t)S)];
+        const RenderScene& R = scenes[a.scene];
+        const int ti = h_ti[p];
+        const f3 ray = h_ray[p];
+        Surf sp;
+        sp.ti = ti;
+        sp.P = h_p[p];
+        const float4 A = R.bvh.tris[3 * ti], B = R.bvh.tris[3 * ti + 1], C = R.bvh.tris[3 * ti + 2];
+        sp.Ng = mk3(A.w, B.w, C.w);
+        sp.N = sp.Ng;
+        create_cs(sp.N, sp.NU, sp.NV);
+        sp.obj = R.bvh.tobj[ti];
+        sp.mat = R.bvh.tmat[ti];
+        // scene_t::light with indirect = true (sun lights only inside GI paths)
+        const uint32_t mask = vis[p];
+        const f3 eye = (sp.P + (-ray)) - sp.P;
+        const c3 raycolor = (j == 0) ? mkc(1, 1, 1) : st_col[p];
+        const float s = st_s[p];
+        c3 rho;
+        if (!ADJ) {
+            c3 fl = mkc(0, 0, 0);
+            for (int li = 0; li < R.nl; li++) {
+                const DevLight& L = R.lights[li];
+                if (!L.use_in_indirect || L.type != YG_LIGHT_SUN) continue;
+                if (L.cast_shadows && (mask >> li & 1u)) fl = cadd(fl, mkc(0, 0, 0));
+                else fl = cadd(fl, from_light(R, sp, L.dir, L.col, eye));
+            }
+            contri[(size_t)p * D + j] = cmul(cscale(raycolor, s), fl);
+            rho = get_diffuse(R.mats, R.mods, R.texs, sp.mat, sp.P);
+            nseg[p] = (uint8_t)(j + 1);
+        } else {
+            // diffuse_d's rho is get_diffuse's value (one modulator at most)
+            const RhoD r = diffuse_d(R.mats, R.mods, R.texs, sp.mat, sp.P);
+            rho = r.rho;
+            const c3 W = w_item[p / (uint32_t)S];
+            if (W.r != 0 || W.g != 0 || W.b != 0) {
+                c3 lf = mkc(0, 0, 0), lfp = mkc(0, 0, 0);
+                int sun = -1;
+                for (int li = 0; li < R.nl; li++) {
+                    const DevLight& L = R.lights[li];
+                    if (!L.use_in_indirect || L.type != YG_LIGHT_SUN) continue;
+                    if (sun < 0) sun = li;
+                    if (L.cast_shadows && (mask >> li & 1u)) continue;
+                    const f3 N = face_forward(sp.Ng, sp.N, norm3(eye));
+                    const float inte = dot(N, L.dir);
+                    if (inte < 0) continue;
+                    lf = cadd(lf, cscale(L.col, inte));
+                    lfp = cadd(lfp, cscale(L.color, inte));
+                }
+                c3 La = mkc(0, 0, 0);
+                const int ns = nseg[p];
+                for (int b = j + 1; b < ns; b++) La = cadd(La, contri[(size_t)p * D + b]);
+                if ((W.r != 0 && rho.r == 0) || (W.g != 0 && rho.g == 0) || (W.b != 0 && rho.b == 0)) {
+                    ndeg = 1;
+                } else {
+                    const c3 sR = cscale(raycolor, s);
+                    const c3 div = mkc(rho.r != 0 ? La.r / rho.r : 0.0f, rho.g != 0 ? La.g / rho.g : 0.0f,
+                                       rho.b != 0 ? La.b / rho.b : 0.0f);
+                    adj_add_vertex(Ls, G, lds_ok, sp.mat, r, cmul(W, cadd(cmul(sR, lf), div)));
+                    if (sun >= 0) {
+                        const c3 q = cmul(W, cmul(cmul(sR, lfp), rho));
+                        atomicAdd(lds_ok ? &Ls.light[sun] : &G.light[sun], (double)(q.r + q.g + q.b));
+                    }
+                }
+            }
+        }
+        st_hn[p] = face_forward(sp.Ng, sp.N, -ray);
+        st_col[p] = cmul(raycolor, rho);
+        st_where[p] = sp.P;
+        st_ti[p] = ti;
+        flag[p] = 1;
     }
-    const c3 raycolor = (j == 0) ? mkc(1, 1, 1) : st_col[p];
-    const float s = st_s[p];
-    contri[(size_t)p * D + j] = cmul(cscale(raycolor, s), fl);
-    st_hn[p] = face_forward(sp.Ng, sp.N, -ray);
-    st_col[p] = cmul(raycolor, get_diffuse(R.mats, R.mods, R.texs, sp.mat, sp.P));
-    st_where[p] = sp.P;
-    st_ti[p] = ti;
-    flag[p] = 1;
-    nseg[p] = (uint8_t)(j + 1);
+    if (ADJ) {
+        adj_lds_flush(Ls, G, lds_ok);
+        block_add(degenerate, ndeg);
+    }
 }
 
 // Scatter flagged paths into the next active list and give each its Halton
@@ -593,6 +732,8 @@ struct RenderWS {
     DevBuf<Counters> cnt;
     DevBuf<uint8_t> scan_tmp;
     DevBuf<uint8_t> akey0, akey1;
+    DevBuf<c3> w_item;                   // adjoint: per-item path weight
+    DevBuf<unsigned long long> adeg;     // adjoint: vertices left to the per-path replay
     bool pow_ready = false;
     // shape of the last launch (the adjoint replays it)
     uint32_t nitems = 0, npaths = 0;
@@ -643,13 +784,14 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
     if (!s->has_cam) return yg_fail(YG_ERR_STATE, "no camera (yg_set_camera)");
     if (s->mats.empty()) return yg_fail(YG_ERR_STATE, "no materials (yg_set_shading)");
     if (!o || !d_rgb) return yg_fail(YG_ERR_ARG, "null argument");
-    if (s->checked_gen != s->gen) {
+    if (s->checked_geo != s->geo_gen || s->checked_nmat != s->mats.size()) {
         for (size_t t = 0; t < s->tri_mat.size(); t++)
             if (s->tri_mat[t] < 0 || (size_t)s->tri_mat[t] >= s->mats.size())
                 return yg_fail(YG_ERR_ARG, "triangle material id out of range");
         for (int32_t f : s->obj_flags)
             if (!(f & 1)) return yg_fail(YG_ERR_UNSUPPORTED, "objects that cast no shadows are not implemented");
-        s->checked_gen = s->gen;
+        s->checked_geo = s->geo_gen;
+        s->checked_nmat = s->mats.size();
     }
     P.s = s;
     RenderScene& R = P.R;
@@ -758,6 +900,69 @@ __global__ __launch_bounds__(64) void k_photon_direct(const RenderScene* __restr
                                                       uint32_t nitems, const int32_t* __restrict__ item_ti,
                                                       const float* __restrict__ item_t,
                                                       c3* __restrict__ lightc, int nl_max);
+
+// The GI levels of the launch the workspace holds: per level the extension
+// rays, the shadow rays of their hits, the shading, then the stable
+// compaction that numbers the next level's Halton calls per render area.
+// ADJ replays the forward (same Halton indices, same hits) and accumulates
+// gradients in the shading kernel; the forward's contri/nseg stay untouched.
+template <bool ADJ>
+static int trace_levels(hipStream_t st, uint32_t nactive, bool ktiming, std::vector<hipEvent_t>& kev,
+                        const AdjAcc& G, const c3* w_item, unsigned long long* degenerate)
+{
+    RenderWS& W = g_ws;
+    const int S = W.S, D = W.D, na = W.na;
+    const uint32_t npaths = W.npaths, nitems = W.nitems;
+    int rc;
+    hipLaunchKernelGGL(k_path_init, dim3(grid_for(nitems, 256)), dim3(256), 0, st, nitems, S, W.qual.p,
+                       W.qoff.p, W.active0.p);
+    YG_HIP(hipGetLastError());
+    uint32_t* act = W.active0.p;
+    uint32_t* nxt = W.active1.p;
+    YG_HIP(hipMemsetAsync(W.qcnt.p, 0, sizeof(uint32_t) * D, st));
+    for (int j = 0; j < D && nactive > 0; j++) {
+        YG_HIP(hipMemsetAsync(W.flag.p, 0, sizeof(uint32_t) * npaths, st));
+        hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+        if (ktiming) {
+            for (auto& e : ev) {
+                YG_HIP(hipEventCreate(&e));
+                kev.push_back(e);
+            }
+            YG_HIP(hipEventRecord(ev[0], st));
+        }
+        const unsigned grid = grid_for(nactive, RB);
+        uint32_t* qc = W.qcnt.p + j;
+        hipLaunchKernelGGL(k_ext<ADJ>, dim3(grid), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na, j, act,
+                           nactive, npaths, S, D, W.item_ti.p, W.item_t.p, W.qoff.p, W.hidx.p, W.st_where.p,
+                           W.st_hn.p, W.st_ti.p, W.st_col.p, W.st_s.p, W.contri.p, W.nseg.p, W.h_p.p,
+                           W.h_ray.p, W.h_ti.p, qc, W.queue.p, W.cnt.p);
+        YG_HIP(hipGetLastError());
+        if (ktiming) YG_HIP(hipEventRecord(ev[1], st));
+        hipLaunchKernelGGL(k_shadow<ADJ>, dim3(grid), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na, S, qc,
+                           W.queue.p, W.h_p.p, W.h_ti.p, W.vis.p, W.cnt.p);
+        YG_HIP(hipGetLastError());
+        if (ktiming) YG_HIP(hipEventRecord(ev[2], st));
+        const unsigned sgrid = ADJ ? std::min(grid_for(nactive, SB), ADJ_GRID) : grid_for(nactive, SB);
+        hipLaunchKernelGGL(k_shade<ADJ>, dim3(sgrid), dim3(SB), ADJ ? sizeof(AdjLds) : 0, st,
+                           W.scenes.p, W.areas.p, na, j, S, D, qc, W.queue.p, W.h_p.p, W.h_ray.p, W.h_ti.p,
+                           W.vis.p, W.st_where.p, W.st_hn.p, W.st_ti.p, W.st_col.p, W.st_s.p, W.contri.p,
+                           W.nseg.p, W.flag.p, G, w_item, degenerate);
+        YG_HIP(hipGetLastError());
+        if (ktiming) YG_HIP(hipEventRecord(ev[3], st));
+        if (j + 1 >= D) break;
+        if ((rc = exclusive_scan(W.flag.p, W.ex.p, npaths, st))) return rc;
+        hipLaunchKernelGGL(k_compact, dim3(grid_for(npaths, 256)), dim3(256), 0, st, W.areas.p, na, npaths, S,
+                           W.flag.p, W.ex.p, nxt, W.hidx.p + (size_t)(j + 1) * npaths);
+        YG_HIP(hipGetLastError());
+        uint32_t lf = 0, le = 0;
+        YG_HIP(hipMemcpyAsync(&lf, W.flag.p + npaths - 1, 4, hipMemcpyDeviceToHost, st));
+        YG_HIP(hipMemcpyAsync(&le, W.ex.p + npaths - 1, 4, hipMemcpyDeviceToHost, st));
+        YG_HIP(hipStreamSynchronize(st));
+        nactive = lf + le;
+        std::swap(act, nxt);
+    }
+    return YG_OK;
+}
 
 // Launch the whole pipeline over a set of planned scenes (one launch per stage).
 static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render_stats* stats,
@@ -908,52 +1113,8 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         YG_HIP(hipMemcpyAsync(&last_q, W.qual.p + nitems - 1, 4, hipMemcpyDeviceToHost, st));
         YG_HIP(hipMemcpyAsync(&last_off, W.qoff.p + nitems - 1, 4, hipMemcpyDeviceToHost, st));
         YG_HIP(hipStreamSynchronize(st));
-        uint32_t nactive = (last_q + last_off) * (uint32_t)S;
-        hipLaunchKernelGGL(k_path_init, dim3(grid_for(nitems, 256)), dim3(256), 0, st, nitems, S, W.qual.p,
-                           W.qoff.p, W.active0.p);
-        YG_HIP(hipGetLastError());
-        uint32_t* act = W.active0.p;
-        uint32_t* nxt = W.active1.p;
-        YG_HIP(hipMemsetAsync(W.qcnt.p, 0, sizeof(uint32_t) * D, st));
-        for (int j = 0; j < D && nactive > 0; j++) {
-            YG_HIP(hipMemsetAsync(W.flag.p, 0, sizeof(uint32_t) * npaths, st));
-            hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-            if (ktiming) {
-                for (auto& e : ev) {
-                    YG_HIP(hipEventCreate(&e));
-                    kev.push_back(e);
-                }
-                YG_HIP(hipEventRecord(ev[0], st));
-            }
-            const unsigned grid = grid_for(nactive, RB);
-            uint32_t* qc = W.qcnt.p + j;
-            hipLaunchKernelGGL(k_ext, dim3(grid), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na, j, act,
-                               nactive, npaths, S, D, W.item_ti.p, W.item_t.p, W.qoff.p, W.hidx.p, W.st_where.p,
-                               W.st_hn.p, W.st_ti.p, W.st_col.p, W.st_s.p, W.contri.p, W.nseg.p, W.h_p.p,
-                               W.h_ray.p, W.h_ti.p, qc, W.queue.p, W.cnt.p);
-            YG_HIP(hipGetLastError());
-            if (ktiming) YG_HIP(hipEventRecord(ev[1], st));
-            hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na, S, qc,
-                               W.queue.p, W.h_p.p, W.h_ti.p, W.vis.p, W.cnt.p);
-            YG_HIP(hipGetLastError());
-            if (ktiming) YG_HIP(hipEventRecord(ev[2], st));
-            hipLaunchKernelGGL(k_shade, dim3(grid), dim3(RB), 0, st, W.scenes.p, W.areas.p, na, j, S, D, qc,
-                               W.queue.p, W.h_p.p, W.h_ray.p, W.h_ti.p, W.vis.p, W.st_where.p, W.st_hn.p,
-                               W.st_ti.p, W.st_col.p, W.st_s.p, W.contri.p, W.nseg.p, W.flag.p);
-            YG_HIP(hipGetLastError());
-            if (ktiming) YG_HIP(hipEventRecord(ev[3], st));
-            if (j + 1 >= D) break;
-            if ((rc = exclusive_scan(W.flag.p, W.ex.p, npaths, st))) return rc;
-            hipLaunchKernelGGL(k_compact, dim3(grid_for(npaths, 256)), dim3(256), 0, st, W.areas.p, na, npaths,
-                               S, W.flag.p, W.ex.p, nxt, W.hidx.p + (size_t)(j + 1) * npaths);
-            YG_HIP(hipGetLastError());
-            uint32_t lf = 0, le = 0;
-            YG_HIP(hipMemcpyAsync(&lf, W.flag.p + npaths - 1, 4, hipMemcpyDeviceToHost, st));
-            YG_HIP(hipMemcpyAsync(&le, W.ex.p + npaths - 1, 4, hipMemcpyDeviceToHost, st));
-            YG_HIP(hipStreamSynchronize(st));
-            nactive = lf + le;
-            std::swap(act, nxt);
-        }
+        const uint32_t nactive = (last_q + last_off) * (uint32_t)S;
+        if ((rc = trace_levels<false>(st, nactive, ktiming, kev, AdjAcc{}, nullptr, nullptr))) return rc;
     }
     if (timing) YG_HIP(hipEventRecord(e1, st));
     hipLaunchKernelGGL(k_finalize, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
@@ -1071,105 +1232,68 @@ __global__ __launch_bounds__(64) void k_photon_direct(const RenderScene* __restr
 // light term linearly (sunlight.cc:35-57).  Supported scenes: one pathlight
 // plus sun lights, materials with at most one modulator.
 
-#define ADJ_MAXD 8
-#define ADJ_LDS_MAT 256
-#define ADJ_LDS_MOD 16
-#define ADJ_LDS_LIGHT 8
-
-struct AdjAcc {
-    double* mat;   // nmat*3
-    double* mod;   // nmod*2 (d/dcolour, d/dlambda)
-    double* light; // nl
-    int nmat, nmod, nl;
-};
-
-// block-private accumulators in LDS, flushed once per block
-struct AdjLds {
-    double mat[ADJ_LDS_MAT * 3];
-    double mod[ADJ_LDS_MOD * 2];
-    double light[ADJ_LDS_LIGHT];
-};
-
-__device__ __forceinline__ void adj_add_vertex(AdjLds& L, const AdjAcc& G, bool lds_ok, int mat, const RhoD& r,
-                                               c3 gamma)
-{
-    const c3 ga = cmul(gamma, r.d_alb);
-    double* dst = lds_ok ? L.mat : G.mat;
-    atomicAdd(&dst[3 * mat], (double)ga.r);
-    atomicAdd(&dst[3 * mat + 1], (double)ga.g);
-    atomicAdd(&dst[3 * mat + 2], (double)ga.b);
-    if (r.mod >= 0) {
-        double* dm = lds_ok ? L.mod : G.mod;
-        const c3 a = cmul(gamma, r.d_amp), l = cmul(gamma, r.d_lam);
-        atomicAdd(&dm[2 * r.mod], (double)(a.r + a.g + a.b));
-        atomicAdd(&dm[2 * r.mod + 1], (double)(l.r + l.g + l.b));
-    }
-}
-
-__device__ __forceinline__ void adj_lds_init(AdjLds& L)
-{
-    for (int k = threadIdx.x; k < ADJ_LDS_MAT * 3; k += blockDim.x) L.mat[k] = 0.0;
-    for (int k = threadIdx.x; k < ADJ_LDS_MOD * 2; k += blockDim.x) L.mod[k] = 0.0;
-    for (int k = threadIdx.x; k < ADJ_LDS_LIGHT; k += blockDim.x) L.light[k] = 0.0;
-    __syncthreads();
-}
-
-__device__ __forceinline__ void adj_lds_flush(AdjLds& L, const AdjAcc& G, bool lds_ok)
-{
-    __syncthreads();
-    if (!lds_ok) return;
-    for (int k = threadIdx.x; k < G.nmat * 3; k += blockDim.x)
-        if (L.mat[k] != 0.0) atomicAdd(&G.mat[k], L.mat[k]);
-    for (int k = threadIdx.x; k < G.nmod * 2; k += blockDim.x)
-        if (L.mod[k] != 0.0) atomicAdd(&G.mod[k], L.mod[k]);
-    for (int k = threadIdx.x; k < G.nl; k += blockDim.x)
-        if (L.light[k] != 0.0) atomicAdd(&G.light[k], L.light[k]);
-}
-
-__device__ __forceinline__ bool item_output(const Area& a, uint32_t item, int& i, int& j)
-{
-    item_pixel(a, item, i, j);
-    return j >= a.rx && j < a.rx + a.rw && i >= a.ry && i < a.ry + a.rh;
-}
-
-// Direct sun term at the primary hit: Dir = vis * inte * (colour*power) * rho0.
+// Per output pixel: the direct sun term at the primary hit,
+// Dir = vis * inte * (colour*power) * rho0.  With path_terms, also the
+// pathlight's dependence on rho0 through its pixel term
+// rho0 * mult * power * sum_{samples, b} contri_b (from the forward's contri),
+// and the per-item weight W = dLoss/dpixel * rho0 * mult * power that the
+// replay's shading kernel applies at every path vertex (0 where the pixel
+// has no pathlight term).
 __global__ __launch_bounds__(RB) void k_adjoint_direct(const RenderScene* __restrict__ scenes,
                                                        const Area* __restrict__ areas, int nareas, uint32_t nitems,
                                                        const int32_t* __restrict__ item_ti,
                                                        const float* __restrict__ item_t,
-                                                       const float* __restrict__ grad_rgb, AdjAcc G)
+                                                       const float* __restrict__ grad_rgb, AdjAcc G,
+                                                       int path_terms, int S, int D,
+                                                       const uint32_t* __restrict__ qual,
+                                                       const c3* __restrict__ contri,
+                                                       const uint8_t* __restrict__ nseg, c3* __restrict__ w_item)
 {
     extern __shared__ int s_stk[];
     __shared__ AdjLds L;
     const bool lds_ok = G.nmat <= ADJ_LDS_MAT && G.nmod <= ADJ_LDS_MOD && G.nl <= ADJ_LDS_LIGHT;
     adj_lds_init(L);
-    const uint32_t item = blockIdx.x * RB + threadIdx.x;
-    if (item < nitems && item_ti[item] >= 0) {
-        const Area a = areas[find_area(areas, nareas, item)];
-        int i, j;
-        if (item_output(a, item, i, j)) {
-            const RenderScene& R = scenes[a.scene];
-            const float* gp = grad_rgb + 3 * ((size_t)i * R.resx + j);
-            const c3 g = mkc(gp[0], gp[1], gp[2]);
-            LaneStack st;
-            st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = RB; st.sp = 0;
-            f3 eye;
-            const Surf sp = primary_surf(R, a, item, item_ti[item], item_t[item], eye);
-            const RhoD r0 = diffuse_d(R.mats, R.mods, R.texs, sp.mat, sp.P);
-            for (int li = 0; li < R.nl; li++) {
-                const DevLight& Lt = R.lights[li];
-                if (!Lt.use_in_render || Lt.type != YG_LIGHT_SUN) continue;
-                if (Lt.cast_shadows && is_shadowed(R.bvh, sp.P, Lt.dir, sp.obj, false, R.min_raydis, R.self_bias, st))
-                    continue;
-                const f3 N = face_forward(sp.Ng, sp.N, norm3(eye));
-                const float inte = dot(N, Lt.dir);
-                if (inte < 0) continue;
-                const c3 fac = cscale(Lt.col, inte);  // d Dir / d rho0
-                adj_add_vertex(L, G, lds_ok, sp.mat, r0, cmul(g, fac));
-                const c3 dp = cmul(cscale(Lt.color, inte), r0.rho);  // d Dir / d power
-                atomicAdd(lds_ok ? &L.light[li] : &G.light[li], (double)(g.r * dp.r + g.g * dp.g + g.b * dp.b));
+    for (uint32_t item = blockIdx.x * RB + threadIdx.x; item < nitems; item += gridDim.x * RB) {
+        c3 w = mkc(0, 0, 0);
+        if (item_ti[item] >= 0) {
+            const Area a = areas[find_area(areas, nareas, item)];
+            int i, j;
+            if (item_output(a, item, i, j)) {
+                const RenderScene& R = scenes[a.scene];
+                const float* gp = grad_rgb + 3 * ((size_t)i * R.resx + j);
+                const c3 g = mkc(gp[0], gp[1], gp[2]);
+                LaneStack st;
+                st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = RB; st.sp = 0;
+                f3 eye;
+                const Surf sp = primary_surf(R, a, item, item_ti[item], item_t[item], eye);
+                const RhoD r0 = diffuse_d(R.mats, R.mods, R.texs, sp.mat, sp.P);
+                for (int li = 0; li < R.nl; li++) {
+                    const DevLight& Lt = R.lights[li];
+                    if (!Lt.use_in_render || Lt.type != YG_LIGHT_SUN) continue;
+                    if (Lt.cast_shadows && is_shadowed(R.bvh, sp.P, Lt.dir, sp.obj, false, R.min_raydis, R.self_bias, st))
+                        continue;
+                    const f3 N = face_forward(sp.Ng, sp.N, norm3(eye));
+                    const float inte = dot(N, Lt.dir);
+                    if (inte < 0) continue;
+                    const c3 fac = cscale(Lt.col, inte);  // d Dir / d rho0
+                    adj_add_vertex(L, G, lds_ok, sp.mat, r0, cmul(g, fac));
+                    const c3 dp = cmul(cscale(Lt.color, inte), r0.rho);  // d Dir / d power
+                    atomicAdd(lds_ok ? &L.light[li] : &G.light[li], (double)(g.r * dp.r + g.g * dp.g + g.b * dp.b));
+                }
+                if (path_terms && R.path_light >= 0 && R.lights[R.path_light].use_in_render && qual[item]) {
+                    const float pm = (float)(1.0 / (double)(float)S) * R.path_power;
+                    c3 T = mkc(0, 0, 0);
+                    for (int smp = 0; smp < S; smp++) {
+                        const size_t p = (size_t)item * S + smp;
+                        const int ns = nseg[p];
+                        for (int b = 0; b < ns; b++) T = cadd(T, contri[p * D + b]);
+                    }
+                    adj_add_vertex(L, G, lds_ok, sp.mat, r0, cmul(g, cscale(T, pm)));
+                    w = cmul(g, cscale(r0.rho, pm));
+                }
             }
         }
+        if (path_terms) w_item[item] = w;
     }
     adj_lds_flush(L, G, lds_ok);
 }
@@ -1360,15 +1484,44 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
         YG_HIP(hipEventCreate(&e1));
         YG_HIP(hipEventRecord(e0, st));
     }
-    hipLaunchKernelGGL(k_adjoint_direct, dim3(grid_for(W.nitems, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, W.na,
-                       W.nitems, W.item_ti.p, W.item_t.p, d_grad_rgb, G);
-    YG_HIP(hipGetLastError());
+    // replay: the forward's GI levels are traced again and every path vertex
+    // adds its gradient terms; vertices with a zero albedo channel fall back
+    // to the per-path replay below
+    bool fused = !W.any_path;
+    uint32_t n0 = 0;
     if (W.any_path) {
         uint32_t last_q = 0, last_off = 0;
         YG_HIP(hipMemcpyAsync(&last_q, W.qual.p + W.nitems - 1, 4, hipMemcpyDeviceToHost, st));
         YG_HIP(hipMemcpyAsync(&last_off, W.qoff.p + W.nitems - 1, 4, hipMemcpyDeviceToHost, st));
         YG_HIP(hipStreamSynchronize(st));
-        const uint32_t n0 = (last_q + last_off) * (uint32_t)W.S;
+        n0 = (last_q + last_off) * (uint32_t)W.S;
+    }
+    if (!fused) {
+        if ((rc = W.w_item.ensure(W.nitems)) || (rc = W.adeg.ensure(1))) return rc;
+        YG_HIP(hipMemsetAsync(W.adeg.p, 0, sizeof(unsigned long long), st));
+        hipLaunchKernelGGL(k_adjoint_direct, dim3(std::min(grid_for(W.nitems, RB), 4 * ADJ_GRID)), dim3(RB), W.stk_bytes, st, W.scenes.p,
+                           W.areas.p, W.na, W.nitems, W.item_ti.p, W.item_t.p, d_grad_rgb, G, 1, W.S, W.D,
+                           W.qual.p, W.contri.p, W.nseg.p, W.w_item.p);
+        YG_HIP(hipGetLastError());
+        std::vector<hipEvent_t> none;
+        if (n0 > 0 && (rc = trace_levels<true>(st, n0, false, none, G, W.w_item.p, W.adeg.p))) return rc;
+        unsigned long long ndeg = 0;
+        YG_HIP(hipMemcpyAsync(&ndeg, W.adeg.p, sizeof(ndeg), hipMemcpyDeviceToHost, st));
+        YG_HIP(hipStreamSynchronize(st));
+        if (ndeg > 0) {
+            fused = true;
+            YG_HIP(hipMemsetAsync(d_grad_mat, 0, sizeof(double) * 3 * G.nmat, st));
+            if (G.nmod) YG_HIP(hipMemsetAsync(d_grad_mod, 0, sizeof(double) * 2 * G.nmod, st));
+            YG_HIP(hipMemsetAsync(d_grad_light, 0, sizeof(double) * G.nl, st));
+        }
+    }
+    if (fused) {
+        hipLaunchKernelGGL(k_adjoint_direct, dim3(std::min(grid_for(W.nitems, RB), 4 * ADJ_GRID)), dim3(RB), W.stk_bytes, st, W.scenes.p,
+                           W.areas.p, W.na, W.nitems, W.item_ti.p, W.item_t.p, d_grad_rgb, G, 0, W.S, W.D,
+                           W.qual.p, W.contri.p, W.nseg.p, (c3*)nullptr);
+        YG_HIP(hipGetLastError());
+    }
+    if (fused && W.any_path) {
         hipLaunchKernelGGL(k_path_init, dim3(grid_for(W.nitems, 256)), dim3(256), 0, st, W.nitems, W.S, W.qual.p,
                            W.qoff.p, W.active0.p);
         YG_HIP(hipGetLastError());

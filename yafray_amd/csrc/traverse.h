@@ -39,9 +39,38 @@ __device__ __forceinline__ void tri_bary(f3 n, f3 a, f3 b, f3 c, f3 h, float& fb
     fc = beta_numer / denum;
 }
 
+// Loads from scene arrays in device memory.  The arrays reach the kernels
+// through structs in memory, so the compiler cannot infer their address
+// space and would emit flat loads, which share the LDS counter: every wait
+// for a node would also drain the stack's LDS traffic.
+#define YG_GLOBAL __attribute__((address_space(1)))
+typedef float yg_v4f __attribute__((ext_vector_type(4)));
+typedef int yg_v4i __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int gload(const int32_t* p) { return *(const YG_GLOBAL int32_t*)p; }
+
+__device__ __forceinline__ float4 gload(const float4* p)
+{
+    const yg_v4f v = *(const YG_GLOBAL yg_v4f*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+__device__ __forceinline__ GNode gload(const GNode* p)
+{
+    const YG_GLOBAL yg_v4f* q = (const YG_GLOBAL yg_v4f*)p;
+    const yg_v4f a = q[0], b = q[1], c = q[2];
+    const yg_v4i d = ((const YG_GLOBAL yg_v4i*)p)[3];
+    GNode n;
+    n.a = make_float4(a.x, a.y, a.z, a.w);
+    n.b = make_float4(b.x, b.y, b.z, b.w);
+    n.c = make_float4(c.x, c.y, c.z, c.w);
+    n.d = make_int4(d.x, d.y, d.z, d.w);
+    return n;
+}
+
 __device__ __forceinline__ void load_tri(const float4* tris, int ti, f3& a, f3& b, f3& c, f3& n)
 {
-    const float4 A = tris[3 * ti], B = tris[3 * ti + 1], C = tris[3 * ti + 2];
+    const float4 A = gload(tris + 3 * ti), B = gload(tris + 3 * ti + 1), C = gload(tris + 3 * ti + 2);
     a = mk3(A.x, A.y, A.z);
     b = mk3(B.x, B.y, B.z);
     c = mk3(C.x, C.y, C.z);
@@ -128,7 +157,7 @@ __device__ int trace_closest(const BVHView& bvh, f3 o, f3 d, float tmin, float& 
     st.sp = 0;
     for (;;) {
         if (node >= 0) {
-            const GNode n = bvh.nodes[node];
+            const GNode n = gload(bvh.nodes + node);
             if (COUNT) cs->nodes++;
             float t0, t1;
             bool h0, h1;
@@ -157,9 +186,9 @@ __device__ int trace_closest(const BVHView& bvh, f3 o, f3 d, float tmin, float& 
                     tbest = t;
                     tcull = cull_bound(t);
                     best = ti;
-                    bestprim = bvh.tprim[ti];
+                    bestprim = gload(bvh.tprim + ti);
                 } else if (t == tbest && best >= 0) {
-                    const int p = bvh.tprim[ti];
+                    const int p = gload(bvh.tprim + ti);
                     if (p < bestprim) { best = ti; bestprim = p; }
                 }
             }
@@ -184,7 +213,7 @@ __device__ bool trace_any(const BVHView& bvh, f3 o, f3 d, float tmin, float tmax
     st.sp = 0;
     for (;;) {
         if (node >= 0) {
-            const GNode n = bvh.nodes[node];
+            const GNode n = gload(bvh.nodes + node);
             if (COUNT) cs->nodes++;
             float t0, t1;
             bool h0, h1;
@@ -206,7 +235,9 @@ __device__ bool trace_any(const BVHView& bvh, f3 o, f3 d, float tmin, float tmax
                 f3 a, b, c, nn;
                 load_tri(bvh.tris, ti, a, b, c, nn);
                 if (COUNT) cs->tris++;
-                const f3 org = (self_obj >= 0 && bvh.tobj[ti] == self_obj) ? oself : o;
+                // (one-object scenes need no per-triangle object id)
+                const bool same = self_obj >= 0 && (bvh.nobj == 1 ? self_obj == 0 : gload(bvh.tobj + ti) == self_obj);
+                const f3 org = same ? oself : o;
                 if (!tri_hit(a, b, c, nn, org, d)) continue;
                 const float t = tri_t(a, nn, org, d);
                 if (t > tmin && t < tmax) return true;

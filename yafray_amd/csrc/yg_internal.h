@@ -29,9 +29,12 @@ struct alignas(64) GNode {
 
 // Dynamic LDS bytes of a traversal launch with `lanes` threads per block over
 // trees at most `max_depth` deep.
+#ifndef YG_STACK_EXTRA
+#define YG_STACK_EXTRA 0  // extra entries per lane (occupancy tuning builds)
+#endif
 static inline size_t stack_lds_bytes(int max_depth, int lanes)
 {
-    return (size_t)(max_depth + 1) * (size_t)lanes * sizeof(int);
+    return (size_t)(max_depth + 1 + YG_STACK_EXTRA) * (size_t)lanes * sizeof(int);
 }
 
 // Leaf-ordered triangle record, 48 B: (a.xyz, n.x) (b.xyz, n.y) (c.xyz, n.z),
@@ -43,6 +46,7 @@ struct BVHView {
     const int32_t* tobj;    // object id per leaf-ordered triangle
     const int32_t* tmat;    // material id per leaf-ordered triangle
     int32_t ntris;
+    int32_t nobj;           // objects in the scene (1: every triangle is object 0)
 };
 
 struct HostBVH {
@@ -103,7 +107,9 @@ struct yg_scene {
 
     PhotonMapDev pm;
     uint64_t gen = 0;  // bumped by every change of geometry, shading, lights or camera
-    uint64_t checked_gen = ~0ull;  // gen at which the render inputs were last validated
+    uint64_t geo_gen = 0;          // bumped by geometry changes only
+    uint64_t checked_geo = ~0ull;  // geo_gen / material count at which the
+    size_t checked_nmat = 0;       //   triangle material ids were last validated
     uint64_t uid = 0;              // process-unique scene number (never reused)
 
     // per-area Halton tables of the last render (dimension-major, hstride
@@ -117,6 +123,7 @@ struct yg_scene {
         BVHView v;
         v.nodes = d_nodes; v.tris = d_tris; v.tprim = d_tprim; v.tobj = d_tobj; v.tmat = d_tmat;
         v.ntris = (int32_t)tri_mat.size();
+        v.nobj = nobj;
         return v;
     }
 };
