@@ -33,7 +33,7 @@ extern "C" {
 #define YG_ERR_UNSUPPORTED (-4)
 #define YG_ERR_NOMEM (-5)
 
-#define YG_ABI_VERSION 2
+#define YG_ABI_VERSION 3
 
 /* ---------------------------------------------------------------- rays -- */
 
@@ -195,6 +195,7 @@ typedef struct yg_build_opts {
     int32_t max_leaf;      /* triangles per leaf, 1..8 */
     int32_t sah_bins;      /* 8..64 */
     int32_t threads;       /* host build threads, 0 = all */
+    int32_t width;         /* BVH node width: 2 or 4 (0 = default, 4) */
 } yg_build_opts;
 
 typedef struct yg_scene yg_scene;

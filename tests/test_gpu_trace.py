@@ -52,15 +52,15 @@ def _check(prim_g, t_g, u_g, v_g, prim_c, t_c, u_c, v_c):
 
 def test_library_loads(torch_cuda):
     Y = _lib(torch_cuda)
-    assert Y.lib().yg_abi_version() == 2
+    assert Y.lib().yg_abi_version() == 3
 
 
-@pytest.mark.parametrize("seed", [0, 1])
-def test_closest_any_room(torch_cuda, seed):
+@pytest.mark.parametrize("seed,width", [(0, 2), (1, 2), (0, 4), (1, 4)])
+def test_closest_any_room(torch_cuda, seed, width):
     torch = torch_cuda
     Y = _lib(torch)
     d = S.direct_room(64, 48, subdiv=5)
-    sc = Y.Scene.from_desc(d)
+    sc = Y.Scene.from_desc(d, width=width)
     oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
     rays = _rays(200000, -2.9, 2.9, seed, zshift=3.0)
     pc, tc, uc, vc = oc.trace(rays, threads=8)
@@ -75,7 +75,8 @@ def test_closest_any_room(torch_cuda, seed):
     assert np.mean(ag == (ac >= 0)) >= 0.9999
 
 
-def test_tmin_and_empty(torch_cuda):
+@pytest.mark.parametrize("width", [2, 4])
+def test_tmin_and_empty(torch_cuda, width):
     torch = torch_cuda
     Y = _lib(torch)
     # one triangle, rays with tmin excluding the hit
@@ -83,7 +84,7 @@ def test_tmin_and_empty(torch_cuda):
     f = np.array([[0, 1, 2]], np.uint32)
     sc = Y.Scene()
     sc.add_mesh(v, f)
-    sc.build()
+    sc.build(width=width)
     rays = np.array([[0.2, 0.2, 1, 0, 0, 0, -1, np.inf],
                      [0.2, 0.2, 1, 1.5, 0, 0, -1, np.inf],
                      [0.2, 0.2, 1, 0, 0, 0, -1, 0.5],
@@ -93,14 +94,15 @@ def test_tmin_and_empty(torch_cuda):
     assert t[0] == 1.0 and abs(u[0] - 0.2) < 1e-6 and abs(vv[0] - 0.2) < 1e-6
     # an empty scene builds and misses everything
     e = Y.Scene()
-    e.build()
+    e.build(width=width)
     p, t, _, _ = _gpu_trace(Y, torch, e, rays)
     assert np.all(p == -1)
 
 
-def test_terrain_parity_full_size(torch_cuda):
-    """Config-2 terrain (1,045,506 triangles): 1M incoherent rays on the GPU,
-    a 100k-ray sample checked against the oracle."""
+@pytest.mark.parametrize("width", [2, 4])
+def test_terrain_parity_full_size(torch_cuda, width):
+    """1,045,506-triangle fBm terrain: 1M incoherent rays on the GPU, a
+    100k-ray sample checked against the oracle."""
     torch = torch_cuda
     Y = _lib(torch)
     verts, idx, band = S.terrain_mesh(1024, 512, Y.device_heights)
@@ -109,7 +111,7 @@ def test_terrain_parity_full_size(torch_cuda):
     assert np.array_equal(hv, verts)
     sc = Y.Scene()
     sc.add_mesh(verts, idx, band)
-    sc.build()
+    sc.build(width=width)
     info = sc.info()
     assert info["tris"] == 1045506
     rng = np.random.default_rng(7)
@@ -131,11 +133,12 @@ def test_terrain_parity_full_size(torch_cuda):
     assert np.mean(pg >= 0) > 0.3
 
 
-def test_first_hit_is_shadowed(torch_cuda):
+@pytest.mark.parametrize("width", [2, 4])
+def test_first_hit_is_shadowed(torch_cuda, width):
     torch = torch_cuda
     Y = _lib(torch)
     d = S.direct_room(64, 48, subdiv=4)
-    sc = Y.Scene.from_desc(d)
+    sc = Y.Scene.from_desc(d, width=width)
     oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
     rng = np.random.default_rng(3)
     n = 50000

@@ -193,8 +193,92 @@ static void inflate(const BBox& b, float G, float lo[3], float hi[3])
 
 }  // namespace
 
+// Collapses the binary tree into four-wide nodes: a node's children start as
+// the binary node's two and the largest-area inner child is replaced by its
+// own two children until there are four or only leaves remain.  Boxes and
+// leaf references are those of the binary tree, so the width changes the
+// traversal order only, never which triangles a ray is tested against.
+static void export_width4(const BNode* bn, float G, HostBVH& out,
+                          int32_t (*enc_leaf)(const BNode&), void (*inflate_fn)(const BBox&, float, float*, float*))
+{
+    out.nodes4.clear();
+    out.stack4 = 0;
+    auto set_empty = [](GNode4& g, int k) {
+        float* f[6] = {&g.lox.x, &g.hix.x, &g.loy.x, &g.hiy.x, &g.loz.x, &g.hiz.x};
+        for (int c = 0; c < 6; c++) f[c][k] = INFINITY;
+        (&g.child.x)[k] = 0;
+    };
+    auto set_child = [&](GNode4& g, int k, const BNode& c, int32_t ref) {
+        float lo[3], hi[3];
+        inflate_fn(c.box, G, lo, hi);
+        (&g.lox.x)[k] = lo[0]; (&g.hix.x)[k] = hi[0];
+        (&g.loy.x)[k] = lo[1]; (&g.hiy.x)[k] = hi[1];
+        (&g.loz.x)[k] = lo[2]; (&g.hiz.x)[k] = hi[2];
+        (&g.child.x)[k] = ref;
+    };
+    auto children = [&](int64_t bi, int64_t* ch) -> int {
+        int n = 0;
+        ch[n++] = bn[bi].left;
+        ch[n++] = bn[bi].right;
+        while (n < 4) {
+            int best = -1;
+            double ba = -1.0;
+            for (int k = 0; k < n; k++)
+                if (bn[ch[k]].left >= 0 && bn[ch[k]].box.area() > ba) { ba = bn[ch[k]].box.area(); best = k; }
+            if (best < 0) break;
+            const int64_t x = ch[best];
+            for (int k = n; k > best + 1; k--) ch[k] = ch[k - 1];  // keep left-to-right order
+            ch[best] = bn[x].left;
+            ch[best + 1] = bn[x].right;
+            n++;
+        }
+        return n;
+    };
+    const BNode& root = bn[0];
+    if (root.left < 0) {
+        GNode4 g;
+        memset(&g, 0, sizeof(g));
+        for (int k = 0; k < 4; k++) set_empty(g, k);
+        if (root.count > 0) set_child(g, 0, root, enc_leaf(root));
+        out.nodes4.push_back(g);
+        out.stack4 = 0;
+        return;
+    }
+    // (binary node, slot, stack entries pending above this node)
+    struct Work { int64_t bi; int64_t gi; int32_t pend; };
+    std::vector<Work> work;
+    out.nodes4.emplace_back();
+    work.push_back({0, 0, 0});
+    while (!work.empty()) {
+        const Work w = work.back();
+        work.pop_back();
+        int64_t ch[4];
+        const int n = children(w.bi, ch);
+        GNode4 g;
+        memset(&g, 0, sizeof(g));
+        int32_t refs[4];
+        for (int k = 0; k < n; k++) {
+            if (bn[ch[k]].left < 0) refs[k] = enc_leaf(bn[ch[k]]);
+            else {
+                refs[k] = (int32_t)out.nodes4.size();
+                out.nodes4.emplace_back();
+            }
+        }
+        for (int k = 0; k < 4; k++) {
+            if (k < n) set_child(g, k, bn[ch[k]], refs[k]);
+            else set_empty(g, k);
+        }
+        out.nodes4[w.gi] = g;
+        // a visit pushes at most n-1 siblings on top of what its ancestors left
+        const int32_t pend = w.pend + (n - 1);
+        out.stack4 = std::max(out.stack4, pend);
+        for (int k = n - 1; k >= 0; k--)
+            if (bn[ch[k]].left >= 0) work.push_back({ch[k], refs[k], pend});
+    }
+}
+
 void build_bvh(const float* verts, const uint32_t* idx, int64_t nt, const int32_t* tri_obj,
-               const int32_t* tri_mat, int max_leaf, int bins, int threads, HostBVH& out)
+               const int32_t* tri_mat, int max_leaf, int bins, int threads, int width, HostBVH& out)
 {
     out.nodes.clear();
     out.tris.clear();
@@ -304,6 +388,9 @@ void build_bvh(const float* verts, const uint32_t* idx, int64_t nt, const int32_
         set_child(g, 1, emptyn, ~0);
         if (root.count == 0) set_child(g, 0, emptyn, ~0);
         out.nodes.push_back(g);
+        if (width == 4)
+            export_width4(bn.get(), G, out, [](const BNode& n) -> int32_t { return ~(int32_t)(n.start * YG_LEAF_MAX + (n.count - 1)); },
+                          inflate);
         return;
     }
     out.nodes.emplace_back();
@@ -329,4 +416,7 @@ void build_bvh(const float* verts, const uint32_t* idx, int64_t nt, const int32_
         for (int k = 1; k >= 0; k--)
             if (ch[k]->left >= 0) work.push_back({(int64_t)(k == 0 ? n.left : n.right), refs[k]});
     }
+    if (width == 4)
+        export_width4(bn.get(), G, out, [](const BNode& n) -> int32_t { return ~(int32_t)(n.start * YG_LEAF_MAX + (n.count - 1)); },
+                      inflate);
 }

@@ -79,6 +79,7 @@ __global__ void k_emit_pick(uint32_t natt, uint32_t np, const uint32_t* __restri
 
 // One photon segment at depth d (shoot_photon_diffuse, photonlight.cc:203-244)
 #define PB 64
+template <int W>
 __global__ __launch_bounds__(PB) void k_photon_seg(PhotonSrc src, BVHView bvh, const yg_material* __restrict__ mats,
                                                     const yg_modulator* __restrict__ mods,
                                                     const yg_texture* __restrict__ texs,
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(PB) void k_photon_seg(PhotonSrc src, BVHView bvh, c
     }
     atomicAdd(nrays, 1ull);
     float t;
-    const int ti = first_hit(bvh, pos, dir, min_raydis, t, st);
+    const int ti = first_hit<W>(bvh, pos, dir, min_raydis, t, st);
     if (ti < 0) return;
     const f3 f = pos + dir * min_raydis;
     const f3 hp = f + dir * t;
@@ -398,9 +399,10 @@ extern "C" int yg_photon_build(yg_scene* s, uint32_t li, const yg_render_opts* o
         YG_HIP(hipMemsetAsync(bflag, 0, sizeof(uint32_t) * np, st));
         const float* h0 = (d == 1) ? hdim[0] : hdim[2 * (d - 1)];
         const float* h1 = (d == 1) ? hdim[1] : hdim[2 * (d - 1) + 1];
-        hipLaunchKernelGGL(k_photon_seg, dim3(grid_for(nactive, PB)), dim3(PB), stack_lds_bytes(s->max_depth, PB), st, src, s->view(), s->d_mats,
-                           s->d_mods, s->d_texs, s->d_obj_flags, opts->min_raydis, d, act, nactive, att, h0, h1,
-                           bidx, pst, slots, mvalid, bflag, nrays);
+        YG_LAUNCH_W(s->width, (k_photon_seg<4>), (k_photon_seg<2>), dim3(grid_for(nactive, PB)), dim3(PB),
+                    stack_lds_bytes(s->stack_entries, PB), st, src, s->view(), s->d_mats, s->d_mods, s->d_texs,
+                    s->d_obj_flags, opts->min_raydis, d, act, nactive, att, h0, h1, bidx, pst, slots, mvalid, bflag,
+                    nrays);
         YG_HIP(hipGetLastError());
         if (d > L.depth) break;
         if ((rc = scan_u32(F, bflag, bex, np, st))) return rc;

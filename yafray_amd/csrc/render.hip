@@ -64,6 +64,7 @@ __device__ __forceinline__ void block_add(unsigned long long* dst, uint32_t v)
 // ------------------------------------------------------------- kernels --
 
 // Camera rays and their closest hits (scene_t::raytrace, scene.cc:200-235).
+template <int W>
 __global__ __launch_bounds__(RB) void k_primary(const RenderScene* __restrict__ scenes,
                                                 const Area* __restrict__ areas, int nareas,
                                                 uint32_t nitems, int32_t* __restrict__ item_ti,
@@ -81,7 +82,7 @@ __global__ __launch_bounds__(RB) void k_primary(const RenderScene* __restrict__ 
     const f3 ray = cam_ray(R, (float)j + 0.5f, (float)i + 0.5f);
     float t = INFINITY;
     int ti = -1;
-    if (R.raydepth > 0) ti = first_hit(R.bvh, R.cam_pos, ray, R.min_raydis, t, st);
+    if (R.raydepth > 0) ti = first_hit<W>(R.bvh, R.cam_pos, ray, R.min_raydis, t, st);
     item_ti[item] = ti;
     item_t[item] = t;
 }
@@ -130,6 +131,7 @@ __global__ __launch_bounds__(RB) void k_prep(const RenderScene* __restrict__ sce
 }
 
 // Sun and area lights at the primary hit, one colour per light.
+template <int W>
 __global__ __launch_bounds__(RB) void k_direct(const RenderScene* __restrict__ scenes,
                                                const Area* __restrict__ areas, int nareas,
                                                uint32_t nitems, const int32_t* __restrict__ item_ti,
@@ -156,14 +158,14 @@ __global__ __launch_bounds__(RB) void k_direct(const RenderScene* __restrict__ s
                 const DevLight& L = R.lights[li];
                 c3 c = mkc(0, 0, 0);
                 if (L.use_in_render) {
-                    if (L.type == YG_LIGHT_SUN) c = sun_illuminate(R, L, sp, eye, st, nshadow);
+                    if (L.type == YG_LIGHT_SUN) c = sun_illuminate<W>(R, L, sp, eye, st, nshadow);
                     else if (L.type == YG_LIGHT_AREA) {
                         if (!seeded) {
                             seed = lcg_jump(R.lcg_seed, (uint64_t)(lcg_off[item] - lcg_off[R.item_base]), YG_LCG_POW);
                             seeded = true;
                         }
                         if (dot(sp.P - L.from, L.dir) > 0) ndraws += 2u * (uint32_t)L.samples;
-                        c = area_illuminate(R, L, sp, eye, seed, st, nshadow);
+                        c = area_illuminate<W>(R, L, sp, eye, seed, st, nshadow);
                     }
                 }
                 lightc[(size_t)item * nl_max + li] = c;
@@ -267,7 +269,7 @@ __device__ __forceinline__ void queue_push(bool want, uint32_t p, uint32_t* qcou
 #ifndef YG_BOUNCE_WPE
 #define YG_BOUNCE_WPE 1
 #endif
-template <bool ADJ>  // ADJ: replay for the adjoint (contri/nseg of the forward are kept)
+template <bool ADJ, int W>  // ADJ: replay for the adjoint (contri/nseg of the forward are kept)
 __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WPE))) void k_ext(
     const RenderScene* __restrict__ scenes, const Area* __restrict__ areas, int nareas, int j,
     const uint32_t* __restrict__ active, uint32_t nactive, uint32_t nactive_cap, int S, int D,
@@ -327,7 +329,7 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WP
             next = 1;
             float t;
             const f3 f = where + ray * R.min_raydis;
-            const int ti = first_hit(R.bvh, where, ray, R.min_raydis, t, st);
+            const int ti = first_hit<W>(R.bvh, where, ray, R.min_raydis, t, st);
             if (ti < 0) {
                 if (!ADJ) {
                     contri[(size_t)p * D + j] = cmul(cscale(raycolor, s), R.bg);
@@ -345,7 +347,7 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WP
     if (!ADJ) block_add(&cnt->ext, next);
 }
 
-template <bool ADJ>
+template <bool ADJ, int W>
 __global__ __launch_bounds__(RB) void k_shadow(const RenderScene* __restrict__ scenes,
                                                const Area* __restrict__ areas, int nareas, int S,
                                                const uint32_t* __restrict__ qcount,
@@ -369,7 +371,7 @@ __global__ __launch_bounds__(RB) void k_shadow(const RenderScene* __restrict__ s
             const DevLight& L = R.lights[li];
             if (!L.use_in_indirect || L.type != YG_LIGHT_SUN || !L.cast_shadows) continue;
             nshadow++;
-            if (is_shadowed(R.bvh, P, L.dir, obj, false, R.min_raydis, R.self_bias, st)) mask |= 1u << li;
+            if (is_shadowed<W>(R.bvh, P, L.dir, obj, false, R.min_raydis, R.self_bias, st)) mask |= 1u << li;
         }
         vis[p] = mask;
     }
@@ -740,6 +742,7 @@ struct RenderWS {
     int S = 0, D = 0, na = 0, nl_max = 0;
     bool any_path = false;
     size_t stk_bytes = 0;  // dynamic LDS of the traversal launches
+    int width = 2;         // BVH width of the scenes in the launch
     // the single-scene render the workspace holds (for YG_RENDER_REUSE)
     uint64_t last_uid = 0;  // yg_scene::uid, 0 = none
     uint64_t last_gen = 0;
@@ -932,13 +935,13 @@ static int trace_levels(hipStream_t st, uint32_t nactive, bool ktiming, std::vec
         }
         const unsigned grid = grid_for(nactive, RB);
         uint32_t* qc = W.qcnt.p + j;
-        hipLaunchKernelGGL(k_ext<ADJ>, dim3(grid), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na, j, act,
+        YG_LAUNCH_W(W.width, (k_ext<ADJ, 4>), (k_ext<ADJ, 2>), dim3(grid), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na, j, act,
                            nactive, npaths, S, D, W.item_ti.p, W.item_t.p, W.qoff.p, W.hidx.p, W.st_where.p,
                            W.st_hn.p, W.st_ti.p, W.st_col.p, W.st_s.p, W.contri.p, W.nseg.p, W.h_p.p,
                            W.h_ray.p, W.h_ti.p, qc, W.queue.p, W.cnt.p);
         YG_HIP(hipGetLastError());
         if (ktiming) YG_HIP(hipEventRecord(ev[1], st));
-        hipLaunchKernelGGL(k_shadow<ADJ>, dim3(grid), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na, S, qc,
+        YG_LAUNCH_W(W.width, (k_shadow<ADJ, 4>), (k_shadow<ADJ, 2>), dim3(grid), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na, S, qc,
                            W.queue.p, W.h_p.p, W.h_ti.p, W.vis.p, W.cnt.p);
         YG_HIP(hipGetLastError());
         if (ktiming) YG_HIP(hipEventRecord(ev[2], st));
@@ -1072,9 +1075,14 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     }
     YG_HIP(hipMemsetAsync(W.cnt.p, 0, sizeof(Counters), st));
 
-    int max_depth = 1;
-    for (auto& P : plans) max_depth = std::max(max_depth, P.s->max_depth);
+    int max_depth = 1;  // stack entries
+    for (auto& P : plans) {
+        max_depth = std::max(max_depth, P.s->stack_entries);
+        if (P.s->width != plans[0].s->width)
+            return yg_fail(YG_ERR_UNSUPPORTED, "batched scenes need BVHs of one width");
+    }
     W.stk_bytes = stack_lds_bytes(max_depth, RB);
+    W.width = plans[0].s->width;
     W.nitems = nitems;
     W.npaths = npaths;
     W.S = S;
@@ -1092,14 +1100,14 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         YG_HIP(hipEventRecord(e0, st));
     }
     const int na = (int)areas.size();
-    hipLaunchKernelGGL(k_primary, dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na,
+    YG_LAUNCH_W(W.width, (k_primary<4>), (k_primary<2>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na,
                        nitems, W.item_ti.p, W.item_t.p);
     YG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_prep, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
                        nitems, W.item_ti.p, W.item_t.p, W.draws.p, W.qual.p);
     YG_HIP(hipGetLastError());
     if (any_area && (rc = exclusive_scan(W.draws.p, W.lcg_off.p, nitems, st))) return rc;
-    hipLaunchKernelGGL(k_direct, dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na,
+    YG_LAUNCH_W(W.width, (k_direct<4>), (k_direct<2>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na,
                        nitems, W.item_ti.p, W.item_t.p, W.lcg_off.p, W.lightc.p, nl_max, W.cnt.p);
     YG_HIP(hipGetLastError());
     if (any_photon) {
@@ -1239,6 +1247,7 @@ __global__ __launch_bounds__(64) void k_photon_direct(const RenderScene* __restr
 // and the per-item weight W = dLoss/dpixel * rho0 * mult * power that the
 // replay's shading kernel applies at every path vertex (0 where the pixel
 // has no pathlight term).
+template <int W>
 __global__ __launch_bounds__(RB) void k_adjoint_direct(const RenderScene* __restrict__ scenes,
                                                        const Area* __restrict__ areas, int nareas, uint32_t nitems,
                                                        const int32_t* __restrict__ item_ti,
@@ -1270,7 +1279,8 @@ __global__ __launch_bounds__(RB) void k_adjoint_direct(const RenderScene* __rest
                 for (int li = 0; li < R.nl; li++) {
                     const DevLight& Lt = R.lights[li];
                     if (!Lt.use_in_render || Lt.type != YG_LIGHT_SUN) continue;
-                    if (Lt.cast_shadows && is_shadowed(R.bvh, sp.P, Lt.dir, sp.obj, false, R.min_raydis, R.self_bias, st))
+                    if (Lt.cast_shadows &&
+                        is_shadowed<W>(R.bvh, sp.P, Lt.dir, sp.obj, false, R.min_raydis, R.self_bias, st))
                         continue;
                     const f3 N = face_forward(sp.Ng, sp.N, norm3(eye));
                     const float inte = dot(N, Lt.dir);
@@ -1299,6 +1309,7 @@ __global__ __launch_bounds__(RB) void k_adjoint_direct(const RenderScene* __rest
 }
 
 // Pathlight sample replay (one thread per qualifying path).
+template <int W>
 __global__ __launch_bounds__(RB) void k_adjoint_path(const RenderScene* __restrict__ scenes,
                                                      const Area* __restrict__ areas, int nareas,
                                                      const uint32_t* __restrict__ active, uint32_t nactive,
@@ -1348,7 +1359,7 @@ __global__ __launch_bounds__(RB) void k_adjoint_path(const RenderScene* __restri
                 if (b == 0) s = dot(ray, N);
                 if (cenergy(raycolor) < 0.05f) break;
                 float t;
-                const int ti = first_hit(R.bvh, where, ray, R.min_raydis, t, st);
+                const int ti = first_hit<W>(R.bvh, where, ray, R.min_raydis, t, st);
                 nb = b + 1;
                 if (ti < 0) {
                     hit[b] = false;
@@ -1366,7 +1377,7 @@ __global__ __launch_bounds__(RB) void k_adjoint_path(const RenderScene* __restri
                     const DevLight& Lt = R.lights[li];
                     if (!Lt.use_in_indirect || Lt.type != YG_LIGHT_SUN) continue;
                     if (Lt.cast_shadows &&
-                        is_shadowed(R.bvh, sp.P, Lt.dir, sp.obj, false, R.min_raydis, R.self_bias, st))
+                        is_shadowed<W>(R.bvh, sp.P, Lt.dir, sp.obj, false, R.min_raydis, R.self_bias, st))
                         continue;
                     const f3 Nl = face_forward(sp.Ng, sp.N, norm3(eye2));
                     const float inte = dot(Nl, Lt.dir);
@@ -1395,7 +1406,7 @@ __global__ __launch_bounds__(RB) void k_adjoint_path(const RenderScene* __restri
                 }
             }
             adj_add_vertex(L, G, lds_ok, sp0.mat, r0, cmul(g, cscale(T, pm)));
-            const c3 W = cmul(g, cscale(r0.rho, pm));
+            const c3 Wp = cmul(g, cscale(r0.rho, pm));
             // d T / d rho(H_k): bounces after k through R_b, bounce k through X_k
             for (int k = 0; k < nb; k++) {
                 if (!hit[k]) continue;
@@ -1409,13 +1420,13 @@ __global__ __launch_bounds__(RB) void k_adjoint_path(const RenderScene* __restri
                     }
                     if (hit[b] && b != k) Rex = cmul(Rex, rv[b].rho);
                 }
-                adj_add_vertex(L, G, lds_ok, mv[k], rv[k], cmul(W, dk));
+                adj_add_vertex(L, G, lds_ok, mv[k], rv[k], cmul(Wp, dk));
             }
             // sun power: every hit term is linear in it (one sun light assumed)
             for (int li = 0; li < R.nl; li++) {
                 const DevLight& Lt = R.lights[li];
                 if (Lt.use_in_indirect && Lt.type == YG_LIGHT_SUN) {
-                    const c3 q = cmul(W, dpow);
+                    const c3 q = cmul(Wp, dpow);
                     atomicAdd(lds_ok ? &L.light[li] : &G.light[li], (double)(q.r + q.g + q.b));
                     break;
                 }
@@ -1499,7 +1510,7 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
     if (!fused) {
         if ((rc = W.w_item.ensure(W.nitems)) || (rc = W.adeg.ensure(1))) return rc;
         YG_HIP(hipMemsetAsync(W.adeg.p, 0, sizeof(unsigned long long), st));
-        hipLaunchKernelGGL(k_adjoint_direct, dim3(std::min(grid_for(W.nitems, RB), 4 * ADJ_GRID)), dim3(RB), W.stk_bytes, st, W.scenes.p,
+        YG_LAUNCH_W(W.width, (k_adjoint_direct<4>), (k_adjoint_direct<2>), dim3(std::min(grid_for(W.nitems, RB), 4 * ADJ_GRID)), dim3(RB), W.stk_bytes, st, W.scenes.p,
                            W.areas.p, W.na, W.nitems, W.item_ti.p, W.item_t.p, d_grad_rgb, G, 1, W.S, W.D,
                            W.qual.p, W.contri.p, W.nseg.p, W.w_item.p);
         YG_HIP(hipGetLastError());
@@ -1516,7 +1527,7 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
         }
     }
     if (fused) {
-        hipLaunchKernelGGL(k_adjoint_direct, dim3(std::min(grid_for(W.nitems, RB), 4 * ADJ_GRID)), dim3(RB), W.stk_bytes, st, W.scenes.p,
+        YG_LAUNCH_W(W.width, (k_adjoint_direct<4>), (k_adjoint_direct<2>), dim3(std::min(grid_for(W.nitems, RB), 4 * ADJ_GRID)), dim3(RB), W.stk_bytes, st, W.scenes.p,
                            W.areas.p, W.na, W.nitems, W.item_ti.p, W.item_t.p, d_grad_rgb, G, 0, W.S, W.D,
                            W.qual.p, W.contri.p, W.nseg.p, (c3*)nullptr);
         YG_HIP(hipGetLastError());
@@ -1539,7 +1550,7 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
             if ((rc = W.scan_tmp.ensure(bytes))) return rc;
             YG_HIP(hipcub::DeviceRadixSort::SortPairsDescending(W.scan_tmp.p, bytes, W.akey0.p, W.akey1.p,
                                                                 W.active0.p, W.active1.p, (int)n0, 0, 4, st));
-            hipLaunchKernelGGL(k_adjoint_path, dim3(grid_for(n0, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, W.na,
+            YG_LAUNCH_W(W.width, (k_adjoint_path<4>), (k_adjoint_path<2>), dim3(grid_for(n0, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, W.na,
                                W.active1.p, n0, W.npaths, W.S, W.D, W.item_ti.p, W.item_t.p, W.qoff.p, W.hidx.p,
                                d_grad_rgb, G);
             YG_HIP(hipGetLastError());

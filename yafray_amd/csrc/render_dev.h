@@ -103,12 +103,13 @@ __device__ __forceinline__ c3 from_light(const RenderScene& R, const Surf& sp, f
 }
 
 // sunLight_t::illuminate (sunlight.cc:35-57)
+template <int W>
 __device__ __forceinline__ c3 sun_illuminate(const RenderScene& R, const DevLight& L, const Surf& sp,
                                              f3 eye, LaneStack& st, uint32_t& nshadow)
 {
     if (L.cast_shadows) {
         nshadow++;
-        if (is_shadowed(R.bvh, sp.P, L.dir, sp.obj, false, R.min_raydis, R.self_bias, st))
+        if (is_shadowed<W>(R.bvh, sp.P, L.dir, sp.obj, false, R.min_raydis, R.self_bias, st))
             return mkc(0, 0, 0);
     }
     return from_light(R, sp, L.dir, L.col, eye);
@@ -137,6 +138,7 @@ __device__ __forceinline__ int32_t lcg_jump(uint32_t seed, uint64_t n, const uin
 
 // areaLight_t::illuminate, PENUMBRA branch (arealight.cc:93-174), draws taken
 // from the ourRandom() stream starting at *seed
+template <int W>
 __device__ __forceinline__ c3 area_illuminate(const RenderScene& R, const DevLight& L, const Surf& sp,
                                               f3 eye, int32_t& seed, LaneStack& st, uint32_t& nshadow)
 {
@@ -152,7 +154,7 @@ __device__ __forceinline__ c3 area_illuminate(const RenderScene& R, const DevLig
         const f3 Lv = sampleP - sp.P;
         if (dot(Lv, N) < 0) continue;
         nshadow++;
-        if (!is_shadowed(R.bvh, sp.P, sampleP, sp.obj, true, R.min_raydis, R.self_bias, st)) {
+        if (!is_shadowed<W>(R.bvh, sp.P, sampleP, sp.obj, true, R.min_raydis, R.self_bias, st)) {
             f3 dir = Lv;
             float LD2 = normlensqr3(dir);
             LD2 = (LD2 != 0) ? (1.0f / LD2) : 1.0f;
@@ -177,6 +179,7 @@ __device__ __forceinline__ f3 hemi_direction(float z1, float z2, f3 N, f3 Ru, f3
 // scene_t::light with indirect = true restricted to sun lights (the only
 // lights the GPU path supports inside GI paths): the getLight of
 // pathlight.cc:186-191 at a path vertex, eye = (P + (-ray)) - P.
+template <int W>
 __device__ __forceinline__ c3 indirect_light(const RenderScene& R, const Surf& sp, f3 ray, LaneStack& st,
                                              uint32_t& nshadow)
 {
@@ -186,7 +189,7 @@ __device__ __forceinline__ c3 indirect_light(const RenderScene& R, const Surf& s
     for (int li = 0; li < R.nl; li++) {
         const DevLight& L = R.lights[li];
         if (!L.use_in_indirect) continue;
-        if (L.type == YG_LIGHT_SUN) fl = cadd(fl, sun_illuminate(R, L, sp, eye, st, nshadow));
+        if (L.type == YG_LIGHT_SUN) fl = cadd(fl, sun_illuminate<W>(R, L, sp, eye, st, nshadow));
     }
     return fl;
 }

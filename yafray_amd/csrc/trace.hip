@@ -8,7 +8,7 @@
 
 // --------------------------------------------------------------- kernels --
 
-template <bool COUNT>
+template <bool COUNT, int W>
 __global__ __launch_bounds__(TB) void k_trace_closest(BVHView bvh, const yg_ray* __restrict__ rays,
                                                        uint32_t n, yg_hit* __restrict__ hits,
                                                        unsigned long long* counters)
@@ -22,7 +22,7 @@ __global__ __launch_bounds__(TB) void k_trace_closest(BVHView bvh, const yg_ray*
     const f3 o = mk3(r.ox, r.oy, r.oz), d = mk3(r.dx, r.dy, r.dz);
     float t = r.tmax;
     TraceStats cs = {0, 0};
-    const int ti = trace_closest<COUNT>(bvh, o, d, r.tmin, t, st, &cs);
+    const int ti = trace_closest<COUNT, W>(bvh, o, d, r.tmin, t, st, &cs);
     yg_hit h;
     if (ti >= 0) {
         f3 a, b, c, nn;
@@ -41,6 +41,7 @@ __global__ __launch_bounds__(TB) void k_trace_closest(BVHView bvh, const yg_ray*
     }
 }
 
+template <int W>
 __global__ __launch_bounds__(TB) void k_trace_any(BVHView bvh, const yg_ray* __restrict__ rays,
                                                   uint32_t n, uint8_t* __restrict__ occ)
 {
@@ -52,9 +53,10 @@ __global__ __launch_bounds__(TB) void k_trace_any(BVHView bvh, const yg_ray* __r
     const yg_ray r = rays[i];
     const f3 o = mk3(r.ox, r.oy, r.oz), d = mk3(r.dx, r.dy, r.dz);
     const float blo = r.tmin - 1e-4f * fabsf(r.tmin) - 1e-30f;
-    occ[i] = trace_any<false>(bvh, o, d, r.tmin, r.tmax, -1, o, blo, r.tmax, st) ? 1 : 0;
+    occ[i] = trace_any<false, W>(bvh, o, d, r.tmin, r.tmax, -1, o, blo, r.tmax, st) ? 1 : 0;
 }
 
+template <int W>
 __global__ __launch_bounds__(TB) void k_first_hit(BVHView bvh, const float* __restrict__ from,
                                                   const float* __restrict__ dir, uint32_t n,
                                                   float min_raydis, yg_hit* __restrict__ hits)
@@ -66,7 +68,7 @@ __global__ __launch_bounds__(TB) void k_first_hit(BVHView bvh, const float* __re
     st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = TB; st.sp = 0;
     const f3 fr = ld3(from + 3 * (size_t)i), d = ld3(dir + 3 * (size_t)i);
     float t;
-    const int ti = first_hit(bvh, fr, d, min_raydis, t, st);
+    const int ti = first_hit<W>(bvh, fr, d, min_raydis, t, st);
     yg_hit h;
     h.prim = (ti >= 0) ? bvh.tprim[ti] : -1;
     h.t = t;
@@ -74,6 +76,7 @@ __global__ __launch_bounds__(TB) void k_first_hit(BVHView bvh, const float* __re
     hits[i] = h;
 }
 
+template <int W>
 __global__ __launch_bounds__(TB) void k_is_shadowed(BVHView bvh, const float* __restrict__ p,
                                                     const float* __restrict__ l,
                                                     const int32_t* __restrict__ self_obj,
@@ -86,7 +89,7 @@ __global__ __launch_bounds__(TB) void k_is_shadowed(BVHView bvh, const float* __
     LaneStack st;
     st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = TB; st.sp = 0;
     const int so = self_obj ? self_obj[i] : -1;
-    out[i] = is_shadowed(bvh, ld3(p + 3 * (size_t)i), ld3(l + 3 * (size_t)i), so, to_point != 0,
+    out[i] = is_shadowed<W>(bvh, ld3(p + 3 * (size_t)i), ld3(l + 3 * (size_t)i), so, to_point != 0,
                          min_raydis, self_bias, st) ? 1 : 0;
 }
 
@@ -137,8 +140,8 @@ extern "C" int yg_trace_closest(yg_scene* s, const yg_ray* d_rays, uint32_t n, y
     if (rc) return rc;
     if (n == 0) return YG_OK;
     if (!d_rays || !d_hits) return yg_fail(YG_ERR_ARG, "null buffer");
-    hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid_for(n, TB)), dim3(TB), stack_lds_bytes(s->max_depth, TB),
-                       (hipStream_t)stream, s->view(), d_rays, n, d_hits, nullptr);
+    YG_LAUNCH_W(s->width, (k_trace_closest<false, 4>), (k_trace_closest<false, 2>), dim3(grid_for(n, TB)), dim3(TB),
+                stack_lds_bytes(s->stack_entries, TB), (hipStream_t)stream, s->view(), d_rays, n, d_hits, nullptr);
     YG_HIP(hipGetLastError());
     return YG_OK;
 }
@@ -151,8 +154,8 @@ extern "C" int yg_trace_closest_counted(yg_scene* s, const yg_ray* d_rays, uint3
     int rc = check_scene(s);
     if (rc) return rc;
     if (n == 0) return YG_OK;
-    hipLaunchKernelGGL(k_trace_closest<true>, dim3(grid_for(n, TB)), dim3(TB), stack_lds_bytes(s->max_depth, TB),
-                       (hipStream_t)stream, s->view(), d_rays, n, d_hits, d_counters);
+    YG_LAUNCH_W(s->width, (k_trace_closest<true, 4>), (k_trace_closest<true, 2>), dim3(grid_for(n, TB)), dim3(TB),
+                stack_lds_bytes(s->stack_entries, TB), (hipStream_t)stream, s->view(), d_rays, n, d_hits, d_counters);
     YG_HIP(hipGetLastError());
     return YG_OK;
 }
@@ -164,8 +167,8 @@ extern "C" int yg_trace_any(yg_scene* s, const yg_ray* d_rays, uint32_t n, uint8
     if (rc) return rc;
     if (n == 0) return YG_OK;
     if (!d_rays || !d_occluded) return yg_fail(YG_ERR_ARG, "null buffer");
-    hipLaunchKernelGGL(k_trace_any, dim3(grid_for(n, TB)), dim3(TB), stack_lds_bytes(s->max_depth, TB), (hipStream_t)stream,
-                       s->view(), d_rays, n, d_occluded);
+    YG_LAUNCH_W(s->width, (k_trace_any<4>), (k_trace_any<2>), dim3(grid_for(n, TB)), dim3(TB),
+                stack_lds_bytes(s->stack_entries, TB), (hipStream_t)stream, s->view(), d_rays, n, d_occluded);
     YG_HIP(hipGetLastError());
     return YG_OK;
 }
@@ -177,8 +180,9 @@ extern "C" int yg_first_hit(yg_scene* s, const float* d_from, const float* d_dir
     if (rc) return rc;
     if (n == 0) return YG_OK;
     if (!d_from || !d_dir || !d_hits) return yg_fail(YG_ERR_ARG, "null buffer");
-    hipLaunchKernelGGL(k_first_hit, dim3(grid_for(n, TB)), dim3(TB), stack_lds_bytes(s->max_depth, TB), (hipStream_t)stream,
-                       s->view(), d_from, d_dir, n, min_raydis, d_hits);
+    YG_LAUNCH_W(s->width, (k_first_hit<4>), (k_first_hit<2>), dim3(grid_for(n, TB)), dim3(TB),
+                stack_lds_bytes(s->stack_entries, TB), (hipStream_t)stream, s->view(), d_from, d_dir, n, min_raydis,
+                d_hits);
     YG_HIP(hipGetLastError());
     return YG_OK;
 }
@@ -191,8 +195,9 @@ extern "C" int yg_is_shadowed(yg_scene* s, const float* d_p, const float* d_l,
     if (rc) return rc;
     if (n == 0) return YG_OK;
     if (!d_p || !d_l || !d_out) return yg_fail(YG_ERR_ARG, "null buffer");
-    hipLaunchKernelGGL(k_is_shadowed, dim3(grid_for(n, TB)), dim3(TB), stack_lds_bytes(s->max_depth, TB), (hipStream_t)stream,
-                       s->view(), d_p, d_l, d_self_obj, n, to_point, min_raydis, self_bias, d_out);
+    YG_LAUNCH_W(s->width, (k_is_shadowed<4>), (k_is_shadowed<2>), dim3(grid_for(n, TB)), dim3(TB),
+                stack_lds_bytes(s->stack_entries, TB), (hipStream_t)stream, s->view(), d_p, d_l, d_self_obj, n, to_point,
+                min_raydis, self_bias, d_out);
     YG_HIP(hipGetLastError());
     return YG_OK;
 }

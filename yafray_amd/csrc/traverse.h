@@ -55,6 +55,22 @@ __device__ __forceinline__ float4 gload(const float4* p)
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
+__device__ __forceinline__ GNode4 gload(const GNode4* p)
+{
+    const YG_GLOBAL yg_v4f* q = (const YG_GLOBAL yg_v4f*)p;
+    const yg_v4f a = q[0], b = q[1], c = q[2], d = q[3], e = q[4], f = q[5];
+    const yg_v4i ch = ((const YG_GLOBAL yg_v4i*)p)[6];
+    GNode4 n;
+    n.lox = make_float4(a.x, a.y, a.z, a.w);
+    n.hix = make_float4(b.x, b.y, b.z, b.w);
+    n.loy = make_float4(c.x, c.y, c.z, c.w);
+    n.hiy = make_float4(d.x, d.y, d.z, d.w);
+    n.loz = make_float4(e.x, e.y, e.z, e.w);
+    n.hiz = make_float4(f.x, f.y, f.z, f.w);
+    n.child = make_int4(ch.x, ch.y, ch.z, ch.w);
+    return n;
+}
+
 __device__ __forceinline__ GNode gload(const GNode* p)
 {
     const YG_GLOBAL yg_v4f* q = (const YG_GLOBAL yg_v4f*)p;
@@ -137,6 +153,90 @@ __device__ __forceinline__ void node_test(const GNode& n, const RayBox& r, float
     t1 = n1;
 }
 
+// Slab test of the four children of a width-4 node: t[k] = entry distance of
+// child k, or +inf when the ray misses it (also for the +inf empty slots,
+// whose widened test evaluates to NaN and fails).
+__device__ __forceinline__ void node_test4(const GNode4& n, const RayBox& r, float lo, float hi, float t[4])
+{
+    const float* lx = &n.lox.x; const float* hx = &n.hix.x;
+    const float* ly = &n.loy.x; const float* hy = &n.hiy.x;
+    const float* lz = &n.loz.x; const float* hz = &n.hiz.x;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float a0 = __builtin_fmaf(lx[k], r.ix, -r.ox), a1 = __builtin_fmaf(hx[k], r.ix, -r.ox);
+        const float a2 = __builtin_fmaf(ly[k], r.iy, -r.oy), a3 = __builtin_fmaf(hy[k], r.iy, -r.oy);
+        const float a4 = __builtin_fmaf(lz[k], r.iz, -r.oz), a5 = __builtin_fmaf(hz[k], r.iz, -r.oz);
+        const float nr = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), lo));
+        const float fr = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), hi));
+        const bool h = __builtin_fmaf(-fabsf(nr), YG_TEPS, nr) <= __builtin_fmaf(fabsf(fr), YG_TEPS, fr);
+        t[k] = h ? nr : INFINITY;
+    }
+}
+
+#define YG_POP 0x7fffffff  // "no child to descend into": pop the stack
+
+__device__ __forceinline__ void cswap(float& ta, int& ca, float& tb, int& cb)
+{
+    const bool s = tb < ta;
+    const float t = s ? tb : ta;
+    const int c = s ? cb : ca;
+    tb = s ? ta : tb;
+    cb = s ? ca : cb;
+    ta = t;
+    ca = c;
+}
+
+// One inner-node visit: the hit children nearest first, the nearest
+// returned, the others pushed far-to-near (so the next nearest pops first).
+__device__ __forceinline__ int visit2(const BVHView& bvh, int node, const RayBox& rb, float lo, float hi,
+                                      LaneStack& st)
+{
+    const GNode n = gload(bvh.nodes + node);
+    float t0, t1;
+    bool h0, h1;
+    node_test(n, rb, lo, hi, t0, t1, h0, h1);
+    if (h0 && h1) {
+        const bool first0 = t0 <= t1;
+        st.push(first0 ? n.d.y : n.d.x);
+        return first0 ? n.d.x : n.d.y;
+    }
+    if (h0) return n.d.x;
+    if (h1) return n.d.y;
+    return YG_POP;
+}
+
+__device__ __forceinline__ int visit4(const BVHView& bvh, int node, const RayBox& rb, float lo, float hi,
+                                      LaneStack& st)
+{
+    const GNode4 n = gload(bvh.nodes4 + node);
+    float t[4];
+    node_test4(n, rb, lo, hi, t);
+    float t0 = t[0], t1 = t[1], t2 = t[2], t3 = t[3];
+    int c0 = n.child.x, c1 = n.child.y, c2 = n.child.z, c3 = n.child.w;
+    cswap(t0, c0, t1, c1);
+    cswap(t2, c2, t3, c3);
+    cswap(t0, c0, t2, c2);
+    cswap(t1, c1, t3, c3);
+    cswap(t1, c1, t2, c2);
+    if (t0 == INFINITY) return YG_POP;
+    if (t3 != INFINITY) st.push(c3);
+    if (t2 != INFINITY) st.push(c2);
+    if (t1 != INFINITY) st.push(c1);
+    return c0;
+}
+
+// Kernels are instantiated per BVH width W (2 or 4); a launch traverses
+// trees of one width only.  Keeping the two node formats in separate kernels
+// keeps each traversal loop small (one binary with both loops ran a third
+// slower on either).
+template <int W>
+__device__ __forceinline__ int visit(const BVHView& bvh, int node, const RayBox& rb, float lo, float hi,
+                                     LaneStack& st)
+{
+    if constexpr (W == 4) return visit4(bvh, node, rb, lo, hi, st);
+    else return visit2(bvh, node, rb, lo, hi, st);
+}
+
 struct TraceStats {
     uint32_t nodes, tris;
 };
@@ -144,7 +244,7 @@ struct TraceStats {
 // Closest hit: accepts tmin <= t < tbest (kdtree.cc:690-694); exact ties go to
 // the lowest primitive id so the result does not depend on the tree.
 // Returns the leaf-ordered triangle index or -1; tbest holds the distance.
-template <bool COUNT>
+template <bool COUNT, int W>
 __device__ int trace_closest(const BVHView& bvh, f3 o, f3 d, float tmin, float& tbest,
                              LaneStack& st, TraceStats* cs = nullptr)
 {
@@ -157,20 +257,9 @@ __device__ int trace_closest(const BVHView& bvh, f3 o, f3 d, float tmin, float& 
     st.sp = 0;
     for (;;) {
         if (node >= 0) {
-            const GNode n = gload(bvh.nodes + node);
             if (COUNT) cs->nodes++;
-            float t0, t1;
-            bool h0, h1;
-            node_test(n, rb, lo, tcull, t0, t1, h0, h1);
-            if (h0 && h1) {
-                int nearc = (t0 <= t1) ? n.d.x : n.d.y;
-                int farc = (t0 <= t1) ? n.d.y : n.d.x;
-                st.push(farc);
-                node = nearc;
-                continue;
-            }
-            if (h0) { node = n.d.x; continue; }
-            if (h1) { node = n.d.y; continue; }
+            node = visit<W>(bvh, node, rb, lo, tcull, st);
+            if (node != YG_POP) continue;
         } else {
             const int code = ~node;
             const int first = code >> 4, cnt = (code & 15) + 1;
@@ -203,7 +292,7 @@ __device__ int trace_closest(const BVHView& bvh, f3 o, f3 d, float tmin, float& 
 // self_obj are tested from oself instead of o (scene_t::isShadowed's
 // self_bias rule, scene.cc:115-141); [blo, bhi] is the box interval in o's
 // parameter and must cover both origins' segments.
-template <bool COUNT>
+template <bool COUNT, int W>
 __device__ bool trace_any(const BVHView& bvh, f3 o, f3 d, float tmin, float tmax, int self_obj,
                           f3 oself, float blo, float bhi, LaneStack& st, TraceStats* cs = nullptr)
 {
@@ -213,20 +302,9 @@ __device__ bool trace_any(const BVHView& bvh, f3 o, f3 d, float tmin, float tmax
     st.sp = 0;
     for (;;) {
         if (node >= 0) {
-            const GNode n = gload(bvh.nodes + node);
             if (COUNT) cs->nodes++;
-            float t0, t1;
-            bool h0, h1;
-            node_test(n, rb, blo, hi, t0, t1, h0, h1);
-            if (h0 && h1) {
-                int nearc = (t0 <= t1) ? n.d.x : n.d.y;
-                int farc = (t0 <= t1) ? n.d.y : n.d.x;
-                st.push(farc);
-                node = nearc;
-                continue;
-            }
-            if (h0) { node = n.d.x; continue; }
-            if (h1) { node = n.d.y; continue; }
+            node = visit<W>(bvh, node, rb, blo, hi, st);
+            if (node != YG_POP) continue;
         } else {
             const int code = ~node;
             const int first = code >> 4, cnt = (code & 15) + 1;
@@ -251,17 +329,19 @@ __device__ bool trace_any(const BVHView& bvh, f3 o, f3 d, float tmin, float tmax
 
 // scene_t::firstHit (scene.cc:688-721): origin from + ray*min_raydis, closest
 // hit with Z > 0 (a zero-distance closest hit is rejected by the scene).
+template <int W>
 __device__ __forceinline__ int first_hit(const BVHView& bvh, f3 from, f3 ray, float min_raydis,
                                          float& t, LaneStack& st)
 {
     const f3 f = from + ray * min_raydis;
     t = INFINITY;
-    int ti = trace_closest<false>(bvh, f, ray, 0.0f, t, st);
+    int ti = trace_closest<false, W>(bvh, f, ray, 0.0f, t, st);
     if (ti >= 0 && !(t > 0.0f)) ti = -1;
     return ti;
 }
 
 // scene_t::isShadowed, point form (scene.cc:107-146) or direction form (:148-185)
+template <int W>
 __device__ __forceinline__ bool is_shadowed(const BVHView& bvh, f3 p, f3 l, int self_obj,
                                             bool to_point, float min_raydis, float self_bias,
                                             LaneStack& st)
@@ -282,5 +362,5 @@ __device__ __forceinline__ bool is_shadowed(const BVHView& bvh, f3 p, f3 l, int 
     const float slack = 1e-4f * (fabsf(self_bias) + fabsf(min_raydis)) + 1e-30f;
     float blo = fminf(0.0f, delta) - slack;
     float bhi = dist + fmaxf(0.0f, delta) + slack;
-    return trace_any<false>(bvh, q, ray, 0.0f, dist, self_obj, self, blo, bhi, st);
+    return trace_any<false, W>(bvh, q, ray, 0.0f, dist, self_obj, self, blo, bhi, st);
 }

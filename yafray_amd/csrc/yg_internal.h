@@ -21,26 +21,41 @@ struct alignas(64) GNode {
     int4 d;    // child0 child1 - -
 };
 
-#define YG_LEAF_MAX 16
-// Deepest tree the traversal accepts.  Each lane's stack lives in LDS and
-// holds at most one entry per tree level, so a launch reserves
-// (max_depth + 1) entries per lane (see stack_lds_bytes).
-#define YG_MAX_DEPTH 200
+// Four-wide BVH node, 128 bytes (one L2 line): the boxes of up to four
+// children as structure-of-arrays (component k = child k) and their
+// references, encoded as for GNode.  Unused child slots have +inf boxes,
+// which every slab test rejects.
+struct alignas(128) GNode4 {
+    float4 lox, hix, loy, hiy, loz, hiz;
+    int4 child;
+    int4 pad;
+};
 
-// Dynamic LDS bytes of a traversal launch with `lanes` threads per block over
-// trees at most `max_depth` deep.
+#define YG_LEAF_MAX 16
+#ifndef YG_DEFAULT_WIDTH
+#define YG_DEFAULT_WIDTH 4
+#endif
+// Deepest traversal stack the kernels accept.  Each lane's stack lives in
+// LDS; a launch reserves the deepest stack its trees can need (one entry per
+// level of a two-wide tree, up to three per level of a four-wide one).
+#define YG_MAX_STACK 200
+
+// Dynamic LDS bytes of a traversal launch with `lanes` threads per block and
+// `entries` stack entries per lane.
 #ifndef YG_STACK_EXTRA
 #define YG_STACK_EXTRA 0  // extra entries per lane (occupancy tuning builds)
 #endif
-static inline size_t stack_lds_bytes(int max_depth, int lanes)
+static inline size_t stack_lds_bytes(int entries, int lanes)
 {
-    return (size_t)(max_depth + 1 + YG_STACK_EXTRA) * (size_t)lanes * sizeof(int);
+    return (size_t)(entries + YG_STACK_EXTRA) * (size_t)lanes * sizeof(int);
 }
 
 // Leaf-ordered triangle record, 48 B: (a.xyz, n.x) (b.xyz, n.y) (c.xyz, n.z),
 // n = the unit face normal of triangle_t::recNormal (triangle.cc:45-49).
 struct BVHView {
-    const GNode* nodes;
+    const GNode* nodes;     // width 2
+    const GNode4* nodes4;   // width 4
+    int32_t width;
     const float4* tris;     // 3 per triangle
     const int32_t* tprim;   // original primitive id per leaf-ordered triangle
     const int32_t* tobj;    // object id per leaf-ordered triangle
@@ -51,6 +66,8 @@ struct BVHView {
 
 struct HostBVH {
     std::vector<GNode> nodes;
+    std::vector<GNode4> nodes4;  // the same tree collapsed to width 4 (when asked)
+    int32_t stack4 = 0;          // deepest traversal stack the width-4 tree can need
     std::vector<float4> tris;
     std::vector<int32_t> tprim, tobj, tmat;
     int32_t max_depth = 0;
@@ -59,7 +76,7 @@ struct HostBVH {
 // Builds the BVH over world-space triangles (binned SAH).  prim ids are
 // positions in idx/3.
 void build_bvh(const float* verts, const uint32_t* idx, int64_t nt, const int32_t* tri_obj,
-               const int32_t* tri_mat, int max_leaf, int bins, int threads, HostBVH& out);
+               const int32_t* tri_mat, int max_leaf, int bins, int threads, int width, HostBVH& out);
 
 // Face normal exactly as triangle_t::recNormal computes it.
 YG_HD f3 tri_normal(f3 a, f3 b, f3 c) { return norm3(cross(b - a, c - a)); }
@@ -86,8 +103,11 @@ struct yg_scene {
 
     bool built = false;
     int32_t max_depth = 0;
+    int32_t width = 2;
+    int32_t stack_entries = 1;  // traversal stack entries per lane the tree can need
     uint64_t n_nodes = 0;
     GNode* d_nodes = nullptr;
+    GNode4* d_nodes4 = nullptr;
     float4* d_tris = nullptr;
     int32_t* d_tprim = nullptr;
     int32_t* d_tobj = nullptr;
@@ -121,7 +141,7 @@ struct yg_scene {
     BVHView view() const
     {
         BVHView v;
-        v.nodes = d_nodes; v.tris = d_tris; v.tprim = d_tprim; v.tobj = d_tobj; v.tmat = d_tmat;
+        v.nodes = d_nodes; v.nodes4 = d_nodes4; v.width = width; v.tris = d_tris; v.tprim = d_tprim; v.tobj = d_tobj; v.tmat = d_tmat;
         v.ntris = (int32_t)tri_mat.size();
         v.nobj = nobj;
         return v;
@@ -153,3 +173,11 @@ static inline int dev_upload(T** dst, const T* src, size_t n, uint64_t* acct = n
 }
 
 static inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
+
+// Launches the instantiation of a traversal kernel for the BVH width in use
+// (K4 / K2: the kernel template-ids for widths 4 and 2, parenthesised).
+#define YG_LAUNCH_W(width, K4, K2, grid, block, shm, stream, ...)                  \
+    do {                                                                         \
+        if ((width) == 4) hipLaunchKernelGGL(K4, grid, block, shm, stream, __VA_ARGS__); \
+        else hipLaunchKernelGGL(K2, grid, block, shm, stream, __VA_ARGS__);       \
+    } while (0)

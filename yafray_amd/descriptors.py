@@ -72,7 +72,8 @@ class yg_render_stats(C.Structure):
 
 
 class yg_build_opts(C.Structure):
-    _fields_ = [("max_leaf", C.c_int32), ("sah_bins", C.c_int32), ("threads", C.c_int32)]
+    _fields_ = [("max_leaf", C.c_int32), ("sah_bins", C.c_int32), ("threads", C.c_int32),
+                ("width", C.c_int32)]
 
 
 class yg_photon_stats(C.Structure):

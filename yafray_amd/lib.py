@@ -124,8 +124,8 @@ class Scene:
                                 C.byref(oid)), "yg_mesh_add")
         return oid.value
 
-    def build(self, max_leaf=4, bins=32, threads=0):
-        o = D.yg_build_opts(max_leaf, bins, threads)
+    def build(self, max_leaf=4, bins=32, threads=0, width=0):
+        o = D.yg_build_opts(max_leaf, bins, threads, width)
         check(lib().yg_build(self.h, C.byref(o)), "yg_build")
 
     def info(self):
