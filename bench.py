@@ -91,6 +91,20 @@ def copy_probe():
     return {"bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 2}
 
 
+TRAFFIC_PROFILE = "profiles/gi_terrain_1280x720_hbm_traffic.md"
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE / WRITE_SIZE passes scaled by a 1 GiB copy), or None."""
+    try:
+        txt = open(os.path.join(ROOT, TRAFFIC_PROFILE)).read()
+        block = txt.split("```json", 1)[1].split("```", 1)[0]
+        return int(json.loads(block)["kernels"][kernel]["hbm_bytes_per_launch"])
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def _timed(fn, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -298,6 +312,7 @@ def main():
         k["gbs"] = k["bytes"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else 0.0
     dom = max(kern, key=lambda k: kern[k]["ms"])
     achieved = kern[dom]["gbs"]
+    traffic = measured_traffic(dom)
     line = {
         "metric": "GI Mrays/s",
         "value": round(value, 3),
@@ -327,7 +342,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": TRAFFIC_PROFILE if traffic is not None else None,
             "launches": levels,
             "ms_per_launch": round(kern[dom]["ms"] / levels, 4),
             "alg_bytes_per_launch": round(kern[dom]["bytes"] / levels),
@@ -350,7 +366,8 @@ def main():
             "cores": cores,
             "kind": "port",
             "sample": "oracle render of the full 1280x720 frame (4 spp, 3 bounces), "
-                      "%d rays in %.1f s" % (p["cpu_rays"], p["cpu_dt"]),
+                      "%d rays in %.1f s on %d threads (%.0f core-seconds)"
+                      % (p["cpu_rays"], p["cpu_dt"], cores, p["cpu_dt"] * cores),
         }
         line["rmse_vs_oracle"] = {"rel_rmse": p["rel_rmse"], "bit_exact_pixels": p["exact_px"],
                                   "sample": "full 1280x720 frame, GPU image vs oracle image"}

@@ -170,3 +170,33 @@ def test_photon_small_matches_oracle(torch_cuda):
     img = out.cpu().numpy().reshape(d.cam.resy, d.cam.resx, 3)
     ref, _ = oc.render(d)
     assert rel_rmse(img, ref) <= 1e-3
+
+
+@pytest.mark.parametrize("width", [2, 4])
+def test_batched_equals_single_renders(torch_cuda, width):
+    """yg_render_batch over scenes of uneven size and resolution: every image
+    equals that scene's own yg_render bit for bit, and one of them the oracle."""
+    torch = torch_cuda
+    Y = _Y()
+    specs = [(40, 20, 64, 48), (160, 80, 96, 64), (24, 12, 33, 17), (96, 48, 128, 72), (60, 30, 80, 80)]
+    scenes, outs, singles, descs, counts = [], [], [], [], []
+    for k, (nx, ny, rx, ry) in enumerate(specs):
+        d = S.terrain_gi(po.heightfield, nx=nx, ny=ny, resx=rx, resy=ry, samples=2, depth=3, seed_key=300 + k)
+        sc = Y.Scene.from_desc(d, width=width)
+        out = torch.zeros(rx * ry * 3, dtype=torch.float32, device="cuda")
+        s1 = sc.render(out)
+        counts.append((s1.rays_camera, s1.rays_extension, s1.rays_shadow))
+        singles.append(out.clone())
+        outs.append(torch.full_like(out, -1.0))
+        scenes.append(sc)
+        descs.append(d)
+    st = Y.render_batch(scenes, outs)
+    torch.cuda.synchronize()
+    for a, b in zip(outs, singles):
+        assert torch.equal(a, b)
+    assert (st.rays_camera, st.rays_extension, st.rays_shadow) == tuple(sum(c[i] for c in counts) for i in range(3))
+    d = descs[1]
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    ref, _ = oc.render(d, threads=8)
+    img = outs[1].cpu().numpy().reshape(ref.shape)
+    assert rel_rmse(img, ref) <= 1e-3
