@@ -38,10 +38,12 @@ def gpu_leg(args):
         sc.render(out, opts)
     torch.cuda.synchronize()
     rays = 0
+    modes = set()
     t_start = time.perf_counter()
     for _ in range(args.steps):
         st = sc.render(out, opts)
         rays += st.rays_camera + st.rays_extension + st.rays_shadow
+        modes.add(int(st.graph))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t_start
     # kernel-level timing of the GI bounce launches (HIP events on the render stream)
@@ -53,6 +55,7 @@ def gpu_leg(args):
     return {
         "desc": d, "img": img,
         "dt": dt, "rays": rays, "stats": ks, "info": info, "setup_s": setup_s,
+        "graph_modes": sorted(modes),
         "finite": bool(np.all(np.isfinite(img))), "mean": float(img.mean()),
     }
 
@@ -354,6 +357,8 @@ def main():
         "bvh": {"nodes": int(g["info"]["nodes"]), "max_depth": int(g["info"]["max_depth"]),
                 "dev_bytes": int(g["info"]["dev_bytes"])},
         "image_finite": g["finite"],
+        # 1 = frame replayed from its captured hipGraph, 2 = captured in that call, 0 = kernel by kernel
+        "frame_graph_modes": g["graph_modes"],
     }
     if args.copy_probe:
         line["copy_probe"] = copy_probe()

@@ -33,7 +33,7 @@ extern "C" {
 #define YG_ERR_UNSUPPORTED (-4)
 #define YG_ERR_NOMEM (-5)
 
-#define YG_ABI_VERSION 3
+#define YG_ABI_VERSION 4
 
 /* ---------------------------------------------------------------- rays -- */
 
@@ -175,6 +175,8 @@ typedef struct yg_render_opts {
 #define YG_RENDER_KTIMING 2  /* also time each GI bounce launch (ms_bounce) */
 #define YG_RENDER_REUSE 4    /* yg_render_adjoint: replay the last yg_render of this scene
                                 if nothing changed since, instead of rendering again */
+#define YG_RENDER_NO_GRAPH 8 /* launch every kernel of the frame from the host instead of
+                                replaying the captured frame graph (see yg_render) */
 
 typedef struct yg_render_stats {
     uint64_t rays_camera;
@@ -189,6 +191,8 @@ typedef struct yg_render_stats {
     uint64_t rays_bounce;     /* extension + shadow rays traced inside the bounce launches */
     float ms_ext;             /* sum of the extension-ray (closest hit) launches */
     float ms_shadow;          /* sum of the bounce shadow-ray (any hit) launches */
+    int32_t graph;            /* 1: the frame ran as a replay of its captured hipGraph,
+                                 2: it was captured by this call, 0: launched kernel by kernel */
 } yg_render_stats;
 
 typedef struct yg_build_opts {

@@ -200,3 +200,41 @@ def test_batched_equals_single_renders(torch_cuda, width):
     ref, _ = oc.render(d, threads=8)
     img = outs[1].cpu().numpy().reshape(ref.shape)
     assert rel_rmse(img, ref) <= 1e-3
+
+
+def test_frame_graph_replays_equal_eager(torch_cuda):
+    """Frames replayed from the captured hipGraph equal the frame launched
+    kernel by kernel, on every replay, with other device work and a render
+    of another launch shape in between."""
+    torch = torch_cuda
+    Y = _Y()
+    d = small_terrain(nx=256, ny=128, resx=320, resy=180, samples=2)
+    sc = Y.Scene.from_desc(d)
+    n = d.cam.resx * d.cam.resy * 3
+    eager = D.yg_render_opts.from_buffer_copy(d.opts)
+    eager.flags |= 8  # YG_RENDER_NO_GRAPH
+    ref = torch.zeros(n, dtype=torch.float32, device="cuda")
+    st0 = sc.render(ref, eager)
+    torch.cuda.synchronize()
+    assert st0.graph == 0
+    ref_h = ref.cpu()
+    small = small_terrain(nx=64, ny=32, resx=64, resy=48, samples=2)
+    sc2 = Y.Scene.from_desc(small)
+    out2 = torch.zeros(small.cam.resx * small.cam.resy * 3, dtype=torch.float32, device="cuda")
+    modes = []
+    for k in range(5):
+        if k == 3:  # another launch shape between two replays
+            sc2.render(out2)
+        out = torch.full((n,), -1.0, dtype=torch.float32, device="cuda")
+        st = sc.render(out)
+        torch.cuda.synchronize()
+        modes.append(int(st.graph))
+        assert torch.equal(out.cpu(), ref_h), "frame %d (graph mode %d) differs" % (k, st.graph)
+        for f in ("rays_camera", "rays_extension", "rays_shadow", "lcg_draws"):
+            assert getattr(st, f) == getattr(st0, f), f
+        x = torch.randn(1 << 20, device="cuda")  # eager work between replays
+        assert torch.isfinite(x.sum()).item()
+    # captured on the second consecutive frame of the shape (the eager one
+    # was the first), replayed from then on
+    assert modes[0] == 2 and modes[1] == 1 and modes[2] == 1, modes
+    assert all(m in (0, 1, 2) for m in modes[3:]), modes

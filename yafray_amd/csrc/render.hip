@@ -8,6 +8,7 @@
 // stream the area light draws from (arealight.cc:139) -- is turned into
 // indices by exclusive prefix sums over the work items and path segments, so
 // every path reads exactly the sample values the CPU renderer would.
+#include <array>
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -25,15 +26,17 @@ __constant__ uint64_t YG_LCG_POW[64];
 
 // ------------------------------------------------------------- helpers --
 
+// A launch's area array (nareas entries, ascending item_start) is followed in
+// the same buffer by its bucket index: entry b = the area holding item
+// b << YG_AREA_LUT_SHIFT.  The area of an item is then one index load plus a
+// forward scan over the few areas that start inside its bucket.
+#define YG_AREA_LUT_SHIFT 8
 __device__ __forceinline__ int find_area(const Area* areas, int nareas, uint32_t item)
 {
-    int lo = 0, hi = nareas - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (areas[mid].item_start <= item) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
+    const uint32_t* lut = reinterpret_cast<const uint32_t*>(areas + nareas);
+    int k = (int)lut[item >> YG_AREA_LUT_SHIFT];
+    while (k + 1 < nareas && areas[k + 1].item_start <= item) k++;
+    return k;
 }
 
 // camera_t::shootRay perspective (camera.cc:184-189)
@@ -68,10 +71,12 @@ template <int W>
 __global__ __launch_bounds__(RB) void k_primary(const RenderScene* __restrict__ scenes,
                                                 const Area* __restrict__ areas, int nareas,
                                                 uint32_t nitems, int32_t* __restrict__ item_ti,
-                                                float* __restrict__ item_t)
+                                                float* __restrict__ item_t, Counters* __restrict__ cnt)
 {
     extern __shared__ int s_stk[];
     const uint32_t item = blockIdx.x * RB + threadIdx.x;
+    // the frame's ray counters start here (kernels after this one add to them)
+    if (item == 0) *cnt = Counters{};
     if (item >= nitems) return;
     LaneStack st;
     st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = RB; st.sp = 0;
@@ -272,23 +277,26 @@ __device__ __forceinline__ void queue_push(bool want, uint32_t p, uint32_t* qcou
 template <bool ADJ, int W>  // ADJ: replay for the adjoint (contri/nseg of the forward are kept)
 __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WPE))) void k_ext(
     const RenderScene* __restrict__ scenes, const Area* __restrict__ areas, int nareas, int j,
-    const uint32_t* __restrict__ active, uint32_t nactive, uint32_t nactive_cap, int S, int D,
+    const uint32_t* __restrict__ active, const uint32_t* __restrict__ nact, uint32_t nactive_cap, int S, int D,
     const int32_t* __restrict__ item_ti, const float* __restrict__ item_t, const uint32_t* __restrict__ qoff,
     const uint32_t* __restrict__ hidx, const f3* __restrict__ st_where, const f3* __restrict__ st_hn,
     const int32_t* __restrict__ st_ti, const c3* __restrict__ st_col, float* __restrict__ st_s,
     c3* __restrict__ contri, uint8_t* __restrict__ nseg, f3* __restrict__ h_p, f3* __restrict__ h_ray,
     int32_t* __restrict__ h_ti, uint32_t* __restrict__ qcount, uint32_t* __restrict__ queue,
-    Counters* __restrict__ cnt)
+    uint32_t* __restrict__ flag, Counters* __restrict__ cnt)
 {
     extern __shared__ int s_stk[];
     const uint32_t idx = blockIdx.x * RB + threadIdx.x;
     uint32_t next = 0;
     bool hit = false;
     uint32_t p = 0;
-    if (idx < nactive) {
+    // *nact = this level's active paths (written on the device by k_path_init
+    // or k_compact); the grid covers every path
+    if (idx < *nact) {
         LaneStack st;
         st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = RB; st.sp = 0;
         p = active[idx];
+        flag[p] = 0;  // set again by k_shade if the path continues
         const uint32_t item = p / (uint32_t)S;
         const int smp = (int)(p % (uint32_t)S);
         const Area a = areas[find_area(areas, nareas, item)];
@@ -499,9 +507,11 @@ __global__ __launch_bounds__(SB) void k_shade(const RenderScene* __restrict__ sc
 // call index within its render area (hidx_level: npaths entries of this level).
 __global__ void k_compact(const Area* __restrict__ areas, int nareas, uint32_t npaths, int S,
                           const uint32_t* __restrict__ flag, const uint32_t* __restrict__ ex,
-                          uint32_t* __restrict__ active, uint32_t* __restrict__ hidx_level)
+                          uint32_t* __restrict__ active, uint32_t* __restrict__ hidx_level,
+                          uint32_t* __restrict__ nact_next)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == npaths - 1) *nact_next = ex[p] + flag[p];
     if (p >= npaths || !flag[p]) return;
     const uint32_t item = p / (uint32_t)S;
     const Area& a = areas[find_area(areas, nareas, item)];
@@ -510,12 +520,20 @@ __global__ void k_compact(const Area* __restrict__ areas, int nareas, uint32_t n
     hidx_level[p] = e - ex[(size_t)a.item_start * S];
 }
 
-// level-0 active list: every sample of every qualifying item
-__global__ void k_path_init(uint32_t nitems, int S, const uint32_t* __restrict__ qual,
-                            const uint32_t* __restrict__ qoff, uint32_t* __restrict__ active)
+// Level-0 active list (every sample of every qualifying item) and its
+// length nact[0]; continuation flags and per-level queue lengths start at 0.
+__global__ void k_path_init(uint32_t nitems, int S, int D, const uint32_t* __restrict__ qual,
+                            const uint32_t* __restrict__ qoff, uint32_t* __restrict__ active,
+                            uint32_t* __restrict__ flag, uint32_t* __restrict__ nact,
+                            uint32_t* __restrict__ qcnt)
 {
     const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= nitems || !qual[item]) return;
+    if (item == 0)
+        for (int j = 0; j < D; j++) qcnt[j] = 0;
+    if (item >= nitems) return;
+    if (item == nitems - 1) nact[0] = (qoff[item] + qual[item]) * (uint32_t)S;
+    for (int i = 0; i < S; i++) flag[(size_t)item * S + i] = 0;
+    if (!qual[item]) return;
     for (int i = 0; i < S; i++) active[(size_t)qoff[item] * S + i] = item * (uint32_t)S + i;
 }
 
@@ -692,10 +710,16 @@ static void area_setup(const yg_light& Lg, DevLight& L, std::vector<f3>& pts, st
     L.from = (((a + b) + c) + d) * 0.25f;
 }
 
-template <class T>
+// Counts render-buffer allocations: a changed value means some buffer may
+// have moved (a freed block can come back at the same address, so pointers
+// alone do not tell).
+static uint64_t g_alloc_serial = 0;
+
+template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t cap = 0;
+    uint64_t serial = 0;  // g_alloc_serial of this allocation
     int ensure(size_t n)
     {
         if (n <= cap && p) return YG_OK;
@@ -706,9 +730,31 @@ struct DevBuf {
         hipError_t e = hipMalloc((void**)&p, sizeof(T) * want);
         if (e != hipSuccess) return yg_fail(YG_ERR_NOMEM, std::string("render buffer: ") + hipGetErrorString(e));
         cap = want;
+        serial = ++g_alloc_serial;
         return YG_OK;
     }
 };
+
+// Host bytes last copied into one device buffer.
+struct UploadCache {
+    uint64_t serial = 0;
+    std::vector<uint8_t> bytes;
+};
+
+// Copies h into d unless d's allocation already holds exactly these bytes
+// from the previous call: a repeated frame uploads nothing.
+template <typename T>
+static int upload_cached(DevBuf<T>& d, const std::vector<T>& h, UploadCache& c, hipStream_t st)
+{
+    const size_t n = sizeof(T) * h.size();
+    if (n == 0) return YG_OK;
+    const uint8_t* b = reinterpret_cast<const uint8_t*>(h.data());
+    if (c.serial == d.serial && c.bytes.size() == n && memcmp(c.bytes.data(), b, n) == 0) return YG_OK;
+    YG_HIP(hipMemcpyAsync(d.p, h.data(), n, hipMemcpyHostToDevice, st));
+    c.bytes.assign(b, b + n);
+    c.serial = d.serial;
+    return YG_OK;
+}
 
 // Per-process render workspace (grow-only, reused across calls).
 struct RenderWS {
@@ -731,6 +777,7 @@ struct RenderWS {
     DevBuf<int32_t> h_ti;
     DevBuf<uint32_t> vis, queue;  // shadow visibility bits; queue of hit paths
     DevBuf<uint32_t> qcnt;        // queue length per level
+    DevBuf<uint32_t> nact;        // active paths per level
     DevBuf<Counters> cnt;
     DevBuf<uint8_t> scan_tmp;
     DevBuf<uint8_t> akey0, akey1;
@@ -743,10 +790,20 @@ struct RenderWS {
     bool any_path = false;
     size_t stk_bytes = 0;  // dynamic LDS of the traversal launches
     int width = 2;         // BVH width of the scenes in the launch
+    bool any_area = false, any_photon = false;
     // the single-scene render the workspace holds (for YG_RENDER_REUSE)
     uint64_t last_uid = 0;  // yg_scene::uid, 0 = none
     uint64_t last_gen = 0;
     yg_render_opts last_opts{};
+    // descriptor uploads of the last frame
+    UploadCache up_scenes, up_areas, up_lights, up_apts, up_ajh, up_ajv;
+    Counters* h_cnt = nullptr;  // pinned read-back of the frame's counters
+    // captured frame graph: replayed for frames of the same launch shape
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t frame_exec = nullptr;
+    std::array<uint64_t, 13> frame_key{}, seen_key{};
+    bool seen_valid = false;
+    bool graphs_off = false;  // a capture failed: launch kernel by kernel
 };
 
 static bool same_opts(const yg_render_opts& a, const yg_render_opts& b)
@@ -757,6 +814,14 @@ static bool same_opts(const yg_render_opts& a, const yg_render_opts& b)
 }
 
 static RenderWS g_ws;
+
+// temporary bytes of an exclusive scan of n elements
+static int scan_bytes(size_t n, size_t& bytes, hipStream_t st)
+{
+    bytes = 0;
+    YG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, st));
+    return YG_OK;
+}
 
 static int exclusive_scan(const uint32_t* in, uint32_t* out, size_t n, hipStream_t st)
 {
@@ -907,24 +972,26 @@ __global__ __launch_bounds__(64) void k_photon_direct(const RenderScene* __restr
 // The GI levels of the launch the workspace holds: per level the extension
 // rays, the shadow rays of their hits, the shading, then the stable
 // compaction that numbers the next level's Halton calls per render area.
+// Level lengths stay on the device (W.nact): every launch covers all paths
+// and its threads beyond the level's length return at once, so the host
+// never waits inside a frame.
 // ADJ replays the forward (same Halton indices, same hits) and accumulates
 // gradients in the shading kernel; the forward's contri/nseg stay untouched.
 template <bool ADJ>
-static int trace_levels(hipStream_t st, uint32_t nactive, bool ktiming, std::vector<hipEvent_t>& kev,
-                        const AdjAcc& G, const c3* w_item, unsigned long long* degenerate)
+static int trace_levels(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& kev, const AdjAcc& G,
+                        const c3* w_item, unsigned long long* degenerate)
 {
     RenderWS& W = g_ws;
     const int S = W.S, D = W.D, na = W.na;
     const uint32_t npaths = W.npaths, nitems = W.nitems;
     int rc;
-    hipLaunchKernelGGL(k_path_init, dim3(grid_for(nitems, 256)), dim3(256), 0, st, nitems, S, W.qual.p,
-                       W.qoff.p, W.active0.p);
+    hipLaunchKernelGGL(k_path_init, dim3(grid_for(nitems, 256)), dim3(256), 0, st, nitems, S, D, W.qual.p,
+                       W.qoff.p, W.active0.p, W.flag.p, W.nact.p, W.qcnt.p);
     YG_HIP(hipGetLastError());
     uint32_t* act = W.active0.p;
     uint32_t* nxt = W.active1.p;
-    YG_HIP(hipMemsetAsync(W.qcnt.p, 0, sizeof(uint32_t) * D, st));
-    for (int j = 0; j < D && nactive > 0; j++) {
-        YG_HIP(hipMemsetAsync(W.flag.p, 0, sizeof(uint32_t) * npaths, st));
+    const unsigned grid = grid_for(npaths, RB);
+    for (int j = 0; j < D; j++) {
         hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
         if (ktiming) {
             for (auto& e : ev) {
@@ -933,19 +1000,18 @@ static int trace_levels(hipStream_t st, uint32_t nactive, bool ktiming, std::vec
             }
             YG_HIP(hipEventRecord(ev[0], st));
         }
-        const unsigned grid = grid_for(nactive, RB);
         uint32_t* qc = W.qcnt.p + j;
-        YG_LAUNCH_W(W.width, (k_ext<ADJ, 4>), (k_ext<ADJ, 2>), dim3(grid), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na, j, act,
-                           nactive, npaths, S, D, W.item_ti.p, W.item_t.p, W.qoff.p, W.hidx.p, W.st_where.p,
-                           W.st_hn.p, W.st_ti.p, W.st_col.p, W.st_s.p, W.contri.p, W.nseg.p, W.h_p.p,
-                           W.h_ray.p, W.h_ti.p, qc, W.queue.p, W.cnt.p);
+        YG_LAUNCH_W(W.width, (k_ext<ADJ, 4>), (k_ext<ADJ, 2>), dim3(grid), dim3(RB), W.stk_bytes, st, W.scenes.p,
+                    W.areas.p, na, j, act, W.nact.p + j, npaths, S, D, W.item_ti.p, W.item_t.p, W.qoff.p,
+                    W.hidx.p, W.st_where.p, W.st_hn.p, W.st_ti.p, W.st_col.p, W.st_s.p, W.contri.p, W.nseg.p,
+                    W.h_p.p, W.h_ray.p, W.h_ti.p, qc, W.queue.p, W.flag.p, W.cnt.p);
         YG_HIP(hipGetLastError());
         if (ktiming) YG_HIP(hipEventRecord(ev[1], st));
-        YG_LAUNCH_W(W.width, (k_shadow<ADJ, 4>), (k_shadow<ADJ, 2>), dim3(grid), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na, S, qc,
-                           W.queue.p, W.h_p.p, W.h_ti.p, W.vis.p, W.cnt.p);
+        YG_LAUNCH_W(W.width, (k_shadow<ADJ, 4>), (k_shadow<ADJ, 2>), dim3(grid), dim3(RB), W.stk_bytes, st,
+                    W.scenes.p, W.areas.p, na, S, qc, W.queue.p, W.h_p.p, W.h_ti.p, W.vis.p, W.cnt.p);
         YG_HIP(hipGetLastError());
         if (ktiming) YG_HIP(hipEventRecord(ev[2], st));
-        const unsigned sgrid = ADJ ? std::min(grid_for(nactive, SB), ADJ_GRID) : grid_for(nactive, SB);
+        const unsigned sgrid = ADJ ? std::min(grid_for(npaths, SB), ADJ_GRID) : grid_for(npaths, SB);
         hipLaunchKernelGGL(k_shade<ADJ>, dim3(sgrid), dim3(SB), ADJ ? sizeof(AdjLds) : 0, st,
                            W.scenes.p, W.areas.p, na, j, S, D, qc, W.queue.p, W.h_p.p, W.h_ray.p, W.h_ti.p,
                            W.vis.p, W.st_where.p, W.st_hn.p, W.st_ti.p, W.st_col.p, W.st_s.p, W.contri.p,
@@ -955,21 +1021,84 @@ static int trace_levels(hipStream_t st, uint32_t nactive, bool ktiming, std::vec
         if (j + 1 >= D) break;
         if ((rc = exclusive_scan(W.flag.p, W.ex.p, npaths, st))) return rc;
         hipLaunchKernelGGL(k_compact, dim3(grid_for(npaths, 256)), dim3(256), 0, st, W.areas.p, na, npaths, S,
-                           W.flag.p, W.ex.p, nxt, W.hidx.p + (size_t)(j + 1) * npaths);
+                           W.flag.p, W.ex.p, nxt, W.hidx.p + (size_t)(j + 1) * npaths, W.nact.p + j + 1);
         YG_HIP(hipGetLastError());
-        uint32_t lf = 0, le = 0;
-        YG_HIP(hipMemcpyAsync(&lf, W.flag.p + npaths - 1, 4, hipMemcpyDeviceToHost, st));
-        YG_HIP(hipMemcpyAsync(&le, W.ex.p + npaths - 1, 4, hipMemcpyDeviceToHost, st));
-        YG_HIP(hipStreamSynchronize(st));
-        nactive = lf + le;
         std::swap(act, nxt);
     }
     return YG_OK;
 }
 
+// One frame of the launch the workspace holds, camera rays to pixel pass,
+// enqueued on st.  Nothing in it waits for the host or reads host memory,
+// so the same sequence can be captured as a graph.  before_final (if set) is
+// recorded ahead of the pixel pass.
+static int launch_frame(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& kev, hipEvent_t before_final)
+{
+    RenderWS& W = g_ws;
+    const uint32_t nitems = W.nitems;
+    const int na = W.na, nl_max = W.nl_max;
+    int rc;
+    YG_LAUNCH_W(W.width, (k_primary<4>), (k_primary<2>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes, st,
+                W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.cnt.p);
+    YG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_prep, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
+                       nitems, W.item_ti.p, W.item_t.p, W.draws.p, W.qual.p);
+    YG_HIP(hipGetLastError());
+    if (W.any_area && (rc = exclusive_scan(W.draws.p, W.lcg_off.p, nitems, st))) return rc;
+    YG_LAUNCH_W(W.width, (k_direct<4>), (k_direct<2>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes, st,
+                W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lcg_off.p, W.lightc.p, nl_max,
+                W.cnt.p);
+    YG_HIP(hipGetLastError());
+    if (W.any_photon) {
+        hipLaunchKernelGGL(k_photon_direct, dim3(grid_for(nitems, 64)), dim3(64), 0, st, W.scenes.p,
+                           W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lightc.p, nl_max);
+        YG_HIP(hipGetLastError());
+    }
+    if (W.any_path) {
+        if ((rc = exclusive_scan(W.qual.p, W.qoff.p, nitems, st))) return rc;
+        if ((rc = trace_levels<false>(st, ktiming, kev, AdjAcc{}, nullptr, nullptr))) return rc;
+    }
+    if (before_final) YG_HIP(hipEventRecord(before_final, st));
+    hipLaunchKernelGGL(k_finalize, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
+                       nitems, W.S, W.D, W.item_ti.p, W.item_t.p, W.qual.p, W.lightc.p, nl_max, W.contri.p,
+                       W.nseg.p, W.cnt.p);
+    YG_HIP(hipGetLastError());
+    return YG_OK;
+}
+
+// Captures launch_frame on a private stream into W.frame_exec for the launch
+// shape `key`.  A capture that fails leaves graphs off for the process: the
+// frames are then launched kernel by kernel, as with YG_RENDER_NO_GRAPH.
+static int capture_frame(const std::array<uint64_t, 13>& key)
+{
+    RenderWS& W = g_ws;
+    if (!W.cap_stream) YG_HIP(hipStreamCreateWithFlags(&W.cap_stream, hipStreamNonBlocking));
+    const uint64_t serial = g_alloc_serial;
+    YG_HIP(hipStreamBeginCapture(W.cap_stream, hipStreamCaptureModeThreadLocal));
+    std::vector<hipEvent_t> none;
+    const int rc = launch_frame(W.cap_stream, false, none, nullptr);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(W.cap_stream, &g);
+    hipGraphExec_t x = nullptr;
+    hipError_t ie = hipErrorUnknown;
+    // a buffer (re)allocated while capturing would leave stale pointers in the graph
+    if (rc == YG_OK && ce == hipSuccess && g && serial == g_alloc_serial)
+        ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+        (void)hipGetLastError();
+        W.graphs_off = true;
+        return YG_OK;
+    }
+    if (W.frame_exec) (void)hipGraphExecDestroy(W.frame_exec);
+    W.frame_exec = x;
+    W.frame_key = key;
+    return YG_OK;
+}
+
 // Launch the whole pipeline over a set of planned scenes (one launch per stage).
 static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render_stats* stats,
-                        bool timing, bool ktiming)
+                        bool timing, bool ktiming, bool use_graph)
 {
     int rc;
     g_ws.last_uid = 0;  // the workspace is about to hold this launch
@@ -1034,6 +1163,21 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     const uint64_t npaths64 = (uint64_t)nitems * S;
     if (npaths64 >= 0xffffffffull) return yg_fail(YG_ERR_UNSUPPORTED, "too many path samples in one launch");
     const uint32_t npaths = (uint32_t)npaths64;
+    // bucket index of the areas, stored behind them (see find_area)
+    const int na_real = (int)areas.size();
+    {
+        const uint32_t nb = (nitems >> YG_AREA_LUT_SHIFT) + 1;
+        std::vector<uint32_t> lut(nb);
+        size_t k = 0;
+        for (uint32_t b = 0; b < nb; b++) {
+            const uint64_t first = (uint64_t)b << YG_AREA_LUT_SHIFT;
+            while (k + 1 < areas.size() && areas[k + 1].item_start <= first) k++;
+            lut[b] = (uint32_t)k;
+        }
+        const size_t extra = (sizeof(uint32_t) * nb + sizeof(Area) - 1) / sizeof(Area);
+        areas.resize(areas.size() + extra);
+        memcpy(areas.data() + na_real, lut.data(), sizeof(uint32_t) * nb);
+    }
     if ((rc = W.scenes.ensure(scenes.size()))) return rc;
     if ((rc = W.areas.ensure(areas.size()))) return rc;
     if ((rc = W.item_ti.ensure(nitems))) return rc;
@@ -1063,17 +1207,12 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         if ((rc = W.vis.ensure(npaths))) return rc;
         if ((rc = W.queue.ensure(npaths))) return rc;
         if ((rc = W.qcnt.ensure(D))) return rc;
+        if ((rc = W.nact.ensure(D))) return rc;
     }
-    YG_HIP(hipMemcpyAsync(W.scenes.p, scenes.data(), sizeof(RenderScene) * scenes.size(), hipMemcpyHostToDevice, st));
-    YG_HIP(hipMemcpyAsync(W.areas.p, areas.data(), sizeof(Area) * areas.size(), hipMemcpyHostToDevice, st));
-    if (!lights.empty())
-        YG_HIP(hipMemcpyAsync(W.lights.p, lights.data(), sizeof(DevLight) * lights.size(), hipMemcpyHostToDevice, st));
-    if (!apts.empty()) {
-        YG_HIP(hipMemcpyAsync(W.apts.p, apts.data(), sizeof(f3) * apts.size(), hipMemcpyHostToDevice, st));
-        YG_HIP(hipMemcpyAsync(W.ajh.p, ajh.data(), sizeof(f3) * ajh.size(), hipMemcpyHostToDevice, st));
-        YG_HIP(hipMemcpyAsync(W.ajv.p, ajv.data(), sizeof(f3) * ajv.size(), hipMemcpyHostToDevice, st));
-    }
-    YG_HIP(hipMemsetAsync(W.cnt.p, 0, sizeof(Counters), st));
+    if ((rc = upload_cached(W.scenes, scenes, W.up_scenes, st)) || (rc = upload_cached(W.areas, areas, W.up_areas, st)) ||
+        (rc = upload_cached(W.lights, lights, W.up_lights, st)) || (rc = upload_cached(W.apts, apts, W.up_apts, st)) ||
+        (rc = upload_cached(W.ajh, ajh, W.up_ajh, st)) || (rc = upload_cached(W.ajv, ajv, W.up_ajv, st)))
+        return rc;
 
     int max_depth = 1;  // stack entries
     for (auto& P : plans) {
@@ -1087,9 +1226,11 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     W.npaths = npaths;
     W.S = S;
     W.D = D;
-    W.na = (int)areas.size();
+    W.na = na_real;
     W.nl_max = nl_max;
     W.any_path = any_path;
+    W.any_area = any_area;
+    W.any_photon = any_photon;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
     std::vector<hipEvent_t> kev;
     if (ktiming) timing = true;
@@ -1099,41 +1240,41 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         YG_HIP(hipEventCreate(&e2));
         YG_HIP(hipEventRecord(e0, st));
     }
-    const int na = (int)areas.size();
-    YG_LAUNCH_W(W.width, (k_primary<4>), (k_primary<2>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na,
-                       nitems, W.item_ti.p, W.item_t.p);
-    YG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_prep, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
-                       nitems, W.item_ti.p, W.item_t.p, W.draws.p, W.qual.p);
-    YG_HIP(hipGetLastError());
-    if (any_area && (rc = exclusive_scan(W.draws.p, W.lcg_off.p, nitems, st))) return rc;
-    YG_LAUNCH_W(W.width, (k_direct<4>), (k_direct<2>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na,
-                       nitems, W.item_ti.p, W.item_t.p, W.lcg_off.p, W.lightc.p, nl_max, W.cnt.p);
-    YG_HIP(hipGetLastError());
-    if (any_photon) {
-        hipLaunchKernelGGL(k_photon_direct, dim3(grid_for(nitems, 64)), dim3(64), 0, st, W.scenes.p,
-                           W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lightc.p, nl_max);
-        YG_HIP(hipGetLastError());
+    // every buffer of the frame is allocated before its shape is taken
+    {
+        size_t b1 = 0, b2 = 0;
+        if ((rc = scan_bytes(nitems, b1, st)) || (rc = scan_bytes(any_path ? npaths : 0, b2, st)) ||
+            (rc = W.scan_tmp.ensure(std::max(b1, b2))))
+            return rc;
     }
-    if (any_path) {
-        if ((rc = exclusive_scan(W.qual.p, W.qoff.p, nitems, st))) return rc;
-        uint32_t last_q = 0, last_off = 0;
-        YG_HIP(hipMemcpyAsync(&last_q, W.qual.p + nitems - 1, 4, hipMemcpyDeviceToHost, st));
-        YG_HIP(hipMemcpyAsync(&last_off, W.qoff.p + nitems - 1, 4, hipMemcpyDeviceToHost, st));
-        YG_HIP(hipStreamSynchronize(st));
-        const uint32_t nactive = (last_q + last_off) * (uint32_t)S;
-        if ((rc = trace_levels<false>(st, nactive, ktiming, kev, AdjAcc{}, nullptr, nullptr))) return rc;
+    // The launch shape: a graph captured for it replays any frame of the same
+    // shape (scenes, areas, lights and output pointers are read from the
+    // uploaded descriptors, not baked into the launches).
+    const std::array<uint64_t, 13> key = {g_alloc_serial, nitems, npaths, (uint64_t)S, (uint64_t)D,
+                                          (uint64_t)W.na, (uint64_t)nl_max, (uint64_t)W.width, W.stk_bytes,
+                                          W.scan_tmp.cap, any_area, any_photon, any_path};
+    int graph = 0;
+    if (use_graph && !timing && !W.graphs_off) {
+        // capture on the second consecutive frame of a shape, replay from then on
+        bool captured = false;
+        if (!(W.frame_exec && W.frame_key == key) && W.seen_valid && W.seen_key == key) {
+            if ((rc = capture_frame(key))) return rc;
+            captured = true;
+        }
+        if (W.frame_exec && W.frame_key == key) {
+            graph = captured ? 2 : 1;
+            YG_HIP(hipGraphLaunch(W.frame_exec, st));
+        }
     }
-    if (timing) YG_HIP(hipEventRecord(e1, st));
-    hipLaunchKernelGGL(k_finalize, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
-                       nitems, S, D, W.item_ti.p, W.item_t.p, W.qual.p, W.lightc.p, nl_max, W.contri.p,
-                       W.nseg.p, W.cnt.p);
-    YG_HIP(hipGetLastError());
+    W.seen_key = key;
+    W.seen_valid = true;
+    if (!graph && (rc = launch_frame(st, ktiming, kev, timing ? e1 : nullptr))) return rc;
     if (stats) {
-        Counters c;
-        YG_HIP(hipMemcpyAsync(&c, W.cnt.p, sizeof(c), hipMemcpyDeviceToHost, st));
+        if (!W.h_cnt) YG_HIP(hipHostMalloc((void**)&W.h_cnt, sizeof(Counters), hipHostMallocDefault));
+        YG_HIP(hipMemcpyAsync(W.h_cnt, W.cnt.p, sizeof(Counters), hipMemcpyDeviceToHost, st));
         if (timing) YG_HIP(hipEventRecord(e2, st));
         YG_HIP(hipStreamSynchronize(st));
+        const Counters c = *W.h_cnt;
         memset(stats, 0, sizeof(*stats));
         stats->rays_camera = nitems;
         stats->rays_extension = c.ext;
@@ -1141,6 +1282,7 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         stats->adaptive_extra = c.extra;
         stats->lcg_draws = c.draws;
         stats->rays_bounce = c.ext + c.bshadow;
+        stats->graph = graph;
         if (timing) {
             float ms = 0, ms2 = 0;
             YG_HIP(hipEventElapsedTime(&ms, e0, e2));
@@ -1175,8 +1317,8 @@ extern "C" int yg_render(yg_scene* s, const yg_render_opts* opts, float* d_rgb, 
     int rc = plan_scene(s, opts, d_rgb, plans[0]);
     if (rc) return rc;
     g_ws.last_uid = 0;
-    rc = render_plans(plans, (hipStream_t)stream, stats, opts && (opts->flags & YG_RENDER_TIMING),
-                      opts && (opts->flags & YG_RENDER_KTIMING));
+    rc = render_plans(plans, (hipStream_t)stream, stats, opts->flags & YG_RENDER_TIMING,
+                      opts->flags & YG_RENDER_KTIMING, !(opts->flags & YG_RENDER_NO_GRAPH));
     if (rc == YG_OK) {
         g_ws.last_uid = s->uid;
         g_ws.last_gen = s->gen;
@@ -1196,12 +1338,13 @@ extern "C" int yg_render_batch(yg_scene** scenes, uint32_t n, const yg_render_op
         int rc = plan_scene(scenes[k], &opts[k], d_rgb[k], plans[k]);
         if (rc) return rc;
     }
-    bool timing = false, ktiming = false;
+    bool timing = false, ktiming = false, graph = true;
     for (uint32_t k = 0; k < n; k++) {
         timing = timing || (opts[k].flags & YG_RENDER_TIMING);
         ktiming = ktiming || (opts[k].flags & YG_RENDER_KTIMING);
+        graph = graph && !(opts[k].flags & YG_RENDER_NO_GRAPH);
     }
-    return render_plans(plans, (hipStream_t)stream, stats, timing, ktiming);
+    return render_plans(plans, (hipStream_t)stream, stats, timing, ktiming, graph);
 }
 
 // photonLight_t::illuminate at every primary hit (photonlight.cc:297-395)
@@ -1474,7 +1617,7 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
                        same_opts(W.last_opts, *opts);
     if (!reuse) {
         W.last_uid = 0;
-        if ((rc = render_plans(plans, st, stats, timing, false))) return rc;
+        if ((rc = render_plans(plans, st, stats, timing, false, !(opts->flags & YG_RENDER_NO_GRAPH)))) return rc;
         W.last_uid = s->uid;
         W.last_gen = s->gen;
         W.last_opts = *opts;
@@ -1515,7 +1658,7 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
                            W.qual.p, W.contri.p, W.nseg.p, W.w_item.p);
         YG_HIP(hipGetLastError());
         std::vector<hipEvent_t> none;
-        if (n0 > 0 && (rc = trace_levels<true>(st, n0, false, none, G, W.w_item.p, W.adeg.p))) return rc;
+        if (n0 > 0 && (rc = trace_levels<true>(st, false, none, G, W.w_item.p, W.adeg.p))) return rc;
         unsigned long long ndeg = 0;
         YG_HIP(hipMemcpyAsync(&ndeg, W.adeg.p, sizeof(ndeg), hipMemcpyDeviceToHost, st));
         YG_HIP(hipStreamSynchronize(st));
@@ -1533,8 +1676,8 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
         YG_HIP(hipGetLastError());
     }
     if (fused && W.any_path) {
-        hipLaunchKernelGGL(k_path_init, dim3(grid_for(W.nitems, 256)), dim3(256), 0, st, W.nitems, W.S, W.qual.p,
-                           W.qoff.p, W.active0.p);
+        hipLaunchKernelGGL(k_path_init, dim3(grid_for(W.nitems, 256)), dim3(256), 0, st, W.nitems, W.S, W.D,
+                           W.qual.p, W.qoff.p, W.active0.p, W.flag.p, W.nact.p, W.qcnt.p);
         YG_HIP(hipGetLastError());
         if (n0 > 0) {
             // Replay order: longest paths first (stable, so pixel order is kept

@@ -68,7 +68,8 @@ class yg_render_stats(C.Structure):
                 ("rays_shadow", C.c_uint64), ("adaptive_extra", C.c_uint64),
                 ("lcg_draws", C.c_uint64), ("ms", C.c_float), ("ms_trace", C.c_float),
                 ("ms_bounce", C.c_float), ("bounce_launches", C.c_uint32),
-                ("rays_bounce", C.c_uint64), ("ms_ext", C.c_float), ("ms_shadow", C.c_float)]
+                ("rays_bounce", C.c_uint64), ("ms_ext", C.c_float), ("ms_shadow", C.c_float),
+                ("graph", C.c_int32)]
 
 
 class yg_build_opts(C.Structure):
