@@ -272,13 +272,46 @@ def bench_direct(args):
     }
 
 
+def bench_autotune(args):
+    """Autotuner on the GI terrain frame (1280x720, 4 spp, 3 bounces): every
+    (BVH width, leaf size) candidate built and timed on the frame."""
+    import torch
+
+    from yafray_amd import lib as Y
+    from yafray_amd import scenes as S
+
+    d = S.terrain_gi(Y.device_heights)
+    sc = Y.Scene.from_desc(d)
+    out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+    default_dt, _ = _timed(lambda: sc.render(out), args.steps, args.warmup)
+    t0 = time.perf_counter()
+    best, best_ms, cands = sc.autotune(out, widths=(2, 4), leaves=(2, 4, 8), repeats=2)
+    tune_s = time.perf_counter() - t0
+    tuned_dt, _ = _timed(lambda: sc.render(out), args.steps, args.warmup)
+    return {
+        "metric": "autotuned GI ms/frame", "value": round(tuned_dt * 1e3, 3), "unit": "ms/frame",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(tuned_dt * 1e3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "path-traced GI on the 1,045,506-triangle fBm terrain, 1280x720, 4 spp, "
+                               "3 bounces; BVH width 2/4 x max_leaf 2/4/8, 32 SAH bins"},
+        "default_build_ms_per_frame": round(default_dt * 1e3, 3),
+        "best": {"width": best.width, "max_leaf": best.max_leaf, "sah_bins": best.sah_bins,
+                 "frame_device_ms": round(best_ms, 3)},
+        "candidates": [{"width": w, "max_leaf": lf, "sah_bins": b, "frame_device_ms": round(ms, 3),
+                        "build_ms": round(bms, 1)} for (w, lf, b, ms, bms) in cands],
+        "tune_s": round(tune_s, 2),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="gi", choices=["gi", "batched", "adjoint", "photon", "direct"])
+    ap.add_argument("--workload", default="gi",
+                    choices=["gi", "batched", "adjoint", "photon", "direct", "autotune"])
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy after the timed region (counter calibration)")
@@ -291,7 +324,7 @@ def main():
         import torch
         torch.cuda.set_device(0)
         fn = {"batched": bench_batched, "adjoint": bench_adjoint, "photon": bench_photon,
-              "direct": bench_direct}[args.workload]
+              "direct": bench_direct, "autotune": bench_autotune}[args.workload]
         print(json.dumps(fn(args)))
         return
 

@@ -307,6 +307,36 @@ int yg_render(yg_scene* s, const yg_render_opts* opts, float* d_rgb,
 int yg_render_batch(yg_scene** scenes, uint32_t n, const yg_render_opts* opts, float** d_rgb,
                     yg_render_stats* stats, void* stream);
 
+/* --------------------------------------------------------- autotuner -- */
+
+typedef struct yg_tune_opts {
+    int32_t widths;   /* BVH node widths to try, as bits: 1<<2 (two-wide), 1<<4 (four-wide); 0 = both */
+    int32_t leaves;   /* max_leaf values to try, as bits 1<<n, n in 1..8; 0 = 2, 4 and 8 */
+    int32_t bins;     /* SAH bin counts to try, as bits 1<<k for 2^k bins, k in 3..6; 0 = 32 only */
+    int32_t repeats;  /* timed renders per candidate after one untimed render; 0 = 2 */
+    int32_t threads;  /* host build threads; 0 = all */
+} yg_tune_opts;
+
+#define YG_TUNE_MAX 32
+typedef struct yg_tune_result {
+    yg_build_opts best;               /* the build the scene holds on return */
+    float best_ms;                    /* its frame time */
+    int32_t ncand;                    /* candidates tried, in the order below */
+    yg_build_opts cand[YG_TUNE_MAX];
+    float cand_ms[YG_TUNE_MAX];       /* fastest frame of each candidate (device time) */
+    float cand_build_ms[YG_TUNE_MAX]; /* build + upload of each candidate (host time) */
+} yg_tune_result;
+
+/* Autotuner (no reference counterpart: the reference's kd-tree has fixed
+ * build constants, kdtree.cc:57-138).  Builds the scene with every candidate
+ * (node width x leaf size x SAH bins), times yg_render of the caller's frame
+ * (opts, d_rgb: the image of the last render is left there) and leaves the
+ * scene built with the fastest.  Images do not depend on the build (exact
+ * distance ties go to the lowest primitive id).  At most YG_TUNE_MAX
+ * candidates. */
+int yg_autotune(yg_scene* s, const yg_render_opts* opts, float* d_rgb, const yg_tune_opts* t,
+                yg_tune_result* r, void* stream);
+
 /* Gradient of a loss through the image (no reference counterpart: the
  * reference has no derivatives).  Renders the image into d_rgb, then replays
  * every pathlight path with the same Halton samples and visibility held

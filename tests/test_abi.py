@@ -48,3 +48,52 @@ def test_error_path_without_gpu():
 def test_oracle_loads():
     from oracle import pyoracle as po
     assert po.lib().yo_next_prime(7) == 11
+
+
+STRUCTS = ["yg_ray", "yg_hit", "yg_camera", "yg_texture", "yg_modulator", "yg_material", "yg_light",
+           "yg_render_opts", "yg_render_stats", "yg_build_opts", "yg_tune_opts", "yg_tune_result",
+           "yg_photon_stats"]
+
+
+def test_ctypes_layouts_match_header(tmp_path):
+    """Every ABI struct has the same size and field offsets in the ctypes
+    mirror as a C compiler gives it from include/yafgpu.h."""
+    import subprocess
+    from yafray_amd import descriptors as D
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "yafgpu.h"', 'int main(void) {']
+    for s in STRUCTS:
+        lines.append('printf("%s sizeof() %%zu\\n", sizeof(%s));' % (s, s))
+        for f, _ in getattr(D, s)._fields_:
+            # (a trailing underscore marks a C field named by a Python keyword)
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (s, f, s, f.rstrip("_")))
+    lines.append('return 0; }')
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True, capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for s in STRUCTS:
+        cls = getattr(D, s)
+        assert got[(s, "sizeof()")] == ctypes.sizeof(cls), s
+        for f, _ in cls._fields_:
+            assert got[(s, f)] == getattr(cls, f).offset, (s, f)
+
+
+def test_autotune_argument_errors_without_gpu():
+    from yafray_amd import descriptors as D
+    from yafray_amd import lib as Y
+    L = Y.lib()
+    r = D.yg_tune_result()
+    assert L.yg_autotune(None, None, None, None, ctypes.byref(r), None) == -1
+    h = ctypes.c_void_p()
+    assert L.yg_scene_create(ctypes.byref(h)) == 0
+    o = D.yg_render_opts()
+    dummy = ctypes.c_void_p(16)  # never dereferenced: the masks are checked first
+    for widths, leaves, bins, msg in ((1 << 3, 0, 0, "widths"), (0, 1 << 9, 0, "leaves"),
+                                      (0, 0, 1 << 7, "bins")):
+        t = D.yg_tune_opts(widths=widths, leaves=leaves, bins=bins)
+        assert L.yg_autotune(h, ctypes.byref(o), dummy, ctypes.byref(t), ctypes.byref(r), None) == -1
+        assert msg in Y.last_error()
+    assert L.yg_scene_destroy(h) == 0

@@ -238,3 +238,31 @@ def test_frame_graph_replays_equal_eager(torch_cuda):
     # was the first), replayed from then on
     assert modes[0] == 2 and modes[1] == 1 and modes[2] == 1, modes
     assert all(m in (0, 1, 2) for m in modes[3:]), modes
+
+
+def test_autotune_keeps_fastest_build(torch_cuda):
+    """yg_autotune times every (width, leaf) candidate on the frame, leaves the
+    fastest build in place, and the image does not depend on the build."""
+    torch = torch_cuda
+    Y = _Y()
+    d = small_terrain(nx=256, ny=128, resx=160, resy=90, samples=2)
+    sc = Y.Scene.from_desc(d)
+    n = d.cam.resx * d.cam.resy * 3
+    ref = torch.zeros(n, dtype=torch.float32, device="cuda")
+    sc.render(ref)
+    best, best_ms, cands = sc.autotune(ref, widths=(2, 4), leaves=(2, 4), repeats=2)
+    assert len(cands) == 4
+    assert sorted((c[0], c[1]) for c in cands) == [(2, 2), (2, 4), (4, 2), (4, 4)]
+    assert all(c[3] > 0 and c[4] > 0 for c in cands)
+    assert best_ms == min(c[3] for c in cands)
+    assert (best.width, best.max_leaf) == next((c[0], c[1]) for c in cands if c[3] == best_ms)
+    # the scene holds the chosen build: same tree as building it afresh
+    other = Y.Scene.from_desc(d, max_leaf=best.max_leaf, bins=best.sah_bins, width=best.width)
+    assert sc.info()["nodes"] == other.info()["nodes"]
+    out = torch.zeros(n, dtype=torch.float32, device="cuda")
+    sc.render(out)
+    torch.cuda.synchronize()
+    a, b = out.cpu().numpy(), ref.cpu().numpy()
+    assert np.mean(a == b) >= 0.9999
+    ref_img, _ = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat).render(d, threads=8)
+    assert rel_rmse(a.reshape(ref_img.shape), ref_img) <= 1e-3

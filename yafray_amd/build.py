@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libyafgpu.so")
 
-SOURCES = ["capi.cpp", "qmc.cpp", "bvh_build.cpp", "trace.hip", "render.hip", "photon.hip",
+SOURCES = ["capi.cpp", "qmc.cpp", "bvh_build.cpp", "autotune.cpp", "trace.hip", "render.hip", "photon.hip",
            "../host/yafgpu_host.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

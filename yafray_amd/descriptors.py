@@ -77,6 +77,20 @@ class yg_build_opts(C.Structure):
                 ("width", C.c_int32)]
 
 
+YG_TUNE_MAX = 32
+
+
+class yg_tune_opts(C.Structure):
+    _fields_ = [("widths", C.c_int32), ("leaves", C.c_int32), ("bins", C.c_int32),
+                ("repeats", C.c_int32), ("threads", C.c_int32)]
+
+
+class yg_tune_result(C.Structure):
+    _fields_ = [("best", yg_build_opts), ("best_ms", C.c_float), ("ncand", C.c_int32),
+                ("cand", yg_build_opts * YG_TUNE_MAX), ("cand_ms", C.c_float * YG_TUNE_MAX),
+                ("cand_build_ms", C.c_float * YG_TUNE_MAX)]
+
+
 class yg_photon_stats(C.Structure):
     _fields_ = [("emitted", C.c_uint64), ("stored", C.c_uint64), ("clustered", C.c_uint64),
                 ("ms", C.c_float)]

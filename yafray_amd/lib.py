@@ -60,6 +60,7 @@ def lib():
     }
     sigs["yg_render_batch"] = (C.c_int, [C.POINTER(vp), u32, vp, C.POINTER(vp), vp, vp])
     sigs["yg_render_adjoint"] = (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp])
+    sigs["yg_autotune"] = (C.c_int, [vp, vp, vp, vp, vp, vp])
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
         fn.restype = res
@@ -201,6 +202,27 @@ class Scene:
         check(lib().yg_render(self.h, C.byref(o), _ptr(out_rgb), C.byref(st), _stream()),
               "yg_render")
         return st
+
+    def autotune(self, out_rgb, opts=None, widths=(2, 4), leaves=(2, 4, 8), bins=(32,), repeats=2,
+                 threads=0):
+        """Rebuild with every (width, max_leaf, bins) candidate, time the frame
+        of `opts` into out_rgb, keep the fastest build.  Returns
+        (best yg_build_opts, best ms, [(width, leaf, bins, ms, build_ms), ...])."""
+        o = opts if opts is not None else self.desc.opts
+        t = D.yg_tune_opts()
+        t.widths = sum(1 << w for w in widths)
+        t.leaves = sum(1 << n for n in leaves)
+        t.bins = sum(1 << (int(b).bit_length() - 1) for b in bins)
+        if any(b & (b - 1) for b in bins):
+            raise ValueError("SAH bin counts must be powers of two")
+        t.repeats = repeats
+        t.threads = threads
+        r = D.yg_tune_result()
+        check(lib().yg_autotune(self.h, C.byref(o), _ptr(out_rgb), C.byref(t), C.byref(r), _stream()),
+              "yg_autotune")
+        cands = [(r.cand[k].width, r.cand[k].max_leaf, r.cand[k].sah_bins, r.cand_ms[k], r.cand_build_ms[k])
+                 for k in range(r.ncand)]
+        return r.best, r.best_ms, cands
 
     def render_adjoint(self, grad_rgb, out_rgb, gmat, gmod, glight, opts=None):
         """Forward render into out_rgb and dLoss/dparams from dLoss/dpixel."""
