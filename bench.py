@@ -304,6 +304,37 @@ def bench_autotune(args):
     }
 
 
+def bench_aa(args):
+    """Adaptive antialiasing on the GI terrain frame: 1280x720, 4 spp, 3 bounces,
+    AA_passes 2, AA_minsamples 4, AA_threshold 0.03, jittered first pass."""
+    import torch
+
+    from yafray_amd import descriptors as D
+    from yafray_amd import lib as Y
+    from yafray_amd import scenes as S
+
+    d = S.terrain_gi(Y.device_heights)
+    d.opts = D.render_opts(bias=2e-3, raydepth=3, background=(0.4, 0.5, 0.7), aa_passes=2, aa_minsamples=4,
+                           aa_threshold=0.03, aa_jitterfirst=True)
+    sc = Y.Scene.from_desc(d)
+    out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+    dt, st = _timed(lambda: sc.render(out), args.steps, args.warmup)
+    rays = st.rays_camera + st.rays_extension + st.rays_shadow
+    plain = D.yg_render_opts.from_buffer_copy(d.opts)
+    plain.aa_passes = 0
+    n_first = sc.render(out, plain).rays_camera  # first-pass camera rays (area overlaps included)
+    return {
+        "metric": "antialiased GI Mrays/s", "value": round(rays / dt / 1e6, 2), "unit": "Mrays/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "path-traced GI on the 1,045,506-triangle fBm terrain, 1280x720, 4 spp, "
+                               "3 bounces, AA_passes 2, AA_minsamples 4, AA_threshold 0.03, AA_jitterfirst"},
+        "rays_per_frame": int(rays), "camera_rays": int(st.rays_camera),
+        "aa_samples_per_first_pass_ray": round((st.rays_camera - n_first) / n_first, 4),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,7 +342,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="gi",
-                    choices=["gi", "batched", "adjoint", "photon", "direct", "autotune"])
+                    choices=["gi", "batched", "adjoint", "photon", "direct", "autotune", "aa"])
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy after the timed region (counter calibration)")
@@ -324,7 +355,7 @@ def main():
         import torch
         torch.cuda.set_device(0)
         fn = {"batched": bench_batched, "adjoint": bench_adjoint, "photon": bench_photon,
-              "direct": bench_direct, "autotune": bench_autotune}[args.workload]
+              "direct": bench_direct, "autotune": bench_autotune, "aa": bench_aa}[args.workload]
         print(json.dumps(fn(args)))
         return
 

@@ -33,7 +33,7 @@ extern "C" {
 #define YG_ERR_UNSUPPORTED (-4)
 #define YG_ERR_NOMEM (-5)
 
-#define YG_ABI_VERSION 4
+#define YG_ABI_VERSION 5
 
 /* ---------------------------------------------------------------- rays -- */
 
@@ -169,6 +169,13 @@ typedef struct yg_render_opts {
     uint32_t rand_seed;      /* srand() seed before the block shuffle (glibc default 1) */
     uint32_t lcg_seed;       /* ourRandom() state at render start (vector3d.cc:160: 123212) */
     int32_t flags;           /* YG_RENDER_* */
+    /* adaptive antialiasing of scene_t::render(renderArea_t&) (scene.cc:740-816),
+     * with the ranges interface_impl.cc:501-533 enforces; aa_passes = 0: off */
+    int32_t aa_passes;       /* AA_passes */
+    int32_t aa_minsamples;   /* AA_minsamples, >= 1 when aa_passes > 0 */
+    float aa_pixelwidth;     /* AA_pixelwidth, 1..2 */
+    float aa_threshold;      /* AA_threshold */
+    int32_t aa_jitterfirst;  /* AA_jitterfirst */
 } yg_render_opts;
 
 #define YG_RENDER_TIMING 1   /* fill yg_render_stats.ms with device time */
@@ -292,8 +299,11 @@ int yg_set_shading(yg_scene* s, const yg_material* mats, uint32_t nmat,
                    const yg_texture* texs, uint32_t ntex);
 int yg_set_lights(yg_scene* s, const yg_light* lights, uint32_t n);
 
-/* scene_t::render(colorOutput_t&) in mono mode (scene.cc:538-600) with
- * AA_passes = 0: d_rgb receives resx*resy*3 floats (row-major, top row first).
+/* scene_t::render(colorOutput_t&) in mono mode (scene.cc:538-600), each area
+ * rendered as scene_t::render(renderArea_t&) (scene.cc:724-816) including the
+ * antialiasing passes of opts->aa_* (not with area lights, whose ourRandom()
+ * stream would have to run through each area's passes in turn): d_rgb
+ * receives resx*resy*3 floats (row-major, top row first).
  * Lights supported: sun, area (psamples 0), path (use_QMC on, cache off),
  * photon (diffuse mode; yg_photon_build must have run). */
 int yg_render(yg_scene* s, const yg_render_opts* opts, float* d_rgb,

@@ -72,6 +72,11 @@ def lib():
         L.yo_render_mt.restype = C.c_int
         L.yo_render_mt.argtypes = [vp, vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int,
                                    vp, C.c_int, vp, F32P, U64P, C.c_int]
+        L.yo_trace_camera.restype = C.c_int
+        L.yo_trace_camera.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp, C.c_int, vp, C.c_int, vp,
+                                      F32P, F32P, C.c_int64, F32P]
+        L.yo_block_regions_into.restype = C.c_int
+        L.yo_block_regions_into.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, I32P, C.c_int]
         L.yo_render_adjoint.restype = C.c_int
         L.yo_render_adjoint.argtypes = [vp, vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int,
                                         vp, C.c_int, vp, F32P, F32P, vp, vp, vp]
@@ -164,6 +169,20 @@ class OracleScene:
             raise RuntimeError("yo_render failed: %d" % rc)
         return rgb.reshape(desc.cam.resy, desc.cam.resx, 3), stats
 
+    def trace_camera(self, desc, px, py):
+        """Colours of camera rays through camera-plane points (px, py) (sun
+        lights only): [n, 3] float32."""
+        px = np.ascontiguousarray(px, np.float32)
+        py = np.ascontiguousarray(py, np.float32)
+        rgb = np.zeros((len(px), 3), np.float32)
+        rc = lib().yo_trace_camera(self.h, C.addressof(desc.cam), _addr(desc.mats), len(desc.mats),
+                                   _addr(desc.mods), _addr(desc.texs), _addr(desc.lights), len(desc.lights),
+                                   _ptr(desc.obj_flags), len(desc.obj_flags), C.addressof(desc.opts),
+                                   px, py, len(px), rgb)
+        if rc != 0:
+            raise RuntimeError("yo_trace_camera failed: %d" % rc)
+        return rgb
+
     def render_logged(self, desc, threads=1, ext_cap=0, sh_cap=0):
         """render() that also returns every first-hit and shadow query the
         render made: (img, stats, ext[n,8], sh[n,8]) with the record layouts
@@ -236,6 +255,14 @@ def _addr(arr):
     if arr is None or len(arr) == 0:
         return None
     return C.addressof(arr)
+
+
+def block_regions(width, height, block=64, seed=1):
+    """blockSpliter_t regions in render order: [n, 8] int32 (x y w h rx ry rw rh)."""
+    cap = ((width + block - 1) // block) * ((height + block - 1) // block)
+    out = np.zeros((cap, 8), np.int32)
+    n = lib().yo_block_regions_into(width, height, block, seed, out, cap)
+    return out[:n]
 
 
 def halton_seq(base, n):

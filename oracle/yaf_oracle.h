@@ -102,6 +102,15 @@ int yo_render_mt(yo_scene* s, const yg_camera* cam, const yg_material* mats, int
                  const yg_light* lights, int nl, const int32_t* obj_flags, int nobj,
                  const yg_render_opts* opts, float* rgb, uint64_t* stats, int nthreads);
 
+/* blockSpliter_t regions (renderblock.cc:71-101) in render order, 8 ints each */
+int yo_block_regions_into(int width, int height, int b, uint32_t seed, int32_t* out, int cap);
+
+/* raytrace of camera rays through camera-plane points (px, py); sun lights only */
+int yo_trace_camera(yo_scene* s, const yg_camera* cam, const yg_material* mats, int nmat,
+                    const yg_modulator* mods, const yg_texture* texs, const yg_light* lights, int nl,
+                    const int32_t* obj_flags, int nobj, const yg_render_opts* opts, const float* px,
+                    const float* py, int64_t n, float* rgb);
+
 /* FP64 reference gradient of a loss through the image (single thread):
  * gmat nmat*3, gmod nmod*2 (colour, size scale), glight nl (power) */
 int yo_render_adjoint(yo_scene* s, const yg_camera* cam, const yg_material* mats, int nmat,

@@ -1049,6 +1049,20 @@ int yo_block_regions(int width, int height, int b, uint32_t seed, yo_region** ou
     return n;
 }
 
+/* the same regions into a caller buffer of cap x 8 ints (x y w h rx ry rw rh);
+ * returns the region count (regions beyond cap are not written) */
+int yo_block_regions_into(int width, int height, int b, uint32_t seed, int32_t* out, int cap)
+{
+    yo_region* r = NULL;
+    const int n = yo_block_regions(width, height, b, seed, &r);
+    for (int q = 0; q < n && q < cap; q++) {
+        const int32_t v[8] = {r[q].x, r[q].y, r[q].w, r[q].h, r[q].rx, r[q].ry, r[q].rw, r[q].rh};
+        memcpy(out + 8 * q, v, sizeof(v));
+    }
+    free(r);
+    return n;
+}
+
 typedef struct {
     yo_scene* s;
     const yg_camera* cam;
@@ -1069,31 +1083,115 @@ typedef struct {
     double *gmat, *gmod, *glight;
 } render_job;
 
+/* renderArea_t::checkResample, renderblock.cc:22-69: a pixel of the area's
+ * image (W x H, the overlap border included) is resampled when any of its 8
+ * neighbours (clamped to the area) differs by abscol2bri >= threshold. */
+static float abscol2bri(yo_col a, yo_col b)
+{
+    /* color.h:71 on the difference (colorA_t operator-, color.h:293-296, in
+     * float).  color.h includes only <cmath>, so its unqualified fabs is the C
+     * library's fabs(double): the weighted sum is formed in double and
+     * returned as float. */
+    return (float)(0.299f * fabs((double)(a.r - b.r)) + 0.587f * fabs((double)(a.g - b.g)) +
+                   0.114f * fabs((double)(a.b - b.b)));
+}
+
+static void check_resample(const yo_col* img, int W, int H, float threshold, unsigned char* mask)
+{
+    for (int i = 0; i < H; ++i) {
+        const int im = i > 0 ? i - 1 : 0, ip = i + 1 < H ? i + 1 : H - 1;
+        for (int j = 0; j < W; ++j) {
+            const int jm = j > 0 ? j - 1 : 0, jp = j + 1 < W ? j + 1 : W - 1;
+            const yo_col c = img[i * W + j];
+            const int nb[8][2] = {{im, jm}, {im, j}, {im, jp}, {i, jm}, {i, jp}, {ip, jm}, {ip, j}, {ip, jp}};
+            int need = 0;
+            for (int q = 0; q < 8 && !need; q++) need = abscol2bri(c, img[nb[q][0] * W + nb[q][1]]) >= threshold;
+            mask[i * W + j] = (unsigned char)need;
+        }
+    }
+}
+
+/* scene_t::render(renderArea_t&), scene.cc:724-816: the first pass (pixel
+ * centres, or with AA_jitterfirst the area's n-th pixel at (RI_vdC(n),
+ * RI_S(n))), then AA_passes passes that resample the pixels the contrast mask
+ * marks with AA_minsamples camera rays each and fold them into a running
+ * average.  The area keeps its own image, overlap border included; only the
+ * real rectangle is written out. */
 static void render_area(rctx* R, const render_job* J, int ri)
 {
     const yo_region* a = &J->regions[ri];
+    const yg_render_opts* o = J->opts;
     /* a fresh renderState_t (and sampler context) per area, scene.cc:726 */
     for (int li = 0; li < J->nl; li++)
         if (J->lights[li].type == YG_LIGHT_PATH) {
             int nd = (J->lights[li].depth + 1) * 2, base = 2;
             for (int d = 0; d < nd; d++) { halton_init(&R->hseq[li][d], base); base = yo_next_prime(base); }
         }
+    const int W = a->w, H = a->h, resx = J->cam->resx;
+    const int passes = o->aa_passes > 0 ? o->aa_passes : 0;
+    yo_col* img = (yo_col*)malloc(sizeof(yo_col) * (size_t)W * H);
+    unsigned int sc = 0;
     for (int i = a->y; i < a->y + a->h; ++i)
         for (int j = a->x; j < a->x + a->w; ++j) {
-            yo_v3 ray = cam_ray(J->k, (float)j + 0.5f, (float)i + 0.5f);
+            float fx = 0.5f, fy = 0.5f;
+            if (o->aa_jitterfirst && passes != 0) {
+                ++sc;
+                fx = yo_ri_vdc(sc, 0);
+                fy = yo_ri_s(sc, 0);
+            }
+            yo_v3 ray = cam_ray(J->k, (float)j + fx, (float)i + fy);
             const int out = j >= a->rx && j < a->rx + a->rw && i >= a->ry && i < a->ry + a->rh;
             R->gpx_on = 0;
             if (R->adj && out) {
-                const float* g = J->grad_rgb + 3 * ((int64_t)i * J->cam->resx + j);
+                const float* g = J->grad_rgb + 3 * ((int64_t)i * resx + j);
                 R->gpx[0] = g[0]; R->gpx[1] = g[1]; R->gpx[2] = g[2];
                 R->gpx_on = (g[0] != 0 || g[1] != 0 || g[2] != 0);
             }
-            yo_col c = raytrace(R, J->k->pos, ray);
-            if (out) {
-                int64_t px = (int64_t)i * J->cam->resx + j;
-                J->rgb[3 * px] = c.r; J->rgb[3 * px + 1] = c.g; J->rgb[3 * px + 2] = c.b;
-            }
+            img[(i - a->y) * W + (j - a->x)] = raytrace(R, J->k->pos, ray);
         }
+    if (passes > 0) {
+        unsigned char* mask = (unsigned char*)malloc((size_t)W * H);
+        const int mins = o->aa_minsamples;
+        float totsamdiv = (float)(mins * passes);
+        if (totsamdiv != 0) totsamdiv = (float)(1.0 / (double)totsamdiv);
+        for (int pass = 0; pass < passes; pass++) {
+            check_resample(img, W, H, o->aa_threshold, mask);
+            for (int i = a->y; i < a->y + a->h; ++i)
+                for (int j = a->x; j < a->x + a->w; ++j) {
+                    yo_col* px = &img[(i - a->y) * W + (j - a->x)];
+                    if (!mask[(i - a->y) * W + (j - a->x)]) continue;
+                    const unsigned int pixnum = (unsigned int)(j + i * resx);
+                    yo_col tot = col(0, 0, 0);
+                    int n = 0;
+                    for (int ms = 0; ms < mins; ms++) {
+                        const unsigned int cursam = (unsigned int)(pass * mins + ms);
+                        /* 0.5 is a double literal: the offsets are formed in double */
+                        const float fx = (float)(0.5 + (double)o->aa_pixelwidth *
+                                                           ((double)yo_ri_lp(cursam + pixnum, 0) - 0.5));
+                        const float fy = (float)(0.5 + (double)o->aa_pixelwidth *
+                                                           ((double)((float)cursam * totsamdiv) - 0.5));
+                        const yo_col c = raytrace(R, J->k->pos, cam_ray(J->k, (float)j + fx, (float)i + fy));
+                        tot = col_add(tot, c);
+                        n++;
+                    }
+                    /* (mf*image + totcol) / (mf + n), colorA_t / f = * (1/f) (color.h:282-286) */
+                    const float mf = (float)(pass * n + 1);
+                    float den = mf + (float)n;
+                    if (den != 0) den = (float)(1.0 / (double)den);
+                    px->r = (mf * px->r + tot.r) * den;
+                    px->g = (mf * px->g + tot.g) * den;
+                    px->b = (mf * px->b + tot.b) * den;
+                }
+        }
+        free(mask);
+    }
+    for (int i = a->ry; i < a->ry + a->rh; ++i)
+        for (int j = a->rx; j < a->rx + a->rw; ++j) {
+            const yo_col c = img[(i - a->y) * W + (j - a->x)];
+            const int64_t px = (int64_t)i * resx + j;
+            J->rgb[3 * px] = c.r; J->rgb[3 * px + 1] = c.g; J->rgb[3 * px + 2] = c.b;
+        }
+    free(img);
 }
 
 static void* render_worker(void* arg)
@@ -1161,6 +1259,29 @@ int yo_render_mt(yo_scene* s, const yg_camera* cam, const yg_material* mats, int
     return 0;
 }
 
+/* scene_t::raytrace of camera rays through camera-plane points (px[q], py[q])
+ * (camera_t::shootRay coordinates), one after another.  Test hook for scenes
+ * whose lights keep no per-area state (sun lights, background): returns -6
+ * for path, area and photon lights. */
+int yo_trace_camera(yo_scene* s, const yg_camera* cam, const yg_material* mats, int nmat,
+                    const yg_modulator* mods, const yg_texture* texs, const yg_light* lights, int nl,
+                    const int32_t* obj_flags, int nobj, const yg_render_opts* opts, const float* px,
+                    const float* py, int64_t n, float* rgb)
+{
+    for (int li = 0; li < nl; li++)
+        if (lights[li].type != YG_LIGHT_SUN) return -6;
+    rctx R;
+    rctx_init(&R, s, mats, nmat, mods, texs, lights, nl, obj_flags, nobj, opts);
+    yo_cam k;
+    cam_setup(cam, &k);
+    for (int64_t q = 0; q < n; q++) {
+        const yo_col c = raytrace(&R, k.pos, cam_ray(&k, px[q], py[q]));
+        rgb[3 * q] = c.r; rgb[3 * q + 1] = c.g; rgb[3 * q + 2] = c.b;
+    }
+    rctx_free(&R);
+    return 0;
+}
+
 int yo_render(yo_scene* s, const yg_camera* cam, const yg_material* mats, int nmat,
               const yg_modulator* mods, int nmod, const yg_texture* texs, int ntex,
               const yg_light* lights, int nl, const int32_t* obj_flags, int nobj,
@@ -1181,6 +1302,7 @@ int yo_render_adjoint(yo_scene* s, const yg_camera* cam, const yg_material* mats
                       double* gmod, double* glight)
 {
     (void)ntex;
+    if (opts->aa_passes > 0) return -5; /* antialiasing passes are not differentiated */
     memset(gmat, 0, sizeof(double) * 3 * (size_t)nmat);
     if (gmod) memset(gmod, 0, sizeof(double) * 2 * (size_t)nmod);
     memset(glight, 0, sizeof(double) * (size_t)nl);

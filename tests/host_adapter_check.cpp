@@ -64,6 +64,22 @@ int main()
     for (auto& m : mods)
         printf("mod %d %d %g %g %g %g\n", m.tex, m.mode, m.color, m.size[0], m.size[1], m.size[2]);
     for (auto& t : texs) printf("tex %d %g %g %g %g\n", t.type, t.H, t.lacunarity, t.octaves, t.iscale);
+    // render options: defaults, and the AA values clamped to their ranges
+    paramMap_t r0, r1, r2;
+    r1.setInt("AA_passes", 2);
+    r1.setInt("AA_minsamples", 0);
+    r1.setFloat("AA_pixelwidth", 2.5f);
+    r1.setFloat("AA_threshold", 0.03f);
+    r1.setBool("AA_jitterfirst", true);
+    r1.setFloat("bias", 0.002f);
+    r2.setInt("samples", 3);  // obsolete name: first pass included
+    r2.setFloat("tolerance", 0.1f);
+    r2.setFloat("AA_pixelwidth", 0.5f);
+    for (const paramMap_t* p : {&r0, &r1, &r2}) {
+        const yg_render_opts o = yafgpu::gpuInterface_t::render_opts(*p);
+        printf("ropts %g %d %d %d %g %g %d\n", o.self_bias, o.raydepth, o.aa_passes, o.aa_minsamples,
+               o.aa_pixelwidth, o.aa_threshold, o.aa_jitterfirst);
+    }
     // unsupported features are refused with a status, not an exit
     paramMap_t cache;
     cache.setBool("cache", true);

@@ -1,0 +1,89 @@
+"""Antialiasing passes on the GPU (yg_render with aa_passes > 0) against the
+oracle's scene_t::render(renderArea_t&) (scene.cc:724-816): jittered first
+pass, per-area contrast mask, RI_LP sample offsets, running average, and the
+pathlight's per-area Halton sequences continued through the passes."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from yafray_amd import descriptors as D
+from yafray_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _Y():
+    from yafray_amd import lib as Y
+    return Y
+
+
+def rel_rmse(a, b):
+    a, b = a.astype(np.float64), b.astype(np.float64)
+    return float(np.sqrt(np.mean((a - b) ** 2)) / max(np.sqrt(np.mean(b ** 2)), 1e-30))
+
+
+def terrain(resx, resy, samples, sun_only=False, **aa):
+    d = S.terrain_gi(po.heightfield, nx=160, ny=80, resx=resx, resy=resy, samples=samples, depth=3)
+    if sun_only:
+        d.lights = D.carray(D.yg_light, [d.lights[1]])
+    d.opts = D.render_opts(bias=2e-3, raydepth=3, background=(0.4, 0.5, 0.7), **aa)
+    return d
+
+
+def gpu(d, torch):
+    Y = _Y()
+    sc = Y.Scene.from_desc(d)
+    out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+    st = sc.render(out)
+    torch.cuda.synchronize()
+    return out.cpu().numpy().reshape(d.cam.resy, d.cam.resx, 3), st, sc
+
+
+CASES = [
+    dict(sun_only=True, samples=1, aa_passes=2, aa_minsamples=3, aa_threshold=0.05),
+    dict(sun_only=True, samples=1, aa_passes=1, aa_minsamples=4, aa_threshold=0.02, aa_jitterfirst=True,
+         aa_pixelwidth=2.0),
+    dict(samples=2, aa_passes=2, aa_minsamples=2, aa_threshold=0.05),
+    dict(samples=4, aa_passes=3, aa_minsamples=2, aa_threshold=0.03, aa_jitterfirst=True),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_aa_matches_oracle(torch_cuda, case):
+    c = dict(CASES[case])
+    print("aa case", case, c, flush=True)
+    d = terrain(192, 108, c.pop("samples"), c.pop("sun_only", False), **c)
+    img, st, _ = gpu(d, torch_cuda)
+    ref, rs = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat).render(d, threads=8)
+    assert np.all(np.isfinite(img))
+    assert rel_rmse(img, ref) <= 1e-3
+    assert np.mean(np.all(img == ref, axis=2)) >= 0.999
+    # same camera rays (first pass + AA samples), extension and shadow rays
+    assert st.rays_camera == rs[0]
+    assert st.rays_extension == rs[1]
+    assert st.rays_shadow == rs[2]
+    assert st.rays_camera > 192 * 108 * 1.05  # the passes resampled something
+    assert st.graph == 0
+
+
+def test_aa_unreachable_threshold_equals_plain_render(torch_cuda):
+    img, st, sc = gpu(terrain(160, 90, 2, aa_passes=2, aa_minsamples=3, aa_threshold=1e30), torch_cuda)
+    plain, pst, _ = gpu(terrain(160, 90, 2), torch_cuda)
+    assert np.array_equal(img, plain)
+    assert st.rays_camera == pst.rays_camera and st.rays_extension == pst.rays_extension
+
+
+def test_aa_unsupported_combinations(torch_cuda):
+    torch = torch_cuda
+    Y = _Y()
+    room = S.direct_room(64, 48, subdiv=3)
+    room.opts = D.render_opts(bias=1e-3, aa_passes=1, aa_minsamples=2)
+    sc = Y.Scene.from_desc(room)
+    out = torch.zeros(64 * 48 * 3, dtype=torch.float32, device="cuda")
+    with pytest.raises(RuntimeError, match="area lights"):
+        sc.render(out)
+    d = terrain(64, 48, 2, aa_passes=1, aa_minsamples=2)
+    a, b = Y.Scene.from_desc(d), Y.Scene.from_desc(d)
+    outs = [torch.zeros(64 * 48 * 3, dtype=torch.float32, device="cuda") for _ in range(2)]
+    with pytest.raises(RuntimeError, match="batched"):
+        Y.render_batch([a, b], outs)

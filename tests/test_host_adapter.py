@@ -61,3 +61,16 @@ def test_unsupported_is_refused(adapter_output):
 
 
 D_ERR_UNSUPPORTED = -4
+
+
+def test_render_options_match_python_spec(adapter_output):
+    rows = [l for l in adapter_output if l[0] == "ropts"]
+    specs = [D.render_opts(),
+             D.render_opts(bias=0.002, aa_passes=2, aa_minsamples=0, aa_pixelwidth=2.5, aa_threshold=0.03,
+                           aa_jitterfirst=True),
+             D.render_opts(aa_passes=2, aa_threshold=0.1, aa_pixelwidth=0.5)]
+    assert len(rows) == len(specs)
+    for r, o in zip(rows, specs):
+        assert _f(r[1]) == pytest.approx(o.self_bias) and int(r[2]) == o.raydepth
+        assert (int(r[3]), int(r[4]), int(r[7])) == (o.aa_passes, o.aa_minsamples, o.aa_jitterfirst)
+        assert _f(r[5]) == pytest.approx(o.aa_pixelwidth) and _f(r[6]) == pytest.approx(o.aa_threshold)

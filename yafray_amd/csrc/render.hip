@@ -52,6 +52,40 @@ __device__ __forceinline__ void item_pixel(const Area& a, uint32_t item, int& i,
     j = a.x + (int)(li % (uint32_t)a.w);
 }
 
+// Pixel (i, j) and camera-plane point (px, py) of a work item
+// (scene_t::render(renderArea_t&), scene.cc:740-796).  First-pass items are
+// the area's pixels in row-major order, shot through the pixel centre or,
+// with AA_jitterfirst, through (RI_vdC(n), RI_S(n)) for the area's n-th
+// pixel.  An item of AA pass p is sample ms of a marked pixel, cursam =
+// p * AA_minsamples + ms, at
+//   fx = 0.5 + AA_pixelwidth * (RI_LP(cursam + pixelNumber) - 0.5),
+//   fy = 0.5 + AA_pixelwidth * (cursam / (AA_minsamples * AA_passes) - 0.5),
+// formed in double as the reference's double literals make it.
+__device__ __forceinline__ void item_camera(const RenderScene& R, const Area& a, uint32_t item, int& i, int& j,
+                                            float& px, float& py)
+{
+    float fx = 0.5f, fy = 0.5f;
+    if (R.aa_pass < 0) {
+        item_pixel(a, item, i, j);
+        if (R.aa_jitter) {
+            const uint32_t n = item - a.item_start + 1;
+            fx = ri_vdc(n);
+            fy = ri_s(n);
+        }
+    } else {
+        const uint2 src = R.aa_items[item];
+        const uint32_t li = src.x - a.src_start;
+        i = a.y + (int)(li / (uint32_t)a.w);
+        j = a.x + (int)(li % (uint32_t)a.w);
+        const uint32_t cursam = (uint32_t)R.aa_pass * (uint32_t)R.aa_mins + src.y;
+        const uint32_t pixnum = (uint32_t)(j + i * R.resx);
+        fx = (float)(0.5 + (double)R.aa_pw * ((double)ri_lp(cursam + pixnum) - 0.5));
+        fy = (float)(0.5 + (double)R.aa_pw * ((double)((float)cursam * R.aa_totsamdiv) - 0.5));
+    }
+    px = (float)j + fx;
+    py = (float)i + fy;
+}
+
 struct Counters {
     unsigned long long ext, shadow, extra, bshadow, draws;
 };
@@ -71,20 +105,23 @@ template <int W>
 __global__ __launch_bounds__(RB) void k_primary(const RenderScene* __restrict__ scenes,
                                                 const Area* __restrict__ areas, int nareas,
                                                 uint32_t nitems, int32_t* __restrict__ item_ti,
-                                                float* __restrict__ item_t, Counters* __restrict__ cnt)
+                                                float* __restrict__ item_t, Counters* __restrict__ cnt,
+                                                int zero_counters)
 {
     extern __shared__ int s_stk[];
     const uint32_t item = blockIdx.x * RB + threadIdx.x;
-    // the frame's ray counters start here (kernels after this one add to them)
-    if (item == 0) *cnt = Counters{};
+    // the frame's ray counters start here (kernels after this one add to
+    // them); antialiasing passes keep counting
+    if (item == 0 && zero_counters) *cnt = Counters{};
     if (item >= nitems) return;
     LaneStack st;
     st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = RB; st.sp = 0;
     const Area a = areas[find_area(areas, nareas, item)];
     const RenderScene& R = scenes[a.scene];
     int i, j;
-    item_pixel(a, item, i, j);
-    const f3 ray = cam_ray(R, (float)j + 0.5f, (float)i + 0.5f);
+    float px, py;
+    item_camera(R, a, item, i, j, px, py);
+    const f3 ray = cam_ray(R, px, py);
     float t = INFINITY;
     int ti = -1;
     if (R.raydepth > 0) ti = first_hit<W>(R.bvh, R.cam_pos, ray, R.min_raydis, t, st);
@@ -96,8 +133,9 @@ __device__ __forceinline__ Surf primary_surf(const RenderScene& R, const Area& a
                                              float t, f3& eye)
 {
     int i, j;
-    item_pixel(a, item, i, j);
-    const f3 ray = cam_ray(R, (float)j + 0.5f, (float)i + 0.5f);
+    float px, py;
+    item_camera(R, a, item, i, j, px, py);
+    const f3 ray = cam_ray(R, px, py);
     const f3 f = R.cam_pos + ray * R.min_raydis;
     Surf sp = make_surf(R.bvh, ti, f, ray, t);
     eye = R.cam_pos - sp.P;
@@ -283,7 +321,7 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WP
     const int32_t* __restrict__ st_ti, const c3* __restrict__ st_col, float* __restrict__ st_s,
     c3* __restrict__ contri, uint8_t* __restrict__ nseg, f3* __restrict__ h_p, f3* __restrict__ h_ray,
     int32_t* __restrict__ h_ti, uint32_t* __restrict__ qcount, uint32_t* __restrict__ queue,
-    uint32_t* __restrict__ flag, Counters* __restrict__ cnt)
+    uint32_t* __restrict__ flag, const uint32_t* __restrict__ hbase, Counters* __restrict__ cnt)
 {
     extern __shared__ int s_stk[];
     const uint32_t idx = blockIdx.x * RB + threadIdx.x;
@@ -299,7 +337,8 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WP
         flag[p] = 0;  // set again by k_shade if the path continues
         const uint32_t item = p / (uint32_t)S;
         const int smp = (int)(p % (uint32_t)S);
-        const Area a = areas[find_area(areas, nareas, item)];
+        const int ai = find_area(areas, nareas, item);
+        const Area a = areas[ai];
         const RenderScene& R = scenes[a.scene];
         f3 where, N, Ru, Rv;
         c3 raycolor;
@@ -313,6 +352,7 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WP
             Rv = sp.NV;
             raycolor = mkc(1, 1, 1);
             k = (qoff[item] - qoff[a.item_start]) * (uint32_t)S + (uint32_t)smp;
+            if (hbase) k += hbase[(size_t)ai * D];  // level-0 draws of the area's earlier passes
         } else {
             where = st_where[p];
             N = st_hn[p];
@@ -508,16 +548,37 @@ __global__ __launch_bounds__(SB) void k_shade(const RenderScene* __restrict__ sc
 __global__ void k_compact(const Area* __restrict__ areas, int nareas, uint32_t npaths, int S,
                           const uint32_t* __restrict__ flag, const uint32_t* __restrict__ ex,
                           uint32_t* __restrict__ active, uint32_t* __restrict__ hidx_level,
-                          uint32_t* __restrict__ nact_next)
+                          uint32_t* __restrict__ nact_next, const uint32_t* __restrict__ hbase_level, int D)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p == npaths - 1) *nact_next = ex[p] + flag[p];
     if (p >= npaths || !flag[p]) return;
     const uint32_t item = p / (uint32_t)S;
-    const Area& a = areas[find_area(areas, nareas, item)];
+    const int ai = find_area(areas, nareas, item);
     const uint32_t e = ex[p];
     active[e] = p;
-    hidx_level[p] = e - ex[(size_t)a.item_start * S];
+    // the path's rank among the area's paths at this level (plus, with
+    // antialiasing, the level's draws in the area's earlier passes)
+    hidx_level[p] = e - ex[(size_t)areas[ai].item_start * S] + (hbase_level ? hbase_level[(size_t)ai * D] : 0u);
+}
+
+// Antialiasing passes: adds this pass's draws of one GI level to each area's
+// Halton counter (level 0: every sample of every qualifying item; level L:
+// the paths the compaction kept), so that the next pass continues the area's
+// sequences where this one stopped.
+__global__ void k_area_draws(const Area* __restrict__ areas, int nareas, uint32_t nitems, int S, int D,
+                             int level, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ ex,
+                             uint32_t* __restrict__ hbase)
+{
+    const int ai = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ai >= nareas) return;
+    const uint32_t s = areas[ai].item_start, e = (ai + 1 < nareas) ? areas[ai + 1].item_start : nitems;
+    if (e <= s) return;
+    const uint32_t k = (level == 0) ? 1u : (uint32_t)S;  // level 0 counts items, levels > 0 paths
+    const size_t lo = (size_t)s * k, hi = (size_t)e * k - 1;
+    uint32_t n = ex[hi] + flag[hi] - ex[lo];
+    if (level == 0) n *= (uint32_t)S;
+    hbase[(size_t)ai * D + level] += n;
 }
 
 // Level-0 active list (every sample of every qualifying item) and its
@@ -537,6 +598,97 @@ __global__ void k_path_init(uint32_t nitems, int S, int D, const uint32_t* __res
     for (int i = 0; i < S; i++) active[(size_t)qoff[item] * S + i] = item * (uint32_t)S + i;
 }
 
+// ------------------------------------------------------ antialiasing --
+//
+// scene_t::render(renderArea_t&) passes (scene.cc:777-816) over the
+// first-pass items: every area keeps its own colours (overlap border
+// included) in item_col.
+
+// abscol2bri of a colour difference (color.h:71): the difference in float
+// (colorA_t operator-), the weighted sum in double (color.h's unqualified
+// fabs is the C library's double fabs), returned as float.
+__device__ __forceinline__ float abscol2bri(c3 a, c3 b)
+{
+    return (float)(0.299f * fabs((double)(a.r - b.r)) + 0.587f * fabs((double)(a.g - b.g)) +
+                   0.114f * fabs((double)(a.b - b.b)));
+}
+
+// renderArea_t::checkResample (renderblock.cc:22-69): an item is resampled
+// when one of its 8 neighbours in the area (clamped at the area's edge)
+// differs by abscol2bri >= threshold; count = AA_minsamples items then, else 0.
+__global__ void k_aa_mask(const Area* __restrict__ areas, int nareas, uint32_t nitems, const c3* __restrict__ item_col,
+                          float threshold, uint32_t mins, uint32_t* __restrict__ count)
+{
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= nitems) return;
+    const Area& a = areas[find_area(areas, nareas, item)];
+    const int l = (int)(item - a.item_start), i = l / a.w, j = l % a.w;
+    const int im = i > 0 ? i - 1 : 0, ip = i + 1 < a.h ? i + 1 : a.h - 1;
+    const int jm = j > 0 ? j - 1 : 0, jp = j + 1 < a.w ? j + 1 : a.w - 1;
+    const c3* A = item_col + a.item_start;
+    const c3 c = A[l];
+    const int nb[8] = {im * a.w + jm, im * a.w + j, im * a.w + jp, i * a.w + jm,
+                       i * a.w + jp, ip * a.w + jm, ip * a.w + j, ip * a.w + jp};
+    bool need = false;
+    for (int q = 0; q < 8 && !need; q++) need = abscol2bri(c, A[nb[q]]) >= threshold;
+    count[item] = need ? mins : 0u;
+}
+
+// The pass's items: for each marked first-pass item its samples 0..mins-1,
+// in item order (areas, then the area's pixels row-major, then samples).
+__global__ void k_aa_items(uint32_t nitems, const uint32_t* __restrict__ count, const uint32_t* __restrict__ off,
+                           uint2* __restrict__ aa_items)
+{
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= nitems) return;
+    for (uint32_t ms = 0; ms < count[item]; ms++) aa_items[off[item] + ms] = make_uint2(item, ms);
+}
+
+// The running average of pass `pass` (scene.cc:809-814):
+// colour = (mf * colour + totcol) / (mf + n), mf = pass * n + 1, where the
+// colour division multiplies by the float reciprocal (color.h:282-286).
+__global__ void k_aa_combine(uint32_t nitems, const uint32_t* __restrict__ count, const uint32_t* __restrict__ off,
+                             const c3* __restrict__ aa_col, int pass, c3* __restrict__ item_col)
+{
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= nitems) return;
+    const uint32_t n = count[item];
+    if (!n) return;
+    c3 tot = mkc(0, 0, 0);
+    for (uint32_t ms = 0; ms < n; ms++) tot = cadd(tot, aa_col[off[item] + ms]);
+    const float mf = (float)(pass * (int)n + 1);
+    float den = mf + (float)n;
+    if (den != 0) den = (float)(1.0 / (double)den);
+    const c3 c = item_col[item];
+    item_col[item] = mkc((mf * c.r + tot.r) * den, (mf * c.g + tot.g) * den, (mf * c.b + tot.b) * den);
+}
+
+// First item of each area in the pass (starts[na] = the pass's item count).
+__global__ void k_aa_starts(const Area* __restrict__ areas, int nareas, uint32_t nitems,
+                            const uint32_t* __restrict__ count, const uint32_t* __restrict__ off,
+                            uint32_t* __restrict__ starts)
+{
+    const int ai = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ai < nareas) starts[ai] = off[areas[ai].item_start];
+    else if (ai == nareas) starts[ai] = off[nitems - 1] + count[nitems - 1];
+}
+
+// renderArea_t::out (renderblock.cc:8-20): the real rectangle of each area.
+__global__ void k_aa_output(const RenderScene* __restrict__ scenes, const Area* __restrict__ areas, int nareas,
+                            uint32_t nitems, const c3* __restrict__ item_col)
+{
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= nitems) return;
+    const Area& a = areas[find_area(areas, nareas, item)];
+    int i, j;
+    item_pixel(a, item, i, j);
+    if (j < a.rx || j >= a.rx + a.rw || i < a.ry || i >= a.ry + a.rh) return;
+    const RenderScene& R = scenes[a.scene];
+    const c3 c = item_col[item];
+    float* o = R.out_rgb + 3 * ((size_t)i * R.resx + j);
+    o[0] = c.r; o[1] = c.g; o[2] = c.b;
+}
+
 // Ordered light sum, pathlight estimate (pathlight.cc:349-403) and pixel output.
 __global__ __launch_bounds__(RB) void k_finalize(const RenderScene* __restrict__ scenes,
                                                  const Area* __restrict__ areas, int nareas,
@@ -547,7 +699,7 @@ __global__ __launch_bounds__(RB) void k_finalize(const RenderScene* __restrict__
                                                  const c3* __restrict__ lightc, int nl_max,
                                                  const c3* __restrict__ contri,
                                                  const uint8_t* __restrict__ nseg,
-                                                 Counters* __restrict__ cnt)
+                                                 Counters* __restrict__ cnt, c3* __restrict__ item_col)
 {
     const uint32_t item = blockIdx.x * RB + threadIdx.x;
     uint32_t extra_n = 0;
@@ -598,7 +750,8 @@ __global__ __launch_bounds__(RB) void k_finalize(const RenderScene* __restrict__
                 col = cadd(col, (li == R.path_light) ? pathc : lightc[(size_t)item * nl_max + li]);
             }
         }
-        if (j >= a.rx && j < a.rx + a.rw && i >= a.ry && i < a.ry + a.rh) {
+        if (item_col) item_col[item] = col;  // antialiasing: the area keeps every colour
+        else if (j >= a.rx && j < a.rx + a.rw && i >= a.ry && i < a.ry + a.rh) {
             float* o = R.out_rgb + 3 * ((size_t)i * R.resx + j);
             o[0] = col.r; o[1] = col.g; o[2] = col.b;
         }
@@ -791,6 +944,14 @@ struct RenderWS {
     size_t stk_bytes = 0;  // dynamic LDS of the traversal launches
     int width = 2;         // BVH width of the scenes in the launch
     bool any_area = false, any_photon = false;
+    // antialiasing passes: first-pass areas and colours, the pass's samples
+    DevBuf<Area> areas0;
+    DevBuf<c3> item_col, aa_col;
+    DevBuf<uint32_t> aa_cnt, aa_off, aa_starts, hbase;
+    DevBuf<uint2> aa_items;
+    int aa_pass = -1;             // pass of the launch being enqueued (-1: first pass)
+    bool hbase_on = false;        // per-area Halton counters carried across passes
+    c3* finalize_to = nullptr;    // per-item colours instead of image pixels
     // the single-scene render the workspace holds (for YG_RENDER_REUSE)
     uint64_t last_uid = 0;  // yg_scene::uid, 0 = none
     uint64_t last_gen = 0;
@@ -843,6 +1004,8 @@ struct ScenePlan {
     std::vector<f3> apts, ajh, ajv;
     uint32_t nitems = 0;
     int max_area_items = 0;
+    int aa_passes = 0;
+    float aa_threshold = 0;
 };
 
 static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, ScenePlan& P)
@@ -924,17 +1087,41 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
         if (R.depth > 255) return yg_fail(YG_ERR_UNSUPPORTED, "pathlight depth");
         if (R.nl > 32) return yg_fail(YG_ERR_UNSUPPORTED, "more than 32 lights with a pathlight");
     }
+    if (o->aa_passes < 0) return yg_fail(YG_ERR_ARG, "aa_passes must be >= 0");
+    if (o->aa_passes > 0) {
+        if (o->aa_minsamples < 1) return yg_fail(YG_ERR_ARG, "aa_minsamples must be >= 1");
+        if (!(o->aa_pixelwidth >= 1.0f && o->aa_pixelwidth <= 2.0f))
+            return yg_fail(YG_ERR_ARG, "aa_pixelwidth must be in [1, 2]");
+        for (const auto& L : P.lights)
+            if (L.type == YG_LIGHT_AREA)
+                return yg_fail(YG_ERR_UNSUPPORTED,
+                               "antialiasing passes with area lights (their ourRandom() stream runs through "
+                               "each area's passes before the next area starts)");
+    }
+    R.aa_pass = -1;
+    R.aa_mins = o->aa_passes > 0 ? o->aa_minsamples : 0;
+    R.aa_jitter = (o->aa_jitterfirst && o->aa_passes != 0) ? 1 : 0;
+    R.aa_pw = o->aa_pixelwidth;
+    float tsd = (float)(o->aa_minsamples * o->aa_passes);
+    if (tsd != 0) tsd = (float)(1.0 / (double)tsd);
+    R.aa_totsamdiv = tsd;
+    P.aa_passes = o->aa_passes;
+    P.aa_threshold = o->aa_threshold;
     block_regions(R.resx, R.resy, o->block > 0 ? o->block : 64, o->rand_seed, P.areas);
     uint32_t acc = 0;
     for (auto& a : P.areas) {
-        a.item_start = acc;
+        a.item_start = a.src_start = acc;
         acc += (uint32_t)(a.w * a.h);
         P.max_area_items = std::max(P.max_area_items, a.w * a.h);
     }
     P.nitems = acc;
     if (R.path_light >= 0) {
         const int ndim = 2 * (R.depth + 1);
-        R.hstride = P.max_area_items * R.samples + 1;
+        // an area's draws per level: its first pass, then up to AA_minsamples
+        // per pixel in each antialiasing pass
+        const int64_t hs = (int64_t)P.max_area_items * R.samples * (1 + (int64_t)P.aa_passes * std::max(R.aa_mins, 0)) + 1;
+        if (hs > INT32_MAX) return yg_fail(YG_ERR_UNSUPPORTED, "too many sampler draws per render area");
+        R.hstride = (int)hs;
         if (!s->d_halton || s->halton_ndim != ndim || s->halton_stride != R.hstride) {
             std::vector<float> all((size_t)ndim * R.hstride), tab;
             int base = 2;
@@ -1004,7 +1191,8 @@ static int trace_levels(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
         YG_LAUNCH_W(W.width, (k_ext<ADJ, 4>), (k_ext<ADJ, 2>), dim3(grid), dim3(RB), W.stk_bytes, st, W.scenes.p,
                     W.areas.p, na, j, act, W.nact.p + j, npaths, S, D, W.item_ti.p, W.item_t.p, W.qoff.p,
                     W.hidx.p, W.st_where.p, W.st_hn.p, W.st_ti.p, W.st_col.p, W.st_s.p, W.contri.p, W.nseg.p,
-                    W.h_p.p, W.h_ray.p, W.h_ti.p, qc, W.queue.p, W.flag.p, W.cnt.p);
+                    W.h_p.p, W.h_ray.p, W.h_ti.p, qc, W.queue.p, W.flag.p, W.hbase_on ? W.hbase.p : nullptr,
+                    W.cnt.p);
         YG_HIP(hipGetLastError());
         if (ktiming) YG_HIP(hipEventRecord(ev[1], st));
         YG_LAUNCH_W(W.width, (k_shadow<ADJ, 4>), (k_shadow<ADJ, 2>), dim3(grid), dim3(RB), W.stk_bytes, st,
@@ -1021,10 +1209,66 @@ static int trace_levels(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
         if (j + 1 >= D) break;
         if ((rc = exclusive_scan(W.flag.p, W.ex.p, npaths, st))) return rc;
         hipLaunchKernelGGL(k_compact, dim3(grid_for(npaths, 256)), dim3(256), 0, st, W.areas.p, na, npaths, S,
-                           W.flag.p, W.ex.p, nxt, W.hidx.p + (size_t)(j + 1) * npaths, W.nact.p + j + 1);
+                           W.flag.p, W.ex.p, nxt, W.hidx.p + (size_t)(j + 1) * npaths, W.nact.p + j + 1,
+                           W.hbase_on ? W.hbase.p + j + 1 : nullptr, D);
         YG_HIP(hipGetLastError());
+        if (W.hbase_on) {
+            hipLaunchKernelGGL(k_area_draws, dim3(grid_for(na, 256)), dim3(256), 0, st, W.areas.p, na, nitems, S, D,
+                               j + 1, W.flag.p, W.ex.p, W.hbase.p);
+            YG_HIP(hipGetLastError());
+        }
         std::swap(act, nxt);
     }
+    if (W.hbase_on) {
+        hipLaunchKernelGGL(k_area_draws, dim3(grid_for(na, 256)), dim3(256), 0, st, W.areas.p, na, nitems, S, D, 0,
+                           W.qual.p, W.qoff.p, W.hbase.p);
+        YG_HIP(hipGetLastError());
+    }
+    return YG_OK;
+}
+
+// Appends the bucket index of `areas` (ascending item_start, nitems items in
+// all) behind them, as find_area expects it.
+static void append_area_index(std::vector<Area>& areas, uint32_t nitems)
+{
+    const size_t na = areas.size();
+    const uint32_t nb = (nitems >> YG_AREA_LUT_SHIFT) + 1;
+    std::vector<uint32_t> lut(nb);
+    size_t k = 0;
+    for (uint32_t b = 0; b < nb; b++) {
+        const uint64_t first = (uint64_t)b << YG_AREA_LUT_SHIFT;
+        while (k + 1 < na && areas[k + 1].item_start <= first) k++;
+        lut[b] = (uint32_t)k;
+    }
+    areas.resize(na + (sizeof(uint32_t) * nb + sizeof(Area) - 1) / sizeof(Area));
+    memcpy(areas.data() + na, lut.data(), sizeof(uint32_t) * nb);
+}
+
+// Per-item and per-path buffers of a launch of nitems work items.
+static int ensure_frame_buffers(uint32_t nitems, uint32_t npaths, int nl_max, bool any_path, int D, hipStream_t st)
+{
+    RenderWS& W = g_ws;
+    int rc;
+    if ((rc = W.item_ti.ensure(nitems)) || (rc = W.item_t.ensure(nitems)) || (rc = W.draws.ensure(nitems)) ||
+        (rc = W.lcg_off.ensure(nitems)) || (rc = W.qual.ensure(nitems)) || (rc = W.qoff.ensure(nitems)) ||
+        (rc = W.lightc.ensure((size_t)nitems * nl_max)) || (rc = W.cnt.ensure(1)))
+        return rc;
+    if (any_path) {
+        if ((rc = W.active0.ensure(npaths)) || (rc = W.active1.ensure(npaths)) || (rc = W.flag.ensure(npaths)) ||
+            (rc = W.ex.ensure(npaths)) || (rc = W.hidx.ensure((size_t)npaths * D)) ||
+            (rc = W.st_where.ensure(npaths)) || (rc = W.st_hn.ensure(npaths)) || (rc = W.st_ti.ensure(npaths)) ||
+            (rc = W.st_col.ensure(npaths)) || (rc = W.st_s.ensure(npaths)) ||
+            (rc = W.contri.ensure((size_t)npaths * D)) || (rc = W.nseg.ensure(npaths)) ||
+            (rc = W.h_p.ensure(npaths)) || (rc = W.h_ray.ensure(npaths)) || (rc = W.h_ti.ensure(npaths)) ||
+            (rc = W.vis.ensure(npaths)) || (rc = W.queue.ensure(npaths)) || (rc = W.qcnt.ensure(D)) ||
+            (rc = W.nact.ensure(D)))
+            return rc;
+    }
+    // hipcub scan storage for both scans of the launch
+    size_t b1 = 0, b2 = 0;
+    if ((rc = scan_bytes(nitems, b1, st)) || (rc = scan_bytes(any_path ? npaths : 0, b2, st)) ||
+        (rc = W.scan_tmp.ensure(std::max(b1, b2))))
+        return rc;
     return YG_OK;
 }
 
@@ -1039,7 +1283,7 @@ static int launch_frame(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
     const int na = W.na, nl_max = W.nl_max;
     int rc;
     YG_LAUNCH_W(W.width, (k_primary<4>), (k_primary<2>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes, st,
-                W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.cnt.p);
+                W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.cnt.p, W.aa_pass < 0 ? 1 : 0);
     YG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_prep, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
                        nitems, W.item_ti.p, W.item_t.p, W.draws.p, W.qual.p);
@@ -1061,7 +1305,7 @@ static int launch_frame(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
     if (before_final) YG_HIP(hipEventRecord(before_final, st));
     hipLaunchKernelGGL(k_finalize, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
                        nitems, W.S, W.D, W.item_ti.p, W.item_t.p, W.qual.p, W.lightc.p, nl_max, W.contri.p,
-                       W.nseg.p, W.cnt.p);
+                       W.nseg.p, W.cnt.p, W.finalize_to);
     YG_HIP(hipGetLastError());
     return YG_OK;
 }
@@ -1133,6 +1377,7 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         for (auto a : P.areas) {
             a.scene = (int)scenes.size();
             a.item_start += nitems;
+            a.src_start += nitems;
             areas.push_back(a);
         }
         nitems += P.nitems;
@@ -1163,52 +1408,19 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     const uint64_t npaths64 = (uint64_t)nitems * S;
     if (npaths64 >= 0xffffffffull) return yg_fail(YG_ERR_UNSUPPORTED, "too many path samples in one launch");
     const uint32_t npaths = (uint32_t)npaths64;
-    // bucket index of the areas, stored behind them (see find_area)
     const int na_real = (int)areas.size();
-    {
-        const uint32_t nb = (nitems >> YG_AREA_LUT_SHIFT) + 1;
-        std::vector<uint32_t> lut(nb);
-        size_t k = 0;
-        for (uint32_t b = 0; b < nb; b++) {
-            const uint64_t first = (uint64_t)b << YG_AREA_LUT_SHIFT;
-            while (k + 1 < areas.size() && areas[k + 1].item_start <= first) k++;
-            lut[b] = (uint32_t)k;
+    int aa_passes = 0;
+    float aa_threshold = 0;
+    for (auto& P : plans)
+        if (P.aa_passes > 0) {
+            if (plans.size() > 1) return yg_fail(YG_ERR_UNSUPPORTED, "antialiasing passes in a batched render");
+            aa_passes = P.aa_passes;
+            aa_threshold = P.aa_threshold;
         }
-        const size_t extra = (sizeof(uint32_t) * nb + sizeof(Area) - 1) / sizeof(Area);
-        areas.resize(areas.size() + extra);
-        memcpy(areas.data() + na_real, lut.data(), sizeof(uint32_t) * nb);
-    }
-    if ((rc = W.scenes.ensure(scenes.size()))) return rc;
-    if ((rc = W.areas.ensure(areas.size()))) return rc;
-    if ((rc = W.item_ti.ensure(nitems))) return rc;
-    if ((rc = W.item_t.ensure(nitems))) return rc;
-    if ((rc = W.draws.ensure(nitems))) return rc;
-    if ((rc = W.lcg_off.ensure(nitems))) return rc;
-    if ((rc = W.qual.ensure(nitems))) return rc;
-    if ((rc = W.qoff.ensure(nitems))) return rc;
-    if ((rc = W.lightc.ensure((size_t)nitems * nl_max))) return rc;
-    if ((rc = W.cnt.ensure(1))) return rc;
-    if (any_path) {
-        if ((rc = W.active0.ensure(npaths))) return rc;
-        if ((rc = W.active1.ensure(npaths))) return rc;
-        if ((rc = W.flag.ensure(npaths))) return rc;
-        if ((rc = W.ex.ensure(npaths))) return rc;
-        if ((rc = W.hidx.ensure((size_t)npaths * D))) return rc;
-        if ((rc = W.st_where.ensure(npaths))) return rc;
-        if ((rc = W.st_hn.ensure(npaths))) return rc;
-        if ((rc = W.st_ti.ensure(npaths))) return rc;
-        if ((rc = W.st_col.ensure(npaths))) return rc;
-        if ((rc = W.st_s.ensure(npaths))) return rc;
-        if ((rc = W.contri.ensure((size_t)npaths * D))) return rc;
-        if ((rc = W.nseg.ensure(npaths))) return rc;
-        if ((rc = W.h_p.ensure(npaths))) return rc;
-        if ((rc = W.h_ray.ensure(npaths))) return rc;
-        if ((rc = W.h_ti.ensure(npaths))) return rc;
-        if ((rc = W.vis.ensure(npaths))) return rc;
-        if ((rc = W.queue.ensure(npaths))) return rc;
-        if ((rc = W.qcnt.ensure(D))) return rc;
-        if ((rc = W.nact.ensure(D))) return rc;
-    }
+    const std::vector<Area> areas0(areas);  // the first pass's areas
+    append_area_index(areas, nitems);
+    if ((rc = W.scenes.ensure(scenes.size())) || (rc = W.areas.ensure(areas.size()))) return rc;
+    if ((rc = ensure_frame_buffers(nitems, npaths, nl_max, any_path, D, st))) return rc;
     if ((rc = upload_cached(W.scenes, scenes, W.up_scenes, st)) || (rc = upload_cached(W.areas, areas, W.up_areas, st)) ||
         (rc = upload_cached(W.lights, lights, W.up_lights, st)) || (rc = upload_cached(W.apts, apts, W.up_apts, st)) ||
         (rc = upload_cached(W.ajh, ajh, W.up_ajh, st)) || (rc = upload_cached(W.ajv, ajv, W.up_ajv, st)))
@@ -1240,13 +1452,7 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         YG_HIP(hipEventCreate(&e2));
         YG_HIP(hipEventRecord(e0, st));
     }
-    // every buffer of the frame is allocated before its shape is taken
-    {
-        size_t b1 = 0, b2 = 0;
-        if ((rc = scan_bytes(nitems, b1, st)) || (rc = scan_bytes(any_path ? npaths : 0, b2, st)) ||
-            (rc = W.scan_tmp.ensure(std::max(b1, b2))))
-            return rc;
-    }
+    // (every buffer of the frame was allocated above, before its shape is taken)
     // The launch shape: a graph captured for it replays any frame of the same
     // shape (scenes, areas, lights and output pointers are read from the
     // uploaded descriptors, not baked into the launches).
@@ -1254,7 +1460,19 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
                                           (uint64_t)W.na, (uint64_t)nl_max, (uint64_t)W.width, W.stk_bytes,
                                           W.scan_tmp.cap, any_area, any_photon, any_path};
     int graph = 0;
-    if (use_graph && !timing && !W.graphs_off) {
+    W.aa_pass = -1;
+    W.hbase_on = aa_passes > 0 && any_path;
+    W.finalize_to = nullptr;
+    if (aa_passes > 0) {
+        if ((rc = W.areas0.ensure(areas.size())) || (rc = W.item_col.ensure(nitems)) ||
+            (rc = W.aa_cnt.ensure(nitems)) || (rc = W.aa_off.ensure(nitems)) ||
+            (rc = W.aa_starts.ensure((size_t)na_real + 1)) || (rc = W.hbase.ensure((size_t)na_real * D)))
+            return rc;
+        YG_HIP(hipMemcpyAsync(W.areas0.p, areas.data(), sizeof(Area) * areas.size(), hipMemcpyHostToDevice, st));
+        YG_HIP(hipMemsetAsync(W.hbase.p, 0, sizeof(uint32_t) * na_real * D, st));
+        W.finalize_to = W.item_col.p;
+    }
+    if (use_graph && !timing && !W.graphs_off && aa_passes == 0) {
         // capture on the second consecutive frame of a shape, replay from then on
         bool captured = false;
         if (!(W.frame_exec && W.frame_key == key) && W.seen_valid && W.seen_key == key) {
@@ -1269,6 +1487,59 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     W.seen_key = key;
     W.seen_valid = true;
     if (!graph && (rc = launch_frame(st, ktiming, kev, timing ? e1 : nullptr))) return rc;
+    uint64_t aa_rays = 0;
+    if (aa_passes > 0) {
+        // scene.cc:777-816: each pass resamples the pixels the contrast mask
+        // marks (per area, from the area's colours) with AA_minsamples camera
+        // rays each, through the whole pipeline again, then folds them in
+        const uint32_t n0 = nitems;
+        const uint32_t mins = (uint32_t)plans[0].R.aa_mins;
+        std::vector<uint32_t> starts((size_t)na_real + 1);
+        std::vector<hipEvent_t> none;
+        for (int pass = 0; pass < aa_passes; pass++) {
+            hipLaunchKernelGGL(k_aa_mask, dim3(grid_for(n0, 256)), dim3(256), 0, st, W.areas0.p, na_real, n0,
+                               W.item_col.p, aa_threshold, mins, W.aa_cnt.p);
+            YG_HIP(hipGetLastError());
+            if ((rc = exclusive_scan(W.aa_cnt.p, W.aa_off.p, n0, st))) return rc;
+            hipLaunchKernelGGL(k_aa_starts, dim3(grid_for((uint64_t)na_real + 1, 256)), dim3(256), 0, st, W.areas0.p,
+                               na_real, n0, W.aa_cnt.p, W.aa_off.p, W.aa_starts.p);
+            YG_HIP(hipGetLastError());
+            YG_HIP(hipMemcpyAsync(starts.data(), W.aa_starts.p, sizeof(uint32_t) * starts.size(),
+                                  hipMemcpyDeviceToHost, st));
+            YG_HIP(hipStreamSynchronize(st));
+            const uint32_t total = starts[na_real];
+            if (total == 0) break;  // no colour changes, so no later pass resamples either
+            const uint64_t np64 = (uint64_t)total * S;
+            if (np64 >= 0xffffffffull) return yg_fail(YG_ERR_UNSUPPORTED, "too many path samples in one launch");
+            std::vector<Area> aa_areas(areas0);
+            for (int a = 0; a < na_real; a++) aa_areas[a].item_start = starts[a];
+            append_area_index(aa_areas, total);
+            if ((rc = W.areas.ensure(aa_areas.size())) || (rc = W.aa_items.ensure(total)) ||
+                (rc = W.aa_col.ensure(total)) ||
+                (rc = ensure_frame_buffers(total, (uint32_t)np64, nl_max, any_path, D, st)))
+                return rc;
+            scenes[0].aa_pass = pass;
+            scenes[0].aa_items = W.aa_items.p;
+            if ((rc = upload_cached(W.scenes, scenes, W.up_scenes, st)) ||
+                (rc = upload_cached(W.areas, aa_areas, W.up_areas, st)))
+                return rc;
+            hipLaunchKernelGGL(k_aa_items, dim3(grid_for(n0, 256)), dim3(256), 0, st, n0, W.aa_cnt.p, W.aa_off.p,
+                               W.aa_items.p);
+            YG_HIP(hipGetLastError());
+            W.nitems = total;
+            W.npaths = (uint32_t)np64;
+            W.aa_pass = pass;
+            W.finalize_to = W.aa_col.p;
+            if ((rc = launch_frame(st, false, none, nullptr))) return rc;
+            hipLaunchKernelGGL(k_aa_combine, dim3(grid_for(n0, 256)), dim3(256), 0, st, n0, W.aa_cnt.p, W.aa_off.p,
+                               W.aa_col.p, pass, W.item_col.p);
+            YG_HIP(hipGetLastError());
+            aa_rays += total;
+        }
+        hipLaunchKernelGGL(k_aa_output, dim3(grid_for(n0, 256)), dim3(256), 0, st, W.scenes.p, W.areas0.p, na_real,
+                           n0, W.item_col.p);
+        YG_HIP(hipGetLastError());
+    }
     if (stats) {
         if (!W.h_cnt) YG_HIP(hipHostMalloc((void**)&W.h_cnt, sizeof(Counters), hipHostMallocDefault));
         YG_HIP(hipMemcpyAsync(W.h_cnt, W.cnt.p, sizeof(Counters), hipMemcpyDeviceToHost, st));
@@ -1276,7 +1547,7 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         YG_HIP(hipStreamSynchronize(st));
         const Counters c = *W.h_cnt;
         memset(stats, 0, sizeof(*stats));
-        stats->rays_camera = nitems;
+        stats->rays_camera = nitems + aa_rays;
         stats->rays_extension = c.ext;
         stats->rays_shadow = c.shadow;
         stats->adaptive_extra = c.extra;
@@ -1596,6 +1867,7 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
 {
     if (!s || !opts || !d_grad_rgb || !d_rgb || !d_grad_mat || !d_grad_light)
         return yg_fail(YG_ERR_ARG, "null argument");
+    if (opts->aa_passes > 0) return yg_fail(YG_ERR_UNSUPPORTED, "the adjoint does not differentiate antialiasing passes");
     int nsun = 0, npath = 0;
     for (const auto& L : s->lights) {
         if (L.type == YG_LIGHT_SUN) nsun++;

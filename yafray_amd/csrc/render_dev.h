@@ -60,6 +60,10 @@ struct RenderScene {
     float* out_rgb;
     uint32_t item_base;      // first item of this scene
     uint32_t area_base;      // first area of this scene
+    // antialiasing (scene.cc:740-816): the launch's pass, -1 = the first pass
+    int aa_pass, aa_mins, aa_jitter;
+    float aa_pw, aa_totsamdiv;
+    const uint2* aa_items;   // AA-pass items: (first-pass item, sample number)
 };
 
 // A render area (renderArea_t): the extended rectangle x,y,w,h is rendered,
@@ -67,8 +71,32 @@ struct RenderScene {
 struct Area {
     int scene;
     int x, y, w, h, rx, ry, rw, rh;
-    uint32_t item_start;
+    uint32_t item_start;  // first item of the area in this launch
+    uint32_t src_start;   // first item of the area in the first pass
 };
+
+// RI_vdC / RI_S / RI_LP (mcqmc.h:59-81): radical inverses by bit reversal and
+// the Sobol' and Larcher-Pillichshammer generator matrices, as float.
+__device__ __forceinline__ float ri_vdc(uint32_t bits)
+{
+    return (float)((double)__builtin_bitreverse32(bits) * (1.0 / 4294967296.0));
+}
+
+__device__ __forceinline__ float ri_s(uint32_t i)
+{
+    uint32_t r = 0;
+    for (uint32_t v = 1u << 31; i; i >>= 1, v ^= v >> 1)
+        if (i & 1) r ^= v;
+    return (float)((double)r * (1.0 / 4294967296.0));
+}
+
+__device__ __forceinline__ float ri_lp(uint32_t i)
+{
+    uint32_t r = 0;
+    for (uint32_t v = 1u << 31; i; i >>= 1, v |= v >> 1)
+        if (i & 1) r ^= v;
+    return (float)((double)r * (1.0 / 4294967296.0));
+}
 
 // surfacePoint_t subset for a hit (mesh.cc:517-577, triangle.cc:138-141):
 // P = from + Z*ray; N = Ng = face normal; NU, NV = createCS(N).

@@ -60,7 +60,9 @@ class yg_light(C.Structure):
 class yg_render_opts(C.Structure):
     _fields_ = [("min_raydis", C.c_float), ("self_bias", C.c_float), ("raydepth", C.c_int32),
                 ("background", F3), ("has_background", C.c_int32), ("block", C.c_int32),
-                ("rand_seed", C.c_uint32), ("lcg_seed", C.c_uint32), ("flags", C.c_int32)]
+                ("rand_seed", C.c_uint32), ("lcg_seed", C.c_uint32), ("flags", C.c_int32),
+                ("aa_passes", C.c_int32), ("aa_minsamples", C.c_int32), ("aa_pixelwidth", C.c_float),
+                ("aa_threshold", C.c_float), ("aa_jitterfirst", C.c_int32)]
 
 
 class yg_render_stats(C.Structure):
@@ -224,8 +226,21 @@ def material(color, mod_first=0, mod_count=0):
     return yg_material(_f3(color), mod_first, mod_count)
 
 
-def render_opts(bias=0.1, raydepth=3, background=None, block=64, rand_seed=1, lcg_seed=123212):
+def render_opts(bias=0.1, raydepth=3, background=None, block=64, rand_seed=1, lcg_seed=123212,
+                aa_passes=0, aa_minsamples=1, aa_pixelwidth=1.5, aa_threshold=0.05, aa_jitterfirst=False):
+    """Render parameters with the reference's defaults; the AA values are
+    clamped as interface_impl.cc:501-533 clamps them."""
     o = yg_render_opts()
+    if aa_passes < 0:
+        aa_passes = 1
+    if aa_passes and aa_minsamples < 1:
+        aa_minsamples = 1
+    aa_pixelwidth = min(max(aa_pixelwidth, 1.0), 2.0)
+    o.aa_passes = aa_passes
+    o.aa_minsamples = aa_minsamples
+    o.aa_pixelwidth = aa_pixelwidth
+    o.aa_threshold = aa_threshold
+    o.aa_jitterfirst = int(bool(aa_jitterfirst))
     o.min_raydis = MIN_RAYDIST
     o.self_bias = bias
     o.raydepth = raydepth

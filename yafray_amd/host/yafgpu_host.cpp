@@ -393,6 +393,40 @@ int gpuInterface_t::describe(std::vector<yg_light>& lights, std::vector<yg_mater
     return YG_OK;
 }
 
+yg_render_opts gpuInterface_t::render_opts(const paramMap_t& params)
+{
+    yg_render_opts o;
+    memset(&o, 0, sizeof(o));
+    o.min_raydis = 0.00005f;  // MIN_RAYDIST (configio.py:95-100)
+    o.self_bias = 0.1f;       // scene_t ctor (scene.cc:56)
+    o.raydepth = 3;           // maxraylevel
+    o.block = 64;
+    o.rand_seed = 1;
+    o.lcg_seed = 123212;      // myseed (vector3d.cc:160)
+    params.getParam("bias", o.self_bias);
+    params.getParam("raydepth", o.raydepth);
+    // antialiasing parameters and their ranges (interface_impl.cc:501-533)
+    int aa_passes = 0, aa_min = 1;
+    float aa_pw = 1.5f, aa_thr = 0.05f;
+    bool aa_jit = false;
+    if (params.getParam("samples", aa_passes)) aa_passes -= 1;  // obsolete name, counts the first pass
+    else params.getParam("AA_passes", aa_passes);
+    if (aa_passes < 0) aa_passes = 1;
+    params.getParam("AA_minsamples", aa_min);
+    if (aa_passes && aa_min < 1) aa_min = 1;
+    params.getParam("AA_pixelwidth", aa_pw);
+    if (aa_pw < 1.0f) aa_pw = 1.0f;
+    else if (aa_pw > 2.0f) aa_pw = 2.0f;
+    if (!params.getParam("tolerance", aa_thr)) params.getParam("AA_threshold", aa_thr);
+    params.getParam("AA_jitterfirst", aa_jit);
+    o.aa_passes = aa_passes;
+    o.aa_minsamples = aa_min;
+    o.aa_pixelwidth = aa_pw;
+    o.aa_threshold = aa_thr;
+    o.aa_jitterfirst = aa_jit ? 1 : 0;
+    return o;
+}
+
 int gpuInterface_t::render(const paramMap_t& params, std::vector<float>& rgb, yg_render_stats* stats)
 {
     std::vector<yg_light> lights;
@@ -439,16 +473,7 @@ int gpuInterface_t::render(const paramMap_t& params, std::vector<float>& rgb, yg
                              texs.data(), (uint32_t)texs.size())))
         return rc;
     if ((rc = yg_set_lights(s, lights.data(), (uint32_t)lights.size()))) return rc;
-    yg_render_opts o;
-    memset(&o, 0, sizeof(o));
-    o.min_raydis = 0.00005f;  // MIN_RAYDIST (configio.py:95-100)
-    o.self_bias = 0.1f;       // scene_t ctor (scene.cc:56)
-    o.raydepth = 3;           // maxraylevel
-    o.block = 64;
-    o.rand_seed = 1;
-    o.lcg_seed = 123212;      // myseed (vector3d.cc:160)
-    params.getParam("bias", o.self_bias);
-    params.getParam("raydepth", o.raydepth);
+    yg_render_opts o = render_opts(params);
     auto bit = backgrounds_.find(bgname);
     if (bit != backgrounds_.end()) {
         o.has_background = 1;

@@ -70,9 +70,13 @@ public:
     int addObject_trimesh(const std::string& name, const std::vector<float>& verts,
                           const std::vector<uint32_t>& faces, const std::vector<std::string>& shaders,
                           bool cast_shadows = true, bool recv_rad = true, bool use_rad = true);
-    // render params: camera_name, background_name, bias, raydepth; rgb gets
+    // render params: camera_name, background_name, bias, raydepth, AA_*; rgb gets
     // resx*resy*3 floats (HDR, no tone mapping).  Photon lights are built first.
     int render(const paramMap_t& params, std::vector<float>& rgb, yg_render_stats* stats = nullptr);
+
+    // the render options render() passes to yg_render for these params
+    // (interfaceImpl_t::render's defaults and ranges, interface_impl.cc:480-533)
+    static yg_render_opts render_opts(const paramMap_t& params);
 
     // the descriptors the next render would use (object order, light order)
     int describe(std::vector<yg_light>& lights, std::vector<yg_material>& mats,
