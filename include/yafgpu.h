@@ -33,7 +33,7 @@ extern "C" {
 #define YG_ERR_UNSUPPORTED (-4)
 #define YG_ERR_NOMEM (-5)
 
-#define YG_ABI_VERSION 5
+#define YG_ABI_VERSION 6
 
 /* ---------------------------------------------------------------- rays -- */
 
@@ -176,6 +176,12 @@ typedef struct yg_render_opts {
     float aa_pixelwidth;     /* AA_pixelwidth, 1..2 */
     float aa_threshold;      /* AA_threshold */
     int32_t aa_jitterfirst;  /* AA_jitterfirst */
+    /* tone mapping of every raytraced colour (scene.cc:763 with color.h:175-195);
+     * the reference turns it on for 8-bit (targa) output, off for hdr */
+    int32_t tonemap;
+    float exposure;          /* render param "exposure", 0 = off */
+    float gamma;             /* render param "gamma", 1 = off */
+    int32_t clamp_rgb;       /* render param "clamp_rgb" */
 } yg_render_opts;
 
 #define YG_RENDER_TIMING 1   /* fill yg_render_stats.ms with device time */

@@ -1083,6 +1083,36 @@ typedef struct {
     double *gmat, *gmod, *glight;
 } render_job;
 
+/* color_t::expgam_Adjust (color.h:175-195) with scene_t's stored values
+ * exposure = -e and gamma_R = 1/g (0 for g = 0) (scene.h:161-164); exp and
+ * pow are the C library's double functions there. */
+static float cl0(float v) { return v < 0 ? 0.f : v; }
+
+static yo_col tonemap(yo_col c, const yg_render_opts* o)
+{
+    if (!o->tonemap) return c;
+    const float e = -o->exposure, g = (o->gamma != 0.0f) ? (float)(1.0 / (double)o->gamma) : 0.0f;
+    if (e == 0.f && g == 1.f) {
+        if (o->clamp_rgb) {
+            c.r = c.r < 0 ? 0.f : (c.r > 1 ? 1.f : c.r);
+            c.g = c.g < 0 ? 0.f : (c.g > 1 ? 1.f : c.g);
+            c.b = c.b < 0 ? 0.f : (c.b > 1 ? 1.f : c.b);
+        }
+        return c;
+    }
+    if (e != 0.f) {
+        c.r = (float)(1.0 - exp((double)(cl0(c.r) * e)));
+        c.g = (float)(1.0 - exp((double)(cl0(c.g) * e)));
+        c.b = (float)(1.0 - exp((double)(cl0(c.b) * e)));
+    }
+    if (g != 1.f) {
+        c.r = (float)pow((double)cl0(c.r), (double)g);
+        c.g = (float)pow((double)cl0(c.g), (double)g);
+        c.b = (float)pow((double)cl0(c.b), (double)g);
+    }
+    return c;
+}
+
 /* renderArea_t::checkResample, renderblock.cc:22-69: a pixel of the area's
  * image (W x H, the overlap border included) is resampled when any of its 8
  * neighbours (clamped to the area) differs by abscol2bri >= threshold. */
@@ -1147,7 +1177,7 @@ static void render_area(rctx* R, const render_job* J, int ri)
                 R->gpx[0] = g[0]; R->gpx[1] = g[1]; R->gpx[2] = g[2];
                 R->gpx_on = (g[0] != 0 || g[1] != 0 || g[2] != 0);
             }
-            img[(i - a->y) * W + (j - a->x)] = raytrace(R, J->k->pos, ray);
+            img[(i - a->y) * W + (j - a->x)] = tonemap(raytrace(R, J->k->pos, ray), o);
         }
     if (passes > 0) {
         unsigned char* mask = (unsigned char*)malloc((size_t)W * H);
@@ -1170,7 +1200,8 @@ static void render_area(rctx* R, const render_job* J, int ri)
                                                            ((double)yo_ri_lp(cursam + pixnum, 0) - 0.5));
                         const float fy = (float)(0.5 + (double)o->aa_pixelwidth *
                                                            ((double)((float)cursam * totsamdiv) - 0.5));
-                        const yo_col c = raytrace(R, J->k->pos, cam_ray(J->k, (float)j + fx, (float)i + fy));
+                        const yo_col c =
+                            tonemap(raytrace(R, J->k->pos, cam_ray(J->k, (float)j + fx, (float)i + fy)), o);
                         tot = col_add(tot, c);
                         n++;
                     }

@@ -109,3 +109,34 @@ def test_aa_unreachable_threshold_is_the_plain_render():
     base, bstats = oc.render(sun_terrain(), threads=4)
     assert np.array_equal(img, base)
     assert int(stats[0]) == int(bstats[0])
+
+
+def tonemap_np(img, exposure, gamma, clamp):
+    """color_t::expgam_Adjust (color.h:175-195) with scene_t's exposure = -e,
+    gamma_R = 1/g; exp and pow in double, each step stored as float."""
+    e = F32(-exposure)
+    g = F32(1.0 / F64(F32(gamma))) if gamma != 0 else F32(0)
+    c = img.astype(F32)
+    if e == 0 and g == 1:
+        return np.clip(c, 0, 1) if clamp else c
+    if e != 0:
+        c = (1.0 - np.exp((np.maximum(c, F32(0)) * e).astype(F64))).astype(F32)
+    if g != 1:
+        c = np.power(np.maximum(c, F32(0)).astype(F64), F64(g)).astype(F32)
+    return c
+
+
+@pytest.mark.parametrize("exposure,gamma,clamp", [(0.0, 1.0, True), (1.5, 1.0, False), (0.0, 2.2, False),
+                                                  (0.8, 1.8, True)])
+def test_tonemapping_is_expgam_adjust_of_each_colour(exposure, gamma, clamp):
+    d = sun_terrain()
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    plain, _ = oc.render(d, threads=4)
+    d.opts = D.render_opts(bias=2e-3, raydepth=3, background=(0.4, 0.5, 0.7), tonemap=True, exposure=exposure,
+                           gamma=gamma, clamp_rgb=clamp)
+    img, _ = oc.render(d, threads=4)
+    ref = tonemap_np(plain, exposure, gamma, clamp)
+    assert np.mean(img == ref) >= 0.9999
+    assert np.max(np.abs(img - ref)) <= 1e-6
+    if exposure or gamma != 1:
+        assert not np.array_equal(img, plain)

@@ -1,4 +1,5 @@
-"""Build libyafgpu.so (hipcc, gfx950) in-tree.
+"""Build libyafgpu.so (hipcc, gfx950) and the yafgpu-render scene-file
+renderer in-tree.
 
 Usage: python -m yafray_amd.build  (or __graft_entry__.build()).
 """
@@ -11,9 +12,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libyafgpu.so")
+CLI = os.path.join(HERE, "yafgpu-render")
 
 SOURCES = ["capi.cpp", "qmc.cpp", "bvh_build.cpp", "autotune.cpp", "trace.hip", "render.hip", "photon.hip",
-           "../host/yafgpu_host.cpp"]
+           "../host/yafgpu_host.cpp", "../host/yafgpu_xml.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("YG_OFFLOAD_ARCH", "gfx950")
@@ -52,6 +54,15 @@ def build(verbose=True, defines=(), lib=LIB, obj_dir=OBJ):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, lib)
+    if lib == LIB:
+        # the command-line renderer, linked against the library beside it
+        cmd = ["g++", "-std=c++17", "-O2", "-I", os.path.join(HERE, "host"),
+               os.path.join(HERE, "host", "yafgpu_render_main.cpp"), "-L", HERE, "-lyafgpu",
+               "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib", "-o", CLI + ".tmp"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
+        os.replace(CLI + ".tmp", CLI)
     if verbose:
         print("built", lib)
     return lib

@@ -750,6 +750,7 @@ __global__ __launch_bounds__(RB) void k_finalize(const RenderScene* __restrict__
                 col = cadd(col, (li == R.path_light) ? pathc : lightc[(size_t)item * nl_max + li]);
             }
         }
+        if (R.tm) col = expgam_adjust(col, R.tm_e, R.tm_g, R.tm_clamp != 0);
         if (item_col) item_col[item] = col;  // antialiasing: the area keeps every colour
         else if (j >= a.rx && j < a.rx + a.rw && i >= a.ry && i < a.ry + a.rh) {
             float* o = R.out_rgb + 3 * ((size_t)i * R.resx + j);
@@ -1105,6 +1106,10 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
     float tsd = (float)(o->aa_minsamples * o->aa_passes);
     if (tsd != 0) tsd = (float)(1.0 / (double)tsd);
     R.aa_totsamdiv = tsd;
+    R.tm = o->tonemap ? 1 : 0;
+    R.tm_clamp = o->clamp_rgb ? 1 : 0;
+    R.tm_e = -o->exposure;
+    R.tm_g = (o->gamma != 0.0f) ? (float)(1.0 / (double)o->gamma) : 0.0f;
     P.aa_passes = o->aa_passes;
     P.aa_threshold = o->aa_threshold;
     block_regions(R.resx, R.resy, o->block > 0 ? o->block : 64, o->rand_seed, P.areas);

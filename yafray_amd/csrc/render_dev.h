@@ -64,7 +64,35 @@ struct RenderScene {
     int aa_pass, aa_mins, aa_jitter;
     float aa_pw, aa_totsamdiv;
     const uint2* aa_items;   // AA-pass items: (first-pass item, sample number)
+    // tone mapping (scene_t::setExposure / setGamma, scene.h:161-164)
+    int tm, tm_clamp;
+    float tm_e, tm_g;        // -exposure, 1/gamma (0 for gamma 0)
 };
+
+// color_t::expgam_Adjust (color.h:175-195).  color.h includes only <cmath>, so
+// its unqualified exp and pow are the C library's double functions: each
+// channel is formed in double and stored as float.
+__device__ __forceinline__ float clamp0(float v) { return (v < 0) ? 0.f : v; }
+__device__ __forceinline__ float clamp01(float v) { return (v < 0) ? 0.f : ((v > 1) ? 1.f : v); }
+
+__device__ __forceinline__ c3 expgam_adjust(c3 c, float e, float g, bool clamp_rgb)
+{
+    if (e == 0.f && g == 1.f) {
+        if (clamp_rgb) c = mkc(clamp01(c.r), clamp01(c.g), clamp01(c.b));
+        return c;
+    }
+    if (e != 0.f) {
+        c = mkc(clamp0(c.r), clamp0(c.g), clamp0(c.b));
+        c = mkc((float)(1.0 - exp((double)(c.r * e))), (float)(1.0 - exp((double)(c.g * e))),
+                (float)(1.0 - exp((double)(c.b * e))));
+    }
+    if (g != 1.f) {
+        c = mkc(clamp0(c.r), clamp0(c.g), clamp0(c.b));
+        c = mkc((float)pow((double)c.r, (double)g), (float)pow((double)c.g, (double)g),
+                (float)pow((double)c.b, (double)g));
+    }
+    return c;
+}
 
 // A render area (renderArea_t): the extended rectangle x,y,w,h is rendered,
 // the real rectangle rx,ry,rw,rh is written out (renderblock.cc:8-20, 71-101).

@@ -62,7 +62,9 @@ class yg_render_opts(C.Structure):
                 ("background", F3), ("has_background", C.c_int32), ("block", C.c_int32),
                 ("rand_seed", C.c_uint32), ("lcg_seed", C.c_uint32), ("flags", C.c_int32),
                 ("aa_passes", C.c_int32), ("aa_minsamples", C.c_int32), ("aa_pixelwidth", C.c_float),
-                ("aa_threshold", C.c_float), ("aa_jitterfirst", C.c_int32)]
+                ("aa_threshold", C.c_float), ("aa_jitterfirst", C.c_int32),
+                ("tonemap", C.c_int32), ("exposure", C.c_float), ("gamma", C.c_float),
+                ("clamp_rgb", C.c_int32)]
 
 
 class yg_render_stats(C.Structure):
@@ -227,7 +229,8 @@ def material(color, mod_first=0, mod_count=0):
 
 
 def render_opts(bias=0.1, raydepth=3, background=None, block=64, rand_seed=1, lcg_seed=123212,
-                aa_passes=0, aa_minsamples=1, aa_pixelwidth=1.5, aa_threshold=0.05, aa_jitterfirst=False):
+                aa_passes=0, aa_minsamples=1, aa_pixelwidth=1.5, aa_threshold=0.05, aa_jitterfirst=False,
+                tonemap=False, exposure=0.0, gamma=1.0, clamp_rgb=False):
     """Render parameters with the reference's defaults; the AA values are
     clamped as interface_impl.cc:501-533 clamps them."""
     o = yg_render_opts()
@@ -241,6 +244,10 @@ def render_opts(bias=0.1, raydepth=3, background=None, block=64, rand_seed=1, lc
     o.aa_pixelwidth = aa_pixelwidth
     o.aa_threshold = aa_threshold
     o.aa_jitterfirst = int(bool(aa_jitterfirst))
+    o.tonemap = int(bool(tonemap))
+    o.exposure = exposure
+    o.gamma = gamma
+    o.clamp_rgb = int(bool(clamp_rgb))
     o.min_raydis = MIN_RAYDIST
     o.self_bias = bias
     o.raydepth = raydepth

@@ -19,6 +19,7 @@ static int fail(int code, const std::string& msg)
 }
 
 const std::string& lastError() { return g_host_err; }
+int hostFail(int code, const std::string& msg) { return fail(code, msg); }
 
 bool paramMap_t::getParam(const std::string& k, float& out) const
 {
@@ -319,6 +320,14 @@ int gpuInterface_t::addLight(const std::string& name, const std::string& type, c
     yg_light L;
     int rc = lightFactory(type, p, L);
     if (rc) return rc;
+    // render_t::light (render.cc:781-815) applies use_in_render / use_in_indirect
+    // after the factory; path and photon lights leave indirect light again in
+    // their init (pathlight.cc:100, photonlight.cc:102), which runs later
+    bool in_render = true, in_indirect = true;
+    p.getParam("use_in_render", in_render);
+    p.getParam("use_in_indirect", in_indirect);
+    L.use_in_render = in_render ? 1 : 0;
+    if (L.type != YG_LIGHT_PATH && L.type != YG_LIGHT_PHOTON) L.use_in_indirect = in_indirect ? 1 : 0;
     lights_[name] = L;
     return YG_OK;
 }
@@ -332,6 +341,15 @@ int gpuInterface_t::addBackground(const std::string& name, const paramMap_t& p)
     c.assign(col, col + 3);
     backgrounds_[name] = c;
     return YG_OK;
+}
+
+bool gpuInterface_t::cameraResolution(const std::string& name, int& resx, int& resy) const
+{
+    auto it = cameras_.find(name);
+    if (it == cameras_.end()) return false;
+    resx = it->second.resx;
+    resy = it->second.resy;
+    return true;
 }
 
 int gpuInterface_t::addCamera(const std::string& name, const paramMap_t& p)
@@ -424,6 +442,18 @@ yg_render_opts gpuInterface_t::render_opts(const paramMap_t& params)
     o.aa_pixelwidth = aa_pw;
     o.aa_threshold = aa_thr;
     o.aa_jitterfirst = aa_jit ? 1 : 0;
+    // exposure and gamma (interface_impl.cc:545-548); the reference tone-maps
+    // only for 8-bit output (render.cc:1233-1257)
+    bool clamp = false;
+    std::string out_type = "tga";
+    o.exposure = 0.0f;
+    o.gamma = 1.0f;
+    params.getParam("exposure", o.exposure);
+    params.getParam("gamma", o.gamma);
+    params.getParam("clamp_rgb", clamp);
+    params.getParam("output_type", out_type);
+    o.clamp_rgb = clamp ? 1 : 0;
+    o.tonemap = (out_type == "hdr" || out_type.find("exr") != std::string::npos) ? 0 : 1;
     return o;
 }
 

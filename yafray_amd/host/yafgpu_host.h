@@ -53,6 +53,8 @@ int shaderFactory(const std::string& type, const paramMap_t& p, const std::vecto
                   const std::map<std::string, int>& textures, yg_material& out,
                   std::vector<yg_modulator>& mods_out);
 const std::string& lastError();
+// records an error (lastError, cerr with the reference's prefix) and returns code
+int hostFail(int code, const std::string& msg);
 
 // renderEnvironment / yafrayInterface_t subset for the GPU path.
 class gpuInterface_t {
@@ -77,6 +79,9 @@ public:
     // the render options render() passes to yg_render for these params
     // (interfaceImpl_t::render's defaults and ranges, interface_impl.cc:480-533)
     static yg_render_opts render_opts(const paramMap_t& params);
+
+    // resolution of a defined camera; false when there is none of that name
+    bool cameraResolution(const std::string& name, int& resx, int& resy) const;
 
     // the descriptors the next render would use (object order, light order)
     int describe(std::vector<yg_light>& lights, std::vector<yg_material>& mats,
