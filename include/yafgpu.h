@@ -307,9 +307,10 @@ int yg_set_lights(yg_scene* s, const yg_light* lights, uint32_t n);
 
 /* scene_t::render(colorOutput_t&) in mono mode (scene.cc:538-600), each area
  * rendered as scene_t::render(renderArea_t&) (scene.cc:724-816) including the
- * antialiasing passes of opts->aa_* (not with area lights, whose ourRandom()
- * stream would have to run through each area's passes in turn): d_rgb
- * receives resx*resy*3 floats (row-major, top row first).
+ * antialiasing passes of opts->aa_* (with area lights, whose ourRandom()
+ * stream runs through each area's passes in turn, the areas are then launched
+ * one after another): d_rgb receives resx*resy*3 floats (row-major, top row
+ * first).
  * Lights supported: sun, area (psamples 0), path (use_QMC on, cache off),
  * photon (diffuse mode; yg_photon_build must have run). */
 int yg_render(yg_scene* s, const yg_render_opts* opts, float* d_rgb,

@@ -73,15 +73,24 @@ def test_aa_unreachable_threshold_equals_plain_render(torch_cuda):
     assert st.rays_camera == pst.rays_camera and st.rays_extension == pst.rays_extension
 
 
-def test_aa_unsupported_combinations(torch_cuda):
+@pytest.mark.parametrize("jitter", [False, True])
+def test_aa_with_area_light_matches_oracle(torch_cuda, jitter):
+    """Area lights draw from the global ourRandom() stream: the areas (and
+    their passes) are rendered in turn, each continuing the stream."""
+    room = S.direct_room(96, 72, subdiv=3)
+    room.opts = D.render_opts(bias=1e-3, aa_passes=2, aa_minsamples=2, aa_threshold=0.05, aa_jitterfirst=jitter)
+    print("aa room jitter", jitter, flush=True)
+    img, st, _ = gpu(room, torch_cuda)
+    ref, rs = po.OracleScene(room.verts, room.idx, room.tri_obj, room.tri_mat).render(room, threads=1)
+    assert rel_rmse(img, ref) <= 1e-3
+    assert np.mean(np.all(img == ref, axis=2)) >= 0.999
+    assert (st.rays_camera, st.rays_shadow, st.lcg_draws) == (rs[0], rs[2], rs[4])
+    assert st.rays_camera > 96 * 72 * 1.05
+
+
+def test_aa_batched_is_refused(torch_cuda):
     torch = torch_cuda
     Y = _Y()
-    room = S.direct_room(64, 48, subdiv=3)
-    room.opts = D.render_opts(bias=1e-3, aa_passes=1, aa_minsamples=2)
-    sc = Y.Scene.from_desc(room)
-    out = torch.zeros(64 * 48 * 3, dtype=torch.float32, device="cuda")
-    with pytest.raises(RuntimeError, match="area lights"):
-        sc.render(out)
     d = terrain(64, 48, 2, aa_passes=1, aa_minsamples=2)
     a, b = Y.Scene.from_desc(d), Y.Scene.from_desc(d)
     outs = [torch.zeros(64 * 48 * 3, dtype=torch.float32, device="cuda") for _ in range(2)]

@@ -1007,6 +1007,7 @@ struct ScenePlan {
     int max_area_items = 0;
     int aa_passes = 0;
     float aa_threshold = 0;
+    bool area_lcg = false;  // antialiasing with area lights: areas in turn
 };
 
 static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, ScenePlan& P)
@@ -1094,10 +1095,7 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
         if (!(o->aa_pixelwidth >= 1.0f && o->aa_pixelwidth <= 2.0f))
             return yg_fail(YG_ERR_ARG, "aa_pixelwidth must be in [1, 2]");
         for (const auto& L : P.lights)
-            if (L.type == YG_LIGHT_AREA)
-                return yg_fail(YG_ERR_UNSUPPORTED,
-                               "antialiasing passes with area lights (their ourRandom() stream runs through "
-                               "each area's passes before the next area starts)");
+            if (L.type == YG_LIGHT_AREA) P.area_lcg = true;
     }
     R.aa_pass = -1;
     R.aa_mins = o->aa_passes > 0 ? o->aa_minsamples : 0;
@@ -1345,6 +1343,22 @@ static int capture_frame(const std::array<uint64_t, 13>& key)
     return YG_OK;
 }
 
+// ourRandom()'s state after n more draws: seed * 16807^n mod (2^31 - 1)
+// (vector3d.h:264-298), by the same power table the kernels jump with.
+static uint32_t lcg_advance(uint32_t seed, uint64_t n)
+{
+    static uint64_t pw[64];
+    static bool ready = false;
+    if (!ready) {
+        lcg_pow_table(pw);
+        ready = true;
+    }
+    uint64_t acc = seed;
+    for (int k = 0; k < 64 && (n >> k); k++)
+        if ((n >> k) & 1) acc = (acc * pw[k]) % 2147483647ull;
+    return (uint32_t)acc;
+}
+
 // Launch the whole pipeline over a set of planned scenes (one launch per stage).
 static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render_stats* stats,
                         bool timing, bool ktiming, bool use_graph)
@@ -1419,6 +1433,8 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     for (auto& P : plans)
         if (P.aa_passes > 0) {
             if (plans.size() > 1) return yg_fail(YG_ERR_UNSUPPORTED, "antialiasing passes in a batched render");
+            if (P.area_lcg && P.areas.size() > 1)
+                return yg_fail(YG_ERR_STATE, "antialiasing with area lights renders one area per launch");
             aa_passes = P.aa_passes;
             aa_threshold = P.aa_threshold;
         }
@@ -1511,8 +1527,15 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
             YG_HIP(hipGetLastError());
             YG_HIP(hipMemcpyAsync(starts.data(), W.aa_starts.p, sizeof(uint32_t) * starts.size(),
                                   hipMemcpyDeviceToHost, st));
+            if (any_area) {
+                if (!W.h_cnt) YG_HIP(hipHostMalloc((void**)&W.h_cnt, sizeof(Counters), hipHostMallocDefault));
+                YG_HIP(hipMemcpyAsync(W.h_cnt, W.cnt.p, sizeof(Counters), hipMemcpyDeviceToHost, st));
+            }
             YG_HIP(hipStreamSynchronize(st));
             const uint32_t total = starts[na_real];
+            // (one area per launch with area lights: the pass continues the
+            // area's ourRandom() stream after the draws taken so far)
+            if (any_area) scenes[0].lcg_seed = lcg_advance(plans[0].R.lcg_seed, W.h_cnt->draws);
             if (total == 0) break;  // no colour changes, so no later pass resamples either
             const uint64_t np64 = (uint64_t)total * S;
             if (np64 >= 0xffffffffull) return yg_fail(YG_ERR_UNSUPPORTED, "too many path samples in one launch");
@@ -1586,12 +1609,51 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     return YG_OK;
 }
 
+// scene_t::render in mono mode renders the areas one after another.  With
+// anti-aliasing and area lights, an area's passes take draws from the global
+// ourRandom() stream before the next area starts, and which pixels the passes
+// resample depends on the colours those draws give: each area is then its
+// own launch (first pass and passes), its stream starting where the previous
+// areas' draws left it.
+static int render_areas_in_turn(const ScenePlan& P, hipStream_t st, yg_render_stats* stats, bool timing)
+{
+    yg_render_stats tot;
+    memset(&tot, 0, sizeof(tot));
+    uint64_t draws = 0;
+    for (const Area& a : P.areas) {
+        std::vector<ScenePlan> one(1, P);
+        Area b = a;
+        b.item_start = b.src_start = 0;
+        one[0].areas.assign(1, b);
+        one[0].nitems = (uint32_t)(a.w * a.h);
+        one[0].R.lcg_seed = lcg_advance(P.R.lcg_seed, draws);
+        yg_render_stats s;
+        int rc = render_plans(one, st, &s, timing, false, false);
+        if (rc) return rc;
+        draws += s.lcg_draws;
+        tot.rays_camera += s.rays_camera;
+        tot.rays_extension += s.rays_extension;
+        tot.rays_shadow += s.rays_shadow;
+        tot.adaptive_extra += s.adaptive_extra;
+        tot.rays_bounce += s.rays_bounce;
+        tot.ms += s.ms;
+        tot.ms_trace += s.ms_trace;
+    }
+    tot.lcg_draws = draws;
+    if (stats) *stats = tot;
+    return YG_OK;
+}
+
 extern "C" int yg_render(yg_scene* s, const yg_render_opts* opts, float* d_rgb, yg_render_stats* stats,
                          void* stream)
 {
     std::vector<ScenePlan> plans(1);
     int rc = plan_scene(s, opts, d_rgb, plans[0]);
     if (rc) return rc;
+    if (plans[0].aa_passes > 0 && plans[0].area_lcg) {
+        g_ws.last_uid = 0;
+        return render_areas_in_turn(plans[0], (hipStream_t)stream, stats, opts->flags & YG_RENDER_TIMING);
+    }
     g_ws.last_uid = 0;
     rc = render_plans(plans, (hipStream_t)stream, stats, opts->flags & YG_RENDER_TIMING,
                       opts->flags & YG_RENDER_KTIMING, !(opts->flags & YG_RENDER_NO_GRAPH));
