@@ -285,7 +285,7 @@ def bench_autotune(args):
     out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
     default_dt, _ = _timed(lambda: sc.render(out), args.steps, args.warmup)
     t0 = time.perf_counter()
-    best, best_ms, cands = sc.autotune(out, widths=(2, 4), leaves=(2, 4, 8), repeats=2)
+    best, best_ms, cands = sc.autotune(out, widths=(2, 4), leaves=(2, 4, 8), stacks=(-1, 8, 16, 32), repeats=2)
     tune_s = time.perf_counter() - t0
     tuned_dt, _ = _timed(lambda: sc.render(out), args.steps, args.warmup)
     return {
@@ -294,12 +294,13 @@ def bench_autotune(args):
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": "path-traced GI on the 1,045,506-triangle fBm terrain, 1280x720, 4 spp, "
-                               "3 bounces; BVH width 2/4 x max_leaf 2/4/8, 32 SAH bins"},
+                               "3 bounces; BVH width 2/4 x max_leaf 2/4/8 x LDS stack entries all/8/16/32, "
+                               "32 SAH bins"},
         "default_build_ms_per_frame": round(default_dt * 1e3, 3),
         "best": {"width": best.width, "max_leaf": best.max_leaf, "sah_bins": best.sah_bins,
-                 "frame_device_ms": round(best_ms, 3)},
-        "candidates": [{"width": w, "max_leaf": lf, "sah_bins": b, "frame_device_ms": round(ms, 3),
-                        "build_ms": round(bms, 1)} for (w, lf, b, ms, bms) in cands],
+                 "stack_lds": best.stack_lds, "frame_device_ms": round(best_ms, 3)},
+        "candidates": [{"width": w, "max_leaf": lf, "sah_bins": b, "stack_lds": sl, "frame_device_ms": round(ms, 3),
+                        "build_ms": round(bms, 1)} for (w, lf, b, ms, bms, sl) in cands],
         "tune_s": round(tune_s, 2),
     }
 

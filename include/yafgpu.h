@@ -33,7 +33,7 @@ extern "C" {
 #define YG_ERR_UNSUPPORTED (-4)
 #define YG_ERR_NOMEM (-5)
 
-#define YG_ABI_VERSION 6
+#define YG_ABI_VERSION 7
 
 /* ---------------------------------------------------------------- rays -- */
 
@@ -213,6 +213,9 @@ typedef struct yg_build_opts {
     int32_t sah_bins;      /* 8..64 */
     int32_t threads;       /* host build threads, 0 = all */
     int32_t width;         /* BVH node width: 2 or 4 (0 = default, 4) */
+    int32_t stack_lds;     /* traversal stack entries per lane kept in LDS by the render
+                              launches, deeper ones in device memory (0 = default 16,
+                              -1 = the whole stack in LDS) */
 } yg_build_opts;
 
 typedef struct yg_scene yg_scene;
@@ -332,6 +335,8 @@ typedef struct yg_tune_opts {
     int32_t bins;     /* SAH bin counts to try, as bits 1<<k for 2^k bins, k in 3..6; 0 = 32 only */
     int32_t repeats;  /* timed renders per candidate after one untimed render; 0 = 2 */
     int32_t threads;  /* host build threads; 0 = all */
+    int32_t stacks;   /* stack_lds values to try, as bits: 1<<0 = whole stack in LDS,
+                         1<<k = 2^k entries, k in 3..6; 0 = the build's default only */
 } yg_tune_opts;
 
 #define YG_TUNE_MAX 32
@@ -346,7 +351,7 @@ typedef struct yg_tune_result {
 
 /* Autotuner (no reference counterpart: the reference's kd-tree has fixed
  * build constants, kdtree.cc:57-138).  Builds the scene with every candidate
- * (node width x leaf size x SAH bins), times yg_render of the caller's frame
+ * (node width x leaf size x SAH bins x LDS stack entries), times yg_render of the caller's frame
  * (opts, d_rgb: the image of the last render is left there) and leaves the
  * scene built with the fastest.  Images do not depend on the build (exact
  * distance ties go to the lowest primitive id).  At most YG_TUNE_MAX

@@ -266,3 +266,30 @@ def test_autotune_keeps_fastest_build(torch_cuda):
     assert np.mean(a == b) >= 0.9999
     ref_img, _ = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat).render(d, threads=8)
     assert rel_rmse(a.reshape(ref_img.shape), ref_img) <= 1e-3
+
+
+def test_stack_lds_split_changes_nothing_but_speed(torch_cuda):
+    """However many stack entries per lane stay in LDS (the rest in device
+    memory), traversal and image are the same; the tuner's choice is kept."""
+    torch = torch_cuda
+    Y = _Y()
+    d = small_terrain(nx=256, ny=128, resx=160, resy=90, samples=2)
+    n = d.cam.resx * d.cam.resy * 3
+    imgs, stats = [], []
+    for sl in (-1, 4, 8, 16):
+        sc = Y.Scene.from_desc(d, stack_lds=sl)
+        out = torch.zeros(n, dtype=torch.float32, device="cuda")
+        st = sc.render(out)
+        torch.cuda.synchronize()
+        imgs.append(out.cpu().numpy())
+        stats.append((st.rays_camera, st.rays_extension, st.rays_shadow))
+    for im, s in zip(imgs[1:], stats[1:]):
+        assert np.array_equal(im, imgs[0]) and s == stats[0]
+    sc = Y.Scene.from_desc(d)
+    out = torch.zeros(n, dtype=torch.float32, device="cuda")
+    best, best_ms, cands = sc.autotune(out, widths=(4,), leaves=(4,), stacks=(-1, 8, 16), repeats=1)
+    assert [c[5] for c in cands] == [-1, 8, 16]
+    assert best_ms == min(c[3] for c in cands) and best.stack_lds in (-1, 8, 16)
+    sc.render(out)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), imgs[0])

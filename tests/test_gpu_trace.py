@@ -52,7 +52,7 @@ def _check(prim_g, t_g, u_g, v_g, prim_c, t_c, u_c, v_c):
 
 def test_library_loads(torch_cuda):
     Y = _lib(torch_cuda)
-    assert Y.lib().yg_abi_version() == 6
+    assert Y.lib().yg_abi_version() == 7
 
 
 @pytest.mark.parametrize("seed,width", [(0, 2), (1, 2), (0, 4), (1, 4)])

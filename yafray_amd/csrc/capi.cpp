@@ -100,6 +100,8 @@ extern "C" int yg_build(yg_scene* s, const yg_build_opts* opts)
     int threads = opts ? opts->threads : 0;
     int width = (opts && opts->width) ? opts->width : YG_DEFAULT_WIDTH;
     if (width != 2 && width != 4) return yg_fail(YG_ERR_ARG, "BVH width must be 2 or 4");
+    const int stack_lds = (opts && opts->stack_lds) ? opts->stack_lds : YG_DEFAULT_STACK_LDS;
+    if (stack_lds < -1 || stack_lds > YG_MAX_STACK) return yg_fail(YG_ERR_ARG, "stack_lds must be -1 or 0..YG_MAX_STACK");
     HostBVH h;
     const int64_t nt = (int64_t)s->tri_mat.size();
     build_bvh(s->verts.data(), s->idx.data(), nt, s->tri_obj.data(), s->tri_mat.data(), max_leaf,
@@ -134,6 +136,7 @@ extern "C" int yg_build(yg_scene* s, const yg_build_opts* opts)
     s->max_depth = h.max_depth;
     s->width = width;
     s->stack_entries = entries;
+    s->stack_lds = stack_lds < 0 ? 0 : stack_lds;
     s->built = true;
     s->gen++;
     return YG_OK;

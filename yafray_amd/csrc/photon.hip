@@ -95,7 +95,7 @@ __global__ __launch_bounds__(PB) void k_photon_seg(PhotonSrc src, BVHView bvh, c
     const uint32_t idx = blockIdx.x * PB + threadIdx.x;
     if (idx >= nactive) return;
     LaneStack st;
-    st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = PB; st.sp = 0;
+    st.init(s_stk, threadIdx.x, PB);
     const uint32_t p = active[idx];
     f3 pos, lastpos, dir;
     c3 col;

@@ -106,7 +106,7 @@ __global__ __launch_bounds__(RB) void k_primary(const RenderScene* __restrict__ 
                                                 const Area* __restrict__ areas, int nareas,
                                                 uint32_t nitems, int32_t* __restrict__ item_ti,
                                                 float* __restrict__ item_t, Counters* __restrict__ cnt,
-                                                int zero_counters)
+                                                int zero_counters, int lds_cap, int* __restrict__ stk_ovf)
 {
     extern __shared__ int s_stk[];
     const uint32_t item = blockIdx.x * RB + threadIdx.x;
@@ -115,7 +115,8 @@ __global__ __launch_bounds__(RB) void k_primary(const RenderScene* __restrict__ 
     if (item == 0 && zero_counters) *cnt = Counters{};
     if (item >= nitems) return;
     LaneStack st;
-    st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = RB; st.sp = 0;
+    st.init(s_stk, threadIdx.x, RB);
+    st.spill_to(lds_cap, stk_ovf + item, gridDim.x * RB);
     const Area a = areas[find_area(areas, nareas, item)];
     const RenderScene& R = scenes[a.scene];
     int i, j;
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(RB) void k_direct(const RenderScene* __restrict__ s
                                                const float* __restrict__ item_t,
                                                const uint32_t* __restrict__ lcg_off,
                                                c3* __restrict__ lightc, int nl_max,
-                                               Counters* __restrict__ cnt)
+                                               Counters* __restrict__ cnt, int lds_cap, int* __restrict__ stk_ovf)
 {
     extern __shared__ int s_stk[];
     const uint32_t item = blockIdx.x * RB + threadIdx.x;
@@ -190,7 +191,8 @@ __global__ __launch_bounds__(RB) void k_direct(const RenderScene* __restrict__ s
         const int ti = item_ti[item];
         if (ti >= 0) {
             LaneStack st;
-            st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = RB; st.sp = 0;
+            st.init(s_stk, threadIdx.x, RB);
+            st.spill_to(lds_cap, stk_ovf + item, gridDim.x * RB);
             const Area a = areas[find_area(areas, nareas, item)];
             const RenderScene& R = scenes[a.scene];
             f3 eye;
@@ -321,7 +323,8 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WP
     const int32_t* __restrict__ st_ti, const c3* __restrict__ st_col, float* __restrict__ st_s,
     c3* __restrict__ contri, uint8_t* __restrict__ nseg, f3* __restrict__ h_p, f3* __restrict__ h_ray,
     int32_t* __restrict__ h_ti, uint32_t* __restrict__ qcount, uint32_t* __restrict__ queue,
-    uint32_t* __restrict__ flag, const uint32_t* __restrict__ hbase, Counters* __restrict__ cnt)
+    uint32_t* __restrict__ flag, const uint32_t* __restrict__ hbase, Counters* __restrict__ cnt,
+    int lds_cap, int* __restrict__ stk_ovf)
 {
     extern __shared__ int s_stk[];
     const uint32_t idx = blockIdx.x * RB + threadIdx.x;
@@ -332,7 +335,8 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WP
     // or k_compact); the grid covers every path
     if (idx < *nact) {
         LaneStack st;
-        st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = RB; st.sp = 0;
+        st.init(s_stk, threadIdx.x, RB);
+        st.spill_to(lds_cap, stk_ovf + idx, gridDim.x * RB);
         p = active[idx];
         flag[p] = 0;  // set again by k_shade if the path continues
         const uint32_t item = p / (uint32_t)S;
@@ -401,14 +405,16 @@ __global__ __launch_bounds__(RB) void k_shadow(const RenderScene* __restrict__ s
                                                const uint32_t* __restrict__ qcount,
                                                const uint32_t* __restrict__ queue,
                                                const f3* __restrict__ h_p, const int32_t* __restrict__ h_ti,
-                                               uint32_t* __restrict__ vis, Counters* __restrict__ cnt)
+                                               uint32_t* __restrict__ vis, Counters* __restrict__ cnt,
+                                               int lds_cap, int* __restrict__ stk_ovf)
 {
     extern __shared__ int s_stk[];
     const uint32_t idx = blockIdx.x * RB + threadIdx.x;
     uint32_t nshadow = 0;
     if (idx < *qcount) {
         LaneStack st;
-        st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = RB; st.sp = 0;
+        st.init(s_stk, threadIdx.x, RB);
+        st.spill_to(lds_cap, stk_ovf + idx, gridDim.x * RB);
         const uint32_t p = queue[idx];
         const Area& a = areas[find_area(areas, nareas, p / (uint32_t)S)];
         const RenderScene& R = scenes[a.scene];
@@ -942,7 +948,12 @@ struct RenderWS {
     uint32_t nitems = 0, npaths = 0;
     int S = 0, D = 0, na = 0, nl_max = 0;
     bool any_path = false;
-    size_t stk_bytes = 0;  // dynamic LDS of the traversal launches
+    size_t stk_bytes = 0;      // dynamic LDS of the traversal launches
+    size_t stk_bytes_lvl = 0;  // ... of the frame launches (camera, direct, GI levels)
+    int stk_entries = 1;       // stack bound of the launch's trees
+    int stk_lds = 0;           // entries the frame launches keep in LDS (0: all)
+    int lds_cap = 0;           // lds_cap argument of those launches (0: no overflow)
+    DevBuf<int> stk_ovf;       // their stack entries beyond the LDS part
     int width = 2;         // BVH width of the scenes in the launch
     bool any_area = false, any_photon = false;
     // antialiasing passes: first-pass areas and colours, the pass's samples
@@ -1191,15 +1202,16 @@ static int trace_levels(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
             YG_HIP(hipEventRecord(ev[0], st));
         }
         uint32_t* qc = W.qcnt.p + j;
-        YG_LAUNCH_W(W.width, (k_ext<ADJ, 4>), (k_ext<ADJ, 2>), dim3(grid), dim3(RB), W.stk_bytes, st, W.scenes.p,
+        YG_LAUNCH_W(W.width, (k_ext<ADJ, 4>), (k_ext<ADJ, 2>), dim3(grid), dim3(RB), W.stk_bytes_lvl, st, W.scenes.p,
                     W.areas.p, na, j, act, W.nact.p + j, npaths, S, D, W.item_ti.p, W.item_t.p, W.qoff.p,
                     W.hidx.p, W.st_where.p, W.st_hn.p, W.st_ti.p, W.st_col.p, W.st_s.p, W.contri.p, W.nseg.p,
                     W.h_p.p, W.h_ray.p, W.h_ti.p, qc, W.queue.p, W.flag.p, W.hbase_on ? W.hbase.p : nullptr,
-                    W.cnt.p);
+                    W.cnt.p, W.lds_cap, W.stk_ovf.p);
         YG_HIP(hipGetLastError());
         if (ktiming) YG_HIP(hipEventRecord(ev[1], st));
-        YG_LAUNCH_W(W.width, (k_shadow<ADJ, 4>), (k_shadow<ADJ, 2>), dim3(grid), dim3(RB), W.stk_bytes, st,
-                    W.scenes.p, W.areas.p, na, S, qc, W.queue.p, W.h_p.p, W.h_ti.p, W.vis.p, W.cnt.p);
+        YG_LAUNCH_W(W.width, (k_shadow<ADJ, 4>), (k_shadow<ADJ, 2>), dim3(grid), dim3(RB), W.stk_bytes_lvl, st,
+                    W.scenes.p, W.areas.p, na, S, qc, W.queue.p, W.h_p.p, W.h_ti.p, W.vis.p, W.cnt.p,
+                    W.lds_cap, W.stk_ovf.p);
         YG_HIP(hipGetLastError());
         if (ktiming) YG_HIP(hipEventRecord(ev[2], st));
         const unsigned sgrid = ADJ ? std::min(grid_for(npaths, SB), ADJ_GRID) : grid_for(npaths, SB);
@@ -1267,6 +1279,17 @@ static int ensure_frame_buffers(uint32_t nitems, uint32_t npaths, int nl_max, bo
             (rc = W.nact.ensure(D)))
             return rc;
     }
+    // the frame launches keep W.stk_lds entries per lane in LDS and the rest
+    // in global memory, one slot per thread of the largest of them
+    W.stk_bytes_lvl = stack_lds_bytes(W.stk_entries, RB);
+    W.lds_cap = 0;
+    if (W.stk_lds > 0 && W.stk_entries > W.stk_lds) {
+        const size_t threads = (size_t)grid_for(std::max<uint64_t>(nitems, any_path ? npaths : 0), RB) * RB;
+        if ((rc = W.stk_ovf.ensure((size_t)(W.stk_entries - W.stk_lds) * threads))) return rc;
+        W.stk_bytes_lvl = stack_lds_bytes(W.stk_lds, RB);
+        W.lds_cap = W.stk_lds;
+    }
+    if ((rc = W.stk_ovf.ensure(1))) return rc;  // a valid pointer for launches that do not spill
     // hipcub scan storage for both scans of the launch
     size_t b1 = 0, b2 = 0;
     if ((rc = scan_bytes(nitems, b1, st)) || (rc = scan_bytes(any_path ? npaths : 0, b2, st)) ||
@@ -1285,16 +1308,17 @@ static int launch_frame(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
     const uint32_t nitems = W.nitems;
     const int na = W.na, nl_max = W.nl_max;
     int rc;
-    YG_LAUNCH_W(W.width, (k_primary<4>), (k_primary<2>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes, st,
-                W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.cnt.p, W.aa_pass < 0 ? 1 : 0);
+    YG_LAUNCH_W(W.width, (k_primary<4>), (k_primary<2>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes_lvl, st,
+                W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.cnt.p, W.aa_pass < 0 ? 1 : 0,
+                W.lds_cap, W.stk_ovf.p);
     YG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_prep, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
                        nitems, W.item_ti.p, W.item_t.p, W.draws.p, W.qual.p);
     YG_HIP(hipGetLastError());
     if (W.any_area && (rc = exclusive_scan(W.draws.p, W.lcg_off.p, nitems, st))) return rc;
-    YG_LAUNCH_W(W.width, (k_direct<4>), (k_direct<2>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes, st,
+    YG_LAUNCH_W(W.width, (k_direct<4>), (k_direct<2>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes_lvl, st,
                 W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lcg_off.p, W.lightc.p, nl_max,
-                W.cnt.p);
+                W.cnt.p, W.lds_cap, W.stk_ovf.p);
     YG_HIP(hipGetLastError());
     if (W.any_photon) {
         hipLaunchKernelGGL(k_photon_direct, dim3(grid_for(nitems, 64)), dim3(64), 0, st, W.scenes.p,
@@ -1438,6 +1462,15 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
             aa_passes = P.aa_passes;
             aa_threshold = P.aa_threshold;
         }
+    int max_depth = 1;  // stack entries
+    for (auto& P : plans) {
+        max_depth = std::max(max_depth, P.s->stack_entries);
+        if (P.s->width != plans[0].s->width)
+            return yg_fail(YG_ERR_UNSUPPORTED, "batched scenes need BVHs of one width");
+    }
+    W.stk_entries = max_depth;
+    W.stk_lds = 0;
+    for (auto& P : plans) W.stk_lds = std::max(W.stk_lds, P.s->stack_lds);
     const std::vector<Area> areas0(areas);  // the first pass's areas
     append_area_index(areas, nitems);
     if ((rc = W.scenes.ensure(scenes.size())) || (rc = W.areas.ensure(areas.size()))) return rc;
@@ -1447,12 +1480,6 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         (rc = upload_cached(W.ajh, ajh, W.up_ajh, st)) || (rc = upload_cached(W.ajv, ajv, W.up_ajv, st)))
         return rc;
 
-    int max_depth = 1;  // stack entries
-    for (auto& P : plans) {
-        max_depth = std::max(max_depth, P.s->stack_entries);
-        if (P.s->width != plans[0].s->width)
-            return yg_fail(YG_ERR_UNSUPPORTED, "batched scenes need BVHs of one width");
-    }
     W.stk_bytes = stack_lds_bytes(max_depth, RB);
     W.width = plans[0].s->width;
     W.nitems = nitems;
@@ -1753,7 +1780,7 @@ __global__ __launch_bounds__(RB) void k_adjoint_direct(const RenderScene* __rest
                 const float* gp = grad_rgb + 3 * ((size_t)i * R.resx + j);
                 const c3 g = mkc(gp[0], gp[1], gp[2]);
                 LaneStack st;
-                st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = RB; st.sp = 0;
+                st.init(s_stk, threadIdx.x, RB);
                 f3 eye;
                 const Surf sp = primary_surf(R, a, item, item_ti[item], item_t[item], eye);
                 const RhoD r0 = diffuse_d(R.mats, R.mods, R.texs, sp.mat, sp.P);
@@ -1817,7 +1844,7 @@ __global__ __launch_bounds__(RB) void k_adjoint_path(const RenderScene* __restri
             const float* gp = grad_rgb + 3 * ((size_t)pi * R.resx + pj);
             const c3 g = mkc(gp[0], gp[1], gp[2]);
             LaneStack st;
-            st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = RB; st.sp = 0;
+            st.init(s_stk, threadIdx.x, RB);
             f3 eye;
             const Surf sp0 = primary_surf(R, a, item, item_ti[item], item_t[item], eye);
             const RhoD r0 = diffuse_d(R.mats, R.mods, R.texs, sp0.mat, sp0.P);

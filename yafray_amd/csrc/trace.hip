@@ -17,7 +17,7 @@ __global__ __launch_bounds__(TB) void k_trace_closest(BVHView bvh, const yg_ray*
     const uint32_t i = blockIdx.x * TB + threadIdx.x;
     if (i >= n) return;
     LaneStack st;
-    st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = TB; st.sp = 0;
+    st.init(s_stk, threadIdx.x, TB);
     const yg_ray r = rays[i];
     const f3 o = mk3(r.ox, r.oy, r.oz), d = mk3(r.dx, r.dy, r.dz);
     float t = r.tmax;
@@ -49,7 +49,7 @@ __global__ __launch_bounds__(TB) void k_trace_any(BVHView bvh, const yg_ray* __r
     const uint32_t i = blockIdx.x * TB + threadIdx.x;
     if (i >= n) return;
     LaneStack st;
-    st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = TB; st.sp = 0;
+    st.init(s_stk, threadIdx.x, TB);
     const yg_ray r = rays[i];
     const f3 o = mk3(r.ox, r.oy, r.oz), d = mk3(r.dx, r.dy, r.dz);
     const float blo = r.tmin - 1e-4f * fabsf(r.tmin) - 1e-30f;
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(TB) void k_first_hit(BVHView bvh, const float* __re
     const uint32_t i = blockIdx.x * TB + threadIdx.x;
     if (i >= n) return;
     LaneStack st;
-    st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = TB; st.sp = 0;
+    st.init(s_stk, threadIdx.x, TB);
     const f3 fr = ld3(from + 3 * (size_t)i), d = ld3(dir + 3 * (size_t)i);
     float t;
     const int ti = first_hit<W>(bvh, fr, d, min_raydis, t, st);
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(TB) void k_is_shadowed(BVHView bvh, const float* __
     const uint32_t i = blockIdx.x * TB + threadIdx.x;
     if (i >= n) return;
     LaneStack st;
-    st.lds = s_stk; st.lane = threadIdx.x; st.nlanes = TB; st.sp = 0;
+    st.init(s_stk, threadIdx.x, TB);
     const int so = self_obj ? self_obj[i] : -1;
     out[i] = is_shadowed<W>(bvh, ld3(p + 3 * (size_t)i), ld3(l + 3 * (size_t)i), so, to_point != 0,
                          min_raydis, self_bias, st) ? 1 : 0;

@@ -96,11 +96,36 @@ __device__ __forceinline__ void load_tri(const float4* tris, int ti, f3& a, f3& 
 // Per-lane traversal stack in LDS: entry k of lane t at lds[k*nlanes + t]
 // (conflict free).  Launches size the dynamic LDS for the deepest tree they
 // traverse, so no entry ever goes to private (scratch) memory.
+// A lane's traversal stack: the first `lcap` entries in LDS ([entry][lane],
+// conflict free), deeper ones (rare) in global memory ([entry][thread], one
+// slot per thread of the launch).  A short LDS part lets more waves share a
+// compute unit; lcap = INT_MAX keeps the whole stack in LDS.
 struct LaneStack {
     int* lds;
     int lane, nlanes, sp;
-    __device__ void push(int v) { lds[sp * nlanes + lane] = v; sp++; }
-    __device__ int pop() { sp--; return lds[sp * nlanes + lane]; }
+    int lcap = 0x7fffffff;
+    int* ovf = nullptr;
+    uint32_t ostride = 0;
+    __device__ void init(int* l, int ln, int nl)
+    {
+        lds = l; lane = ln; nlanes = nl; sp = 0;
+    }
+    // entries from cap on go to ovf + (entry - cap) * stride (cap 0: none)
+    __device__ void spill_to(int cap, int* o, uint32_t stride)
+    {
+        if (cap > 0 && o) { lcap = cap; ovf = o; ostride = stride; }
+    }
+    __device__ void push(int v)
+    {
+        if (sp < lcap) lds[sp * nlanes + lane] = v;
+        else ovf[(size_t)(sp - lcap) * ostride] = v;
+        sp++;
+    }
+    __device__ int pop()
+    {
+        sp--;
+        return (sp < lcap) ? lds[sp * nlanes + lane] : ovf[(size_t)(sp - lcap) * ostride];
+    }
 };
 
 struct RayBox {

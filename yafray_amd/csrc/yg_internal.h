@@ -35,9 +35,14 @@ struct alignas(128) GNode4 {
 #ifndef YG_DEFAULT_WIDTH
 #define YG_DEFAULT_WIDTH 4
 #endif
-// Deepest traversal stack the kernels accept.  Each lane's stack lives in
-// LDS; a launch reserves the deepest stack its trees can need (one entry per
-// level of a two-wide tree, up to three per level of a four-wide one).
+#ifndef YG_DEFAULT_STACK_LDS
+#define YG_DEFAULT_STACK_LDS 16  // render launches: stack entries per lane in LDS
+#endif
+// Deepest traversal stack the kernels accept.  A launch sizes each lane's
+// stack for the deepest its trees can need (one entry per level of a
+// two-wide tree, up to three per level of a four-wide one): in LDS, or with
+// the render launches the first stack_lds entries in LDS and the rest in
+// device memory.
 #define YG_MAX_STACK 200
 
 // Dynamic LDS bytes of a traversal launch with `lanes` threads per block and
@@ -105,6 +110,7 @@ struct yg_scene {
     int32_t max_depth = 0;
     int32_t width = 2;
     int32_t stack_entries = 1;  // traversal stack entries per lane the tree can need
+    int32_t stack_lds = YG_DEFAULT_STACK_LDS;  // of those, kept in LDS by render launches (0: all)
     uint64_t n_nodes = 0;
     GNode* d_nodes = nullptr;
     GNode4* d_nodes4 = nullptr;

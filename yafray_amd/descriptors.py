@@ -78,7 +78,7 @@ class yg_render_stats(C.Structure):
 
 class yg_build_opts(C.Structure):
     _fields_ = [("max_leaf", C.c_int32), ("sah_bins", C.c_int32), ("threads", C.c_int32),
-                ("width", C.c_int32)]
+                ("width", C.c_int32), ("stack_lds", C.c_int32)]
 
 
 YG_TUNE_MAX = 32
@@ -86,7 +86,7 @@ YG_TUNE_MAX = 32
 
 class yg_tune_opts(C.Structure):
     _fields_ = [("widths", C.c_int32), ("leaves", C.c_int32), ("bins", C.c_int32),
-                ("repeats", C.c_int32), ("threads", C.c_int32)]
+                ("repeats", C.c_int32), ("threads", C.c_int32), ("stacks", C.c_int32)]
 
 
 class yg_tune_result(C.Structure):

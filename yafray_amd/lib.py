@@ -125,8 +125,10 @@ class Scene:
                                 C.byref(oid)), "yg_mesh_add")
         return oid.value
 
-    def build(self, max_leaf=4, bins=32, threads=0, width=0):
-        o = D.yg_build_opts(max_leaf, bins, threads, width)
+    def build(self, max_leaf=4, bins=32, threads=0, width=0, stack_lds=0):
+        """stack_lds: stack entries per lane the render launches keep in LDS
+        (0 = library default, -1 = the whole stack)."""
+        o = D.yg_build_opts(max_leaf, bins, threads, width, stack_lds)
         check(lib().yg_build(self.h, C.byref(o)), "yg_build")
 
     def info(self):
@@ -204,10 +206,12 @@ class Scene:
         return st
 
     def autotune(self, out_rgb, opts=None, widths=(2, 4), leaves=(2, 4, 8), bins=(32,), repeats=2,
-                 threads=0):
-        """Rebuild with every (width, max_leaf, bins) candidate, time the frame
-        of `opts` into out_rgb, keep the fastest build.  Returns
-        (best yg_build_opts, best ms, [(width, leaf, bins, ms, build_ms), ...])."""
+                 threads=0, stacks=()):
+        """Rebuild with every (width, max_leaf, bins) candidate and try every
+        LDS stack part in `stacks` (entries, -1 = whole stack; () = the
+        default), time the frame of `opts` into out_rgb, keep the fastest.
+        Returns (best yg_build_opts, best ms,
+        [(width, leaf, bins, ms, build_ms, stack_lds), ...])."""
         o = opts if opts is not None else self.desc.opts
         t = D.yg_tune_opts()
         t.widths = sum(1 << w for w in widths)
@@ -217,11 +221,18 @@ class Scene:
             raise ValueError("SAH bin counts must be powers of two")
         t.repeats = repeats
         t.threads = threads
+        for v in stacks:
+            if v == -1:
+                t.stacks |= 1
+            elif v in (8, 16, 32, 64):
+                t.stacks |= 1 << (v.bit_length() - 1)
+            else:
+                raise ValueError("stack_lds candidates: -1, 8, 16, 32, 64")
         r = D.yg_tune_result()
         check(lib().yg_autotune(self.h, C.byref(o), _ptr(out_rgb), C.byref(t), C.byref(r), _stream()),
               "yg_autotune")
-        cands = [(r.cand[k].width, r.cand[k].max_leaf, r.cand[k].sah_bins, r.cand_ms[k], r.cand_build_ms[k])
-                 for k in range(r.ncand)]
+        cands = [(r.cand[k].width, r.cand[k].max_leaf, r.cand[k].sah_bins, r.cand_ms[k], r.cand_build_ms[k],
+                  r.cand[k].stack_lds) for k in range(r.ncand)]
         return r.best, r.best_ms, cands
 
     def render_adjoint(self, grad_rgb, out_rgb, gmat, gmod, glight, opts=None):
