@@ -100,9 +100,24 @@ __device__ __forceinline__ void block_add(unsigned long long* dst, uint32_t v)
 
 // ------------------------------------------------------------- kernels --
 
+// Minimum waves per SIMD the compiler must leave room for (register budget)
+// in the traversal kernels; 1 = no constraint.
+// k_primary fits 8 waves (64 VGPRs) and the shadow-ray kernels 6 (79) without
+// spilling; tighter budgets spill to scratch.
+#ifndef YG_PRIMARY_WPE
+#define YG_PRIMARY_WPE 8
+#endif
+#ifndef YG_DIRECT_WPE
+#define YG_DIRECT_WPE 1
+#endif
+#ifndef YG_SHADOW_WPE
+#define YG_SHADOW_WPE 6
+#endif
+
 // Camera rays and their closest hits (scene_t::raytrace, scene.cc:200-235).
 template <int W>
-__global__ __launch_bounds__(RB) void k_primary(const RenderScene* __restrict__ scenes,
+__global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_PRIMARY_WPE))) void k_primary(
+                                                const RenderScene* __restrict__ scenes,
                                                 const Area* __restrict__ areas, int nareas,
                                                 uint32_t nitems, int32_t* __restrict__ item_ti,
                                                 float* __restrict__ item_t, Counters* __restrict__ cnt,
@@ -175,14 +190,52 @@ __global__ __launch_bounds__(RB) void k_prep(const RenderScene* __restrict__ sce
 }
 
 // Sun and area lights at the primary hit, one colour per light.
+// Sun shadows of the camera rays' hits for k_direct<W, false>: bit li of
+// sun_vis[item] is set when sun light li (casting shadows, used in render)
+// is blocked (scene_t::isShadowed, direction form, scene.cc:148-185).
 template <int W>
-__global__ __launch_bounds__(RB) void k_direct(const RenderScene* __restrict__ scenes,
+__global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_SHADOW_WPE))) void k_sun_vis(
+    const RenderScene* __restrict__ scenes, const Area* __restrict__ areas, int nareas, uint32_t nitems,
+    const int32_t* __restrict__ item_ti, const float* __restrict__ item_t, uint32_t* __restrict__ sun_vis,
+    Counters* __restrict__ cnt, int lds_cap, int* __restrict__ stk_ovf)
+{
+    extern __shared__ int s_stk[];
+    const uint32_t item = blockIdx.x * RB + threadIdx.x;
+    uint32_t nshadow = 0, mask = 0;
+    if (item < nitems && item_ti[item] >= 0) {
+        LaneStack st;
+        st.init(s_stk, threadIdx.x, RB);
+        st.spill_to(lds_cap, stk_ovf + item, gridDim.x * RB);
+        const Area a = areas[find_area(areas, nareas, item)];
+        const RenderScene& R = scenes[a.scene];
+        f3 eye;
+        const Surf sp = primary_surf(R, a, item, item_ti[item], item_t[item], eye);
+        for (int li = 0; li < R.nl && li < 32; li++) {
+            const DevLight& L = R.lights[li];
+            if (!L.use_in_render || L.type != YG_LIGHT_SUN || !L.cast_shadows) continue;
+            nshadow++;
+            if (is_shadowed<W>(R.bvh, sp.P, L.dir, sp.obj, false, R.min_raydis, R.self_bias, st)) mask |= 1u << li;
+        }
+    }
+    if (item < nitems) sun_vis[item] = mask;
+    block_add(&cnt->shadow, nshadow);
+}
+
+// Light sum of the camera rays' hits (scene_t::light, scene.cc:259-277).
+// AREA: the launch has area lights, whose samples take ourRandom() draws
+// here and trace their shadow rays here, as do the sun lights.  Without area
+// lights the sun shadows come from k_sun_vis and this kernel only shades
+// (no traversal: far fewer registers).
+template <int W, bool AREA>
+__global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_DIRECT_WPE))) void k_direct(
+                                               const RenderScene* __restrict__ scenes,
                                                const Area* __restrict__ areas, int nareas,
                                                uint32_t nitems, const int32_t* __restrict__ item_ti,
                                                const float* __restrict__ item_t,
                                                const uint32_t* __restrict__ lcg_off,
                                                c3* __restrict__ lightc, int nl_max,
-                                               Counters* __restrict__ cnt, int lds_cap, int* __restrict__ stk_ovf)
+                                               Counters* __restrict__ cnt, int lds_cap, int* __restrict__ stk_ovf,
+                                               const uint32_t* __restrict__ sun_vis)
 {
     extern __shared__ int s_stk[];
     const uint32_t item = blockIdx.x * RB + threadIdx.x;
@@ -191,8 +244,10 @@ __global__ __launch_bounds__(RB) void k_direct(const RenderScene* __restrict__ s
         const int ti = item_ti[item];
         if (ti >= 0) {
             LaneStack st;
-            st.init(s_stk, threadIdx.x, RB);
-            st.spill_to(lds_cap, stk_ovf + item, gridDim.x * RB);
+            if (AREA) {
+                st.init(s_stk, threadIdx.x, RB);
+                st.spill_to(lds_cap, stk_ovf + item, gridDim.x * RB);
+            }
             const Area a = areas[find_area(areas, nareas, item)];
             const RenderScene& R = scenes[a.scene];
             f3 eye;
@@ -203,8 +258,11 @@ __global__ __launch_bounds__(RB) void k_direct(const RenderScene* __restrict__ s
                 const DevLight& L = R.lights[li];
                 c3 c = mkc(0, 0, 0);
                 if (L.use_in_render) {
-                    if (L.type == YG_LIGHT_SUN) c = sun_illuminate<W>(R, L, sp, eye, st, nshadow);
-                    else if (L.type == YG_LIGHT_AREA) {
+                    if (L.type == YG_LIGHT_SUN) {
+                        if (AREA) c = sun_illuminate<W>(R, L, sp, eye, st, nshadow);
+                        else if (!(L.cast_shadows && li < 32 && ((sun_vis[item] >> li) & 1u)))
+                            c = from_light(R, sp, L.dir, L.col, eye);  // sunLight_t::illuminate (sunlight.cc:35-57)
+                    } else if (AREA && L.type == YG_LIGHT_AREA) {
                         if (!seeded) {
                             seed = lcg_jump(R.lcg_seed, (uint64_t)(lcg_off[item] - lcg_off[R.item_base]), YG_LCG_POW);
                             seeded = true;
@@ -400,7 +458,8 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WP
 }
 
 template <bool ADJ, int W>
-__global__ __launch_bounds__(RB) void k_shadow(const RenderScene* __restrict__ scenes,
+__global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_SHADOW_WPE))) void k_shadow(
+                                               const RenderScene* __restrict__ scenes,
                                                const Area* __restrict__ areas, int nareas, int S,
                                                const uint32_t* __restrict__ qcount,
                                                const uint32_t* __restrict__ queue,
@@ -925,6 +984,7 @@ struct RenderWS {
     DevBuf<int32_t> item_ti;
     DevBuf<float> item_t;
     DevBuf<uint32_t> draws, lcg_off, qual, qoff;
+    DevBuf<uint32_t> sun_vis;  // camera-ray hits: blocked sun lights (bit per light)
     DevBuf<c3> lightc;
     DevBuf<uint32_t> active0, active1, flag, ex, hidx;
     DevBuf<f3> st_where, st_hn;
@@ -1265,6 +1325,7 @@ static int ensure_frame_buffers(uint32_t nitems, uint32_t npaths, int nl_max, bo
     RenderWS& W = g_ws;
     int rc;
     if ((rc = W.item_ti.ensure(nitems)) || (rc = W.item_t.ensure(nitems)) || (rc = W.draws.ensure(nitems)) ||
+        (rc = W.sun_vis.ensure(nitems)) ||
         (rc = W.lcg_off.ensure(nitems)) || (rc = W.qual.ensure(nitems)) || (rc = W.qoff.ensure(nitems)) ||
         (rc = W.lightc.ensure((size_t)nitems * nl_max)) || (rc = W.cnt.ensure(1)))
         return rc;
@@ -1316,9 +1377,21 @@ static int launch_frame(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
                        nitems, W.item_ti.p, W.item_t.p, W.draws.p, W.qual.p);
     YG_HIP(hipGetLastError());
     if (W.any_area && (rc = exclusive_scan(W.draws.p, W.lcg_off.p, nitems, st))) return rc;
-    YG_LAUNCH_W(W.width, (k_direct<4>), (k_direct<2>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes_lvl, st,
-                W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lcg_off.p, W.lightc.p, nl_max,
-                W.cnt.p, W.lds_cap, W.stk_ovf.p);
+    if (W.any_area || nl_max > 32) {
+        // registers hold this kernel to 2 waves per SIMD, so LDS has room for
+        // the whole stack
+        YG_LAUNCH_W(W.width, (k_direct<4, true>), (k_direct<2, true>), dim3(grid_for(nitems, RB)), dim3(RB),
+                    W.stk_bytes, st, W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lcg_off.p,
+                    W.lightc.p, nl_max, W.cnt.p, 0, W.stk_ovf.p, (const uint32_t*)nullptr);
+    } else {
+        YG_LAUNCH_W(W.width, (k_sun_vis<4>), (k_sun_vis<2>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes_lvl,
+                    st, W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.sun_vis.p, W.cnt.p,
+                    W.lds_cap, W.stk_ovf.p);
+        YG_HIP(hipGetLastError());
+        YG_LAUNCH_W(W.width, (k_direct<4, false>), (k_direct<2, false>), dim3(grid_for(nitems, RB)), dim3(RB), 0,
+                    st, W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lcg_off.p, W.lightc.p,
+                    nl_max, W.cnt.p, 0, W.stk_ovf.p, (const uint32_t*)W.sun_vis.p);
+    }
     YG_HIP(hipGetLastError());
     if (W.any_photon) {
         hipLaunchKernelGGL(k_photon_direct, dim3(grid_for(nitems, 64)), dim3(64), 0, st, W.scenes.p,
