@@ -336,6 +336,52 @@ def bench_aa(args):
     }
 
 
+def bench_rays(args):
+    """Ray-query entry points on the 1,045,506-triangle terrain: 4M incoherent
+    rays per batch (origins above the terrain, random downward-biased
+    directions), closest hit (kdTree_t::Intersect), any hit (IntersectS) and
+    the scene_t::firstHit / isShadowed forms, device arrays in HBM."""
+    import numpy as np
+    import torch
+
+    from yafray_amd import lib as Y
+    from yafray_amd import scenes as S
+
+    verts, idx, band = S.terrain_mesh(1024, 512, Y.device_heights)
+    sc = Y.Scene()
+    sc.add_mesh(verts, idx, band)
+    sc.build()
+    n = 1 << 22
+    rng = np.random.default_rng(7)
+    o = np.stack([rng.uniform(0, 2, n), rng.uniform(0, 1, n), rng.uniform(0.3, 0.8, n)], 1).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d[:, 2] = -np.abs(d[:, 2]) - 0.2
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3], rays[:, 4:7], rays[:, 7] = o, d, np.inf
+    dr = torch.from_numpy(rays).cuda()
+    hits = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    occ = torch.empty(n, dtype=torch.uint8, device="cuda")
+    frm, dd = torch.from_numpy(o).cuda(), torch.from_numpy(d).cuda()
+    sun = torch.from_numpy(np.tile(np.float32([0.53, 0.27, 0.80]), (n, 1))).cuda()
+    self_obj = torch.zeros(n, dtype=torch.int32, device="cuda")
+    res = {}
+    for name, fn in (("closest", lambda: sc.trace_closest(dr, hits)),
+                     ("any", lambda: sc.trace_any(dr, occ)),
+                     ("first_hit", lambda: sc.first_hit(frm, dd, hits)),
+                     ("is_shadowed_dir", lambda: sc.is_shadowed(frm, sun, self_obj, occ, False))):
+        dt, _ = _timed(fn, args.steps, args.warmup)
+        res[name] = {"ms": round(dt * 1e3, 3), "mrays_per_s": round(n / dt / 1e6, 1)}
+    return {
+        "metric": "ray queries Mrays/s (closest hit)", "value": res["closest"]["mrays_per_s"], "unit": "Mrays/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["closest"]["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "4,194,304 incoherent rays per batch on the 1,045,506-triangle fBm terrain "
+                               "(BVH width 4)"},
+        "entry_points": res,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -343,7 +389,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="gi",
-                    choices=["gi", "batched", "adjoint", "photon", "direct", "autotune", "aa"])
+                    choices=["gi", "batched", "adjoint", "photon", "direct", "autotune", "aa", "rays"])
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy after the timed region (counter calibration)")
@@ -356,7 +402,8 @@ def main():
         import torch
         torch.cuda.set_device(0)
         fn = {"batched": bench_batched, "adjoint": bench_adjoint, "photon": bench_photon,
-              "direct": bench_direct, "autotune": bench_autotune, "aa": bench_aa}[args.workload]
+              "direct": bench_direct, "autotune": bench_autotune, "aa": bench_aa,
+              "rays": bench_rays}[args.workload]
         print(json.dumps(fn(args)))
         return
 
