@@ -395,8 +395,8 @@ def main():
                     help="run a 1 GiB device copy after the timed region (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(rendering would shard by image tiles; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(rendering would shard by image tiles over GPUs)", file=sys.stderr)
         sys.exit(2)
     if args.workload != "gi":
         import torch
