@@ -72,8 +72,9 @@ public:
     int addObject_trimesh(const std::string& name, const std::vector<float>& verts,
                           const std::vector<uint32_t>& faces, const std::vector<std::string>& shaders,
                           bool cast_shadows = true, bool recv_rad = true, bool use_rad = true);
-    // render params: camera_name, background_name, bias, raydepth, AA_*; rgb gets
-    // resx*resy*3 floats (HDR, no tone mapping).  Photon lights are built first.
+    // render params: camera_name, background_name, bias, raydepth, AA_*, exposure,
+    // gamma, clamp_rgb, output_type; rgb gets resx*resy*3 floats, tone-mapped
+    // unless output_type is hdr or exr.  Photon lights are built first.
     int render(const paramMap_t& params, std::vector<float>& rgb, yg_render_stats* stats = nullptr);
 
     // the render options render() passes to yg_render for these params
