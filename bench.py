@@ -73,10 +73,45 @@ def parity_leg(d, img, cpu_threads):
     cpu_dt = time.perf_counter() - t0
     cpu_rays = int(rs[0] + rs[1] + rs[2])
     img = img.reshape(ref.shape).astype(np.float64)
-    ref = ref.astype(np.float64)
-    rel = float(np.sqrt(np.mean((img - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)))
-    exact = float(np.mean(np.all(img == ref, axis=2)))
-    return {"cpu_dt": cpu_dt, "cpu_rays": cpu_rays, "rel_rmse": rel, "exact_px": exact}
+    ref64 = ref.astype(np.float64)
+    rel = float(np.sqrt(np.mean((img - ref64) ** 2)) / np.sqrt(np.mean(ref64 ** 2)))
+    exact = float(np.mean(np.all(img == ref64, axis=2)))
+    return {"cpu_dt": cpu_dt, "cpu_rays": cpu_rays, "rel_rmse": rel, "exact_px": exact,
+            "oracle": oc, "ref": ref}
+
+
+FP64_SAMPLES = 4096
+
+
+def cpu_cores():
+    return min(os.cpu_count() or 1, 16)
+
+
+def fp64_leg(d, images, n_pix, oc=None, seed=7):
+    """Error against the converged FP64 render.  A bounded sample: `n_pix`
+    pixels drawn with a fixed seed, each rendered through its centre with 4096
+    samples of every sampled light (pathlight paths, area-light points) in
+    double precision by the oracle (yo_render_fp64); each image in `images`
+    ([resy, resx, 3] or flat) is compared with it on those pixels."""
+    import numpy as np
+
+    from oracle import pyoracle as po
+
+    if oc is None:
+        oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    rx, ry = d.cam.resx, d.cam.resy
+    q = np.random.default_rng(seed).choice(rx * ry, size=min(n_pix, rx * ry), replace=False)
+    px, py = (q % rx).astype(np.int32), (q // rx).astype(np.int32)
+    threads = cpu_cores()
+    t0 = time.perf_counter()
+    conv = oc.render_fp64(d, samples=FP64_SAMPLES, px=px, py=py, seed=seed, threads=threads)
+    dt = time.perf_counter() - t0
+    out = {"samples": FP64_SAMPLES, "pixels": int(len(q)), "cpu_s": round(dt, 1), "threads": threads}
+    den = np.sqrt(np.mean(conv ** 2))
+    for name, img in images.items():
+        v = np.asarray(img, np.float64).reshape(ry, rx, 3)[py, px]
+        out["rel_rmse_" + name] = float(np.sqrt(np.mean((v - conv) ** 2)) / den)
+    return out
 
 
 def copy_probe():
@@ -143,6 +178,18 @@ def bench_batched(args):
     bat_dt, st = _timed(lambda: Y.render_batch(scenes, outs), args.steps, args.warmup)
     same = all(torch.equal(a, b) for a, b in zip(ref, outs))
     rays = st.rays_camera + st.rays_extension + st.rays_shadow
+    fp64 = None
+    if not args.no_cpu:
+        # the first scene of each resolution, 256 pixels each
+        fp64, seen = [], set()
+        for s, o in zip(scenes, outs):
+            key = (s.desc.cam.resx, s.desc.cam.resy)
+            if key in seen:
+                continue
+            seen.add(key)
+            e = fp64_leg(s.desc, {"gpu": o.cpu().numpy()}, 256)
+            e["scene"] = "%dx%d, %d tris" % (key[0], key[1], s.desc.n_tris)
+            fp64.append(e)
     return {
         "metric": "batched GI ms/iter", "value": round(bat_dt * 1e3, 3), "unit": "ms/iter",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(bat_dt * 1e3, 3),
@@ -155,6 +202,7 @@ def bench_batched(args):
         "speedup_vs_sequential": round(seq_dt / bat_dt, 3),
         "batched_mrays_per_s": round(rays / bat_dt / 1e6, 2),
         "images_identical_to_sequential": bool(same),
+        "error_vs_fp64": fp64,
     }
 
 
@@ -203,6 +251,12 @@ def bench_adjoint(args):
         st = sc.render_adjoint(gt, out, gm, go, gl, opts=o)
         adj_ms.append(st.ms_trace)
     fst = sc.render(out, timing=True)
+    fp64 = None
+    if not args.no_cpu:
+        # forward image at the starting parameters against the converged FP64 render
+        sc.render(out)
+        torch.cuda.synchronize()
+        fp64 = fp64_leg(d, {"gpu": out.cpu().numpy()}, 512)
     return {
         "metric": "fwd+adjoint ms/iter", "value": round(both_dt * 1e3, 3), "unit": "ms/iter",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(both_dt * 1e3, 3),
@@ -216,6 +270,7 @@ def bench_adjoint(args):
         "adjoint_replay_device_ms": round(min(adj_ms), 3),
         "forward_device_ms": round(fst.ms, 3),
         "loss": loss,
+        "error_vs_fp64": fp64,
     }
 
 
@@ -261,6 +316,9 @@ def bench_direct(args):
     out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
     dt, st = _timed(lambda: sc.render(out), args.steps, args.warmup)
     rays = st.rays_camera + st.rays_extension + st.rays_shadow
+    fp64 = None
+    if not args.no_cpu:
+        fp64 = fp64_leg(d, {"gpu": out.cpu().numpy()}, 1024)
     return {
         "metric": "direct-lighting Mrays/s", "value": round(rays / dt / 1e6, 3), "unit": "Mrays/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
@@ -269,6 +327,7 @@ def bench_direct(args):
         "config": {"workload": "room + icosphere(20480) + 2 boxes, area light 16 samples, 640x480",
                    "triangles": int(d.n_tris)},
         "rays_per_frame": int(rays), "lcg_draws": int(st.lcg_draws),
+        "error_vs_fp64": fp64,
     }
 
 
@@ -475,7 +534,7 @@ def main():
     if args.copy_probe:
         line["copy_probe"] = copy_probe()
     if not args.no_cpu:
-        cores = min(os.cpu_count() or 1, 16)
+        cores = cpu_cores()
         p = parity_leg(g["desc"], g["img"], cores)
         line["cpu_baseline"] = {
             "value": round(p["cpu_rays"] / p["cpu_dt"] / 1e6, 4),
@@ -488,6 +547,9 @@ def main():
         }
         line["rmse_vs_oracle"] = {"rel_rmse": p["rel_rmse"], "bit_exact_pixels": p["exact_px"],
                                   "sample": "full 1280x720 frame, GPU image vs oracle image"}
+        # both 4-spp images (GPU and the FP32 oracle) against the converged FP64 render
+        line["error_vs_fp64"] = fp64_leg(g["desc"], {"gpu": g["img"], "oracle_fp32": p["ref"]}, 1024,
+                                         oc=p["oracle"])
     print(json.dumps(line))
 
 

@@ -80,6 +80,9 @@ def lib():
         L.yo_render_adjoint.restype = C.c_int
         L.yo_render_adjoint.argtypes = [vp, vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int,
                                         vp, C.c_int, vp, F32P, F32P, vp, vp, vp]
+        L.yo_render_fp64.restype = C.c_int
+        L.yo_render_fp64.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp, C.c_int, vp, C.c_int, vp,
+                                     C.c_int, C.c_int, vp, vp, C.c_int64, C.c_uint64, vp, C.c_int]
         L.yo_photon_build.restype = C.c_int
         L.yo_photon_build.argtypes = [vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int,
                                       C.c_int, vp, C.c_int, vp, U64P]
@@ -218,6 +221,30 @@ class OracleScene:
             raise RuntimeError("yo_render_adjoint failed: %d" % rc)
         return (rgb.reshape(desc.cam.resy, desc.cam.resx, 3), gmat.reshape(-1, 3),
                 gmod[: 2 * len(desc.mods)].reshape(-1, 2), glight)
+
+    def render_fp64(self, desc, samples=None, px=None, py=None, seed=1, threads=1):
+        """FP64 render through pixel centres.  samples=None: the whole image
+        with the reference's sample streams, [resy, resx, 3] float64.  Else
+        the pixels (px[q], py[q]) with `samples` samples of every sampled
+        light (the converged estimate), [n, 3] float64."""
+        L = lib()
+        args = (self.h, C.addressof(desc.cam), _addr(desc.mats), len(desc.mats), _addr(desc.mods),
+                _addr(desc.texs), _addr(desc.lights), len(desc.lights), _ptr(desc.obj_flags),
+                len(desc.obj_flags), C.addressof(desc.opts))
+        if samples is None:
+            rgb = np.zeros(desc.cam.resx * desc.cam.resy * 3, np.float64)
+            rc = L.yo_render_fp64(*args, 0, 0, None, None, 0, 0, _ptr(rgb), threads)
+            out = rgb.reshape(desc.cam.resy, desc.cam.resx, 3)
+        else:
+            px = np.ascontiguousarray(px, np.int32)
+            py = np.ascontiguousarray(py, np.int32)
+            rgb = np.zeros(len(px) * 3, np.float64)
+            rc = L.yo_render_fp64(*args, 1, int(samples), _ptr(px), _ptr(py), len(px), seed,
+                                  _ptr(rgb), threads)
+            out = rgb.reshape(-1, 3)
+        if rc != 0:
+            raise RuntimeError("yo_render_fp64 failed: %d" % rc)
+        return out
 
     def photon_build(self, desc, li):
         L = lib()

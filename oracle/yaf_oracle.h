@@ -119,6 +119,21 @@ int yo_render_adjoint(yo_scene* s, const yg_camera* cam, const yg_material* mats
                       const yg_render_opts* opts, const float* grad_rgb, float* rgb, double* gmat,
                       double* gmod, double* glight);
 
+/* FP64 render of camera rays through pixel centres (yo_fp64.c): sun, area
+ * and path lights in double precision.  mode YO_FP64_REFERENCE: the whole
+ * image with the reference's sample streams (rgb: resx*resy*3; px, py, n,
+ * samples, seed unused); YO_FP64_CONVERGED: the n pixels (px[q], py[q]) with
+ * `samples` samples of every sampled light (rgb: n*3).  Returns 0, -2 bad
+ * arguments, -4 non-shadow-casting objects, -5 AA or tone mapping, -6 photon
+ * lights or a path light used in indirect lighting, -7 several path lights. */
+#define YO_FP64_REFERENCE 0
+#define YO_FP64_CONVERGED 1
+int yo_render_fp64(yo_scene* s, const yg_camera* cam, const yg_material* mats, int nmat,
+                   const yg_modulator* mods, const yg_texture* texs, const yg_light* lights, int nl,
+                   const int32_t* obj_flags, int nobj, const yg_render_opts* opts, int mode,
+                   int samples, const int32_t* px, const int32_t* py, int64_t n, uint64_t seed,
+                   double* rgb, int nthreads);
+
 /* photonLight_t::init for light li; keeps the clustered photons in s */
 int yo_photon_build(yo_scene* s, const yg_material* mats, int nmat, const yg_modulator* mods,
                     int nmod, const yg_texture* texs, int ntex, const yg_light* lights,

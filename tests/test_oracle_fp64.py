@@ -1,0 +1,139 @@
+"""Pins of the FP64 oracle (oracle/yo_fp64.c), the converged reference that
+rendered images are measured against.
+
+* With the reference's own sample streams it must reproduce the FP32
+  restatement up to float rounding (terrain with pathlight GI; room with an
+  area light drawing from the global ourRandom() stream).
+* With many samples per pixel it must reach closed forms: sky light through
+  the pathlight's cosine-distributed first bounce (E[cos] = 2/3), and a
+  square area light over a plane (through the light's solid angle).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from yafray_amd import descriptors as D
+from yafray_amd import scenes as S
+
+
+def rel_rmse(a, b):
+    return float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2)))
+
+
+def test_reference_streams_match_fp32_terrain_gi():
+    d = S.terrain_gi(po.heightfield, nx=64, ny=32, resx=80, resy=72)
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    f32, _ = oc.render(d, threads=4)
+    f64 = oc.render_fp64(d, threads=4)
+    f32 = f32.astype(np.float64)
+    assert rel_rmse(f32, f64) < 1e-6
+    assert np.max(np.abs(f32 - f64) / np.maximum(np.abs(f64), 1e-6)) < 1e-4
+
+
+def test_reference_streams_match_fp32_area_light():
+    """Sequential ourRandom() draws over the render areas: one misplaced draw
+    would shift every later pixel."""
+    d = S.direct_room(64, 48, subdiv=2)
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    f32, _ = oc.render(d)
+    f64 = oc.render_fp64(d)
+    f32 = f32.astype(np.float64)
+    assert rel_rmse(f32, f64) < 1e-5
+    r = np.abs(f32 - f64).max(axis=2) / np.maximum(np.abs(f64).max(axis=2), 1e-6)
+    assert np.mean(r < 1e-4) > 0.999
+
+
+def _plane(lights, res=(48, 40), bg=(0.4, 0.5, 0.7)):
+    E = 50.0
+    v = np.array([[-E, -E, 0], [E, -E, 0], [E, E, 0], [-E, E, 0]], np.float32)
+    f = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+    cam = D.camera((0, 0, 3), (0, 0.01, 0), (0, 1, 4), res[0], res[1], 1.0, 1.0)
+    opts = D.render_opts(bias=1e-3, background=bg)
+    return D.SceneDesc(v, f, [0, 0], [0, 0], [7], cam, [D.material((0.6, 0.5, 0.3))], [], [],
+                       lights, opts)
+
+
+def _floor_points(d, px, py):
+    """Camera rays of pixel centres (camera.cc:27-87, 184-189) met with z = 0."""
+    c = d.cam
+    pos = np.array(c.pos[:], np.float64)
+    vup = np.array(c.up[:], np.float64) - pos
+    vto = np.array(c.look[:], np.float64) - pos
+    vr = np.cross(vup, vto)
+    vup = np.cross(vr, vto)
+    vup /= np.linalg.norm(vup)
+    vr = -vr / np.linalg.norm(vr)
+    vto /= np.linalg.norm(vto)
+    vup *= c.aspect * c.resy / c.resx
+    vto = vto * c.focal - (vup + vr) * 0.5
+    ray = np.outer(px + 0.5, vr / c.resx) + np.outer(py + 0.5, vup / c.resy) + vto
+    return pos + ray * (-pos[2] / ray[:, 2])[:, None]
+
+
+def test_converged_sky_light_closed_form():
+    """Every pathlight sample escapes: pixel = rho * power * B * E[N.w] + rho (N.s) E_sun,
+    with N.w = sqrt(z1), z1 uniform: E = 2/3 (pathtools.cc:32-43)."""
+    sun = np.array([1.0, 0.5, 1.5])
+    sun /= np.linalg.norm(sun)
+    d = _plane([D.pathlight(samples=4, depth=2, use_qmc=True, threshold=1e3),
+                D.sunlight(sun, power=2.0)])
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    px = np.array([0, 11, 23, 47, 30], np.int32)
+    py = np.array([0, 39, 20, 5, 33], np.int32)
+    got = oc.render_fp64(d, samples=4096, px=px, py=py, threads=2)
+    rho, B = np.array([0.6, 0.5, 0.3]), np.array([0.4, 0.5, 0.7])
+    exp = rho * B * (2.0 / 3.0) + rho * sun[2] * 2.0
+    assert np.max(np.abs(got - exp) / exp) < 3e-4
+    # the value of a pixel does not depend on which other pixels are listed
+    again = oc.render_fp64(d, samples=4096, px=px[::-1], py=py[::-1], threads=1)
+    assert np.array_equal(again, got[::-1])
+
+
+def _corner_solid_angle(a, b, h):
+    """Solid angle of an a x b rectangle seen from height h above one corner."""
+    return math.atan(a * b / (h * math.sqrt(a * a + b * b + h * h)))
+
+
+def test_converged_area_light_closed_form():
+    """Square light of side 2s at height h over the plane, facing down.
+    arealight.cc:93-174 averages pc * rho * plane_at * (N.L) / |L|^2 over its
+    samples; with the light parallel to the plane plane_at = h and N.L = h/d,
+    so the limit is pc * rho * h * mean(h / d^3) = pc * rho * h * Omega / A
+    (Omega: the light's solid angle, four corner rectangles)."""
+    s, h = 0.5, 1.5
+    L = D.arealight((-s, -s, h), (-s, s, h), (s, s, h), (s, -s, h), color=(1.0, 0.9, 0.8),
+                    power=3.0, samples=16)
+    d = _plane([L], bg=(0, 0, 0))
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    px = np.array([24, 20, 30, 17, 28], np.int32)
+    py = np.array([20, 14, 25, 22, 17], np.int32)
+    P = _floor_points(d, px.astype(np.float64), py.astype(np.float64))
+    assert np.all(np.abs(P[:, :2]) < s)
+    got = oc.render_fp64(d, samples=4096, px=px, py=py, threads=2)
+    pc = np.array([1.0, 0.9, 0.8]) * 3.0
+    rho = np.array([0.6, 0.5, 0.3])
+    omega = [sum(_corner_solid_angle(a, b, h) for a in (s - x, s + x) for b in (s - y, s + y))
+             for x, y in P[:, :2]]
+    for q in range(len(px)):
+        exp = pc * rho * h * omega[q] / (4 * s * s)
+        assert np.max(np.abs(got[q] - exp) / exp) < 2e-4, (q, got[q], exp)
+    # the solid angle against a midpoint quadrature of h / d^3
+    g = (np.arange(400) + 0.5) / 400 * 2 * s - s
+    X, Y = np.meshgrid(g, g)
+    d2 = (X - P[0, 0]) ** 2 + (Y - P[0, 1]) ** 2 + h * h
+    quad = np.mean(h / d2 ** 1.5) * 4 * s * s
+    assert abs(quad - omega[0]) < 1e-5 * omega[0]
+
+
+def test_unsupported_inputs_fail():
+    d = _plane([D.sunlight((0, 0, 1), power=1.0)])
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    aa = D.render_opts(bias=1e-3, background=(1, 1, 1), aa_passes=1, aa_minsamples=2)
+    d2 = D.SceneDesc(d.verts, d.idx, d.tri_obj, d.tri_mat, d.obj_flags, d.cam, list(d.mats), [], [],
+                     list(d.lights), aa)
+    with pytest.raises(RuntimeError, match="-5"):
+        oc.render_fp64(d2)
+    with pytest.raises(RuntimeError, match="-2"):
+        oc.render_fp64(d, samples=0, px=[0], py=[0])
