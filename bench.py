@@ -83,6 +83,12 @@ def parity_leg(d, img, cpu_threads):
 FP64_SAMPLES = 4096
 
 
+def path_spp(d):
+    """Pathlight samples per pixel of a scene description (1 without one)."""
+    from yafray_amd import descriptors as D
+    return next((int(L.samples) for L in d.lights if L.type == D.YG_LIGHT_PATH), 1)
+
+
 def cpu_cores():
     return min(os.cpu_count() or 1, 16)
 
@@ -201,6 +207,7 @@ def bench_batched(args):
         "sequential_ms_per_iter": round(seq_dt * 1e3, 3),
         "speedup_vs_sequential": round(seq_dt / bat_dt, 3),
         "batched_mrays_per_s": round(rays / bat_dt / 1e6, 2),
+        "batched_msamples_per_s": round(rays_px * path_spp(scenes[0].desc) / bat_dt / 1e6, 2),
         "images_identical_to_sequential": bool(same),
         "error_vs_fp64": fp64,
     }
@@ -266,6 +273,7 @@ def bench_adjoint(args):
                                "256 materials, Musgrave albedo texture (mul)",
                    "params": 256 * 3 + 3},
         "forward_ms": round(fwd_dt * 1e3, 3),
+        "forward_msamples_per_s": round(d.cam.resx * d.cam.resy * path_spp(d) / fwd_dt / 1e6, 2),
         "ratio_fwd_adj_over_fwd": round(both_dt / fwd_dt, 3),
         "adjoint_replay_device_ms": round(min(adj_ms), 3),
         "forward_device_ms": round(fst.ms, 3),
@@ -500,6 +508,9 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
+        # pixel samples: camera rays x pathlight samples per frame
+        "msamples_per_s": round(g["desc"].cam.resx * g["desc"].cam.resy * path_spp(g["desc"])
+                                / (ms * 1e-3) / 1e6, 3),
         "config": {
             "workload": "path-traced GI: 1024x512-vertex fBm terrain (1,045,506 tris), 1280x720, "
                         "4 spp, 3 bounces, sun + constant sky, 64 height-band materials",
