@@ -332,3 +332,62 @@ def test_heightfield_grid():
     assert np.allclose(hf[:5, 0], [0, 0.5, 1.0, 1.5, 2.0]) and np.allclose(hf[::5, 1], [0, 0.5, 1.0])
     L = po.lib()
     assert hf[7, 2] == np.float32(np.float32(0.4) * L.yo_fbm(np.float32(4.0), np.float32(2.0), 0.0, 1.0, 2.0, 6.0))
+
+
+# ---------------------------------------------------- marble and wood --
+
+def _tex_float(tex, pts):
+    return po.noise_eval(3, tex, np.asarray(pts, np.float32))
+
+
+def test_marble_wood_closed_forms():
+    """Without turbulence the textures are closed forms of the point
+    (basictex.cc:133-153, 194-216): marble = (1/2 + 1/2 sin(5 s))^(1/sharpness)
+    with s = x + y + z; wood = 1/2 + 1/2 sin(20 |p|) with rings, sin(10 s)
+    without; the saw shape is frac(w / 2 pi), the triangle |2 frac(w / 2 pi) - 1|.
+    Points are chosen at the sine's peaks, zeros and quarter periods."""
+    from yafray_amd import descriptors as D
+    # marble, sine shape, sharpness 1 and 2 (exponent 1/2)
+    s = np.array([math.pi / 10, -math.pi / 10, 0.0, math.pi / 20, 3 * math.pi / 10])
+    pts = np.stack([s / 3, s / 3, s / 3], 1)
+    w = 0.5 + 0.5 * np.sin(5 * s)
+    for sharp, ex in ((1.0, 1.0), (2.0, 0.5)):
+        got = _tex_float(D.marble(turbulence=0.0, sharpness=sharp), pts)
+        assert np.allclose(got, w ** ex, atol=2e-6), (sharp, got, w ** ex)
+    # marble saw and triangle: w = 5 s / (2 pi)
+    s2 = np.array([0.1, 0.5, 1.3, 2.9, 4.4])
+    pts2 = np.stack([s2, np.zeros(5), np.zeros(5)], 1)
+    f = 5 * s2 / (2 * math.pi)
+    fr = f - np.floor(f)
+    assert np.allclose(_tex_float(D.marble(turbulence=0.0, shape=1), pts2), fr, atol=2e-6)
+    assert np.allclose(_tex_float(D.marble(turbulence=0.0, shape=2), pts2), np.abs(2 * fr - 1), atol=4e-6)
+    # wood with rings: the radius alone matters
+    r = np.array([math.pi / 40, math.pi / 20, 3 * math.pi / 40, 0.3, 0.77])
+    dirs = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [0.6, 0.8, 0], [0, 0.6, -0.8]])
+    got = _tex_float(D.wood(turbulence=0.0, rings=True), dirs * r[:, None])
+    assert np.allclose(got, 0.5 + 0.5 * np.sin(20 * r), atol=2e-6)
+    # wood without rings: 10 s
+    got = _tex_float(D.wood(turbulence=0.0), pts)
+    assert np.allclose(got, 0.5 + 0.5 * np.sin(10 * s), atol=2e-6)
+    # the colour is color1 + f (color2 - color1) (basictex.cc:155-159)
+    t = D.marble(color1=(0.1, 0.2, 0.3), color2=(0.9, 0.6, 0.5), turbulence=0.0)
+    c = po.noise_eval(4, t, pts.astype(np.float32)).reshape(-1, 3)
+    c1, c2 = np.array([0.1, 0.2, 0.3]), np.array([0.9, 0.6, 0.5])
+    assert np.allclose(c, c1 + w[:, None] * (c2 - c1), atol=3e-6)
+
+
+def test_marble_wood_turbulence_term():
+    """With turbulence t the phase gains t x turbulence(p) (the turbulence
+    itself is pinned above): at points with s = 0 marble's sine shape gives
+    1/2 + 1/2 sin(t T(p)); scaling t scales the phase."""
+    from yafray_amd import descriptors as D
+    rng = np.random.default_rng(5)
+    xy = rng.uniform(-3, 3, (64, 2))
+    pts = np.c_[xy, -xy.sum(1)].astype(np.float32)   # x + y + z = 0 up to rounding
+    assert np.max(np.abs(pts.astype(np.float64).sum(1))) < 1e-5
+    # turbulence(p, depth, size, hard) of the same parameters (any texture carries them)
+    T = po.noise_eval(2, D.clouds(depth=3, size=1.7), pts)
+    for tb in (0.5, 2.0):
+        got = _tex_float(D.marble(turbulence=tb, depth=3, size=1.7), pts)
+        exp = 0.5 + 0.5 * np.sin(5 * pts.astype(np.float64).sum(1) + tb * T.astype(np.float64))
+        assert np.allclose(got, exp, atol=5e-6), tb

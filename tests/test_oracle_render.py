@@ -1,4 +1,6 @@
 """Pins of the oracle's render loop: closed-form scenes and invariants."""
+import math
+
 import numpy as np
 
 from oracle import pyoracle as po
@@ -123,3 +125,35 @@ def test_photon_counts():
     c = np.floor(ph[:, 3:6] / cell)
     keys = [tuple(r) for r in c.astype(np.int64)]
     assert keys == sorted(keys)
+
+
+def test_photon_estimate_closed_form():
+    """Spot light at height h over a plane, photons stored at their first hit
+    (depth 0, mindepth 0, clusters smaller than the photon spacing).  Each
+    photon carries power x colour x Omega (Omega = 2 pi (1 - cos angle),
+    photonlight.cc:77-90) and the stored density at horizontal distance rho
+    is Np cos^3(theta) / (Omega h^2).  The K-nearest estimate
+    (photonlight.cc:297-395) sums cone weights 2 (1 - d/r), whose mean over
+    the disc is 2/3, times N.L = cos(theta), scales by 4 pi / (Np pi r^2 fa)
+    with fillArea's fa = 1 for a uniform field (weighted mean radius r/2 times
+    sqrt(2) x 1.41421), so Np and Omega cancel:
+        E = (8 pi / 3) power colour albedo cos^4(theta) / h^2."""
+    E = 50.0
+    v = np.array([[-E, -E, 0], [E, -E, 0], [E, E, 0], [-E, E, 0]], np.float32)
+    f = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+    h, P = 1.0, 3.0
+    L = D.photonlight(frm=(0, 0, h), to=(0, 0, 0), color=(1.0, 0.8, 0.6), power=P, angle=40.0,
+                      photons=2_000_000, search=600, depth=0, mindepth=0, fixedradius=5.0, cluster=1e-6)
+    cam = D.camera((0, 0, 3), (0, 0.01, 0), (0, 1, 4), 8, 8, 1.0, 1.0)
+    d = D.SceneDesc(v, f, [0, 0], [0, 0], [7], cam, [D.material((0.6, 0.5, 0.3))], [], [], [L],
+                    D.render_opts(bias=1e-3))
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    st = oc.photon_build(d, 0)
+    assert st[1] == st[2] == st[0]   # every photon stored, none merged
+    pts = np.array([[0, 0, 0], [0.5, 0, 0], [0, -0.4, 0], [0.3, 0.3, 0], [-0.6, 0.2, 0]], np.float32)
+    n = np.tile([0, 0, 1], (len(pts), 1)).astype(np.float32)
+    got = oc.photon_gather(d, 0, pts, n, n, n, np.zeros(len(pts), np.int32))
+    c = h / np.sqrt(h * h + np.hypot(pts[:, 0], pts[:, 1]) ** 2)
+    exp = ((8 * math.pi / 3) * P * np.array([1.0, 0.8, 0.6]) * np.array([0.6, 0.5, 0.3]) / h ** 2
+           * (c ** 4)[:, None])
+    assert np.max(np.abs(got / exp - 1)) < 0.03, got / exp
