@@ -311,6 +311,61 @@ def bench_photon(args):
     }
 
 
+VALU_FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak (packed FMA); unpacked code reaches half
+FBM_FLOP_PER_POINT = 384       # 6 octaves x ~64 FLOP (improved Perlin: 8 gradients, 7 lerps, 3 fades)
+
+
+def bench_noise(args):
+    """Improved-Perlin fBm on the device, the terrain's parameters (H 1,
+    lacunarity 2, 6 octaves), over 64M points resident in HBM."""
+    import numpy as np
+    import torch
+
+    from yafray_amd import descriptors as D
+    from yafray_amd import lib as Y
+
+    n = 1 << 26
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    pts = (torch.rand((n, 3), device="cuda", generator=g) * 64.0 - 32.0).contiguous()
+    out = torch.empty(n, dtype=torch.float32, device="cuda")
+    tex = D.musgrave(H=1.0, lacunarity=2.0, octaves=6.0)
+    for _ in range(args.warmup):
+        Y.noise_eval(D.YG_EVAL_FBM, tex, pts, out)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(args.steps):
+        Y.noise_eval(D.YG_EVAL_FBM, tex, pts, out)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    tflops = n * FBM_FLOP_PER_POINT / (ms * 1e-3) / 1e12
+    line = {
+        "metric": "fBm Gpoints/s", "value": round(n / (ms * 1e-3) / 1e9, 3), "unit": "Gpoints/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "improved-Perlin fBm, H 1, lacunarity 2, 6 octaves, 67,108,864 points "
+                               "uniform in [-32, 32]^3", "points": n},
+        "roofline": {"bound": "valu", "achieved": round(tflops, 2), "peak": VALU_FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(tflops / VALU_FP32_PEAK_TFLOPS, 4),
+                     "flop_per_point": FBM_FLOP_PER_POINT,
+                     "hbm_gbs": round(n * 16 / (ms * 1e-3) / 1e9, 1)},
+    }
+    if not args.no_cpu:
+        from oracle import pyoracle as po
+        q = np.random.default_rng(3).choice(n, 65536, replace=False)
+        host = pts.cpu().numpy()[q]
+        t0 = time.perf_counter()
+        ref = po.noise_eval(1, tex, host)
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(len(q) / dt / 1e9, 6), "unit": "Gpoints/s", "cores": 1,
+                                "kind": "port", "sample": "oracle fBm of 65,536 of the points, one thread"}
+        line["exact_vs_oracle"] = float(np.mean(out.cpu().numpy()[q] == ref))
+    return line
+
+
 def bench_direct(args):
     """Plain direct lighting: box room + icosphere + two boxes, 1x1 area light
     with 16 samples, 640x480."""
@@ -456,7 +511,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="gi",
-                    choices=["gi", "batched", "adjoint", "photon", "direct", "autotune", "aa", "rays"])
+                    choices=["gi", "batched", "adjoint", "photon", "direct", "autotune", "aa", "rays",
+                             "noise"])
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy after the timed region (counter calibration)")
@@ -470,7 +526,7 @@ def main():
         torch.cuda.set_device(0)
         fn = {"batched": bench_batched, "adjoint": bench_adjoint, "photon": bench_photon,
               "direct": bench_direct, "autotune": bench_autotune, "aa": bench_aa,
-              "rays": bench_rays}[args.workload]
+              "rays": bench_rays, "noise": bench_noise}[args.workload]
         print(json.dumps(fn(args)))
         return
 

@@ -190,6 +190,11 @@ typedef struct yg_render_opts {
                                 if nothing changed since, instead of rendering again */
 #define YG_RENDER_NO_GRAPH 8 /* launch every kernel of the frame from the host instead of
                                 replaying the captured frame graph (see yg_render) */
+#define YG_RENDER_ROULETTE 16 /* Russian roulette from the third bounce on: a path continues
+                                 with probability q = min(1, energy of its throughput) and is
+                                 divided by q.  Not in the reference (pathlight.cc:313-315 ends
+                                 paths only at energy < 0.05 or the depth): images differ from
+                                 it; rejected by yg_render_adjoint */
 
 typedef struct yg_render_stats {
     uint64_t rays_camera;

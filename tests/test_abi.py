@@ -97,3 +97,11 @@ def test_autotune_argument_errors_without_gpu():
         assert L.yg_autotune(h, ctypes.byref(o), dummy, ctypes.byref(t), ctypes.byref(r), None) == -1
         assert msg in Y.last_error()
     assert L.yg_scene_destroy(h) == 0
+
+
+def test_render_flags_match_header():
+    import re
+    from yafray_amd import descriptors as D
+    hdr = open(os.path.join(ROOT, "include", "yafgpu.h")).read()
+    flags = dict(re.findall(r"#define (YG_RENDER_\w+) (\d+)", hdr))
+    assert flags and all(getattr(D, k) == int(v) for k, v in flags.items()), flags

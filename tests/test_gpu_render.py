@@ -293,3 +293,35 @@ def test_stack_lds_split_changes_nothing_but_speed(torch_cuda):
     sc.render(out)
     torch.cuda.synchronize()
     assert np.array_equal(out.cpu().numpy(), imgs[0])
+
+
+def test_roulette_mode_keeps_the_mean(torch_cuda):
+    """YG_RENDER_ROULETTE (not a reference feature) ends paths at random from
+    the third bounce on and divides survivors by their survival probability:
+    fewer extension rays, the same expected image, reproducible frames; the
+    adjoint refuses it."""
+    torch = torch_cuda
+    Y = _Y()
+    d = small_terrain(samples=64, depth=4)
+    sc = Y.Scene.from_desc(d)
+    out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+    st0 = sc.render(out)
+    torch.cuda.synchronize()
+    ref = out.cpu().numpy().astype(np.float64)
+    o = D.yg_render_opts.from_buffer_copy(d.opts)
+    o.flags |= D.YG_RENDER_ROULETTE
+    st1 = sc.render(out, o)
+    torch.cuda.synchronize()
+    rr = out.cpu().numpy().astype(np.float64)
+    st2 = sc.render(out, o)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), rr.astype(np.float32))
+    assert st2.rays_extension == st1.rays_extension
+    assert st1.rays_extension < st0.rays_extension
+    assert abs(rr.mean() / ref.mean() - 1) < 0.01
+    assert rel_rmse(rr, ref) < 0.05
+    g = torch.zeros_like(out)
+    gm = torch.zeros(len(d.mats) * 3, dtype=torch.float64, device="cuda")
+    gl = torch.zeros(len(d.lights), dtype=torch.float64, device="cuda")
+    with pytest.raises(Exception, match="roulette"):
+        sc.render_adjoint(g, out, gm, None, gl, opts=o)

@@ -23,6 +23,15 @@ static int perm_upload()
     return YG_OK;
 }
 
+// Copies the permutation into a block's LDS (all threads of the block call
+// it, before any returns): the hash lookups then read LDS instead of three
+// dependent levels of global loads per octave.
+__device__ __forceinline__ void perm_to_lds(unsigned char* s)
+{
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) s[i] = YG_PERM[i];
+    __syncthreads();
+}
+
 // newPerlin_t::fade / grad (noise.h:33-40)
 __device__ __forceinline__ float pfade(float t) { return t * t * t * (t * (t * 6 - 15) + 10); }
 __device__ __forceinline__ float pgrad(int hash, float x, float y, float z)
@@ -33,10 +42,11 @@ __device__ __forceinline__ float pgrad(int hash, float x, float y, float z)
 }
 __device__ __forceinline__ float plerp(float t, float a, float b) { return a + t * (b - a); }
 
-// newPerlin_t::operator() (noise.cc:161-183), value in [0,1]
-__device__ __forceinline__ float new_perlin(float x, float y, float z)
+// newPerlin_t::operator() (noise.cc:161-183), value in [0,1].  H: the
+// permutation, YG_PERM or a block's LDS copy of it (perm_to_lds); the noise
+// functions below pass it on.
+__device__ __forceinline__ float new_perlin(float x, float y, float z, const unsigned char* H = YG_PERM)
 {
-    const unsigned char* H = YG_PERM;
     float u = floorf(x), v = floorf(y), w = floorf(z);
     int X = ((int)u) & 255, Y = ((int)v) & 255, Z = ((int)w) & 255;
     x -= u; y -= v; z -= w;
@@ -51,29 +61,42 @@ __device__ __forceinline__ float new_perlin(float x, float y, float z)
     return 0.5f + 0.5f * nv;
 }
 
-__device__ __forceinline__ float signed_noise(float x, float y, float z) { return 2.0f * new_perlin(x, y, z) - 1.0f; }
-
-// fBm_t::operator() (noise.cc:638-650)
-__device__ __forceinline__ float fbm(float x, float y, float z, float H, float lac, float oct)
+__device__ __forceinline__ float signed_noise(float x, float y, float z, const unsigned char* perm = YG_PERM)
 {
-    float value = 0, pwr = 1, pwHL = dpow(lac, -H);
+    return 2.0f * new_perlin(x, y, z, perm) - 1.0f;
+}
+
+// fBm_t::operator() (noise.cc:638-650) with pwHL = pow(lacunarity, -H)
+// given: it is the same for every point of a texture, so bulk kernels take
+// it from the host (dpow there equals the oracle's).
+__device__ __forceinline__ float fbm_w(float x, float y, float z, float pwHL, float lac, float oct,
+                                       const unsigned char* perm = YG_PERM)
+{
+    float value = 0, pwr = 1;
     for (int i = 0; i < (int)oct; i++) {
-        value += signed_noise(x, y, z) * pwr;
+        value += signed_noise(x, y, z, perm) * pwr;
         pwr *= pwHL;
         x *= lac; y *= lac; z *= lac;
     }
     float rmd = oct - floorf(oct);
-    if (rmd != 0.f) value += rmd * signed_noise(x, y, z) * pwr;
+    if (rmd != 0.f) value += rmd * signed_noise(x, y, z, perm) * pwr;
     return value;
 }
 
+__device__ __forceinline__ float fbm(float x, float y, float z, float H, float lac, float oct,
+                                     const unsigned char* perm = YG_PERM)
+{
+    return fbm_w(x, y, z, dpow(lac, -H), lac, oct, perm);
+}
+
 // turbulence (noise.cc:808-819); newPerlin_t adds no coordinate offset
-__device__ __forceinline__ float turbulence(float x, float y, float z, int oct, float size, int hard)
+__device__ __forceinline__ float turbulence(float x, float y, float z, int oct, float size, int hard,
+                                            const unsigned char* perm = YG_PERM)
 {
     float val, amp = 1, sum = 0;
     x *= size; y *= size; z *= size;
     for (int i = 0; i <= oct; i++, amp *= 0.5f, x *= 2.0f, y *= 2.0f, z *= 2.0f) {
-        val = new_perlin(x, y, z);
+        val = new_perlin(x, y, z, perm);
         if (hard) val = fabsf(2.0f * val - 1.0f);
         sum += amp * val;
     }
@@ -81,11 +104,11 @@ __device__ __forceinline__ float turbulence(float x, float y, float z, int oct, 
 }
 
 // texture getFloat (basictex.cc:82-90, 133-153, 194-216, 677-680)
-__device__ __forceinline__ float tex_float(const yg_texture& t, f3 p)
+__device__ __forceinline__ float tex_float(const yg_texture& t, f3 p, const unsigned char* perm = YG_PERM)
 {
     switch (t.type) {
     case YG_TEX_CLOUDS: {
-        float v = turbulence(p.x, p.y, p.z, t.depth, t.size, t.hard);
+        float v = turbulence(p.x, p.y, p.z, t.depth, t.size, t.hard, perm);
         if (t.bias) {
             v *= v;
             if (t.bias == 1) return -v;
@@ -97,11 +120,11 @@ __device__ __forceinline__ float tex_float(const yg_texture& t, f3 p)
         float w;
         if (t.type == YG_TEX_MARBLE) {
             w = (float)((double)(p.x + p.y + p.z) * 5.0 +
-                        ((t.turb == 0.0f) ? 0.0 : (double)(t.turb * turbulence(p.x, p.y, p.z, t.depth, t.size, t.hard))));
+                        ((t.turb == 0.0f) ? 0.0 : (double)(t.turb * turbulence(p.x, p.y, p.z, t.depth, t.size, t.hard, perm))));
         } else {
             if (t.rings) w = (float)((double)sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 20.0);
             else w = (float)((double)(p.x + p.y + p.z) * 10.0);
-            w = (float)((double)w + ((t.turb == 0.0f) ? 0.0 : (double)(t.turb * turbulence(p.x, p.y, p.z, t.depth, t.size, t.hard))));
+            w = (float)((double)w + ((t.turb == 0.0f) ? 0.0 : (double)(t.turb * turbulence(p.x, p.y, p.z, t.depth, t.size, t.hard, perm))));
         }
         switch (t.shape) {
         case YG_SHAPE_SAW:
@@ -119,15 +142,15 @@ __device__ __forceinline__ float tex_float(const yg_texture& t, f3 p)
         return w;
     }
     case YG_TEX_MUSGRAVE:
-        return t.iscale * fbm(p.x * t.size, p.y * t.size, p.z * t.size, t.H, t.lacunarity, t.octaves);
+        return t.iscale * fbm(p.x * t.size, p.y * t.size, p.z * t.size, t.H, t.lacunarity, t.octaves, perm);
     }
     return 0.0f;
 }
 
 // getColor = color1 + getFloat(p)*(color2 - color1)
-__device__ __forceinline__ c3 tex_color(const yg_texture& t, f3 p)
+__device__ __forceinline__ c3 tex_color(const yg_texture& t, f3 p, const unsigned char* perm = YG_PERM)
 {
-    float f = tex_float(t, p);
+    float f = tex_float(t, p, perm);
     c3 c1 = mkc(t.color1[0], t.color1[1], t.color1[2]);
     c3 c2 = mkc(t.color2[0], t.color2[1], t.color2[2]);
     return cadd(c1, cscale(csub(c2, c1), f));

@@ -597,10 +597,18 @@ __global__ __launch_bounds__(SB) void k_shade(const RenderScene* __restrict__ sc
             }
         }
         st_hn[p] = face_forward(sp.Ng, sp.N, -ray);
-        st_col[p] = cmul(raycolor, rho);
+        c3 next_col = cmul(raycolor, rho);
+        uint32_t cont = 1;
+        if (!ADJ && R.rr_from > 0 && j + 1 >= R.rr_from) {
+            // Russian roulette before the path's next level
+            const float q = fminf(1.0f, cenergy(next_col));
+            if (rr_unit(p, j + 1, R.rr_seed) < q) next_col = cscale(next_col, 1.0f / q);
+            else cont = 0;
+        }
+        st_col[p] = next_col;
         st_where[p] = sp.P;
         st_ti[p] = ti;
-        flag[p] = 1;
+        flag[p] = cont;
     }
     if (ADJ) {
         adj_lds_flush(Ls, G, lds_ok);
@@ -1042,7 +1050,8 @@ struct RenderWS {
 static bool same_opts(const yg_render_opts& a, const yg_render_opts& b)
 {
     yg_render_opts x = a, y = b;
-    x.flags = y.flags = 0;
+    x.flags &= YG_RENDER_ROULETTE;  // the other flags do not change the image
+    y.flags &= YG_RENDER_ROULETTE;
     return memcmp(&x, &y, sizeof(x)) == 0;
 }
 
@@ -1179,6 +1188,8 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
     R.tm_clamp = o->clamp_rgb ? 1 : 0;
     R.tm_e = -o->exposure;
     R.tm_g = (o->gamma != 0.0f) ? (float)(1.0 / (double)o->gamma) : 0.0f;
+    R.rr_from = (o->flags & YG_RENDER_ROULETTE) ? 2 : 0;
+    R.rr_seed = o->lcg_seed;
     P.aa_passes = o->aa_passes;
     P.aa_threshold = o->aa_threshold;
     block_regions(R.resx, R.resy, o->block > 0 ? o->block : 64, o->rand_seed, P.areas);
@@ -2035,6 +2046,8 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
     if (!s || !opts || !d_grad_rgb || !d_rgb || !d_grad_mat || !d_grad_light)
         return yg_fail(YG_ERR_ARG, "null argument");
     if (opts->aa_passes > 0) return yg_fail(YG_ERR_UNSUPPORTED, "the adjoint does not differentiate antialiasing passes");
+    if (opts->flags & YG_RENDER_ROULETTE)
+        return yg_fail(YG_ERR_UNSUPPORTED, "the adjoint does not differentiate Russian roulette");
     int nsun = 0, npath = 0;
     for (const auto& L : s->lights) {
         if (L.type == YG_LIGHT_SUN) nsun++;

@@ -67,7 +67,24 @@ struct RenderScene {
     // tone mapping (scene_t::setExposure / setGamma, scene.h:161-164)
     int tm, tm_clamp;
     float tm_e, tm_g;        // -exposure, 1/gamma (0 for gamma 0)
+    // Russian roulette (YG_RENDER_ROULETTE): first GI level it decides, 0 = off
+    int rr_from;
+    uint32_t rr_seed;
 };
+
+// Uniform [0, 1) of a path's roulette at one level: an integer hash of
+// (path, level, seed) (Wellons' lowbias32 finaliser), so a frame is
+// reproducible and launch order does not matter.
+__device__ __forceinline__ float rr_unit(uint32_t p, int level, uint32_t seed)
+{
+    uint32_t h = p * 0x9E3779B1u ^ (uint32_t)(level + 1) * 0x85EBCA77u ^ seed * 0xC2B2AE3Du;
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    h *= 0x846CA68Bu;
+    h ^= h >> 16;
+    return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
 
 // color_t::expgam_Adjust (color.h:175-195).  color.h includes only <cmath>, so
 // its unqualified exp and pow are the C library's double functions: each

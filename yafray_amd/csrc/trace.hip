@@ -93,34 +93,38 @@ __global__ __launch_bounds__(TB) void k_is_shadowed(BVHView bvh, const float* __
                          min_raydis, self_bias, st) ? 1 : 0;
 }
 
-__global__ void k_noise_eval(int what, yg_texture tex, const float* __restrict__ pts, uint32_t n,
+__global__ void k_noise_eval(int what, yg_texture tex, float pwHL, const float* __restrict__ pts, uint32_t n,
                              float* __restrict__ out)
 {
+    __shared__ unsigned char perm[512];
+    perm_to_lds(perm);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const f3 p = ld3(pts + 3 * (size_t)i);
     switch (what) {
-    case YG_EVAL_NOISE: out[i] = new_perlin(p.x, p.y, p.z); break;
-    case YG_EVAL_FBM: out[i] = fbm(p.x, p.y, p.z, tex.H, tex.lacunarity, tex.octaves); break;
-    case YG_EVAL_TURBULENCE: out[i] = turbulence(p.x, p.y, p.z, tex.depth, tex.size, tex.hard); break;
-    case YG_EVAL_TEX_FLOAT: out[i] = tex_float(tex, p); break;
+    case YG_EVAL_NOISE: out[i] = new_perlin(p.x, p.y, p.z, perm); break;
+    case YG_EVAL_FBM: out[i] = fbm_w(p.x, p.y, p.z, pwHL, tex.lacunarity, tex.octaves, perm); break;
+    case YG_EVAL_TURBULENCE: out[i] = turbulence(p.x, p.y, p.z, tex.depth, tex.size, tex.hard, perm); break;
+    case YG_EVAL_TEX_FLOAT: out[i] = tex_float(tex, p, perm); break;
     case YG_EVAL_TEX_COLOR: {
-        const c3 c = tex_color(tex, p);
+        const c3 c = tex_color(tex, p, perm);
         out[3 * (size_t)i] = c.r; out[3 * (size_t)i + 1] = c.g; out[3 * (size_t)i + 2] = c.b;
         break;
     }
     }
 }
 
-__global__ void k_heightfield(int nx, int ny, float ex, float ey, float amp, float freq, float H,
+__global__ void k_heightfield(int nx, int ny, float ex, float ey, float amp, float freq, float pwHL,
                               float lac, float oct, float* __restrict__ xyz)
 {
+    __shared__ unsigned char perm[512];
+    perm_to_lds(perm);
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (int64_t)nx * ny) return;
     const int i = (int)(k % nx), j = (int)(k / nx);
     const float x = ex * (float)i / (float)(nx - 1);
     const float y = ey * (float)j / (float)(ny - 1);
-    const float z = amp * fbm(freq * x, freq * y, 0.0f, H, lac, oct);
+    const float z = amp * fbm_w(freq * x, freq * y, 0.0f, pwHL, lac, oct, perm);
     xyz[3 * k] = x; xyz[3 * k + 1] = y; xyz[3 * k + 2] = z;
 }
 
@@ -217,7 +221,7 @@ extern "C" int yg_noise_eval(int32_t what, const yg_texture* tex, const float* d
     if (rc) return rc;
     if (n == 0) return YG_OK;
     hipLaunchKernelGGL(k_noise_eval, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream,
-                       what, t, d_pts, n, d_out);
+                       what, t, dpow(t.lacunarity, -t.H), d_pts, n, d_out);
     YG_HIP(hipGetLastError());
     return YG_OK;
 }
@@ -231,7 +235,7 @@ extern "C" int yg_gen_heightfield(int32_t nx, int32_t ny, float ex, float ey, fl
     if (rc) return rc;
     const uint64_t n = (uint64_t)nx * ny;
     hipLaunchKernelGGL(k_heightfield, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream,
-                       nx, ny, ex, ey, amp, freq, H, lacunarity, octaves, d_xyz);
+                       nx, ny, ex, ey, amp, freq, dpow(lacunarity, -H), lacunarity, octaves, d_xyz);
     YG_HIP(hipGetLastError());
     return YG_OK;
 }

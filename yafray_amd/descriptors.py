@@ -228,12 +228,19 @@ def material(color, mod_first=0, mod_count=0):
     return yg_material(_f3(color), mod_first, mod_count)
 
 
+# yg_render_opts.flags (include/yafgpu.h)
+YG_RENDER_TIMING, YG_RENDER_KTIMING, YG_RENDER_REUSE, YG_RENDER_NO_GRAPH, YG_RENDER_ROULETTE = 1, 2, 4, 8, 16
+
+
 def render_opts(bias=0.1, raydepth=3, background=None, block=64, rand_seed=1, lcg_seed=123212,
                 aa_passes=0, aa_minsamples=1, aa_pixelwidth=1.5, aa_threshold=0.05, aa_jitterfirst=False,
-                tonemap=False, exposure=0.0, gamma=1.0, clamp_rgb=False):
+                tonemap=False, exposure=0.0, gamma=1.0, clamp_rgb=False, roulette=False):
     """Render parameters with the reference's defaults; the AA values are
-    clamped as interface_impl.cc:501-533 clamps them."""
+    clamped as interface_impl.cc:501-533 clamps them.  roulette: the
+    non-reference Russian roulette mode (YG_RENDER_ROULETTE)."""
     o = yg_render_opts()
+    if roulette:
+        o.flags |= YG_RENDER_ROULETTE
     if aa_passes < 0:
         aa_passes = 1
     if aa_passes and aa_minsamples < 1:
