@@ -70,8 +70,8 @@ typedef struct yg_camera {
 
 enum {
     YG_NOISE_NEWPERLIN = 0, /* newPerlin_t, noise.cc:161-183 */
-    YG_NOISE_STDPERLIN = 1, /* stdPerlin_t (not implemented: YG_ERR_UNSUPPORTED) */
-    YG_NOISE_BLENDER = 2    /* blenderNoise_t (not implemented: YG_ERR_UNSUPPORTED) */
+    YG_NOISE_STDPERLIN = 1, /* stdPerlin_t, noise.cc:372-435 */
+    YG_NOISE_BLENDER = 2    /* blenderNoise_t, noise.cc:439-509 (turbulence offsets its point by 1) */
 };
 
 enum {

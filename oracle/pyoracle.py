@@ -54,6 +54,10 @@ def lib():
         L.yo_glibc_rand.argtypes = [C.c_uint32, C.c_int64, I32P]
         L.yo_newperlin.restype = C.c_float
         L.yo_newperlin.argtypes = [C.c_float] * 3
+        for f in ("yo_stdperlin", "yo_blendernoise"):
+            getattr(L, f).restype = C.c_float
+            getattr(L, f).argtypes = [C.c_float] * 3
+        L.yo_grad_table.argtypes = [F32P]
         L.yo_fbm.restype = C.c_float
         L.yo_fbm.argtypes = [C.c_float] * 6
         L.yo_turbulence.restype = C.c_float

@@ -80,6 +80,10 @@ void yo_glibc_rand(uint32_t seed, int64_t n, int32_t* out);
 /* ---------------------------------------------------------------- noise -- */
 void yo_perm(unsigned char* out512);
 float yo_newperlin(float x, float y, float z);
+float yo_stdperlin(float x, float y, float z);
+float yo_blendernoise(float x, float y, float z);
+/* the 256 unit gradients (768 floats) of stdPerlin_t / blenderNoise_t */
+void yo_grad_table(float* out768);
 float yo_fbm(float x, float y, float z, float H, float lac, float oct);
 float yo_turbulence(float x, float y, float z, int oct, float size, int hard);
 /* what: YG_EVAL_* ; out gets 1 float (3 for TEX_COLOR) per point */

@@ -116,11 +116,14 @@ def radinv2(i):
     return r
 
 
-def test_gi_textured_musgrave_matches_oracle(torch_cuda):
-    """Albedo modulated by a Musgrave fBm texture (mode mul) on the terrain."""
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_gi_textured_musgrave_matches_oracle(torch_cuda, kind):
+    """Albedo modulated by a Musgrave fBm texture (mode mul) on the terrain, its
+    fBm over newPerlin_t, stdPerlin_t or blenderNoise_t (kind 0, 1, 2)."""
     torch = torch_cuda
     d = small_terrain(nx=96, ny=48, resx=96, resy=64, samples=4, depth=3, nmat=8)
     texs = [D.musgrave(color1=(0.2, 0.2, 0.2), color2=(1, 1, 1), octaves=4.0, size=1.0)]
+    texs[0].noise = kind
     mods = [D.modulator(0, D.YG_MOD_MUL, color=0.7, size=3.0)]
     mats = [D.material(tuple(m.color), 0, 1) for m in d.mats]
     d = D.SceneDesc(d.verts, d.idx, d.tri_obj, d.tri_mat, d.obj_flags, d.cam, mats, mods, texs,

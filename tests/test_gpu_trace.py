@@ -171,8 +171,11 @@ def test_first_hit_is_shadowed(torch_cuda, width):
             assert np.mean(out.cpu().numpy() == oc_o) >= 0.9999
 
 
+@pytest.mark.parametrize("kind", [0, 1, 2])
 @pytest.mark.parametrize("what", [0, 1, 2, 3, 4])
-def test_noise_parity(torch_cuda, what):
+def test_noise_parity(torch_cuda, what, kind):
+    """newPerlin_t, stdPerlin_t and blenderNoise_t (kind 0, 1, 2) through every
+    evaluation: bit-exact with the oracle."""
     torch = torch_cuda
     Y = _lib(torch)
     rng = np.random.default_rng(11)
@@ -187,6 +190,7 @@ def test_noise_parity(torch_cuda, what):
     if what <= 2:
         texs = [texs[0], texs[1]]
     for tex in texs:
+        tex.noise = kind
         out = torch.empty(len(pts) * (3 if what == 4 else 1), dtype=torch.float32, device="cuda")
         Y.noise_eval(what, tex, torch.from_numpy(pts).cuda(), out)
         g = out.cpu().numpy()

@@ -391,3 +391,94 @@ def test_marble_wood_turbulence_term():
         got = _tex_float(D.marble(turbulence=tb, depth=3, size=1.7), pts)
         exp = 0.5 + 0.5 * np.sin(5 * pts.astype(np.float64).sum(1) + tb * T.astype(np.float64))
         assert np.allclose(got, exp, atol=5e-6), tb
+
+
+# ------------------------------------------- stdPerlin and Blender noise --
+
+def _grad_table():
+    g = np.zeros(768, np.float32)
+    po.lib().yo_grad_table(g)
+    return g.reshape(256, 3).astype(np.float64)
+
+
+def _perm_np():
+    return np.array(_perm(), np.int64)
+
+
+def test_gradient_table_is_256_unit_vectors():
+    """Perlin's original gradients (noise.cc:200-370) are normalised random
+    directions: 256 distinct rows of length 1 to the table's 6 digits."""
+    g = _grad_table()
+    assert np.allclose(np.linalg.norm(g, axis=1), 1.0, atol=1e-4)
+    assert len(np.unique(g.round(6), axis=0)) == 256
+
+
+def test_stdperlin_lattice_and_gradients():
+    """stdPerlin_t (noise.cc:372-435) is 1/2 at lattice points and its
+    gradient there is 0.75 x the unit vector of row
+    (p[p[bx] + by] + bz) & 255, b = (coordinate + 10000) & 255."""
+    L = po.lib()
+    g, H = _grad_table(), _perm_np()
+    # one ulp of the shifted coordinate (x + 10000 in float): the step stays
+    # exact, and the other corners' O(h^2) weights bias the difference by ~1.5 h
+    h = 2.0 ** -10
+    rng = np.random.default_rng(11)
+    for X, Y, Z in rng.integers(-40, 40, (12, 3)):
+        assert L.yo_stdperlin(float(X), float(Y), float(Z)) == 0.5
+        b = [(int(c) + 10000) & 255 for c in (X, Y, Z)]
+        row = (H[H[b[0]] + b[1]] + b[2]) & 255
+        d = [(L.yo_stdperlin(X + h * e[0], Y + h * e[1], Z + h * e[2])
+              - L.yo_stdperlin(X - h * e[0], Y - h * e[1], Z - h * e[2])) / (2 * h)
+             for e in np.eye(3)]
+        assert np.allclose(d, 0.75 * g[row], atol=0.004), (X, Y, Z, d, 0.75 * g[row])
+
+
+def test_blender_noise_lattice_and_gradients():
+    """blenderNoise_t (noise.cc:439-509) is 1/2 at lattice points; the corner
+    weights 1-3t^2+2|t|^3 have zero slope there, so the gradient is the unit
+    vector hashvectf[p[(z & 255) + p[p[x & 255] + (y & 255)]]]."""
+    L = po.lib()
+    g, H = _grad_table(), _perm_np()
+    h = 2.0 ** -12   # the other corners enter with weight 3h^2: a ~1.5 h bias
+    rng = np.random.default_rng(12)
+    for X, Y, Z in rng.integers(-40, 40, (12, 3)):
+        assert L.yo_blendernoise(float(X), float(Y), float(Z)) == 0.5
+        row = H[(Z & 255) + H[H[X & 255] + (Y & 255)]]
+        d = [(L.yo_blendernoise(X + h * e[0], Y + h * e[1], Z + h * e[2])
+              - L.yo_blendernoise(X - h * e[0], Y - h * e[1], Z - h * e[2])) / (2 * h)
+             for e in np.eye(3)]
+        assert np.allclose(d, g[row], atol=0.002), (X, Y, Z, d, g[row])
+
+
+def test_turbulence_and_fbm_over_each_generator():
+    """turbulence sums octaves of the texture's generator, Blender's at
+    p + (1,1,1) (noise.h:60-61, noise.cc:808-819); fBm sums signed octaves
+    without the offset (noise.cc:638-650)."""
+    from yafray_amd import descriptors as D
+    L = po.lib()
+    gens = {1: L.yo_stdperlin, 2: L.yo_blendernoise}
+    pts = np.random.default_rng(13).uniform(-5, 5, (40, 3)).astype(np.float32)
+    for kind, fn in gens.items():
+        t = D.clouds(depth=3, size=1.3, hard=False)
+        t.noise = kind
+        got = po.noise_eval(2, t, pts)
+        for q, p in enumerate(pts):
+            x = (p + 1.0).astype(np.float32) if kind == 2 else p.copy()
+            x = (x * np.float32(1.3)).astype(np.float32)
+            s, amp = np.float32(0), np.float32(1)
+            for _ in range(4):
+                s = np.float32(s + amp * np.float32(fn(*map(float, x))))
+                amp, x = np.float32(amp * 0.5), (x * np.float32(2)).astype(np.float32)
+            exp = np.float32(s * np.float32(np.float32(8) / np.float32(15)))
+            assert got[q] == exp, (kind, q, got[q], exp)
+        m = D.musgrave(H=0.8, lacunarity=2.1, octaves=3.5)
+        m.noise = kind
+        f = po.noise_eval(1, m, pts)
+        pw = np.float32(np.power(2.1, -0.8))
+        for q, p in enumerate(pts[:10]):
+            x, val, pwr = p.copy(), np.float32(0), np.float32(1)
+            for _ in range(3):
+                val = np.float32(val + np.float32(np.float32(2) * np.float32(fn(*map(float, x))) - 1) * pwr)
+                pwr, x = np.float32(pwr * pw), (x * np.float32(2.1)).astype(np.float32)
+            val = np.float32(val + np.float32(0.5) * np.float32(np.float32(2) * np.float32(fn(*map(float, x))) - 1) * pwr)
+            assert abs(f[q] - val) <= 1e-6 * max(1.0, abs(val)), (kind, q, f[q], val)

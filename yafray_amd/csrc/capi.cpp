@@ -179,8 +179,8 @@ extern "C" int yg_set_shading(yg_scene* s, const yg_material* mats, uint32_t nma
         if (mods[k].tex < 0 || (uint32_t)mods[k].tex >= ntex)
             return yg_fail(YG_ERR_ARG, "modulator texture index out of bounds");
     for (uint32_t k = 0; k < ntex; k++) {
-        if (texs[k].noise != YG_NOISE_NEWPERLIN)
-            return yg_fail(YG_ERR_UNSUPPORTED, "only newPerlin_t noise is implemented");
+        if (texs[k].noise < YG_NOISE_NEWPERLIN || texs[k].noise > YG_NOISE_BLENDER)
+            return yg_fail(YG_ERR_ARG, "bad noise generator");
         if (texs[k].type < YG_TEX_CLOUDS || texs[k].type > YG_TEX_MUSGRAVE)
             return yg_fail(YG_ERR_ARG, "bad texture type");
     }

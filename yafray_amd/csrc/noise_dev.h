@@ -9,7 +9,10 @@
 // Each translation unit holds its own copy (no relocatable device code);
 // perm_upload() fills the copy of the unit it is called from.
 static __constant__ unsigned char YG_PERM[512];
+// The unit gradient vectors of stdPerlin_t and blenderNoise_t, 3 per entry.
+static __constant__ float YG_GRAD[768];
 void perm_fill(unsigned char out[512]);
+void grad_fill(float out[768]);
 
 static int perm_upload()
 {
@@ -17,8 +20,11 @@ static int perm_upload()
     if (done) return YG_OK;
     unsigned char h[512];
     perm_fill(h);
+    float g[768];
+    grad_fill(g);
     hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(YG_PERM), h, 512);
-    if (e != hipSuccess) return yg_fail(YG_ERR_HIP, std::string("perm upload: ") + hipGetErrorString(e));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(YG_GRAD), g, sizeof(g));
+    if (e != hipSuccess) return yg_fail(YG_ERR_HIP, std::string("noise table upload: ") + hipGetErrorString(e));
     done = true;
     return YG_OK;
 }
@@ -29,6 +35,16 @@ static int perm_upload()
 __device__ __forceinline__ void perm_to_lds(unsigned char* s)
 {
     for (int i = threadIdx.x; i < 512; i += blockDim.x) s[i] = YG_PERM[i];
+    __syncthreads();
+}
+
+// Both tables into LDS (kernels that may evaluate any noise generator).
+__device__ __forceinline__ void noise_tables_to_lds(unsigned char* perm, float* grad)
+{
+    for (int i = threadIdx.x; i < 768; i += blockDim.x) {
+        if (i < 512) perm[i] = YG_PERM[i];
+        grad[i] = YG_GRAD[i];
+    }
     __syncthreads();
 }
 
@@ -61,42 +77,130 @@ __device__ __forceinline__ float new_perlin(float x, float y, float z, const uns
     return 0.5f + 0.5f * nv;
 }
 
-__device__ __forceinline__ float signed_noise(float x, float y, float z, const unsigned char* perm = YG_PERM)
+// stdPerlin_t::operator() (noise.cc:372-435): Perlin's original noise over
+// the permutation and the unit gradients, coordinates shifted by 10000 in
+// float (which quantises them), the fade and the result formed in double.
+__device__ __forceinline__ float std_perlin(float x, float y, float z, const unsigned char* H, const float* G)
 {
-    return 2.0f * new_perlin(x, y, z, perm) - 1.0f;
+    int b0[3], b1[3];
+    float r0[3], r1[3];
+    const float vec[3] = {x, y, z};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float t = (float)((double)vec[k] + 10000.0);
+        b0[k] = ((int)t) & 255;
+        b1[k] = (b0[k] + 1) & 255;
+        r0[k] = t - (float)(int)t;
+        r1[k] = (float)((double)r0[k] - 1.0);
+    }
+    const int i = H[b0[0]], j = H[b1[0]];
+    const int b00 = H[i + b0[1]], b10 = H[j + b0[1]], b01 = H[i + b1[1]], b11 = H[j + b1[1]];
+    auto surve = [](float t) { return (float)((double)(t * t) * (3.0 - 2.0 * (double)t)); };
+    const float sx = surve(r0[0]), sy = surve(r0[1]), sz = surve(r0[2]);
+    // stdp_g has 514 rows (256 twice + 2): row k is gradient k & 255
+    auto at = [&](int row, float rx, float ry, float rz) {
+        const float* q = G + 3 * (row & 255);
+        return rx * q[0] + ry * q[1] + rz * q[2];
+    };
+    float u = at(b00 + b0[2], r0[0], r0[1], r0[2]), v = at(b10 + b0[2], r1[0], r0[1], r0[2]);
+    float a = plerp(sx, u, v);
+    u = at(b01 + b0[2], r0[0], r1[1], r0[2]);
+    v = at(b11 + b0[2], r1[0], r1[1], r0[2]);
+    float b = plerp(sx, u, v);
+    const float c = plerp(sy, a, b);
+    u = at(b00 + b1[2], r0[0], r0[1], r1[2]);
+    v = at(b10 + b1[2], r1[0], r0[1], r1[2]);
+    a = plerp(sx, u, v);
+    u = at(b01 + b1[2], r0[0], r1[1], r1[2]);
+    v = at(b11 + b1[2], r1[0], r1[1], r1[2]);
+    b = plerp(sx, u, v);
+    const float d = plerp(sy, a, b);
+    return (float)(0.5 + 0.75 * (double)plerp(sz, c, d));
+}
+
+// blenderNoise_t::operator() (noise.cc:439-509): corner weights
+// 1 - 3t^2 + 2|t|^3 formed in double, gradients from the same unit vectors
+// hashed through the permutation, clamped to [0, 1].
+__device__ __forceinline__ float blender_noise(float x, float y, float z, const unsigned char* H, const float* G)
+{
+    const int ix = (int)floorf(x), iy = (int)floorf(y), iz = (int)floorf(z);
+    const float ox = x - (float)ix, oy = y - (float)iy, oz = z - (float)iz;
+    const float jx = ox - 1.0f, jy = oy - 1.0f, jz = oz - 1.0f;
+    auto wp = [](float o) { const float c = o * o; return (float)(1.0 - 3.0 * (double)c + 2.0 * (double)c * (double)o); };
+    auto wn = [](float j) { const float c = j * j; return (float)(1.0 - 3.0 * (double)c - 2.0 * (double)c * (double)j); };
+    const float cn1 = wp(ox), cn2 = wp(oy), cn3 = wp(oz), cn4 = wn(jx), cn5 = wn(jy), cn6 = wn(jz);
+    const int b00 = H[H[ix & 255] + (iy & 255)], b10 = H[H[(ix + 1) & 255] + (iy & 255)];
+    const int b01 = H[H[ix & 255] + ((iy + 1) & 255)], b11 = H[H[(ix + 1) & 255] + ((iy + 1) & 255)];
+    const int b20 = iz & 255, b21 = (iz + 1) & 255;
+    auto term = [&](float i, int hidx, float px, float py, float pz) {
+        const float* h = G + 3 * H[hidx];
+        return i * (h[0] * px + h[1] * py + h[2] * pz);
+    };
+    float n = 0.5f;
+    n += term(cn1 * cn2 * cn3, b20 + b00, ox, oy, oz);
+    n += term(cn1 * cn2 * cn6, b21 + b00, ox, oy, jz);
+    n += term(cn1 * cn5 * cn3, b20 + b01, ox, jy, oz);
+    n += term(cn1 * cn5 * cn6, b21 + b01, ox, jy, jz);
+    n += term(cn4 * cn2 * cn3, b20 + b10, jx, oy, oz);
+    n += term(cn4 * cn2 * cn6, b21 + b10, jx, oy, jz);
+    n += term(cn4 * cn5 * cn3, b20 + b11, jx, jy, oz);
+    n += term(cn4 * cn5 * cn6, b21 + b11, jx, jy, jz);
+    return (n < 0.0f) ? 0.0f : ((n > 1.0f) ? 1.0f : n);
+}
+
+// The noise generator of a texture (newNoise, basictex.cc:38-63)
+__device__ __forceinline__ float noise3(int kind, float x, float y, float z, const unsigned char* perm,
+                                        const float* grad)
+{
+    if (kind == YG_NOISE_STDPERLIN) return std_perlin(x, y, z, perm, grad);
+    if (kind == YG_NOISE_BLENDER) return blender_noise(x, y, z, perm, grad);
+    return new_perlin(x, y, z, perm);
+}
+
+// getSignedNoise (noise.h:245-248)
+__device__ __forceinline__ float signed_noise(float x, float y, float z, const unsigned char* perm = YG_PERM,
+                                              int kind = YG_NOISE_NEWPERLIN, const float* grad = YG_GRAD)
+{
+    return 2.0f * noise3(kind, x, y, z, perm, grad) - 1.0f;
 }
 
 // fBm_t::operator() (noise.cc:638-650) with pwHL = pow(lacunarity, -H)
 // given: it is the same for every point of a texture, so bulk kernels take
 // it from the host (dpow there equals the oracle's).
 __device__ __forceinline__ float fbm_w(float x, float y, float z, float pwHL, float lac, float oct,
-                                       const unsigned char* perm = YG_PERM)
+                                       const unsigned char* perm = YG_PERM, int kind = YG_NOISE_NEWPERLIN,
+                                       const float* grad = YG_GRAD)
 {
     float value = 0, pwr = 1;
     for (int i = 0; i < (int)oct; i++) {
-        value += signed_noise(x, y, z, perm) * pwr;
+        value += signed_noise(x, y, z, perm, kind, grad) * pwr;
         pwr *= pwHL;
         x *= lac; y *= lac; z *= lac;
     }
     float rmd = oct - floorf(oct);
-    if (rmd != 0.f) value += rmd * signed_noise(x, y, z, perm) * pwr;
+    if (rmd != 0.f) value += rmd * signed_noise(x, y, z, perm, kind, grad) * pwr;
     return value;
 }
 
 __device__ __forceinline__ float fbm(float x, float y, float z, float H, float lac, float oct,
-                                     const unsigned char* perm = YG_PERM)
+                                     const unsigned char* perm = YG_PERM, int kind = YG_NOISE_NEWPERLIN,
+                                     const float* grad = YG_GRAD)
 {
-    return fbm_w(x, y, z, dpow(lac, -H), lac, oct, perm);
+    return fbm_w(x, y, z, dpow(lac, -H), lac, oct, perm, kind, grad);
 }
 
-// turbulence (noise.cc:808-819); newPerlin_t adds no coordinate offset
+// turbulence (noise.cc:808-819) over the texture's noise generator
 __device__ __forceinline__ float turbulence(float x, float y, float z, int oct, float size, int hard,
-                                            const unsigned char* perm = YG_PERM)
+                                            const unsigned char* perm = YG_PERM, int kind = YG_NOISE_NEWPERLIN,
+                                            const float* grad = YG_GRAD)
 {
     float val, amp = 1, sum = 0;
+    if (kind == YG_NOISE_BLENDER) {  // only blenderNoise_t offsets the point (noise.h:60-61)
+        x += 1.0f; y += 1.0f; z += 1.0f;
+    }
     x *= size; y *= size; z *= size;
     for (int i = 0; i <= oct; i++, amp *= 0.5f, x *= 2.0f, y *= 2.0f, z *= 2.0f) {
-        val = new_perlin(x, y, z, perm);
+        val = noise3(kind, x, y, z, perm, grad);
         if (hard) val = fabsf(2.0f * val - 1.0f);
         sum += amp * val;
     }
@@ -104,11 +208,12 @@ __device__ __forceinline__ float turbulence(float x, float y, float z, int oct, 
 }
 
 // texture getFloat (basictex.cc:82-90, 133-153, 194-216, 677-680)
-__device__ __forceinline__ float tex_float(const yg_texture& t, f3 p, const unsigned char* perm = YG_PERM)
+__device__ __forceinline__ float tex_float(const yg_texture& t, f3 p, const unsigned char* perm = YG_PERM,
+                                           const float* grad = YG_GRAD)
 {
     switch (t.type) {
     case YG_TEX_CLOUDS: {
-        float v = turbulence(p.x, p.y, p.z, t.depth, t.size, t.hard, perm);
+        float v = turbulence(p.x, p.y, p.z, t.depth, t.size, t.hard, perm, t.noise, grad);
         if (t.bias) {
             v *= v;
             if (t.bias == 1) return -v;
@@ -120,11 +225,11 @@ __device__ __forceinline__ float tex_float(const yg_texture& t, f3 p, const unsi
         float w;
         if (t.type == YG_TEX_MARBLE) {
             w = (float)((double)(p.x + p.y + p.z) * 5.0 +
-                        ((t.turb == 0.0f) ? 0.0 : (double)(t.turb * turbulence(p.x, p.y, p.z, t.depth, t.size, t.hard, perm))));
+                        ((t.turb == 0.0f) ? 0.0 : (double)(t.turb * turbulence(p.x, p.y, p.z, t.depth, t.size, t.hard, perm, t.noise, grad))));
         } else {
             if (t.rings) w = (float)((double)sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 20.0);
             else w = (float)((double)(p.x + p.y + p.z) * 10.0);
-            w = (float)((double)w + ((t.turb == 0.0f) ? 0.0 : (double)(t.turb * turbulence(p.x, p.y, p.z, t.depth, t.size, t.hard, perm))));
+            w = (float)((double)w + ((t.turb == 0.0f) ? 0.0 : (double)(t.turb * turbulence(p.x, p.y, p.z, t.depth, t.size, t.hard, perm, t.noise, grad))));
         }
         switch (t.shape) {
         case YG_SHAPE_SAW:
@@ -142,15 +247,16 @@ __device__ __forceinline__ float tex_float(const yg_texture& t, f3 p, const unsi
         return w;
     }
     case YG_TEX_MUSGRAVE:
-        return t.iscale * fbm(p.x * t.size, p.y * t.size, p.z * t.size, t.H, t.lacunarity, t.octaves, perm);
+        return t.iscale * fbm(p.x * t.size, p.y * t.size, p.z * t.size, t.H, t.lacunarity, t.octaves, perm, t.noise, grad);
     }
     return 0.0f;
 }
 
 // getColor = color1 + getFloat(p)*(color2 - color1)
-__device__ __forceinline__ c3 tex_color(const yg_texture& t, f3 p, const unsigned char* perm = YG_PERM)
+__device__ __forceinline__ c3 tex_color(const yg_texture& t, f3 p, const unsigned char* perm = YG_PERM,
+                                        const float* grad = YG_GRAD)
 {
-    float f = tex_float(t, p, perm);
+    float f = tex_float(t, p, perm, grad);
     c3 c1 = mkc(t.color1[0], t.color1[1], t.color1[2]);
     c3 c2 = mkc(t.color2[0], t.color2[1], t.color2[2]);
     return cadd(c1, cscale(csub(c2, c1), f));

@@ -97,17 +97,22 @@ __global__ void k_noise_eval(int what, yg_texture tex, float pwHL, const float* 
                              float* __restrict__ out)
 {
     __shared__ unsigned char perm[512];
-    perm_to_lds(perm);
+    __shared__ float grad[768];
+    if (tex.noise == YG_NOISE_NEWPERLIN) perm_to_lds(perm);  // (tex is a kernel argument: uniform)
+    else noise_tables_to_lds(perm, grad);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const f3 p = ld3(pts + 3 * (size_t)i);
+    const int kind = tex.noise;
     switch (what) {
-    case YG_EVAL_NOISE: out[i] = new_perlin(p.x, p.y, p.z, perm); break;
-    case YG_EVAL_FBM: out[i] = fbm_w(p.x, p.y, p.z, pwHL, tex.lacunarity, tex.octaves, perm); break;
-    case YG_EVAL_TURBULENCE: out[i] = turbulence(p.x, p.y, p.z, tex.depth, tex.size, tex.hard, perm); break;
-    case YG_EVAL_TEX_FLOAT: out[i] = tex_float(tex, p, perm); break;
+    case YG_EVAL_NOISE: out[i] = noise3(kind, p.x, p.y, p.z, perm, grad); break;
+    case YG_EVAL_FBM: out[i] = fbm_w(p.x, p.y, p.z, pwHL, tex.lacunarity, tex.octaves, perm, kind, grad); break;
+    case YG_EVAL_TURBULENCE:
+        out[i] = turbulence(p.x, p.y, p.z, tex.depth, tex.size, tex.hard, perm, kind, grad);
+        break;
+    case YG_EVAL_TEX_FLOAT: out[i] = tex_float(tex, p, perm, grad); break;
     case YG_EVAL_TEX_COLOR: {
-        const c3 c = tex_color(tex, p, perm);
+        const c3 c = tex_color(tex, p, perm, grad);
         out[3 * (size_t)i] = c.r; out[3 * (size_t)i + 1] = c.g; out[3 * (size_t)i + 2] = c.b;
         break;
     }
@@ -213,8 +218,7 @@ extern "C" int yg_noise_eval(int32_t what, const yg_texture* tex, const float* d
     yg_texture t{};
     if (tex) t = *tex;
     if ((what >= YG_EVAL_TEX_FLOAT) && !tex) return yg_fail(YG_ERR_ARG, "texture required");
-    if (tex && t.noise != YG_NOISE_NEWPERLIN)
-        return yg_fail(YG_ERR_UNSUPPORTED, "only newPerlin_t noise is implemented");
+    if (t.noise < YG_NOISE_NEWPERLIN || t.noise > YG_NOISE_BLENDER) return yg_fail(YG_ERR_ARG, "bad noise generator");
     if (what >= YG_EVAL_TEX_FLOAT && (t.type < YG_TEX_CLOUDS || t.type > YG_TEX_MUSGRAVE))
         return yg_fail(YG_ERR_ARG, "bad texture type");
     int rc = perm_upload();

@@ -172,11 +172,12 @@ int textureFactory(const std::string& type, const paramMap_t& p, yg_texture& T)
     memset(&T, 0, sizeof(T));
     set3(T.color1, 0, 0, 0);
     set3(T.color2, 1, 1, 1);
+    // newNoise (basictex.cc:38-63): any other name is newPerlin_t
     std::string ntype;
     p.getParam("noise_type", ntype);
-    if (!ntype.empty() && ntype != "newperlin")
+    if (ntype.find("voronoi") != std::string::npos || ntype == "cellnoise")
         return fail(YG_ERR_UNSUPPORTED, "noise_type '" + ntype + "' is not on the GPU path");
-    T.noise = YG_NOISE_NEWPERLIN;
+    T.noise = (ntype == "blender") ? YG_NOISE_BLENDER : (ntype == "stdperlin") ? YG_NOISE_STDPERLIN : YG_NOISE_NEWPERLIN;
     p.getParam("color1", T.color1);
     p.getParam("color2", T.color2);
     std::string shape;
