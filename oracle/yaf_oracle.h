@@ -128,7 +128,7 @@ int yo_render_adjoint(yo_scene* s, const yg_camera* cam, const yg_material* mats
  * image with the reference's sample streams (rgb: resx*resy*3; px, py, n,
  * samples, seed unused); YO_FP64_CONVERGED: the n pixels (px[q], py[q]) with
  * `samples` samples of every sampled light (rgb: n*3).  Returns 0, -2 bad
- * arguments, -4 non-shadow-casting objects, -5 AA or tone mapping, -6 photon
+ * arguments, -5 AA or tone mapping, -6 photon
  * lights or a path light used in indirect lighting, -7 several path lights. */
 #define YO_FP64_REFERENCE 0
 #define YO_FP64_CONVERGED 1

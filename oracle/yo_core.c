@@ -303,6 +303,13 @@ static int box_hit(const yo_node* nd, const double* o, const double* d, double t
 int yo_query(const yo_scene* s, yo_v3 o, yo_v3 d, float tmin, float tmax, int mode,
              int filter_obj, int include, int32_t* prim, float* t, uint64_t* cn, uint64_t* ct)
 {
+    return yo_query_s(s, o, d, tmin, tmax, mode, filter_obj, include, 0, prim, t, cn, ct);
+}
+
+int yo_query_s(const yo_scene* s, yo_v3 o, yo_v3 d, float tmin, float tmax, int mode,
+               int filter_obj, int include, int shadow, int32_t* prim, float* t, uint64_t* cn,
+               uint64_t* ct)
+{
     int32_t best = -1;
     float bt = tmax;
     if (s->nt == 0) { *prim = -1; *t = tmax; return 0; }
@@ -325,6 +332,7 @@ int yo_query(const yo_scene* s, yo_v3 o, yo_v3 d, float tmin, float tmax, int mo
         if (nd->left < 0) {
             for (int32_t i = nd->start; i < nd->start + nd->count; i++) {
                 int32_t tri = s->order[i];
+                if (shadow && yo_skips_shadow(s, s->tri_obj[tri])) continue;
                 if (filter_obj >= 0) {
                     int same = s->tri_obj[tri] == filter_obj;
                     if (include ? !same : same) continue;
@@ -482,9 +490,17 @@ static void log_put(float* buf, int64_t cap, int64_t* n, const float* rec)
 int yo_first_hit_one(const yo_scene* s, yo_v3 from, yo_v3 ray, float min_raydis, int32_t* prim,
                      float* t)
 {
+    return yo_first_hit_one_s(s, from, ray, min_raydis, 0, prim, t);
+}
+
+/* shadow: firstHit's shadow flag (scene.cc:698, objects that cast no shadows
+ * are skipped) */
+int yo_first_hit_one_s(const yo_scene* s, yo_v3 from, yo_v3 ray, float min_raydis, int shadow,
+                       int32_t* prim, float* t)
+{
     yo_v3 f = v3_add(from, v3_mul(ray, min_raydis));
     int32_t p; float tt;
-    yo_query(s, f, ray, 0.0f, INFINITY, 0, -1, 0, &p, &tt, NULL, NULL);
+    yo_query_s(s, f, ray, 0.0f, INFINITY, 0, -1, 0, shadow, &p, &tt, NULL, NULL);
     if (p >= 0 && !(tt > 0.0f)) p = -1;
     *prim = p;
     *t = tt;
@@ -541,10 +557,10 @@ static int is_shadowed_q(const yo_scene* s, yo_v3 p, yo_v3 l, int self_obj, int 
     yo_v3 q = v3_add(p, v3_mul(ray, min_raydis));
     int32_t pr; float tt;
     if (self_obj >= 0) {
-        if (yo_query(s, self, ray, 0.0f, dist, 1, self_obj, 1, &pr, &tt, NULL, NULL)) return 1;
-        return yo_query(s, q, ray, 0.0f, dist, 1, self_obj, 0, &pr, &tt, NULL, NULL);
+        if (yo_query_s(s, self, ray, 0.0f, dist, 1, self_obj, 1, 1, &pr, &tt, NULL, NULL)) return 1;
+        return yo_query_s(s, q, ray, 0.0f, dist, 1, self_obj, 0, 1, &pr, &tt, NULL, NULL);
     }
-    return yo_query(s, q, ray, 0.0f, dist, 1, -1, 0, &pr, &tt, NULL, NULL);
+    return yo_query_s(s, q, ray, 0.0f, dist, 1, -1, 0, 1, &pr, &tt, NULL, NULL);
 }
 
 void yo_is_shadowed(yo_scene* s, const float* p, const float* l, const int32_t* self_obj,

@@ -143,6 +143,8 @@ typedef struct {
     const yg_texture* texs;
     const yg_light* lights;
     int nl;
+    const int32_t* oflags;   /* per object, bit0 cast shadows (NULL: all cast) */
+    int nobj;
     yg_render_opts o;
     int mode, samples;
     unsigned bases[MAXDIM];
@@ -176,7 +178,7 @@ static int dbox(const yo_node* nd, const double* o, const double* d, double t0, 
  * double; mode 0 closest (tmin <= t < best, ties to the lower id), 1 any
  * (tmin < t < tmax); the object filter of yo_query */
 static int dquery(const fctx* F, d3 o, d3 d, double tmin, double tmax, int mode, int filter_obj,
-                  int include, int32_t* prim, double* t)
+                  int include, int shadow, int32_t* prim, double* t)
 {
     const yo_scene* s = F->s;
     int32_t best = -1;
@@ -205,6 +207,10 @@ static int dquery(const fctx* F, d3 o, d3 d, double tmin, double tmax, int mode,
         }
         for (int32_t i = nd->start; i < nd->start + nd->count; i++) {
             const int32_t tri = s->order[i];
+            if (shadow && F->oflags) {  /* shadow queries skip objects that cast no shadows */
+                const int32_t ob = s->tri_obj[tri];
+                if (ob >= 0 && ob < F->nobj && !(F->oflags[ob] & 1)) continue;
+            }
             if (filter_obj >= 0) {
                 const int same = s->tri_obj[tri] == filter_obj;
                 if (include ? !same : same) continue;
@@ -256,13 +262,14 @@ static void make_sp(const fctx* F, int32_t prim, d3 f, d3 ray, double Z, dsp* sp
     sp->mat = F->s->tri_mat[prim];
 }
 
-/* scene_t::firstHit (scene.cc:688-721): closest hit from from + ray*min_raydis, Z > 0 */
-static int first_hit(const fctx* F, d3 from, d3 ray, dsp* sp)
+/* scene_t::firstHit (scene.cc:688-721): closest hit from from + ray*min_raydis,
+ * Z > 0; shadow: its shadow flag (objects that cast no shadows are skipped) */
+static int first_hit(const fctx* F, d3 from, d3 ray, dsp* sp, int shadow)
 {
     const d3 f = dadd(from, dmul(ray, (double)F->o.min_raydis));
     int32_t p;
     double t;
-    dquery(F, f, ray, 0.0, INFINITY, 0, -1, 0, &p, &t);
+    dquery(F, f, ray, 0.0, INFINITY, 0, -1, 0, shadow, &p, &t);
     if (p < 0 || !(t > 0)) return 0;
     make_sp(F, p, f, ray, t, sp);
     return 1;
@@ -280,10 +287,10 @@ static int is_shadowed(const fctx* F, d3 p, d3 l, int self_obj, int to_point)
     int32_t pr;
     double tt;
     if (self_obj >= 0) {
-        if (dquery(F, self, ray, 0.0, dist, 1, self_obj, 1, &pr, &tt)) return 1;
-        return dquery(F, q, ray, 0.0, dist, 1, self_obj, 0, &pr, &tt);
+        if (dquery(F, self, ray, 0.0, dist, 1, self_obj, 1, 1, &pr, &tt)) return 1;
+        return dquery(F, q, ray, 0.0, dist, 1, self_obj, 0, 1, &pr, &tt);
     }
-    return dquery(F, q, ray, 0.0, dist, 1, -1, 0, &pr, &tt);
+    return dquery(F, q, ray, 0.0, dist, 1, -1, 0, 1, &pr, &tt);
 }
 
 /* ------------------------------------------------------------- shading -- */
@@ -465,7 +472,7 @@ static dc path_take_sample(const fctx* F, fsamp* S, int li, d3 N, const dsp* sp)
             for (int j = 0; j < L->depth; ++j) {
                 if (cenergy(raycolor) < 0.05) break;
                 dsp t;
-                if (!first_hit(F, where, ray, &t)) {
+                if (!first_hit(F, where, ray, &t, 1)) {
                     const dc contri = cmul(cscale(raycolor, cos0), background(F));
                     total = cadd(total, contri);
                     if (first) subtotal[i % 4] = cadd(subtotal[i % 4], contri);
@@ -542,7 +549,7 @@ static dc raytrace_pixel(const fctx* F, fsamp* S, const dcam* k, int x, int y)
 {
     const d3 ray = dnorm(dadd(dadd(dmul(k->vright, x + 0.5), dmul(k->vup, y + 0.5)), k->vto));
     dsp sp;
-    if (first_hit(F, k->pos, ray, &sp)) return scene_light(F, S, &sp, k->pos, 0);
+    if (first_hit(F, k->pos, ray, &sp, 0)) return scene_light(F, S, &sp, k->pos, 0);
     return background(F);
 }
 
@@ -636,8 +643,6 @@ int yo_render_fp64(yo_scene* s, const yg_camera* cam, const yg_material* mats, i
     if (mode != YO_FP64_REFERENCE && mode != YO_FP64_CONVERGED) return -2;
     if (mode == YO_FP64_CONVERGED && (samples < 1 || (n > 0 && (!px || !py)))) return -2;
     if (opts->aa_passes > 0 || opts->tonemap) return -5;
-    for (int o = 0; o < nobj; o++)
-        if (obj_flags && !(obj_flags[o] & 1)) return -4;
     int npath = 0, sequential = 0;
     for (int li = 0; li < nl; li++) {
         const yg_light* L = &lights[li];
@@ -653,6 +658,8 @@ int yo_render_fp64(yo_scene* s, const yg_camera* cam, const yg_material* mats, i
     fctx F;
     memset(&F, 0, sizeof(F));
     F.s = s; F.mats = mats; F.mods = mods; F.texs = texs; F.lights = lights; F.nl = nl;
+    F.oflags = obj_flags;
+    F.nobj = nobj;
     F.o = *opts;
     F.mode = mode;
     F.samples = samples;

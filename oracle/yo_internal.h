@@ -147,12 +147,28 @@ struct yo_scene {
     int64_t nnodes;
     int32_t* order;
     yo_photonmap pm;
+    /* object flags of the render in progress (bit0 cast shadows), or NULL */
+    const int32_t* oflags;
+    int64_t noflags;
 };
+
+/* scene shadow queries (isShadowed; firstHit with its shadow flag,
+ * scene.cc:698) skip objects that cast no shadows */
+static inline int yo_skips_shadow(const yo_scene* s, int32_t obj)
+{
+    return s->oflags && obj >= 0 && obj < s->noflags && !(s->oflags[obj] & 1);
+}
 
 /* closest query with object filter: only triangles with (filter_obj < 0) or
  * (include ? obj == filter_obj : obj != filter_obj).  mode 0 closest, 1 any. */
 int yo_query(const yo_scene* s, yo_v3 o, yo_v3 d, float tmin, float tmax, int mode,
              int filter_obj, int include, int32_t* prim, float* t, uint64_t* cn, uint64_t* ct);
+int yo_first_hit_one_s(const yo_scene* s, yo_v3 from, yo_v3 ray, float min_raydis, int shadow,
+                       int32_t* prim, float* t);
+/* the same, skipping objects that cast no shadows when shadow != 0 */
+int yo_query_s(const yo_scene* s, yo_v3 o, yo_v3 d, float tmin, float tmax, int mode,
+               int filter_obj, int include, int shadow, int32_t* prim, float* t, uint64_t* cn,
+               uint64_t* ct);
 
 static inline yo_v3 tri_vert(const yo_scene* s, int64_t tri, int k)
 {

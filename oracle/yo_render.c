@@ -88,11 +88,12 @@ static void make_sp(const yo_scene* s, int32_t prim, yo_v3 f, yo_v3 ray, float Z
     sp->mat = s->tri_mat[prim];
 }
 
-/* scene_t::firstHit (scene.cc:688-721) */
-static int first_hit(rctx* R, yo_v3 from, yo_v3 ray, yo_sp* sp)
+/* scene_t::firstHit (scene.cc:688-721); shadow: its shadow flag, which
+ * skips objects that cast no shadows (the pathlight's rays, pathlight.cc:317) */
+static int first_hit(rctx* R, yo_v3 from, yo_v3 ray, yo_sp* sp, int shadow)
 {
     int32_t p; float t;
-    if (!yo_first_hit_one(R->s, from, ray, R->opts.min_raydis, &p, &t)) return 0;
+    if (!yo_first_hit_one_s(R->s, from, ray, R->opts.min_raydis, shadow, &p, &t)) return 0;
     yo_v3 f = v3_add(from, v3_mul(ray, R->opts.min_raydis));
     make_sp(R->s, p, f, ray, t, sp);
     return 1;
@@ -305,7 +306,7 @@ static yo_col path_take_sample(rctx* R, int li, yo_v3 N, const yo_sp* sp)
                 if (col_energy(raycolor) < 0.05f) break;
                 yo_sp t;
                 R->stats[1]++;
-                if (!first_hit(R, where, ray, &t)) {
+                if (!first_hit(R, where, ray, &t, 1)) {
                     yo_col contri = col_mul(col_scale(raycolor, v3_dot(startray, N)), background(R));
                     total = col_add(total, contri);
                     if (first) subtotal[i % 4] = col_add(subtotal[i % 4], contri);
@@ -626,7 +627,7 @@ static void shoot_diffuse(pctx* P, yo_photon* ph, yo_v3 dir)
     P->depth++;
     yo_sp sp;
     R->stats[1]++;
-    if (!first_hit(R, ph->pos, dir, &sp)) { P->depth--; return; }
+    if (!first_hit(R, ph->pos, dir, &sp, 0)) { P->depth--; return; }
     /* photon_t::position, photonlight.cc:39-47 */
     float ad = v3_len(v3_sub(ph->pos, sp.P));
     if (ad > P->L->bias) ph->lastpos = ph->pos;
@@ -679,6 +680,8 @@ static void rctx_init(rctx* R, yo_scene* s, const yg_material* mats, int nmat,
 {
     memset(R, 0, sizeof(*R));
     R->s = s; R->mats = mats; R->nmat = nmat; R->mods = mods; R->texs = texs;
+    s->oflags = obj_flags;  /* the scene's shadow queries see the render's object flags */
+    s->noflags = nobj;
     R->lights = lights; R->nl = nl; R->obj_flags = obj_flags; R->nobj = nobj;
     R->opts = *opts;
     R->lcg = (int32_t)opts->lcg_seed;
@@ -1000,7 +1003,7 @@ static yo_col raytrace(rctx* R, yo_v3 from, yo_v3 ray)
 {
     yo_sp sp;
     R->stats[0]++;
-    if (first_hit(R, from, ray, &sp)) {
+    if (first_hit(R, from, ray, &sp, 0)) {
         R->primary = 1;
         yo_col c = scene_light(R, &sp, from, 0);
         R->primary = 0;
@@ -1260,8 +1263,6 @@ int yo_render_mt(yo_scene* s, const yg_camera* cam, const yg_material* mats, int
                  const yg_render_opts* opts, float* rgb, uint64_t* stats, int nthreads)
 {
     (void)nmod; (void)ntex;
-    for (int o = 0; o < nobj; o++)
-        if (obj_flags && !(obj_flags[o] & 1)) return -4; /* non-shadow-casting objects */
     int sequential = 0;
     for (int li = 0; li < nl; li++)
         if (lights[li].type == YG_LIGHT_AREA) sequential = 1;

@@ -328,3 +328,27 @@ def test_roulette_mode_keeps_the_mean(torch_cuda):
     gl = torch.zeros(len(d.lights), dtype=torch.float64, device="cuda")
     with pytest.raises(Exception, match="roulette"):
         sc.render_adjoint(g, out, gm, None, gl, opts=o)
+
+
+@pytest.mark.parametrize("cast", [False, True])
+def test_objects_that_cast_no_shadows(torch_cuda, cast):
+    """A quad above the camera that casts no shadows: isShadowed and the
+    pathlight's shadow-flag firstHit skip it (GPU = oracle, same ray counts),
+    while the raw any-hit query (kdTree_t::IntersectS semantics) still hits it."""
+    import test_oracle_render as TR
+    torch = torch_cuda
+    d, _ = TR._occluded_plane(cast)
+    img, st, sc = gpu_render(d, torch)
+    ref, rs = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat).render(d)
+    assert rel_rmse(img, ref) <= 1e-3
+    assert st.rays_extension == rs[1] and st.rays_shadow == rs[2]
+    p = torch.tensor([[0.3, 0.2, 0.0]], dtype=torch.float32, device="cuda")
+    up = torch.tensor([[0.0, 0.0, 1.0]], dtype=torch.float32, device="cuda")
+    so = torch.tensor([0], dtype=torch.int32, device="cuda")
+    out = torch.empty(1, dtype=torch.uint8, device="cuda")
+    sc.is_shadowed(p, up, so, out, 0, self_bias=1e-3)
+    assert int(out.item()) == int(cast)
+    ray = torch.tensor([[0.3, 0.2, 0.01, 0.0, 0.0, 0.0, 1.0, float("inf")]], dtype=torch.float32, device="cuda")
+    occ = torch.empty(1, dtype=torch.uint8, device="cuda")
+    sc.trace_any(ray, occ)
+    assert int(occ.item()) == 1

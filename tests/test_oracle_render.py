@@ -157,3 +157,31 @@ def test_photon_estimate_closed_form():
     exp = ((8 * math.pi / 3) * P * np.array([1.0, 0.8, 0.6]) * np.array([0.6, 0.5, 0.3]) / h ** 2
            * (c ** 4)[:, None])
     assert np.max(np.abs(got / exp - 1)) < 0.03, got / exp
+
+
+def _occluded_plane(cast):
+    """plane_desc plus a quad at z = 5 over the whole view (object 1): camera
+    rays (z = 3, looking down) never reach it; sun and path rays do."""
+    d, sun = plane_desc(S_=4, resx=48, resy=40)
+    v = np.vstack([d.verts, np.array([[-60, -60, 5], [60, -60, 5], [60, 60, 5], [-60, 60, 5]], np.float32)])
+    f = np.vstack([d.idx, np.array([[4, 5, 6], [4, 6, 7]], np.uint32)])
+    return D.SceneDesc(v, f, [0, 0, 1, 1], [0, 0, 0, 0], [7, 7 if cast else 6], d.cam, list(d.mats), [], [],
+                       list(d.lights), d.opts), sun
+
+
+def test_objects_that_cast_no_shadows():
+    """scene_t::isShadowed and the pathlight's firstHit(shadow = true) skip
+    objects whose castShadows() is false (scene.cc:107-185, 698); camera rays
+    still see them.  With the quad not casting, the image is the open-sky
+    closed form exactly; casting, it blocks the sun and the sky."""
+    d, sun = _occluded_plane(cast=False)
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    img, st = oc.render(d)
+    exp = plane_expected(4, 48, 40, sun, 2.0)
+    assert np.max(np.abs(img - exp) / exp) < 2e-5
+    d2, _ = _occluded_plane(cast=True)
+    img2, _ = oc.render(d2)
+    assert np.all(img2 < 0.05 * exp)
+    # the FP64 oracle follows the same rule
+    f64 = oc.render_fp64(d)
+    assert np.max(np.abs(f64 - exp) / exp) < 2e-5

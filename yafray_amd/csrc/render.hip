@@ -439,7 +439,8 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WP
             next = 1;
             float t;
             const f3 f = where + ray * R.min_raydis;
-            const int ti = first_hit<W>(R.bvh, where, ray, R.min_raydis, t, st);
+            // pathlight.cc:317: firstHit with the shadow flag
+            const int ti = first_hit<W>(R.bvh, where, ray, R.min_raydis, t, st, true);
             if (ti < 0) {
                 if (!ADJ) {
                     contri[(size_t)p * D + j] = cmul(cscale(raycolor, s), R.bg);
@@ -1023,6 +1024,7 @@ struct RenderWS {
     int lds_cap = 0;           // lds_cap argument of those launches (0: no overflow)
     DevBuf<int> stk_ovf;       // their stack entries beyond the LDS part
     int width = 2;         // BVH width of the scenes in the launch
+    bool nocast = false;   // some scene of the launch has objects that cast no shadows
     bool any_area = false, any_photon = false;
     // antialiasing passes: first-pass areas and colours, the pass's samples
     DevBuf<Area> areas0;
@@ -1101,8 +1103,6 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
         for (size_t t = 0; t < s->tri_mat.size(); t++)
             if (s->tri_mat[t] < 0 || (size_t)s->tri_mat[t] >= s->mats.size())
                 return yg_fail(YG_ERR_ARG, "triangle material id out of range");
-        for (int32_t f : s->obj_flags)
-            if (!(f & 1)) return yg_fail(YG_ERR_UNSUPPORTED, "objects that cast no shadows are not implemented");
         s->checked_geo = s->geo_gen;
         s->checked_nmat = s->mats.size();
     }
@@ -1273,14 +1273,14 @@ static int trace_levels(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
             YG_HIP(hipEventRecord(ev[0], st));
         }
         uint32_t* qc = W.qcnt.p + j;
-        YG_LAUNCH_W(W.width, (k_ext<ADJ, 4>), (k_ext<ADJ, 2>), dim3(grid), dim3(RB), W.stk_bytes_lvl, st, W.scenes.p,
+        YG_LAUNCH_WS(W.width, W.nocast, (k_ext<ADJ, 4>), (k_ext<ADJ, 2>), (k_ext<ADJ, 12>), (k_ext<ADJ, 10>), dim3(grid), dim3(RB), W.stk_bytes_lvl, st, W.scenes.p,
                     W.areas.p, na, j, act, W.nact.p + j, npaths, S, D, W.item_ti.p, W.item_t.p, W.qoff.p,
                     W.hidx.p, W.st_where.p, W.st_hn.p, W.st_ti.p, W.st_col.p, W.st_s.p, W.contri.p, W.nseg.p,
                     W.h_p.p, W.h_ray.p, W.h_ti.p, qc, W.queue.p, W.flag.p, W.hbase_on ? W.hbase.p : nullptr,
                     W.cnt.p, W.lds_cap, W.stk_ovf.p);
         YG_HIP(hipGetLastError());
         if (ktiming) YG_HIP(hipEventRecord(ev[1], st));
-        YG_LAUNCH_W(W.width, (k_shadow<ADJ, 4>), (k_shadow<ADJ, 2>), dim3(grid), dim3(RB), W.stk_bytes_lvl, st,
+        YG_LAUNCH_WS(W.width, W.nocast, (k_shadow<ADJ, 4>), (k_shadow<ADJ, 2>), (k_shadow<ADJ, 12>), (k_shadow<ADJ, 10>), dim3(grid), dim3(RB), W.stk_bytes_lvl, st,
                     W.scenes.p, W.areas.p, na, S, qc, W.queue.p, W.h_p.p, W.h_ti.p, W.vis.p, W.cnt.p,
                     W.lds_cap, W.stk_ovf.p);
         YG_HIP(hipGetLastError());
@@ -1391,11 +1391,11 @@ static int launch_frame(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
     if (W.any_area || nl_max > 32) {
         // registers hold this kernel to 2 waves per SIMD, so LDS has room for
         // the whole stack
-        YG_LAUNCH_W(W.width, (k_direct<4, true>), (k_direct<2, true>), dim3(grid_for(nitems, RB)), dim3(RB),
+        YG_LAUNCH_WS(W.width, W.nocast, (k_direct<4, true>), (k_direct<2, true>), (k_direct<12, true>), (k_direct<10, true>), dim3(grid_for(nitems, RB)), dim3(RB),
                     W.stk_bytes, st, W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lcg_off.p,
                     W.lightc.p, nl_max, W.cnt.p, 0, W.stk_ovf.p, (const uint32_t*)nullptr);
     } else {
-        YG_LAUNCH_W(W.width, (k_sun_vis<4>), (k_sun_vis<2>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes_lvl,
+        YG_LAUNCH_WS(W.width, W.nocast, (k_sun_vis<4>), (k_sun_vis<2>), (k_sun_vis<12>), (k_sun_vis<10>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes_lvl,
                     st, W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.sun_vis.p, W.cnt.p,
                     W.lds_cap, W.stk_ovf.p);
         YG_HIP(hipGetLastError());
@@ -1555,6 +1555,10 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     W.stk_entries = max_depth;
     W.stk_lds = 0;
     for (auto& P : plans) W.stk_lds = std::max(W.stk_lds, P.s->stack_lds);
+    W.nocast = false;
+    for (auto& P : plans)
+        for (int32_t f : P.s->obj_flags)
+            if (!(f & 1)) W.nocast = true;
     const std::vector<Area> areas0(areas);  // the first pass's areas
     append_area_index(areas, nitems);
     if ((rc = W.scenes.ensure(scenes.size())) || (rc = W.areas.ensure(areas.size()))) return rc;
@@ -1590,7 +1594,8 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     // uploaded descriptors, not baked into the launches).
     const std::array<uint64_t, 13> key = {g_alloc_serial, nitems, npaths, (uint64_t)S, (uint64_t)D,
                                           (uint64_t)W.na, (uint64_t)nl_max, (uint64_t)W.width, W.stk_bytes,
-                                          W.scan_tmp.cap, any_area, any_photon, any_path};
+                                          W.scan_tmp.cap, any_area, any_photon,
+                                          (uint64_t)any_path | ((uint64_t)W.nocast << 1)};
     int graph = 0;
     W.aa_pass = -1;
     W.hbase_on = aa_passes > 0 && any_path;
@@ -1951,7 +1956,7 @@ __global__ __launch_bounds__(RB) void k_adjoint_path(const RenderScene* __restri
                 if (b == 0) s = dot(ray, N);
                 if (cenergy(raycolor) < 0.05f) break;
                 float t;
-                const int ti = first_hit<W>(R.bvh, where, ray, R.min_raydis, t, st);
+                const int ti = first_hit<W>(R.bvh, where, ray, R.min_raydis, t, st, true);
                 nb = b + 1;
                 if (ti < 0) {
                     hit[b] = false;
@@ -2108,7 +2113,7 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
     if (!fused) {
         if ((rc = W.w_item.ensure(W.nitems)) || (rc = W.adeg.ensure(1))) return rc;
         YG_HIP(hipMemsetAsync(W.adeg.p, 0, sizeof(unsigned long long), st));
-        YG_LAUNCH_W(W.width, (k_adjoint_direct<4>), (k_adjoint_direct<2>), dim3(std::min(grid_for(W.nitems, RB), 4 * ADJ_GRID)), dim3(RB), W.stk_bytes, st, W.scenes.p,
+        YG_LAUNCH_WS(W.width, W.nocast, (k_adjoint_direct<4>), (k_adjoint_direct<2>), (k_adjoint_direct<12>), (k_adjoint_direct<10>), dim3(std::min(grid_for(W.nitems, RB), 4 * ADJ_GRID)), dim3(RB), W.stk_bytes, st, W.scenes.p,
                            W.areas.p, W.na, W.nitems, W.item_ti.p, W.item_t.p, d_grad_rgb, G, 1, W.S, W.D,
                            W.qual.p, W.contri.p, W.nseg.p, W.w_item.p);
         YG_HIP(hipGetLastError());
@@ -2125,7 +2130,7 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
         }
     }
     if (fused) {
-        YG_LAUNCH_W(W.width, (k_adjoint_direct<4>), (k_adjoint_direct<2>), dim3(std::min(grid_for(W.nitems, RB), 4 * ADJ_GRID)), dim3(RB), W.stk_bytes, st, W.scenes.p,
+        YG_LAUNCH_WS(W.width, W.nocast, (k_adjoint_direct<4>), (k_adjoint_direct<2>), (k_adjoint_direct<12>), (k_adjoint_direct<10>), dim3(std::min(grid_for(W.nitems, RB), 4 * ADJ_GRID)), dim3(RB), W.stk_bytes, st, W.scenes.p,
                            W.areas.p, W.na, W.nitems, W.item_ti.p, W.item_t.p, d_grad_rgb, G, 0, W.S, W.D,
                            W.qual.p, W.contri.p, W.nseg.p, (c3*)nullptr);
         YG_HIP(hipGetLastError());
@@ -2148,7 +2153,7 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
             if ((rc = W.scan_tmp.ensure(bytes))) return rc;
             YG_HIP(hipcub::DeviceRadixSort::SortPairsDescending(W.scan_tmp.p, bytes, W.akey0.p, W.akey1.p,
                                                                 W.active0.p, W.active1.p, (int)n0, 0, 4, st));
-            YG_LAUNCH_W(W.width, (k_adjoint_path<4>), (k_adjoint_path<2>), dim3(grid_for(n0, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, W.na,
+            YG_LAUNCH_WS(W.width, W.nocast, (k_adjoint_path<4>), (k_adjoint_path<2>), (k_adjoint_path<12>), (k_adjoint_path<10>), dim3(grid_for(n0, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, W.na,
                                W.active1.p, n0, W.npaths, W.S, W.D, W.item_ti.p, W.item_t.p, W.qoff.p, W.hidx.p,
                                d_grad_rgb, G);
             YG_HIP(hipGetLastError());

@@ -204,7 +204,8 @@ extern "C" int yg_is_shadowed(yg_scene* s, const float* d_p, const float* d_l,
     if (rc) return rc;
     if (n == 0) return YG_OK;
     if (!d_p || !d_l || !d_out) return yg_fail(YG_ERR_ARG, "null buffer");
-    YG_LAUNCH_W(s->width, (k_is_shadowed<4>), (k_is_shadowed<2>), dim3(grid_for(n, TB)), dim3(TB),
+    YG_LAUNCH_WS(s->width, s->view().nocast != 0, (k_is_shadowed<4>), (k_is_shadowed<2>), (k_is_shadowed<12>),
+                 (k_is_shadowed<10>), dim3(grid_for(n, TB)), dim3(TB),
                 stack_lds_bytes(s->stack_entries, TB), (hipStream_t)stream, s->view(), d_p, d_l, d_self_obj, n, to_point,
                 min_raydis, self_bias, d_out);
     YG_HIP(hipGetLastError());

@@ -258,7 +258,7 @@ template <int W>
 __device__ __forceinline__ int visit(const BVHView& bvh, int node, const RayBox& rb, float lo, float hi,
                                      LaneStack& st)
 {
-    if constexpr (W == 4) return visit4(bvh, node, rb, lo, hi, st);
+    if constexpr ((W & 7) == 4) return visit4(bvh, node, rb, lo, hi, st);
     else return visit2(bvh, node, rb, lo, hi, st);
 }
 
@@ -266,13 +266,40 @@ struct TraceStats {
     uint32_t nodes, tris;
 };
 
+// The traversal kernels' W template parameter is the BVH width (2 or 4),
+// plus 8 for launches with objects that cast no shadows: only those
+// instantiations test object flags in their shadow queries, so the leaf loops
+// of every other launch stay as they are.
+#define YG_W_NOCAST 8
+template <int W>
+constexpr bool yg_nocast_w() { return (W & YG_W_NOCAST) != 0; }
+
+// Whether a shadow query in this wave must look at object flags (lanes of
+// scenes whose objects all cast shadows read flags that never skip).
+template <int W>
+__device__ __forceinline__ bool wave_nocast(const BVHView& bvh, bool shadow)
+{
+    if constexpr (!yg_nocast_w<W>()) return false;
+    else return shadow && __ballot(bvh.nocast != 0) != 0ull;
+}
+
+// Whether a scene shadow query (scene_t::isShadowed, and firstHit with its
+// shadow flag, scene.cc:698) skips triangle ti: its object casts no shadows.
+__device__ __forceinline__ bool skips_shadow(const BVHView& bvh, int ti)
+{
+    const int o = (bvh.nobj == 1) ? 0 : gload(bvh.tobj + ti);
+    return !(gload(bvh.oflags + o) & 1);
+}
+
 // Closest hit: accepts tmin <= t < tbest (kdtree.cc:690-694); exact ties go to
 // the lowest primitive id so the result does not depend on the tree.
 // Returns the leaf-ordered triangle index or -1; tbest holds the distance.
+// shadow: a scene shadow query (objects that cast no shadows are skipped).
 template <bool COUNT, int W>
 __device__ int trace_closest(const BVHView& bvh, f3 o, f3 d, float tmin, float& tbest,
-                             LaneStack& st, TraceStats* cs = nullptr)
+                             LaneStack& st, TraceStats* cs = nullptr, bool shadow = false)
 {
+    const bool skip = wave_nocast<W>(bvh, shadow);
     const RayBox rb = make_raybox(o, d);
     const float lo = tmin - 1e-4f * fabsf(tmin) - 1e-30f;
     int best = -1;
@@ -290,6 +317,7 @@ __device__ int trace_closest(const BVHView& bvh, f3 o, f3 d, float tmin, float& 
             const int first = code >> 4, cnt = (code & 15) + 1;
             for (int k = 0; k < cnt; k++) {
                 const int ti = first + k;
+                if (skip && skips_shadow(bvh, ti)) continue;
                 f3 a, b, c, nn;
                 load_tri(bvh.tris, ti, a, b, c, nn);
                 if (COUNT) cs->tris++;
@@ -319,8 +347,10 @@ __device__ int trace_closest(const BVHView& bvh, f3 o, f3 d, float tmin, float& 
 // parameter and must cover both origins' segments.
 template <bool COUNT, int W>
 __device__ bool trace_any(const BVHView& bvh, f3 o, f3 d, float tmin, float tmax, int self_obj,
-                          f3 oself, float blo, float bhi, LaneStack& st, TraceStats* cs = nullptr)
+                          f3 oself, float blo, float bhi, LaneStack& st, TraceStats* cs = nullptr,
+                          bool shadow = false)
 {
+    const bool skip = wave_nocast<W>(bvh, shadow);
     const RayBox rb = make_raybox(o, d);
     const float hi = cull_bound(bhi);
     int node = 0;
@@ -335,6 +365,7 @@ __device__ bool trace_any(const BVHView& bvh, f3 o, f3 d, float tmin, float tmax
             const int first = code >> 4, cnt = (code & 15) + 1;
             for (int k = 0; k < cnt; k++) {
                 const int ti = first + k;
+                if (skip && skips_shadow(bvh, ti)) continue;
                 f3 a, b, c, nn;
                 load_tri(bvh.tris, ti, a, b, c, nn);
                 if (COUNT) cs->tris++;
@@ -354,13 +385,14 @@ __device__ bool trace_any(const BVHView& bvh, f3 o, f3 d, float tmin, float tmax
 
 // scene_t::firstHit (scene.cc:688-721): origin from + ray*min_raydis, closest
 // hit with Z > 0 (a zero-distance closest hit is rejected by the scene).
+// shadow: firstHit's shadow flag (objects that cast no shadows are skipped).
 template <int W>
 __device__ __forceinline__ int first_hit(const BVHView& bvh, f3 from, f3 ray, float min_raydis,
-                                         float& t, LaneStack& st)
+                                         float& t, LaneStack& st, bool shadow = false)
 {
     const f3 f = from + ray * min_raydis;
     t = INFINITY;
-    int ti = trace_closest<false, W>(bvh, f, ray, 0.0f, t, st);
+    int ti = trace_closest<false, W>(bvh, f, ray, 0.0f, t, st, nullptr, shadow);
     if (ti >= 0 && !(t > 0.0f)) ti = -1;
     return ti;
 }
@@ -387,5 +419,5 @@ __device__ __forceinline__ bool is_shadowed(const BVHView& bvh, f3 p, f3 l, int 
     const float slack = 1e-4f * (fabsf(self_bias) + fabsf(min_raydis)) + 1e-30f;
     float blo = fminf(0.0f, delta) - slack;
     float bhi = dist + fmaxf(0.0f, delta) + slack;
-    return trace_any<false, W>(bvh, q, ray, 0.0f, dist, self_obj, self, blo, bhi, st);
+    return trace_any<false, W>(bvh, q, ray, 0.0f, dist, self_obj, self, blo, bhi, st, nullptr, true);
 }

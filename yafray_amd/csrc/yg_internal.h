@@ -67,6 +67,8 @@ struct BVHView {
     const int32_t* tmat;    // material id per leaf-ordered triangle
     int32_t ntris;
     int32_t nobj;           // objects in the scene (1: every triangle is object 0)
+    const int32_t* oflags;  // per object: bit0 cast shadows, bit1 receive, bit2 use for radiosity
+    int32_t nocast;         // some object casts no shadows (scene shadow queries skip it)
 };
 
 struct HostBVH {
@@ -150,6 +152,10 @@ struct yg_scene {
         v.nodes = d_nodes; v.nodes4 = d_nodes4; v.width = width; v.tris = d_tris; v.tprim = d_tprim; v.tobj = d_tobj; v.tmat = d_tmat;
         v.ntris = (int32_t)tri_mat.size();
         v.nobj = nobj;
+        v.oflags = d_obj_flags;
+        v.nocast = 0;
+        for (int32_t f : obj_flags)
+            if (!(f & 1)) v.nocast = 1;
         return v;
     }
 };
@@ -186,4 +192,11 @@ static inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)(
     do {                                                                         \
         if ((width) == 4) hipLaunchKernelGGL(K4, grid, block, shm, stream, __VA_ARGS__); \
         else hipLaunchKernelGGL(K2, grid, block, shm, stream, __VA_ARGS__);       \
+    } while (0)
+// The same for kernels with shadow queries: K12 / K10 are the width-4 / width-2
+// instantiations that test object flags (W + 8), used when nocast is set.
+#define YG_LAUNCH_WS(width, nocast, K4, K2, K12, K10, grid, block, shm, stream, ...)            \
+    do {                                                                                        \
+        if (nocast) YG_LAUNCH_W(width, K12, K10, grid, block, shm, stream, __VA_ARGS__);        \
+        else YG_LAUNCH_W(width, K4, K2, grid, block, shm, stream, __VA_ARGS__);                 \
     } while (0)
