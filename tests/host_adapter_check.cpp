@@ -80,6 +80,14 @@ int main()
         printf("ropts %g %d %d %d %g %g %d\n", o.self_bias, o.raydepth, o.aa_passes, o.aa_minsamples,
                o.aa_pixelwidth, o.aa_threshold, o.aa_jitterfirst);
     }
+    // noise_type names as newNoise (basictex.cc:38-63) reads them
+    for (const char* nt : {"blender", "stdperlin", "newperlin", "", "anything", "voronoi_f1", "cellnoise"}) {
+        paramMap_t tp;
+        tp.setString("noise_type", nt);
+        yg_texture T;
+        const int rc = yafgpu::textureFactory("clouds", tp, T);
+        printf("noise %s %d %d\n", *nt ? nt : "-", rc, rc ? -1 : T.noise);
+    }
     // unsupported features are refused with a status, not an exit
     paramMap_t cache;
     cache.setBool("cache", true);

@@ -74,3 +74,12 @@ def test_render_options_match_python_spec(adapter_output):
         assert _f(r[1]) == pytest.approx(o.self_bias) and int(r[2]) == o.raydepth
         assert (int(r[3]), int(r[4]), int(r[7])) == (o.aa_passes, o.aa_minsamples, o.aa_jitterfirst)
         assert _f(r[5]) == pytest.approx(o.aa_pixelwidth) and _f(r[6]) == pytest.approx(o.aa_threshold)
+
+
+def test_noise_type_names(adapter_output):
+    """blender / stdperlin select those generators; voronoi and cellnoise are
+    refused; any other name (or none) is newPerlin_t, as in newNoise."""
+    got = {l[1]: (int(l[2]), int(l[3])) for l in adapter_output if l[0] == "noise"}
+    assert got["blender"] == (0, 2) and got["stdperlin"] == (0, 1)
+    assert got["newperlin"] == got["-"] == got["anything"] == (0, 0)
+    assert got["voronoi_f1"][0] == D_ERR_UNSUPPORTED and got["cellnoise"][0] == D_ERR_UNSUPPORTED
