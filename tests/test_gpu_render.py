@@ -352,3 +352,25 @@ def test_objects_that_cast_no_shadows(torch_cuda, cast):
     occ = torch.empty(1, dtype=torch.uint8, device="cuda")
     sc.trace_any(ray, occ)
     assert int(occ.item()) == 1
+
+
+@pytest.mark.parametrize("path_first", [False, True])
+def test_area_light_inside_gi_paths(torch_cuda, path_first):
+    """A pathlight whose path vertices light an area light (use_in_indirect,
+    pathlight.cc:186-191): the vertices' ourRandom() draws sit in the stream
+    between the primary hit's lights in light order, sample by sample and
+    bounce by bounce (scene.cc:260-277, pathlight.cc:305-347).  GPU = oracle,
+    same shadow rays and draws, with the pathlight before or after the area
+    light."""
+    torch = torch_cuda
+    d = S.direct_room(64, 48, subdiv=2)
+    path = D.pathlight(samples=4, depth=2, use_qmc=True, threshold=1e3)
+    lights = [path] + list(d.lights) if path_first else list(d.lights) + [path]
+    d = D.SceneDesc(d.verts, d.idx, d.tri_obj, d.tri_mat, d.obj_flags, d.cam, list(d.mats), [], [], lights,
+                    d.opts)
+    img, st, _ = gpu_render(d, torch)
+    ref, rs = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat).render(d)
+    assert np.all(np.isfinite(img))
+    assert rel_rmse(img, ref) <= 1e-3
+    assert st.rays_camera == rs[0] and st.rays_extension == rs[1] and st.rays_shadow == rs[2]
+    assert st.lcg_draws == rs[4]

@@ -164,12 +164,13 @@ __global__ __launch_bounds__(RB) void k_prep(const RenderScene* __restrict__ sce
                                              const Area* __restrict__ areas, int nareas,
                                              uint32_t nitems, const int32_t* __restrict__ item_ti,
                                              const float* __restrict__ item_t,
-                                             uint32_t* __restrict__ draws, uint32_t* __restrict__ qual)
+                                             uint32_t* __restrict__ draws, uint32_t* __restrict__ qual,
+                                             uint32_t* __restrict__ pre)
 {
     const uint32_t item = blockIdx.x * RB + threadIdx.x;
     if (item >= nitems) return;
     const int ti = item_ti[item];
-    uint32_t dr = 0, q = 0;
+    uint32_t dr = 0, q = 0, drp = 0;
     if (ti >= 0) {
         const Area a = areas[find_area(areas, nareas, item)];
         const RenderScene& R = scenes[a.scene];
@@ -178,7 +179,10 @@ __global__ __launch_bounds__(RB) void k_prep(const RenderScene* __restrict__ sce
         for (int li = 0; li < R.nl; li++) {
             const DevLight& L = R.lights[li];
             if (!L.use_in_render || L.type != YG_LIGHT_AREA) continue;
-            if (dot(sp.P - L.from, L.dir) > 0) dr += 2u * (uint32_t)L.samples;
+            if (dot(sp.P - L.from, L.dir) > 0) {
+                dr += 2u * (uint32_t)L.samples;
+                if (li < R.path_light) drp += 2u * (uint32_t)L.samples;
+            }
         }
         if (R.path_light >= 0 && R.lights[R.path_light].use_in_render) {
             const c3 dif = get_diffuse(R.mats, R.mods, R.texs, sp.mat, sp.P);
@@ -187,6 +191,7 @@ __global__ __launch_bounds__(RB) void k_prep(const RenderScene* __restrict__ sce
     }
     draws[item] = dr;
     qual[item] = q;
+    if (pre) pre[item] = drp;  // draws of the area lights before the pathlight
 }
 
 // Sun and area lights at the primary hit, one colour per light.
@@ -235,7 +240,8 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_DIRECT_WP
                                                const uint32_t* __restrict__ lcg_off,
                                                c3* __restrict__ lightc, int nl_max,
                                                Counters* __restrict__ cnt, int lds_cap, int* __restrict__ stk_ovf,
-                                               const uint32_t* __restrict__ sun_vis)
+                                               const uint32_t* __restrict__ sun_vis,
+                                               const uint32_t* __restrict__ pathd = nullptr)
 {
     extern __shared__ int s_stk[];
     const uint32_t item = blockIdx.x * RB + threadIdx.x;
@@ -254,9 +260,15 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_DIRECT_WP
             const Surf sp = primary_surf(R, a, item, ti, item_t[item], eye);
             int32_t seed = 0;
             bool seeded = false;
+            uint64_t skip = 0;  // draws of the pathlight's vertices passed before the first area light
             for (int li = 0; li < R.nl; li++) {
                 const DevLight& L = R.lights[li];
                 c3 c = mkc(0, 0, 0);
+                if (AREA && pathd && li == R.path_light) {
+                    // the pathlight's GI paths draw here in the stream (area lights at their vertices)
+                    if (seeded) seed = lcg_jump((uint32_t)seed, pathd[item], YG_LCG_POW);
+                    else skip += pathd[item];
+                }
                 if (L.use_in_render) {
                     if (L.type == YG_LIGHT_SUN) {
                         if (AREA) c = sun_illuminate<W>(R, L, sp, eye, st, nshadow);
@@ -264,7 +276,8 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_DIRECT_WP
                             c = from_light(R, sp, L.dir, L.col, eye);  // sunLight_t::illuminate (sunlight.cc:35-57)
                     } else if (AREA && L.type == YG_LIGHT_AREA) {
                         if (!seeded) {
-                            seed = lcg_jump(R.lcg_seed, (uint64_t)(lcg_off[item] - lcg_off[R.item_base]), YG_LCG_POW);
+                            seed = lcg_jump(R.lcg_seed, (uint64_t)(lcg_off[item] - lcg_off[R.item_base]) + skip,
+                                            YG_LCG_POW);
                             seeded = true;
                         }
                         if (dot(sp.P - L.from, L.dir) > 0) ndraws += 2u * (uint32_t)L.samples;
@@ -497,6 +510,18 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_SHADOW_WP
 
 #define SB 256  // shading block size (no traversal stack)
 
+// Hit vertices of the GI paths whose area lights are evaluated after the
+// levels (pathlight + area lights used in indirect light): entry p*D + j.
+// Null pointers: the launch has no such lights.
+struct VertexRec {
+    f3* p;             // hit point
+    f3* ray;           // direction the path arrived along
+    int32_t* ti;       // leaf-ordered triangle
+    uint32_t* vis;     // sun shadow bits of the vertex (k_shadow)
+    c3* rs;            // raycolor * (N.w0) of the path at this vertex
+    uint32_t* draws;   // ourRandom() draws of its area lights (0: no vertex or none lit)
+};
+
 // Forward: the level's light sum, contribution and next-vertex state.
 // ADJ (replay): the same state update, plus the gradient terms of this path
 // vertex.  With W = dLoss/dpixel * rho0 * mult * power (per item) and the
@@ -519,7 +544,7 @@ __global__ __launch_bounds__(SB) void k_shade(const RenderScene* __restrict__ sc
                                               const float* __restrict__ st_s, c3* __restrict__ contri,
                                               uint8_t* __restrict__ nseg, uint32_t* __restrict__ flag,
                                               AdjAcc G, const c3* __restrict__ w_item,
-                                              unsigned long long* __restrict__ degenerate)
+                                              unsigned long long* __restrict__ degenerate, VertexRec vx)
 {
     extern __shared__ double s_adj[];
     AdjLds& Ls = *reinterpret_cast<AdjLds*>(s_adj);
@@ -561,6 +586,26 @@ __global__ __launch_bounds__(SB) void k_shade(const RenderScene* __restrict__ sc
             contri[(size_t)p * D + j] = cmul(cscale(raycolor, s), fl);
             rho = get_diffuse(R.mats, R.mods, R.texs, sp.mat, sp.P);
             nseg[p] = (uint8_t)(j + 1);
+            if (vx.draws) {
+                // area lights used in indirect light draw 2 x samples at a vertex
+                // in front of their plane (arealight.cc:93-153); k_vertex_light
+                // redoes this vertex's light sum once their stream positions are known
+                uint32_t dr = 0;
+                for (int li = 0; li < R.nl; li++) {
+                    const DevLight& L = R.lights[li];
+                    if (L.use_in_indirect && L.type == YG_LIGHT_AREA && dot(sp.P - L.from, L.dir) > 0)
+                        dr += 2u * (uint32_t)L.samples;
+                }
+                const size_t v = (size_t)p * D + j;
+                vx.draws[v] = dr;
+                if (dr) {
+                    vx.p[v] = sp.P;
+                    vx.ray[v] = ray;
+                    vx.ti[v] = ti;
+                    vx.vis[v] = mask;
+                    vx.rs[v] = cscale(raycolor, s);
+                }
+            }
         } else {
             // diffuse_d's rho is get_diffuse's value (one modulator at most)
             const RhoD r = diffuse_d(R.mats, R.mods, R.texs, sp.mat, sp.P);
@@ -634,6 +679,87 @@ __global__ void k_compact(const Area* __restrict__ areas, int nareas, uint32_t n
     // the path's rank among the area's paths at this level (plus, with
     // antialiasing, the level's draws in the area's earlier passes)
     hidx_level[p] = e - ex[(size_t)areas[ai].item_start * S] + (hbase_level ? hbase_level[(size_t)ai * D] : 0u);
+}
+
+// Clears the vertex draw counts of a frame (graph-safe: no memset node).
+__global__ void k_fill_u32(uint32_t* __restrict__ a, size_t n, uint32_t v)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// An item's ourRandom() draws: its primary-hit area lights plus every draw
+// its GI paths' vertices take (vx_off: exclusive scan of the vertex draws
+// in (item, sample, level) order, the order takeSample visits them).
+__global__ void k_item_pathd(uint32_t nitems, int S, int D, const uint32_t* __restrict__ vx_draws,
+                             const uint32_t* __restrict__ vx_off, const uint32_t* __restrict__ draws,
+                             uint32_t* __restrict__ pathd, uint32_t* __restrict__ tdraws)
+{
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= nitems) return;
+    const size_t b = (size_t)item * S * D, e = b + (size_t)S * D - 1;
+    const uint32_t pd = vx_off[e] + vx_draws[e] - vx_off[b];
+    pathd[item] = pd;
+    tdraws[item] = draws[item] + pd;
+}
+
+// scene_t::light(indirect) at the GI vertices that light area lights
+// (pathlight.cc:186-191): the sun terms from k_shadow's bits and the area
+// lights drawing from the stream position of the vertex -- after the item's
+// draws, its area lights before the pathlight and the earlier vertices of its
+// paths -- summed in light order like k_shade's sum, which this replaces.
+template <int W>
+__global__ __launch_bounds__(RB) void k_vertex_light(const RenderScene* __restrict__ scenes,
+                                                    const Area* __restrict__ areas, int nareas, uint32_t nv,
+                                                    int S, int D, VertexRec vx, const uint32_t* __restrict__ vx_off,
+                                                    const uint32_t* __restrict__ lcg_off,
+                                                    const uint32_t* __restrict__ pre, c3* __restrict__ contri,
+                                                    Counters* __restrict__ cnt, int lds_cap,
+                                                    int* __restrict__ stk_ovf)
+{
+    extern __shared__ int s_stk[];
+    const uint32_t v = blockIdx.x * RB + threadIdx.x;
+    uint32_t nshadow = 0, ndraws = 0;
+    if (v < nv && vx.draws[v] > 0) {
+        LaneStack st;
+        st.init(s_stk, threadIdx.x, RB);
+        st.spill_to(lds_cap, stk_ovf + v, gridDim.x * RB);
+        const uint32_t p = v / (uint32_t)D, item = p / (uint32_t)S;
+        const Area& a = areas[find_area(areas, nareas, item)];
+        const RenderScene& R = scenes[a.scene];
+        const int ti = vx.ti[v];
+        const f3 ray = vx.ray[v];
+        Surf sp;
+        sp.ti = ti;
+        sp.P = vx.p[v];
+        const float4 A = R.bvh.tris[3 * ti], B = R.bvh.tris[3 * ti + 1], C = R.bvh.tris[3 * ti + 2];
+        sp.Ng = mk3(A.w, B.w, C.w);
+        sp.N = sp.Ng;
+        create_cs(sp.N, sp.NU, sp.NV);
+        sp.obj = R.bvh.tobj[ti];
+        sp.mat = R.bvh.tmat[ti];
+        const f3 eye = (sp.P + (-ray)) - sp.P;
+        const uint64_t pos = (uint64_t)(lcg_off[item] - lcg_off[R.item_base]) + pre[item] +
+                             (vx_off[v] - vx_off[(size_t)item * S * D]);
+        int32_t seed = lcg_jump(R.lcg_seed, pos, YG_LCG_POW);
+        const uint32_t mask = vx.vis[v];
+        c3 fl = mkc(0, 0, 0);
+        for (int li = 0; li < R.nl; li++) {
+            const DevLight& L = R.lights[li];
+            if (!L.use_in_indirect) continue;
+            if (L.type == YG_LIGHT_SUN) {
+                if (L.cast_shadows && (mask >> li & 1u)) fl = cadd(fl, mkc(0, 0, 0));
+                else fl = cadd(fl, from_light(R, sp, L.dir, L.col, eye));
+            } else if (L.type == YG_LIGHT_AREA) {
+                fl = cadd(fl, area_illuminate<W>(R, L, sp, eye, seed, st, nshadow));
+            }
+        }
+        contri[v] = cmul(vx.rs[v], fl);
+        ndraws = vx.draws[v];
+    }
+    block_add(&cnt->shadow, nshadow);
+    block_add(&cnt->bshadow, nshadow);
+    block_add(&cnt->draws, ndraws);
 }
 
 // Antialiasing passes: adds this pass's draws of one GI level to each area's
@@ -1025,6 +1151,13 @@ struct RenderWS {
     DevBuf<int> stk_ovf;       // their stack entries beyond the LDS part
     int width = 2;         // BVH width of the scenes in the launch
     bool nocast = false;   // some scene of the launch has objects that cast no shadows
+    // a pathlight with area lights used in indirect light: their draws at GI
+    // vertices are placed in the stream after the levels (k_vertex_light)
+    bool area_path = false;
+    DevBuf<f3> vx_p, vx_ray;
+    DevBuf<int32_t> vx_ti;
+    DevBuf<uint32_t> vx_vis, vx_draws, vx_off, pre, pathd, tdraws;
+    DevBuf<c3> vx_rs;
     bool any_area = false, any_photon = false;
     // antialiasing passes: first-pass areas and colours, the pass's samples
     DevBuf<Area> areas0;
@@ -1161,11 +1294,6 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
         P.lights.push_back(L);
     }
     if (R.path_light >= 0) {
-        for (int li = 0; li < R.nl; li++) {
-            const yg_light& Lg = s->lights[li];
-            if (Lg.use_in_indirect && (Lg.type == YG_LIGHT_AREA))
-                return yg_fail(YG_ERR_UNSUPPORTED, "area lights inside pathlight GI paths are not implemented");
-        }
         if (R.depth > 255) return yg_fail(YG_ERR_UNSUPPORTED, "pathlight depth");
         if (R.nl > 32) return yg_fail(YG_ERR_UNSUPPORTED, "more than 32 lights with a pathlight");
     }
@@ -1289,7 +1417,10 @@ static int trace_levels(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
         hipLaunchKernelGGL(k_shade<ADJ>, dim3(sgrid), dim3(SB), ADJ ? sizeof(AdjLds) : 0, st,
                            W.scenes.p, W.areas.p, na, j, S, D, qc, W.queue.p, W.h_p.p, W.h_ray.p, W.h_ti.p,
                            W.vis.p, W.st_where.p, W.st_hn.p, W.st_ti.p, W.st_col.p, W.st_s.p, W.contri.p,
-                           W.nseg.p, W.flag.p, G, w_item, degenerate);
+                           W.nseg.p, W.flag.p, G, w_item, degenerate,
+                           (!ADJ && W.area_path) ? VertexRec{W.vx_p.p, W.vx_ray.p, W.vx_ti.p, W.vx_vis.p, W.vx_rs.p,
+                                                             W.vx_draws.p}
+                                                 : VertexRec{});
         YG_HIP(hipGetLastError());
         if (ktiming) YG_HIP(hipEventRecord(ev[3], st));
         if (j + 1 >= D) break;
@@ -1362,10 +1493,19 @@ static int ensure_frame_buffers(uint32_t nitems, uint32_t npaths, int nl_max, bo
         W.lds_cap = W.stk_lds;
     }
     if ((rc = W.stk_ovf.ensure(1))) return rc;  // a valid pointer for launches that do not spill
-    // hipcub scan storage for both scans of the launch
-    size_t b1 = 0, b2 = 0;
+    const size_t nv = W.area_path && any_path ? (size_t)npaths * D : 0;
+    if (nv) {
+        if ((rc = W.vx_p.ensure(nv)) || (rc = W.vx_ray.ensure(nv)) || (rc = W.vx_ti.ensure(nv)) ||
+            (rc = W.vx_vis.ensure(nv)) || (rc = W.vx_draws.ensure(nv)) || (rc = W.vx_off.ensure(nv)) ||
+            (rc = W.vx_rs.ensure(nv)) || (rc = W.pre.ensure(nitems)) || (rc = W.pathd.ensure(nitems)) ||
+            (rc = W.tdraws.ensure(nitems)))
+            return rc;
+        if (nv >= 0x7fffffffull) return yg_fail(YG_ERR_UNSUPPORTED, "too many GI vertices in one launch");
+    }
+    // hipcub scan storage for every scan of the launch
+    size_t b1 = 0, b2 = 0, b3 = 0;
     if ((rc = scan_bytes(nitems, b1, st)) || (rc = scan_bytes(any_path ? npaths : 0, b2, st)) ||
-        (rc = W.scan_tmp.ensure(std::max(b1, b2))))
+        (rc = scan_bytes(nv, b3, st)) || (rc = W.scan_tmp.ensure(std::max(b1, std::max(b2, b3)))))
         return rc;
     return YG_OK;
 }
@@ -1384,9 +1524,45 @@ static int launch_frame(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
                 W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.cnt.p, W.aa_pass < 0 ? 1 : 0,
                 W.lds_cap, W.stk_ovf.p);
     YG_HIP(hipGetLastError());
+    const bool area_path = W.area_path && W.any_path;
     hipLaunchKernelGGL(k_prep, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
-                       nitems, W.item_ti.p, W.item_t.p, W.draws.p, W.qual.p);
+                       nitems, W.item_ti.p, W.item_t.p, W.draws.p, W.qual.p, area_path ? W.pre.p : nullptr);
     YG_HIP(hipGetLastError());
+    if (area_path) {
+        // the GI levels first: the items' ourRandom() positions depend on their paths' vertices
+        if ((rc = exclusive_scan(W.qual.p, W.qoff.p, nitems, st))) return rc;
+        const size_t nv = (size_t)W.npaths * W.D;
+        hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nv, 256)), dim3(256), 0, st, W.vx_draws.p, nv, 0u);
+        YG_HIP(hipGetLastError());
+        if ((rc = trace_levels<false>(st, ktiming, kev, AdjAcc{}, nullptr, nullptr))) return rc;
+        if ((rc = exclusive_scan(W.vx_draws.p, W.vx_off.p, nv, st))) return rc;
+        hipLaunchKernelGGL(k_item_pathd, dim3(grid_for(nitems, 256)), dim3(256), 0, st, nitems, W.S, W.D,
+                           W.vx_draws.p, W.vx_off.p, W.draws.p, W.pathd.p, W.tdraws.p);
+        YG_HIP(hipGetLastError());
+        if ((rc = exclusive_scan(W.tdraws.p, W.lcg_off.p, nitems, st))) return rc;
+        YG_LAUNCH_WS(W.width, W.nocast, (k_direct<4, true>), (k_direct<2, true>), (k_direct<12, true>),
+                     (k_direct<10, true>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p,
+                     W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lcg_off.p, W.lightc.p, nl_max, W.cnt.p, 0,
+                     W.stk_ovf.p, (const uint32_t*)nullptr, (const uint32_t*)W.pathd.p);
+        YG_HIP(hipGetLastError());
+        const VertexRec vx{W.vx_p.p, W.vx_ray.p, W.vx_ti.p, W.vx_vis.p, W.vx_rs.p, W.vx_draws.p};
+        YG_LAUNCH_WS(W.width, W.nocast, (k_vertex_light<4>), (k_vertex_light<2>), (k_vertex_light<12>),
+                     (k_vertex_light<10>), dim3(grid_for(nv, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p,
+                     na, (uint32_t)nv, W.S, W.D, vx, W.vx_off.p, W.lcg_off.p, W.pre.p, W.contri.p, W.cnt.p, 0,
+                     W.stk_ovf.p);
+        YG_HIP(hipGetLastError());
+        if (W.any_photon) {
+            hipLaunchKernelGGL(k_photon_direct, dim3(grid_for(nitems, 64)), dim3(64), 0, st, W.scenes.p,
+                               W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lightc.p, nl_max);
+            YG_HIP(hipGetLastError());
+        }
+        if (before_final) YG_HIP(hipEventRecord(before_final, st));
+        hipLaunchKernelGGL(k_finalize, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
+                           nitems, W.S, W.D, W.item_ti.p, W.item_t.p, W.qual.p, W.lightc.p, nl_max, W.contri.p,
+                           W.nseg.p, W.cnt.p, W.finalize_to);
+        YG_HIP(hipGetLastError());
+        return YG_OK;
+    }
     if (W.any_area && (rc = exclusive_scan(W.draws.p, W.lcg_off.p, nitems, st))) return rc;
     if (W.any_area || nl_max > 32) {
         // registers hold this kernel to 2 waves per SIMD, so LDS has room for
@@ -1559,6 +1735,30 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     for (auto& P : plans)
         for (int32_t f : P.s->obj_flags)
             if (!(f & 1)) W.nocast = true;
+    W.area_path = false;
+    for (auto& P : plans)
+        if (P.R.path_light >= 0)
+            for (const auto& L : P.s->lights)
+                if (L.use_in_indirect && L.type == YG_LIGHT_AREA) W.area_path = true;
+    if (W.area_path && aa_passes > 0)
+        return yg_fail(YG_ERR_UNSUPPORTED, "antialiasing with area lights inside GI paths");
+    if (W.area_path) {
+        // stream offsets are 32-bit scans: bound the launch's draws (every
+        // primary hit and every path vertex in front of every area light)
+        uint64_t prim = 0, vert = 0;
+        for (auto& P : plans) {
+            uint64_t pp = 0, pv = 0;
+            for (const auto& L : P.s->lights)
+                if (L.type == YG_LIGHT_AREA) {
+                    if (L.use_in_render) pp += 2ull * (uint64_t)L.samples;
+                    if (L.use_in_indirect) pv += 2ull * (uint64_t)L.samples;
+                }
+            prim = std::max(prim, pp);
+            vert = std::max(vert, pv);
+        }
+        if ((double)nitems * (double)prim + (double)npaths * (double)D * (double)vert >= 4294967295.0)
+            return yg_fail(YG_ERR_UNSUPPORTED, "too many area-light draws in one launch with GI paths");
+    }
     const std::vector<Area> areas0(areas);  // the first pass's areas
     append_area_index(areas, nitems);
     if ((rc = W.scenes.ensure(scenes.size())) || (rc = W.areas.ensure(areas.size()))) return rc;
@@ -1595,7 +1795,8 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     const std::array<uint64_t, 13> key = {g_alloc_serial, nitems, npaths, (uint64_t)S, (uint64_t)D,
                                           (uint64_t)W.na, (uint64_t)nl_max, (uint64_t)W.width, W.stk_bytes,
                                           W.scan_tmp.cap, any_area, any_photon,
-                                          (uint64_t)any_path | ((uint64_t)W.nocast << 1)};
+                                          (uint64_t)any_path | ((uint64_t)W.nocast << 1) |
+                                              ((uint64_t)W.area_path << 2)};
     int graph = 0;
     W.aa_pass = -1;
     W.hbase_on = aa_passes > 0 && any_path;
