@@ -374,3 +374,15 @@ def test_area_light_inside_gi_paths(torch_cuda, path_first):
     assert rel_rmse(img, ref) <= 1e-3
     assert st.rays_camera == rs[0] and st.rays_extension == rs[1] and st.rays_shadow == rs[2]
     assert st.lcg_draws == rs[4]
+
+
+def test_area_light_inside_gi_paths_refuses_antialiasing(torch_cuda):
+    """Antialiasing passes continue each area's ourRandom() stream pass by
+    pass; with area lights at path vertices the launch is refused, not
+    rendered differently from the reference."""
+    d = S.direct_room(32, 24, subdiv=1, samples=2)
+    o = D.render_opts(bias=1e-3, raydepth=3, aa_passes=1, aa_minsamples=2)
+    d = D.SceneDesc(d.verts, d.idx, d.tri_obj, d.tri_mat, d.obj_flags, d.cam, list(d.mats), [], [],
+                    list(d.lights) + [D.pathlight(samples=2, depth=2, use_qmc=True, threshold=1e3)], o)
+    with pytest.raises(RuntimeError, match="antialiasing with area lights inside GI paths"):
+        gpu_render(d, torch_cuda)

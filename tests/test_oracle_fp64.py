@@ -45,6 +45,27 @@ def test_reference_streams_match_fp32_area_light():
     assert np.mean(r < 1e-4) > 0.999
 
 
+
+@pytest.mark.parametrize("path_first", [False, True])
+def test_reference_streams_match_fp32_area_light_inside_gi_paths(path_first):
+    """Area lights used in indirect light draw at every path vertex: the FP64
+    render walks pixel, light, sample and bounce in the reference's order, so
+    its stream placement checks the FP32 restatement's (which the GPU's
+    deferred vertex pass is compared with)."""
+    d = S.direct_room(48, 36, subdiv=2, samples=4)
+    path = D.pathlight(samples=4, depth=2, use_qmc=True, threshold=1e3)
+    lights = [path] + list(d.lights) if path_first else list(d.lights) + [path]
+    d = D.SceneDesc(d.verts, d.idx, d.tri_obj, d.tri_mat, d.obj_flags, d.cam, list(d.mats), [], [], lights,
+                    d.opts)
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    f32, rs = oc.render(d)
+    f64 = oc.render_fp64(d, threads=4)
+    assert rs[4] > 4 * 48 * 36 * 8  # the vertices draw far more than the primary hits
+    f32 = f32.astype(np.float64)
+    assert rel_rmse(f32, f64) < 1e-6
+    r = np.abs(f32 - f64).max(axis=2) / np.maximum(np.abs(f64).max(axis=2), 1e-6)
+    assert np.mean(r < 1e-4) > 0.999
+
 def _plane(lights, res=(48, 40), bg=(0.4, 0.5, 0.7)):
     E = 50.0
     v = np.array([[-E, -E, 0], [E, -E, 0], [E, E, 0], [-E, E, 0]], np.float32)
