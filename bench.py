@@ -394,6 +394,46 @@ def bench_direct(args):
     }
 
 
+def bench_room_gi(args):
+    """Pathlight GI whose path vertices light an area light (use_in_indirect):
+    the direct-lighting room, 640x480, area light 16 samples, pathlight 4
+    samples and 2 bounces.  Also times the same frame with the area light kept
+    out of the paths, the cost of the deferred vertex pass beside it."""
+    import torch
+
+    from yafray_amd import descriptors as D
+    from yafray_amd import lib as Y
+    from yafray_amd import scenes as S
+
+    base = S.direct_room()
+    out = torch.zeros(base.cam.resx * base.cam.resy * 3, dtype=torch.float32, device="cuda")
+    res = {}
+    for in_paths in (1, 0):
+        lights = list(base.lights)
+        lights[0].use_in_indirect = in_paths
+        lights.append(D.pathlight(samples=4, depth=2, use_qmc=True, threshold=1e3))
+        d = D.SceneDesc(base.verts, base.idx, base.tri_obj, base.tri_mat, base.obj_flags, base.cam,
+                        list(base.mats), [], [], lights, base.opts)
+        sc = Y.Scene.from_desc(d)
+        dt, st = _timed(lambda: sc.render(out), args.steps, args.warmup)
+        res[in_paths] = (dt, st.rays_camera + st.rays_extension + st.rays_shadow, st.lcg_draws)
+    dt, rays, draws = res[1]
+    return {
+        "metric": "GI Mrays/s", "value": round(rays / dt / 1e6, 3), "unit": "Mrays/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "room + icosphere(20480) + 2 boxes, 640x480, area light 16 samples lit at "
+                               "the primary hit and at every GI vertex, pathlight 4 spp, 2 bounces",
+                   "triangles": int(base.n_tris)},
+        "rays_per_frame": int(rays), "lcg_draws": int(draws),
+        "area_light_outside_paths": {"ms_per_step": round(res[0][0] * 1e3, 3),
+                                     "rays_per_frame": int(res[0][1]),
+                                     "mrays_per_s": round(res[0][1] / res[0][0] / 1e6, 3)},
+        "error_vs_fp64": None,
+    }
+
+
 def bench_autotune(args):
     """Autotuner on the GI terrain frame (1280x720, 4 spp, 3 bounces): every
     (BVH width, leaf size) candidate built and timed on the frame."""
@@ -512,7 +552,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="gi",
                     choices=["gi", "batched", "adjoint", "photon", "direct", "autotune", "aa", "rays",
-                             "noise"])
+                             "noise", "room_gi"])
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy after the timed region (counter calibration)")
@@ -526,7 +566,7 @@ def main():
         torch.cuda.set_device(0)
         fn = {"batched": bench_batched, "adjoint": bench_adjoint, "photon": bench_photon,
               "direct": bench_direct, "autotune": bench_autotune, "aa": bench_aa,
-              "rays": bench_rays, "noise": bench_noise}[args.workload]
+              "rays": bench_rays, "noise": bench_noise, "room_gi": bench_room_gi}[args.workload]
         print(json.dumps(fn(args)))
         return
 

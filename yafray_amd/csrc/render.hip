@@ -703,6 +703,24 @@ __global__ void k_item_pathd(uint32_t nitems, int S, int D, const uint32_t* __re
     tdraws[item] = draws[item] + pd;
 }
 
+// 1 for a vertex that draws (the lit list's flags).
+__global__ void k_vx_flag(size_t n, const uint32_t* __restrict__ vx_draws, uint32_t* __restrict__ flag)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = vx_draws[i] > 0 ? 1u : 0u;
+}
+
+// The drawing vertices in vertex order, and their count (pos: exclusive
+// scan of the flags), so that k_vertex_light's waves hold lit vertices only.
+__global__ void k_vx_list(size_t n, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                          uint32_t* __restrict__ lit, uint32_t* __restrict__ nlit)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (flag[i]) lit[pos[i]] = (uint32_t)i;
+    if (i == n - 1) *nlit = pos[i] + flag[i];
+}
+
 // scene_t::light(indirect) at the GI vertices that light area lights
 // (pathlight.cc:186-191): the sun terms from k_shadow's bits and the area
 // lights drawing from the stream position of the vertex -- after the item's
@@ -710,7 +728,9 @@ __global__ void k_item_pathd(uint32_t nitems, int S, int D, const uint32_t* __re
 // paths -- summed in light order like k_shade's sum, which this replaces.
 template <int W>
 __global__ __launch_bounds__(RB) void k_vertex_light(const RenderScene* __restrict__ scenes,
-                                                    const Area* __restrict__ areas, int nareas, uint32_t nv,
+                                                    const Area* __restrict__ areas, int nareas,
+                                                    const uint32_t* __restrict__ lit,
+                                                    const uint32_t* __restrict__ nlit,
                                                     int S, int D, VertexRec vx, const uint32_t* __restrict__ vx_off,
                                                     const uint32_t* __restrict__ lcg_off,
                                                     const uint32_t* __restrict__ pre, c3* __restrict__ contri,
@@ -718,12 +738,13 @@ __global__ __launch_bounds__(RB) void k_vertex_light(const RenderScene* __restri
                                                     int* __restrict__ stk_ovf)
 {
     extern __shared__ int s_stk[];
-    const uint32_t v = blockIdx.x * RB + threadIdx.x;
+    const uint32_t idx = blockIdx.x * RB + threadIdx.x;
     uint32_t nshadow = 0, ndraws = 0;
-    if (v < nv && vx.draws[v] > 0) {
+    if (idx < *nlit) {
+        const uint32_t v = lit[idx];
         LaneStack st;
         st.init(s_stk, threadIdx.x, RB);
-        st.spill_to(lds_cap, stk_ovf + v, gridDim.x * RB);
+        st.spill_to(lds_cap, stk_ovf + idx, gridDim.x * RB);
         const uint32_t p = v / (uint32_t)D, item = p / (uint32_t)S;
         const Area& a = areas[find_area(areas, nareas, item)];
         const RenderScene& R = scenes[a.scene];
@@ -1157,6 +1178,7 @@ struct RenderWS {
     DevBuf<f3> vx_p, vx_ray;
     DevBuf<int32_t> vx_ti;
     DevBuf<uint32_t> vx_vis, vx_draws, vx_off, pre, pathd, tdraws;
+    DevBuf<uint32_t> vx_flag, vx_pos, vx_lit, vx_nlit;  // the drawing vertices, compacted
     DevBuf<c3> vx_rs;
     bool any_area = false, any_photon = false;
     // antialiasing passes: first-pass areas and colours, the pass's samples
@@ -1498,7 +1520,8 @@ static int ensure_frame_buffers(uint32_t nitems, uint32_t npaths, int nl_max, bo
         if ((rc = W.vx_p.ensure(nv)) || (rc = W.vx_ray.ensure(nv)) || (rc = W.vx_ti.ensure(nv)) ||
             (rc = W.vx_vis.ensure(nv)) || (rc = W.vx_draws.ensure(nv)) || (rc = W.vx_off.ensure(nv)) ||
             (rc = W.vx_rs.ensure(nv)) || (rc = W.pre.ensure(nitems)) || (rc = W.pathd.ensure(nitems)) ||
-            (rc = W.tdraws.ensure(nitems)))
+            (rc = W.tdraws.ensure(nitems)) || (rc = W.vx_flag.ensure(nv)) || (rc = W.vx_pos.ensure(nv)) ||
+            (rc = W.vx_lit.ensure(nv)) || (rc = W.vx_nlit.ensure(1)))
             return rc;
         if (nv >= 0x7fffffffull) return yg_fail(YG_ERR_UNSUPPORTED, "too many GI vertices in one launch");
     }
@@ -1545,11 +1568,17 @@ static int launch_frame(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
                      W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lcg_off.p, W.lightc.p, nl_max, W.cnt.p, 0,
                      W.stk_ovf.p, (const uint32_t*)nullptr, (const uint32_t*)W.pathd.p);
         YG_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_vx_flag, dim3(grid_for(nv, 256)), dim3(256), 0, st, nv, W.vx_draws.p, W.vx_flag.p);
+        YG_HIP(hipGetLastError());
+        if ((rc = exclusive_scan(W.vx_flag.p, W.vx_pos.p, nv, st))) return rc;
+        hipLaunchKernelGGL(k_vx_list, dim3(grid_for(nv, 256)), dim3(256), 0, st, nv, W.vx_flag.p, W.vx_pos.p,
+                           W.vx_lit.p, W.vx_nlit.p);
+        YG_HIP(hipGetLastError());
         const VertexRec vx{W.vx_p.p, W.vx_ray.p, W.vx_ti.p, W.vx_vis.p, W.vx_rs.p, W.vx_draws.p};
         YG_LAUNCH_WS(W.width, W.nocast, (k_vertex_light<4>), (k_vertex_light<2>), (k_vertex_light<12>),
                      (k_vertex_light<10>), dim3(grid_for(nv, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p,
-                     na, (uint32_t)nv, W.S, W.D, vx, W.vx_off.p, W.lcg_off.p, W.pre.p, W.contri.p, W.cnt.p, 0,
-                     W.stk_ovf.p);
+                     na, W.vx_lit.p, W.vx_nlit.p, W.S, W.D, vx, W.vx_off.p, W.lcg_off.p, W.pre.p, W.contri.p,
+                     W.cnt.p, 0, W.stk_ovf.p);
         YG_HIP(hipGetLastError());
         if (W.any_photon) {
             hipLaunchKernelGGL(k_photon_direct, dim3(grid_for(nitems, 64)), dim3(64), 0, st, W.scenes.p,
