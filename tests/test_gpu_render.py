@@ -386,3 +386,32 @@ def test_area_light_inside_gi_paths_refuses_antialiasing(torch_cuda):
                     list(d.lights) + [D.pathlight(samples=2, depth=2, use_qmc=True, threshold=1e3)], o)
     with pytest.raises(RuntimeError, match="antialiasing with area lights inside GI paths"):
         gpu_render(d, torch_cuda)
+
+
+def test_batched_area_light_inside_gi_paths(torch_cuda):
+    """A batch mixing rooms whose GI vertices draw from the area light (both
+    light orders, two resolutions) with a terrain without area lights: each
+    image equals its own yg_render bit for bit, and the draw counts add up."""
+    torch = torch_cuda
+    Y = _Y()
+    descs = []
+    for k, (rx, ry, path_first) in enumerate([(48, 36, False), (40, 30, True)]):
+        d = S.direct_room(rx, ry, subdiv=2, samples=3 + k)
+        path = D.pathlight(samples=3, depth=2, use_qmc=True, threshold=1e3)
+        lights = [path] + list(d.lights) if path_first else list(d.lights) + [path]
+        descs.append(D.SceneDesc(d.verts, d.idx, d.tri_obj, d.tri_mat, d.obj_flags, d.cam, list(d.mats), [], [],
+                                 lights, d.opts))
+    descs.insert(1, S.terrain_gi(po.heightfield, nx=40, ny=20, resx=32, resy=24, samples=3, depth=2))
+    scenes, outs, singles, draws = [], [], [], 0
+    for d in descs:
+        sc = Y.Scene.from_desc(d)
+        out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+        draws += sc.render(out).lcg_draws
+        singles.append(out.clone())
+        outs.append(torch.full_like(out, -1.0))
+        scenes.append(sc)
+    st = Y.render_batch(scenes, outs)
+    torch.cuda.synchronize()
+    for a, b in zip(outs, singles):
+        assert torch.equal(a, b)
+    assert st.lcg_draws == draws
