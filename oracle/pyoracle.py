@@ -64,6 +64,7 @@ def lib():
         L.yo_turbulence.argtypes = [C.c_float] * 3 + [C.c_int, C.c_float, C.c_int]
         L.yo_noise_eval.restype = C.c_int
         L.yo_noise_eval.argtypes = [C.c_int, vp, F32P, C.c_int64, F32P]
+        L.yo_diffuse_at.argtypes = [vp, vp, vp, C.c_int, F32P, C.c_int64, F32P]
         L.yo_gen_heightfield.argtypes = [C.c_int, C.c_int] + [C.c_float] * 7 + [F32P]
         L.yo_tri_hit.restype = C.c_int
         L.yo_tri_hit.argtypes = [F32P] * 6
@@ -316,6 +317,16 @@ def noise_eval(what, tex, pts):
     if rc != 0:
         raise RuntimeError("yo_noise_eval failed")
     return out
+
+
+def diffuse_at(desc, mat, pts):
+    """Albedo of material `mat` of a SceneDesc at points [n, 3] after its
+    modulators: [n, 3] float32."""
+    pts = np.ascontiguousarray(pts, np.float32).reshape(-1)
+    n = len(pts) // 3
+    out = np.empty(n * 3, np.float32)
+    lib().yo_diffuse_at(_addr(desc.mats), _addr(desc.mods), _addr(desc.texs), mat, pts, n, out)
+    return out.reshape(n, 3)
 
 
 def heightfield(nx, ny, ex, ey, amp, freq, H, lac, octv):

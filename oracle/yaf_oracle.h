@@ -90,6 +90,10 @@ float yo_turbulence(float x, float y, float z, int oct, float size, int hard);
 int yo_noise_eval(int what, const yg_texture* tex, const float* pts, int64_t n, float* out);
 void yo_gen_heightfield(int nx, int ny, float ex, float ey, float amp, float freq, float H,
                         float lac, float oct, float* xyz);
+/* genericShader_t::getDiffuse (basicshaders.cc:191-204) of material `mat` at
+ * n points: its colour passed through its modulators in order. out: 3 per point */
+void yo_diffuse_at(const yg_material* mats, const yg_modulator* mods, const yg_texture* texs, int mat,
+                   const float* pts, int64_t n, float* out);
 
 /* ------------------------------------------------------------ rendering -- */
 /* scene_t::render(colorOutput_t&) mono mode, AA_passes 0 (scene.cc:538-600,

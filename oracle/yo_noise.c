@@ -407,6 +407,17 @@ yo_col yo_modulate(const yg_modulator* m, const yg_texture* texs, yo_col C, yo_v
     return C;
 }
 
+void yo_diffuse_at(const yg_material* mats, const yg_modulator* mods, const yg_texture* texs, int mat,
+                   const float* pts, int64_t n, float* out)
+{
+    const yg_material* m = &mats[mat];
+    for (int64_t i = 0; i < n; i++) {
+        yo_col c = col(m->color[0], m->color[1], m->color[2]);
+        for (int k = 0; k < m->mod_count; k++) c = yo_modulate(&mods[m->mod_first + k], texs, c, v3_ld(pts + 3 * i));
+        out[3 * i] = c.r; out[3 * i + 1] = c.g; out[3 * i + 2] = c.b;
+    }
+}
+
 int yo_noise_eval(int what, const yg_texture* tex, const float* pts, int64_t n, float* out)
 {
     for (int64_t i = 0; i < n; i++) {
