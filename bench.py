@@ -417,6 +417,11 @@ def bench_room_gi(args):
         sc = Y.Scene.from_desc(d)
         dt, st = _timed(lambda: sc.render(out), args.steps, args.warmup)
         res[in_paths] = (dt, st.rays_camera + st.rays_extension + st.rays_shadow, st.lcg_draws)
+        if in_paths:
+            finite = bool(torch.isfinite(out).all().item())
+            if not finite:
+                raise RuntimeError("room_gi: non-finite pixels in the GPU image")
+            parity = None if args.no_cpu else _room_gi_parity(d)
     dt, rays, draws = res[1]
     return {
         "metric": "GI Mrays/s", "value": round(rays / dt / 1e6, 3), "unit": "Mrays/s",
@@ -430,8 +435,40 @@ def bench_room_gi(args):
         "area_light_outside_paths": {"ms_per_step": round(res[0][0] * 1e3, 3),
                                      "rays_per_frame": int(res[0][1]),
                                      "mrays_per_s": round(res[0][1] / res[0][0] / 1e6, 3)},
+        "image_finite": finite,
+        "rmse_vs_oracle": parity,
         "error_vs_fp64": None,
     }
+
+
+def _room_gi_parity(d, res=(160, 120)):
+    """The room_gi frame at reduced resolution (same camera, scene and
+    lights) on the GPU and on the oracle: image relative RMSE, and the
+    ourRandom() draws and ray counts of both, which must be equal."""
+    import numpy as np
+    import torch
+
+    from oracle import pyoracle as po
+    from yafray_amd import descriptors as D
+    from yafray_amd import lib as Y
+
+    cam = D.yg_camera.from_buffer_copy(d.cam)
+    cam.resx, cam.resy = res
+    ds = D.SceneDesc(d.verts, d.idx, d.tri_obj, d.tri_mat, d.obj_flags, cam, list(d.mats), [], [],
+                     list(d.lights), d.opts)
+    sc = Y.Scene.from_desc(ds)
+    out = torch.zeros(res[0] * res[1] * 3, dtype=torch.float32, device="cuda")
+    st = sc.render(out)
+    torch.cuda.synchronize()
+    img = out.cpu().numpy().reshape(res[1], res[0], 3).astype(np.float64)
+    ref, rs = po.OracleScene(ds.verts, ds.idx, ds.tri_obj, ds.tri_mat).render(ds, threads=cpu_cores())
+    ref = ref.astype(np.float64)
+    rel = float(np.sqrt(np.mean((img - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)))
+    same = (int(st.lcg_draws) == int(rs[4]) and
+            (st.rays_camera, st.rays_extension, st.rays_shadow) == tuple(int(x) for x in rs[:3]))
+    if not (rel <= 1e-3 and same):
+        raise RuntimeError("room_gi parity failed: rel rmse %.3g, draws %d vs %d" % (rel, st.lcg_draws, rs[4]))
+    return {"rel_rmse": rel, "lcg_draws_equal": same, "sample": "same scene at %dx%d, GPU vs oracle" % res}
 
 
 def bench_autotune(args):

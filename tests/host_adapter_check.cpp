@@ -92,5 +92,11 @@ int main()
     paramMap_t cache;
     cache.setBool("cache", true);
     printf("refused %d\n", I.addLight("cached", "pathlight", cache));
+    // warnings of a finished render: adaptive repetitions counted but not run
+    yg_render_stats st{};
+    printf("warn0 %d\n", (int)yafgpu::gpuInterface_t::renderWarnings(st).size());
+    st.adaptive_extra = 17;
+    const std::string w = yafgpu::gpuInterface_t::renderWarnings(st);
+    printf("warn1 %d %d\n", (int)(w.find("[Warning]: 17 pixels") == 0), (int)(w.find("threshold") != std::string::npos));
     return 0;
 }

@@ -376,6 +376,30 @@ def test_area_light_inside_gi_paths(torch_cuda, path_first):
     assert st.lcg_draws == rs[4]
 
 
+def test_two_area_lights_around_the_pathlight(torch_cuda):
+    """Area lights on both sides of the pathlight in light order (area,
+    pathlight, area, sun): the split of each primary hit's draws around the
+    pathlight's path draws (k_prep's `pre`, k_direct's jump) and the vertex
+    pass's order.  GPU = oracle in image, rays and draws."""
+    from test_oracle_fp64 import room_two_area_lights
+    d = room_two_area_lights(res=(64, 48))
+    img, st, _ = gpu_render(d, torch_cuda)
+    ref, rs = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat).render(d)
+    assert np.all(np.isfinite(img))
+    assert rel_rmse(img, ref) <= 1e-3
+    assert (st.rays_camera, st.rays_extension, st.rays_shadow) == tuple(int(x) for x in rs[:3])
+    assert st.lcg_draws == rs[4]
+
+
+def test_area_light_draws_beyond_32_bits_are_refused(torch_cuda):
+    """The primary hits' ourRandom() stream offsets are 32-bit scans: a scene
+    whose draws in one pass reach 2^32 (4096 pixels x 2 x 2^20 samples) is
+    refused before any launch instead of drawing from wrong stream positions."""
+    d = S.direct_room(64, 64, subdiv=1, samples=1 << 20)
+    with pytest.raises(RuntimeError, match="too many area-light draws"):
+        gpu_render(d, torch_cuda)
+
+
 def test_area_light_inside_gi_paths_refuses_antialiasing(torch_cuda):
     """Antialiasing passes continue each area's ourRandom() stream pass by
     pass; with area lights at path vertices the launch is refused, not

@@ -83,3 +83,13 @@ def test_noise_type_names(adapter_output):
     assert got["blender"] == (0, 2) and got["stdperlin"] == (0, 1)
     assert got["newperlin"] == got["-"] == got["anything"] == (0, 0)
     assert got["voronoi_f1"][0] == D_ERR_UNSUPPORTED and got["cellnoise"][0] == D_ERR_UNSUPPORTED
+
+
+def test_adaptive_repetitions_are_reported(adapter_output):
+    """A render whose pathlight asked for adaptive extra repetitions
+    (pathlight.cc:349-374), which the GPU path counts but does not run, is
+    reported by the command-line renderer; a render without them is not."""
+    w0 = [l for l in adapter_output if l[0] == "warn0"][0]
+    w1 = [l for l in adapter_output if l[0] == "warn1"][0]
+    assert w0 == ["warn0", "0"]
+    assert w1 == ["warn1", "1", "1"]

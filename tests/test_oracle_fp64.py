@@ -66,6 +66,36 @@ def test_reference_streams_match_fp32_area_light_inside_gi_paths(path_first):
     r = np.abs(f32 - f64).max(axis=2) / np.maximum(np.abs(f64).max(axis=2), 1e-6)
     assert np.mean(r < 1e-4) > 0.999
 
+def room_two_area_lights(res=(48, 36), samples=(4, 3)):
+    """The room with a second area light (facing -z, off centre) and a sun,
+    in the light order area, pathlight, area, sun: draws at every primary hit
+    and every path vertex fall before and after the pathlight's own draws."""
+    d = S.direct_room(res[0], res[1], subdiv=2, samples=samples[0])
+    h = 5.8
+    a2 = D.arealight((1.0, 0.6, h), (1.0, 1.4, h), (1.6, 1.4, h), (1.6, 0.6, h), samples=samples[1],
+                     power=6.0, color=(1.0, 0.8, 0.7))
+    sun = D.sunlight((0.3, -0.4, 1.0), power=0.5)
+    path = D.pathlight(samples=3, depth=2, use_qmc=True, threshold=1e3)
+    lights = [d.lights[0], path, a2, sun]
+    return D.SceneDesc(d.verts, d.idx, d.tri_obj, d.tri_mat, d.obj_flags, d.cam, list(d.mats), [], [], lights,
+                       d.opts)
+
+
+def test_reference_streams_two_area_lights_around_the_pathlight():
+    """Area lights before and after the pathlight in light order: the FP64
+    render's sequential walk places every draw; the FP32 restatement's
+    prefix-sum placement must agree."""
+    d = room_two_area_lights()
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    f32, rs = oc.render(d)
+    f64 = oc.render_fp64(d, threads=4)
+    assert rs[4] > 0
+    f32 = f32.astype(np.float64)
+    assert rel_rmse(f32, f64) < 1e-6
+    r = np.abs(f32 - f64).max(axis=2) / np.maximum(np.abs(f64).max(axis=2), 1e-6)
+    assert np.mean(r < 1e-4) > 0.999
+
+
 def _plane(lights, res=(48, 40), bg=(0.4, 0.5, 0.7)):
     E = 50.0
     v = np.array([[-E, -E, 0], [E, -E, 0], [E, E, 0], [-E, E, 0]], np.float32)

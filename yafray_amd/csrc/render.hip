@@ -1787,6 +1787,18 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         }
         if ((double)nitems * (double)prim + (double)npaths * (double)D * (double)vert >= 4294967295.0)
             return yg_fail(YG_ERR_UNSUPPORTED, "too many area-light draws in one launch with GI paths");
+    } else {
+        // the primary hits' ourRandom() offsets are 32-bit scans too: a scene's
+        // draws in one pass (all its items, or an AA pass's samples) must stay
+        // below 2^32 (differences between the scenes of a batch survive a wrap)
+        for (auto& P : plans) {
+            uint64_t pp = 0;
+            for (const auto& L : P.s->lights)
+                if (L.type == YG_LIGHT_AREA && L.use_in_render) pp += 2ull * (uint64_t)L.samples;
+            const double per_pass = (double)P.nitems * (double)std::max(1, P.aa_passes > 0 ? P.R.aa_mins : 1);
+            if (per_pass * (double)pp >= 4294967295.0)
+                return yg_fail(YG_ERR_UNSUPPORTED, "too many area-light draws in one scene's pass (2^32 or more)");
+        }
     }
     const std::vector<Area> areas0(areas);  // the first pass's areas
     append_area_index(areas, nitems);

@@ -522,6 +522,15 @@ int gpuInterface_t::render(const paramMap_t& params, std::vector<float>& rgb, yg
     return YG_OK;
 }
 
+std::string gpuInterface_t::renderWarnings(const yg_render_stats& st)
+{
+    if (st.adaptive_extra == 0) return std::string();
+    return "[Warning]: " + std::to_string(st.adaptive_extra) +
+           " pixels asked the pathlight for adaptive extra repetitions (its \"threshold\" is too low for "
+           "them to stay off); the GPU path renders the first round only, so these pixels differ from the "
+           "reference's. Set the pathlight threshold to 1000 or more for the reference's image.";
+}
+
 }  // namespace yafgpu
 
 // C entry for callers without the C++ types: the factory defaults, so a

@@ -81,6 +81,12 @@ public:
     // (interfaceImpl_t::render's defaults and ranges, interface_impl.cc:480-533)
     static yg_render_opts render_opts(const paramMap_t& params);
 
+    // What a finished render did differently from the reference, as warning
+    // text ("" when nothing): pixels whose pathlight asked for adaptive extra
+    // repetitions (pathlight.cc:349-374; the GPU path counts them in
+    // stats.adaptive_extra but renders the first round only).
+    static std::string renderWarnings(const yg_render_stats& st);
+
     // resolution of a defined camera; false when there is none of that name
     bool cameraResolution(const std::string& name, int& resx, int& resy) const;
 

@@ -45,6 +45,8 @@ int main(int argc, char** argv)
         if (I.render(p, rgb, &st)) return 1;
         std::cout << "Render: " << st.rays_camera << " camera, " << st.rays_extension << " extension, "
                   << st.rays_shadow << " shadow rays\n";
+        const std::string warn = yafgpu::gpuInterface_t::renderWarnings(st);
+        if (!warn.empty()) std::cerr << warn << "\n";
         int rc;
         if (type == "hdr" || type.find("exr") != std::string::npos) {
             if (type != "hdr") std::cout << "Built without OpenEXR: image saved in hdr format instead.\n";

@@ -214,6 +214,13 @@ void add_label(paramMap_t& p, const xmlNode_t& l)
     else if (l.attrs.size() == 1 && l.attrs[0].first == "value") set_value(p, l.name, l.attrs[0].second);
 }
 
+static bool has_descendant(const xmlNode_t& n, const char* name)
+{
+    for (const auto& k : n.kids)
+        if (k.name == name || (k.name != "mesh" && has_descendant(k, name))) return true;
+    return false;
+}
+
 paramMap_t generic_params(const xmlNode_t& n)
 {
     paramMap_t p;
@@ -268,6 +275,31 @@ struct Loader {
             if (k.name == "mesh") mesh = &k;
             else if (k.name == "sphere") return hostFail(YG_ERR_UNSUPPORTED, "sphere objects are not on the GPU path");
         if (!mesh) return hostFail(YG_ERR_UNSUPPORTED, "object " + *name + ": only meshes are on the GPU path");
+        // Settings that change what the reference renders for this object and
+        // that the GPU path does not implement are refused, not ignored:
+        // caustic settings turn on obj->caustics (msin_object.cc:448-476),
+        // which makes the pathlight follow caustic bounces (pathlight.cc:194-222);
+        // face uv and has_orco give the surface uv tangents
+        // (meshObject_t::tangentsFromUV, mesh.cc:284-340, render.cc:765), which
+        // the pathlight samples its directions around.
+        if (attr(n, "caus_IOR") || has_descendant(n, "caus_rcolor") || has_descendant(n, "caus_tcolor"))
+            return hostFail(YG_ERR_UNSUPPORTED, "object " + *name + ": caustic settings (caus_IOR, caus_rcolor, "
+                                                "caus_tcolor) are not on the GPU path");
+        {
+            const std::string* orco = attr(*mesh, "has_orco");
+            if (orco && *orco == "on")
+                return hostFail(YG_ERR_UNSUPPORTED, "object " + *name + ": has_orco is not on the GPU path");
+        }
+        float smooth_angle = 0;
+        if (const std::string* as = attr(*mesh, "autosmooth")) {
+            // render.cc:764 with msin_object.cc:271-280: a float angle; below 1
+            // degree meshObject_t::autoSmooth does nothing (mesh.cc:204)
+            if (!as_float(*as, smooth_angle))
+                std::cerr << "[Warning]: Only float value accepted for autosmooth\n";
+            else if (smooth_angle >= 1)
+                return hostFail(YG_ERR_UNSUPPORTED, "object " + *name + ": autosmooth >= 1 degree "
+                                                    "(interpolated vertex normals) is not on the GPU path");
+        }
         bool shadow = true, emit = true, recv = true;
         paramMap_t op = generic_params(n);
         op.getParam("shadow", shadow);
@@ -297,6 +329,11 @@ struct Loader {
                     f.push_back((uint32_t)a);
                     f.push_back((uint32_t)b);
                     f.push_back((uint32_t)c);
+                    for (const auto& a : fc.attrs)
+                        if (a.first.compare(0, 2, "u_") == 0 || a.first.compare(0, 2, "v_") == 0 ||
+                            a.first.compare(0, 5, "vcol_") == 0)
+                            return hostFail(YG_ERR_UNSUPPORTED, "object " + *name + ": face " + a.first +
+                                                                " (uv or vertex colours) is not on the GPU path");
                     const std::string* fs = attr(fc, "shader_name");
                     sh.push_back(fs ? *fs : (shader ? *shader : std::string()));
                 }
