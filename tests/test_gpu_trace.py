@@ -131,6 +131,16 @@ def test_terrain_parity_full_size(torch_cuda, width):
     _check(pg[sub], tg[sub], ug[sub], vg[sub], pc, tc, uc, vc)
     # a good part of the downward rays leave the [0,2]x[0,1] terrain before landing
     assert np.mean(pg >= 0) > 0.3
+    # any hit with a finite length: this tree's traversal stack (max depth
+    # well past the 16 entries kept in LDS) spills into the overflow buffer
+    assert info["max_depth"] > 16
+    rays[:, 7] = 0.8
+    occ = torch.empty(n, dtype=torch.uint8, device="cuda")
+    sc.trace_any(torch.from_numpy(rays).cuda(), occ)
+    ag = occ.cpu().numpy()
+    ac, _, _, _ = oc.trace(rays[sub], mode=1, threads=8)
+    assert np.mean(ag[sub] == (ac >= 0)) >= 0.9999
+    assert 0.05 < np.mean(ag) < 0.95
 
 
 @pytest.mark.parametrize("width", [2, 4])

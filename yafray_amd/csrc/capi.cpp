@@ -50,13 +50,14 @@ static void free_dev(yg_scene* s)
 {
     void* ptrs[] = {s->d_nodes, s->d_nodes4, s->d_tris, s->d_tprim, s->d_tobj, s->d_tmat, s->d_obj_flags,
                     s->d_mats, s->d_mods, s->d_texs, s->pm.d_photons, s->pm.d_gstart,
-                    s->pm.d_gitems, s->d_halton};
+                    s->pm.d_gitems, s->d_halton, s->d_any_ovf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     s->d_nodes = nullptr; s->d_nodes4 = nullptr; s->d_tris = nullptr; s->d_tprim = nullptr; s->d_tobj = nullptr;
     s->d_tmat = nullptr; s->d_obj_flags = nullptr; s->d_mats = nullptr; s->d_mods = nullptr;
     s->d_texs = nullptr; s->pm = PhotonMapDev();
     s->d_halton = nullptr; s->halton_ndim = 0; s->halton_stride = 0;
+    s->d_any_ovf = nullptr; s->any_ovf_ints = 0;
 }
 
 extern "C" int yg_scene_destroy(yg_scene* s)

@@ -1,6 +1,9 @@
 // Ray-query entry points of libyafgpu: closest / any hit over ray batches,
 // scene_t::firstHit / isShadowed semantics, noise evaluation and heightfield
 // generation on the device.
+#include <algorithm>
+#include <cstdint>
+
 #include "noise_dev.h"
 #include "traverse.h"
 
@@ -41,19 +44,66 @@ __global__ __launch_bounds__(TB) void k_trace_closest(BVHView bvh, const yg_ray*
     }
 }
 
+// Any-hit batches: each lane keeps the first `lcap` entries of its traversal
+// stack in LDS and deeper ones (only rays under deep subtrees reach them) in
+// its own column of an overflow buffer, ovf[(entry - lcap) * nthreads +
+// thread].  The short LDS part lets more waves share a compute unit than a
+// whole stack per lane would (terrain, 4M rays: 4.7 vs 4.0 Grays/s).  One ray
+// per thread while the overflow buffer stays within its cap; beyond that the
+// grid shrinks and each thread loops over several rays.
+#ifndef YG_ANY_LCAP
+#define YG_ANY_LCAP 16
+#endif
+#define YG_ANY_OVF_MAX ((size_t)1 << 28)  // overflow ints of one launch (1 GiB)
+
 template <int W>
 __global__ __launch_bounds__(TB) void k_trace_any(BVHView bvh, const yg_ray* __restrict__ rays,
-                                                  uint32_t n, uint8_t* __restrict__ occ)
+                                                  uint32_t n, uint8_t* __restrict__ occ, int lcap,
+                                                  int* __restrict__ ovf)
 {
     extern __shared__ int s_stk[];
-    const uint32_t i = blockIdx.x * TB + threadIdx.x;
-    if (i >= n) return;
+    const uint32_t t = blockIdx.x * TB + threadIdx.x, nt = gridDim.x * TB;
     LaneStack st;
     st.init(s_stk, threadIdx.x, TB);
-    const yg_ray r = rays[i];
-    const f3 o = mk3(r.ox, r.oy, r.oz), d = mk3(r.dx, r.dy, r.dz);
-    const float blo = r.tmin - 1e-4f * fabsf(r.tmin) - 1e-30f;
-    occ[i] = trace_any<false, W>(bvh, o, d, r.tmin, r.tmax, -1, o, blo, r.tmax, st) ? 1 : 0;
+    if (ovf) st.spill_to(lcap, ovf + t, nt);
+    for (uint32_t i = t; i < n; i += nt) {
+        const yg_ray r = rays[i];
+        const f3 o = mk3(r.ox, r.oy, r.oz), d = mk3(r.dx, r.dy, r.dz);
+        const float blo = r.tmin - 1e-4f * fabsf(r.tmin) - 1e-30f;
+        occ[i] = trace_any<false, W>(bvh, o, d, r.tmin, r.tmax, -1, o, blo, r.tmax, st) ? 1 : 0;
+    }
+}
+
+// Launch shape of an any-hit batch: grid, LDS stack entries per lane, and the
+// overflow entries per lane (0: the whole stack fits in LDS).  max_ovf_ints
+// caps the overflow buffer: the grid shrinks until it fits.
+struct AnyShape {
+    unsigned grid = 0;
+    int lcap = 0;
+    int ovf_entries = 0;
+    size_t ovf_ints() const { return (size_t)ovf_entries * grid * TB; }
+};
+
+static AnyShape any_shape(const yg_scene* s, uint32_t n, size_t max_ovf_ints)
+{
+    AnyShape a;
+    a.grid = grid_for(n, TB);
+    a.lcap = std::min(s->stack_entries, YG_ANY_LCAP);
+    a.ovf_entries = s->stack_entries - a.lcap;
+    if (a.ovf_entries > 0) {
+        const size_t fit = max_ovf_ints / ((size_t)a.ovf_entries * TB);
+        if (fit < a.grid) a.grid = (unsigned)fit;
+    }
+    return a;
+}
+
+static int launch_trace_any(yg_scene* s, const yg_ray* d_rays, uint32_t n, uint8_t* d_occ, const AnyShape& a,
+                            int* ovf, hipStream_t st)
+{
+    YG_LAUNCH_W(s->width, (k_trace_any<4>), (k_trace_any<2>), dim3(a.grid), dim3(TB), stack_lds_bytes(a.lcap, TB),
+                st, s->view(), d_rays, n, d_occ, a.lcap, a.ovf_entries > 0 ? ovf : nullptr);
+    YG_HIP(hipGetLastError());
+    return YG_OK;
 }
 
 template <int W>
@@ -176,10 +226,17 @@ extern "C" int yg_trace_any(yg_scene* s, const yg_ray* d_rays, uint32_t n, uint8
     if (rc) return rc;
     if (n == 0) return YG_OK;
     if (!d_rays || !d_occluded) return yg_fail(YG_ERR_ARG, "null buffer");
-    YG_LAUNCH_W(s->width, (k_trace_any<4>), (k_trace_any<2>), dim3(grid_for(n, TB)), dim3(TB),
-                stack_lds_bytes(s->stack_entries, TB), (hipStream_t)stream, s->view(), d_rays, n, d_occluded);
-    YG_HIP(hipGetLastError());
-    return YG_OK;
+    const AnyShape a = any_shape(s, n, YG_ANY_OVF_MAX);
+    if (a.ovf_ints() > s->any_ovf_ints) {
+        // grow-only per scene; hipFree waits for work still using the old buffer
+        if (s->d_any_ovf) (void)hipFree(s->d_any_ovf);
+        s->d_any_ovf = nullptr;
+        s->any_ovf_ints = 0;
+        hipError_t e = hipMalloc((void**)&s->d_any_ovf, sizeof(int) * a.ovf_ints());
+        if (e != hipSuccess) return yg_fail(YG_ERR_NOMEM, std::string("any-hit stack buffer: ") + hipGetErrorString(e));
+        s->any_ovf_ints = a.ovf_ints();
+    }
+    return launch_trace_any(s, d_rays, n, d_occluded, a, s->d_any_ovf, (hipStream_t)stream);
 }
 
 extern "C" int yg_first_hit(yg_scene* s, const float* d_from, const float* d_dir, uint32_t n,

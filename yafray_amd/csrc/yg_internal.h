@@ -134,6 +134,8 @@ struct yg_scene {
     yg_texture* d_texs = nullptr;
 
     PhotonMapDev pm;
+    int* d_any_ovf = nullptr;  // yg_trace_any: traversal stack entries beyond the LDS part
+    size_t any_ovf_ints = 0;
     uint64_t gen = 0;  // bumped by every change of geometry, shading, lights or camera
     uint64_t geo_gen = 0;          // bumped by geometry changes only
     uint64_t checked_geo = ~0ull;  // geo_gen / material count at which the
