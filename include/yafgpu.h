@@ -263,6 +263,22 @@ int yg_trace_closest(yg_scene* s, const yg_ray* d_rays, uint32_t n, yg_hit* d_hi
 int yg_trace_any(yg_scene* s, const yg_ray* d_rays, uint32_t n, uint8_t* d_occluded,
                  void* stream);
 
+/* yg_trace_any without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid it).  No counterpart in the reference.  Same results as
+ * yg_trace_any; the traversal stack entries beyond the LDS part go to a
+ * 64 MiB scratch array that the library holds on each GPU until the process
+ * ends, instead of a buffer it allocates.  The launch shrinks its grid (each
+ * thread then traces several rays) until the scratch holds every thread's
+ * entries, so no ray count is refused; a tree whose stack does not fit the
+ * scratch even for one block of 64 lanes returns YG_ERR_UNSUPPORTED (deeper
+ * than any tree yg_build accepts).  The scratch keeps the last call's
+ * contents and every call on one GPU shares it: calls that overlap on one
+ * GPU (other streams or threads) race and must be serialised by the caller.
+ * The first call on a device looks up the scratch's address there. */
+int yg_trace_any_noalloc(yg_scene* s, const yg_ray* d_rays, uint32_t n, uint8_t* d_occluded,
+                         void* stream);
+
 /* scene_t::firstHit (scene.cc:688-721): origin from + ray*min_raydis, closest
  * with Z > 0.  d_from, d_dir: n float3.  Writes prim/t/u/v per query. */
 int yg_first_hit(yg_scene* s, const float* d_from, const float* d_dir, uint32_t n,

@@ -3,6 +3,8 @@
 // generation on the device.
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
+#include <string>
 
 #include "noise_dev.h"
 #include "traverse.h"
@@ -237,6 +239,54 @@ extern "C" int yg_trace_any(yg_scene* s, const yg_ray* d_rays, uint32_t n, uint8
         s->any_ovf_ints = a.ovf_ints();
     }
     return launch_trace_any(s, d_rays, n, d_occluded, a, s->d_any_ovf, (hipStream_t)stream);
+}
+
+// ------------------------------------------- allocation-free any-hit --
+// yg_trace_any_noalloc keeps the stack overflow in this file-scope array
+// instead of a buffer the library allocates.  Each GPU holds its own copy
+// until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (the array's name is a
+// host-side stand-in, not a device address).
+#define YG_SCRATCH_INTS ((size_t)16 << 20)  // 64 MiB
+#define YG_SCRATCH_DEVICES 64
+static __device__ __align__(256) int scratch[YG_SCRATCH_INTS];
+static int* g_scratch_dev[YG_SCRATCH_DEVICES];
+static std::mutex g_scratch_mu;
+
+// scratch's device address on the current device, looked up once per device
+static int scratch_on_device(int** out)
+{
+    int dev = 0;
+    YG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= YG_SCRATCH_DEVICES) return yg_fail(YG_ERR_UNSUPPORTED, "device index beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return yg_fail(YG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = (int*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return YG_OK;
+}
+
+extern "C" int yg_trace_any_noalloc(yg_scene* s, const yg_ray* d_rays, uint32_t n, uint8_t* d_occluded,
+                                    void* stream)
+{
+    int rc = check_scene(s);
+    if (rc) return rc;
+    if (n == 0) return YG_OK;
+    if (!d_rays || !d_occluded) return yg_fail(YG_ERR_ARG, "null buffer");
+    int* scr = nullptr;
+    if ((rc = scratch_on_device(&scr))) return rc;
+    // the grid shrinks until every thread's overflow column fits the scratch
+    const AnyShape a = any_shape(s, n, YG_SCRATCH_INTS);
+    if (a.grid == 0 || a.ovf_ints() > YG_SCRATCH_INTS)
+        return yg_fail(YG_ERR_UNSUPPORTED, "traversal stack of " + std::to_string(s->stack_entries) +
+                                               " entries per lane does not fit the allocation-free scratch (" +
+                                               std::to_string(YG_SCRATCH_INTS) + " ints)");
+    return launch_trace_any(s, d_rays, n, d_occluded, a, scr, (hipStream_t)stream);
 }
 
 extern "C" int yg_first_hit(yg_scene* s, const float* d_from, const float* d_dir, uint32_t n,

@@ -45,6 +45,7 @@ def lib():
         "yg_trace_closest": (C.c_int, [vp, vp, u32, vp, vp]),
         "yg_trace_closest_counted": (C.c_int, [vp, vp, u32, vp, vp, vp]),
         "yg_trace_any": (C.c_int, [vp, vp, u32, vp, vp]),
+        "yg_trace_any_noalloc": (C.c_int, [vp, vp, u32, vp, vp]),
         "yg_first_hit": (C.c_int, [vp, vp, vp, u32, f32, vp, vp]),
         "yg_is_shadowed": (C.c_int, [vp, vp, vp, vp, u32, i32, f32, f32, vp, vp]),
         "yg_noise_eval": (C.c_int, [i32, vp, vp, u32, vp, vp]),
@@ -185,6 +186,10 @@ class Scene:
     def trace_any(self, rays, occ):
         check(lib().yg_trace_any(self.h, _ptr(rays), rays.shape[0], _ptr(occ), _stream()),
               "yg_trace_any")
+
+    def trace_any_noalloc(self, rays, occ):
+        check(lib().yg_trace_any_noalloc(self.h, _ptr(rays), rays.shape[0], _ptr(occ), _stream()),
+              "yg_trace_any_noalloc")
 
     def first_hit(self, frm, d, hits, min_raydis=D.MIN_RAYDIST):
         check(lib().yg_first_hit(self.h, _ptr(frm), _ptr(d), frm.shape[0], min_raydis, _ptr(hits),
