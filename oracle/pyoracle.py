@@ -10,6 +10,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
+_LIB_FAST = None
 
 F32P = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -18,12 +19,24 @@ U8P = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 U64P = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
 
 
-def lib():
-    global _LIB
+def lib(fast=False):
+    """The oracle library.  fast=True: the same restatement built with the
+    reference's release flags (-O3 -ffast-math), for CPU-baseline timing
+    only; the checker is always the strict build."""
+    global _LIB, _LIB_FAST
+    if fast:
+        if _LIB_FAST is None:
+            _LIB_FAST = _load(os.path.join(_HERE, "liboracle_fast.so"))
+        return _LIB_FAST
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        _LIB = _load(os.path.join(_HERE, "liboracle.so"))
+    return _LIB
+
+
+def _load(path):
+    if True:
         if not os.path.exists(path):
-            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+            raise RuntimeError("%s is not built (run __graft_entry__.build())" % path)
         L = C.CDLL(path)
         vp = C.c_void_p
         L.yo_scene_new.restype = vp
@@ -64,6 +77,9 @@ def lib():
         L.yo_turbulence.argtypes = [C.c_float] * 3 + [C.c_int, C.c_float, C.c_int]
         L.yo_noise_eval.restype = C.c_int
         L.yo_noise_eval.argtypes = [C.c_int, vp, F32P, C.c_int64, F32P]
+        L.yo_canonical_counts.restype = C.c_int64
+        L.yo_canonical_counts.argtypes = [F32P, C.c_int64, U32P, C.c_int64, F32P, C.c_int64, C.c_int, C.c_int,
+                                          U64P, U64P, U64P]
         L.yo_diffuse_at.argtypes = [vp, vp, vp, C.c_int, F32P, C.c_int64, F32P]
         L.yo_gen_heightfield.argtypes = [C.c_int, C.c_int] + [C.c_float] * 7 + [F32P]
         L.yo_tri_hit.restype = C.c_int
@@ -95,8 +111,7 @@ def lib():
         L.yo_photon_read.argtypes = [vp, vp, C.c_int64]
         L.yo_photon_gather.argtypes = [vp, vp, vp, vp, vp, F32P, F32P, F32P, F32P, I32P,
                                        C.c_int64, F32P]
-        _LIB = L
-    return _LIB
+    return L
 
 
 def _ptr(a):
@@ -106,8 +121,9 @@ def _ptr(a):
 class OracleScene:
     """Triangle soup with per-triangle object and material ids."""
 
-    def __init__(self, verts, idx, tri_obj=None, tri_mat=None):
-        L = lib()
+    def __init__(self, verts, idx, tri_obj=None, tri_mat=None, fast=False):
+        self.fast = fast
+        L = lib(fast)
         self.verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
         self.idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1, 3)
         self.tri_obj = None if tri_obj is None else np.ascontiguousarray(tri_obj, dtype=np.int32)
@@ -117,7 +133,7 @@ class OracleScene:
 
     def close(self):
         if self.h:
-            lib().yo_scene_free(self.h)
+            lib(self.fast).yo_scene_free(self.h)
             self.h = None
 
     def __del__(self):
@@ -128,7 +144,7 @@ class OracleScene:
 
     @property
     def n_nodes(self):
-        return lib().yo_scene_nodes(self.h)
+        return lib(self.fast).yo_scene_nodes(self.h)
 
     def trace(self, rays, mode=0, brute=False, threads=1):
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
@@ -138,11 +154,11 @@ class OracleScene:
         u = np.empty(n, np.float32)
         v = np.empty(n, np.float32)
         if brute:
-            lib().yo_trace_brute(self.h, rays.reshape(-1), n, mode, prim, t, u, v)
+            lib(self.fast).yo_trace_brute(self.h, rays.reshape(-1), n, mode, prim, t, u, v)
             return prim, t, u, v
         nodes = np.zeros(1, np.uint64)
         tris = np.zeros(1, np.uint64)
-        lib().yo_trace_bvh(self.h, rays.reshape(-1), n, mode, prim, t, u, v, nodes, tris, threads)
+        lib(self.fast).yo_trace_bvh(self.h, rays.reshape(-1), n, mode, prim, t, u, v, nodes, tris, threads)
         self.last_counts = (int(nodes[0]), int(tris[0]))
         return prim, t, u, v
 
@@ -152,7 +168,7 @@ class OracleScene:
         n = len(frm) // 3
         prim = np.empty(n, np.int32)
         t = np.empty(n, np.float32)
-        lib().yo_first_hit(self.h, frm, d, n, min_raydis, prim, t)
+        lib(self.fast).yo_first_hit(self.h, frm, d, n, min_raydis, prim, t)
         return prim, t
 
     def is_shadowed(self, p, l, self_obj, to_point, min_raydis=5e-5, self_bias=0.1):
@@ -161,12 +177,12 @@ class OracleScene:
         so = None if self_obj is None else np.ascontiguousarray(self_obj, np.int32)
         n = len(p) // 3
         out = np.empty(n, np.uint8)
-        lib().yo_is_shadowed(self.h, p, l, _ptr(so), n, int(to_point), min_raydis, self_bias, out)
+        lib(self.fast).yo_is_shadowed(self.h, p, l, _ptr(so), n, int(to_point), min_raydis, self_bias, out)
         return out
 
     def render(self, desc, threads=1):
         """desc: yafray_amd.descriptors.SceneDesc (its ctypes arrays)."""
-        L = lib()
+        L = lib(self.fast)
         rgb = np.zeros(desc.cam.resx * desc.cam.resy * 3, np.float32)
         stats = np.zeros(5, np.uint64)
         rc = L.yo_render_mt(self.h, C.addressof(desc.cam), _addr(desc.mats), len(desc.mats),
@@ -183,7 +199,7 @@ class OracleScene:
         px = np.ascontiguousarray(px, np.float32)
         py = np.ascontiguousarray(py, np.float32)
         rgb = np.zeros((len(px), 3), np.float32)
-        rc = lib().yo_trace_camera(self.h, C.addressof(desc.cam), _addr(desc.mats), len(desc.mats),
+        rc = lib(self.fast).yo_trace_camera(self.h, C.addressof(desc.cam), _addr(desc.mats), len(desc.mats),
                                    _addr(desc.mods), _addr(desc.texs), _addr(desc.lights), len(desc.lights),
                                    _ptr(desc.obj_flags), len(desc.obj_flags), C.addressof(desc.opts),
                                    px, py, len(px), rgb)
@@ -195,7 +211,7 @@ class OracleScene:
         """render() that also returns every first-hit and shadow query the
         render made: (img, stats, ext[n,8], sh[n,8]) with the record layouts
         of yo_log_begin (int fields are int32 bit patterns in the floats)."""
-        L = lib()
+        L = lib(self.fast)
         ext = np.zeros((max(ext_cap, 1), 8), np.float32)
         sh = np.zeros((max(sh_cap, 1), 8), np.float32)
         L.yo_log_begin(_ptr(ext), ext_cap, _ptr(sh), sh_cap)
@@ -211,7 +227,7 @@ class OracleScene:
 
     def render_adjoint(self, desc, grad_rgb):
         """FP64 reference gradient: returns (image, gmat[nmat,3], gmod[nmod,2], glight[nl])."""
-        L = lib()
+        L = lib(self.fast)
         rgb = np.zeros(desc.cam.resx * desc.cam.resy * 3, np.float32)
         g = np.ascontiguousarray(grad_rgb, np.float32).reshape(-1)
         gmat = np.zeros(len(desc.mats) * 3, np.float64)
@@ -232,7 +248,7 @@ class OracleScene:
         with the reference's sample streams, [resy, resx, 3] float64.  Else
         the pixels (px[q], py[q]) with `samples` samples of every sampled
         light (the converged estimate), [n, 3] float64."""
-        L = lib()
+        L = lib(self.fast)
         args = (self.h, C.addressof(desc.cam), _addr(desc.mats), len(desc.mats), _addr(desc.mods),
                 _addr(desc.texs), _addr(desc.lights), len(desc.lights), _ptr(desc.obj_flags),
                 len(desc.obj_flags), C.addressof(desc.opts))
@@ -252,7 +268,7 @@ class OracleScene:
         return out
 
     def photon_build(self, desc, li):
-        L = lib()
+        L = lib(self.fast)
         stats = np.zeros(5, np.uint64)
         rc = L.yo_photon_build(self.h, _addr(desc.mats), len(desc.mats), _addr(desc.mods),
                                len(desc.mods), _addr(desc.texs), len(desc.texs),
@@ -263,14 +279,14 @@ class OracleScene:
         return stats
 
     def photon_read(self):
-        L = lib()
+        L = lib(self.fast)
         n = L.yo_photon_read(self.h, None, 0)
         out = np.zeros(max(n, 1) * 9, np.float32)
         L.yo_photon_read(self.h, _ptr(out), n)
         return out[: n * 9].reshape(n, 9)
 
     def photon_gather(self, desc, li, p, n, ng, eye, mat):
-        L = lib()
+        L = lib(self.fast)
         cnt = len(mat)
         rgb = np.zeros(cnt * 3, np.float32)
         light = desc.lights[li]
@@ -327,6 +343,21 @@ def diffuse_at(desc, mat, pts):
     out = np.empty(n * 3, np.float32)
     lib().yo_diffuse_at(_addr(desc.mats), _addr(desc.mods), _addr(desc.texs), mat, pts, n, out)
     return out.reshape(n, 3)
+
+
+def canonical_counts(verts, idx, rays, mode, threads=8):
+    """Mean node visits, triangle tests and box tests per ray of the
+    canonical tight BVH2 of the byte model (yo_bytemodel.c):
+    (n_node, n_tri, n_box_tested, tree nodes)."""
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1)
+    f = np.ascontiguousarray(idx, np.uint32).reshape(-1)
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1)
+    n = len(r) // 8
+    vis = np.zeros(1, np.uint64)
+    t = np.zeros(1, np.uint64)
+    b = np.zeros(1, np.uint64)
+    nodes = lib().yo_canonical_counts(v, len(v) // 3, f, len(f) // 3, r, n, mode, threads, vis, t, b)
+    return float(vis[0]) / n, float(t[0]) / n, float(b[0]) / n, int(nodes)
 
 
 def heightfield(nx, ny, ex, ey, amp, freq, H, lac, octv):

@@ -158,4 +158,13 @@ void yo_photon_gather(yo_scene* s, const yg_material* mats, const yg_modulator* 
 #ifdef __cplusplus
 }
 #endif
+/* Canonical BVH2 visit counts of the roofline byte model (yo_bytemodel.c):
+ * builds a tight FP64 binned-SAH BVH2 (32 bins, leaf <= 4) over the
+ * triangles, traces n rays (8 floats: o, tmin, d, tmax) closest (mode 0) or
+ * any hit (mode 1), and returns the node count; *visits / *tris /
+ * *boxes_tested get the summed node visits, triangle tests and box tests. */
+int64_t yo_canonical_counts(const float* verts, int64_t nv, const uint32_t* idx, int64_t nt, const float* rays,
+                            int64_t n, int mode, int threads, uint64_t* visits, uint64_t* tris,
+                            uint64_t* boxes_tested);
+
 #endif
