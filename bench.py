@@ -295,9 +295,18 @@ def bench_adjoint(args):
 
 def bench_photon(args):
     """Photon mapping: 2M photons from a 45-degree spot into 4,096 icospheres
-    (20,971,522 triangles), K = 50 gather at 1920x1080."""
+    (20,971,522 triangles), K = 50 gather at 1920x1080.  One step = photon
+    build (emission, tracing, clustering, gather index) + render.
+
+    Also: the K-nearest gather kernel alone (yg_photon_gather over the
+    frame's primary hit points, HIP events on its stream) against the HBM
+    roofline; a sampled-pixel parity check of the GPU frame against the
+    oracle (same photon map built on the host); and the oracle's photon build
+    plus the sampled pixels as the CPU baseline, scaled to a full frame."""
+    import numpy as np
     import torch
 
+    from yafray_amd import descriptors as D
     from yafray_amd import lib as Y
     from yafray_amd import scenes as S
 
@@ -309,7 +318,44 @@ def bench_photon(args):
     build_dt, _ = _timed(lambda: sc.photon_build(0), max(1, args.steps // 2), 0)
     out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
     rdt, st = _timed(lambda: sc.render(out), args.steps, args.warmup)
-    return {
+    img = out.cpu().numpy().reshape(d.cam.resy, d.cam.resx, 3)
+    finite = bool(np.all(np.isfinite(img)))
+    L = d.lights[0]
+    K = int(L.search)
+    # -- the gather kernel on the frame's primary hit points (pixel centres)
+    rx, ry = d.cam.resx, d.cam.resy
+    o, dirs = _camera_rays(d.cam, np.arange(rx * ry))
+    frm = torch.from_numpy(o.astype(np.float32)).cuda()
+    dd = torch.from_numpy(dirs.astype(np.float32)).cuda()
+    hits = torch.empty((rx * ry, 4), dtype=torch.float32, device="cuda")
+    sc.first_hit(frm, dd, hits)
+    h = hits.cpu().numpy()
+    prim = h[:, 0].view(np.int32)
+    ok = prim >= 0
+    P = (o + dirs * (D.MIN_RAYDIST + h[:, 1:2]))[ok].astype(np.float32)
+    tri = d.verts[d.idx[prim[ok]]].astype(np.float64)
+    ng = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    ng = (ng / np.linalg.norm(ng, axis=1, keepdims=True)).astype(np.float32)
+    eye = (o[ok] - P).astype(np.float32)
+    mat = d.tri_mat[prim[ok]].astype(np.int32)
+    nq = int(ok.sum())
+    tP, tN, tE = (torch.from_numpy(np.ascontiguousarray(x)).cuda() for x in (P, ng, eye))
+    tM = torch.from_numpy(mat).cuda()
+    rgb = torch.empty((nq, 3), dtype=torch.float32, device="cuda")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for _ in range(2):
+        sc.photon_gather(tP, tN, tN, tE, tM, rgb)
+    e0.record()
+    for _ in range(args.steps):
+        sc.photon_gather(tP, tN, tN, tE, tM, rgb)
+    e1.record()
+    torch.cuda.synchronize()
+    g_ms = e0.elapsed_time(e1) / args.steps
+    # algorithmic bytes per query: 48 B query (P, N, Ng, eye) + 12 B estimate
+    # + K x 36 B photons + 27 cells x 8 B (cell ranges of the 3x3x3 block)
+    bq = 48 + 12 + K * 36 + 27 * 8
+    achieved = nq * bq / (g_ms * 1e-3) / 1e9
+    line = {
         "metric": "photon map ms/frame", "value": round((build_dt + rdt) * 1e3, 3), "unit": "ms/frame",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round((build_dt + rdt) * 1e3, 3),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
@@ -318,7 +364,130 @@ def bench_photon(args):
                    "triangles": int(d.n_tris)},
         "photon_build_ms": round(build_dt * 1e3, 3), "render_ms": round(rdt * 1e3, 3),
         "emitted": int(pst.emitted), "stored": int(pst.stored), "clustered": int(pst.clustered),
-        "scene_setup_s": round(setup_s, 1),
+        "scene_setup_s": round(setup_s, 1), "image_finite": finite,
+        "roofline": {"bound": "hbm", "kernel": "k_gather_points", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "queries": nq, "ms_per_launch": round(g_ms, 4),
+                     "alg_bytes_per_query": bq,
+                     "note": "the clustered photon map (%d marks, %d KB) stays in L2: the gather is bound by "
+                             "its LDS heap work, not by HBM" % (int(pst.clustered), int(pst.clustered) * 36 // 1024)},
+    }
+    if not args.no_cpu:
+        line.update(_photon_cpu_legs(d, img, pst, args, sc=sc))
+    return line
+
+
+def _camera_rays(cam, idx):
+    """Camera rays through pixel centres (camera.cc:27-87, scene.cc:736-756)
+    in float64: (origins, unit directions) for flat pixel indices idx."""
+    import numpy as np
+    pos = np.array(cam.pos[:], np.float64)
+    look = np.array(cam.look[:], np.float64)
+    up = np.array(cam.up[:], np.float64)
+    vup, vto = up - pos, look - pos
+    vright = np.cross(vup, vto)
+    vup = np.cross(vright, vto)
+    vup /= np.linalg.norm(vup)
+    vright /= -np.linalg.norm(vright)
+    vto /= np.linalg.norm(vto)
+    vup *= cam.aspect * cam.resy / cam.resx
+    vto = vto * cam.focal - 0.5 * (vup + vright)
+    vup /= cam.resy
+    vright /= cam.resx
+    i, j = idx // cam.resx, idx % cam.resx
+    dv = vright[None] * (j + 0.5)[:, None] + vup[None] * (i + 0.5)[:, None] + vto[None]
+    dv /= np.linalg.norm(dv, axis=1, keepdims=True)
+    return np.repeat(pos[None], len(idx), 0), dv
+
+
+def _photon_cpu_legs(d, img, pst, args, sc=None, n_pix=4096, seed=11):
+    """Oracle legs of the photon workload, on the host.
+
+    * map: the oracle builds the photon map itself (timed: with the sampled
+      pixels below, scaled to the full frame, the CPU baseline); marks are
+      compared cell by cell with the GPU's;
+    * estimate: the oracle renders a fixed-seed sample of pixels with the
+      GPU's map loaded (yo_photon_load), so the K-nearest estimate at full
+      size is compared on equal photons;
+    * camera rays of the sample whose first hit differs are classified by an
+      FP64 edge test of the oracle's triangle.
+
+    The reference draws bounce directions from one Halton sequence in photon
+    order, so a single first hit decided differently (the reference's float
+    edge test is tree-dependent at far silhouettes) shifts the bounce of
+    every later photon: maps built independently differ in a few percent of
+    their cells, and the K-nearest estimate passes that on to the image."""
+    import numpy as np
+    import torch
+
+    from oracle import pyoracle as po
+
+    print("photon: oracle tree over %d triangles" % d.n_tris, file=sys.stderr, flush=True)
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    print("photon: oracle photon build", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    os_ = oc.photon_build(d, 0)
+    cpu_build = time.perf_counter() - t0
+    omap = oc.photon_read()
+    gbuf = torch.zeros(int(pst.clustered) * 9, dtype=torch.float32, device="cuda")
+    sc.photon_read(gbuf)
+    gmap = gbuf.cpu().numpy().reshape(-1, 9)
+    cell = float(d.lights[0].cluster)
+    gk = {tuple(np.floor(r[3:6] / cell).astype(np.int64)): r for r in gmap}
+    ok_ = {tuple(np.floor(r[3:6] / cell).astype(np.int64)): r for r in omap}
+    common = set(gk) & set(ok_)
+    same = sum(1 for k in common if np.allclose(gk[k], ok_[k], rtol=1e-5, atol=1e-5))
+    rx, ry = d.cam.resx, d.cam.resy
+    q = np.random.default_rng(seed).choice(rx * ry, size=n_pix, replace=False)
+    px, py = (q % rx).astype(np.float32) + 0.5, (q // rx).astype(np.float32) + 0.5
+    print("photon: oracle sample of %d pixels" % n_pix, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    oc.trace_camera(d, px, py)
+    cpu_px = time.perf_counter() - t0
+    oc.photon_load(d, 0, gmap)
+    ref = oc.trace_camera(d, px, py).astype(np.float64)
+    g = img.reshape(ry, rx, 3)[q // rx, q % rx].astype(np.float64)
+    # first hits of the sample's camera rays on both sides, FP64 check of mismatches
+    o, dirs = _camera_rays(d.cam, q)
+    o32, d32 = o.astype(np.float32), dirs.astype(np.float32)
+    hits = torch.empty((n_pix, 4), dtype=torch.float32, device="cuda")
+    sc.first_hit(torch.from_numpy(o32).cuda(), torch.from_numpy(d32).cuda(), hits)
+    pg = hits.cpu().numpy()[:, 0].view(np.int32)
+    pc, _ = oc.first_hit(o32, d32)
+    mism = np.nonzero(pg != pc)[0]
+    false_pos = 0
+    for k in mism:
+        p = pc[k] if pc[k] >= 0 else pg[k]
+        A, B, C = d.verts[d.idx[p]].astype(np.float64)
+        n = np.cross(B - A, C - A)
+        f = o32[k].astype(np.float64) + d32[k].astype(np.float64) * 5e-5
+        r = d32[k].astype(np.float64)
+        r = -r if np.dot(r, n) < 0 else r
+        va, vb, vc = A - f, B - f, C - f
+        tp = [np.dot(np.cross(va, vb), r), np.dot(np.cross(vb, vc), r), np.dot(np.cross(vc, va), r)]
+        false_pos += int(min(tp) < 0 < max(tp))
+    agree = np.ones(n_pix, bool)
+    agree[mism] = False
+
+    def rrmse(x, y):
+        return float(np.sqrt(np.mean((x - y) ** 2)) / max(np.sqrt(np.mean(y ** 2)), 1e-30))
+
+    frame_ms = (cpu_build + cpu_px / n_pix * rx * ry) * 1e3
+    return {
+        "cpu_baseline": {"value": round(frame_ms, 1), "unit": "ms/frame", "cores": 1, "kind": "port",
+                         "sample": "oracle photon build (%.1f s) + %d sampled pixels (%.2f s) scaled to the "
+                                   "%dx%d frame, one thread" % (cpu_build, n_pix, cpu_px, rx, ry)},
+        "rmse_vs_oracle": {
+            "rel_rmse": rrmse(g, ref), "rel_rmse_same_first_hit": rrmse(g[agree], ref[agree]),
+            "pixels": n_pix, "camera_first_hit_differs": int(len(mism)),
+            "of_which_float_false_positive": int(false_pos),
+            "sample": "fixed-seed pixel sample of the full frame; the oracle renders it with the GPU's photon "
+                      "map; a differing first hit whose triangle the ray misses in FP64 is a false positive "
+                      "of the reference's float edge test, which trees decide differently"},
+        "photon_map_vs_oracle": {
+            "emitted": [int(pst.emitted), int(os_[0])], "stored": [int(pst.stored), int(os_[1])],
+            "clustered": [int(pst.clustered), int(os_[2])], "cells_equal": int(same),
+            "cells_common": int(len(common))},
     }
 
 

@@ -107,6 +107,8 @@ def _load(path):
         L.yo_photon_build.restype = C.c_int
         L.yo_photon_build.argtypes = [vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int,
                                       C.c_int, vp, C.c_int, vp, U64P]
+        L.yo_photon_load.restype = C.c_int
+        L.yo_photon_load.argtypes = [vp, F32P, C.c_int64, vp]
         L.yo_photon_read.restype = C.c_int64
         L.yo_photon_read.argtypes = [vp, vp, C.c_int64]
         L.yo_photon_gather.argtypes = [vp, vp, vp, vp, vp, F32P, F32P, F32P, F32P, I32P,
@@ -277,6 +279,13 @@ class OracleScene:
         if rc != 0:
             raise RuntimeError("yo_photon_build failed: %d" % rc)
         return stats
+
+    def photon_load(self, desc, li, marks):
+        """Use `marks` ([n, 9]: dir, pos, colour) as light li's clustered map."""
+        m = np.ascontiguousarray(marks, np.float32).reshape(-1)
+        rc = lib(self.fast).yo_photon_load(self.h, m, len(m) // 9, C.addressof(desc.lights[li]))
+        if rc != 0:
+            raise RuntimeError("yo_photon_load failed")
 
     def photon_read(self):
         L = lib(self.fast)

@@ -149,6 +149,10 @@ int yo_photon_build(yo_scene* s, const yg_material* mats, int nmat, const yg_mod
                     const yg_render_opts* opts, uint64_t* stats);
 /* copy clustered photons (9 floats each: dir, pos, colour) */
 int64_t yo_photon_read(yo_scene* s, float* out, int64_t cap);
+/* Installs a clustered photon map (n marks of 9 floats, e.g. read back from
+ * the GPU) as photon light L's map, with its gather grid, so that renders
+ * and gathers use it instead of a map yo_photon_build made. */
+int yo_photon_load(yo_scene* s, const float* data, int64_t n, const yg_light* L);
 /* photonLight_t::illuminate at explicit shading points (see yg_photon_gather) */
 void yo_photon_gather(yo_scene* s, const yg_material* mats, const yg_modulator* mods,
                       const yg_texture* texs, const yg_light* light, const float* p,

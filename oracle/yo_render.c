@@ -835,6 +835,20 @@ int yo_photon_build(yo_scene* s, const yg_material* mats, int nmat, const yg_mod
     return 0;
 }
 
+int yo_photon_load(yo_scene* s, const float* data, int64_t n, const yg_light* L)
+{
+    if (!L || L->type != YG_LIGHT_PHOTON || n < 0) return -1;
+    yo_photonmap* pm = &s->pm;
+    free(pm->data);
+    pm->data = (float*)malloc(sizeof(float) * 9 * (size_t)(n > 0 ? n : 1));
+    if (n > 0) memcpy(pm->data, data, sizeof(float) * 9 * (size_t)n);
+    pm->n = n;
+    pm->cell = L->cluster;
+    pm->np_emit = photon_np(L);
+    grid_build(pm, L->fixedradius);
+    return 0;
+}
+
 int64_t yo_photon_read(yo_scene* s, float* out, int64_t cap)
 {
     int64_t n = s->pm.n < cap ? s->pm.n : cap;
@@ -1293,15 +1307,17 @@ int yo_render_mt(yo_scene* s, const yg_camera* cam, const yg_material* mats, int
 
 /* scene_t::raytrace of camera rays through camera-plane points (px[q], py[q])
  * (camera_t::shootRay coordinates), one after another.  Test hook for scenes
- * whose lights keep no per-area state (sun lights, background): returns -6
- * for path, area and photon lights. */
+ * whose lights keep no per-area state (sun lights, photon lights whose map
+ * yo_photon_build made, background): returns -6 for path and area lights. */
 int yo_trace_camera(yo_scene* s, const yg_camera* cam, const yg_material* mats, int nmat,
                     const yg_modulator* mods, const yg_texture* texs, const yg_light* lights, int nl,
                     const int32_t* obj_flags, int nobj, const yg_render_opts* opts, const float* px,
                     const float* py, int64_t n, float* rgb)
 {
     for (int li = 0; li < nl; li++)
-        if (lights[li].type != YG_LIGHT_SUN) return -6;
+        if (lights[li].type != YG_LIGHT_SUN && lights[li].type != YG_LIGHT_PHOTON) return -6;
+    for (int li = 0; li < nl; li++)
+        if (lights[li].type == YG_LIGHT_PHOTON && !s->pm.data) return -3;  /* photon map not built */
     rctx R;
     rctx_init(&R, s, mats, nmat, mods, texs, lights, nl, obj_flags, nobj, opts);
     yo_cam k;

@@ -175,6 +175,40 @@ def test_photon_small_matches_oracle(torch_cuda):
     assert rel_rmse(img, ref) <= 1e-3
 
 
+@pytest.mark.parametrize("search", [50, 100])
+def test_photon_k50_merged_clusters_match_oracle(torch_cuda, search):
+    """The photon workload's settings at small scale: K = 50 (and 100, past
+    the old 64-entry heap), depth 1 (one diffuse bounce), mindepth 0 and the
+    default cluster size fixedradius / 4, so that stored photons merge into
+    marks (preGathering): map and image equal the oracle's."""
+    torch = torch_cuda
+    Y = _Y()
+    d = S.photon_field(n_side=8, subdiv=2, extent=20.0, resx=160, resy=120, photons=2 * 300 * 300,
+                       search=search, nmat=4)
+    assert abs(d.lights[0].cluster - d.lights[0].fixedradius / 4) < 1e-7 and d.lights[0].depth == 1
+    sc = Y.Scene.from_desc(d)
+    ps = sc.photon_build(0)
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    os_ = oc.photon_build(d, 0)
+    assert (ps.emitted, ps.stored, ps.clustered) == tuple(int(x) for x in os_[:3])
+    assert ps.clustered < 0.6 * ps.stored        # marks really merge
+    buf = torch.zeros(ps.clustered * 9, dtype=torch.float32, device="cuda")
+    sc.photon_read(buf)
+    g = buf.cpu().numpy().reshape(-1, 9)
+    assert np.allclose(g, oc.photon_read(), rtol=1e-5, atol=1e-5)
+    out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+    sc.render(out)
+    torch.cuda.synchronize()
+    img = out.cpu().numpy().reshape(d.cam.resy, d.cam.resx, 3)
+    ref, _ = oc.render(d)
+    assert np.all(np.isfinite(img))
+    assert rel_rmse(img, ref) <= 1e-3
+    # the oracle's per-pixel entry (camera ray + photon estimate) gives the same pixels
+    px = (np.arange(0, d.cam.resx, 7) + 0.5).astype(np.float32)
+    py = np.full_like(px, 60.5)
+    assert np.allclose(oc.trace_camera(d, px, py), img[60, np.arange(0, d.cam.resx, 7)], rtol=1e-4, atol=1e-6)
+
+
 @pytest.mark.parametrize("width", [2, 4])
 def test_batched_equals_single_renders(torch_cuda, width):
     """yg_render_batch over scenes of uneven size and resolution: every image

@@ -251,8 +251,7 @@ __global__ __launch_bounds__(64) void k_gather_points(RenderScene R, PhotonView 
                                                       const int32_t* __restrict__ mat, uint32_t n,
                                                       float* __restrict__ rgb)
 {
-    __shared__ float hd[YG_KNN_MAX * 64];
-    __shared__ int hi[YG_KNN_MAX * 64];
+    extern __shared__ float s_knn[];  // K distances, then K indices per lane
     const uint32_t i = blockIdx.x * 64 + threadIdx.x;
     if (i >= n) return;
     Surf sp;
@@ -263,8 +262,8 @@ __global__ __launch_bounds__(64) void k_gather_points(RenderScene R, PhotonView 
     sp.mat = mat[i];
     sp.obj = -1;
     sp.ti = -1;
-    const c3 c = photon_estimate(R, pm, K, fr, sp, ld3(E + 3 * (size_t)i), true, hd + threadIdx.x,
-                                 hi + threadIdx.x, 64);
+    const c3 c = photon_estimate(R, pm, K, fr, sp, ld3(E + 3 * (size_t)i), true, s_knn + threadIdx.x,
+                                 (int*)(s_knn + (size_t)K * 64) + threadIdx.x, 64);
     rgb[3 * (size_t)i] = c.r;
     rgb[3 * (size_t)i + 1] = c.g;
     rgb[3 * (size_t)i + 2] = c.b;
@@ -326,7 +325,7 @@ extern "C" int yg_photon_build(yg_scene* s, uint32_t li, const yg_render_opts* o
     if (s->mats.empty()) return yg_fail(YG_ERR_STATE, "no materials (yg_set_shading)");
     const yg_light& L = s->lights[li];
     if (L.depth < 0 || L.depth + 1 - L.mindepth <= 0) return yg_fail(YG_ERR_ARG, "photon depth/mindepth");
-    if (L.search < 1 || L.search > YG_KNN_MAX) return yg_fail(YG_ERR_UNSUPPORTED, "photon search must be 1..64");
+    if (L.search < 1 || L.search > YG_KNN_MAX) return yg_fail(YG_ERR_UNSUPPORTED, "photon search must be 1..128");
     if (!(L.cluster > 0) || !(L.fixedradius > 0)) return yg_fail(YG_ERR_ARG, "cluster and fixedradius must be > 0");
     hipStream_t st = (hipStream_t)stream;
     int rc;
@@ -567,7 +566,7 @@ extern "C" int yg_photon_gather(yg_scene* s, const float* d_p, const float* d_n,
     pv.gcell = s->pm.gcell;
     pv.np_emit = s->pm.np_emit;
     pv.n = s->pm.n;
-    hipLaunchKernelGGL(k_gather_points, dim3(grid_for(n, 64)), dim3(64), 0, (hipStream_t)stream, R, pv, L.search,
+    hipLaunchKernelGGL(k_gather_points, dim3(grid_for(n, 64)), dim3(64), knn_lds_bytes(L.search), (hipStream_t)stream, R, pv, L.search,
                        L.fixedradius, d_p, d_n, d_ng, d_eye, d_mat, n, d_rgb);
     YG_HIP(hipGetLastError());
     return YG_OK;

@@ -5,7 +5,11 @@
 
 #include "render_dev.h"
 
-#define YG_KNN_MAX 64
+#define YG_KNN_MAX 128  // photon "search" (K) at most: two K-entry rows per lane, 64 KB per 64-lane block
+
+// Dynamic LDS of a 64-lane K-nearest launch: a K-entry distance heap and a
+// K-entry index heap per lane ([k * 64 + lane]), sized for the launch's largest K
+static inline size_t knn_lds_bytes(int kmax) { return (size_t)2 * (size_t)kmax * 64 * sizeof(float); }
 
 // randomVectorCone (vector3d.cc:162-168)
 __device__ __host__ __forceinline__ f3 random_vector_cone(f3 D, f3 U, f3 V, float cosang, float z1, float z2)

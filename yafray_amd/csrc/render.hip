@@ -1181,6 +1181,7 @@ struct RenderWS {
     DevBuf<uint32_t> vx_flag, vx_pos, vx_lit, vx_nlit;  // the drawing vertices, compacted
     DevBuf<c3> vx_rs;
     bool any_area = false, any_photon = false;
+    int photon_kmax = 0;  // largest K of the launch's photon lights (heap entries per lane)
     // antialiasing passes: first-pass areas and colours, the pass's samples
     DevBuf<Area> areas0;
     DevBuf<c3> item_col, aa_col;
@@ -1308,7 +1309,7 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
         } else if (Lg.type == YG_LIGHT_PHOTON) {
             if (R.photon_light >= 0) return yg_fail(YG_ERR_UNSUPPORTED, "more than one photonlight");
             if (s->pm.light != li) return yg_fail(YG_ERR_STATE, "photon map not built (yg_photon_build)");
-            if (Lg.search < 1 || Lg.search > YG_KNN_MAX) return yg_fail(YG_ERR_UNSUPPORTED, "photon search must be 1..64");
+            if (Lg.search < 1 || Lg.search > YG_KNN_MAX) return yg_fail(YG_ERR_UNSUPPORTED, "photon search must be 1..128");
             R.photon_light = li;
             L.K = Lg.search;
             L.fixedradius = Lg.fixedradius;
@@ -1581,7 +1582,7 @@ static int launch_frame(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
                      W.cnt.p, 0, W.stk_ovf.p);
         YG_HIP(hipGetLastError());
         if (W.any_photon) {
-            hipLaunchKernelGGL(k_photon_direct, dim3(grid_for(nitems, 64)), dim3(64), 0, st, W.scenes.p,
+            hipLaunchKernelGGL(k_photon_direct, dim3(grid_for(nitems, 64)), dim3(64), knn_lds_bytes(W.photon_kmax), st, W.scenes.p,
                                W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lightc.p, nl_max);
             YG_HIP(hipGetLastError());
         }
@@ -1610,7 +1611,7 @@ static int launch_frame(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
     }
     YG_HIP(hipGetLastError());
     if (W.any_photon) {
-        hipLaunchKernelGGL(k_photon_direct, dim3(grid_for(nitems, 64)), dim3(64), 0, st, W.scenes.p,
+        hipLaunchKernelGGL(k_photon_direct, dim3(grid_for(nitems, 64)), dim3(64), knn_lds_bytes(W.photon_kmax), st, W.scenes.p,
                            W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lightc.p, nl_max);
         YG_HIP(hipGetLastError());
     }
@@ -1693,6 +1694,7 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     uint32_t nitems = 0;
     int S = 0, D = 0, nl_max = 1;
     bool any_path = false, any_photon = false, any_area = false;
+    int photon_kmax = 1;
     for (auto& P : plans) {
         RenderScene R = P.R;
         R.item_base = nitems;
@@ -1721,7 +1723,10 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
             S = R.samples;
             D = R.depth;
         }
-        if (R.photon_light >= 0) any_photon = true;
+        if (R.photon_light >= 0) {
+            any_photon = true;
+            photon_kmax = std::max(photon_kmax, P.lights[R.photon_light].K);
+        }
         scenes.push_back(R);
     }
     if (!any_path) { S = 1; D = 1; }
@@ -1820,6 +1825,7 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     W.any_path = any_path;
     W.any_area = any_area;
     W.any_photon = any_photon;
+    W.photon_kmax = photon_kmax;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
     std::vector<hipEvent_t> kev;
     if (ktiming) timing = true;
@@ -2050,8 +2056,9 @@ __global__ __launch_bounds__(64) void k_photon_direct(const RenderScene* __restr
                                                       const float* __restrict__ item_t,
                                                       c3* __restrict__ lightc, int nl_max)
 {
-    __shared__ float hd[YG_KNN_MAX * 64];
-    __shared__ int hi[YG_KNN_MAX * 64];
+    // K-entry heaps of the block's 64 lanes: distances, then indices (entry k
+    // of lane t at [k * 64 + t]), sized by the launch's largest K
+    extern __shared__ float s_knn[];
     const uint32_t item = blockIdx.x * 64 + threadIdx.x;
     if (item >= nitems) return;
     const int ti = item_ti[item];
@@ -2065,7 +2072,8 @@ __global__ __launch_bounds__(64) void k_photon_direct(const RenderScene* __restr
     const Surf sp = primary_surf(R, a, item, ti, item_t[item], eye);
     const bool receive = (R.obj_flags[sp.obj] & 2) != 0;
     lightc[(size_t)item * nl_max + R.photon_light] =
-        photon_estimate(R, R.pm, L.K, L.fixedradius, sp, eye, receive, hd + threadIdx.x, hi + threadIdx.x, 64);
+        photon_estimate(R, R.pm, L.K, L.fixedradius, sp, eye, receive, s_knn + threadIdx.x,
+                        (int*)(s_knn + (size_t)L.K * 64) + threadIdx.x, 64);
 }
 
 // ------------------------------------------------------------ adjoint --
