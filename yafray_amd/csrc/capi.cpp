@@ -58,6 +58,10 @@ static void free_dev(yg_scene* s)
     s->d_texs = nullptr; s->pm = PhotonMapDev();
     s->d_halton = nullptr; s->halton_ndim = 0; s->halton_stride = 0;
     s->d_any_ovf = nullptr; s->any_ovf_ints = 0;
+    for (float* p : s->ph_halton)
+        if (p) (void)hipFree(p);
+    s->ph_halton.clear();
+    s->ph_halton_np = -1;
 }
 
 extern "C" int yg_scene_destroy(yg_scene* s)

@@ -165,23 +165,49 @@ __global__ void k_photon_compact(uint32_t n, const uint32_t* __restrict__ flag, 
     bidx[p] = ex[p];
 }
 
+__device__ __forceinline__ int wave_min(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Stored marks to their insertion-ordered positions, with their hash cells;
+// the cell range of all marks is reduced per wave (one atomic per wave and
+// bound instead of one per mark)
 __global__ void k_mark_gather(uint32_t nslots, const uint32_t* __restrict__ mvalid, const uint32_t* __restrict__ mpos,
                               const Mark* __restrict__ slots, Mark* __restrict__ out, float cell,
                               int* __restrict__ cells, int* __restrict__ bounds)
 {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslots || !mvalid[s]) return;
-    const uint32_t o = mpos[s];
-    const Mark m = slots[s];
-    out[o] = m;
-    int bx, by, bz;
-    find_box(cell, m.pos, bx, by, bz);
-    cells[3 * (size_t)o] = bx;
-    cells[3 * (size_t)o + 1] = by;
-    cells[3 * (size_t)o + 2] = bz;
-    atomicMin(&bounds[0], bx); atomicMax(&bounds[1], bx);
-    atomicMin(&bounds[2], by); atomicMax(&bounds[3], by);
-    atomicMin(&bounds[4], bz); atomicMax(&bounds[5], bz);
+    int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    if (s < nslots && mvalid[s]) {
+        const uint32_t o = mpos[s];
+        const Mark m = slots[s];
+        out[o] = m;
+        int b[3];
+        find_box(cell, m.pos, b[0], b[1], b[2]);
+        for (int k = 0; k < 3; k++) {
+            cells[3 * (size_t)o + k] = b[k];
+            lo[k] = b[k];
+            hi[k] = b[k];
+        }
+    }
+    // every lane of the wave reaches the reductions (no early return above)
+    for (int k = 0; k < 3; k++) {
+        lo[k] = wave_min(lo[k]);
+        hi[k] = wave_max(hi[k]);
+    }
+    if ((threadIdx.x & 63) == 0 && lo[0] != INT32_MAX) {
+        for (int k = 0; k < 3; k++) {
+            atomicMin(&bounds[2 * k], lo[k]);
+            atomicMax(&bounds[2 * k + 1], hi[k]);
+        }
+    }
 }
 
 __global__ void k_mark_keys(uint32_t n, const int* __restrict__ cells, int bx0, int by0, int bz0, int sy, int sz,
@@ -349,19 +375,26 @@ extern "C" int yg_photon_build(yg_scene* s, uint32_t li, const yg_render_opts* o
     const uint32_t natt = (uint32_t)np + 64;
     const int ndim = 2 * (L.depth + 1);
     Freer F;
-    std::vector<float*> hdim(ndim, nullptr);
-    {
+    // Halton tables of the emission (dimensions 0, 1) and bounce dimensions,
+    // built on the host with the reference's incremental rule and kept on the
+    // device while the photon count and depth repeat
+    if (s->ph_halton_np != np || (int)s->ph_halton.size() != ndim) {
+        for (float* p : s->ph_halton)
+            if (p) (void)hipFree(p);
+        s->ph_halton.assign(ndim, nullptr);
+        s->ph_halton_np = -1;
         std::vector<float> tab;
         int base = 2;
         for (int dd = 0; dd < ndim; dd++) {
             const uint64_t len = (dd < 2) ? natt : (uint64_t)np + 1;
             halton_incremental(base, len, tab);
-            if ((rc = F.alloc(&hdim[dd], len))) return rc;
-            YG_HIP(hipMemcpyAsync(hdim[dd], tab.data(), sizeof(float) * len, hipMemcpyHostToDevice, st));
-            YG_HIP(hipStreamSynchronize(st));
+            if ((rc = dmalloc(&s->ph_halton[dd], len))) return rc;
+            YG_HIP(hipMemcpy(s->ph_halton[dd], tab.data(), sizeof(float) * len, hipMemcpyHostToDevice));
             base = next_prime(base);
         }
+        s->ph_halton_np = np;
     }
+    const std::vector<float*>& hdim = s->ph_halton;
     uint32_t *valid, *vpos, *att;
     if ((rc = F.alloc(&valid, natt)) || (rc = F.alloc(&vpos, natt)) || (rc = F.alloc(&att, np))) return rc;
     hipLaunchKernelGGL(k_emit_valid, dim3(grid_for(natt, 256)), dim3(256), 0, st, src, hdim[0], hdim[1], natt, valid);

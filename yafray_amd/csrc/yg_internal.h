@@ -134,6 +134,8 @@ struct yg_scene {
     yg_texture* d_texs = nullptr;
 
     PhotonMapDev pm;
+    std::vector<float*> ph_halton;  // yg_photon_build: device Halton tables per dimension
+    int ph_halton_np = -1;          //   for this emitted-photon count
     int* d_any_ovf = nullptr;  // yg_trace_any: traversal stack entries beyond the LDS part
     size_t any_ovf_ints = 0;
     uint64_t gen = 0;  // bumped by every change of geometry, shading, lights or camera
