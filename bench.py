@@ -30,9 +30,20 @@ def gpu_leg(args):
     t0 = time.time()
     d = S.terrain_gi(Y.device_heights)
     sc = Y.Scene.from_desc(d)
-    info = sc.info()
     setup_s = time.time() - t0
     out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+    # the autotuner picks BVH width, rays per lane and compaction threshold on
+    # this frame (before the timed region) and leaves the scene built with them
+    t0 = time.time()
+    widths = (2, 4, 8) if not args.width else (args.width,)
+    rays_c, comp_c = ((1, 2, 4), (1.0, 0.5, -1)) if not args.width else ((1,), (1.0,))
+    best, best_ms, cands = sc.autotune(out, widths=widths, leaves=(4,), repeats=2, rays=rays_c, compacts=comp_c)
+    tune = {"candidates": len(cands), "seconds": round(time.time() - t0, 1), "best_ms": round(best_ms, 3),
+            "best": {"width": int(best.width), "max_leaf": int(best.max_leaf), "sah_bins": int(best.sah_bins),
+                     "stack_lds": int(best.stack_lds) or 16, "rays_per_lane": int(best.rays_per_lane),
+                     "compact_below": float(best.compact_below)},
+            "slowest_ms": round(max(c[3] for c in cands), 3)}
+    info = sc.info()
     opts = D.yg_render_opts.from_buffer_copy(d.opts)
     for _ in range(args.warmup):
         sc.render(out, opts)
@@ -53,7 +64,7 @@ def gpu_leg(args):
     torch.cuda.synchronize()
     img = out.cpu().numpy().reshape(d.cam.resy, d.cam.resx, 3)
     return {
-        "desc": d, "img": img,
+        "desc": d, "img": img, "tune": tune,
         "dt": dt, "rays": rays, "stats": ks, "info": info, "setup_s": setup_s,
         "graph_modes": sorted(modes),
         "finite": bool(np.all(np.isfinite(img))), "mean": float(img.mean()),
@@ -771,6 +782,8 @@ def main():
                     choices=["gi", "batched", "adjoint", "photon", "direct", "autotune", "aa", "rays",
                              "noise", "room_gi"])
     ap.add_argument("--batch-n", type=int, default=64)
+    ap.add_argument("--width", type=int, default=0, choices=[0, 2, 4, 8],
+                    help="GI workload: BVH width to measure (0 = the autotuner's choice)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy after the timed region (counter calibration)")
     args = ap.parse_args()
@@ -851,6 +864,7 @@ def main():
         },
         "bvh": {"nodes": int(g["info"]["nodes"]), "max_depth": int(g["info"]["max_depth"]),
                 "dev_bytes": int(g["info"]["dev_bytes"])},
+        "autotune": g["tune"],
         "image_finite": g["finite"],
         # 1 = frame replayed from its captured hipGraph, 2 = captured in that call, 0 = kernel by kernel
         "frame_graph_modes": g["graph_modes"],

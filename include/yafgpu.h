@@ -33,7 +33,7 @@ extern "C" {
 #define YG_ERR_UNSUPPORTED (-4)
 #define YG_ERR_NOMEM (-5)
 
-#define YG_ABI_VERSION 7
+#define YG_ABI_VERSION 8
 
 /* ---------------------------------------------------------------- rays -- */
 
@@ -217,10 +217,17 @@ typedef struct yg_build_opts {
     int32_t max_leaf;      /* triangles per leaf, 1..8 */
     int32_t sah_bins;      /* 8..64 */
     int32_t threads;       /* host build threads, 0 = all */
-    int32_t width;         /* BVH node width: 2 or 4 (0 = default, 4) */
+    int32_t width;         /* BVH node width: 2, 4 or 8 (0 = default, 4) */
     int32_t stack_lds;     /* traversal stack entries per lane kept in LDS by the render
                               launches, deeper ones in device memory (0 = default 16,
                               -1 = the whole stack in LDS) */
+    int32_t rays_per_lane; /* rays each lane of the GI level launches (extension and shadow
+                              rays) traces one after another: 64 x this per wave (0 = 1; 1, 2, 4) */
+    float compact_below;   /* a GI level's surviving paths are compacted into a dense list
+                              for the next level when fewer than this fraction of the
+                              level's listed paths survive; otherwise the next level reuses
+                              the list and its ended paths' lanes exit at once
+                              (0 = default 1: always compact; 0 < x <= 1, or -1 = never) */
 } yg_build_opts;
 
 typedef struct yg_scene yg_scene;
@@ -351,13 +358,18 @@ int yg_render_batch(yg_scene** scenes, uint32_t n, const yg_render_opts* opts, f
 /* --------------------------------------------------------- autotuner -- */
 
 typedef struct yg_tune_opts {
-    int32_t widths;   /* BVH node widths to try, as bits: 1<<2 (two-wide), 1<<4 (four-wide); 0 = both */
+    int32_t widths;   /* BVH node widths to try, as bits: 1<<2 (two-wide), 1<<4 (four-wide), 1<<8 (eight-wide);
+                         0 = all three */
     int32_t leaves;   /* max_leaf values to try, as bits 1<<n, n in 1..8; 0 = 2, 4 and 8 */
     int32_t bins;     /* SAH bin counts to try, as bits 1<<k for 2^k bins, k in 3..6; 0 = 32 only */
     int32_t repeats;  /* timed renders per candidate after one untimed render; 0 = 2 */
     int32_t threads;  /* host build threads; 0 = all */
     int32_t stacks;   /* stack_lds values to try, as bits: 1<<0 = whole stack in LDS,
                          1<<k = 2^k entries, k in 3..6; 0 = the build's default only */
+    int32_t rays;     /* rays_per_lane values to try, as bits 1<<k for 2^k rays, k in 0..2;
+                         0 = 1 only */
+    int32_t compacts; /* compact_below values to try, as bits: 1<<k = k/4 for k in 1..4,
+                         1<<0 = never; 0 = always compact only */
 } yg_tune_opts;
 
 #define YG_TUNE_MAX 32
@@ -372,7 +384,8 @@ typedef struct yg_tune_result {
 
 /* Autotuner (no reference counterpart: the reference's kd-tree has fixed
  * build constants, kdtree.cc:57-138).  Builds the scene with every candidate
- * (node width x leaf size x SAH bins x LDS stack entries), times yg_render of the caller's frame
+ * (node width x leaf size x SAH bins, and for each build the launch settings
+ * LDS stack entries x rays per lane x compaction threshold), times yg_render of the caller's frame
  * (opts, d_rgb: the image of the last render is left there) and leaves the
  * scene built with the fastest.  Images do not depend on the build (exact
  * distance ties go to the lowest primitive id).  At most YG_TUNE_MAX

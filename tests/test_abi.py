@@ -30,7 +30,7 @@ def test_error_path_without_gpu():
     """Argument errors are reported through the status code and yg_last_error."""
     from yafray_amd import lib as Y
     L = Y.lib()
-    assert L.yg_abi_version() == 7
+    assert L.yg_abi_version() == 8
     assert L.yg_scene_create(None) == -1
     assert "null" in Y.last_error()
     h = ctypes.c_void_p()
@@ -91,10 +91,27 @@ def test_autotune_argument_errors_without_gpu():
     assert L.yg_scene_create(ctypes.byref(h)) == 0
     o = D.yg_render_opts()
     dummy = ctypes.c_void_p(16)  # never dereferenced: the masks are checked first
-    for widths, leaves, bins, msg in ((1 << 3, 0, 0, "widths"), (0, 1 << 9, 0, "leaves"),
-                                      (0, 0, 1 << 7, "bins")):
-        t = D.yg_tune_opts(widths=widths, leaves=leaves, bins=bins)
+    for widths, leaves, bins, rays, compacts, msg in ((1 << 3, 0, 0, 0, 0, "widths"), (0, 1 << 9, 0, 0, 0, "leaves"),
+                                                      (0, 0, 1 << 7, 0, 0, "bins"), (0, 0, 0, 1 << 3, 0, "rays"),
+                                                      (0, 0, 0, 0, 1 << 5, "compacts")):
+        t = D.yg_tune_opts(widths=widths, leaves=leaves, bins=bins, rays=rays, compacts=compacts)
         assert L.yg_autotune(h, ctypes.byref(o), dummy, ctypes.byref(t), ctypes.byref(r), None) == -1
+        assert msg in Y.last_error()
+    assert L.yg_scene_destroy(h) == 0
+
+
+def test_build_launch_settings_are_checked_without_gpu():
+    """yg_build refuses a width, rays_per_lane or compact_below outside
+    their ranges before anything reaches the device."""
+    from yafray_amd import descriptors as D
+    from yafray_amd import lib as Y
+    L = Y.lib()
+    h = ctypes.c_void_p()
+    assert L.yg_scene_create(ctypes.byref(h)) == 0
+    for kw, msg in ((dict(width=3), "width"), (dict(rays_per_lane=3), "rays_per_lane"),
+                    (dict(compact_below=1.5), "compact_below"), (dict(compact_below=-0.5), "compact_below")):
+        o = D.yg_build_opts(**kw)
+        assert L.yg_build(h, ctypes.byref(o)) == -1
         assert msg in Y.last_error()
     assert L.yg_scene_destroy(h) == 0
 

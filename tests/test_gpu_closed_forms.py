@@ -30,7 +30,7 @@ def _oracle(d):
     return po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
 
 
-@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_two_bounce_pathlight_closed_form_gpu(torch_cuda, width):
     S_, rx, ry = 4, 80, 72
     d, sun = CF.two_bounce(samples=S_, depth=3, resx=rx, resy=ry)
@@ -40,7 +40,7 @@ def test_two_bounce_pathlight_closed_form_gpu(torch_cuda, width):
     assert st.rays_extension == 2 * S_ * st.rays_camera
 
 
-@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_level1_hits_on_a_sunlit_wall_gpu(torch_cuda, width):
     S_ = 4
     d, sun = CF.wall_bounce(samples=S_, resx=48, resy=40)

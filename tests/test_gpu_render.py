@@ -175,6 +175,27 @@ def test_photon_small_matches_oracle(torch_cuda):
     assert rel_rmse(img, ref) <= 1e-3
 
 
+@pytest.mark.parametrize("rays,compact", [(2, 1.0), (4, 1.0), (1, -1.0), (1, 0.5), (2, 0.75)])
+def test_level_launch_settings_keep_the_image(torch_cuda, rays, compact):
+    """Rays per lane of the GI level launches and the compaction threshold
+    change which lane traces which path, never a result: the image and the
+    ray counts equal the default launch's bit for bit (also under graph
+    replay and in the adjoint's level replay)."""
+    torch = torch_cuda
+    d = small_terrain(nx=96, ny=48, resx=120, resy=80, samples=4, depth=4, nmat=8)
+    ref, st0, _ = gpu_render(d, torch)
+    Y = _Y()
+    sc = Y.Scene.from_desc(d, rays_per_lane=rays, compact_below=compact)
+    out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+    for _ in range(3):  # eager, captured, replayed
+        out.zero_()
+        st = sc.render(out)
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy().reshape(ref.shape), ref)
+        assert (st.rays_camera, st.rays_extension, st.rays_shadow) == (st0.rays_camera, st0.rays_extension,
+                                                                       st0.rays_shadow)
+
+
 @pytest.mark.parametrize("search", [50, 100])
 def test_photon_k50_merged_clusters_match_oracle(torch_cuda, search):
     """The photon workload's settings at small scale: K = 50 (and 100, past
@@ -209,7 +230,7 @@ def test_photon_k50_merged_clusters_match_oracle(torch_cuda, search):
     assert np.allclose(oc.trace_camera(d, px, py), img[60, np.arange(0, d.cam.resx, 7)], rtol=1e-4, atol=1e-6)
 
 
-@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_batched_equals_single_renders(torch_cuda, width):
     """yg_render_batch over scenes of uneven size and resolution: every image
     equals that scene's own yg_render bit for bit, and one of them the oracle."""
@@ -278,8 +299,9 @@ def test_frame_graph_replays_equal_eager(torch_cuda):
 
 
 def test_autotune_keeps_fastest_build(torch_cuda):
-    """yg_autotune times every (width, leaf) candidate on the frame, leaves the
-    fastest build in place, and the image does not depend on the build."""
+    """yg_autotune times every (width 2/4/8, leaf) build with every launch
+    setting (rays per lane, compaction threshold) on the frame, leaves the
+    fastest in place, and the image does not depend on the choice."""
     torch = torch_cuda
     Y = _Y()
     d = small_terrain(nx=256, ny=128, resx=160, resy=90, samples=2)
@@ -287,9 +309,11 @@ def test_autotune_keeps_fastest_build(torch_cuda):
     n = d.cam.resx * d.cam.resy * 3
     ref = torch.zeros(n, dtype=torch.float32, device="cuda")
     sc.render(ref)
-    best, best_ms, cands = sc.autotune(ref, widths=(2, 4), leaves=(2, 4), repeats=2)
-    assert len(cands) == 4
-    assert sorted((c[0], c[1]) for c in cands) == [(2, 2), (2, 4), (4, 2), (4, 4)]
+    best, best_ms, cands = sc.autotune(ref, widths=(2, 4, 8), leaves=(2, 4), repeats=2, rays=(1, 2),
+                                       compacts=(1.0, -1))
+    assert len(cands) == 3 * 2 * 2 * 2
+    assert sorted(set((c[0], c[1]) for c in cands)) == [(2, 2), (2, 4), (4, 2), (4, 4), (8, 2), (8, 4)]
+    assert sorted(set((c[6], c[7]) for c in cands)) == [(1, -1.0), (1, 1.0), (2, -1.0), (2, 1.0)]
     assert all(c[3] > 0 and c[4] > 0 for c in cands)
     assert best_ms == min(c[3] for c in cands)
     assert (best.width, best.max_leaf) == next((c[0], c[1]) for c in cands if c[3] == best_ms)

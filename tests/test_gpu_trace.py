@@ -52,10 +52,10 @@ def _check(prim_g, t_g, u_g, v_g, prim_c, t_c, u_c, v_c):
 
 def test_library_loads(torch_cuda):
     Y = _lib(torch_cuda)
-    assert Y.lib().yg_abi_version() == 7
+    assert Y.lib().yg_abi_version() == 8
 
 
-@pytest.mark.parametrize("seed,width", [(0, 2), (1, 2), (0, 4), (1, 4)])
+@pytest.mark.parametrize("seed,width", [(0, 2), (1, 2), (0, 4), (1, 4), (0, 8), (1, 8)])
 def test_closest_any_room(torch_cuda, seed, width):
     torch = torch_cuda
     Y = _lib(torch)
@@ -75,7 +75,7 @@ def test_closest_any_room(torch_cuda, seed, width):
     assert np.mean(ag == (ac >= 0)) >= 0.9999
 
 
-@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_tmin_and_empty(torch_cuda, width):
     torch = torch_cuda
     Y = _lib(torch)
@@ -99,7 +99,7 @@ def test_tmin_and_empty(torch_cuda, width):
     assert np.all(p == -1)
 
 
-@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_terrain_parity_full_size(torch_cuda, width):
     """1,045,506-triangle fBm terrain: 1M incoherent rays on the GPU, a
     100k-ray sample checked against the oracle."""
@@ -143,7 +143,7 @@ def test_terrain_parity_full_size(torch_cuda, width):
     assert 0.05 < np.mean(ag) < 0.95
 
 
-@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_first_hit_is_shadowed(torch_cuda, width):
     torch = torch_cuda
     Y = _lib(torch)

@@ -21,7 +21,7 @@ def _rays(n, seed, tmax):
     return r
 
 
-@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_noalloc_any_hit_equals_yg_trace_any(torch_cuda, width):
     torch = torch_cuda
     from yafray_amd import lib as Y

@@ -193,34 +193,45 @@ static void inflate(const BBox& b, float G, float lo[3], float hi[3])
 
 }  // namespace
 
-// Collapses the binary tree into four-wide nodes: a node's children start as
-// the binary node's two and the largest-area inner child is replaced by its
-// own two children until there are four or only leaves remain.  Boxes and
+// Box rows of a wide node (0..5: lo.x hi.x lo.y hi.y lo.z hi.z, one float
+// per child) and its child references.
+static float* node_row(GNode4& g, int r) { return &(&g.lox)[r].x; }
+static int32_t* node_child(GNode4& g) { return &g.child.x; }
+static float* node_row(GNode8& g, int r) { return g.box[r]; }
+static int32_t* node_child(GNode8& g) { return g.child; }
+
+// Collapses the binary tree into N-wide nodes (N = 4 or 8): a node's children
+// start as the binary node's two and the largest-area inner child is replaced
+// by its own two children until there are N or only leaves remain.  Boxes and
 // leaf references are those of the binary tree, so the width changes the
 // traversal order only, never which triangles a ray is tested against.
-static void export_width4(const BNode* bn, float G, HostBVH& out,
-                          int32_t (*enc_leaf)(const BNode&), void (*inflate_fn)(const BBox&, float, float*, float*))
+// `stack` gets the deepest traversal stack the tree can need: a visit pushes
+// at most (children - 1) entries on top of what its ancestors left.
+template <int N, class Node>
+static void export_wide(const BNode* bn, float G, std::vector<Node>& nodes, int32_t& stack,
+                        int32_t (*enc_leaf)(const BNode&), void (*inflate_fn)(const BBox&, float, float*, float*))
 {
-    out.nodes4.clear();
-    out.stack4 = 0;
-    auto set_empty = [](GNode4& g, int k) {
-        float* f[6] = {&g.lox.x, &g.hix.x, &g.loy.x, &g.hiy.x, &g.loz.x, &g.hiz.x};
+    nodes.clear();
+    stack = 0;
+    auto set_empty = [](Node& g, int k) {
+        float* f[6] = {node_row(g, 0), node_row(g, 1), node_row(g, 2), node_row(g, 3), node_row(g, 4), node_row(g, 5)};
         for (int c = 0; c < 6; c++) f[c][k] = INFINITY;
-        (&g.child.x)[k] = 0;
+        node_child(g)[k] = 0;
     };
-    auto set_child = [&](GNode4& g, int k, const BNode& c, int32_t ref) {
+    auto set_child = [&](Node& g, int k, const BNode& c, int32_t ref) {
         float lo[3], hi[3];
         inflate_fn(c.box, G, lo, hi);
-        (&g.lox.x)[k] = lo[0]; (&g.hix.x)[k] = hi[0];
-        (&g.loy.x)[k] = lo[1]; (&g.hiy.x)[k] = hi[1];
-        (&g.loz.x)[k] = lo[2]; (&g.hiz.x)[k] = hi[2];
-        (&g.child.x)[k] = ref;
+        for (int a = 0; a < 3; a++) {
+            node_row(g, 2 * a)[k] = lo[a];
+            node_row(g, 2 * a + 1)[k] = hi[a];
+        }
+        node_child(g)[k] = ref;
     };
     auto children = [&](int64_t bi, int64_t* ch) -> int {
         int n = 0;
         ch[n++] = bn[bi].left;
         ch[n++] = bn[bi].right;
-        while (n < 4) {
+        while (n < N) {
             int best = -1;
             double ba = -1.0;
             for (int k = 0; k < n; k++)
@@ -236,45 +247,51 @@ static void export_width4(const BNode* bn, float G, HostBVH& out,
     };
     const BNode& root = bn[0];
     if (root.left < 0) {
-        GNode4 g;
+        Node g;
         memset(&g, 0, sizeof(g));
-        for (int k = 0; k < 4; k++) set_empty(g, k);
+        for (int k = 0; k < N; k++) set_empty(g, k);
         if (root.count > 0) set_child(g, 0, root, enc_leaf(root));
-        out.nodes4.push_back(g);
-        out.stack4 = 0;
+        nodes.push_back(g);
+        stack = 0;
         return;
     }
     // (binary node, slot, stack entries pending above this node)
     struct Work { int64_t bi; int64_t gi; int32_t pend; };
     std::vector<Work> work;
-    out.nodes4.emplace_back();
+    nodes.emplace_back();
     work.push_back({0, 0, 0});
     while (!work.empty()) {
         const Work w = work.back();
         work.pop_back();
-        int64_t ch[4];
+        int64_t ch[N];
         const int n = children(w.bi, ch);
-        GNode4 g;
+        Node g;
         memset(&g, 0, sizeof(g));
-        int32_t refs[4];
+        int32_t refs[N];
         for (int k = 0; k < n; k++) {
             if (bn[ch[k]].left < 0) refs[k] = enc_leaf(bn[ch[k]]);
             else {
-                refs[k] = (int32_t)out.nodes4.size();
-                out.nodes4.emplace_back();
+                refs[k] = (int32_t)nodes.size();
+                nodes.emplace_back();
             }
         }
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < N; k++) {
             if (k < n) set_child(g, k, bn[ch[k]], refs[k]);
             else set_empty(g, k);
         }
-        out.nodes4[w.gi] = g;
-        // a visit pushes at most n-1 siblings on top of what its ancestors left
+        nodes[w.gi] = g;
         const int32_t pend = w.pend + (n - 1);
-        out.stack4 = std::max(out.stack4, pend);
+        stack = std::max(stack, pend);
         for (int k = n - 1; k >= 0; k--)
             if (bn[ch[k]].left >= 0) work.push_back({ch[k], refs[k], pend});
     }
+}
+
+static void export_width(int width, const BNode* bn, float G, HostBVH& out)
+{
+    auto enc = [](const BNode& n) -> int32_t { return ~(int32_t)(n.start * YG_LEAF_MAX + (n.count - 1)); };
+    if (width == 4) export_wide<4>(bn, G, out.nodes4, out.stack4, enc, inflate);
+    if (width == 8) export_wide<8>(bn, G, out.nodes8, out.stack8, enc, inflate);
 }
 
 void build_bvh(const float* verts, const uint32_t* idx, int64_t nt, const int32_t* tri_obj,
@@ -388,9 +405,7 @@ void build_bvh(const float* verts, const uint32_t* idx, int64_t nt, const int32_
         set_child(g, 1, emptyn, ~0);
         if (root.count == 0) set_child(g, 0, emptyn, ~0);
         out.nodes.push_back(g);
-        if (width == 4)
-            export_width4(bn.get(), G, out, [](const BNode& n) -> int32_t { return ~(int32_t)(n.start * YG_LEAF_MAX + (n.count - 1)); },
-                          inflate);
+        export_width(width, bn.get(), G, out);
         return;
     }
     out.nodes.emplace_back();
@@ -416,7 +431,5 @@ void build_bvh(const float* verts, const uint32_t* idx, int64_t nt, const int32_
         for (int k = 1; k >= 0; k--)
             if (ch[k]->left >= 0) work.push_back({(int64_t)(k == 0 ? n.left : n.right), refs[k]});
     }
-    if (width == 4)
-        export_width4(bn.get(), G, out, [](const BNode& n) -> int32_t { return ~(int32_t)(n.start * YG_LEAF_MAX + (n.count - 1)); },
-                      inflate);
+    export_width(width, bn.get(), G, out);
 }

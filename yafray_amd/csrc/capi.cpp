@@ -48,13 +48,13 @@ extern "C" int yg_scene_create(yg_scene** out)
 
 static void free_dev(yg_scene* s)
 {
-    void* ptrs[] = {s->d_nodes, s->d_nodes4, s->d_tris, s->d_tprim, s->d_tobj, s->d_tmat, s->d_obj_flags,
+    void* ptrs[] = {s->d_nodes, s->d_nodes4, s->d_nodes8, s->d_tris, s->d_tprim, s->d_tobj, s->d_tmat, s->d_obj_flags,
                     s->d_mats, s->d_mods, s->d_texs, s->pm.d_photons, s->pm.d_gstart,
                     s->pm.d_gitems, s->d_halton, s->d_any_ovf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    s->d_nodes = nullptr; s->d_nodes4 = nullptr; s->d_tris = nullptr; s->d_tprim = nullptr; s->d_tobj = nullptr;
-    s->d_tmat = nullptr; s->d_obj_flags = nullptr; s->d_mats = nullptr; s->d_mods = nullptr;
+    s->d_nodes = nullptr; s->d_nodes4 = nullptr; s->d_nodes8 = nullptr; s->d_tris = nullptr; s->d_tprim = nullptr;
+    s->d_tobj = nullptr; s->d_tmat = nullptr; s->d_obj_flags = nullptr; s->d_mats = nullptr; s->d_mods = nullptr;
     s->d_texs = nullptr; s->pm = PhotonMapDev();
     s->d_halton = nullptr; s->halton_ndim = 0; s->halton_stride = 0;
     s->d_any_ovf = nullptr; s->any_ovf_ints = 0;
@@ -104,25 +104,32 @@ extern "C" int yg_build(yg_scene* s, const yg_build_opts* opts)
     int bins = opts ? opts->sah_bins : 32;
     int threads = opts ? opts->threads : 0;
     int width = (opts && opts->width) ? opts->width : YG_DEFAULT_WIDTH;
-    if (width != 2 && width != 4) return yg_fail(YG_ERR_ARG, "BVH width must be 2 or 4");
+    if (width != 2 && width != 4 && width != 8) return yg_fail(YG_ERR_ARG, "BVH width must be 2, 4 or 8");
     const int stack_lds = (opts && opts->stack_lds) ? opts->stack_lds : YG_DEFAULT_STACK_LDS;
     if (stack_lds < -1 || stack_lds > YG_MAX_STACK) return yg_fail(YG_ERR_ARG, "stack_lds must be -1 or 0..YG_MAX_STACK");
+    const int rays_lane = (opts && opts->rays_per_lane) ? opts->rays_per_lane : 1;
+    if (rays_lane != 1 && rays_lane != 2 && rays_lane != 4) return yg_fail(YG_ERR_ARG, "rays_per_lane must be 1, 2 or 4");
+    float compact_below = (opts && opts->compact_below != 0.0f) ? opts->compact_below : 1.0f;
+    if (compact_below == -1.0f) compact_below = 0.0f;  // never compact (0 < 0 is never true)
+    else if (!(compact_below > 0.0f && compact_below <= 1.0f))
+        return yg_fail(YG_ERR_ARG, "compact_below must be in (0, 1] or -1");
     HostBVH h;
     const int64_t nt = (int64_t)s->tri_mat.size();
     build_bvh(s->verts.data(), s->idx.data(), nt, s->tri_obj.data(), s->tri_mat.data(), max_leaf,
               bins, threads, width, h);
-    const int entries = (width == 4) ? h.stack4 + 1 : h.max_depth + 1;
+    const int entries = (width == 8) ? h.stack8 + 1 : (width == 4) ? h.stack4 + 1 : h.max_depth + 1;
     if (entries > YG_MAX_STACK) return yg_fail(YG_ERR_STATE, "BVH deeper than the traversal stack");
     // the device copy replaces the previous one
-    void* old[] = {s->d_nodes, s->d_nodes4, s->d_tris, s->d_tprim, s->d_tobj, s->d_tmat, s->d_obj_flags};
+    void* old[] = {s->d_nodes, s->d_nodes4, s->d_nodes8, s->d_tris, s->d_tprim, s->d_tobj, s->d_tmat, s->d_obj_flags};
     for (void* p : old)
         if (p) (void)hipFree(p);
-    s->d_nodes = nullptr; s->d_nodes4 = nullptr; s->d_tris = nullptr; s->d_tprim = nullptr; s->d_tobj = nullptr;
-    s->d_tmat = nullptr; s->d_obj_flags = nullptr;
+    s->d_nodes = nullptr; s->d_nodes4 = nullptr; s->d_nodes8 = nullptr; s->d_tris = nullptr; s->d_tprim = nullptr;
+    s->d_tobj = nullptr; s->d_tmat = nullptr; s->d_obj_flags = nullptr;
     s->dev_bytes = 0;
     int rc;
     if ((rc = dev_upload(&s->d_nodes, h.nodes.data(), h.nodes.size(), &s->dev_bytes))) return rc;
     if (width == 4 && (rc = dev_upload(&s->d_nodes4, h.nodes4.data(), h.nodes4.size(), &s->dev_bytes))) return rc;
+    if (width == 8 && (rc = dev_upload(&s->d_nodes8, h.nodes8.data(), h.nodes8.size(), &s->dev_bytes))) return rc;
     if ((rc = dev_upload(&s->d_tris, h.tris.data(), h.tris.size(), &s->dev_bytes))) return rc;
     if ((rc = dev_upload(&s->d_tprim, h.tprim.data(), h.tprim.size(), &s->dev_bytes))) return rc;
     if ((rc = dev_upload(&s->d_tobj, h.tobj.data(), h.tobj.size(), &s->dev_bytes))) return rc;
@@ -137,11 +144,13 @@ extern "C" int yg_build(yg_scene* s, const yg_build_opts* opts)
         if ((rc = dev_upload(&s->d_tobj, &zi, 1))) return rc;
         if ((rc = dev_upload(&s->d_tmat, &zi, 1))) return rc;
     }
-    s->n_nodes = (width == 4) ? h.nodes4.size() : h.nodes.size();
+    s->n_nodes = (width == 8) ? h.nodes8.size() : (width == 4) ? h.nodes4.size() : h.nodes.size();
     s->max_depth = h.max_depth;
     s->width = width;
     s->stack_entries = entries;
     s->stack_lds = stack_lds < 0 ? 0 : stack_lds;
+    s->rays_per_lane = rays_lane;
+    s->compact_below = compact_below;
     s->built = true;
     s->gen++;
     return YG_OK;

@@ -431,7 +431,7 @@ extern "C" int yg_photon_build(yg_scene* s, uint32_t li, const yg_render_opts* o
         YG_HIP(hipMemsetAsync(bflag, 0, sizeof(uint32_t) * np, st));
         const float* h0 = (d == 1) ? hdim[0] : hdim[2 * (d - 1)];
         const float* h1 = (d == 1) ? hdim[1] : hdim[2 * (d - 1) + 1];
-        YG_LAUNCH_W(s->width, (k_photon_seg<4>), (k_photon_seg<2>), dim3(grid_for(nactive, PB)), dim3(PB),
+        YG_LAUNCH_W(s->width, (k_photon_seg<8>), (k_photon_seg<4>), (k_photon_seg<2>), dim3(grid_for(nactive, PB)), dim3(PB),
                     stack_lds_bytes(s->stack_entries, PB), st, src, s->view(), s->d_mats, s->d_mods, s->d_texs,
                     s->d_obj_flags, opts->min_raydis, d, act, nactive, att, h0, h1, bidx, pst, slots, mvalid, bflag,
                     nrays);

@@ -385,7 +385,10 @@ __device__ __forceinline__ void queue_push(bool want, uint32_t p, uint32_t* qcou
 #ifndef YG_BOUNCE_WPE
 #define YG_BOUNCE_WPE 1
 #endif
-template <bool ADJ, int W>  // ADJ: replay for the adjoint (contri/nseg of the forward are kept)
+// ADJ: replay for the adjoint (contri/nseg of the forward are kept); RPL: rays
+// each lane traces, one after another (a compile-time count: a runtime loop
+// costs the one-ray kernel 20 registers)
+template <bool ADJ, int W, int RPL>
 __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WPE))) void k_ext(
     const RenderScene* __restrict__ scenes, const Area* __restrict__ areas, int nareas, int j,
     const uint32_t* __restrict__ active, const uint32_t* __restrict__ nact, uint32_t nactive_cap, int S, int D,
@@ -395,19 +398,26 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WP
     c3* __restrict__ contri, uint8_t* __restrict__ nseg, f3* __restrict__ h_p, f3* __restrict__ h_ray,
     int32_t* __restrict__ h_ti, uint32_t* __restrict__ qcount, uint32_t* __restrict__ queue,
     uint32_t* __restrict__ flag, const uint32_t* __restrict__ hbase, Counters* __restrict__ cnt,
-    int lds_cap, int* __restrict__ stk_ovf)
+    int lds_cap, int* __restrict__ stk_ovf, const uint32_t* __restrict__ dense)
 {
     extern __shared__ int s_stk[];
-    const uint32_t idx = blockIdx.x * RB + threadIdx.x;
+    const uint32_t tid = blockIdx.x * RB + threadIdx.x;
+    LaneStack st;
+    st.init(s_stk, threadIdx.x, RB);
+    st.spill_to(lds_cap, stk_ovf + tid, gridDim.x * RB);
+    // *dense: every listed path is alive (k_path_init, or k_compact compacted
+    // the list); else the list is the previous level's and only paths whose
+    // flag k_shade set continue
+    const bool all_alive = *dense != 0;
+    // each lane traces RPL rays of the block's consecutive slice
+    for (int r = 0; r < RPL; r++) {
+    const uint32_t idx = ((uint32_t)blockIdx.x * (uint32_t)RPL + (uint32_t)r) * RB + threadIdx.x;
     uint32_t next = 0;
     bool hit = false;
     uint32_t p = 0;
-    // *nact = this level's active paths (written on the device by k_path_init
+    // *nact = this level's listed paths (written on the device by k_path_init
     // or k_compact); the grid covers every path
-    if (idx < *nact) {
-        LaneStack st;
-        st.init(s_stk, threadIdx.x, RB);
-        st.spill_to(lds_cap, stk_ovf + idx, gridDim.x * RB);
+    if (idx < *nact && (all_alive || flag[active[idx]])) {
         p = active[idx];
         flag[p] = 0;  // set again by k_shade if the path continues
         const uint32_t item = p / (uint32_t)S;
@@ -469,9 +479,10 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WP
     }
     queue_push(hit, p, qcount, queue);
     if (!ADJ) block_add(&cnt->ext, next);
+    }
 }
 
-template <bool ADJ, int W>
+template <bool ADJ, int W, int RPL>
 __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_SHADOW_WPE))) void k_shadow(
                                                const RenderScene* __restrict__ scenes,
                                                const Area* __restrict__ areas, int nareas, int S,
@@ -482,12 +493,14 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_SHADOW_WP
                                                int lds_cap, int* __restrict__ stk_ovf)
 {
     extern __shared__ int s_stk[];
-    const uint32_t idx = blockIdx.x * RB + threadIdx.x;
+    const uint32_t tid = blockIdx.x * RB + threadIdx.x;
+    LaneStack st;
+    st.init(s_stk, threadIdx.x, RB);
+    st.spill_to(lds_cap, stk_ovf + tid, gridDim.x * RB);
+    for (int r = 0; r < RPL; r++) {
+    const uint32_t idx = ((uint32_t)blockIdx.x * (uint32_t)RPL + (uint32_t)r) * RB + threadIdx.x;
     uint32_t nshadow = 0;
     if (idx < *qcount) {
-        LaneStack st;
-        st.init(s_stk, threadIdx.x, RB);
-        st.spill_to(lds_cap, stk_ovf + idx, gridDim.x * RB);
         const uint32_t p = queue[idx];
         const Area& a = areas[find_area(areas, nareas, p / (uint32_t)S)];
         const RenderScene& R = scenes[a.scene];
@@ -505,6 +518,7 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_SHADOW_WP
     if (!ADJ) {
         block_add(&cnt->shadow, nshadow);
         block_add(&cnt->bshadow, nshadow);
+    }
     }
 }
 
@@ -664,18 +678,30 @@ __global__ __launch_bounds__(SB) void k_shade(const RenderScene* __restrict__ sc
 
 // Scatter flagged paths into the next active list and give each its Halton
 // call index within its render area (hidx_level: npaths entries of this level).
+// The next level's Halton indices (every surviving path's rank in its area)
+// and its path list: the survivors compacted when fewer than `compact_below`
+// of this level's listed paths survive, else this level's list again (its
+// ended paths' lanes exit at once; *dense_next tells the level which).
 __global__ void k_compact(const Area* __restrict__ areas, int nareas, uint32_t npaths, int S,
                           const uint32_t* __restrict__ flag, const uint32_t* __restrict__ ex,
+                          const uint32_t* __restrict__ active_cur, const uint32_t* __restrict__ nact_cur,
                           uint32_t* __restrict__ active, uint32_t* __restrict__ hidx_level,
-                          uint32_t* __restrict__ nact_next, const uint32_t* __restrict__ hbase_level, int D)
+                          uint32_t* __restrict__ nact_next, uint32_t* __restrict__ dense_next, float compact_below,
+                          const uint32_t* __restrict__ hbase_level, int D)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == npaths - 1) *nact_next = ex[p] + flag[p];
+    const uint32_t listed = *nact_cur, alive = ex[npaths - 1] + flag[npaths - 1];
+    const bool compact = (float)alive < compact_below * (float)listed;
+    if (p == npaths - 1) {
+        *nact_next = compact ? alive : listed;
+        *dense_next = compact ? 1u : 0u;
+    }
+    if (!compact && p < listed) active[p] = active_cur[p];
     if (p >= npaths || !flag[p]) return;
     const uint32_t item = p / (uint32_t)S;
     const int ai = find_area(areas, nareas, item);
     const uint32_t e = ex[p];
-    active[e] = p;
+    if (compact) active[e] = p;
     // the path's rank among the area's paths at this level (plus, with
     // antialiasing, the level's draws in the area's earlier passes)
     hidx_level[p] = e - ex[(size_t)areas[ai].item_start * S] + (hbase_level ? hbase_level[(size_t)ai * D] : 0u);
@@ -807,11 +833,13 @@ __global__ void k_area_draws(const Area* __restrict__ areas, int nareas, uint32_
 __global__ void k_path_init(uint32_t nitems, int S, int D, const uint32_t* __restrict__ qual,
                             const uint32_t* __restrict__ qoff, uint32_t* __restrict__ active,
                             uint32_t* __restrict__ flag, uint32_t* __restrict__ nact,
-                            uint32_t* __restrict__ qcnt)
+                            uint32_t* __restrict__ qcnt, uint32_t* __restrict__ dense)
 {
     const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
-    if (item == 0)
+    if (item == 0) {
         for (int j = 0; j < D; j++) qcnt[j] = 0;
+        dense[0] = 1;
+    }
     if (item >= nitems) return;
     if (item == nitems - 1) nact[0] = (qoff[item] + qual[item]) * (uint32_t)S;
     for (int i = 0; i < S; i++) flag[(size_t)item * S + i] = 0;
@@ -1153,7 +1181,10 @@ struct RenderWS {
     DevBuf<int32_t> h_ti;
     DevBuf<uint32_t> vis, queue;  // shadow visibility bits; queue of hit paths
     DevBuf<uint32_t> qcnt;        // queue length per level
-    DevBuf<uint32_t> nact;        // active paths per level
+    DevBuf<uint32_t> nact;        // listed paths per level
+    DevBuf<uint32_t> dense;       // per level: 1 = every listed path is alive
+    int rays_lane = 1;            // rays each lane of k_ext / k_shadow traces
+    float compact_below = 1.0f;   // survivor fraction below which a level is compacted
     DevBuf<Counters> cnt;
     DevBuf<uint8_t> scan_tmp;
     DevBuf<uint8_t> akey0, akey1;
@@ -1409,11 +1440,11 @@ static int trace_levels(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
     const uint32_t npaths = W.npaths, nitems = W.nitems;
     int rc;
     hipLaunchKernelGGL(k_path_init, dim3(grid_for(nitems, 256)), dim3(256), 0, st, nitems, S, D, W.qual.p,
-                       W.qoff.p, W.active0.p, W.flag.p, W.nact.p, W.qcnt.p);
+                       W.qoff.p, W.active0.p, W.flag.p, W.nact.p, W.qcnt.p, W.dense.p);
     YG_HIP(hipGetLastError());
     uint32_t* act = W.active0.p;
     uint32_t* nxt = W.active1.p;
-    const unsigned grid = grid_for(npaths, RB);
+    const unsigned grid = grid_for(npaths, RB * (ADJ ? 1 : W.rays_lane));
     for (int j = 0; j < D; j++) {
         hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
         if (ktiming) {
@@ -1424,16 +1455,31 @@ static int trace_levels(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
             YG_HIP(hipEventRecord(ev[0], st));
         }
         uint32_t* qc = W.qcnt.p + j;
-        YG_LAUNCH_WS(W.width, W.nocast, (k_ext<ADJ, 4>), (k_ext<ADJ, 2>), (k_ext<ADJ, 12>), (k_ext<ADJ, 10>), dim3(grid), dim3(RB), W.stk_bytes_lvl, st, W.scenes.p,
-                    W.areas.p, na, j, act, W.nact.p + j, npaths, S, D, W.item_ti.p, W.item_t.p, W.qoff.p,
-                    W.hidx.p, W.st_where.p, W.st_hn.p, W.st_ti.p, W.st_col.p, W.st_s.p, W.contri.p, W.nseg.p,
-                    W.h_p.p, W.h_ray.p, W.h_ti.p, qc, W.queue.p, W.flag.p, W.hbase_on ? W.hbase.p : nullptr,
-                    W.cnt.p, W.lds_cap, W.stk_ovf.p);
+        const int rl = ADJ ? 1 : W.rays_lane;
+#define YG_EXT(RL)                                                                                          \
+    YG_LAUNCH_WS(W.width, W.nocast, (k_ext<ADJ, 8, RL>), (k_ext<ADJ, 4, RL>), (k_ext<ADJ, 2, RL>),           \
+                 (k_ext<ADJ, 24, RL>), (k_ext<ADJ, 20, RL>), (k_ext<ADJ, 18, RL>), dim3(grid), dim3(RB),      \
+                 W.stk_bytes_lvl, st, W.scenes.p, W.areas.p, na, j, act, W.nact.p + j, npaths, S, D,        \
+                 W.item_ti.p, W.item_t.p, W.qoff.p, W.hidx.p, W.st_where.p, W.st_hn.p, W.st_ti.p, W.st_col.p, \
+                 W.st_s.p, W.contri.p, W.nseg.p, W.h_p.p, W.h_ray.p, W.h_ti.p, qc, W.queue.p, W.flag.p,       \
+                 W.hbase_on ? W.hbase.p : nullptr, W.cnt.p, W.lds_cap, W.stk_ovf.p, W.dense.p + j)
+#define YG_SHADOW(RL)                                                                                       \
+    YG_LAUNCH_WS(W.width, W.nocast, (k_shadow<ADJ, 8, RL>), (k_shadow<ADJ, 4, RL>), (k_shadow<ADJ, 2, RL>),  \
+                 (k_shadow<ADJ, 24, RL>), (k_shadow<ADJ, 20, RL>), (k_shadow<ADJ, 18, RL>), dim3(grid),      \
+                 dim3(RB), W.stk_bytes_lvl, st, W.scenes.p, W.areas.p, na, S, qc, W.queue.p, W.h_p.p,        \
+                 W.h_ti.p, W.vis.p, W.cnt.p, W.lds_cap, W.stk_ovf.p)
+        if constexpr (ADJ) YG_EXT(1);
+        else if (rl == 4) YG_EXT(4);
+        else if (rl == 2) YG_EXT(2);
+        else YG_EXT(1);
         YG_HIP(hipGetLastError());
         if (ktiming) YG_HIP(hipEventRecord(ev[1], st));
-        YG_LAUNCH_WS(W.width, W.nocast, (k_shadow<ADJ, 4>), (k_shadow<ADJ, 2>), (k_shadow<ADJ, 12>), (k_shadow<ADJ, 10>), dim3(grid), dim3(RB), W.stk_bytes_lvl, st,
-                    W.scenes.p, W.areas.p, na, S, qc, W.queue.p, W.h_p.p, W.h_ti.p, W.vis.p, W.cnt.p,
-                    W.lds_cap, W.stk_ovf.p);
+        if constexpr (ADJ) YG_SHADOW(1);
+        else if (rl == 4) YG_SHADOW(4);
+        else if (rl == 2) YG_SHADOW(2);
+        else YG_SHADOW(1);
+#undef YG_EXT
+#undef YG_SHADOW
         YG_HIP(hipGetLastError());
         if (ktiming) YG_HIP(hipEventRecord(ev[2], st));
         const unsigned sgrid = ADJ ? std::min(grid_for(npaths, SB), ADJ_GRID) : grid_for(npaths, SB);
@@ -1449,7 +1495,8 @@ static int trace_levels(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
         if (j + 1 >= D) break;
         if ((rc = exclusive_scan(W.flag.p, W.ex.p, npaths, st))) return rc;
         hipLaunchKernelGGL(k_compact, dim3(grid_for(npaths, 256)), dim3(256), 0, st, W.areas.p, na, npaths, S,
-                           W.flag.p, W.ex.p, nxt, W.hidx.p + (size_t)(j + 1) * npaths, W.nact.p + j + 1,
+                           W.flag.p, W.ex.p, act, W.nact.p + j, nxt, W.hidx.p + (size_t)(j + 1) * npaths,
+                           W.nact.p + j + 1, W.dense.p + j + 1, W.compact_below,
                            W.hbase_on ? W.hbase.p + j + 1 : nullptr, D);
         YG_HIP(hipGetLastError());
         if (W.hbase_on) {
@@ -1502,7 +1549,7 @@ static int ensure_frame_buffers(uint32_t nitems, uint32_t npaths, int nl_max, bo
             (rc = W.contri.ensure((size_t)npaths * D)) || (rc = W.nseg.ensure(npaths)) ||
             (rc = W.h_p.ensure(npaths)) || (rc = W.h_ray.ensure(npaths)) || (rc = W.h_ti.ensure(npaths)) ||
             (rc = W.vis.ensure(npaths)) || (rc = W.queue.ensure(npaths)) || (rc = W.qcnt.ensure(D)) ||
-            (rc = W.nact.ensure(D)))
+            (rc = W.nact.ensure(D)) || (rc = W.dense.ensure(D)))
             return rc;
     }
     // the frame launches keep W.stk_lds entries per lane in LDS and the rest
@@ -1544,7 +1591,7 @@ static int launch_frame(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
     const uint32_t nitems = W.nitems;
     const int na = W.na, nl_max = W.nl_max;
     int rc;
-    YG_LAUNCH_W(W.width, (k_primary<4>), (k_primary<2>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes_lvl, st,
+    YG_LAUNCH_W(W.width, (k_primary<8>), (k_primary<4>), (k_primary<2>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes_lvl, st,
                 W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.cnt.p, W.aa_pass < 0 ? 1 : 0,
                 W.lds_cap, W.stk_ovf.p);
     YG_HIP(hipGetLastError());
@@ -1564,8 +1611,7 @@ static int launch_frame(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
                            W.vx_draws.p, W.vx_off.p, W.draws.p, W.pathd.p, W.tdraws.p);
         YG_HIP(hipGetLastError());
         if ((rc = exclusive_scan(W.tdraws.p, W.lcg_off.p, nitems, st))) return rc;
-        YG_LAUNCH_WS(W.width, W.nocast, (k_direct<4, true>), (k_direct<2, true>), (k_direct<12, true>),
-                     (k_direct<10, true>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p,
+        YG_LAUNCH_WS(W.width, W.nocast, (k_direct<8, true>), (k_direct<4, true>), (k_direct<2, true>), (k_direct<24, true>), (k_direct<20, true>), (k_direct<18, true>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p,
                      W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lcg_off.p, W.lightc.p, nl_max, W.cnt.p, 0,
                      W.stk_ovf.p, (const uint32_t*)nullptr, (const uint32_t*)W.pathd.p);
         YG_HIP(hipGetLastError());
@@ -1576,8 +1622,7 @@ static int launch_frame(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
                            W.vx_lit.p, W.vx_nlit.p);
         YG_HIP(hipGetLastError());
         const VertexRec vx{W.vx_p.p, W.vx_ray.p, W.vx_ti.p, W.vx_vis.p, W.vx_rs.p, W.vx_draws.p};
-        YG_LAUNCH_WS(W.width, W.nocast, (k_vertex_light<4>), (k_vertex_light<2>), (k_vertex_light<12>),
-                     (k_vertex_light<10>), dim3(grid_for(nv, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p,
+        YG_LAUNCH_WS(W.width, W.nocast, (k_vertex_light<8>), (k_vertex_light<4>), (k_vertex_light<2>), (k_vertex_light<24>), (k_vertex_light<20>), (k_vertex_light<18>), dim3(grid_for(nv, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p,
                      na, W.vx_lit.p, W.vx_nlit.p, W.S, W.D, vx, W.vx_off.p, W.lcg_off.p, W.pre.p, W.contri.p,
                      W.cnt.p, 0, W.stk_ovf.p);
         YG_HIP(hipGetLastError());
@@ -1597,15 +1642,15 @@ static int launch_frame(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
     if (W.any_area || nl_max > 32) {
         // registers hold this kernel to 2 waves per SIMD, so LDS has room for
         // the whole stack
-        YG_LAUNCH_WS(W.width, W.nocast, (k_direct<4, true>), (k_direct<2, true>), (k_direct<12, true>), (k_direct<10, true>), dim3(grid_for(nitems, RB)), dim3(RB),
+        YG_LAUNCH_WS(W.width, W.nocast, (k_direct<8, true>), (k_direct<4, true>), (k_direct<2, true>), (k_direct<24, true>), (k_direct<20, true>), (k_direct<18, true>), dim3(grid_for(nitems, RB)), dim3(RB),
                     W.stk_bytes, st, W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lcg_off.p,
                     W.lightc.p, nl_max, W.cnt.p, 0, W.stk_ovf.p, (const uint32_t*)nullptr);
     } else {
-        YG_LAUNCH_WS(W.width, W.nocast, (k_sun_vis<4>), (k_sun_vis<2>), (k_sun_vis<12>), (k_sun_vis<10>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes_lvl,
+        YG_LAUNCH_WS(W.width, W.nocast, (k_sun_vis<8>), (k_sun_vis<4>), (k_sun_vis<2>), (k_sun_vis<24>), (k_sun_vis<20>), (k_sun_vis<18>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes_lvl,
                     st, W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.sun_vis.p, W.cnt.p,
                     W.lds_cap, W.stk_ovf.p);
         YG_HIP(hipGetLastError());
-        YG_LAUNCH_W(W.width, (k_direct<4, false>), (k_direct<2, false>), dim3(grid_for(nitems, RB)), dim3(RB), 0,
+        YG_LAUNCH_W(W.width, (k_direct<8, false>), (k_direct<4, false>), (k_direct<2, false>), dim3(grid_for(nitems, RB)), dim3(RB), 0,
                     st, W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lcg_off.p, W.lightc.p,
                     nl_max, W.cnt.p, 0, W.stk_ovf.p, (const uint32_t*)W.sun_vis.p);
     }
@@ -1765,6 +1810,9 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     W.stk_entries = max_depth;
     W.stk_lds = 0;
     for (auto& P : plans) W.stk_lds = std::max(W.stk_lds, P.s->stack_lds);
+    // launch settings of the GI levels: the first scene's (a batch shares them)
+    W.rays_lane = plans[0].s->rays_per_lane;
+    W.compact_below = plans[0].s->compact_below;
     W.nocast = false;
     for (auto& P : plans)
         for (int32_t f : P.s->obj_flags)
@@ -1840,7 +1888,10 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     // shape (scenes, areas, lights and output pointers are read from the
     // uploaded descriptors, not baked into the launches).
     const std::array<uint64_t, 13> key = {g_alloc_serial, nitems, npaths, (uint64_t)S, (uint64_t)D,
-                                          (uint64_t)W.na, (uint64_t)nl_max, (uint64_t)W.width, W.stk_bytes,
+                                          (uint64_t)W.na, (uint64_t)nl_max,
+                                          (uint64_t)W.width | ((uint64_t)W.rays_lane << 8) |
+                                              ((uint64_t)(uint32_t)__builtin_bit_cast(uint32_t, W.compact_below) << 16),
+                                          W.stk_bytes,
                                           W.scan_tmp.cap, any_area, any_photon,
                                           (uint64_t)any_path | ((uint64_t)W.nocast << 1) |
                                               ((uint64_t)W.area_path << 2)};
@@ -2363,7 +2414,7 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
     if (!fused) {
         if ((rc = W.w_item.ensure(W.nitems)) || (rc = W.adeg.ensure(1))) return rc;
         YG_HIP(hipMemsetAsync(W.adeg.p, 0, sizeof(unsigned long long), st));
-        YG_LAUNCH_WS(W.width, W.nocast, (k_adjoint_direct<4>), (k_adjoint_direct<2>), (k_adjoint_direct<12>), (k_adjoint_direct<10>), dim3(std::min(grid_for(W.nitems, RB), 4 * ADJ_GRID)), dim3(RB), W.stk_bytes, st, W.scenes.p,
+        YG_LAUNCH_WS(W.width, W.nocast, (k_adjoint_direct<8>), (k_adjoint_direct<4>), (k_adjoint_direct<2>), (k_adjoint_direct<24>), (k_adjoint_direct<20>), (k_adjoint_direct<18>), dim3(std::min(grid_for(W.nitems, RB), 4 * ADJ_GRID)), dim3(RB), W.stk_bytes, st, W.scenes.p,
                            W.areas.p, W.na, W.nitems, W.item_ti.p, W.item_t.p, d_grad_rgb, G, 1, W.S, W.D,
                            W.qual.p, W.contri.p, W.nseg.p, W.w_item.p);
         YG_HIP(hipGetLastError());
@@ -2380,14 +2431,14 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
         }
     }
     if (fused) {
-        YG_LAUNCH_WS(W.width, W.nocast, (k_adjoint_direct<4>), (k_adjoint_direct<2>), (k_adjoint_direct<12>), (k_adjoint_direct<10>), dim3(std::min(grid_for(W.nitems, RB), 4 * ADJ_GRID)), dim3(RB), W.stk_bytes, st, W.scenes.p,
+        YG_LAUNCH_WS(W.width, W.nocast, (k_adjoint_direct<8>), (k_adjoint_direct<4>), (k_adjoint_direct<2>), (k_adjoint_direct<24>), (k_adjoint_direct<20>), (k_adjoint_direct<18>), dim3(std::min(grid_for(W.nitems, RB), 4 * ADJ_GRID)), dim3(RB), W.stk_bytes, st, W.scenes.p,
                            W.areas.p, W.na, W.nitems, W.item_ti.p, W.item_t.p, d_grad_rgb, G, 0, W.S, W.D,
                            W.qual.p, W.contri.p, W.nseg.p, (c3*)nullptr);
         YG_HIP(hipGetLastError());
     }
     if (fused && W.any_path) {
         hipLaunchKernelGGL(k_path_init, dim3(grid_for(W.nitems, 256)), dim3(256), 0, st, W.nitems, W.S, W.D,
-                           W.qual.p, W.qoff.p, W.active0.p, W.flag.p, W.nact.p, W.qcnt.p);
+                           W.qual.p, W.qoff.p, W.active0.p, W.flag.p, W.nact.p, W.qcnt.p, W.dense.p);
         YG_HIP(hipGetLastError());
         if (n0 > 0) {
             // Replay order: longest paths first (stable, so pixel order is kept
@@ -2403,7 +2454,7 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
             if ((rc = W.scan_tmp.ensure(bytes))) return rc;
             YG_HIP(hipcub::DeviceRadixSort::SortPairsDescending(W.scan_tmp.p, bytes, W.akey0.p, W.akey1.p,
                                                                 W.active0.p, W.active1.p, (int)n0, 0, 4, st));
-            YG_LAUNCH_WS(W.width, W.nocast, (k_adjoint_path<4>), (k_adjoint_path<2>), (k_adjoint_path<12>), (k_adjoint_path<10>), dim3(grid_for(n0, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, W.na,
+            YG_LAUNCH_WS(W.width, W.nocast, (k_adjoint_path<8>), (k_adjoint_path<4>), (k_adjoint_path<2>), (k_adjoint_path<24>), (k_adjoint_path<20>), (k_adjoint_path<18>), dim3(grid_for(n0, RB)), dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, W.na,
                                W.active1.p, n0, W.npaths, W.S, W.D, W.item_ti.p, W.item_t.p, W.qoff.p, W.hidx.p,
                                d_grad_rgb, G);
             YG_HIP(hipGetLastError());

@@ -102,7 +102,7 @@ static AnyShape any_shape(const yg_scene* s, uint32_t n, size_t max_ovf_ints)
 static int launch_trace_any(yg_scene* s, const yg_ray* d_rays, uint32_t n, uint8_t* d_occ, const AnyShape& a,
                             int* ovf, hipStream_t st)
 {
-    YG_LAUNCH_W(s->width, (k_trace_any<4>), (k_trace_any<2>), dim3(a.grid), dim3(TB), stack_lds_bytes(a.lcap, TB),
+    YG_LAUNCH_W(s->width, (k_trace_any<8>), (k_trace_any<4>), (k_trace_any<2>), dim3(a.grid), dim3(TB), stack_lds_bytes(a.lcap, TB),
                 st, s->view(), d_rays, n, d_occ, a.lcap, a.ovf_entries > 0 ? ovf : nullptr);
     YG_HIP(hipGetLastError());
     return YG_OK;
@@ -201,7 +201,7 @@ extern "C" int yg_trace_closest(yg_scene* s, const yg_ray* d_rays, uint32_t n, y
     if (rc) return rc;
     if (n == 0) return YG_OK;
     if (!d_rays || !d_hits) return yg_fail(YG_ERR_ARG, "null buffer");
-    YG_LAUNCH_W(s->width, (k_trace_closest<false, 4>), (k_trace_closest<false, 2>), dim3(grid_for(n, TB)), dim3(TB),
+    YG_LAUNCH_W(s->width, (k_trace_closest<false, 8>), (k_trace_closest<false, 4>), (k_trace_closest<false, 2>), dim3(grid_for(n, TB)), dim3(TB),
                 stack_lds_bytes(s->stack_entries, TB), (hipStream_t)stream, s->view(), d_rays, n, d_hits, nullptr);
     YG_HIP(hipGetLastError());
     return YG_OK;
@@ -215,7 +215,7 @@ extern "C" int yg_trace_closest_counted(yg_scene* s, const yg_ray* d_rays, uint3
     int rc = check_scene(s);
     if (rc) return rc;
     if (n == 0) return YG_OK;
-    YG_LAUNCH_W(s->width, (k_trace_closest<true, 4>), (k_trace_closest<true, 2>), dim3(grid_for(n, TB)), dim3(TB),
+    YG_LAUNCH_W(s->width, (k_trace_closest<true, 8>), (k_trace_closest<true, 4>), (k_trace_closest<true, 2>), dim3(grid_for(n, TB)), dim3(TB),
                 stack_lds_bytes(s->stack_entries, TB), (hipStream_t)stream, s->view(), d_rays, n, d_hits, d_counters);
     YG_HIP(hipGetLastError());
     return YG_OK;
@@ -296,7 +296,7 @@ extern "C" int yg_first_hit(yg_scene* s, const float* d_from, const float* d_dir
     if (rc) return rc;
     if (n == 0) return YG_OK;
     if (!d_from || !d_dir || !d_hits) return yg_fail(YG_ERR_ARG, "null buffer");
-    YG_LAUNCH_W(s->width, (k_first_hit<4>), (k_first_hit<2>), dim3(grid_for(n, TB)), dim3(TB),
+    YG_LAUNCH_W(s->width, (k_first_hit<8>), (k_first_hit<4>), (k_first_hit<2>), dim3(grid_for(n, TB)), dim3(TB),
                 stack_lds_bytes(s->stack_entries, TB), (hipStream_t)stream, s->view(), d_from, d_dir, n, min_raydis,
                 d_hits);
     YG_HIP(hipGetLastError());
@@ -311,8 +311,7 @@ extern "C" int yg_is_shadowed(yg_scene* s, const float* d_p, const float* d_l,
     if (rc) return rc;
     if (n == 0) return YG_OK;
     if (!d_p || !d_l || !d_out) return yg_fail(YG_ERR_ARG, "null buffer");
-    YG_LAUNCH_WS(s->width, s->view().nocast != 0, (k_is_shadowed<4>), (k_is_shadowed<2>), (k_is_shadowed<12>),
-                 (k_is_shadowed<10>), dim3(grid_for(n, TB)), dim3(TB),
+    YG_LAUNCH_WS(s->width, s->view().nocast != 0, (k_is_shadowed<8>), (k_is_shadowed<4>), (k_is_shadowed<2>), (k_is_shadowed<24>), (k_is_shadowed<20>), (k_is_shadowed<18>), dim3(grid_for(n, TB)), dim3(TB),
                 stack_lds_bytes(s->stack_entries, TB), (hipStream_t)stream, s->view(), d_p, d_l, d_self_obj, n, to_point,
                 min_raydis, self_bias, d_out);
     YG_HIP(hipGetLastError());

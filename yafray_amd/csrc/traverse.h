@@ -71,6 +71,22 @@ __device__ __forceinline__ GNode4 gload(const GNode4* p)
     return n;
 }
 
+__device__ __forceinline__ GNode8 gload(const GNode8* p)
+{
+    const YG_GLOBAL yg_v4f* q = (const YG_GLOBAL yg_v4f*)p;
+    GNode8 n;
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+        const yg_v4f a = q[2 * r], b = q[2 * r + 1];
+        n.box[r][0] = a.x; n.box[r][1] = a.y; n.box[r][2] = a.z; n.box[r][3] = a.w;
+        n.box[r][4] = b.x; n.box[r][5] = b.y; n.box[r][6] = b.z; n.box[r][7] = b.w;
+    }
+    const yg_v4i c0 = ((const YG_GLOBAL yg_v4i*)p)[12], c1 = ((const YG_GLOBAL yg_v4i*)p)[13];
+    n.child[0] = c0.x; n.child[1] = c0.y; n.child[2] = c0.z; n.child[3] = c0.w;
+    n.child[4] = c1.x; n.child[5] = c1.y; n.child[6] = c1.z; n.child[7] = c1.w;
+    return n;
+}
+
 __device__ __forceinline__ GNode gload(const GNode* p)
 {
     const YG_GLOBAL yg_v4f* q = (const YG_GLOBAL yg_v4f*)p;
@@ -198,6 +214,21 @@ __device__ __forceinline__ void node_test4(const GNode4& n, const RayBox& r, flo
     }
 }
 
+// Slab test of the eight children of a width-8 node (as node_test4).
+__device__ __forceinline__ void node_test8(const GNode8& n, const RayBox& r, float lo, float hi, float t[8])
+{
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const float a0 = __builtin_fmaf(n.box[0][k], r.ix, -r.ox), a1 = __builtin_fmaf(n.box[1][k], r.ix, -r.ox);
+        const float a2 = __builtin_fmaf(n.box[2][k], r.iy, -r.oy), a3 = __builtin_fmaf(n.box[3][k], r.iy, -r.oy);
+        const float a4 = __builtin_fmaf(n.box[4][k], r.iz, -r.oz), a5 = __builtin_fmaf(n.box[5][k], r.iz, -r.oz);
+        const float nr = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), lo));
+        const float fr = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), hi));
+        const bool h = __builtin_fmaf(-fabsf(nr), YG_TEPS, nr) <= __builtin_fmaf(fabsf(fr), YG_TEPS, fr);
+        t[k] = h ? nr : INFINITY;
+    }
+}
+
 #define YG_POP 0x7fffffff  // "no child to descend into": pop the stack
 
 __device__ __forceinline__ void cswap(float& ta, int& ca, float& tb, int& cb)
@@ -250,7 +281,33 @@ __device__ __forceinline__ int visit4(const BVHView& bvh, int node, const RayBox
     return c0;
 }
 
-// Kernels are instantiated per BVH width W (2 or 4); a launch traverses
+// Eight children: sorted nearest first by a 19-comparator network (depth 6),
+// the nearest returned, the other hit children pushed far-to-near.
+__device__ __forceinline__ int visit8(const BVHView& bvh, int node, const RayBox& rb, float lo, float hi,
+                                      LaneStack& st)
+{
+    const GNode8 n = gload(bvh.nodes8 + node);
+    float t[8];
+    node_test8(n, rb, lo, hi, t);
+    int c[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) c[k] = n.child[k];
+#define YG_CS(i, j) cswap(t[i], c[i], t[j], c[j])
+    YG_CS(0, 2); YG_CS(1, 3); YG_CS(4, 6); YG_CS(5, 7);
+    YG_CS(0, 4); YG_CS(1, 5); YG_CS(2, 6); YG_CS(3, 7);
+    YG_CS(0, 1); YG_CS(2, 3); YG_CS(4, 5); YG_CS(6, 7);
+    YG_CS(2, 4); YG_CS(3, 5);
+    YG_CS(1, 4); YG_CS(3, 6);
+    YG_CS(1, 2); YG_CS(3, 4); YG_CS(5, 6);
+#undef YG_CS
+    if (t[0] == INFINITY) return YG_POP;
+#pragma unroll
+    for (int k = 7; k >= 1; k--)
+        if (t[k] != INFINITY) st.push(c[k]);
+    return c[0];
+}
+
+// Kernels are instantiated per BVH width W (2, 4 or 8); a launch traverses
 // trees of one width only.  Keeping the two node formats in separate kernels
 // keeps each traversal loop small (one binary with both loops ran a third
 // slower on either).
@@ -258,7 +315,8 @@ template <int W>
 __device__ __forceinline__ int visit(const BVHView& bvh, int node, const RayBox& rb, float lo, float hi,
                                      LaneStack& st)
 {
-    if constexpr ((W & 7) == 4) return visit4(bvh, node, rb, lo, hi, st);
+    if constexpr ((W & 15) == 8) return visit8(bvh, node, rb, lo, hi, st);
+    else if constexpr ((W & 15) == 4) return visit4(bvh, node, rb, lo, hi, st);
     else return visit2(bvh, node, rb, lo, hi, st);
 }
 
@@ -266,11 +324,11 @@ struct TraceStats {
     uint32_t nodes, tris;
 };
 
-// The traversal kernels' W template parameter is the BVH width (2 or 4),
-// plus 8 for launches with objects that cast no shadows: only those
+// The traversal kernels' W template parameter is the BVH width (2, 4 or 8),
+// plus 16 for launches with objects that cast no shadows: only those
 // instantiations test object flags in their shadow queries, so the leaf loops
 // of every other launch stay as they are.
-#define YG_W_NOCAST 8
+#define YG_W_NOCAST 16
 template <int W>
 constexpr bool yg_nocast_w() { return (W & YG_W_NOCAST) != 0; }
 

@@ -31,6 +31,15 @@ struct alignas(128) GNode4 {
     int4 pad;
 };
 
+// Eight-wide BVH node, 256 bytes (two L2 lines): six box rows (lo.x hi.x
+// lo.y hi.y lo.z hi.z) of eight children each, then the eight child
+// references encoded as for GNode; unused slots have +inf boxes.
+struct alignas(128) GNode8 {
+    float box[6][8];
+    int32_t child[8];
+    int32_t pad[8];
+};
+
 #define YG_LEAF_MAX 16
 #ifndef YG_DEFAULT_WIDTH
 #define YG_DEFAULT_WIDTH 4
@@ -60,6 +69,7 @@ static inline size_t stack_lds_bytes(int entries, int lanes)
 struct BVHView {
     const GNode* nodes;     // width 2
     const GNode4* nodes4;   // width 4
+    const GNode8* nodes8;   // width 8
     int32_t width;
     const float4* tris;     // 3 per triangle
     const int32_t* tprim;   // original primitive id per leaf-ordered triangle
@@ -75,6 +85,8 @@ struct HostBVH {
     std::vector<GNode> nodes;
     std::vector<GNode4> nodes4;  // the same tree collapsed to width 4 (when asked)
     int32_t stack4 = 0;          // deepest traversal stack the width-4 tree can need
+    std::vector<GNode8> nodes8;  // ... or to width 8
+    int32_t stack8 = 0;
     std::vector<float4> tris;
     std::vector<int32_t> tprim, tobj, tmat;
     int32_t max_depth = 0;
@@ -113,9 +125,12 @@ struct yg_scene {
     int32_t width = 2;
     int32_t stack_entries = 1;  // traversal stack entries per lane the tree can need
     int32_t stack_lds = YG_DEFAULT_STACK_LDS;  // of those, kept in LDS by render launches (0: all)
+    int32_t rays_per_lane = 1;   // GI level launches: rays each lane traces
+    float compact_below = 1.0f;  // GI levels: compact the survivors below this fraction
     uint64_t n_nodes = 0;
     GNode* d_nodes = nullptr;
     GNode4* d_nodes4 = nullptr;
+    GNode8* d_nodes8 = nullptr;
     float4* d_tris = nullptr;
     int32_t* d_tprim = nullptr;
     int32_t* d_tobj = nullptr;
@@ -153,7 +168,7 @@ struct yg_scene {
     BVHView view() const
     {
         BVHView v;
-        v.nodes = d_nodes; v.nodes4 = d_nodes4; v.width = width; v.tris = d_tris; v.tprim = d_tprim; v.tobj = d_tobj; v.tmat = d_tmat;
+        v.nodes = d_nodes; v.nodes4 = d_nodes4; v.nodes8 = d_nodes8; v.width = width; v.tris = d_tris; v.tprim = d_tprim; v.tobj = d_tobj; v.tmat = d_tmat;
         v.ntris = (int32_t)tri_mat.size();
         v.nobj = nobj;
         v.oflags = d_obj_flags;
@@ -191,16 +206,17 @@ static inline int dev_upload(T** dst, const T* src, size_t n, uint64_t* acct = n
 static inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
 
 // Launches the instantiation of a traversal kernel for the BVH width in use
-// (K4 / K2: the kernel template-ids for widths 4 and 2, parenthesised).
-#define YG_LAUNCH_W(width, K4, K2, grid, block, shm, stream, ...)                  \
-    do {                                                                         \
-        if ((width) == 4) hipLaunchKernelGGL(K4, grid, block, shm, stream, __VA_ARGS__); \
-        else hipLaunchKernelGGL(K2, grid, block, shm, stream, __VA_ARGS__);       \
-    } while (0)
-// The same for kernels with shadow queries: K12 / K10 are the width-4 / width-2
-// instantiations that test object flags (W + 8), used when nocast is set.
-#define YG_LAUNCH_WS(width, nocast, K4, K2, K12, K10, grid, block, shm, stream, ...)            \
+// (K8 / K4 / K2: the kernel template-ids for widths 8, 4 and 2, parenthesised).
+#define YG_LAUNCH_W(width, K8, K4, K2, grid, block, shm, stream, ...)                            \
     do {                                                                                        \
-        if (nocast) YG_LAUNCH_W(width, K12, K10, grid, block, shm, stream, __VA_ARGS__);        \
-        else YG_LAUNCH_W(width, K4, K2, grid, block, shm, stream, __VA_ARGS__);                 \
+        if ((width) == 8) hipLaunchKernelGGL(K8, grid, block, shm, stream, __VA_ARGS__);        \
+        else if ((width) == 4) hipLaunchKernelGGL(K4, grid, block, shm, stream, __VA_ARGS__);   \
+        else hipLaunchKernelGGL(K2, grid, block, shm, stream, __VA_ARGS__);                     \
+    } while (0)
+// The same for kernels with shadow queries: K24 / K20 / K18 are the width-8 /
+// 4 / 2 instantiations that test object flags (W + 16), used when nocast is set.
+#define YG_LAUNCH_WS(width, nocast, K8, K4, K2, K24, K20, K18, grid, block, shm, stream, ...)     \
+    do {                                                                                        \
+        if (nocast) YG_LAUNCH_W(width, K24, K20, K18, grid, block, shm, stream, __VA_ARGS__);   \
+        else YG_LAUNCH_W(width, K8, K4, K2, grid, block, shm, stream, __VA_ARGS__);             \
     } while (0)

@@ -78,7 +78,8 @@ class yg_render_stats(C.Structure):
 
 class yg_build_opts(C.Structure):
     _fields_ = [("max_leaf", C.c_int32), ("sah_bins", C.c_int32), ("threads", C.c_int32),
-                ("width", C.c_int32), ("stack_lds", C.c_int32)]
+                ("width", C.c_int32), ("stack_lds", C.c_int32), ("rays_per_lane", C.c_int32),
+                ("compact_below", C.c_float)]
 
 
 YG_TUNE_MAX = 32
@@ -86,7 +87,8 @@ YG_TUNE_MAX = 32
 
 class yg_tune_opts(C.Structure):
     _fields_ = [("widths", C.c_int32), ("leaves", C.c_int32), ("bins", C.c_int32),
-                ("repeats", C.c_int32), ("threads", C.c_int32), ("stacks", C.c_int32)]
+                ("repeats", C.c_int32), ("threads", C.c_int32), ("stacks", C.c_int32),
+                ("rays", C.c_int32), ("compacts", C.c_int32)]
 
 
 class yg_tune_result(C.Structure):

@@ -126,10 +126,13 @@ class Scene:
                                 C.byref(oid)), "yg_mesh_add")
         return oid.value
 
-    def build(self, max_leaf=4, bins=32, threads=0, width=0, stack_lds=0):
+    def build(self, max_leaf=4, bins=32, threads=0, width=0, stack_lds=0, rays_per_lane=0, compact_below=0.0):
         """stack_lds: stack entries per lane the render launches keep in LDS
-        (0 = library default, -1 = the whole stack)."""
-        o = D.yg_build_opts(max_leaf, bins, threads, width, stack_lds)
+        (0 = library default, -1 = the whole stack); rays_per_lane: rays each
+        lane of the GI level launches traces (0 = 1); compact_below: survivor
+        fraction below which a GI level's paths are compacted (0 = 1, always;
+        -1 = never)."""
+        o = D.yg_build_opts(max_leaf, bins, threads, width, stack_lds, rays_per_lane, compact_below)
         check(lib().yg_build(self.h, C.byref(o)), "yg_build")
 
     def info(self):
@@ -210,13 +213,14 @@ class Scene:
               "yg_render")
         return st
 
-    def autotune(self, out_rgb, opts=None, widths=(2, 4), leaves=(2, 4, 8), bins=(32,), repeats=2,
-                 threads=0, stacks=()):
+    def autotune(self, out_rgb, opts=None, widths=(2, 4, 8), leaves=(2, 4, 8), bins=(32,), repeats=2,
+                 threads=0, stacks=(), rays=(1,), compacts=(1.0,)):
         """Rebuild with every (width, max_leaf, bins) candidate and try every
         LDS stack part in `stacks` (entries, -1 = whole stack; () = the
-        default), time the frame of `opts` into out_rgb, keep the fastest.
-        Returns (best yg_build_opts, best ms,
-        [(width, leaf, bins, ms, build_ms, stack_lds), ...])."""
+        default), rays per lane in `rays` and compaction threshold in
+        `compacts` (-1 = never), time the frame of `opts` into out_rgb, keep
+        the fastest.  Returns (best yg_build_opts, best ms,
+        [(width, leaf, bins, ms, build_ms, stack_lds, rays_per_lane, compact_below), ...])."""
         o = opts if opts is not None else self.desc.opts
         t = D.yg_tune_opts()
         t.widths = sum(1 << w for w in widths)
@@ -233,11 +237,24 @@ class Scene:
                 t.stacks |= 1 << (v.bit_length() - 1)
             else:
                 raise ValueError("stack_lds candidates: -1, 8, 16, 32, 64")
+        for v in rays:
+            if v not in (1, 2, 4):
+                raise ValueError("rays_per_lane candidates: 1, 2, 4")
+            t.rays |= 1 << (v.bit_length() - 1)
+        for v in compacts:
+            if v == -1:
+                t.compacts |= 1
+            elif v in (0.25, 0.5, 0.75, 1.0):
+                t.compacts |= 1 << int(round(v * 4))
+            else:
+                raise ValueError("compact_below candidates: -1, 0.25, 0.5, 0.75, 1")
+        if tuple(compacts) == (1.0,):
+            t.compacts = 0
         r = D.yg_tune_result()
         check(lib().yg_autotune(self.h, C.byref(o), _ptr(out_rgb), C.byref(t), C.byref(r), _stream()),
               "yg_autotune")
         cands = [(r.cand[k].width, r.cand[k].max_leaf, r.cand[k].sah_bins, r.cand_ms[k], r.cand_build_ms[k],
-                  r.cand[k].stack_lds) for k in range(r.ncand)]
+                  r.cand[k].stack_lds, r.cand[k].rays_per_lane, r.cand[k].compact_below) for k in range(r.ncand)]
         return r.best, r.best_ms, cands
 
     def render_adjoint(self, grad_rgb, out_rgb, gmat, gmod, glight, opts=None):
