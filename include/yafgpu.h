@@ -253,6 +253,15 @@ int yg_scene_destroy(yg_scene* s);
 int yg_mesh_add(yg_scene* s, const float* xyz, uint32_t nv, const uint32_t* idx,
                 uint32_t nt, const int32_t* mat, int32_t flags, int32_t* obj_id);
 
+/* meshObject_t::autoSmooth (mesh.cc:199-282), called by the scene loader for a
+ * mesh with autosmooth="angle" (render.cc:764): vertex normals for object
+ * obj_id from the face normals of its triangles that share the vertex and
+ * deviate by less than `angle` degrees (at 180 or more: all of them; below 1:
+ * nothing).  Hits on the object then get the interpolated, normalised
+ * shading normal of triangle_t::getSurface (triangle.cc:142-160) and the
+ * tangent frame around it.  Call after yg_mesh_add, before yg_build. */
+int yg_mesh_autosmooth(yg_scene* s, int32_t obj_id, float angle);
+
 /* kdTree_t::kdTree_t (kdtree.cc:57-138) + buildObjectTree (object3d.cc:30-53):
  * builds the SAH BVH over all meshes and uploads it. */
 int yg_build(yg_scene* s, const yg_build_opts* opts);

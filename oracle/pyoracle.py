@@ -44,6 +44,9 @@ def _load(path):
         L.yo_log_begin.argtypes = [vp, C.c_int64, vp, C.c_int64]
         L.yo_log_end.argtypes = [C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.yo_scene_free.argtypes = [vp]
+        L.yo_mesh_autosmooth.restype = C.c_int
+        L.yo_mesh_autosmooth.argtypes = [vp, C.c_int, C.c_float]
+        L.yo_shading_normals.argtypes = [vp, I32P, F32P, C.c_int64, F32P]
         L.yo_scene_nodes.restype = C.c_int64
         L.yo_scene_nodes.argtypes = [vp]
         L.yo_trace_brute.argtypes = [vp, F32P, C.c_int64, C.c_int, I32P, F32P, F32P, F32P]
@@ -123,6 +126,14 @@ def _ptr(a):
 class OracleScene:
     """Triangle soup with per-triangle object and material ids."""
 
+    @classmethod
+    def from_desc(cls, desc, fast=False):
+        """The scene of a SceneDesc, its autosmooth angles applied."""
+        oc = cls(desc.verts, desc.idx, desc.tri_obj, desc.tri_mat, fast=fast)
+        for o, a in getattr(desc, "smooth", {}).items():
+            oc.autosmooth(o, a)
+        return oc
+
     def __init__(self, verts, idx, tri_obj=None, tri_mat=None, fast=False):
         self.fast = fast
         L = lib(fast)
@@ -132,6 +143,18 @@ class OracleScene:
         self.tri_mat = None if tri_mat is None else np.ascontiguousarray(tri_mat, dtype=np.int32)
         self.h = L.yo_scene_new(self.verts.reshape(-1), len(self.verts), self.idx.reshape(-1),
                                 len(self.idx), _ptr(self.tri_obj), _ptr(self.tri_mat))
+
+    def autosmooth(self, obj, angle):
+        """meshObject_t::autoSmooth of object obj (vertex normals)."""
+        lib(self.fast).yo_mesh_autosmooth(self.h, int(obj), float(angle))
+
+    def shading_normals(self, prim, h):
+        """getSurface's shading normals at hit points h [n, 3] of triangles prim."""
+        prim = np.ascontiguousarray(prim, np.int32)
+        h = np.ascontiguousarray(h, np.float32).reshape(-1)
+        out = np.zeros(len(prim) * 3, np.float32)
+        lib(self.fast).yo_shading_normals(self.h, prim, h, len(prim), out)
+        return out.reshape(-1, 3)
 
     def close(self):
         if self.h:

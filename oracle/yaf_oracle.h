@@ -37,6 +37,10 @@ typedef struct yo_scene yo_scene;
 yo_scene* yo_scene_new(const float* verts, int64_t nv, const uint32_t* idx, int64_t nt,
                        const int32_t* tri_obj, const int32_t* tri_mat);
 void yo_scene_free(yo_scene* s);
+/* meshObject_t::autoSmooth (mesh.cc:199-282) of object obj's triangles */
+int yo_mesh_autosmooth(yo_scene* s, int obj, float angle);
+/* getSurface's shading normal of n hits (prim, hit point) */
+void yo_shading_normals(yo_scene* s, const int32_t* prim, const float* h, int64_t n, float* out);
 /* canonical BVH2 node count (binned SAH, 32 bins, leaf <= 4, FP64 boxes) */
 int64_t yo_scene_nodes(yo_scene* s);
 

@@ -263,9 +263,104 @@ yo_scene* yo_scene_new(const float* verts, int64_t nv, const uint32_t* idx, int6
     return s;
 }
 
+/* meshObject_t::autoSmooth (mesh.cc:199-282) for the triangles of object
+ * obj: no smoothing below 1 degree; at 180 or more one normal per vertex, the
+ * face normals of its triangles summed in triangle order and normalised;
+ * otherwise per triangle corner the normalised sum, in triangle order, of the
+ * face normals of the vertex's triangles whose dot with this triangle's face
+ * normal exceeds cos(angle). */
+int yo_mesh_autosmooth(yo_scene* s, int obj, float angle)
+{
+    if (angle < 1) return 0;
+    if (!s->tri_vn) {
+        s->tri_vn = (float*)calloc((size_t)9 * (size_t)(s->nt ? s->nt : 1), sizeof(float));
+        s->tri_smooth = (uint8_t*)calloc((size_t)(s->nt ? s->nt : 1), 1);
+    }
+    int64_t* vcount = (int64_t*)calloc((size_t)(s->nv ? s->nv : 1), sizeof(int64_t));
+    for (int64_t t = 0; t < s->nt; t++)
+        if (s->tri_obj[t] == obj)
+            for (int k = 0; k < 3; k++) vcount[s->idx[3 * t + k]]++;
+    if (angle >= 180) {
+        yo_v3* vn = (yo_v3*)calloc((size_t)(s->nv ? s->nv : 1), sizeof(yo_v3));
+        for (int64_t t = 0; t < s->nt; t++) {
+            if (s->tri_obj[t] != obj) continue;
+            yo_v3 n = v3_ld(s->tri_n + 3 * t);
+            for (int k = 0; k < 3; k++) {
+                yo_v3* a = &vn[s->idx[3 * t + k]];
+                *a = v3_add(*a, n);
+            }
+        }
+        for (int64_t v = 0; v < s->nv; v++)
+            if (vcount[v]) vn[v] = v3_norm(vn[v]);
+        for (int64_t t = 0; t < s->nt; t++) {
+            if (s->tri_obj[t] != obj) continue;
+            for (int k = 0; k < 3; k++) {
+                yo_v3 n = vn[s->idx[3 * t + k]];
+                s->tri_vn[9 * t + 3 * k] = n.x; s->tri_vn[9 * t + 3 * k + 1] = n.y; s->tri_vn[9 * t + 3 * k + 2] = n.z;
+            }
+            s->tri_smooth[t] = 1;
+        }
+        free(vn);
+    } else {
+        float cosa = (float)cos((double)angle * M_PI / 180.0);
+        /* the triangles of each vertex, in triangle order */
+        int64_t* start = (int64_t*)calloc((size_t)s->nv + 1, sizeof(int64_t));
+        for (int64_t v = 0; v < s->nv; v++) start[v + 1] = start[v] + vcount[v];
+        int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)(s->nv ? s->nv : 1));
+        memcpy(fill, start, sizeof(int64_t) * (size_t)s->nv);
+        int64_t* tris = (int64_t*)malloc(sizeof(int64_t) * (size_t)(start[s->nv] ? start[s->nv] : 1));
+        for (int64_t t = 0; t < s->nt; t++)
+            if (s->tri_obj[t] == obj)
+                for (int k = 0; k < 3; k++) tris[fill[s->idx[3 * t + k]]++] = t;
+        for (int64_t t = 0; t < s->nt; t++) {
+            if (s->tri_obj[t] != obj) continue;
+            yo_v3 N = v3_ld(s->tri_n + 3 * t);
+            for (int k = 0; k < 3; k++) {
+                uint32_t v = s->idx[3 * t + k];
+                yo_v3 sn = v3(0, 0, 0);
+                for (int64_t q = start[v]; q < start[v + 1]; q++) {
+                    yo_v3 o = v3_ld(s->tri_n + 3 * tris[q]);
+                    if (v3_dot(o, N) > cosa) sn = v3_add(sn, o);
+                }
+                sn = v3_norm(sn);
+                s->tri_vn[9 * t + 3 * k] = sn.x; s->tri_vn[9 * t + 3 * k + 1] = sn.y; s->tri_vn[9 * t + 3 * k + 2] = sn.z;
+            }
+            s->tri_smooth[t] = 1;
+        }
+        free(start); free(fill); free(tris);
+    }
+    free(vcount);
+    return 0;
+}
+
+/* triangle_t::getSurface's shading normal (triangle.cc:142-160) of prim at
+ * hit point h: the face normal, or the vertex normals interpolated with
+ * getInterpolation's weights and normalised. */
+yo_v3 yo_shading_normal(const yo_scene* s, int32_t prim, yo_v3 h)
+{
+    yo_v3 ng = v3_ld(s->tri_n + 3 * (int64_t)prim);
+    if (!s->tri_smooth || !s->tri_smooth[prim]) return ng;
+    float fb, fc;
+    tri_bary_v(ng, tri_vert(s, prim, 0), tri_vert(s, prim, 1), tri_vert(s, prim, 2), h, &fb, &fc);
+    float fa = 1.f - fb - fc;
+    const float* n = s->tri_vn + 9 * (int64_t)prim;
+    yo_v3 nn = v3_add(v3_add(v3_mul(v3(n[0], n[1], n[2]), fa), v3_mul(v3(n[3], n[4], n[5]), fb)),
+                      v3_mul(v3(n[6], n[7], n[8]), fc));
+    return v3_norm(nn);
+}
+
+void yo_shading_normals(yo_scene* s, const int32_t* prim, const float* h, int64_t n, float* out)
+{
+    for (int64_t i = 0; i < n; i++) {
+        yo_v3 r = yo_shading_normal(s, prim[i], v3_ld(h + 3 * i));
+        out[3 * i] = r.x; out[3 * i + 1] = r.y; out[3 * i + 2] = r.z;
+    }
+}
+
 void yo_scene_free(yo_scene* s)
 {
     if (!s) return;
+    free(s->tri_vn); free(s->tri_smooth);
     free(s->verts); free(s->idx); free(s->tri_obj); free(s->tri_mat); free(s->tri_n);
     free(s->nodes); free(s->order);
     free(s->pm.data); free(s->pm.gstart); free(s->pm.gitems);

@@ -257,6 +257,31 @@ static void make_sp(const fctx* F, int32_t prim, d3 f, d3 ray, double Z, dsp* sp
     sp->P = dadd(f, dmul(ray, Z));
     sp->Ng = dldd(F->tn + 3 * (int64_t)prim);
     sp->N = sp->Ng;
+    const yo_scene* s = F->s;
+    if (s->tri_smooth && s->tri_smooth[prim]) {
+        /* getSurface's interpolated shading normal (triangle.cc:142-160) in double:
+         * barycentric weights of the hit point (dominant-axis projection) */
+        d3 a = {0, 0, 0}, b = a, c = a;
+        for (int k = 0; k < 3; k++) {
+            (&a.x)[k] = s->verts[3 * (int64_t)s->idx[3 * (int64_t)prim] + k];
+            (&b.x)[k] = s->verts[3 * (int64_t)s->idx[3 * (int64_t)prim + 1] + k];
+            (&c.x)[k] = s->verts[3 * (int64_t)s->idx[3 * (int64_t)prim + 2] + k];
+        }
+        const d3 n = sp->Ng, h = sp->P;
+        const double nx = fabs(n.x), ny = fabs(n.y), nz = fabs(n.z);
+        int u = 0, v = 1;
+        if (nx >= ny && nx >= nz) { u = 1; v = 2; }
+        else if (ny >= nx && ny >= nz) { u = 0; v = 2; }
+        const double uu0 = (&h.x)[u] - (&a.x)[u], vv0 = (&h.x)[v] - (&a.x)[v];
+        const double uu1 = (&b.x)[u] - (&a.x)[u], vv1 = (&b.x)[v] - (&a.x)[v];
+        const double uu2 = (&c.x)[u] - (&a.x)[u], vv2 = (&c.x)[v] - (&a.x)[v];
+        const double den = uu1 * vv2 - vv1 * uu2;
+        const double fb = (uu0 * vv2 - vv0 * uu2) / den, fc = (uu1 * vv0 - vv1 * uu0) / den, fa = 1 - fb - fc;
+        const float* vn = s->tri_vn + 9 * (int64_t)prim;
+        d3 nn = {vn[0] * fa + vn[3] * fb + vn[6] * fc, vn[1] * fa + vn[4] * fb + vn[7] * fc,
+                 vn[2] * fa + vn[5] * fb + vn[8] * fc};
+        sp->N = dnorm(nn);
+    }
     dcreate_cs(sp->N, &sp->NU, &sp->NV);
     sp->obj = F->s->tri_obj[prim];
     sp->mat = F->s->tri_mat[prim];

@@ -147,6 +147,8 @@ struct yo_scene {
     int64_t nnodes;
     int32_t* order;
     yo_photonmap pm;
+    float* tri_vn;         /* autoSmooth vertex normals, 9 per triangle (NULL: none) */
+    uint8_t* tri_smooth;   /* per triangle: has vertex normals */
     /* object flags of the render in progress (bit0 cast shadows), or NULL */
     const int32_t* oflags;
     int64_t noflags;

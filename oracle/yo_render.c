@@ -9,6 +9,7 @@
 #include "yo_internal.h"
 
 yo_col yo_modulate(const yg_modulator* m, const yg_texture* texs, yo_col C, yo_v3 P);
+yo_v3 yo_shading_normal(const yo_scene* s, int32_t prim, yo_v3 h);
 int yo_first_hit_one(const yo_scene* s, yo_v3 from, yo_v3 ray, float min_raydis, int32_t* prim,
                      float* t);
 int yo_is_shadowed_one(const yo_scene* s, yo_v3 p, yo_v3 l, int self_obj, int to_point,
@@ -74,15 +75,16 @@ static int obj_flag(const rctx* R, int obj, int bit)
     return (R->obj_flags[obj] >> bit) & 1;
 }
 
-/* meshObject_t::shoot + triangle_t::getSurface fast path (mesh.cc:517-577,
- * triangle.cc:138-141): P = from + Z*ray, N = Ng = face normal, NU/NV from createCS. */
+/* meshObject_t::shoot + triangle_t::getSurface (mesh.cc:517-577,
+ * triangle.cc:138-160): P = from + Z*ray, Ng = face normal, N = Ng or the
+ * interpolated vertex normal of an autoSmooth'ed mesh, NU/NV from createCS. */
 static void make_sp(const yo_scene* s, int32_t prim, yo_v3 f, yo_v3 ray, float Z, yo_sp* sp)
 {
     sp->prim = prim;
     sp->Z = Z;
     sp->P = v3_add(f, v3_mul(ray, Z));
     sp->Ng = v3_ld(s->tri_n + 3 * (int64_t)prim);
-    sp->N = sp->Ng;
+    sp->N = yo_shading_normal(s, prim, sp->P);
     create_cs(sp->N, &sp->NU, &sp->NV);
     sp->obj = s->tri_obj[prim];
     sp->mat = s->tri_mat[prim];

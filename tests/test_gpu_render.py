@@ -196,6 +196,57 @@ def test_level_launch_settings_keep_the_image(torch_cuda, rays, compact):
                                                                        st0.rays_shadow)
 
 
+@pytest.mark.parametrize("width", [2, 4, 8])
+def test_smoothed_sphere_room_matches_oracle(torch_cuda, width):
+    """The room's icosphere autoSmooth'ed at 180 degrees (every corner
+    smoothed) and at 8 (corners split where adjacent faces meet at more than
+    8 degrees, about half of them at subdivision 3): shading
+    normals interpolated at primary hits, at path vertices (their tangent
+    frames) and in the area light's and sun's N.L; GPU = oracle."""
+    torch = torch_cuda
+    d = S.direct_room(96, 72, subdiv=3, samples=4)
+    lights = list(d.lights) + [D.pathlight(samples=4, depth=3, use_qmc=True, threshold=1e3),
+                               D.sunlight((0.2, -0.3, 1.0), power=0.6)]
+    for angle in (180.0, 8.0):
+        ds = D.SceneDesc(d.verts, d.idx, d.tri_obj, d.tri_mat, d.obj_flags, d.cam, list(d.mats), [], [], lights,
+                         d.opts, smooth={3: angle})
+        img, st, _ = gpu_render(ds, torch, build_kw={"width": width})
+        ref, rs = po.OracleScene.from_desc(ds).render(ds)
+        assert np.all(np.isfinite(img))
+        assert rel_rmse(img, ref) <= 1e-3
+        assert (st.rays_camera, st.rays_extension, st.rays_shadow) == tuple(int(x) for x in rs[:3])
+        assert st.lcg_draws == rs[4]
+    # smoothing changes the image (the pins above are not vacuous)
+    dflat = D.SceneDesc(d.verts, d.idx, d.tri_obj, d.tri_mat, d.obj_flags, d.cam, list(d.mats), [], [], lights,
+                        d.opts)
+    flat, _, _ = gpu_render(dflat, torch)
+    assert rel_rmse(img, flat) > 1e-3
+
+
+def test_smoothed_spheres_photon_map_matches_oracle(torch_cuda):
+    """Photon bounces leave smoothed spheres around their interpolated
+    normals' tangent frames and test fromRadiosity's side with them."""
+    torch = torch_cuda
+    Y = _Y()
+    d = S.photon_field(n_side=6, subdiv=2, extent=20.0, resx=96, resy=72, photons=2 * 200 * 200, search=50, nmat=4)
+    d.smooth = {o: 180.0 for o in range(len(d.obj_flags))}
+    sc = Y.Scene.from_desc(d)
+    ps = sc.photon_build(0)
+    oc = po.OracleScene.from_desc(d)
+    os_ = oc.photon_build(d, 0)
+    assert (ps.emitted, ps.stored, ps.clustered) == tuple(int(x) for x in os_[:3])
+    buf = torch.zeros(ps.clustered * 9, dtype=torch.float32, device="cuda")
+    assert sc.photon_read(buf) == ps.clustered
+    g, c = buf.cpu().numpy().reshape(-1, 9), oc.photon_read()
+    assert np.mean(np.all(g == c, axis=1)) > 0.99
+    assert np.allclose(g, c, rtol=1e-5, atol=1e-6)
+    out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+    sc.render(out)
+    torch.cuda.synchronize()
+    ref, _ = oc.render(d)
+    assert rel_rmse(out.cpu().numpy().reshape(ref.shape), ref) <= 1e-3
+
+
 @pytest.mark.parametrize("search", [50, 100])
 def test_photon_k50_merged_clusters_match_oracle(torch_cuda, search):
     """The photon workload's settings at small scale: K = 50 (and 100, past

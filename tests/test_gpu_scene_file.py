@@ -68,3 +68,16 @@ def test_tonemapped_antialiased_render_matches_oracle(torch_cuda):
     rel = np.sqrt(np.mean((img.astype(np.float64) - ref) ** 2)) / np.sqrt(np.mean(ref.astype(np.float64) ** 2))
     assert rel <= 1e-3
     assert np.mean(np.all(img == ref, axis=2)) >= 0.999
+
+
+def test_cli_autosmooth_equals_api_render(torch_cuda, tmp_path):
+    """A mesh with autosmooth="45": the scene file's render equals the API's
+    with the same object smoothed (yg_mesh_autosmooth)."""
+    d = terrain(exposure=0.7)
+    d.smooth = {0: 45.0}
+    cli(tmp_path, scene_xml(d, "out.hdr", output_type="hdr"))
+    _, img = read_hdr(str(tmp_path / "out.hdr"))
+    smooth = gpu_render(d, torch_cuda)
+    assert np.array_equal(img, rgbe(smooth))
+    d.smooth = {}
+    assert not np.array_equal(img, rgbe(gpu_render(d, torch_cuda)))

@@ -53,7 +53,7 @@ def scene_xml(d, outfile, output_type="tga", render_attrs=None):
         out.append('<object name="O%04d" shader_name="M%03d" shadow="%s" recv_rad="%s" emit_rad="%s">'
                    % (o, int(d.tri_mat[sel[0]]) if len(sel) else 0, _onoff(flags & 1), _onoff(flags & 2),
                       _onoff(flags & 4)))
-        out.append('<attributes></attributes><mesh autosmooth="0"><points>')
+        out.append('<attributes></attributes><mesh autosmooth="%s"><points>' % _g(d.smooth.get(o, 0)))
         out.extend('<p x="%s" y="%s" z="%s"/>' % tuple(_g(c) for c in d.verts[v]) for v in used)
         out.append("</points><faces>")
         out.extend('<f a="%d" b="%d" c="%d" shader_name="M%03d"/>'

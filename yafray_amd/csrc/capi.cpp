@@ -1,6 +1,7 @@
 // C ABI of libyafgpu: errors, scene lifecycle, geometry ingestion, BVH build,
 // shading descriptors, QMC tables.
 #include <atomic>
+#include <cmath>
 #include <chrono>
 #include <cstring>
 #include <mutex>
@@ -48,13 +49,14 @@ extern "C" int yg_scene_create(yg_scene** out)
 
 static void free_dev(yg_scene* s)
 {
-    void* ptrs[] = {s->d_nodes, s->d_nodes4, s->d_nodes8, s->d_tris, s->d_tprim, s->d_tobj, s->d_tmat, s->d_obj_flags,
+    void* ptrs[] = {s->d_nodes, s->d_nodes4, s->d_nodes8, s->d_tvn, s->d_tris, s->d_tprim, s->d_tobj, s->d_tmat, s->d_obj_flags,
                     s->d_mats, s->d_mods, s->d_texs, s->pm.d_photons, s->pm.d_gstart,
                     s->pm.d_gitems, s->d_halton, s->d_any_ovf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    s->d_nodes = nullptr; s->d_nodes4 = nullptr; s->d_nodes8 = nullptr; s->d_tris = nullptr; s->d_tprim = nullptr;
-    s->d_tobj = nullptr; s->d_tmat = nullptr; s->d_obj_flags = nullptr; s->d_mats = nullptr; s->d_mods = nullptr;
+    s->d_nodes = nullptr; s->d_nodes4 = nullptr; s->d_nodes8 = nullptr; s->d_tvn = nullptr; s->d_tris = nullptr;
+    s->d_tprim = nullptr; s->d_tobj = nullptr; s->d_tmat = nullptr; s->d_obj_flags = nullptr; s->d_mats = nullptr;
+    s->d_mods = nullptr;
     s->d_texs = nullptr; s->pm = PhotonMapDev();
     s->d_halton = nullptr; s->halton_ndim = 0; s->halton_stride = 0;
     s->d_any_ovf = nullptr; s->any_ovf_ints = 0;
@@ -90,10 +92,81 @@ extern "C" int yg_mesh_add(yg_scene* s, const float* xyz, uint32_t nv, const uin
     if (mat) s->tri_mat.insert(s->tri_mat.end(), mat, mat + nt);
     else s->tri_mat.insert(s->tri_mat.end(), nt, 0);
     s->obj_flags.push_back(flags);
+    if (!s->tri_smooth.empty()) {
+        s->tri_smooth.resize(s->tri_mat.size(), 0);
+        s->tri_vn.resize(9 * s->tri_mat.size(), 0.0f);
+    }
     s->built = false;
     s->gen++;
     s->geo_gen++;
     if (obj_id) *obj_id = oid;
+    return YG_OK;
+}
+
+extern "C" int yg_mesh_autosmooth(yg_scene* s, int32_t obj_id, float angle)
+{
+    if (!s) return yg_fail(YG_ERR_ARG, "null scene");
+    if (obj_id < 0 || obj_id >= s->nobj) return yg_fail(YG_ERR_ARG, "object id out of range");
+    if (!(angle >= 1.0f)) return YG_OK;  // mesh.cc:204: no smoothing below 1 degree
+    const size_t nt = s->tri_mat.size();
+    // the object's triangles (contiguous in add order) and its vertex range
+    size_t t0 = 0;
+    while (t0 < nt && s->tri_obj[t0] != obj_id) t0++;
+    size_t t1 = t0;
+    while (t1 < nt && s->tri_obj[t1] == obj_id) t1++;
+    if (t0 == t1) return YG_OK;
+    if (s->tri_smooth.empty()) {
+        s->tri_smooth.assign(nt, 0);
+        s->tri_vn.assign(9 * nt, 0.0f);
+    }
+    uint32_t v0 = UINT32_MAX, v1 = 0;
+    for (size_t t = t0; t < t1; t++)
+        for (int k = 0; k < 3; k++) {
+            v0 = std::min(v0, s->idx[3 * t + k]);
+            v1 = std::max(v1, s->idx[3 * t + k] + 1);
+        }
+    auto vert = [&](uint32_t i) { return ld3(&s->verts[3 * (size_t)i]); };
+    // triangle_t::N(): the normal of triangle_t::recNormal (triangle.cc:45-49)
+    std::vector<f3> tn(t1 - t0);
+    for (size_t t = t0; t < t1; t++)
+        tn[t - t0] = tri_normal(vert(s->idx[3 * t]), vert(s->idx[3 * t + 1]), vert(s->idx[3 * t + 2]));
+    const size_t nv = v1 - v0;
+    if (angle >= 180.0f) {
+        // one normal per vertex: the face normals summed in triangle order, normalised
+        std::vector<f3> vn(nv, mk3(0, 0, 0));
+        for (size_t t = t0; t < t1; t++)
+            for (int k = 0; k < 3; k++) {
+                f3& n = vn[s->idx[3 * t + k] - v0];
+                n = n + tn[t - t0];
+            }
+        for (auto& n : vn) n = norm3(n);
+        for (size_t t = t0; t < t1; t++)
+            for (int k = 0; k < 3; k++) {
+                const f3 n = vn[s->idx[3 * t + k] - v0];
+                s->tri_vn[9 * t + 3 * k] = n.x; s->tri_vn[9 * t + 3 * k + 1] = n.y; s->tri_vn[9 * t + 3 * k + 2] = n.z;
+            }
+    } else {
+        // per corner: the face normals of the vertex's triangles (in triangle
+        // order) whose dot with this face's normal exceeds cos(angle)
+        const float cosa = (float)cos((double)angle * M_PI / 180.0);
+        std::vector<std::vector<uint32_t>> cnx(nv);
+        for (size_t t = t0; t < t1; t++)
+            for (int k = 0; k < 3; k++) cnx[s->idx[3 * t + k] - v0].push_back((uint32_t)(t - t0));
+        for (size_t t = t0; t < t1; t++) {
+            const f3 N = tn[t - t0];
+            for (int k = 0; k < 3; k++) {
+                f3 sn = mk3(0, 0, 0);
+                for (uint32_t o : cnx[s->idx[3 * t + k] - v0])
+                    if (dot(tn[o], N) > cosa) sn = sn + tn[o];
+                sn = norm3(sn);
+                s->tri_vn[9 * t + 3 * k] = sn.x; s->tri_vn[9 * t + 3 * k + 1] = sn.y; s->tri_vn[9 * t + 3 * k + 2] = sn.z;
+            }
+        }
+    }
+    for (size_t t = t0; t < t1; t++) s->tri_smooth[t] = 1;
+    s->built = false;
+    s->gen++;
+    s->geo_gen++;
     return YG_OK;
 }
 
@@ -120,11 +193,12 @@ extern "C" int yg_build(yg_scene* s, const yg_build_opts* opts)
     const int entries = (width == 8) ? h.stack8 + 1 : (width == 4) ? h.stack4 + 1 : h.max_depth + 1;
     if (entries > YG_MAX_STACK) return yg_fail(YG_ERR_STATE, "BVH deeper than the traversal stack");
     // the device copy replaces the previous one
-    void* old[] = {s->d_nodes, s->d_nodes4, s->d_nodes8, s->d_tris, s->d_tprim, s->d_tobj, s->d_tmat, s->d_obj_flags};
+    void* old[] = {s->d_nodes, s->d_nodes4, s->d_nodes8, s->d_tvn, s->d_tris, s->d_tprim, s->d_tobj, s->d_tmat,
+                   s->d_obj_flags};
     for (void* p : old)
         if (p) (void)hipFree(p);
-    s->d_nodes = nullptr; s->d_nodes4 = nullptr; s->d_nodes8 = nullptr; s->d_tris = nullptr; s->d_tprim = nullptr;
-    s->d_tobj = nullptr; s->d_tmat = nullptr; s->d_obj_flags = nullptr;
+    s->d_nodes = nullptr; s->d_nodes4 = nullptr; s->d_nodes8 = nullptr; s->d_tvn = nullptr; s->d_tris = nullptr;
+    s->d_tprim = nullptr; s->d_tobj = nullptr; s->d_tmat = nullptr; s->d_obj_flags = nullptr;
     s->dev_bytes = 0;
     int rc;
     if ((rc = dev_upload(&s->d_nodes, h.nodes.data(), h.nodes.size(), &s->dev_bytes))) return rc;
@@ -135,6 +209,18 @@ extern "C" int yg_build(yg_scene* s, const yg_build_opts* opts)
     if ((rc = dev_upload(&s->d_tobj, h.tobj.data(), h.tobj.size(), &s->dev_bytes))) return rc;
     if ((rc = dev_upload(&s->d_tmat, h.tmat.data(), h.tmat.size(), &s->dev_bytes))) return rc;
     if ((rc = dev_upload(&s->d_obj_flags, s->obj_flags.data(), s->obj_flags.size(), &s->dev_bytes))) return rc;
+    if (!s->tri_smooth.empty()) {
+        // vertex normals in leaf order, with a has-normals flag per triangle
+        std::vector<float4> tvn(3 * (size_t)nt);
+        for (int64_t i = 0; i < nt; i++) {
+            const int32_t p = h.tprim[i];
+            const float* n = &s->tri_vn[9 * (size_t)p];
+            tvn[3 * i] = make_float4(n[0], n[1], n[2], s->tri_smooth[p] ? 1.0f : 0.0f);
+            tvn[3 * i + 1] = make_float4(n[3], n[4], n[5], 0.0f);
+            tvn[3 * i + 2] = make_float4(n[6], n[7], n[8], 0.0f);
+        }
+        if ((rc = dev_upload(&s->d_tvn, tvn.data(), tvn.size(), &s->dev_bytes))) return rc;
+    }
     if (nt == 0) {
         // keep valid (dummy) triangle arrays so kernels never see null pointers
         float4 z[3] = {};

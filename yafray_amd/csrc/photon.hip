@@ -111,11 +111,14 @@ __global__ __launch_bounds__(PB) void k_photon_seg(PhotonSrc src, BVHView bvh, c
         f3 NU, NV;
         const float4 A = bvh.tris[3 * s.ti], B = bvh.tris[3 * s.ti + 1], C = bvh.tris[3 * s.ti + 2];
         const f3 nrm = mk3(A.w, B.w, C.w);
-        create_cs(nrm, NU, NV);
+        // sp.N, sp.NU, sp.NV of the previous hit (shading normal: interpolated on
+        // autoSmooth'ed meshes)
+        const f3 nsh = surf_normal(bvh, s.ti, s.pos, nrm);
+        create_cs(nsh, NU, NV);
         dir = random_vector_cone(s.ng2, NU, NV, 0.05f, h0[k], h1[k]);
         // fromRadiosity (basicshaders.cc:30-44) at the previous hit
         const f3 prevP = s.pos;
-        if (dot(face_forward(nrm, nrm, dir), s.edir) < 0) col = mkc(0, 0, 0);
+        if (dot(face_forward(nrm, nsh, dir), s.edir) < 0) col = mkc(0, 0, 0);
         else col = cmul(s.col, get_diffuse(mats, mods, texs, bvh.tmat[s.ti], prevP));
         pos = s.pos;
         lastpos = s.lastpos;

@@ -444,7 +444,8 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WP
             const int pti = st_ti[p];
             const float4 A = gload(R.bvh.tris + 3 * pti), B = gload(R.bvh.tris + 3 * pti + 1),
                          C = gload(R.bvh.tris + 3 * pti + 2);
-            create_cs(mk3(A.w, B.w, C.w), Ru, Rv);
+            // sp.NU, sp.NV of the previous vertex: around its shading normal
+            create_cs(surf_normal(R.bvh, pti, where, mk3(A.w, B.w, C.w)), Ru, Rv);
             raycolor = st_col[p];
             k = hidx[(size_t)j * nactive_cap + p];
         }
@@ -579,7 +580,7 @@ __global__ __launch_bounds__(SB) void k_shade(const RenderScene* __restrict__ sc
         sp.P = h_p[p];
         const float4 A = R.bvh.tris[3 * ti], B = R.bvh.tris[3 * ti + 1], C = R.bvh.tris[3 * ti + 2];
         sp.Ng = mk3(A.w, B.w, C.w);
-        sp.N = sp.Ng;
+        sp.N = surf_normal(R.bvh, ti, sp.P, sp.Ng);
         create_cs(sp.N, sp.NU, sp.NV);
         sp.obj = R.bvh.tobj[ti];
         sp.mat = R.bvh.tmat[ti];
@@ -781,7 +782,7 @@ __global__ __launch_bounds__(RB) void k_vertex_light(const RenderScene* __restri
         sp.P = vx.p[v];
         const float4 A = R.bvh.tris[3 * ti], B = R.bvh.tris[3 * ti + 1], C = R.bvh.tris[3 * ti + 2];
         sp.Ng = mk3(A.w, B.w, C.w);
-        sp.N = sp.Ng;
+        sp.N = surf_normal(R.bvh, ti, sp.P, sp.Ng);
         create_cs(sp.N, sp.NU, sp.NV);
         sp.obj = R.bvh.tobj[ti];
         sp.mat = R.bvh.tmat[ti];

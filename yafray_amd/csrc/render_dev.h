@@ -143,12 +143,31 @@ __device__ __forceinline__ float ri_lp(uint32_t i)
     return (float)((double)r * (1.0 / 4294967296.0));
 }
 
-// surfacePoint_t subset for a hit (mesh.cc:517-577, triangle.cc:138-141):
-// P = from + Z*ray; N = Ng = face normal; NU, NV = createCS(N).
+// surfacePoint_t subset for a hit (mesh.cc:517-577, triangle.cc:138-160):
+// P = from + Z*ray; Ng = face normal; N = Ng or the interpolated vertex
+// normal (surf_normal); NU, NV = createCS(N).
 struct Surf {
     f3 P, N, Ng, NU, NV;
     int obj, mat, ti;
 };
+
+// The shading normal of triangle_t::getSurface (triangle.cc:142-160) at the
+// hit point P of leaf-ordered triangle ti with face normal Ng: Ng itself, or
+// for a triangle of an autoSmooth'ed mesh the vertex normals interpolated
+// with getInterpolation's weights (fa = 1 - fb - fc) and normalised.
+__device__ __forceinline__ f3 surf_normal(const BVHView& bvh, int ti, f3 P, f3 Ng)
+{
+    if (!bvh.tvn) return Ng;
+    const float4 A = bvh.tvn[3 * ti];
+    if (A.w == 0.0f) return Ng;
+    const float4 B = bvh.tvn[3 * ti + 1], C = bvh.tvn[3 * ti + 2];
+    const float4 TA = bvh.tris[3 * ti], TB = bvh.tris[3 * ti + 1], TC = bvh.tris[3 * ti + 2];
+    float fb, fc;
+    tri_bary(Ng, mk3(TA.x, TA.y, TA.z), mk3(TB.x, TB.y, TB.z), mk3(TC.x, TC.y, TC.z), P, fb, fc);
+    const float fa = 1.0f - fb - fc;
+    const f3 nn = mk3(A.x, A.y, A.z) * fa + mk3(B.x, B.y, B.z) * fb + mk3(C.x, C.y, C.z) * fc;
+    return norm3(nn);
+}
 
 __device__ __forceinline__ Surf make_surf(const BVHView& bvh, int ti, f3 f, f3 ray, float Z)
 {
@@ -157,7 +176,7 @@ __device__ __forceinline__ Surf make_surf(const BVHView& bvh, int ti, f3 f, f3 r
     s.P = f + ray * Z;
     const float4 A = bvh.tris[3 * ti], B = bvh.tris[3 * ti + 1], C = bvh.tris[3 * ti + 2];
     s.Ng = mk3(A.w, B.w, C.w);
-    s.N = s.Ng;
+    s.N = surf_normal(bvh, ti, s.P, s.Ng);
     create_cs(s.N, s.NU, s.NV);
     s.obj = bvh.tobj[ti];
     s.mat = bvh.tmat[ti];

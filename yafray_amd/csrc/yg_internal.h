@@ -79,6 +79,7 @@ struct BVHView {
     int32_t nobj;           // objects in the scene (1: every triangle is object 0)
     const int32_t* oflags;  // per object: bit0 cast shadows, bit1 receive, bit2 use for radiosity
     int32_t nocast;         // some object casts no shadows (scene shadow queries skip it)
+    const float4* tvn;      // vertex normals per leaf-ordered triangle (3 float4), or null
 };
 
 struct HostBVH {
@@ -119,6 +120,11 @@ struct yg_scene {
     std::vector<uint32_t> idx;
     std::vector<int32_t> tri_obj, tri_mat, obj_flags;
     int32_t nobj = 0;
+    // autoSmooth vertex normals: 9 floats per triangle (corners a, b, c) in
+    // mesh-add order, and whether the triangle has them (empty: none has)
+    std::vector<float> tri_vn;
+    std::vector<uint8_t> tri_smooth;
+    float4* d_tvn = nullptr;  // leaf order: (na, 1), (nb, 0), (nc, 0); w of the first = has normals
 
     bool built = false;
     int32_t max_depth = 0;
@@ -172,6 +178,7 @@ struct yg_scene {
         v.ntris = (int32_t)tri_mat.size();
         v.nobj = nobj;
         v.oflags = d_obj_flags;
+        v.tvn = d_tvn;
         v.nocast = 0;
         for (int32_t f : obj_flags)
             if (!(f & 1)) v.nocast = 1;

@@ -282,7 +282,7 @@ class SceneDesc:
     """Geometry + shading + lights + camera + render options of one scene."""
 
     def __init__(self, verts, idx, tri_obj, tri_mat, obj_flags, cam, mats, mods, texs, lights,
-                 opts):
+                 opts, smooth=None):
         self.verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
         self.idx = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
         self.tri_obj = np.ascontiguousarray(tri_obj, np.int32)
@@ -294,6 +294,8 @@ class SceneDesc:
         self.texs = carray(yg_texture, texs)
         self.lights = carray(yg_light, lights)
         self.opts = opts
+        # autosmooth angle (degrees) per object id; objects absent are flat
+        self.smooth = dict(smooth or {})
 
     @property
     def n_tris(self):

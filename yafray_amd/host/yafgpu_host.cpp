@@ -376,7 +376,7 @@ int gpuInterface_t::addCamera(const std::string& name, const paramMap_t& p)
 
 int gpuInterface_t::addObject_trimesh(const std::string& name, const std::vector<float>& verts,
                                       const std::vector<uint32_t>& faces, const std::vector<std::string>& shaders,
-                                      bool cast_shadows, bool recv_rad, bool use_rad)
+                                      bool cast_shadows, bool recv_rad, bool use_rad, float smooth_angle)
 {
     if (faces.size() % 3 || verts.size() % 3) return fail(YG_ERR_ARG, "addObject_trimesh: bad array sizes");
     if (shaders.size() != 1 && shaders.size() != faces.size() / 3)
@@ -386,6 +386,7 @@ int gpuInterface_t::addObject_trimesh(const std::string& name, const std::vector
     o.f = faces;
     o.sh = shaders;
     o.flags = (cast_shadows ? 1 : 0) | (recv_rad ? 2 : 0) | (use_rad ? 4 : 0);
+    o.smooth = smooth_angle;
     objects_[name] = o;
     return YG_OK;
 }
@@ -494,9 +495,12 @@ int gpuInterface_t::render(const paramMap_t& params, std::vector<float>& rgb, yg
             if (it == shindex.end()) return fail(YG_ERR_ARG, "shader '" + sn + "' not found");
             fm[t] = it->second;
         }
+        int32_t oid = -1;
         if ((rc = yg_mesh_add(s, o.v.data(), (uint32_t)(o.v.size() / 3), o.f.data(), (uint32_t)fm.size(),
-                              fm.data(), o.flags, nullptr)))
+                              fm.data(), o.flags, &oid)))
             return rc;
+        // render.cc:764: obj->autoSmooth(angle) after the mesh is built
+        if (o.smooth >= 1.0f && (rc = yg_mesh_autosmooth(s, oid, o.smooth))) return rc;
     }
     if ((rc = yg_build(s, nullptr))) return rc;
     if ((rc = yg_set_camera(s, &cit->second))) return rc;

@@ -68,10 +68,12 @@ public:
     int addBackground(const std::string& name, const paramMap_t& p);
     int addCamera(const std::string& name, const paramMap_t& p);
     // addObject_trimesh: world-space vertices, index triples, one shader name
-    // per face (or a single name for all faces)
+    // per face (or a single name for all faces); smooth_angle: the mesh's
+    // autosmooth angle in degrees (meshObject_t::autoSmooth, below 1: none)
     int addObject_trimesh(const std::string& name, const std::vector<float>& verts,
                           const std::vector<uint32_t>& faces, const std::vector<std::string>& shaders,
-                          bool cast_shadows = true, bool recv_rad = true, bool use_rad = true);
+                          bool cast_shadows = true, bool recv_rad = true, bool use_rad = true,
+                          float smooth_angle = 0.0f);
     // render params: camera_name, background_name, bias, raydepth, AA_*, exposure,
     // gamma, clamp_rgb, output_type; rgb gets resx*resy*3 floats, tone-mapped
     // unless output_type is hdr or exr.  Photon lights are built first.
@@ -100,6 +102,7 @@ private:
         std::vector<uint32_t> f;
         std::vector<std::string> sh;
         int flags;
+        float smooth = 0.0f;
     };
     std::map<std::string, yg_texture> textures_;
     std::map<std::string, std::pair<paramMap_t, std::vector<paramMap_t>>> shaders_;

@@ -294,11 +294,10 @@ struct Loader {
         if (const std::string* as = attr(*mesh, "autosmooth")) {
             // render.cc:764 with msin_object.cc:271-280: a float angle; below 1
             // degree meshObject_t::autoSmooth does nothing (mesh.cc:204)
-            if (!as_float(*as, smooth_angle))
+            if (!as_float(*as, smooth_angle)) {
                 std::cerr << "[Warning]: Only float value accepted for autosmooth\n";
-            else if (smooth_angle >= 1)
-                return hostFail(YG_ERR_UNSUPPORTED, "object " + *name + ": autosmooth >= 1 degree "
-                                                    "(interpolated vertex normals) is not on the GPU path");
+                smooth_angle = 0;
+            }
         }
         bool shadow = true, emit = true, recv = true;
         paramMap_t op = generic_params(n);
@@ -345,7 +344,7 @@ struct Loader {
                 std::cerr << "[Warning]: Point " << k << " out of bounds in object\n";  // render.cc:719-731
                 k = 0;
             }
-        return I.addObject_trimesh(*name, v, f, sh, shadow, recv, emit);
+        return I.addObject_trimesh(*name, v, f, sh, shadow, recv, emit, smooth_angle);
     }
 
     int shader(const xmlNode_t& n)

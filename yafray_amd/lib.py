@@ -39,6 +39,7 @@ def lib():
         "yg_scene_create": (C.c_int, [C.POINTER(vp)]),
         "yg_scene_destroy": (C.c_int, [vp]),
         "yg_mesh_add": (C.c_int, [vp, vp, u32, vp, u32, vp, i32, C.POINTER(i32)]),
+        "yg_mesh_autosmooth": (C.c_int, [vp, i32, f32]),
         "yg_build": (C.c_int, [vp, vp]),
         "yg_scene_info": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_uint64), C.POINTER(i32)]),
@@ -116,6 +117,10 @@ class Scene:
         except Exception:
             pass
 
+    def autosmooth(self, obj, angle):
+        """meshObject_t::autoSmooth of object obj (vertex normals; before build)."""
+        check(lib().yg_mesh_autosmooth(self.h, int(obj), float(angle)), "yg_mesh_autosmooth")
+
     def add_mesh(self, verts, idx, mat=None, flags=7):
         v = np.ascontiguousarray(verts, np.float32).reshape(-1)
         f = np.ascontiguousarray(idx, np.uint32).reshape(-1)
@@ -158,7 +163,9 @@ class Scene:
             else:
                 used, inv = np.unique(tri.reshape(-1), return_inverse=True)
                 v, f = desc.verts[used], inv.reshape(-1, 3).astype(np.uint32)
-            s.add_mesh(v, f, desc.tri_mat[sel], int(desc.obj_flags[o]))
+            oid = s.add_mesh(v, f, desc.tri_mat[sel], int(desc.obj_flags[o]))
+            if desc.smooth.get(o, 0) >= 1:
+                s.autosmooth(oid, desc.smooth[o])
         if build:
             s.build(**kw)
         s.set_desc(desc)
