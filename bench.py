@@ -201,10 +201,21 @@ def bench_batched(args):
         outs.append(torch.zeros(rx * ry * 3, dtype=torch.float32, device="cuda"))
         tris += d.n_tris
         rays_px += rx * ry
-    seq_dt, _ = _timed(lambda: [s.render(o) for s, o in zip(scenes, outs)], args.steps, args.warmup)
+    mode = args.batch_mode
+    if mode == "sequential":
+        # per-scene renders only (a kernel profile of that mode alone)
+        seq_dt, _ = _timed(lambda: [s.render(o) for s, o in zip(scenes, outs)], args.steps, args.warmup)
+        return {"metric": "sequential GI ms/iter", "value": round(seq_dt * 1e3, 3), "unit": "ms/iter",
+                "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(seq_dt * 1e3, 3),
+                "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+                "data": "synthetic", "config": {"workload": "%d heightfield scenes rendered one after another"
+                                                            % len(specs), "triangles": int(tris)}}
+    seq_dt = None
+    if mode == "both":
+        seq_dt, _ = _timed(lambda: [s.render(o) for s, o in zip(scenes, outs)], args.steps, args.warmup)
     ref = [o.clone() for o in outs]
     bat_dt, st = _timed(lambda: Y.render_batch(scenes, outs), args.steps, args.warmup)
-    same = all(torch.equal(a, b) for a, b in zip(ref, outs))
+    same = all(torch.equal(a, b) for a, b in zip(ref, outs)) if seq_dt is not None else None
     rays = st.rays_camera + st.rays_extension + st.rays_shadow
     fp64 = None
     if not args.no_cpu:
@@ -226,11 +237,11 @@ def bench_batched(args):
         "config": {"workload": "%d heightfield scenes, triangles log-uniform in [1e3, 2e6], "
                                "resolutions from 256^2..1920x1080, 2 spp, 3 bounces" % len(specs),
                    "triangles": int(tris), "pixels": int(rays_px)},
-        "sequential_ms_per_iter": round(seq_dt * 1e3, 3),
-        "speedup_vs_sequential": round(seq_dt / bat_dt, 3),
+        "sequential_ms_per_iter": None if seq_dt is None else round(seq_dt * 1e3, 3),
+        "speedup_vs_sequential": None if seq_dt is None else round(seq_dt / bat_dt, 3),
         "batched_mrays_per_s": round(rays / bat_dt / 1e6, 2),
         "batched_msamples_per_s": round(rays_px * path_spp(scenes[0].desc) / bat_dt / 1e6, 2),
-        "images_identical_to_sequential": bool(same),
+        "images_identical_to_sequential": same,
         "error_vs_fp64": fp64,
     }
 
@@ -280,12 +291,13 @@ def bench_adjoint(args):
         st = sc.render_adjoint(gt, out, gm, go, gl, opts=o)
         adj_ms.append(st.ms_trace)
     fst = sc.render(out, timing=True)
-    fp64 = None
+    fp64 = grad = None
     if not args.no_cpu:
         # forward image at the starting parameters against the converged FP64 render
         sc.render(out)
         torch.cuda.synchronize()
         fp64 = fp64_leg(d, {"gpu": out.cpu().numpy()}, 512)
+        grad = adjoint_gradient_leg(d, sc, torch)
     return {
         "metric": "fwd+adjoint ms/iter", "value": round(both_dt * 1e3, 3), "unit": "ms/iter",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(both_dt * 1e3, 3),
@@ -301,7 +313,57 @@ def bench_adjoint(args):
         "forward_device_ms": round(fst.ms, 3),
         "loss": loss,
         "error_vs_fp64": fp64,
+        "gradient_vs_oracle": grad,
     }
+
+
+def adjoint_gradient_leg(d, sc, torch, n_sample=64):
+    """The GPU adjoint's gradient against the oracle's double-precision
+    gradient of the same L2 loss (target rendered from other parameters), at
+    the workload's full size: relative L2 error of the whole gradient
+    (768 albedo components, modulator amplitude and frequency, sun power)
+    and the largest relative error over `n_sample` sampled albedo components
+    among those at least 5% of the largest."""
+    import numpy as np
+
+    from oracle import pyoracle as po
+    from yafray_amd import scenes as S
+
+    def render(desc):
+        sc.set_desc(desc)
+        o = torch.zeros(desc.cam.resx * desc.cam.resy * 3, dtype=torch.float32, device="cuda")
+        sc.render(o)
+        torch.cuda.synchronize()
+        return o.cpu().numpy().astype(np.float64)
+
+    alb = np.array([list(m.color) for m in d.mats], np.float64)
+    target = render(S.with_params(d, albedo=alb * 0.9 + 0.04, amp=0.4, size=2.7, sun_power=2.5))
+    g = (2 * (render(d) - target)).astype(np.float32)
+    gt = torch.from_numpy(g).cuda()
+    out = torch.zeros_like(gt)
+    gm = torch.zeros(len(d.mats) * 3, dtype=torch.float64, device="cuda")
+    go = torch.zeros(2, dtype=torch.float64, device="cuda")
+    gl = torch.zeros(len(d.lights), dtype=torch.float64, device="cuda")
+    sc.render_adjoint(gt, out, gm, go, gl)
+    torch.cuda.synchronize()
+    gpu = np.concatenate([gm.cpu().numpy(), go.cpu().numpy(), gl.cpu().numpy()])
+    threads = cpu_cores()
+    t0 = time.perf_counter()
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    _, rm, ro, rl = oc.render_adjoint(d, g, threads=threads)
+    dt = time.perf_counter() - t0
+    ref = np.concatenate([rm.reshape(-1), ro.reshape(-1), rl])
+    nm = 3 * len(d.mats)
+    big = np.flatnonzero(np.abs(ref[:nm]) >= 0.05 * np.abs(ref[:nm]).max())
+    pick = np.random.default_rng(7).choice(big, size=min(n_sample, len(big)), replace=False)
+    rel = np.abs(gpu[pick] - ref[pick]) / np.abs(ref[pick])
+    return {"rel_l2_error": float(np.linalg.norm(gpu - ref) / np.linalg.norm(ref)),
+            "sampled_albedo_max_rel_error": float(rel.max()), "sampled": int(len(pick)),
+            "modulator_amp_rel_error": float(abs(gpu[nm] - ref[nm]) / abs(ref[nm])),
+            "modulator_size_rel_error": float(abs(gpu[nm + 1] - ref[nm + 1]) / abs(ref[nm + 1])),
+            "sun_power_rel_error": float(abs(gpu[-1] - ref[-1]) / abs(ref[-1])),
+            "oracle": "oracle/yo_render.c yo_render_adjoint (double-precision sums), %d threads, %.1f s"
+                      % (threads, dt)}
 
 
 def bench_photon(args):
@@ -782,6 +844,8 @@ def main():
                     choices=["gi", "batched", "adjoint", "photon", "direct", "autotune", "aa", "rays",
                              "noise", "room_gi"])
     ap.add_argument("--batch-n", type=int, default=64)
+    ap.add_argument("--batch-mode", default="both", choices=["both", "sequential", "batched"],
+                    help="batched workload: time both modes (default) or one of them alone")
     ap.add_argument("--width", type=int, default=0, choices=[0, 2, 4, 8],
                     help="GI workload: BVH width to measure (0 = the autotuner's choice)")
     ap.add_argument("--copy-probe", action="store_true",

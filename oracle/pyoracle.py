@@ -103,7 +103,7 @@ def _load(path):
         L.yo_block_regions_into.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, I32P, C.c_int]
         L.yo_render_adjoint.restype = C.c_int
         L.yo_render_adjoint.argtypes = [vp, vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int,
-                                        vp, C.c_int, vp, F32P, F32P, vp, vp, vp]
+                                        vp, C.c_int, vp, F32P, F32P, vp, vp, vp, C.c_int]
         L.yo_render_fp64.restype = C.c_int
         L.yo_render_fp64.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp, C.c_int, vp, C.c_int, vp,
                                      C.c_int, C.c_int, vp, vp, C.c_int64, C.c_uint64, vp, C.c_int]
@@ -250,7 +250,7 @@ class OracleScene:
                                % (ne.value, ext_cap, ns.value, sh_cap))
         return img, stats, ext[:ne.value], sh[:ns.value]
 
-    def render_adjoint(self, desc, grad_rgb):
+    def render_adjoint(self, desc, grad_rgb, threads=1):
         """FP64 reference gradient: returns (image, gmat[nmat,3], gmod[nmod,2], glight[nl])."""
         L = lib(self.fast)
         rgb = np.zeros(desc.cam.resx * desc.cam.resy * 3, np.float32)
@@ -262,7 +262,7 @@ class OracleScene:
                                  _addr(desc.mods), len(desc.mods), _addr(desc.texs), len(desc.texs),
                                  _addr(desc.lights), len(desc.lights), _ptr(desc.obj_flags),
                                  len(desc.obj_flags), C.addressof(desc.opts), g, rgb, _ptr(gmat),
-                                 _ptr(gmod), _ptr(glight))
+                                 _ptr(gmod), _ptr(glight), int(threads))
         if rc != 0:
             raise RuntimeError("yo_render_adjoint failed: %d" % rc)
         return (rgb.reshape(desc.cam.resy, desc.cam.resx, 3), gmat.reshape(-1, 3),

@@ -129,7 +129,7 @@ int yo_render_adjoint(yo_scene* s, const yg_camera* cam, const yg_material* mats
                       const yg_modulator* mods, int nmod, const yg_texture* texs, int ntex,
                       const yg_light* lights, int nl, const int32_t* obj_flags, int nobj,
                       const yg_render_opts* opts, const float* grad_rgb, float* rgb, double* gmat,
-                      double* gmod, double* glight);
+                      double* gmod, double* glight, int nthreads);
 
 /* FP64 render of camera rays through pixel centres (yo_fp64.c): sun, area
  * and path lights in double precision.  mode YO_FP64_REFERENCE: the whole

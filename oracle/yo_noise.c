@@ -7,6 +7,7 @@
  * agree bit for bit; they differ from libm's float functions by at most one
  * unit in the last place.
  */
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 #include "yo_internal.h"
@@ -24,20 +25,21 @@ static const char* PERM_HEX =
     "29a6c5e3f5f74a41266a165e522d21adf091ffea54fa661a4539cf75a488fb5d";
 
 static unsigned char HASH[512];
-static int hash_ready = 0;
+static pthread_once_t hash_once = PTHREAD_ONCE_INIT;
 
 static int hexv(char c) { return (c <= '9') ? c - '0' : c - 'a' + 10; }
 
-static void hash_init(void)
+static void hash_fill(void)
 {
-    if (hash_ready) return;
     for (int i = 0; i < 256; i++) {
         unsigned char v = (unsigned char)(hexv(PERM_HEX[2 * i]) * 16 + hexv(PERM_HEX[2 * i + 1]));
         HASH[i] = v;
         HASH[i + 256] = v;
     }
-    hash_ready = 1;
 }
+
+/* renders on several threads share the tables: filled once */
+static void hash_init(void) { pthread_once(&hash_once, hash_fill); }
 
 void yo_perm(unsigned char* out)
 {
@@ -178,18 +180,18 @@ static const char* GRAD_HEX =
     "bf5c69f93f3d1202bed4180dbf0826033f1003ffbf343dff3eddd80ebf414403"
     "3f25e3fcbdce00083deaf0393d365fd9bf7e0c073f774bf93e7abffdbda9802c";
 static float GRAD[768];
-static int grad_ready = 0;
+static pthread_once_t grad_once = PTHREAD_ONCE_INIT;
 
-static void grad_init(void)
+static void grad_fill(void)
 {
-    if (grad_ready) return;
     for (int i = 0; i < 768; i++) {
         uint32_t b = 0;
         for (int k = 0; k < 8; k++) b = b * 16 + (uint32_t)hexv(GRAD_HEX[8 * i + k]);
         memcpy(&GRAD[i], &b, 4);
     }
-    grad_ready = 1;
 }
+
+static void grad_init(void) { pthread_once(&grad_once, grad_fill); }
 
 void yo_grad_table(float* out)
 {

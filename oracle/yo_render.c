@@ -61,6 +61,8 @@ typedef struct {
     double* gmat;          /* nmat*3 */
     double* gmod;          /* nmod*2 */
     double* glight;        /* nl */
+    struct adj_path_s* ap; /* the current pixel's recorded path samples */
+    int nap, apcap;
 } rctx;
 
 static void adj_path_sample_begin(rctx* R);
@@ -395,15 +397,12 @@ double yo_tex_dF_dlambda(const yg_texture* t, const float* p);
 void yo_tex_color_f(const yg_texture* t, const float* p, float* out);
 
 #define ADJ_MAXD 8
-typedef struct {
+typedef struct adj_path_s {
     int n, hit[ADJ_MAXD], mat[ADJ_MAXD];
     yo_v3 P[ADJ_MAXD];
     yo_col lf[ADJ_MAXD], lfp[ADJ_MAXD];
     float s;
 } adj_path;
-
-static adj_path* g_ap = NULL;     /* per-thread use only: adjoint renders run single-threaded */
-static int g_nap = 0, g_apcap = 0;
 
 /* albedo with its partial derivatives, in double (one modulator at most):
  * rho, d rho/d albedo, d rho/d modulator colour, d rho/d lambda */
@@ -460,18 +459,16 @@ static void adj_vertex_add(rctx* R, int mat, yo_v3 P, const double gamma[3])
 
 static void adj_path_sample_begin(rctx* R)
 {
-    (void)R;
-    if (g_nap == g_apcap) {
-        g_apcap = g_apcap ? 2 * g_apcap : 64;
-        g_ap = (adj_path*)realloc(g_ap, sizeof(adj_path) * (size_t)g_apcap);
+    if (R->nap == R->apcap) {
+        R->apcap = R->apcap ? 2 * R->apcap : 64;
+        R->ap = (adj_path*)realloc(R->ap, sizeof(adj_path) * (size_t)R->apcap);
     }
-    g_ap[g_nap].n = 0;
+    R->ap[R->nap].n = 0;
 }
 
 static void adj_path_vertex(rctx* R, int hit, const yo_sp* sp, yo_col lf, yo_col lfp)
 {
-    (void)R;
-    adj_path* a = &g_ap[g_nap];
+    adj_path* a = &R->ap[R->nap];
     if (a->n >= ADJ_MAXD) return;
     a->hit[a->n] = hit;
     if (hit) { a->mat[a->n] = sp->mat; a->P[a->n] = sp->P; a->lf[a->n] = lf; a->lfp[a->n] = lfp; }
@@ -480,9 +477,8 @@ static void adj_path_vertex(rctx* R, int hit, const yo_sp* sp, yo_col lf, yo_col
 
 static void adj_path_sample_end(rctx* R, float s)
 {
-    (void)R;
-    g_ap[g_nap].s = s;
-    g_nap++;
+    R->ap[R->nap].s = s;
+    R->nap++;
 }
 
 /* product rule over the recorded samples of one pixel (see yg_render_adjoint) */
@@ -500,8 +496,8 @@ static void adj_path_pixel(rctx* R, int li, const yo_sp* sp0, float mult)
     int sun = -1;
     for (int l2 = 0; l2 < R->nl; l2++)
         if (R->lights[l2].type == YG_LIGHT_SUN && R->lights[l2].use_in_indirect) { sun = l2; break; }
-    for (int q = 0; q < g_nap; q++) {
-        const adj_path* a = &g_ap[q];
+    for (int q = 0; q < R->nap; q++) {
+        const adj_path* a = &R->ap[q];
         double rv[ADJ_MAXD][3];
         for (int b = 0; b < a->n; b++)
             if (a->hit[b]) {
@@ -538,7 +534,7 @@ static void adj_path_pixel(rctx* R, int li, const yo_sp* sp0, float mult)
     for (int c = 0; c < 3; c++) g0[c] = R->gpx[c] * T[c] * pm;
     adj_vertex_add(R, sp0->mat, sp0->P, g0);
     if (sun >= 0) R->glight[sun] += W[0] * dpow[0] + W[1] * dpow[1] + W[2] * dpow[2];
-    g_nap = 0;
+    R->nap = 0;
 }
 
 static void adj_direct_sun(rctx* R, int li, const yo_sp* sp, float inte)
@@ -713,6 +709,7 @@ static void rctx_free(rctx* R)
         free(R->hseq[li]);
     }
     free(R->areas); free(R->sun_dir); free(R->sun_col); free(R->hseq);
+    free(R->ap);
 }
 
 static void grid_build(yo_photonmap* pm, float radius)
@@ -1100,6 +1097,7 @@ typedef struct {
     rctx* shared_R;   /* sequential mode only */
     const float* grad_rgb;  /* adjoint: dLoss/dpixel */
     double *gmat, *gmod, *glight;
+    int nmod;
 } render_job;
 
 /* color_t::expgam_Adjust (color.h:175-195) with scene_t's stored values
@@ -1249,11 +1247,15 @@ static void* render_worker(void* arg)
     render_job* J = (render_job*)arg;
     rctx R;
     rctx_init(&R, J->s, J->mats, J->nmat, J->mods, J->texs, J->lights, J->nl, J->obj_flags, J->nobj, J->opts);
+    double* priv = NULL;
+    const size_t ng = 3 * (size_t)J->nmat + 2 * (size_t)J->nmod + (size_t)J->nl;
     if (J->grad_rgb) {
+        /* per-thread gradient sums, added to the caller's under the lock */
+        priv = (double*)calloc(ng, sizeof(double));
         R.adj = 1;
-        R.gmat = J->gmat;
-        R.gmod = J->gmod;
-        R.glight = J->glight;
+        R.gmat = priv;
+        R.gmod = priv + 3 * (size_t)J->nmat;
+        R.glight = priv + 3 * (size_t)J->nmat + 2 * (size_t)J->nmod;
     }
     for (;;) {
         pthread_mutex_lock(&J->mu);
@@ -1264,7 +1266,14 @@ static void* render_worker(void* arg)
     }
     pthread_mutex_lock(&J->mu);
     for (int k = 0; k < 5; k++) J->stats[k] += R.stats[k];
+    if (priv) {
+        for (int q = 0; q < 3 * J->nmat; q++) J->gmat[q] += priv[q];
+        if (J->gmod)
+            for (int q = 0; q < 2 * J->nmod; q++) J->gmod[q] += priv[3 * (size_t)J->nmat + q];
+        for (int q = 0; q < J->nl; q++) J->glight[q] += priv[3 * (size_t)J->nmat + 2 * (size_t)J->nmod + q];
+    }
     pthread_mutex_unlock(&J->mu);
+    free(priv);
     rctx_free(&R);
     return NULL;
 }
@@ -1342,17 +1351,24 @@ int yo_render(yo_scene* s, const yg_camera* cam, const yg_material* mats, int nm
 }
 
 /* FP64 reference gradient of a loss through the image (see yg_render_adjoint):
- * renders like yo_render (single thread) and accumulates, per output pixel,
+ * renders like yo_render and accumulates, per output pixel,
  * dLoss/dparams = sum over the estimator's terms of grad_rgb * d(term)/d(param)
- * in double precision.  gmat: nmat*3, gmod: nmod*2, glight: nl (zeroed here). */
+ * in double precision.  gmat: nmat*3, gmod: nmod*2, glight: nl (zeroed here).
+ * nthreads > 1 renders areas on several threads (each summing its own
+ * gradient; the sums are added at the end, so the last bits depend on the
+ * areas' order); scenes with area lights run on one. */
 int yo_render_adjoint(yo_scene* s, const yg_camera* cam, const yg_material* mats, int nmat,
                       const yg_modulator* mods, int nmod, const yg_texture* texs, int ntex,
                       const yg_light* lights, int nl, const int32_t* obj_flags, int nobj,
                       const yg_render_opts* opts, const float* grad_rgb, float* rgb, double* gmat,
-                      double* gmod, double* glight)
+                      double* gmod, double* glight, int nthreads)
 {
     (void)ntex;
     if (opts->aa_passes > 0) return -5; /* antialiasing passes are not differentiated */
+    for (int li = 0; li < nl; li++)
+        if (lights[li].type == YG_LIGHT_AREA) nthreads = 1;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 64) nthreads = 64;
     memset(gmat, 0, sizeof(double) * 3 * (size_t)nmat);
     if (gmod) memset(gmod, 0, sizeof(double) * 2 * (size_t)nmod);
     memset(glight, 0, sizeof(double) * (size_t)nl);
@@ -1366,9 +1382,14 @@ int yo_render_adjoint(yo_scene* s, const yg_camera* cam, const yg_material* mats
     J.s = s; J.cam = cam; J.mats = mats; J.nmat = nmat; J.mods = mods; J.texs = texs;
     J.lights = lights; J.nl = nl; J.obj_flags = obj_flags; J.nobj = nobj; J.opts = opts;
     J.k = &k; J.regions = regions; J.nreg = nreg; J.rgb = rgb; J.next = 0;
-    J.grad_rgb = grad_rgb; J.gmat = gmat; J.gmod = gmod; J.glight = glight;
+    J.grad_rgb = grad_rgb; J.gmat = gmat; J.gmod = gmod; J.glight = glight; J.nmod = gmod ? nmod : 0;
     pthread_mutex_init(&J.mu, NULL);
-    render_worker(&J);
+    pthread_t th[64];
+    if (nthreads == 1) render_worker(&J);
+    else {
+        for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, render_worker, &J);
+        for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    }
     pthread_mutex_destroy(&J.mu);
     free(regions);
     return 0;

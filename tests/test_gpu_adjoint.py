@@ -22,7 +22,10 @@ def _render(Y, torch, sc, desc):
 def setup(torch_cuda):
     torch = torch_cuda
     from yafray_amd import lib as Y
-    d = S.terrain_diff(po.heightfield, nx=96, ny=48, res=48, samples=4, depth=3, nmat=16)
+    # albedos in [0.35, 0.9]: no path of depth 3 falls below the pathlight's
+    # energy cut-off (pathlight.cc:315), so the loss is smooth in every
+    # parameter and central differences apply
+    d = S.terrain_diff(po.heightfield, nx=96, ny=48, res=48, samples=4, depth=3, nmat=16, albedo_lo=0.35)
     sc = Y.Scene.from_desc(d)
     alb = np.array([list(m.color) for m in d.mats], np.float64)
     target = _render(Y, torch, sc, S.with_params(d, albedo=alb * 0.9 + 0.04, amp=0.4, size=2.7,
@@ -125,3 +128,47 @@ def test_adjoint_zero_albedo_channel(setup):
         err = np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-30)
         assert err <= 1e-3, err
     assert np.any(gm[:, 1] != 0)
+
+
+def test_adjoint_at_workload_size_matches_fp64_oracle(torch_cuda):
+    """The differentiable-GI workload at its own size: 1,045,506-triangle
+    terrain, 512x512, 16 spp, 3 bounces, 256 materials with albedos drawn
+    U[0.1, 0.9], so paths do end at the pathlight's energy cut-off (the
+    adjoint holds each path's length fixed, as it holds visibility).  The
+    GPU gradient of an L2 loss against a target rendered from other
+    parameters equals the oracle's double-precision gradient of the same
+    loss: the whole gradient within 1e-3 relative L2 error, and each of 64
+    sampled albedo components among the larger ones, the modulator's
+    amplitude and frequency and the sun's power within 1e-2."""
+    torch = torch_cuda
+    from yafray_amd import lib as Y
+    d = S.terrain_diff(po.heightfield)
+    assert (d.cam.resx, d.cam.resy, len(d.mats), d.n_tris) == (512, 512, 256, 1045506)
+    sc = Y.Scene.from_desc(d)
+    alb = np.array([list(m.color) for m in d.mats], np.float64)
+    target = _render(Y, torch, sc, S.with_params(d, albedo=alb * 0.9 + 0.04, amp=0.4, size=2.7, sun_power=2.5))
+    img = _render(Y, torch, sc, d)
+    g = (2 * (img - target)).astype(np.float32)
+    gt = torch.from_numpy(g.reshape(-1)).cuda()
+    out = torch.zeros_like(gt)
+    gmat = torch.zeros(len(d.mats) * 3, dtype=torch.float64, device="cuda")
+    gmod = torch.zeros(2, dtype=torch.float64, device="cuda")
+    glight = torch.zeros(len(d.lights), dtype=torch.float64, device="cuda")
+    sc.render_adjoint(gt, out, gmat, gmod, glight)
+    torch.cuda.synchronize()
+    gm_gpu, gmod_gpu, gl_gpu = gmat.cpu().numpy(), gmod.cpu().numpy(), glight.cpu().numpy()
+    oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
+    _, gm, gmo, gl = oc.render_adjoint(d, g, threads=16)
+    gm, gmo = gm.reshape(-1), gmo.reshape(-1)
+    full_gpu = np.concatenate([gm_gpu, gmod_gpu, gl_gpu])
+    full_ref = np.concatenate([gm, gmo, gl])
+    err = np.linalg.norm(full_gpu - full_ref) / np.linalg.norm(full_ref)
+    print("gradient relative L2 error %.3g" % err)
+    assert err <= 1e-3, err
+    big = np.flatnonzero(np.abs(gm) >= 0.05 * np.abs(gm).max())
+    pick = np.random.default_rng(7).choice(big, size=min(64, len(big)), replace=False)
+    rel = np.abs(gm_gpu[pick] - gm[pick]) / np.abs(gm[pick])
+    print("sampled albedo components: max relative error %.3g (of %d)" % (rel.max(), len(big)))
+    assert rel.max() <= 1e-2, rel.max()
+    for a, b in ((gmod_gpu[0], gmo[0]), (gmod_gpu[1], gmo[1]), (gl_gpu[1], gl[1])):
+        assert abs(a - b) <= 1e-2 * abs(b), (a, b)

@@ -9,7 +9,7 @@ from yafray_amd import scenes as S
 
 @pytest.fixture(scope="module")
 def setup():
-    d = S.terrain_diff(po.heightfield, nx=48, ny=24, res=40, samples=4, depth=3, nmat=8)
+    d = S.terrain_diff(po.heightfield, nx=48, ny=24, res=40, samples=4, depth=3, nmat=8, albedo_lo=0.35)
     oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
     alb = np.array([list(m.color) for m in d.mats], np.float64)
     tgt_desc = S.with_params(d, albedo=alb * 0.9 + 0.04, amp=0.4, size=2.7, sun_power=2.5)

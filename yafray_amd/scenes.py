@@ -288,13 +288,15 @@ def photon_field(n_side=64, subdiv=4, extent=100.0, resx=1920, resy=1080,
 
 
 def terrain_diff(heights_fn, nx=1024, ny=512, res=512, samples=16, depth=3, nmat=256,
-                 amp=0.5, size=3.0, sun_power=3.0, albedo_lo=0.35, albedo_hi=0.9, seed_key=5):
+                 amp=0.5, size=3.0, sun_power=3.0, albedo_lo=0.1, albedo_hi=0.9, seed_key=5):
     """Differentiable GI: the terrain at res x res with nmat height-band
     materials whose albedo is modulated (mode mul) by a Musgrave fBm texture;
     amplitude = modulator colour, frequency = modulator size.  Albedos are
-    drawn in [albedo_lo, albedo_hi] so no path of depth 3 crosses the
-    pathlight's energy cut-off (pathlight.cc:315), which keeps the image
-    differentiable in every parameter."""
+    drawn U[albedo_lo, albedo_hi] per channel, by default the GI terrain's
+    U[0.1, 0.9].  With albedo_lo = 0.35 no path of depth 3 falls below the
+    pathlight's energy cut-off (pathlight.cc:315), so the image is smooth in
+    every parameter (what central differences need); below that the cut-off
+    ends some paths, a step that the adjoint, like visibility, holds fixed."""
     ex, ey = 2.0, 1.0
     verts, idx, band = terrain_mesh(nx, ny, heights_fn, ex, ey, nmat)
     rng = SplitMix64(sub_seed(seed_key))
