@@ -63,8 +63,12 @@ def gpu_leg(args):
     ks = sc.render(out, kt)
     torch.cuda.synchronize()
     img = out.cpu().numpy().reshape(d.cam.resy, d.cam.resx, 3)
+    builders = build_compare(sc, out, torch, dict(width=int(best.width), max_leaf=int(best.max_leaf),
+                                                  bins=int(best.sah_bins), stack_lds=int(best.stack_lds),
+                                                  rays_per_lane=int(best.rays_per_lane),
+                                                  compact_below=float(best.compact_below)))
     return {
-        "desc": d, "img": img, "tune": tune,
+        "desc": d, "img": img, "tune": tune, "builders": builders,
         "dt": dt, "rays": rays, "stats": ks, "info": info, "setup_s": setup_s,
         "graph_modes": sorted(modes),
         "finite": bool(np.all(np.isfinite(img))), "mean": float(img.mean()),
@@ -169,6 +173,31 @@ def measured_traffic(kernel):
         return int(json.loads(block)["kernels"][kernel]["hbm_bytes_per_launch"])
     except (OSError, IndexError, KeyError, ValueError):
         return None
+
+
+def build_compare(sc, out, torch, kw, reps=2, frames=5):
+    """The scene's BVH built by the host builder (binned SAH on host threads)
+    and by the device builders (the same binned SAH on the GPU; clustering
+    on the GPU) at the same settings:
+    wall time of yg_build (fastest of `reps`, host data already in memory)
+    and the frame time each tree gives.  Leaves the scene built by the
+    host builder."""
+    from yafray_amd import descriptors as D
+    res = {}
+    for name, b in (("device_ploc", D.YG_BUILDER_DEVICE_PLOC), ("device_sah", D.YG_BUILDER_DEVICE),
+                    ("host", D.YG_BUILDER_HOST)):
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sc.build(builder=b, **kw)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        ms, _ = _timed(lambda: sc.render(out), frames, 1)
+        info = sc.info()
+        res[name] = {"build_s": round(min(ts), 4), "frame_ms": round(ms * 1e3, 3), "nodes": int(info["nodes"]),
+                     "max_depth": int(info["max_depth"])}
+    return res
 
 
 def _timed(fn, steps, warmup):
@@ -385,7 +414,7 @@ def bench_photon(args):
 
     t0 = time.time()
     d = S.photon_field()
-    sc = Y.Scene.from_desc(d)
+    sc = Y.Scene.from_desc(d, builder=D.YG_BUILDER_DEVICE)
     setup_s = time.time() - t0
     pst = sc.photon_build(0)
     build_dt, _ = _timed(lambda: sc.photon_build(0), max(1, args.steps // 2), 0)
@@ -438,6 +467,7 @@ def bench_photon(args):
         "photon_build_ms": round(build_dt * 1e3, 3), "render_ms": round(rdt * 1e3, 3),
         "emitted": int(pst.emitted), "stored": int(pst.stored), "clustered": int(pst.clustered),
         "scene_setup_s": round(setup_s, 1), "image_finite": finite,
+        "bvh": "device binned-SAH builder, width 4",
         "roofline": {"bound": "hbm", "kernel": "k_gather_points", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "queries": nq, "ms_per_launch": round(g_ms, 4),
@@ -445,6 +475,18 @@ def bench_photon(args):
                      "note": "the clustered photon map (%d marks, %d KB) stays in L2: the gather is bound by "
                              "its LDS heap work, not by HBM" % (int(pst.clustered), int(pst.clustered) * 36 // 1024)},
     }
+    # the two builders on this scene (the photon map is rebuilt for each tree)
+    bvh = {}
+    for name, b in (("host", D.YG_BUILDER_HOST), ("device_ploc", D.YG_BUILDER_DEVICE_PLOC),
+                    ("device_sah", D.YG_BUILDER_DEVICE)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sc.build(builder=b)
+        torch.cuda.synchronize()
+        bs = time.perf_counter() - t0
+        fdt, _ = _timed(lambda: (sc.photon_build(0), sc.render(out)), 2, 1)
+        bvh[name] = {"build_s": round(bs, 3), "ms_per_frame": round(fdt * 1e3, 3), "nodes": int(sc.info()["nodes"])}
+    line["bvh_builders"] = bvh
     if not args.no_cpu:
         line.update(_photon_cpu_legs(d, img, pst, args, sc=sc))
     return line
@@ -929,6 +971,8 @@ def main():
         "bvh": {"nodes": int(g["info"]["nodes"]), "max_depth": int(g["info"]["max_depth"]),
                 "dev_bytes": int(g["info"]["dev_bytes"])},
         "autotune": g["tune"],
+        # BVH built on the host and on the device at the tuned settings (after the timed region)
+        "bvh_builders": g["builders"],
         "image_finite": g["finite"],
         # 1 = frame replayed from its captured hipGraph, 2 = captured in that call, 0 = kernel by kernel
         "frame_graph_modes": g["graph_modes"],

@@ -33,7 +33,7 @@ extern "C" {
 #define YG_ERR_UNSUPPORTED (-4)
 #define YG_ERR_NOMEM (-5)
 
-#define YG_ABI_VERSION 8
+#define YG_ABI_VERSION 9
 
 /* ---------------------------------------------------------------- rays -- */
 
@@ -228,7 +228,17 @@ typedef struct yg_build_opts {
                               level's listed paths survive; otherwise the next level reuses
                               the list and its ended paths' lanes exit at once
                               (0 = default 1: always compact; 0 < x <= 1, or -1 = never) */
+    int32_t builder;       /* YG_BUILDER_DEVICE (0, default): binned SAH on the GPU, level by
+                              level (sah_bins at most 64), the host builder's tree;
+                              YG_BUILDER_HOST: binned SAH on host threads;
+                              YG_BUILDER_DEVICE_PLOC: locally-ordered clustering on the GPU
+                              (fastest build, trees that trace more slowly);
+                              scenes of at most 64 triangles are built on the host */
 } yg_build_opts;
+
+#define YG_BUILDER_DEVICE 0
+#define YG_BUILDER_HOST 1
+#define YG_BUILDER_DEVICE_PLOC 2
 
 typedef struct yg_scene yg_scene;
 

@@ -30,7 +30,7 @@ def test_error_path_without_gpu():
     """Argument errors are reported through the status code and yg_last_error."""
     from yafray_amd import lib as Y
     L = Y.lib()
-    assert L.yg_abi_version() == 8
+    assert L.yg_abi_version() == 9
     assert L.yg_scene_create(None) == -1
     assert "null" in Y.last_error()
     h = ctypes.c_void_p()
@@ -101,15 +101,16 @@ def test_autotune_argument_errors_without_gpu():
 
 
 def test_build_launch_settings_are_checked_without_gpu():
-    """yg_build refuses a width, rays_per_lane or compact_below outside
-    their ranges before anything reaches the device."""
+    """yg_build refuses a width, rays_per_lane, compact_below or builder
+    outside their ranges before anything reaches the device."""
     from yafray_amd import descriptors as D
     from yafray_amd import lib as Y
     L = Y.lib()
     h = ctypes.c_void_p()
     assert L.yg_scene_create(ctypes.byref(h)) == 0
     for kw, msg in ((dict(width=3), "width"), (dict(rays_per_lane=3), "rays_per_lane"),
-                    (dict(compact_below=1.5), "compact_below"), (dict(compact_below=-0.5), "compact_below")):
+                    (dict(compact_below=1.5), "compact_below"), (dict(compact_below=-0.5), "compact_below"),
+                    (dict(builder=3), "builder")):
         o = D.yg_build_opts(**kw)
         assert L.yg_build(h, ctypes.byref(o)) == -1
         assert msg in Y.last_error()

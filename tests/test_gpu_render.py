@@ -247,6 +247,39 @@ def test_smoothed_spheres_photon_map_matches_oracle(torch_cuda):
     assert rel_rmse(out.cpu().numpy().reshape(ref.shape), ref) <= 1e-3
 
 
+@pytest.mark.parametrize("builder", [D.YG_BUILDER_DEVICE, D.YG_BUILDER_DEVICE_PLOC])
+@pytest.mark.parametrize("width", [2, 4, 8])
+def test_device_built_tree_renders_like_the_oracle(torch_cuda, width, builder):
+    """Trees built on the GPU (binned SAH, or locally-ordered clustering) in
+    place of the host's: the GI terrain, the smoothed-sphere room with its
+    area light and pathlight (per-corner normals scattered to the device
+    tree's leaf order) and the photon field render as the oracle does, with
+    the same ray counts."""
+    torch = torch_cuda
+    cases = [small_terrain(nx=128, ny=64, resx=120, resy=80, samples=4, depth=3, nmat=16)]
+    r = S.direct_room(80, 60, subdiv=3, samples=4)
+    cases.append(D.SceneDesc(r.verts, r.idx, r.tri_obj, r.tri_mat, r.obj_flags, r.cam, list(r.mats), [], [],
+                             list(r.lights) + [D.pathlight(samples=4, depth=2, use_qmc=True, threshold=1e3)],
+                             r.opts, smooth={3: 180.0}))
+    for d in cases:
+        img, st, sc = gpu_render(d, torch, build_kw={"width": width, "builder": builder})
+        ref, rs = po.OracleScene.from_desc(d).render(d)
+        assert rel_rmse(img, ref) <= 1e-3
+        assert (st.rays_camera, st.rays_extension, st.rays_shadow) == tuple(int(x) for x in rs[:3])
+        assert sc.info()["nodes"] > 0
+    d = S.photon_field(n_side=4, subdiv=2, extent=10.0, resx=96, resy=64, photons=2 * 150 * 150, search=20, nmat=4)
+    sc = _Y().Scene.from_desc(d, width=width, builder=builder)
+    ps = sc.photon_build(0)
+    oc = po.OracleScene.from_desc(d)
+    os_ = oc.photon_build(d, 0)
+    assert (ps.emitted, ps.stored, ps.clustered) == tuple(int(x) for x in os_[:3])
+    out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+    sc.render(out)
+    torch.cuda.synchronize()
+    ref, _ = oc.render(d)
+    assert rel_rmse(out.cpu().numpy().reshape(ref.shape), ref) <= 1e-3
+
+
 @pytest.mark.parametrize("search", [50, 100])
 def test_photon_k50_merged_clusters_match_oracle(torch_cuda, search):
     """The photon workload's settings at small scale: K = 50 (and 100, past

@@ -52,15 +52,16 @@ def _check(prim_g, t_g, u_g, v_g, prim_c, t_c, u_c, v_c):
 
 def test_library_loads(torch_cuda):
     Y = _lib(torch_cuda)
-    assert Y.lib().yg_abi_version() == 8
+    assert Y.lib().yg_abi_version() == 9
 
 
+@pytest.mark.parametrize("builder", [D.YG_BUILDER_HOST, D.YG_BUILDER_DEVICE, D.YG_BUILDER_DEVICE_PLOC])
 @pytest.mark.parametrize("seed,width", [(0, 2), (1, 2), (0, 4), (1, 4), (0, 8), (1, 8)])
-def test_closest_any_room(torch_cuda, seed, width):
+def test_closest_any_room(torch_cuda, seed, width, builder):
     torch = torch_cuda
     Y = _lib(torch)
     d = S.direct_room(64, 48, subdiv=5)
-    sc = Y.Scene.from_desc(d, width=width)
+    sc = Y.Scene.from_desc(d, width=width, builder=builder)
     oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
     rays = _rays(200000, -2.9, 2.9, seed, zshift=3.0)
     pc, tc, uc, vc = oc.trace(rays, threads=8)
@@ -99,10 +100,12 @@ def test_tmin_and_empty(torch_cuda, width):
     assert np.all(p == -1)
 
 
+@pytest.mark.parametrize("builder", [D.YG_BUILDER_HOST, D.YG_BUILDER_DEVICE, D.YG_BUILDER_DEVICE_PLOC])
 @pytest.mark.parametrize("width", [2, 4, 8])
-def test_terrain_parity_full_size(torch_cuda, width):
+def test_terrain_parity_full_size(torch_cuda, width, builder):
     """1,045,506-triangle fBm terrain: 1M incoherent rays on the GPU, a
-    100k-ray sample checked against the oracle."""
+    100k-ray sample checked against the oracle; the tree built on the host or
+    on the device."""
     torch = torch_cuda
     Y = _lib(torch)
     verts, idx, band = S.terrain_mesh(1024, 512, Y.device_heights)
@@ -111,7 +114,7 @@ def test_terrain_parity_full_size(torch_cuda, width):
     assert np.array_equal(hv, verts)
     sc = Y.Scene()
     sc.add_mesh(verts, idx, band)
-    sc.build(width=width)
+    sc.build(width=width, builder=builder)
     info = sc.info()
     assert info["tris"] == 1045506
     rng = np.random.default_rng(7)
@@ -143,12 +146,13 @@ def test_terrain_parity_full_size(torch_cuda, width):
     assert 0.05 < np.mean(ag) < 0.95
 
 
+@pytest.mark.parametrize("builder", [D.YG_BUILDER_HOST, D.YG_BUILDER_DEVICE, D.YG_BUILDER_DEVICE_PLOC])
 @pytest.mark.parametrize("width", [2, 4, 8])
-def test_first_hit_is_shadowed(torch_cuda, width):
+def test_first_hit_is_shadowed(torch_cuda, width, builder):
     torch = torch_cuda
     Y = _lib(torch)
     d = S.direct_room(64, 48, subdiv=4)
-    sc = Y.Scene.from_desc(d, width=width)
+    sc = Y.Scene.from_desc(d, width=width, builder=builder)
     oc = po.OracleScene(d.verts, d.idx, d.tri_obj, d.tri_mat)
     rng = np.random.default_rng(3)
     n = 50000

@@ -14,7 +14,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libyafgpu.so")
 CLI = os.path.join(HERE, "yafgpu-render")
 
-SOURCES = ["capi.cpp", "qmc.cpp", "bvh_build.cpp", "autotune.cpp", "trace.hip", "render.hip", "photon.hip",
+SOURCES = ["capi.cpp", "qmc.cpp", "bvh_build.cpp", "bvh_gpu.hip", "autotune.cpp", "trace.hip", "render.hip",
+           "photon.hip",
            "../host/yafgpu_host.cpp", "../host/yafgpu_xml.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

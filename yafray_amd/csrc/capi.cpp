@@ -186,21 +186,54 @@ extern "C" int yg_build(yg_scene* s, const yg_build_opts* opts)
     if (compact_below == -1.0f) compact_below = 0.0f;  // never compact (0 < 0 is never true)
     else if (!(compact_below > 0.0f && compact_below <= 1.0f))
         return yg_fail(YG_ERR_ARG, "compact_below must be in (0, 1] or -1");
-    HostBVH h;
+    const int builder = opts ? opts->builder : YG_BUILDER_DEVICE;
+    if (builder != YG_BUILDER_HOST && builder != YG_BUILDER_DEVICE && builder != YG_BUILDER_DEVICE_PLOC)
+        return yg_fail(YG_ERR_ARG, "builder must be YG_BUILDER_HOST, YG_BUILDER_DEVICE or YG_BUILDER_DEVICE_PLOC");
     const int64_t nt = (int64_t)s->tri_mat.size();
+    // the device copy replaces the previous one
+    auto free_old = [s]() {
+        void* old[] = {s->d_nodes, s->d_nodes4, s->d_nodes8, s->d_tvn, s->d_tris, s->d_tprim, s->d_tobj,
+                       s->d_tmat, s->d_obj_flags};
+        for (void* p : old)
+            if (p) (void)hipFree(p);
+        s->d_nodes = nullptr; s->d_nodes4 = nullptr; s->d_nodes8 = nullptr; s->d_tvn = nullptr;
+        s->d_tris = nullptr; s->d_tprim = nullptr; s->d_tobj = nullptr; s->d_tmat = nullptr;
+        s->d_obj_flags = nullptr;
+        s->dev_bytes = 0;
+    };
+    int rc;
+    if (builder != YG_BUILDER_HOST && nt > 64) {
+        free_old();
+        DeviceBVHInfo di;
+        if ((rc = build_bvh_device(s, builder, max_leaf, bins, width, di))) {
+            free_old();
+            s->built = false;
+            return rc;
+        }
+        const int entries = (width == 2) ? di.max_depth + 1 : di.stack + 1;
+        if (entries > YG_MAX_STACK) {
+            free_old();
+            s->built = false;
+            return yg_fail(YG_ERR_STATE, "BVH deeper than the traversal stack");
+        }
+        if ((rc = dev_upload(&s->d_obj_flags, s->obj_flags.data(), s->obj_flags.size(), &s->dev_bytes))) return rc;
+        s->n_nodes = di.n_nodes;
+        s->max_depth = di.max_depth;
+        s->width = width;
+        s->stack_entries = entries;
+        s->stack_lds = stack_lds < 0 ? 0 : stack_lds;
+        s->rays_per_lane = rays_lane;
+        s->compact_below = compact_below;
+        s->built = true;
+        s->gen++;
+        return YG_OK;
+    }
+    HostBVH h;
     build_bvh(s->verts.data(), s->idx.data(), nt, s->tri_obj.data(), s->tri_mat.data(), max_leaf,
               bins, threads, width, h);
     const int entries = (width == 8) ? h.stack8 + 1 : (width == 4) ? h.stack4 + 1 : h.max_depth + 1;
     if (entries > YG_MAX_STACK) return yg_fail(YG_ERR_STATE, "BVH deeper than the traversal stack");
-    // the device copy replaces the previous one
-    void* old[] = {s->d_nodes, s->d_nodes4, s->d_nodes8, s->d_tvn, s->d_tris, s->d_tprim, s->d_tobj, s->d_tmat,
-                   s->d_obj_flags};
-    for (void* p : old)
-        if (p) (void)hipFree(p);
-    s->d_nodes = nullptr; s->d_nodes4 = nullptr; s->d_nodes8 = nullptr; s->d_tvn = nullptr; s->d_tris = nullptr;
-    s->d_tprim = nullptr; s->d_tobj = nullptr; s->d_tmat = nullptr; s->d_obj_flags = nullptr;
-    s->dev_bytes = 0;
-    int rc;
+    free_old();
     if ((rc = dev_upload(&s->d_nodes, h.nodes.data(), h.nodes.size(), &s->dev_bytes))) return rc;
     if (width == 4 && (rc = dev_upload(&s->d_nodes4, h.nodes4.data(), h.nodes4.size(), &s->dev_bytes))) return rc;
     if (width == 8 && (rc = dev_upload(&s->d_nodes8, h.nodes8.data(), h.nodes8.size(), &s->dev_bytes))) return rc;

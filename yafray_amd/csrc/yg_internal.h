@@ -98,6 +98,24 @@ struct HostBVH {
 void build_bvh(const float* verts, const uint32_t* idx, int64_t nt, const int32_t* tri_obj,
                const int32_t* tri_mat, int max_leaf, int bins, int threads, int width, HostBVH& out);
 
+struct yg_scene;
+
+// What the device builder reports: node count of the exported width, the
+// two-wide tree's depth (root = 1) and, for widths 4 and 8, the deepest
+// traversal stack the tree can need (entries - 1).
+struct DeviceBVHInfo {
+    int64_t n_nodes = 0;
+    int32_t max_depth = 0;
+    int32_t stack = 0;
+};
+
+// Builds the scene's BVH on the device (bvh_gpu.hip), kind
+// YG_BUILDER_DEVICE (binned SAH) or YG_BUILDER_DEVICE_PLOC (clustering):
+// writes s->d_nodes, d_nodes4 or d_nodes8 (width 4 or 8), d_tris, d_tprim,
+// d_tobj, d_tmat and, for smoothed scenes, d_tvn; sets s->dev_bytes.  Needs
+// at least two triangles and a scene that is not a single leaf.
+int build_bvh_device(yg_scene* s, int kind, int max_leaf, int bins, int width, DeviceBVHInfo& info);
+
 // Face normal exactly as triangle_t::recNormal computes it.
 YG_HD f3 tri_normal(f3 a, f3 b, f3 c) { return norm3(cross(b - a, c - a)); }
 

@@ -79,7 +79,12 @@ class yg_render_stats(C.Structure):
 class yg_build_opts(C.Structure):
     _fields_ = [("max_leaf", C.c_int32), ("sah_bins", C.c_int32), ("threads", C.c_int32),
                 ("width", C.c_int32), ("stack_lds", C.c_int32), ("rays_per_lane", C.c_int32),
-                ("compact_below", C.c_float)]
+                ("compact_below", C.c_float), ("builder", C.c_int32)]
+
+
+YG_BUILDER_DEVICE = 0
+YG_BUILDER_HOST = 1
+YG_BUILDER_DEVICE_PLOC = 2
 
 
 YG_TUNE_MAX = 32

@@ -131,13 +131,16 @@ class Scene:
                                 C.byref(oid)), "yg_mesh_add")
         return oid.value
 
-    def build(self, max_leaf=4, bins=32, threads=0, width=0, stack_lds=0, rays_per_lane=0, compact_below=0.0):
+    def build(self, max_leaf=4, bins=32, threads=0, width=0, stack_lds=0, rays_per_lane=0, compact_below=0.0,
+              builder=0):
         """stack_lds: stack entries per lane the render launches keep in LDS
         (0 = library default, -1 = the whole stack); rays_per_lane: rays each
         lane of the GI level launches traces (0 = 1); compact_below: survivor
         fraction below which a GI level's paths are compacted (0 = 1, always;
-        -1 = never)."""
-        o = D.yg_build_opts(max_leaf, bins, threads, width, stack_lds, rays_per_lane, compact_below)
+        -1 = never); builder: D.YG_BUILDER_DEVICE (default: binned SAH on
+        the GPU), D.YG_BUILDER_HOST (the same binned SAH on host threads) or
+        D.YG_BUILDER_DEVICE_PLOC (clustering on the GPU)."""
+        o = D.yg_build_opts(max_leaf, bins, threads, width, stack_lds, rays_per_lane, compact_below, builder)
         check(lib().yg_build(self.h, C.byref(o)), "yg_build")
 
     def info(self):
