@@ -33,7 +33,7 @@ extern "C" {
 #define YG_ERR_UNSUPPORTED (-4)
 #define YG_ERR_NOMEM (-5)
 
-#define YG_ABI_VERSION 9
+#define YG_ABI_VERSION 10
 
 /* ---------------------------------------------------------------- rays -- */
 
@@ -97,20 +97,36 @@ typedef struct yg_texture {
     float sharpness;     /* marble: 1/shp if shp > 1 else 1 (basictex.cc:125-126) */
     int32_t shape;       /* YG_SHAPE_* */
     int32_t rings;       /* wood "type" == rings */
-    float H, lacunarity, octaves, iscale; /* Musgrave fBm parameters */
+    float H, lacunarity, octaves, iscale; /* Musgrave parameters */
+    int32_t mtype;       /* Musgrave "type": YG_MUSGRAVE_* (textureMusgrave_t ctor, basictex.cc:657-672) */
+    float offset, gain;  /* Musgrave "offset", "gain" (heteroterrain, hybridmf, ridgedmf) */
 } yg_texture;
+
+enum {
+    YG_MUSGRAVE_FBM = 0,          /* fBm_t (noise.cc:638-650) */
+    YG_MUSGRAVE_MULTIFRACTAL = 1, /* mFractal_t (noise.cc:668-681) */
+    YG_MUSGRAVE_HETEROTERRAIN = 2,/* heteroTerrain_t (noise.cc:695-719) */
+    YG_MUSGRAVE_HYBRIDMF = 3,     /* hybridMFractal_t (noise.cc:731-754) */
+    YG_MUSGRAVE_RIDGEDMF = 4      /* ridgedMFractal_t (noise.cc:768-791) */
+};
 
 enum { YG_MOD_MIX = 0, YG_MOD_MUL = 1, YG_MOD_ADD = 2, YG_MOD_SUB = 3 };
 
-/* modulator_t on a procedural texture with global coordinates and
- * projection x,y,z: texpt = size * (P + ofs) (texture.cc:311-317), colour
- * blend by mode and factor "color" (texture.cc:499-534). */
+/* modulator_t on a procedural texture (modulator_t::doMapping,
+ * texture.cc:122-317, procedural branch): the hit point P (texco "global",
+ * or "orco" of an object that was never transformed, where orco = P),
+ * times the texture matrix when there is one (texpt = M * P), then per
+ * output axis a: texpt[a] = size[a] * (T[proj[a]] + ofs[a]); colour blend by
+ * mode and factor "color" (texture.cc:499-534). */
 typedef struct yg_modulator {
     int32_t tex;         /* index into the texture table */
     int32_t mode;        /* YG_MOD_* */
     float color;         /* blend factor ("color" parameter) */
     float size[3];       /* sizex, sizey, sizez (each times "size") */
     float ofs[3];
+    int32_t proj[3];     /* proj_x/y/z: 0 = the same axis, 1..3 = x, y, z, -1 = none ("n") */
+    int32_t has_mtx;     /* 0: no texture matrix (identity) */
+    float mtx[12];       /* rows 0..2 of the texture matrix m00..m23, row-major */
 } yg_modulator;
 
 /* genericShader_t in its diffuse form (basicshaders.cc:30-70,191-204):

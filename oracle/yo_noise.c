@@ -304,6 +304,95 @@ static float fbm_k(int kind, float x, float y, float z, float H, float lac, floa
     return value;
 }
 
+/* mFractal_t::operator(), noise.cc:668-681 */
+static float mfractal_k(int kind, float x, float y, float z, float H, float lac, float oct)
+{
+    float value = 1, pwr = 1, pwHL = dpow(lac, -H);
+    for (int i = 0; i < (int)oct; i++) {
+        value *= (pwr * signed_noise(kind, x, y, z) + 1.0f);
+        pwr *= pwHL;
+        x *= lac; y *= lac; z *= lac;
+    }
+    float rmd = oct - floorf(oct);
+    if (rmd != 0.0f) value *= (rmd * signed_noise(kind, x, y, z) * pwr + 1.0f);
+    return value;
+}
+
+/* heteroTerrain_t::operator(), noise.cc:695-719 */
+static float hetero_k(int kind, float x, float y, float z, float H, float lac, float oct, float offset)
+{
+    float pwHL = dpow(lac, -H), pwr = pwHL;
+    float value = offset + signed_noise(kind, x, y, z), increment;
+    x *= lac; y *= lac; z *= lac;
+    for (int i = 1; i < (int)oct; i++) {
+        increment = (signed_noise(kind, x, y, z) + offset) * pwr * value;
+        value += increment;
+        pwr *= pwHL;
+        x *= lac; y *= lac; z *= lac;
+    }
+    float rmd = oct - floorf(oct);
+    if (rmd != 0.0f) {
+        increment = (signed_noise(kind, x, y, z) + offset) * pwr * value;
+        value += rmd * increment;
+    }
+    return value;
+}
+
+/* hybridMFractal_t::operator(), noise.cc:731-754 */
+static float hybrid_k(int kind, float x, float y, float z, float H, float lac, float oct, float offset, float gain)
+{
+    float pwHL = dpow(lac, -H), pwr = pwHL;
+    float result = signed_noise(kind, x, y, z) + offset;
+    float weight = gain * result;
+    x *= lac; y *= lac; z *= lac;
+    for (int i = 1; (weight > 0.001f) && (i < (int)oct); i++) {
+        if (weight > 1.0f) weight = 1.0f;
+        float signal = (signed_noise(kind, x, y, z) + offset) * pwr;
+        pwr *= pwHL;
+        result += weight * signal;
+        weight *= gain * signal;
+        x *= lac; y *= lac; z *= lac;
+    }
+    float rmd = oct - floorf(oct);
+    if (rmd != 0.0f) result += rmd * ((signed_noise(kind, x, y, z) + offset) * pwr);
+    return result;
+}
+
+/* ridgedMFractal_t::operator(), noise.cc:768-791 */
+static float ridged_k(int kind, float x, float y, float z, float H, float lac, float oct, float offset, float gain)
+{
+    float pwHL = dpow(lac, -H), pwr = pwHL;
+    float signal = offset - fabsf(signed_noise(kind, x, y, z));
+    signal *= signal;
+    float result = signal, weight = 1.0f;
+    for (int i = 1; i < (int)oct; i++) {
+        x *= lac; y *= lac; z *= lac;
+        weight = signal * gain;
+        if (weight > 1.0f) weight = 1.0f;
+        else if (weight < 0.0f) weight = 0.0f;
+        signal = offset - fabsf(signed_noise(kind, x, y, z));
+        signal *= signal;
+        signal *= weight;
+        result += signal * pwr;
+        pwr *= pwHL;
+    }
+    return result;
+}
+
+/* the generator textureMusgrave_t's "type" selects (basictex.cc:657-672) */
+static float musgrave_k(const yg_texture* t, float x, float y, float z)
+{
+    switch (t->mtype) {
+    case YG_MUSGRAVE_MULTIFRACTAL: return mfractal_k(t->noise, x, y, z, t->H, t->lacunarity, t->octaves);
+    case YG_MUSGRAVE_HETEROTERRAIN: return hetero_k(t->noise, x, y, z, t->H, t->lacunarity, t->octaves, t->offset);
+    case YG_MUSGRAVE_HYBRIDMF:
+        return hybrid_k(t->noise, x, y, z, t->H, t->lacunarity, t->octaves, t->offset, t->gain);
+    case YG_MUSGRAVE_RIDGEDMF:
+        return ridged_k(t->noise, x, y, z, t->H, t->lacunarity, t->octaves, t->offset, t->gain);
+    default: return fbm_k(t->noise, x, y, z, t->H, t->lacunarity, t->octaves);
+    }
+}
+
 float yo_fbm(float x, float y, float z, float H, float lac, float oct)
 {
     return fbm_k(YG_NOISE_NEWPERLIN, x, y, z, H, lac, oct);
@@ -369,8 +458,8 @@ static float tex_float(const yg_texture* t, yo_v3 p)
         return w;
     }
     case YG_TEX_MUSGRAVE:
-        return t->iscale * fbm_k(t->noise, p.x * t->size, p.y * t->size, p.z * t->size, t->H, t->lacunarity,
-                                 t->octaves);
+        /* textureMusgrave_t::getFloat (basictex.cc:677-680) */
+        return t->iscale * musgrave_k(t, p.x * t->size, p.y * t->size, p.z * t->size);
     }
     return 0.0f;
 }
@@ -384,13 +473,32 @@ static yo_col tex_color(const yg_texture* t, yo_v3 p)
     return col_add(c1, col_scale(col_sub(c2, c1), f));
 }
 
-/* modulator_t::modulate colour part (texture.cc:499-534) with doMapping's
- * procedural branch for global coordinates, projections x,y,z (:311-317). */
+/* modulator_t::doMapping, procedural branch (texture.cc:122-180, 311-317):
+ * global coordinates (or orco of an untransformed mesh, which is P), the
+ * texture matrix (matrix4x4_t * point3d_t, matrix4.h:86-91) when given, then
+ * size * (T[projected axis] + ofs) per axis, or size * ofs for axis "n". */
+void yo_mod_texpt(const yg_modulator* m, yo_v3 P, float out[3])
+{
+    float T[3] = {P.x, P.y, P.z};
+    if (m->has_mtx) {
+        const float* M = m->mtx;
+        T[0] = M[0] * P.x + M[1] * P.y + M[2] * P.z + M[3];
+        T[1] = M[4] * P.x + M[5] * P.y + M[6] * P.z + M[7];
+        T[2] = M[8] * P.x + M[9] * P.y + M[10] * P.z + M[11];
+    }
+    for (int a = 0; a < 3; a++) {
+        const int pj = m->proj[a];
+        out[a] = (pj < 0) ? m->size[a] * m->ofs[a] : m->size[a] * (T[pj == 0 ? a : pj - 1] + m->ofs[a]);
+    }
+}
+
+/* modulator_t::modulate colour part (texture.cc:499-534) */
 yo_col yo_modulate(const yg_modulator* m, const yg_texture* texs, yo_col C, yo_v3 P)
 {
     const yg_texture* t = &texs[m->tex];
-    yo_v3 tp = v3(m->size[0] * (P.x + m->ofs[0]), m->size[1] * (P.y + m->ofs[1]),
-                  m->size[2] * (P.z + m->ofs[2]));
+    float q[3];
+    yo_mod_texpt(m, P, q);
+    yo_v3 tp = v3(q[0], q[1], q[2]);
     yo_col tc = tex_color(t, tp);
     switch (m->mode) {
     case YG_MOD_MIX:

@@ -9,6 +9,7 @@
 #include "yo_internal.h"
 
 yo_col yo_modulate(const yg_modulator* m, const yg_texture* texs, yo_col C, yo_v3 P);
+void yo_mod_texpt(const yg_modulator* m, yo_v3 P, float out[3]);
 yo_v3 yo_shading_normal(const yo_scene* s, int32_t prim, yo_v3 h);
 int yo_first_hit_one(const yo_scene* s, yo_v3 from, yo_v3 ray, float min_raydis, int32_t* prim,
                      float* t);
@@ -415,8 +416,8 @@ static int diffuse_d(const rctx* R, int mat, yo_v3 P, double rho[3], double dalb
     if (m->mod_count == 0) return -1;
     const yg_modulator* md = &R->mods[m->mod_first];
     const yg_texture* t = &R->texs[md->tex];
-    float tp[3] = {md->size[0] * (P.x + md->ofs[0]), md->size[1] * (P.y + md->ofs[1]),
-                   md->size[2] * (P.z + md->ofs[2])};
+    float tp[3];
+    yo_mod_texpt(md, P, tp);
     float tcf[3];
     yo_tex_color_f(t, tp, tcf);
     yo_col rc = yo_modulate(md, R->texs, col(m->color[0], m->color[1], m->color[2]), P);
@@ -1363,8 +1364,9 @@ int yo_render_adjoint(yo_scene* s, const yg_camera* cam, const yg_material* mats
                       const yg_render_opts* opts, const float* grad_rgb, float* rgb, double* gmat,
                       double* gmod, double* glight, int nthreads)
 {
-    (void)ntex;
     if (opts->aa_passes > 0) return -5; /* antialiasing passes are not differentiated */
+    for (int k = 0; k < ntex; k++)
+        if (texs[k].type == YG_TEX_MUSGRAVE && texs[k].mtype != YG_MUSGRAVE_FBM) return -7; /* fBm only */
     for (int li = 0; li < nl; li++)
         if (lights[li].type == YG_LIGHT_AREA) nthreads = 1;
     if (nthreads < 1) nthreads = 1;

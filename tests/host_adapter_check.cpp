@@ -88,6 +88,39 @@ int main()
         const int rc = yafgpu::textureFactory("clouds", tp, T);
         printf("noise %s %d %d\n", *nt ? nt : "-", rc, rc ? -1 : T.noise);
     }
+    // Musgrave type names (textureMusgrave_t ctor, basictex.cc:657-672), offset and gain
+    for (const char* mt : {"fBm", "multifractal", "heteroterrain", "hybridmf", "ridgedmf", "", "other"}) {
+        paramMap_t tp;
+        tp.setString("type", mt);
+        tp.setFloat("offset", 0.7f);
+        tp.setFloat("gain", 2.0f);
+        yg_texture T;
+        const int rc = yafgpu::textureFactory("musgrave", tp, T);
+        printf("mtype %s %d %d %g %g\n", *mt ? mt : "-", rc, T.mtype, T.offset, T.gain);
+    }
+    // modulator projection axes and texture matrix (basicshaders.cc:300-357)
+    {
+        paramMap_t sh2, m2;
+        m2.setString("texname", "mus");
+        m2.setString("texco", "global");
+        m2.setString("proj_x", "z");
+        m2.setString("proj_y", "n");
+        m2.setString("proj_z", "x");
+        m2.setFloat("m03", 0.25f);
+        m2.setFloat("m11", 2.0f);
+        std::vector<yg_modulator> out;
+        yg_material M;
+        const std::map<std::string, int> tix{{"mus", 0}};
+        const int rc = yafgpu::shaderFactory("generic", sh2, {m2}, tix, M, out);
+        printf("modmap %d %d %d %d %d %g %g %g\n", rc, out[0].proj[0], out[0].proj[1], out[0].proj[2], out[0].has_mtx,
+               out[0].mtx[3], out[0].mtx[5], out[0].mtx[0]);
+        paramMap_t m3;
+        m3.setString("texname", "mus");
+        m3.setString("texco", "global");
+        out.clear();
+        const int rc3 = yafgpu::shaderFactory("generic", sh2, {m3}, tix, M, out);
+        printf("moddef %d %d %d %d %d\n", rc3, out[0].proj[0], out[0].proj[1], out[0].proj[2], out[0].has_mtx);
+    }
     // unsupported features are refused with a status, not an exit
     paramMap_t cache;
     cache.setBool("cache", true);

@@ -102,3 +102,22 @@ def test_camera_pinhole_projection_gpu(torch_cuda):
         img, _ = _render(d, torch_cuda)
         exp, mask = CF.camera_grid_expected(**kw)
         assert np.allclose(img[mask], exp[mask], atol=1e-6)
+
+
+def test_modulator_projection_matrix_and_multifractals_gpu(torch_cuda):
+    """Projection axes (swapped, "n"), a texture matrix and each Musgrave
+    multifractal on the device equal the oracle's render."""
+    M = np.array([[0.8, -0.6, 0.0, 0.3], [0.6, 0.8, 0.0, -1.2], [0.0, 0.0, 1.0, 0.7]], np.float32)
+    for mt in ("multifractal", "heteroterrain", "hybridmf", "ridgedmf"):
+        t = D.musgrave(color1=(0.1, 0.2, 0.3), color2=(0.9, 0.8, 0.7), octaves=4.5, size=0.8, intensity=0.6,
+                       mtype=mt, offset=0.8, gain=1.5)
+        for proj, mtx in (((0, 0, 0), None), ((3, 1, 2), M), ((0, -1, 1), M)):
+            m = D.modulator(0, D.YG_MOD_MUL, color=0.9, sizexyz=(0.7, 1.3, 0.9), ofs=(0.25, -0.5, 0.125),
+                            proj=proj, mtx=mtx)
+            d = CF.textured_plane([(D.YG_MOD_MUL, 0.9)], resx=64, resy=48)
+            d.mods = D.carray(D.yg_modulator, [m])
+            d.texs = D.carray(D.yg_texture, [t])
+            img, _ = _render(d, torch_cuda)
+            ref, _ = _oracle(d).render(d)
+            assert np.allclose(img, ref, rtol=1e-5, atol=1e-6), (mt, proj)
+            assert np.mean(img == ref) > 0.99

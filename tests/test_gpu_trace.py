@@ -52,7 +52,7 @@ def _check(prim_g, t_g, u_g, v_g, prim_c, t_c, u_c, v_c):
 
 def test_library_loads(torch_cuda):
     Y = _lib(torch_cuda)
-    assert Y.lib().yg_abi_version() == 9
+    assert Y.lib().yg_abi_version() == 10
 
 
 @pytest.mark.parametrize("builder", [D.YG_BUILDER_HOST, D.YG_BUILDER_DEVICE, D.YG_BUILDER_DEVICE_PLOC])
@@ -189,7 +189,8 @@ def test_first_hit_is_shadowed(torch_cuda, width, builder):
 @pytest.mark.parametrize("what", [0, 1, 2, 3, 4])
 def test_noise_parity(torch_cuda, what, kind):
     """newPerlin_t, stdPerlin_t and blenderNoise_t (kind 0, 1, 2) through every
-    evaluation: bit-exact with the oracle."""
+    evaluation, the textures including the four Musgrave multifractals:
+    bit-exact with the oracle."""
     torch = torch_cuda
     Y = _lib(torch)
     rng = np.random.default_rng(11)
@@ -201,6 +202,8 @@ def test_noise_parity(torch_cuda, what, kind):
             D.marble(depth=2, turbulence=1.0, size=1.1, shape=2),
             D.wood(depth=2, turbulence=3.0, size=0.9, rings=True, shape=1),
             D.wood(depth=2, turbulence=0.0, size=0.9, rings=False, shape=0)]
+    texs += [D.musgrave(H=0.8, lacunarity=2.1, octaves=o, size=0.45, intensity=0.9, mtype=mt, offset=0.9, gain=1.6)
+             for mt, o in (("multifractal", 5.5), ("heteroterrain", 4.5), ("hybridmf", 6.25), ("ridgedmf", 5.0))]
     if what <= 2:
         texs = [texs[0], texs[1]]
     for tex in texs:

@@ -93,3 +93,26 @@ def test_adaptive_repetitions_are_reported(adapter_output):
     w1 = [l for l in adapter_output if l[0] == "warn1"][0]
     assert w0 == ["warn0", "0"]
     assert w1 == ["warn1", "1", "1"]
+
+
+def test_musgrave_types_offset_gain(adapter_output):
+    """The Musgrave "type" names select the generators as textureMusgrave_t's
+    ctor does (basictex.cc:657-672); any other name, or none, is fBm;
+    offset and gain are read."""
+    got = {l[1]: (int(l[2]), int(l[3]), _f(l[4]), _f(l[5])) for l in adapter_output if l[0] == "mtype"}
+    assert got["fBm"][:2] == got["-"][:2] == got["other"][:2] == (0, D.YG_MUSGRAVE_FBM)
+    assert got["multifractal"][1] == D.YG_MUSGRAVE_MULTIFRACTAL
+    assert got["heteroterrain"][1] == D.YG_MUSGRAVE_HETEROTERRAIN
+    assert got["hybridmf"][1] == D.YG_MUSGRAVE_HYBRIDMF and got["ridgedmf"][1] == D.YG_MUSGRAVE_RIDGEDMF
+    assert all(v[2] == pytest.approx(0.7) and v[3] == pytest.approx(2.0) for v in got.values())
+
+
+def test_modulator_projection_and_matrix(adapter_output):
+    """proj_x/y/z follow "nxyz".find (texture.h:166-173): z -> 3, n -> none
+    (-1), x -> 1; m03 / m11 land in the 3x4 texture matrix; without them
+    the projection is the identity and there is no matrix."""
+    mm = [l for l in adapter_output if l[0] == "modmap"][0]
+    assert [int(x) for x in mm[1:6]] == [0, 3, -1, 1, 1]
+    assert [_f(x) for x in mm[6:9]] == [0.25, 2.0, 1.0]
+    md = [l for l in adapter_output if l[0] == "moddef"][0]
+    assert [int(x) for x in md[1:6]] == [0, 1, 2, 3, 0]

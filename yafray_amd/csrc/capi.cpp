@@ -308,14 +308,19 @@ extern "C" int yg_set_shading(yg_scene* s, const yg_material* mats, uint32_t nma
         if (mats[m].mod_count < 0 || mats[m].mod_first < 0 ||
             (uint64_t)mats[m].mod_first + mats[m].mod_count > nmod)
             return yg_fail(YG_ERR_ARG, "material modulator range out of bounds");
-    for (uint32_t k = 0; k < nmod; k++)
+    for (uint32_t k = 0; k < nmod; k++) {
         if (mods[k].tex < 0 || (uint32_t)mods[k].tex >= ntex)
             return yg_fail(YG_ERR_ARG, "modulator texture index out of bounds");
+        for (int a = 0; a < 3; a++)
+            if (mods[k].proj[a] < -1 || mods[k].proj[a] > 3) return yg_fail(YG_ERR_ARG, "modulator projection out of range");
+    }
     for (uint32_t k = 0; k < ntex; k++) {
         if (texs[k].noise < YG_NOISE_NEWPERLIN || texs[k].noise > YG_NOISE_BLENDER)
             return yg_fail(YG_ERR_ARG, "bad noise generator");
         if (texs[k].type < YG_TEX_CLOUDS || texs[k].type > YG_TEX_MUSGRAVE)
             return yg_fail(YG_ERR_ARG, "bad texture type");
+        if (texs[k].type == YG_TEX_MUSGRAVE && (texs[k].mtype < YG_MUSGRAVE_FBM || texs[k].mtype > YG_MUSGRAVE_RIDGEDMF))
+            return yg_fail(YG_ERR_ARG, "bad Musgrave type");
     }
     // same table sizes: overwrite the device copies in place (parameter
     // updates of a differentiable render do this every step)

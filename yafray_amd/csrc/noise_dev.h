@@ -189,6 +189,112 @@ __device__ __forceinline__ float fbm(float x, float y, float z, float H, float l
     return fbm_w(x, y, z, dpow(lac, -H), lac, oct, perm, kind, grad);
 }
 
+// The Musgrave multifractals over the texture's noise generator, each with
+// pwHL = pow(lacunarity, -H) computed as fBm computes it.
+// mFractal_t::operator() (noise.cc:668-681)
+__device__ __forceinline__ float mfractal(float x, float y, float z, float H, float lac, float oct,
+                                          const unsigned char* perm, int kind, const float* grad)
+{
+    float value = 1, pwr = 1;
+    const float pwHL = dpow(lac, -H);
+    for (int i = 0; i < (int)oct; i++) {
+        value *= (pwr * signed_noise(x, y, z, perm, kind, grad) + 1.0f);
+        pwr *= pwHL;
+        x *= lac; y *= lac; z *= lac;
+    }
+    const float rmd = oct - floorf(oct);
+    if (rmd != 0.0f) value *= (rmd * signed_noise(x, y, z, perm, kind, grad) * pwr + 1.0f);
+    return value;
+}
+
+// heteroTerrain_t::operator() (noise.cc:695-719)
+__device__ __forceinline__ float hetero_terrain(float x, float y, float z, float H, float lac, float oct, float offset,
+                                                const unsigned char* perm, int kind, const float* grad)
+{
+    const float pwHL = dpow(lac, -H);
+    float pwr = pwHL;
+    float value = offset + signed_noise(x, y, z, perm, kind, grad);
+    x *= lac; y *= lac; z *= lac;
+    for (int i = 1; i < (int)oct; i++) {
+        const float increment = (signed_noise(x, y, z, perm, kind, grad) + offset) * pwr * value;
+        value += increment;
+        pwr *= pwHL;
+        x *= lac; y *= lac; z *= lac;
+    }
+    const float rmd = oct - floorf(oct);
+    if (rmd != 0.0f) {
+        const float increment = (signed_noise(x, y, z, perm, kind, grad) + offset) * pwr * value;
+        value += rmd * increment;
+    }
+    return value;
+}
+
+// hybridMFractal_t::operator() (noise.cc:731-754)
+__device__ __forceinline__ float hybrid_mfractal(float x, float y, float z, float H, float lac, float oct,
+                                                 float offset, float gain, const unsigned char* perm, int kind,
+                                                 const float* grad)
+{
+    const float pwHL = dpow(lac, -H);
+    float pwr = pwHL;
+    float result = signed_noise(x, y, z, perm, kind, grad) + offset;
+    float weight = gain * result;
+    x *= lac; y *= lac; z *= lac;
+    for (int i = 1; (weight > 0.001f) && (i < (int)oct); i++) {
+        if (weight > 1.0f) weight = 1.0f;
+        const float signal = (signed_noise(x, y, z, perm, kind, grad) + offset) * pwr;
+        pwr *= pwHL;
+        result += weight * signal;
+        weight *= gain * signal;
+        x *= lac; y *= lac; z *= lac;
+    }
+    const float rmd = oct - floorf(oct);
+    if (rmd != 0.0f) result += rmd * ((signed_noise(x, y, z, perm, kind, grad) + offset) * pwr);
+    return result;
+}
+
+// ridgedMFractal_t::operator() (noise.cc:768-791)
+__device__ __forceinline__ float ridged_mfractal(float x, float y, float z, float H, float lac, float oct,
+                                                 float offset, float gain, const unsigned char* perm, int kind,
+                                                 const float* grad)
+{
+    const float pwHL = dpow(lac, -H);
+    float pwr = pwHL;
+    float signal = offset - fabsf(signed_noise(x, y, z, perm, kind, grad));
+    signal *= signal;
+    float result = signal;
+    float weight = 1.0f;
+    for (int i = 1; i < (int)oct; i++) {
+        x *= lac; y *= lac; z *= lac;
+        weight = signal * gain;
+        if (weight > 1.0f) weight = 1.0f;
+        else if (weight < 0.0f) weight = 0.0f;
+        signal = offset - fabsf(signed_noise(x, y, z, perm, kind, grad));
+        signal *= signal;
+        signal *= weight;
+        result += signal * pwr;
+        pwr *= pwHL;
+    }
+    return result;
+}
+
+// the generator textureMusgrave_t's "type" selects (basictex.cc:657-672)
+__device__ __forceinline__ float musgrave(const yg_texture& t, float x, float y, float z, const unsigned char* perm,
+                                          const float* grad)
+{
+    switch (t.mtype) {
+    case YG_MUSGRAVE_MULTIFRACTAL:
+        return mfractal(x, y, z, t.H, t.lacunarity, t.octaves, perm, t.noise, grad);
+    case YG_MUSGRAVE_HETEROTERRAIN:
+        return hetero_terrain(x, y, z, t.H, t.lacunarity, t.octaves, t.offset, perm, t.noise, grad);
+    case YG_MUSGRAVE_HYBRIDMF:
+        return hybrid_mfractal(x, y, z, t.H, t.lacunarity, t.octaves, t.offset, t.gain, perm, t.noise, grad);
+    case YG_MUSGRAVE_RIDGEDMF:
+        return ridged_mfractal(x, y, z, t.H, t.lacunarity, t.octaves, t.offset, t.gain, perm, t.noise, grad);
+    default:
+        return fbm(x, y, z, t.H, t.lacunarity, t.octaves, perm, t.noise, grad);
+    }
+}
+
 // turbulence (noise.cc:808-819) over the texture's noise generator
 __device__ __forceinline__ float turbulence(float x, float y, float z, int oct, float size, int hard,
                                             const unsigned char* perm = YG_PERM, int kind = YG_NOISE_NEWPERLIN,
@@ -247,7 +353,8 @@ __device__ __forceinline__ float tex_float(const yg_texture& t, f3 p, const unsi
         return w;
     }
     case YG_TEX_MUSGRAVE:
-        return t.iscale * fbm(p.x * t.size, p.y * t.size, p.z * t.size, t.H, t.lacunarity, t.octaves, perm, t.noise, grad);
+        // textureMusgrave_t::getFloat (basictex.cc:677-680): iscale * mGen(p * size)
+        return t.iscale * musgrave(t, p.x * t.size, p.y * t.size, p.z * t.size, perm, grad);
     }
     return 0.0f;
 }
@@ -262,12 +369,31 @@ __device__ __forceinline__ c3 tex_color(const yg_texture& t, f3 p, const unsigne
     return cadd(c1, cscale(csub(c2, c1), f));
 }
 
-// modulator_t::modulate colour part (texture.cc:499-534) with procedural
-// global mapping texpt = size*(P + ofs) (texture.cc:311-317)
+// modulator_t::doMapping for a procedural texture (texture.cc:122-180 and
+// 311-317): the texture matrix (matrix4x4_t * point3d_t, matrix4.h:86-91),
+// then per axis size * (T[projected axis] + ofs), or size * ofs for "n"
+__device__ __forceinline__ f3 mod_texpt(const yg_modulator& m, f3 P)
+{
+    float T[3] = {P.x, P.y, P.z};
+    if (m.has_mtx) {
+        const float* M = m.mtx;
+        T[0] = M[0] * P.x + M[1] * P.y + M[2] * P.z + M[3];
+        T[1] = M[4] * P.x + M[5] * P.y + M[6] * P.z + M[7];
+        T[2] = M[8] * P.x + M[9] * P.y + M[10] * P.z + M[11];
+    }
+    float o[3];
+    for (int a = 0; a < 3; a++) {
+        const int pj = m.proj[a];
+        o[a] = (pj < 0) ? m.size[a] * m.ofs[a] : m.size[a] * (T[pj == 0 ? a : pj - 1] + m.ofs[a]);
+    }
+    return mk3(o[0], o[1], o[2]);
+}
+
+// modulator_t::modulate colour part (texture.cc:499-534)
 __device__ __forceinline__ c3 modulate(const yg_modulator& m, const yg_texture* texs, c3 C, f3 P)
 {
     const yg_texture& t = texs[m.tex];
-    f3 tp = mk3(m.size[0] * (P.x + m.ofs[0]), m.size[1] * (P.y + m.ofs[1]), m.size[2] * (P.z + m.ofs[2]));
+    const f3 tp = mod_texpt(m, P);
     c3 tc = tex_color(t, tp);
     switch (m.mode) {
     case YG_MOD_MIX:
@@ -403,7 +529,7 @@ __device__ __forceinline__ RhoD diffuse_d(const yg_material* mats, const yg_modu
     const float A = md.color;
     if (!(A > 0)) return r;
     const yg_texture& t = texs[md.tex];
-    const f3 tp = mk3(md.size[0] * (P.x + md.ofs[0]), md.size[1] * (P.y + md.ofs[1]), md.size[2] * (P.z + md.ofs[2]));
+    const f3 tp = mod_texpt(md, P);
     const c3 tc = tex_color(t, tp);
     const c3 c21 = mkc(t.color2[0] - t.color1[0], t.color2[1] - t.color1[1], t.color2[2] - t.color1[2]);
     const float dF = dot(tex_float_grad(t, tp), tp);  // d getFloat / d lambda

@@ -2364,9 +2364,12 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
     if (nsun > 1) return yg_fail(YG_ERR_UNSUPPORTED, "adjoint supports one sun light");
     for (const auto& m : s->mats)
         if (m.mod_count > 1) return yg_fail(YG_ERR_UNSUPPORTED, "adjoint supports one modulator per material");
-    for (const auto& t : s->texs)
+    for (const auto& t : s->texs) {
         if (t.noise != YG_NOISE_NEWPERLIN)
             return yg_fail(YG_ERR_UNSUPPORTED, "adjoint differentiates newPerlin_t textures only");
+        if (t.type == YG_TEX_MUSGRAVE && t.mtype != YG_MUSGRAVE_FBM)
+            return yg_fail(YG_ERR_UNSUPPORTED, "adjoint differentiates Musgrave fBm only, not the multifractals");
+    }
     if (!s->mods.empty() && !d_grad_mod) return yg_fail(YG_ERR_ARG, "d_grad_mod required");
     hipStream_t st = (hipStream_t)stream;
     std::vector<ScenePlan> plans(1);

@@ -34,12 +34,20 @@ class yg_texture(C.Structure):
                 ("depth", C.c_int32), ("size", C.c_float), ("hard", C.c_int32),
                 ("bias", C.c_int32), ("turb", C.c_float), ("sharpness", C.c_float),
                 ("shape", C.c_int32), ("rings", C.c_int32), ("H", C.c_float),
-                ("lacunarity", C.c_float), ("octaves", C.c_float), ("iscale", C.c_float)]
+                ("lacunarity", C.c_float), ("octaves", C.c_float), ("iscale", C.c_float),
+                ("mtype", C.c_int32), ("offset", C.c_float), ("gain", C.c_float)]
+
+
+YG_MUSGRAVE_FBM, YG_MUSGRAVE_MULTIFRACTAL, YG_MUSGRAVE_HETEROTERRAIN, YG_MUSGRAVE_HYBRIDMF, YG_MUSGRAVE_RIDGEDMF = \
+    0, 1, 2, 3, 4
+MUSGRAVE_TYPES = {"fBm": YG_MUSGRAVE_FBM, "multifractal": YG_MUSGRAVE_MULTIFRACTAL,
+                  "heteroterrain": YG_MUSGRAVE_HETEROTERRAIN, "hybridmf": YG_MUSGRAVE_HYBRIDMF,
+                  "ridgedmf": YG_MUSGRAVE_RIDGEDMF}
 
 
 class yg_modulator(C.Structure):
     _fields_ = [("tex", C.c_int32), ("mode", C.c_int32), ("color", C.c_float), ("size", F3),
-                ("ofs", F3)]
+                ("ofs", F3), ("proj", C.c_int32 * 3), ("has_mtx", C.c_int32), ("mtx", C.c_float * 12)]
 
 
 class yg_material(C.Structure):
@@ -185,11 +193,17 @@ def photonlight(frm=(0, 0, 1), to=(0, 0, 0), color=(1, 1, 1), power=1.0, angle=4
 
 
 def musgrave(color1=(0, 0, 0), color2=(1, 1, 1), H=1.0, lacunarity=2.0, octaves=2.0, size=1.0,
-             intensity=1.0):
+             intensity=1.0, mtype="fBm", offset=1.0, gain=1.0, noise=None):
+    """textureMusgrave_t (basictex.cc:657-709); mtype: fBm, multifractal,
+    heteroterrain, hybridmf or ridgedmf."""
     T = yg_texture()
     T.type = YG_TEX_MUSGRAVE
+    if noise is not None:
+        T.noise = noise
     T.color1, T.color2 = _f3(color1), _f3(color2)
     T.H, T.lacunarity, T.octaves, T.size, T.iscale = H, lacunarity, octaves, size, intensity
+    T.mtype = MUSGRAVE_TYPES[mtype] if isinstance(mtype, str) else int(mtype)
+    T.offset, T.gain = offset, gain
     return T
 
 
@@ -220,7 +234,10 @@ def wood(color1=(0, 0, 0), color2=(1, 1, 1), depth=2, turbulence=1.0, size=1.0, 
     return T
 
 
-def modulator(tex, mode=YG_MOD_MIX, color=0.0, size=1.0, sizexyz=(1, 1, 1), ofs=(0, 0, 0)):
+def modulator(tex, mode=YG_MOD_MIX, color=0.0, size=1.0, sizexyz=(1, 1, 1), ofs=(0, 0, 0), proj=(0, 0, 0),
+              mtx=None):
+    """proj: per output axis 0 = same axis, 1..3 = x/y/z, -1 = none ("n");
+    mtx: 3x4 (or 4x4) texture matrix applied to the hit point, or None."""
     M = yg_modulator()
     M.tex = tex
     M.mode = mode
@@ -228,6 +245,13 @@ def modulator(tex, mode=YG_MOD_MIX, color=0.0, size=1.0, sizexyz=(1, 1, 1), ofs=
     s = (size, size, size) if size != 1.0 else sizexyz
     M.size = _f3(s)
     M.ofs = _f3(ofs)
+    for a in range(3):
+        M.proj[a] = int(proj[a])
+    if mtx is not None:
+        m = np.asarray(mtx, np.float32).reshape(-1)[:12]
+        M.has_mtx = 1
+        for k in range(12):
+            M.mtx[k] = float(m[k])
     return M
 
 

@@ -219,20 +219,28 @@ int textureFactory(const std::string& type, const paramMap_t& p, yg_texture& T)
         return YG_OK;
     }
     if (type == "musgrave") {
-        // textureMusgrave_t::factory (basictex.cc:687-709), fBm only
+        // textureMusgrave_t::factory (basictex.cc:687-709) and its ctor's
+        // choice of generator (:657-672; any other name is fBm)
         T.type = YG_TEX_MUSGRAVE;
         T.H = 1;
         T.lacunarity = 2;
         T.octaves = 2;
         T.size = 1;
         T.iscale = 1;
+        T.offset = 1;
+        T.gain = 1;
         std::string mtype;
         p.getParam("type", mtype);
-        if (!mtype.empty() && mtype != "fBm")
-            return fail(YG_ERR_UNSUPPORTED, "musgrave type '" + mtype + "' is not on the GPU path");
+        if (mtype == "multifractal") T.mtype = YG_MUSGRAVE_MULTIFRACTAL;
+        else if (mtype == "heteroterrain") T.mtype = YG_MUSGRAVE_HETEROTERRAIN;
+        else if (mtype == "hybridmf") T.mtype = YG_MUSGRAVE_HYBRIDMF;
+        else if (mtype == "ridgedmf") T.mtype = YG_MUSGRAVE_RIDGEDMF;
+        else T.mtype = YG_MUSGRAVE_FBM;
         p.getParam("H", T.H);
         p.getParam("lacunarity", T.lacunarity);
         p.getParam("octaves", T.octaves);
+        p.getParam("offset", T.offset);
+        p.getParam("gain", T.gain);
         p.getParam("size", T.size);
         p.getParam("intensity", T.iscale);
         return YG_OK;
@@ -280,6 +288,26 @@ int shaderFactory(const std::string& type, const paramMap_t& p, const std::vecto
         mp.getParam("texco", texco);
         if (texco != "global")
             return fail(YG_ERR_UNSUPPORTED, "modulator texco '" + texco + "' is not on the GPU path (use global)");
+        // string2texprojection (texture.h:166-173): "nxyz".find(axis), 'n' or
+        // an unknown name = no axis
+        const char* defaults[3] = {"x", "y", "z"};
+        const char* keys[3] = {"proj_x", "proj_y", "proj_z"};
+        for (int a = 0; a < 3; a++) {
+            std::string ax = defaults[a];
+            mp.getParam(keys[a], ax);
+            const size_t k = std::string("nxyz").find(ax);
+            m.proj[a] = (k == std::string::npos || k == 0) ? -1 : (int32_t)k;
+        }
+        // the texture matrix m00..m23 (basicshaders.cc:317-333; row 3 does
+        // not enter matrix4x4_t * point3d_t)
+        float mtx[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 4; c++) mp.getParam("m" + std::to_string(r) + std::to_string(c), mtx[4 * r + c]);
+        const float ident[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+        if (memcmp(mtx, ident, sizeof(mtx)) != 0) {
+            m.has_mtx = 1;
+            memcpy(m.mtx, mtx, sizeof(mtx));
+        }
         set3(m.size, sx, sy, sz);
         if (size != 1.0f) set3(m.size, size, size, size);
         m.color = col;
