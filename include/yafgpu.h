@@ -33,7 +33,7 @@ extern "C" {
 #define YG_ERR_UNSUPPORTED (-4)
 #define YG_ERR_NOMEM (-5)
 
-#define YG_ABI_VERSION 10
+#define YG_ABI_VERSION 11
 
 /* ---------------------------------------------------------------- rays -- */
 
@@ -141,7 +141,10 @@ enum {
     YG_LIGHT_SUN = 1,    /* sunLight_t, sunlight.cc:27-57 */
     YG_LIGHT_AREA = 2,   /* areaLight_t, arealight.cc:31-174 */
     YG_LIGHT_PATH = 3,   /* pathLight_t full-GI, non-cached (pathlight.cc:107-403) */
-    YG_LIGHT_PHOTON = 4  /* photonLight_t, diffuse mode (photonlight.cc:33-501) */
+    YG_LIGHT_PHOTON = 4, /* photonLight_t, diffuse mode (photonlight.cc:33-501) */
+    YG_LIGHT_GLOBALPHOTON = 5 /* globalPhotonLight_t (globalphotonlight.cc:19-287): builds the
+                                 irradiance map the pathlight's path vertices read
+                                 (pathlight.cc:140-185); photons from "dummy" area lights */
 };
 
 /* One light; lights are summed in the order given, which must be the
@@ -170,8 +173,9 @@ typedef struct yg_light {
     int32_t search;          /* K */
     int32_t mindepth;
     float bias;
-    float fixedradius;
+    float fixedradius;       /* photon "fixedradius"; global photon "radius" */
     float cluster;
+    int32_t dummy;           /* area "dummy": shoots global photons only, no direct light */
 } yg_light;
 
 /* Render parameters (interface_impl.cc:480-644 / scene.cc:50-60). */
@@ -469,6 +473,23 @@ int yg_photon_read(yg_scene* s, float* d_out, uint64_t capacity, uint64_t* count
 int yg_photon_gather(yg_scene* s, const float* d_p, const float* d_n, const float* d_ng,
                      const float* d_eye, const int32_t* d_mat, uint32_t n, float* d_rgb,
                      void* stream);
+
+/* globalPhotonLight_t::init (globalphotonlight.cc:234-287) for light index li:
+ * photons from every dummy area light (quadEmitter_t, arealight.cc:200-232),
+ * traced with diffuse bounces up to "depth" (globalphotonlight.cc:19-95),
+ * merged per hash box (storeInHash, :177-195), then the irradiance of every
+ * box from its "search" nearest box photons within "radius"
+ * (computeIrradiances / setIrradiance, :127-175 and :197-228).  The photon
+ * random numbers come from disjoint segments of the ourRandom() sequence,
+ * one per photon (the reference draws them one photon after another).
+ * stats: emitted photons, stored photons, boxes. */
+int yg_gphoton_build(yg_scene* s, uint32_t li, const yg_render_opts* opts, yg_photon_stats* stats,
+                     void* stream);
+
+/* Copy the irradiance boxes (15 floats each: position, normal N, irradiance,
+ * merged photon direction, merged photon colour; in box order x, y, z) to
+ * d_out (capacity in boxes). */
+int yg_gphoton_read(yg_scene* s, float* d_out, uint64_t capacity, uint64_t* count, void* stream);
 
 #ifdef __cplusplus
 }

@@ -116,6 +116,16 @@ def _load(path):
         L.yo_photon_read.argtypes = [vp, vp, C.c_int64]
         L.yo_photon_gather.argtypes = [vp, vp, vp, vp, vp, F32P, F32P, F32P, F32P, I32P,
                                        C.c_int64, F32P]
+        L.yo_gphoton_build.restype = C.c_int
+        L.yo_gphoton_build.argtypes = [vp, vp, C.c_int, vp, vp, vp, C.c_int, vp, C.c_int, vp, C.c_int,
+                                       C.c_int, U64P]
+        L.yo_gphoton_read.restype = C.c_int64
+        L.yo_gphoton_read.argtypes = [vp, vp, C.c_int64]
+        L.yo_gphoton_load.restype = C.c_int
+        L.yo_gphoton_load.argtypes = [vp, F32P, C.c_int64, vp, C.c_int, C.c_int, C.c_int]
+        L.yo_rgbe_roundtrip.argtypes = [F32P, F32P, C.c_int64]
+        L.yo_dir_roundtrip.argtypes = [F32P, F32P, vp, C.c_int64]
+        L.yo_dirconv_tables.argtypes = [F32P]
     return L
 
 
@@ -317,6 +327,35 @@ class OracleScene:
         L.yo_photon_read(self.h, _ptr(out), n)
         return out[: n * 9].reshape(n, 9)
 
+    def gphoton_build(self, desc, li, stateless=False):
+        """globalPhotonLight_t::init for light li (serial, from opts.lcg_seed);
+        returns [emitted, stored, boxes, rays, draws]."""
+        L = lib(self.fast)
+        stats = np.zeros(5, np.uint64)
+        rc = L.yo_gphoton_build(self.h, _addr(desc.mats), len(desc.mats), _addr(desc.mods), _addr(desc.texs),
+                                _addr(desc.lights), len(desc.lights), _ptr(desc.obj_flags), len(desc.obj_flags),
+                                C.addressof(desc.opts), li, int(stateless), stats)
+        if rc != 0:
+            raise RuntimeError("yo_gphoton_build failed: %d" % rc)
+        return stats
+
+    def gphoton_read(self):
+        """[n, 15]: position, N, irradiance, merged direction, merged colour per box."""
+        L = lib(self.fast)
+        n = L.yo_gphoton_read(self.h, None, 0)
+        out = np.zeros(max(n, 1) * 15, np.float32)
+        L.yo_gphoton_read(self.h, _ptr(out), n)
+        return out[: n * 15].reshape(n, 15)
+
+    def gphoton_load(self, desc, li, boxes, stateless=True, recompute=False):
+        """Use `boxes` ([n, 15], as gphoton_read gives them) as light li's
+        irradiance boxes; recompute: pre-gather their irradiance here."""
+        b = np.ascontiguousarray(boxes, np.float32).reshape(-1)
+        rc = lib(self.fast).yo_gphoton_load(self.h, b, len(b) // 15, C.addressof(desc.lights[li]), li,
+                                            int(stateless), int(recompute))
+        if rc != 0:
+            raise RuntimeError("yo_gphoton_load failed")
+
     def photon_gather(self, desc, li, p, n, ng, eye, mat):
         L = lib(self.fast)
         cnt = len(mat)
@@ -396,3 +435,25 @@ def heightfield(nx, ny, ex, ey, amp, freq, H, lac, octv):
     out = np.empty(nx * ny * 3, np.float32)
     lib().yo_gen_heightfield(nx, ny, ex, ey, amp, freq, H, lac, octv, out)
     return out.reshape(-1, 3)
+
+
+def rgbe_roundtrip(c):
+    c = np.ascontiguousarray(c, np.float32).reshape(-1)
+    out = np.zeros_like(c)
+    lib().yo_rgbe_roundtrip(c, out, len(c) // 3)
+    return out.reshape(-1, 3)
+
+
+def dir_roundtrip(d):
+    """storedPhoton_t direction quantisation: (decoded [n,3], codes [n,2] theta/phi)."""
+    d = np.ascontiguousarray(d, np.float32).reshape(-1)
+    out = np.zeros_like(d)
+    code = np.zeros(len(d) // 3 * 2, np.int32)
+    lib().yo_dir_roundtrip(d, out, code.ctypes.data_as(C.c_void_p), len(d) // 3)
+    return out.reshape(-1, 3), code.reshape(-1, 2)
+
+
+def dirconv_tables():
+    t = np.zeros(1022, np.float32)
+    lib().yo_dirconv_tables(t)
+    return t[:255], t[255:510], t[510:766], t[766:]

@@ -157,6 +157,22 @@ int64_t yo_photon_read(yo_scene* s, float* out, int64_t cap);
  * the GPU) as photon light L's map, with its gather grid, so that renders
  * and gathers use it instead of a map yo_photon_build made. */
 int yo_photon_load(yo_scene* s, const float* data, int64_t n, const yg_light* L);
+
+/* globalPhotonLight_t::init (globalphotonlight.cc:234-278) for light li, one
+ * photon after another from the ourRandom() stream at opts->lcg_seed; a
+ * later yo_render continues that stream.  stateless: every gather starts at
+ * the light's radius (no radius carried between calls).  stats: emitted,
+ * stored, boxes, rays, draws. */
+int yo_gphoton_build(yo_scene* s, const yg_material* mats, int nmat, const yg_modulator* mods,
+                     const yg_texture* texs, const yg_light* lights, int nl, const int32_t* obj_flags,
+                     int nobj, const yg_render_opts* opts, int li, int stateless, uint64_t* stats);
+/* boxes: 15 floats each (position, N, irradiance, merged direction, merged colour) */
+int64_t yo_gphoton_read(yo_scene* s, float* out, int64_t cap);
+int yo_gphoton_load(yo_scene* s, const float* data, int64_t n, const yg_light* GL, int li, int stateless,
+                    int recompute);
+void yo_rgbe_roundtrip(const float* in, float* out, int64_t n);
+void yo_dir_roundtrip(const float* in, float* out, int32_t* code, int64_t n);
+void yo_dirconv_tables(float* out);
 /* photonLight_t::illuminate at explicit shading points (see yg_photon_gather) */
 void yo_photon_gather(yo_scene* s, const yg_material* mats, const yg_modulator* mods,
                       const yg_texture* texs, const yg_light* light, const float* p,

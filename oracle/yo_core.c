@@ -364,6 +364,7 @@ void yo_scene_free(yo_scene* s)
     free(s->verts); free(s->idx); free(s->tri_obj); free(s->tri_mat); free(s->tri_n);
     free(s->nodes); free(s->order);
     free(s->pm.data); free(s->pm.gstart); free(s->pm.gitems);
+    free(s->gm.box); free(s->gm.gstart); free(s->gm.gitems);
     free(s);
 }
 

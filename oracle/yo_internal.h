@@ -136,6 +136,36 @@ typedef struct {
     int np_emit;           /* Np of the light */
 } yo_photonmap;
 
+/* globalPhotonLight_t's hash boxes after pre-gathering (globalphotonlight.cc
+ * :177-228): one record per box, boxes sorted by (bx, by, bz) */
+typedef struct {
+    int32_t bx, by, bz;
+    float pos[3];          /* the box photon's position (the first stored) */
+    uint8_t th, ph;        /* its quantised direction (th 255: none) */
+    uint8_t rgbe[4];       /* its RGBE colour */
+    float N[3];            /* normal of the first stored photon */
+    float irr[3];          /* irradiance (setIrradiance) */
+} yo_gbox;
+
+typedef struct {
+    int built;
+    int light;             /* light index of the global photon light */
+    float cell;            /* hash box size: radius / sqrt(search) */
+    float radius;          /* maxradius of the photon and irradiance maps */
+    int64_t n;             /* boxes */
+    yo_gbox* box;
+    int64_t stored, emitted;
+    int stateless;         /* gathers start at radius each time (no per-call radius state) */
+    int32_t lcg_after;     /* ourRandom() state after the photon pass (render continues it) */
+    int own_stream;        /* the pass ran here (1) or the boxes were loaded (0) */
+    /* uniform grid (cell = radius) over the box positions for the gathers */
+    float gmin[3];
+    int gdim[3];
+    int64_t* gstart;
+    int64_t* gitems;
+    float irr_radius;      /* pathLight_t's photonData_t radius state (pathlight.cc:131) */
+} yo_gmap;
+
 struct yo_scene {
     int64_t nv, nt;
     float* verts;
@@ -147,6 +177,7 @@ struct yo_scene {
     int64_t nnodes;
     int32_t* order;
     yo_photonmap pm;
+    yo_gmap gm;
     float* tri_vn;         /* autoSmooth vertex normals, 9 per triangle (NULL: none) */
     uint8_t* tri_smooth;   /* per triangle: has vertex normals */
     /* object flags of the render in progress (bit0 cast shadows), or NULL */

@@ -64,6 +64,8 @@ typedef struct {
     double* glight;        /* nl */
     struct adj_path_s* ap; /* the current pixel's recorded path samples */
     int nap, apcap;
+    float gm_radius;       /* photonData_t::radius of this render thread (pathlight.cc:131) */
+    int gp;                /* the scene has a global photon light whose map is built */
 } rctx;
 
 static void adj_path_sample_begin(rctx* R);
@@ -247,6 +249,7 @@ static yo_col area_illuminate(rctx* R, int li, const yo_sp* sp, yo_v3 eye)
 
 static yo_col photon_illuminate(rctx* R, int li, const yo_sp* sp, yo_v3 eye);
 static yo_col path_illuminate(rctx* R, int li, const yo_sp* sp, yo_v3 eye);
+static yo_col gm_irradiance(rctx* R, const yo_sp* sp, yo_v3 N);
 
 /* scene_t::light, scene.cc:260-277 (fromWorld of a diffuse shader is 0) */
 static yo_col scene_light(rctx* R, const yo_sp* sp, yo_v3 from, int indirect)
@@ -259,7 +262,10 @@ static yo_col scene_light(rctx* R, const yo_sp* sp, yo_v3 from, int indirect)
         if (indirect && !L->use_in_indirect) continue;
         switch (L->type) {
         case YG_LIGHT_SUN: fl = col_add(fl, sun_illuminate(R, li, sp, eye)); break;
-        case YG_LIGHT_AREA: fl = col_add(fl, area_illuminate(R, li, sp, eye)); break;
+        case YG_LIGHT_AREA:
+            /* a dummy area light only shoots global photons (arealight.cc:98) */
+            if (!L->dummy) fl = col_add(fl, area_illuminate(R, li, sp, eye));
+            break;
         case YG_LIGHT_PATH: fl = col_add(fl, path_illuminate(R, li, sp, eye)); break;
         case YG_LIGHT_PHOTON: fl = col_add(fl, photon_illuminate(R, li, sp, eye)); break;
         }
@@ -339,6 +345,12 @@ static yo_col path_take_sample(rctx* R, int li, yo_v3 N, const yo_sp* sp)
                     adj_path_vertex(R, 1, &t, lf, lfp);
                 }
                 yo_col light = scene_light(R, &t, v3_add(t.P, eye), 1);
+                if (R->gp) {
+                    /* getLight with the irradiance map (pathlight.cc:140-185):
+                     * irradiance x diffuse, then the lights */
+                    const yo_v3 Nv = face_forward(t.Ng, t.N, eye);
+                    light = col_add(col_mul(gm_irradiance(R, &t, Nv), get_diffuse(R, &t)), light);
+                }
                 yo_col contri = col_mul(col_scale(raycolor, v3_dot(startray, N)), light);
                 total = col_add(total, contri);
                 if (first) subtotal[i % 4] = col_add(subtotal[i % 4], contri);
@@ -684,6 +696,13 @@ static void rctx_init(rctx* R, yo_scene* s, const yg_material* mats, int nmat,
     R->lights = lights; R->nl = nl; R->obj_flags = obj_flags; R->nobj = nobj;
     R->opts = *opts;
     R->lcg = (int32_t)opts->lcg_seed;
+    /* the global photon pass ran in init, before the render: the render's
+     * ourRandom() draws continue after the photons' (when the pass ran here) */
+    for (int li = 0; li < nl; li++)
+        if (lights[li].type == YG_LIGHT_GLOBALPHOTON && s->gm.built && s->gm.own_stream) R->lcg = s->gm.lcg_after;
+    R->gm_radius = s->gm.radius;
+    for (int li = 0; li < nl; li++)
+        if (lights[li].type == YG_LIGHT_GLOBALPHOTON && s->gm.built) R->gp = 1;
     R->areas = (yo_area*)calloc((size_t)(nl > 0 ? nl : 1), sizeof(yo_area));
     R->sun_dir = (yo_v3*)calloc((size_t)(nl > 0 ? nl : 1), sizeof(yo_v3));
     R->sun_col = (yo_col*)calloc((size_t)(nl > 0 ? nl : 1), sizeof(yo_col));
@@ -985,6 +1004,576 @@ void yo_photon_gather(yo_scene* s, const yg_material* mats, const yg_modulator* 
 }
 
 /* ------------------------------------------------------------- render -- */
+
+
+/* ------------------------------------------------- global photon map -- */
+
+/* dirConverter_t (photon.cc:11-25, photon.h:14-37): direction <-> (theta,
+ * phi) bytes.  Table entries: the float angle's cosine and sine computed in
+ * double and rounded to float. */
+static float g_costh[255], g_sinth[255], g_cosph[256], g_sinph[256];
+static pthread_once_t g_dirconv_once = PTHREAD_ONCE_INIT;
+static void dirconv_fill(void)
+{
+    for (int i = 0; i < 255; ++i) {
+        const float angle = (float)((double)(float)i * (1.0 / 255.0) * M_PI);
+        g_costh[i] = (float)cos((double)angle);
+        g_sinth[i] = (float)sin((double)angle);
+    }
+    for (int i = 0; i < 256; ++i) {
+        const float angle = (float)((double)(float)i * (1.0 / 256.0) * 2.0 * M_PI);
+        g_cosph[i] = (float)cos((double)angle);
+        g_sinph[i] = (float)sin((double)angle);
+    }
+}
+void yo_dirconv_tables(float* out)  /* costheta[255] sintheta[255] cosphi[256] sinphi[256] */
+{
+    pthread_once(&g_dirconv_once, dirconv_fill);
+    memcpy(out, g_costh, sizeof(g_costh));
+    memcpy(out + 255, g_sinth, sizeof(g_sinth));
+    memcpy(out + 510, g_cosph, sizeof(g_cosph));
+    memcpy(out + 766, g_sinph, sizeof(g_sinph));
+}
+/* storedPhoton_t::direction(dir) (photon.h:108-117) with convert (:24-34) */
+static void dir_enc(yo_v3 d, uint8_t* th, uint8_t* ph)
+{
+    if (d.x == 0 && d.y == 0 && d.z == 0) { *th = 255; *ph = 0; return; }
+    int t = (int)(acos((double)d.z) * (255.0 / M_PI));
+    int p = (int)(atan2((double)d.y, (double)d.x) * (256.0 / (2.0 * M_PI)));
+    if (t > 254) t = 254;
+    else if (t < 0) t = 0;
+    if (p > 255) p = 255;
+    else if (p < 0) p += 256;
+    *th = (uint8_t)t;
+    *ph = (uint8_t)p;
+}
+static yo_v3 dir_dec(uint8_t th, uint8_t ph)
+{
+    pthread_once(&g_dirconv_once, dirconv_fill);
+    if (th == 255) return v3(0, 0, 0);
+    return v3(g_sinth[th] * g_cosph[ph], g_sinth[th] * g_sinph[ph], g_costh[th]);
+}
+/* rgbe_t (color.cc:140-155, color.h:154-170) */
+static void rgbe_enc(yo_col c, uint8_t* e4)
+{
+    float v = c.r;
+    if (c.g > v) v = c.g;
+    if (c.b > v) v = c.b;
+    if (v < 1e-32f) { e4[0] = e4[1] = e4[2] = e4[3] = 0; return; }
+    int e;
+    v = (float)((double)frexpf(v, &e) * 256.0 / (double)v);
+    e4[0] = (uint8_t)(c.r * v);
+    e4[1] = (uint8_t)(c.g * v);
+    e4[2] = (uint8_t)(c.b * v);
+    e4[3] = (uint8_t)(e + 128);
+}
+static yo_col rgbe_dec(const uint8_t* e4)
+{
+    if (!e4[3]) return col(0, 0, 0);
+    const float f = (float)ldexp(1.0, (int)e4[3] - (int)(128 + 8));
+    return col((float)e4[0] * f, (float)e4[1] * f, (float)e4[2] * f);
+}
+void yo_rgbe_roundtrip(const float* in, float* out, int64_t n)
+{
+    for (int64_t i = 0; i < n; i++) {
+        uint8_t e[4];
+        rgbe_enc(col(in[3 * i], in[3 * i + 1], in[3 * i + 2]), e);
+        const yo_col c = rgbe_dec(e);
+        out[3 * i] = c.r; out[3 * i + 1] = c.g; out[3 * i + 2] = c.b;
+    }
+}
+void yo_dir_roundtrip(const float* in, float* out, int32_t* code, int64_t n)
+{
+    for (int64_t i = 0; i < n; i++) {
+        uint8_t th, ph;
+        dir_enc(v3(in[3 * i], in[3 * i + 1], in[3 * i + 2]), &th, &ph);
+        const yo_v3 d = dir_dec(th, ph);
+        out[3 * i] = d.x; out[3 * i + 1] = d.y; out[3 * i + 2] = d.z;
+        if (code) { code[2 * i] = th; code[2 * i + 1] = ph; }
+    }
+}
+
+/* HemiVec_CONE (globalphotonlight.cc:10-16, arealight.cc:219-225) */
+static yo_v3 hemivec_cone(yo_v3 nrm, yo_v3 Ru, yo_v3 Rv, float cosang, float z1, float z2)
+{
+    const float t1 = (float)(2.0 * M_PI * (double)z1);
+    const float t2 = (float)(1.0 - (1.0 - (double)cosang) * (double)z2);
+    const yo_v3 a = v3_add(v3_mul(Ru, dcos(t1)), v3_mul(Rv, dsin(t1)));
+    return v3_add(v3_mul(a, (float)sqrt(1.0 - (double)(t2 * t2))), v3_mul(nrm, t2));
+}
+
+typedef struct {
+    yo_v3 corner, toX, toY, direction, NU, NV;
+    yo_col scolor;
+} yo_qemit;
+
+/* areaLight_t::getEmitter + quadEmitter_t ctor / numSamples (arealight.h:90-95,
+ * arealight.cc:200-217): colour = colour * power * |toX x toY| * 0.5 / n */
+static void qemit_setup(const yg_light* L, int n, yo_qemit* q)
+{
+    yo_area A;
+    memset(&A, 0, sizeof(A));
+    yg_light L1 = *L;
+    L1.samples = 1;  /* only the frame of the quad is needed */
+    area_setup(&L1, &A);
+    free(A.points); free(A.jit_h); free(A.jit_v);
+    q->corner = A.corner;
+    q->toX = A.toX;
+    q->toY = A.toY;
+    q->direction = A.direction;
+    q->NU = v3_norm(A.toX);
+    q->NV = v3_norm(A.toY);
+    const float area = v3_len(v3_cross(A.toX, A.toY));
+    const yo_col c = col_scale(col_scale(col(L->color[0], L->color[1], L->color[2]), L->power), area);
+    const yo_col c2 = col_scale(c, 0.5f);
+    q->scolor = col(c2.r / (float)n, c2.g / (float)n, c2.b / (float)n);
+}
+
+typedef struct {
+    int32_t bx, by, bz;
+    int64_t seq;
+    yo_v3 pos, N;
+    uint8_t th, ph, rgbe[4];
+} yo_grec;
+
+typedef struct { yo_grec* v; int64_t n, cap; } grecvec;
+
+static void grec_push(grecvec* m, yo_grec x)
+{
+    if (m->n == m->cap) {
+        m->cap = m->cap ? 2 * m->cap : 4096;
+        m->v = (yo_grec*)realloc(m->v, sizeof(yo_grec) * (size_t)m->cap);
+    }
+    m->v[m->n++] = x;
+}
+
+typedef struct {
+    rctx* R;
+    int maxdepth;
+    float cell;
+    grecvec* recs;
+    int64_t stored;
+} gpctx;
+
+/* globalPhotonLight_t::shoot (globalphotonlight.cc:19-95), no caustic objects */
+static void gshoot(gpctx* G, yo_photon* ph, yo_v3 dir, int depth)
+{
+    rctx* R = G->R;
+    if (depth > G->maxdepth) return;
+    yo_sp sp;
+    R->stats[1]++;
+    if (!first_hit(R, ph->pos, dir, &sp, 0)) return;
+    /* runningPhoton_t::position(p, MIN_RAYDIST), photon.cc:27-35 */
+    if (v3_len(v3_sub(ph->pos, sp.P)) > R->opts.min_raydis) ph->lastpos = ph->pos;
+    ph->pos = sp.P;
+    const yo_v3 edir = v3_norm(v3_sub(ph->lastpos, ph->pos));
+    const yo_v3 N = face_forward(sp.Ng, sp.N, edir);
+    const yo_v3 Ng = face_forward(sp.Ng, sp.Ng, edir);
+    /* storeFirst: quad emitters store the first hit (arealight.h:43) */
+    if (obj_flag(R, sp.obj, 1)) {
+        yo_grec r;
+        r.pos = ph->pos;
+        r.N = N;
+        rgbe_enc(ph->c, r.rgbe);
+        dir_enc(v3_norm(v3_sub(ph->lastpos, ph->pos)), &r.th, &r.ph);
+        find_box(G->cell, r.pos, &r.bx, &r.by, &r.bz);
+        r.seq = G->recs->n;
+        grec_push(G->recs, r);
+        G->stored++;
+    }
+    float diffuse = 0.0f;
+    yo_col diffcolor = col(0, 0, 0);
+    if (obj_flag(R, sp.obj, 2)) {
+        diffcolor = get_diffuse(R, &sp);
+        diffuse = col_energy(diffcolor);
+    }
+    float sum = diffuse;  /* trans = 0 */
+    if (sum > 0.0f) sum = (float)(1.0 / (double)sum);
+    diffuse *= sum;
+    if (sum > 0.0f) {
+        R->stats[4]++;
+        (void)our_random(&R->lcg);  /* ourRandom() < trans (= 0): never taken */
+        diffcolor = col_scale(diffcolor, (float)(1.0 / (double)diffuse));
+        const float r1 = our_random(&R->lcg), r2 = our_random(&R->lcg);
+        R->stats[4] += 2;
+        const yo_v3 refDir = hemivec_cone(Ng, sp.NU, sp.NV, 0.05f, r1, r2);
+        ph->c = col_mul(ph->c, diffcolor);
+        gshoot(G, ph, refDir, depth + 1);
+    }
+}
+
+static int grec_cmp(const void* a, const void* b)
+{
+    const yo_grec* x = (const yo_grec*)a;
+    const yo_grec* y = (const yo_grec*)b;
+    if (x->bx != y->bx) return x->bx < y->bx ? -1 : 1;
+    if (x->by != y->by) return x->by < y->by ? -1 : 1;
+    if (x->bz != y->bz) return x->bz < y->bz ? -1 : 1;
+    return x->seq < y->seq ? -1 : (x->seq > y->seq);
+}
+
+/* storeInHash (globalphotonlight.cc:177-195): the records of one box, in
+ * the order they were stored */
+static void gbox_fold(const yo_grec* r, int64_t n, yo_gbox* B)
+{
+    B->bx = r[0].bx; B->by = r[0].by; B->bz = r[0].bz;
+    B->th = 255; B->ph = 0;
+    memset(B->rgbe, 0, 4);
+    memset(B->irr, 0, sizeof(B->irr));
+    for (int64_t k = 0; k < n; k++) {
+        const yo_grec* q = &r[k];
+        if (B->th == 255) {
+            B->pos[0] = q->pos.x; B->pos[1] = q->pos.y; B->pos[2] = q->pos.z;
+            B->th = q->th; B->ph = q->ph;
+            memcpy(B->rgbe, q->rgbe, 4);
+            B->N[0] = q->N.x; B->N[1] = q->N.y; B->N[2] = q->N.z;
+        } else {
+            const yo_v3 nd = dir_dec(q->th, q->ph);
+            if (v3_dot(nd, v3_ld(B->N)) > 0.0f) {
+                const yo_col ac = rgbe_dec(B->rgbe), nc = rgbe_dec(q->rgbe);
+                yo_v3 mixedd = v3_add(v3_mul(dir_dec(B->th, B->ph), col_energy(ac)), v3_mul(nd, col_energy(nc)));
+                mixedd = v3_norm(mixedd);
+                dir_enc(mixedd, &B->th, &B->ph);
+                rgbe_enc(col_add(ac, nc), B->rgbe);
+            }
+        }
+    }
+}
+
+typedef struct { float dis; int64_t idx; } yo_gfound;
+
+static int gfound_cmp(const void* a, const void* b)
+{
+    const yo_gfound* x = (const yo_gfound*)a;
+    const yo_gfound* y = (const yo_gfound*)b;
+    if (x->dis != y->dis) return x->dis < y->dis ? -1 : 1;
+    return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+/* uniform grid over the box positions (cell = radius >= any gather radius) */
+static void gm_grid(yo_gmap* M)
+{
+    free(M->gstart); free(M->gitems);
+    M->gstart = NULL; M->gitems = NULL;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = 0; i < M->n; i++)
+        for (int a = 0; a < 3; a++) {
+            lo[a] = fminf(lo[a], M->box[i].pos[a]);
+            hi[a] = fmaxf(hi[a], M->box[i].pos[a]);
+        }
+    int64_t nc = 1;
+    for (int a = 0; a < 3; a++) {
+        M->gmin[a] = M->n ? lo[a] : 0.0f;
+        int d = M->n ? (int)floorf((hi[a] - lo[a]) / M->radius) + 1 : 1;
+        if (d < 1) d = 1;
+        if (d > 256) d = 256;  /* items beyond the last cell stay in it (clamped) */
+        M->gdim[a] = d;
+        nc *= d;
+    }
+    M->gstart = (int64_t*)calloc((size_t)nc + 1, sizeof(int64_t));
+    M->gitems = (int64_t*)malloc(sizeof(int64_t) * (size_t)(M->n > 0 ? M->n : 1));
+    int64_t* cellof = (int64_t*)malloc(sizeof(int64_t) * (size_t)(M->n > 0 ? M->n : 1));
+    for (int64_t i = 0; i < M->n; i++) {
+        int c[3];
+        for (int a = 0; a < 3; a++) {
+            int k = (int)floorf((M->box[i].pos[a] - M->gmin[a]) / M->radius);
+            if (k < 0) k = 0;
+            if (k >= M->gdim[a]) k = M->gdim[a] - 1;
+            c[a] = k;
+        }
+        cellof[i] = ((int64_t)c[2] * M->gdim[1] + c[1]) * M->gdim[0] + c[0];
+        M->gstart[cellof[i] + 1]++;
+    }
+    for (int64_t c = 0; c < nc; c++) M->gstart[c + 1] += M->gstart[c];
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)nc);
+    memcpy(fill, M->gstart, sizeof(int64_t) * (size_t)nc);
+    for (int64_t i = 0; i < M->n; i++) M->gitems[fill[cellof[i]]++] = i;
+    free(fill); free(cellof);
+}
+
+/* globalPhotonMap_t::gather (photon.cc:167-210) over the boxes' photons:
+ * direction and colour of box i as `which` says (0: the merged photon, only
+ * boxes with a direction; 1: the irradiance photon, direction N, colour
+ * RGBE(irr)).  *radius: the caller's radius state.  Returns the number
+ * found, sorted by (distance, box). */
+static int gm_gather(const yo_gmap* M, int which, yo_v3 P, yo_v3 N, int K, float* radius, float mincos,
+                     yo_gfound* found, int cap)
+{
+    int reached = 0, nf = 0;
+    int first = 1;
+    while (first || ((reached < K) && (*radius <= M->radius))) {
+        first = 0;
+        if (!(*radius <= M->radius)) break;
+        reached = 0;
+        nf = 0;
+        const float r = *radius;
+        int g0[3], g1[3];
+        const float pp[3] = {P.x, P.y, P.z};
+        for (int a = 0; a < 3; a++) {
+            long lo = (long)floorf((pp[a] - r - M->gmin[a]) / M->radius);
+            long hi = (long)floorf((pp[a] + r - M->gmin[a]) / M->radius);
+            if (lo < 0) lo = 0;
+            if (hi >= M->gdim[a]) hi = M->gdim[a] - 1;
+            g0[a] = (int)lo; g1[a] = (int)hi;
+        }
+        for (int z = g0[2]; z <= g1[2]; z++)
+            for (int y = g0[1]; y <= g1[1]; y++)
+                for (int x = g0[0]; x <= g1[0]; x++) {
+                    const int64_t c = ((int64_t)z * M->gdim[1] + y) * M->gdim[0] + x;
+                    for (int64_t q = M->gstart[c]; q < M->gstart[c + 1]; q++) {
+                        const int64_t i = M->gitems[q];
+                        const yo_gbox* B = &M->box[i];
+                        yo_v3 d;
+                        if (which == 0) {
+                            if (B->th == 255) continue;
+                            d = dir_dec(B->th, B->ph);
+                        } else {
+                            uint8_t th, ph;
+                            dir_enc(v3_ld(B->N), &th, &ph);
+                            d = dir_dec(th, ph);
+                        }
+                        const float D = v3_len(v3_sub(v3_ld(B->pos), P));
+                        if ((D > r) || (v3_dot(d, N) <= mincos)) continue;
+                        reached++;
+                        yo_gfound f = {D, i};
+                        if (nf < K && nf < cap) { found[nf++] = f; continue; }
+                        /* keep the K nearest: replace the farthest when nearer */
+                        int far = 0;
+                        for (int k = 1; k < nf; k++)
+                            if (gfound_cmp(&found[k], &found[far]) > 0) far = k;
+                        if (gfound_cmp(&f, &found[far]) < 0) found[far] = f;
+                    }
+                }
+        if (reached < K) *radius *= 2;
+        if (M->stateless) break;
+    }
+    if (reached > K) {
+        const float f = (float)K / (float)reached;
+        if (f < (float)(0.7 * 0.7)) *radius *= 0.95f;
+    }
+    if (*radius > M->radius) *radius = M->radius;
+    qsort(found, (size_t)nf, sizeof(yo_gfound), gfound_cmp);
+    return nf;
+}
+
+static yo_col gbox_irr_colour(const yo_gbox* B)
+{
+    uint8_t e[4];
+    rgbe_enc(col(B->irr[0], B->irr[1], B->irr[2]), e);
+    return rgbe_dec(e);
+}
+
+/* setIrradiance (globalphotonlight.cc:127-175) of every box, boxes in hash
+ * order (x, y, z), the gather radius carried from box to box */
+static void gm_irradiances(yo_gmap* M, int search)
+{
+    yo_gfound* found = (yo_gfound*)malloc(sizeof(yo_gfound) * (size_t)(search + 1));
+    float radius = M->radius;
+    for (int64_t i = 0; i < M->n; i++) {
+        yo_gbox* B = &M->box[i];
+        if (M->stateless) radius = M->radius;
+        const yo_v3 N = v3_ld(B->N);
+        const int nf = gm_gather(M, 0, v3_ld(B->pos), N, search, &radius, 0.0f, found, search + 1);
+        yo_col total = col(0, 0, 0);
+        if (nf == 0) {
+            B->irr[0] = B->irr[1] = B->irr[2] = 0;
+            continue;
+        }
+        const yo_gbox* F0 = &M->box[found[nf - 1].idx];  /* the heap's front: the farthest */
+        if (nf == 1 || found[nf - 1].dis == 0) {
+            float factor = v3_dot(dir_dec(F0->th, F0->ph), N);
+            if (factor < 0.0f) factor = 0.0f;
+            const yo_col c = col_scale(rgbe_dec(F0->rgbe), factor);
+            B->irr[0] = c.r; B->irr[1] = c.g; B->irr[2] = c.b;
+            continue;
+        }
+        const float farest = found[nf - 1].dis;
+        for (int k = 0; k < nf; k++) {
+            const yo_gbox* F = &M->box[found[k].idx];
+            const float factor = (1.0f - found[k].dis / farest) * v3_dot(dir_dec(F->th, F->ph), N);
+            if (factor > 0) total = col_add(total, col_scale(rgbe_dec(F->rgbe), factor));
+        }
+        float area = (float)(M_PI * (double)farest * (double)farest);
+        if (area < 0.00005f) area = 0.00005f;
+        const yo_col c = col_scale(total, (float)(4.0 * M_PI / (double)area));
+        B->irr[0] = c.r; B->irr[1] = c.g; B->irr[2] = c.b;
+    }
+    free(found);
+}
+
+/* getLight's irradiance part (pathlight.cc:147-176): the box at P if its
+ * normal agrees (N.N > 0.7), else the weighted mean of the 5 nearest
+ * irradiance photons within the thread's radius (directions . N > 0.25) */
+static yo_col gm_irradiance(rctx* R, const yo_sp* sp, yo_v3 N)
+{
+    const yo_gmap* M = &R->s->gm;
+    int32_t bx, by, bz;
+    hash_cell(M->cell, sp->P, &bx, &by, &bz);
+    int64_t lo = 0, hi = M->n;
+    while (lo < hi) {  /* boxes sorted by (bx, by, bz) */
+        const int64_t mid = (lo + hi) / 2;
+        const yo_gbox* B = &M->box[mid];
+        const int less = B->bx != bx ? B->bx < bx : (B->by != by ? B->by < by : B->bz < bz);
+        if (less) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo < M->n && M->box[lo].bx == bx && M->box[lo].by == by && M->box[lo].bz == bz) {
+        const yo_gbox* B = &M->box[lo];
+        const float factor = v3_dot(v3_ld(B->N), N);
+        if (factor > 0.7f) return col_scale(col(B->irr[0], B->irr[1], B->irr[2]), factor);
+    }
+    yo_gfound found[6];
+    if (M->stateless) R->gm_radius = M->radius;
+    const int nf = gm_gather(M, 1, sp->P, N, 5, &R->gm_radius, 0.25f, found, 6);
+    yo_col total = col(0, 0, 0);
+    if (!nf) return total;
+    float farest = (nf == 1) ? R->gm_radius : found[nf - 1].dis;
+    if (farest == 0.0f) farest = 1.0f;
+    float div = 0.0f;
+    for (int k = 0; k < nf; k++) {
+        const yo_gbox* B = &M->box[found[k].idx];
+        uint8_t th, ph;
+        dir_enc(v3_ld(B->N), &th, &ph);
+        const float factor = v3_dot(dir_dec(th, ph), N) * (1.0f - found[k].dis / farest);
+        if (factor > 0) {
+            total = col_add(total, col_scale(gbox_irr_colour(B), factor));
+            div += factor;
+        }
+    }
+    if (div > 0.0f) total = col_scale(total, (float)(1.0 / (double)div));
+    return total;
+}
+
+/* globalPhotonLight_t::init (globalphotonlight.cc:234-278) for light li:
+ * stats = {emitted, stored, boxes, ray count, ourRandom() draws} */
+int yo_gphoton_build(yo_scene* s, const yg_material* mats, int nmat, const yg_modulator* mods,
+                     const yg_texture* texs, const yg_light* lights, int nl, const int32_t* obj_flags,
+                     int nobj, const yg_render_opts* opts, int li, int stateless, uint64_t* stats)
+{
+    if (li < 0 || li >= nl || lights[li].type != YG_LIGHT_GLOBALPHOTON) return -1;
+    const yg_light* GL = &lights[li];
+    if (GL->search < 1 || GL->photons < 1 || !(GL->fixedradius > 0)) return -2;
+    rctx R;
+    rctx_init(&R, s, mats, nmat, mods, texs, lights, nl, obj_flags, nobj, opts);
+    R.lcg = (int32_t)opts->lcg_seed;
+    yo_gmap* M = &s->gm;
+    free(M->box); free(M->gstart); free(M->gitems);
+    memset(M, 0, sizeof(*M));
+    M->light = li;
+    M->radius = GL->fixedradius;
+    M->cell = GL->fixedradius / sqrtf((float)GL->search);
+    M->stateless = stateless;
+    int numemitters = 0;
+    for (int k = 0; k < nl; k++)
+        if (lights[k].type == YG_LIGHT_AREA && lights[k].dummy) numemitters++;
+    grecvec recs = {NULL, 0, 0};
+    gpctx G = {&R, GL->depth, M->cell, &recs, 0};
+    int64_t emitted = 0;
+    if (numemitters) {
+        const int ppl = GL->photons / numemitters;
+        for (int k = 0; k < nl; k++) {
+            if (lights[k].type != YG_LIGHT_AREA || !lights[k].dummy) continue;
+            yo_qemit q;
+            qemit_setup(&lights[k], ppl, &q);
+            for (int j = 0; j < ppl; ++j) {
+                /* quadEmitter_t::getDirection (arealight.cc:227-232); the
+                 * argument and operand draws in GCC's order (right to left) */
+                const float z2 = our_random(&R.lcg), z1 = our_random(&R.lcg);
+                const yo_v3 dir = hemivec_cone(q.direction, q.NU, q.NV, 0.0001f, z1, z2);
+                const float ry = our_random(&R.lcg), rx = our_random(&R.lcg);
+                R.stats[4] += 4;
+                const yo_v3 from = v3_add(v3_add(q.corner, v3_mul(q.toX, rx)), v3_mul(q.toY, ry));
+                yo_photon ph;
+                ph.pos = from;
+                ph.lastpos = from;
+                ph.c = col_scale(q.scolor, v3_dot(q.direction, dir));
+                gshoot(&G, &ph, dir, 0);
+                emitted++;
+            }
+        }
+    }
+    /* the hash boxes in (x, y, z) order, records of a box in store order */
+    qsort(recs.v, (size_t)recs.n, sizeof(yo_grec), grec_cmp);
+    int64_t nbox = 0;
+    for (int64_t i = 0; i < recs.n; i++)
+        if (i == 0 || recs.v[i].bx != recs.v[i - 1].bx || recs.v[i].by != recs.v[i - 1].by ||
+            recs.v[i].bz != recs.v[i - 1].bz)
+            nbox++;
+    M->box = (yo_gbox*)calloc((size_t)(nbox > 0 ? nbox : 1), sizeof(yo_gbox));
+    int64_t b = 0;
+    for (int64_t i = 0; i < recs.n;) {
+        int64_t j = i + 1;
+        while (j < recs.n && recs.v[j].bx == recs.v[i].bx && recs.v[j].by == recs.v[i].by &&
+               recs.v[j].bz == recs.v[i].bz)
+            j++;
+        gbox_fold(&recs.v[i], j - i, &M->box[b++]);
+        i = j;
+    }
+    M->n = nbox;
+    free(recs.v);
+    gm_grid(M);
+    gm_irradiances(M, GL->search);
+    M->stored = G.stored;
+    M->emitted = emitted;
+    M->lcg_after = R.lcg;
+    M->own_stream = 1;
+    M->built = 1;
+    if (stats) {
+        stats[0] = (uint64_t)emitted; stats[1] = (uint64_t)G.stored; stats[2] = (uint64_t)nbox;
+        stats[3] = R.stats[1]; stats[4] = R.stats[4];
+    }
+    rctx_free(&R);
+    return 0;
+}
+
+/* the boxes as 15 floats each (position, N, irradiance, merged direction,
+ * merged colour), like yg_gphoton_read */
+int64_t yo_gphoton_read(yo_scene* s, float* out, int64_t cap)
+{
+    const yo_gmap* M = &s->gm;
+    if (!M->built) return -1;
+    for (int64_t i = 0; i < M->n && i < cap; i++) {
+        const yo_gbox* B = &M->box[i];
+        const yo_v3 d = dir_dec(B->th, B->ph);
+        const yo_col c = rgbe_dec(B->rgbe);
+        float* o = out + 15 * i;
+        memcpy(o, B->pos, 12); memcpy(o + 3, B->N, 12); memcpy(o + 6, B->irr, 12);
+        o[9] = d.x; o[10] = d.y; o[11] = d.z;
+        o[12] = c.r; o[13] = c.g; o[14] = c.b;
+    }
+    return M->n;
+}
+
+/* the irradiance boxes of another implementation (15 floats each, in box
+ * order; box indices recomputed from the positions with the light's box
+ * size): renders then read them instead of a pass made here */
+int yo_gphoton_load(yo_scene* s, const float* data, int64_t n, const yg_light* GL, int li, int stateless,
+                    int recompute)
+{
+    yo_gmap* M = &s->gm;
+    free(M->box); free(M->gstart); free(M->gitems);
+    memset(M, 0, sizeof(*M));
+    M->light = li;
+    M->radius = GL->fixedradius;
+    M->cell = GL->fixedradius / sqrtf((float)GL->search);
+    M->stateless = stateless;
+    M->box = (yo_gbox*)calloc((size_t)(n > 0 ? n : 1), sizeof(yo_gbox));
+    for (int64_t i = 0; i < n; i++) {
+        const float* o = data + 15 * i;
+        yo_gbox* B = &M->box[i];
+        memcpy(B->pos, o, 12); memcpy(B->N, o + 3, 12); memcpy(B->irr, o + 6, 12);
+        dir_enc(v3(o[9], o[10], o[11]), &B->th, &B->ph);
+        rgbe_enc(col(o[12], o[13], o[14]), B->rgbe);
+        find_box(M->cell, v3_ld(B->pos), &B->bx, &B->by, &B->bz);
+    }
+    M->n = n;
+    gm_grid(M);
+    /* recompute: pre-gather the loaded boxes' irradiance here (setIrradiance) */
+    if (recompute) gm_irradiances(M, GL->search);
+    M->built = 1;
+    M->own_stream = 0;
+    return 0;
+}
 
 typedef struct { yo_v3 pos, vright, vup, vto; } yo_cam;
 
@@ -1289,6 +1878,13 @@ int yo_render_mt(yo_scene* s, const yg_camera* cam, const yg_material* mats, int
                  const yg_render_opts* opts, float* rgb, uint64_t* stats, int nthreads)
 {
     (void)nmod; (void)ntex;
+    for (int li = 0; li < nl; li++)
+        if (lights[li].type == YG_LIGHT_GLOBALPHOTON) {
+            if (!s->gm.built || s->gm.light != li) return -3;  /* map not built */
+            for (int lj = 0; lj < nl; lj++)
+                if (lights[lj].type == YG_LIGHT_PATH && lights[lj].samples > 96) return -6;  /* photonSampler_t */
+            nthreads = 1;  /* the gathers carry a radius from call to call */
+        }
     int sequential = 0;
     for (int li = 0; li < nl; li++)
         if (lights[li].type == YG_LIGHT_AREA) sequential = 1;
@@ -1367,6 +1963,8 @@ int yo_render_adjoint(yo_scene* s, const yg_camera* cam, const yg_material* mats
     if (opts->aa_passes > 0) return -5; /* antialiasing passes are not differentiated */
     for (int k = 0; k < ntex; k++)
         if (texs[k].type == YG_TEX_MUSGRAVE && texs[k].mtype != YG_MUSGRAVE_FBM) return -7; /* fBm only */
+    for (int li = 0; li < nl; li++)
+        if (lights[li].type == YG_LIGHT_GLOBALPHOTON) return -8; /* no irradiance-map derivatives */
     for (int li = 0; li < nl; li++)
         if (lights[li].type == YG_LIGHT_AREA) nthreads = 1;
     if (nthreads < 1) nthreads = 1;

@@ -1,0 +1,107 @@
+"""The global photon map on the GPU (yg_gphoton_build) against the oracle.
+
+* Render parity: the oracle loads the GPU's irradiance boxes and renders the
+  same frame; image and ray counts agree (the boxes, and every lookup at the
+  path vertices, are the same arithmetic).
+* The map itself is statistical: the reference shoots photons one after
+  another from one ourRandom() stream, the GPU gives every photon its own
+  segment of that sequence.  The GPU frame differs from the oracle's own
+  frame (serial photons, the reference's gathers) by no more than the
+  oracle's frames differ from each other over photon seeds, and photon,
+  record and box counts agree within their sampling spread."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from yafray_amd import descriptors as D
+from yafray_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _Y():
+    from yafray_amd import lib as Y
+    return Y
+
+
+def rel_rmse(a, b):
+    return float(np.sqrt(np.mean((np.asarray(a, np.float64) - b) ** 2)) / np.sqrt(np.mean(np.asarray(b, np.float64) ** 2)))
+
+
+def gpu_frame(d, torch, width=4):
+    sc = _Y().Scene.from_desc(d, width=width)
+    gst = sc.gphoton_build(1)
+    out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+    st = sc.render(out)
+    torch.cuda.synchronize()
+    n = sc.gphoton_read()
+    boxes = torch.zeros(max(n, 1) * 15, dtype=torch.float32, device="cuda")
+    sc.gphoton_read(boxes)
+    torch.cuda.synchronize()
+    return (out.cpu().numpy().reshape(d.cam.resy, d.cam.resx, 3), st, gst,
+            boxes.cpu().numpy()[: n * 15].reshape(n, 15))
+
+
+@pytest.mark.parametrize("width", [2, 4, 8])
+def test_render_with_the_gpu_map_matches_oracle(torch_cuda, width):
+    d = S.gphoton_room(64, 48, photons=60000, depth=2, search=50, radius=1.0, samples=6, path_depth=2,
+                       sun=(0.3, -0.2, 1.0))
+    img, st, gst, boxes = gpu_frame(d, torch_cuda, width)
+    assert gst.emitted == 60000 and gst.clustered == len(boxes) > 1000
+    assert np.all(np.isfinite(boxes))
+    oc = po.OracleScene.from_desc(d)
+    oc.gphoton_load(d, 1, boxes, stateless=True)
+    ref, rs = oc.render(d)
+    assert rel_rmse(img, ref) <= 1e-4
+    assert (st.rays_camera, st.rays_extension, st.rays_shadow) == tuple(int(x) for x in rs[:3])
+    # the map carries light: without it (irradiance zero) the frame is darker
+    assert img.mean() > 0
+
+
+def test_map_statistics_match_the_serial_reference(torch_cuda):
+    d = S.gphoton_room(64, 48, photons=200000, depth=2, search=100, radius=1.0, samples=8, path_depth=1)
+    img, st, gst, boxes = gpu_frame(d, torch_cuda)
+    frames, counts = [], []
+    for seed in (123212, 98765, 4242):
+        o = D.yg_render_opts.from_buffer_copy(d.opts)
+        o.lcg_seed = seed
+        dd = D.SceneDesc(d.verts, d.idx, d.tri_obj, d.tri_mat, d.obj_flags, d.cam, list(d.mats), [], [],
+                         list(d.lights), o)
+        oc = po.OracleScene.from_desc(dd)
+        counts.append(oc.gphoton_build(dd, 1))
+        frames.append(oc.render(dd)[0])
+    spread = max(rel_rmse(frames[a], frames[b]) for a, b in ((0, 1), (0, 2), (1, 2)))
+    gap = np.mean([rel_rmse(img, f) for f in frames])
+    print("oracle seed-to-seed frame difference %.4f, GPU to oracle %.4f" % (spread, gap))
+    assert gap <= 1.5 * spread + 1e-3
+    # the mean brightness agrees to within the frames' own spread
+    means = [f.mean() for f in frames]
+    assert abs(img.mean() - np.mean(means)) <= 3 * np.std(means) + 0.01 * np.mean(means)
+    assert all(int(c[0]) == gst.emitted for c in counts)
+    for k, g in ((1, gst.stored), (2, gst.clustered)):
+        vals = np.array([float(c[k]) for c in counts])
+        assert abs(g - vals.mean()) <= max(4 * vals.std(), 0.02 * vals.mean()), (k, g, vals)
+
+
+def test_global_map_refusals(torch_cuda):
+    Y = _Y()
+    d = S.gphoton_room(32, 24, photons=2000, search=20)
+    sc = Y.Scene.from_desc(d)
+    out = torch_cuda.zeros(32 * 24 * 3, dtype=torch_cuda.float32, device="cuda")
+    with pytest.raises(RuntimeError, match="global photon map not built"):
+        sc.render(out)
+    # an area light drawing at the path vertices next to the map
+    r = S.direct_room(32, 24, samples=4, subdiv=2)
+    lights = list(d.lights) + [r.lights[0]]
+    d2 = D.SceneDesc(d.verts, d.idx, d.tri_obj, d.tri_mat, d.obj_flags, d.cam, list(d.mats), [], [], lights, d.opts)
+    sc2 = Y.Scene.from_desc(d2)
+    sc2.gphoton_build(1)
+    with pytest.raises(RuntimeError, match="area lights inside GI paths together with a global photon map"):
+        sc2.render(out)
+    # more than 96 pathlight samples sample by photonSampler_t (pathlight.cc:731)
+    lights = [d.lights[0], d.lights[1], D.pathlight(samples=100, depth=1, use_qmc=True, threshold=1e3)]
+    d3 = D.SceneDesc(d.verts, d.idx, d.tri_obj, d.tri_mat, d.obj_flags, d.cam, list(d.mats), [], [], lights, d.opts)
+    sc3 = Y.Scene.from_desc(d3)
+    sc3.gphoton_build(1)
+    with pytest.raises(RuntimeError, match="photonSampler_t"):
+        sc3.render(out)

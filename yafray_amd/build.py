@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "libyafgpu.so")
 CLI = os.path.join(HERE, "yafgpu-render")
 
 SOURCES = ["capi.cpp", "qmc.cpp", "bvh_build.cpp", "bvh_gpu.hip", "autotune.cpp", "trace.hip", "render.hip",
-           "photon.hip",
+           "photon.hip", "gphoton.hip",
            "../host/yafgpu_host.cpp", "../host/yafgpu_xml.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

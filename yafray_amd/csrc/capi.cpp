@@ -58,6 +58,7 @@ static void free_dev(yg_scene* s)
     s->d_tprim = nullptr; s->d_tobj = nullptr; s->d_tmat = nullptr; s->d_obj_flags = nullptr; s->d_mats = nullptr;
     s->d_mods = nullptr;
     s->d_texs = nullptr; s->pm = PhotonMapDev();
+    gphoton_free(s->gp);
     s->d_halton = nullptr; s->halton_ndim = 0; s->halton_stride = 0;
     s->d_any_ovf = nullptr; s->any_ovf_ints = 0;
     for (float* p : s->ph_halton)
@@ -354,7 +355,7 @@ extern "C" int yg_set_lights(yg_scene* s, const yg_light* lights, uint32_t n)
     if (!s) return yg_fail(YG_ERR_ARG, "null scene");
     for (uint32_t i = 0; i < n; i++) {
         const yg_light& L = lights[i];
-        if (L.type < YG_LIGHT_SUN || L.type > YG_LIGHT_PHOTON) return yg_fail(YG_ERR_ARG, "bad light type");
+        if (L.type < YG_LIGHT_SUN || L.type > YG_LIGHT_GLOBALPHOTON) return yg_fail(YG_ERR_ARG, "bad light type");
         if (L.type == YG_LIGHT_AREA && L.psamples != 0)
             return yg_fail(YG_ERR_UNSUPPORTED, "area light psamples > 0 (guessLight's rand()) is not implemented");
         if (L.type == YG_LIGHT_PATH && !L.use_qmc)
@@ -368,6 +369,7 @@ extern "C" int yg_set_lights(yg_scene* s, const yg_light* lights, uint32_t n)
     }
     s->lights.assign(lights, lights + n);
     s->pm.light = -1;
+    s->gp.light = -1;
     s->gen++;
     return YG_OK;
 }

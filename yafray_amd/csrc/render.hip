@@ -598,8 +598,14 @@ __global__ __launch_bounds__(SB) void k_shade(const RenderScene* __restrict__ sc
                 if (L.cast_shadows && (mask >> li & 1u)) fl = cadd(fl, mkc(0, 0, 0));
                 else fl = cadd(fl, from_light(R, sp, L.dir, L.col, eye));
             }
-            contri[(size_t)p * D + j] = cmul(cscale(raycolor, s), fl);
             rho = get_diffuse(R.mats, R.mods, R.texs, sp.mat, sp.P);
+            if (R.gp_on) {
+                // getLight with the irradiance map (pathlight.cc:140-185):
+                // irradiance x diffuse, then the lights
+                const f3 Nv = face_forward(sp.Ng, sp.N, -ray);
+                fl = cadd(cmul(gp_irradiance(R.gp, sp.P, Nv), rho), fl);
+            }
+            contri[(size_t)p * D + j] = cmul(cscale(raycolor, s), fl);
             nseg[p] = (uint8_t)(j + 1);
             if (vx.draws) {
                 // area lights used in indirect light draw 2 x samples at a vertex
@@ -1314,6 +1320,7 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
     R.out_rgb = d_rgb;
     R.path_light = -1;
     R.photon_light = -1;
+    R.gp_on = 0;
     R.nl = (int)s->lights.size();
     for (int li = 0; li < R.nl; li++) {
         const yg_light& Lg = s->lights[li];
@@ -1329,8 +1336,19 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
             // sunLight_t ctor (sunlight.cc:27-33)
             L.dir = norm3(ld3(Lg.dir));
             L.col = cscale(mkc(Lg.color[0], Lg.color[1], Lg.color[2]), Lg.power);
+        } else if (Lg.type == YG_LIGHT_AREA && Lg.dummy) {
+            // a dummy area light only shoots global photons (arealight.cc:98):
+            // no light, no draws
+            L.type = 0;
         } else if (Lg.type == YG_LIGHT_AREA) {
             area_setup(Lg, L, P.apts, P.ajh, P.ajv);
+        } else if (Lg.type == YG_LIGHT_GLOBALPHOTON) {
+            if (R.gp_on) return yg_fail(YG_ERR_UNSUPPORTED, "more than one global photon light");
+            if (s->gp.light != li || s->gp.gen != s->geo_gen)
+                return yg_fail(YG_ERR_STATE, "global photon map not built (yg_gphoton_build)");
+            R.gp_on = 1;
+            R.gp = s->gp.view();
+            L.type = 0;
         } else if (Lg.type == YG_LIGHT_PATH) {
             if (R.path_light >= 0) return yg_fail(YG_ERR_UNSUPPORTED, "more than one pathlight");
             R.path_light = li;
@@ -1347,6 +1365,16 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
             L.fixedradius = Lg.fixedradius;
         }
         P.lights.push_back(L);
+    }
+    if (R.gp_on) {
+        for (const auto& Lg : s->lights)
+            if (Lg.type == YG_LIGHT_AREA && !Lg.dummy && Lg.use_in_indirect)
+                return yg_fail(YG_ERR_UNSUPPORTED,
+                               "area lights inside GI paths together with a global photon map are not on the GPU path");
+        // with the global map published, a pathlight of more than 96 samples
+        // samples by photonSampler_t (pathlight.cc:731), which is not implemented
+        if (R.path_light >= 0 && R.samples > 96)
+            return yg_fail(YG_ERR_UNSUPPORTED, "pathlight samples > 96 with a global photon map (photonSampler_t)");
     }
     if (R.path_light >= 0) {
         if (R.depth > 255) return yg_fail(YG_ERR_UNSUPPORTED, "pathlight depth");
@@ -1822,7 +1850,7 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     for (auto& P : plans)
         if (P.R.path_light >= 0)
             for (const auto& L : P.s->lights)
-                if (L.use_in_indirect && L.type == YG_LIGHT_AREA) W.area_path = true;
+                if (L.use_in_indirect && L.type == YG_LIGHT_AREA && !L.dummy) W.area_path = true;
     if (W.area_path && aa_passes > 0)
         return yg_fail(YG_ERR_UNSUPPORTED, "antialiasing with area lights inside GI paths");
     if (W.area_path) {
@@ -1832,7 +1860,7 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         for (auto& P : plans) {
             uint64_t pp = 0, pv = 0;
             for (const auto& L : P.s->lights)
-                if (L.type == YG_LIGHT_AREA) {
+                if (L.type == YG_LIGHT_AREA && !L.dummy) {
                     if (L.use_in_render) pp += 2ull * (uint64_t)L.samples;
                     if (L.use_in_indirect) pv += 2ull * (uint64_t)L.samples;
                 }
@@ -1848,7 +1876,7 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         for (auto& P : plans) {
             uint64_t pp = 0;
             for (const auto& L : P.s->lights)
-                if (L.type == YG_LIGHT_AREA && L.use_in_render) pp += 2ull * (uint64_t)L.samples;
+                if (L.type == YG_LIGHT_AREA && !L.dummy && L.use_in_render) pp += 2ull * (uint64_t)L.samples;
             const double per_pass = (double)P.nitems * (double)std::max(1, P.aa_passes > 0 ? P.R.aa_mins : 1);
             if (per_pass * (double)pp >= 4294967295.0)
                 return yg_fail(YG_ERR_UNSUPPORTED, "too many area-light draws in one scene's pass (2^32 or more)");

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "noise_dev.h"
+#include "gphoton_dev.h"
 #include "traverse.h"
 
 // One light of the render's light table (see yg_light); derived quantities
@@ -57,6 +58,8 @@ struct RenderScene {
     uint32_t lcg_seed;
     int photon_light;        // index of the photon light or -1
     PhotonView pm;
+    int gp_on;               // a global photon light's irradiance map feeds the path vertices
+    GPhotonView gp;
     float* out_rgb;
     uint32_t item_base;      // first item of this scene
     uint32_t area_base;      // first area of this scene

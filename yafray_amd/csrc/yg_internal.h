@@ -132,6 +132,42 @@ struct PhotonMapDev {
     int light = -1;
 };
 
+// ------------------------------------------------- global photon map --
+#include "gphoton_dev.h"
+
+// globalPhotonLight_t's pre-gathered boxes on the device (gphoton.hip)
+struct GPhotonDev {
+    int light = -1;                 // light index it was built for, -1: none
+    uint64_t gen = 0;               // geometry generation it was built on
+    int64_t n = 0;                  // boxes
+    float cell = 1, radius = 1;
+    int bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+    int sy = 0, sz = 0;
+    float gmin[3] = {0, 0, 0};
+    int gdim[3] = {1, 1, 1};
+    unsigned long long* d_key = nullptr;
+    f3* d_N = nullptr;
+    c3* d_irr = nullptr;
+    f3* d_pos = nullptr;
+    f3* d_qN = nullptr;
+    c3* d_qirr = nullptr;
+    int32_t* d_gstart = nullptr;
+    int32_t* d_gitems = nullptr;
+    float* d_read = nullptr;        // 15 floats per box (yg_gphoton_read)
+    GPhotonView view() const
+    {
+        GPhotonView v;
+        v.key = d_key; v.N = d_N; v.irr = d_irr; v.pos = d_pos; v.qN = d_qN; v.qirr = d_qirr;
+        v.gstart = d_gstart; v.gitems = d_gitems;
+        for (int a = 0; a < 3; a++) {
+            v.gmin[a] = gmin[a]; v.gdim[a] = gdim[a]; v.bmin[a] = bmin[a]; v.bmax[a] = bmax[a];
+        }
+        v.cell = cell; v.radius = radius; v.sy = sy; v.sz = sz; v.n = n;
+        return v;
+    }
+};
+void gphoton_free(GPhotonDev& g);
+
 // --------------------------------------------------------- scene handle --
 struct yg_scene {
     std::vector<float> verts;
@@ -173,6 +209,7 @@ struct yg_scene {
     yg_texture* d_texs = nullptr;
 
     PhotonMapDev pm;
+    GPhotonDev gp;
     std::vector<float*> ph_halton;  // yg_photon_build: device Halton tables per dimension
     int ph_halton_np = -1;          //   for this emitted-photon count
     int* d_any_ovf = nullptr;  // yg_trace_any: traversal stack entries beyond the LDS part

@@ -62,7 +62,8 @@ class yg_light(C.Structure):
                 ("psamples", C.c_int32), ("depth", C.c_int32), ("use_qmc", C.c_int32),
                 ("threshold", C.c_float), ("from_", F3), ("to", F3), ("angle", C.c_float),
                 ("photons", C.c_int32), ("search", C.c_int32), ("mindepth", C.c_int32),
-                ("bias", C.c_float), ("fixedradius", C.c_float), ("cluster", C.c_float)]
+                ("bias", C.c_float), ("fixedradius", C.c_float), ("cluster", C.c_float),
+                ("dummy", C.c_int32)]
 
 
 class yg_render_opts(C.Structure):
@@ -117,7 +118,7 @@ class yg_photon_stats(C.Structure):
 
 YG_TEX_CLOUDS, YG_TEX_MARBLE, YG_TEX_WOOD, YG_TEX_MUSGRAVE = 1, 2, 3, 4
 YG_MOD_MIX, YG_MOD_MUL, YG_MOD_ADD, YG_MOD_SUB = 0, 1, 2, 3
-YG_LIGHT_SUN, YG_LIGHT_AREA, YG_LIGHT_PATH, YG_LIGHT_PHOTON = 1, 2, 3, 4
+YG_LIGHT_SUN, YG_LIGHT_AREA, YG_LIGHT_PATH, YG_LIGHT_PHOTON, YG_LIGHT_GLOBALPHOTON = 1, 2, 3, 4, 5
 YG_EVAL_NOISE, YG_EVAL_FBM, YG_EVAL_TURBULENCE, YG_EVAL_TEX_FLOAT, YG_EVAL_TEX_COLOR = range(5)
 
 MIN_RAYDIST = 0.00005  # configio.py:95-100
@@ -143,7 +144,8 @@ def sunlight(direction, color=(1, 1, 1), power=1.0, cast_shadows=True):
     return L
 
 
-def arealight(a, b, c, d, samples=50, color=(1, 1, 1), power=1.0, psamples=0):
+def arealight(a, b, c, d, samples=50, color=(1, 1, 1), power=1.0, psamples=0, dummy=False):
+    """dummy: shoots global photons only (arealight.cc:98, arealight.h:90-95)."""
     L = yg_light()
     L.type = YG_LIGHT_AREA
     L.use_in_render = 1
@@ -153,6 +155,21 @@ def arealight(a, b, c, d, samples=50, color=(1, 1, 1), power=1.0, psamples=0):
     L.psamples = psamples
     L.color = _f3(color)
     L.power = power
+    L.dummy = int(dummy)
+    return L
+
+
+def globalphotonlight(radius=1.0, depth=2, photons=50000, search=200):
+    """globalPhotonLight_t::factory (globalphotonlight.cc:280-294); no
+    caustics (caus_depth is not used)."""
+    L = yg_light()
+    L.type = YG_LIGHT_GLOBALPHOTON
+    L.use_in_render = 1
+    L.use_in_indirect = 1
+    L.fixedradius = radius
+    L.depth = depth
+    L.photons = photons
+    L.search = search
     return L
 
 

@@ -59,6 +59,8 @@ def lib():
         "yg_photon_build": (C.c_int, [vp, u32, vp, vp, vp]),
         "yg_photon_read": (C.c_int, [vp, vp, C.c_uint64, C.POINTER(C.c_uint64), vp]),
         "yg_photon_gather": (C.c_int, [vp, vp, vp, vp, vp, vp, u32, vp, vp]),
+        "yg_gphoton_build": (C.c_int, [vp, u32, vp, vp, vp]),
+        "yg_gphoton_read": (C.c_int, [vp, vp, C.c_uint64, C.POINTER(C.c_uint64), vp]),
     }
     sigs["yg_render_batch"] = (C.c_int, [C.POINTER(vp), u32, vp, C.POINTER(vp), vp, vp])
     sigs["yg_render_adjoint"] = (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp])
@@ -282,6 +284,22 @@ class Scene:
         check(lib().yg_photon_build(self.h, li, C.byref(o), C.byref(st), _stream()),
               "yg_photon_build")
         return st
+
+    def gphoton_build(self, li, opts=None):
+        """globalPhotonLight_t::init on the device for light li (yg_gphoton_build)."""
+        o = opts if opts is not None else self.desc.opts
+        st = D.yg_photon_stats()
+        check(lib().yg_gphoton_build(self.h, li, C.byref(o), C.byref(st), _stream()), "yg_gphoton_build")
+        return st
+
+    def gphoton_read(self, out=None):
+        """The irradiance boxes into a float32 CUDA tensor of 15 floats each;
+        returns the box count (out=None: count only)."""
+        cnt = C.c_uint64()
+        check(lib().yg_gphoton_read(self.h, _ptr(out) if out is not None else None,
+                                    out.numel() // 15 if out is not None else 0, C.byref(cnt), _stream()),
+              "yg_gphoton_read")
+        return cnt.value
 
     def photon_read(self, out):
         cnt = C.c_uint64()

@@ -352,3 +352,20 @@ def batch_specs(n=64, lo=1e3, hi=2e6):
         rx, ry = BATCH_RES[min(3, int(u[k, 1] * 4))]
         specs.append((nx, ny, rx, ry))
     return specs
+
+
+def gphoton_room(resx=80, resy=60, subdiv=3, photons=200000, depth=2, search=100, radius=1.0, samples=8,
+                 path_depth=1, sun=None):
+    """The direct-lighting room with its area light made a dummy emitter of
+    global photons (arealight.cc:98), a global photon light
+    (globalphotonlight.cc:280-294) and a pathlight whose path vertices read
+    the irradiance map (pathlight.cc:140-185); optionally a sun."""
+    r = direct_room(resx, resy, samples=4, subdiv=subdiv)
+    A = r.lights[0]
+    lights = [D.arealight(A.a, A.b, A.c, A.d, samples=1, color=tuple(A.color), power=A.power, dummy=True),
+              D.globalphotonlight(radius=radius, depth=depth, photons=photons, search=search),
+              D.pathlight(samples=samples, depth=path_depth, use_qmc=True, threshold=1e3)]
+    if sun is not None:
+        lights.append(D.sunlight(sun, power=0.8))
+    return D.SceneDesc(r.verts, r.idx, r.tri_obj, r.tri_mat, r.obj_flags, r.cam, list(r.mats), [], [], lights,
+                       r.opts)
