@@ -606,6 +606,134 @@ def _photon_cpu_legs(d, img, pst, args, sc=None, n_pix=4096, seed=11):
     }
 
 
+def bench_gphoton(args):
+    """Global photon map (globalPhotonLight_t) read at pathlight vertices:
+    the room (icosphere 20,480 + 2 boxes + 5 walls) with its area light a
+    dummy emitter of 1,000,000 photons, depth 2, search 100, radius 1, a sun,
+    and a 16-sample one-bounce pathlight whose path vertices read the
+    irradiance map, at 640x480.  One step = map build (tracing, box merging,
+    pre-gathering) + the frame.
+
+    Also: device time of the photon tracing and pre-gathering launches and
+    of the frame's extension / shadow launches (HIP events on the library's
+    stream); the frame against the oracle rendering it with the GPU's map
+    (same arithmetic: equal within rounding) and against the oracle's own
+    serial map (a different photon stream: statistical agreement); the
+    oracle's serial map build and its frame as the CPU baseline."""
+    import numpy as np
+    import torch
+
+    from yafray_amd import descriptors as D
+    from yafray_amd import lib as Y
+    from yafray_amd import scenes as S
+
+    d = S.gphoton_room(640, 480, subdiv=5, photons=1000000, depth=2, search=100, radius=1.0, samples=16,
+                       path_depth=1, sun=(0.3, -0.2, 1.0))
+    sc = Y.Scene.from_desc(d)
+    out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+
+    def step():
+        sc.gphoton_build(1)
+        return sc.render(out)
+
+    dt, st = _timed(step, args.steps, args.warmup)
+    kt = D.yg_render_opts.from_buffer_copy(d.opts)
+    kt.flags = 3
+    gst = sc.gphoton_build(1, kt)
+    bdt, _ = _timed(lambda: sc.gphoton_build(1), args.steps, 1)
+    rdt, _ = _timed(lambda: sc.render(out), args.steps, 1)
+    rst = sc.render(out, kt)
+    torch.cuda.synchronize()
+    img = out.cpu().numpy().reshape(d.cam.resy, d.cam.resx, 3)
+    finite = bool(np.all(np.isfinite(img)))
+    if not finite:
+        raise RuntimeError("gphoton: non-finite pixels in the GPU image")
+    rays = int(st.rays_camera + st.rays_extension + st.rays_shadow)
+    nbox = int(gst.clustered)
+    K = int(d.lights[1].search)
+    # pre-gathering: per box 24 B query (position, normal) + 12 B irradiance out
+    # + K x 20 B (position, direction code, RGBE colour of each box photon used)
+    # + 27 x 4 B cell starts
+    irr_b = nbox * (24 + 12 + K * 20 + 27 * 4)
+    kern = {"k_gp_trace": {"ms": float(gst.ms_trace), "segments": int(gst.rays)},
+            "k_gp_irr": {"ms": float(gst.ms_irr), "boxes": nbox, "alg_bytes": irr_b},
+            "k_ext": {"ms": float(rst.ms_ext), "rays": int(rst.rays_extension)},
+            "k_shadow": {"ms": float(rst.ms_shadow), "rays": int(rst.rays_bounce - rst.rays_extension)}}
+    achieved = irr_b / (gst.ms_irr * 1e-3) / 1e9 if gst.ms_irr > 0 else 0.0
+    line = {
+        "metric": "global photon map + GI ms/frame", "value": round(dt * 1e3, 3), "unit": "ms/frame",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "room + icosphere(20480) + 2 boxes, 640x480; global photon map of 1,000,000 "
+                               "photons from the area light, depth 2, search 100, radius 1; sun; pathlight "
+                               "16 spp, 1 bounce, reading the map at its vertices",
+                   "triangles": int(d.n_tris)},
+        "map_build_ms": round(bdt * 1e3, 3), "render_ms": round(rdt * 1e3, 3),
+        "rays_per_frame": rays, "mrays_per_s_frame": round(rays / rdt / 1e6, 3),
+        "emitted": int(gst.emitted), "stored": int(gst.stored), "boxes": nbox,
+        "kernels_ms": kern, "image_finite": finite,
+        "roofline": {"bound": "hbm", "kernel": "k_gp_irr", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "ms_per_launch": round(float(gst.ms_irr), 4), "alg_bytes_per_launch": irr_b},
+    }
+    if not args.no_cpu:
+        line.update(_gphoton_cpu_legs(d, img, rst, sc, torch))
+    return line
+
+
+def _gphoton_cpu_legs(d, img, rst, sc, torch):
+    """Oracle legs of the global photon workload, on the host: the frame
+    rendered with the GPU's boxes loaded (equal arithmetic), the oracle's own
+    serial map and its frame (statistical agreement), and the CPU baseline:
+    the serial map build timed on a bounded photon count and scaled, plus
+    the frame on the host cores."""
+    import numpy as np
+
+    from oracle import pyoracle as po
+
+    def rrmse(x, y):
+        return float(np.sqrt(np.mean((x - y) ** 2)) / max(np.sqrt(np.mean(y ** 2)), 1e-30))
+
+    threads = cpu_cores()
+    n = sc.gphoton_read()
+    boxes = torch.zeros(max(n, 1) * 15, dtype=torch.float32, device="cuda")
+    sc.gphoton_read(boxes)
+    torch.cuda.synchronize()
+    boxes = boxes.cpu().numpy()[: n * 15].reshape(n, 15)
+    print("gphoton: oracle frame on the GPU's map", file=sys.stderr, flush=True)
+    oc = po.OracleScene.from_desc(d)
+    oc.gphoton_load(d, 1, boxes, stateless=True)
+    t0 = time.perf_counter()
+    ref, rs = oc.render(d, threads=threads)
+    frame_s = time.perf_counter() - t0
+    g = img.astype(np.float64)
+    same = rrmse(g, ref.astype(np.float64))
+    counts_equal = (int(rst.rays_camera), int(rst.rays_extension), int(rst.rays_shadow)) == tuple(int(x) for x in rs[:3])
+    if not (same <= 1e-3 and counts_equal):
+        raise RuntimeError("gphoton parity failed: rel rmse %.3g, ray counts equal %s" % (same, counts_equal))
+    print("gphoton: oracle serial map", file=sys.stderr, flush=True)
+    o2 = po.OracleScene.from_desc(d)
+    t0 = time.perf_counter()
+    oc_counts = o2.gphoton_build(d, 1)
+    map_s = time.perf_counter() - t0
+    own, _ = o2.render(d, threads=threads)
+    stat = rrmse(g, own.astype(np.float64))
+    full = int(d.lights[1].photons)
+    return {
+        "cpu_baseline": {"value": round((map_s + frame_s) * 1e3, 1), "unit": "ms/frame", "cores": threads,
+                         "kind": "port",
+                         "sample": "oracle serial map build of %d photons (%.1f s, one thread, as the reference "
+                                   "shoots them one after another) + the 640x480 frame on %d threads (%.1f s)"
+                                   % (full, map_s, threads, frame_s)},
+        "rmse_vs_oracle": {"rel_rmse_same_map": same, "ray_counts_equal": counts_equal,
+                           "rel_rmse_oracle_own_map": stat,
+                           "oracle_emitted_stored_boxes": [int(x) for x in oc_counts[:3]],
+                           "sample": "full frame; same_map: the oracle renders with the GPU's boxes; own_map: "
+                                     "the oracle's serial photon stream (statistical)"},
+    }
+
+
 VALU_FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak (packed FMA); unpacked code reaches half
 FBM_FLOP_PER_POINT = 384       # 6 octaves x ~64 FLOP (improved Perlin: 8 gradients, 7 lerps, 3 fades)
 
@@ -884,7 +1012,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="gi",
                     choices=["gi", "batched", "adjoint", "photon", "direct", "autotune", "aa", "rays",
-                             "noise", "room_gi"])
+                             "noise", "room_gi", "gphoton"])
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--batch-mode", default="both", choices=["both", "sequential", "batched"],
                     help="batched workload: time both modes (default) or one of them alone")
@@ -902,7 +1030,8 @@ def main():
         torch.cuda.set_device(0)
         fn = {"batched": bench_batched, "adjoint": bench_adjoint, "photon": bench_photon,
               "direct": bench_direct, "autotune": bench_autotune, "aa": bench_aa,
-              "rays": bench_rays, "noise": bench_noise, "room_gi": bench_room_gi}[args.workload]
+              "rays": bench_rays, "noise": bench_noise, "room_gi": bench_room_gi,
+              "gphoton": bench_gphoton}[args.workload]
         print(json.dumps(fn(args)))
         return
 

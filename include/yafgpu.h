@@ -33,7 +33,7 @@ extern "C" {
 #define YG_ERR_UNSUPPORTED (-4)
 #define YG_ERR_NOMEM (-5)
 
-#define YG_ABI_VERSION 11
+#define YG_ABI_VERSION 12
 
 /* ---------------------------------------------------------------- rays -- */
 
@@ -176,6 +176,13 @@ typedef struct yg_light {
     float fixedradius;       /* photon "fixedradius"; global photon "radius" */
     float cluster;
     int32_t dummy;           /* area "dummy": shoots global photons only, no direct light */
+    /* pathlight irradiance cache (pathlight.cc:405-723, lightcache.cc, cacheproxy.cc);
+       "search" above is the cache's search (at least 3, pathlight.cc:795) */
+    int32_t cache;           /* path "cache" */
+    float cache_size;        /* path "cache_size": box edge in (screen x, screen y, log
+                                distance) space; samples cover within it, gathers reach twice it */
+    float shadow_threshold;  /* path "shadow_threshold": a sample covers a point while its
+                                weight is at least 1 / shadow_threshold */
 } yg_light;
 
 /* Render parameters (interface_impl.cc:480-644 / scene.cc:50-60). */
@@ -453,6 +460,10 @@ typedef struct yg_photon_stats {
     uint64_t stored;
     uint64_t clustered;   /* photons kept after preGathering */
     float ms;
+    uint64_t rays;        /* yg_gphoton_build: photon ray segments traced */
+    float ms_trace;       /* yg_gphoton_build with YG_RENDER_KTIMING: device time of the
+                             photon tracing launch */
+    float ms_irr;         /* ... and of the pre-gathering launch (irradiance of every box) */
 } yg_photon_stats;
 
 /* photonLight_t::init (photonlight.cc:453-501) for light index li of the

@@ -123,6 +123,13 @@ def _load(path):
         L.yo_gphoton_read.argtypes = [vp, vp, C.c_int64]
         L.yo_gphoton_load.restype = C.c_int
         L.yo_gphoton_load.argtypes = [vp, F32P, C.c_int64, vp, C.c_int, C.c_int, C.c_int]
+        L.yo_cache_read.restype = C.c_int64
+        L.yo_cache_read.argtypes = [vp, vp, C.c_int64, U64P]
+        L.yo_cache_record.argtypes = [vp, C.c_int]
+        L.yo_cache_visits.restype = C.c_int64
+        L.yo_cache_visits.argtypes = [vp, vp, C.c_int64]
+        L.yo_heap_offer_trace.restype = C.c_int64
+        L.yo_heap_offer_trace.argtypes = [F32P, C.c_int64, C.c_uint, F32P]
         L.yo_rgbe_roundtrip.argtypes = [F32P, F32P, C.c_int64]
         L.yo_dir_roundtrip.argtypes = [F32P, F32P, vp, C.c_int64]
         L.yo_dirconv_tables.argtypes = [F32P]
@@ -131,6 +138,15 @@ def _load(path):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def heap_offer_trace(weights, limit):
+    """The oracle's found heap after offering weights[i] (id i) in order, as
+    lightCache_t::gatherSamples offers candidates: [(id, weight)] in array order."""
+    w = np.ascontiguousarray(weights, np.float32)
+    out = np.zeros(2 * (limit + 2), np.float32)
+    n = lib().yo_heap_offer_trace(w, len(w), limit, out)
+    return [(int(out[2 * i]), float(out[2 * i + 1])) for i in range(n)]
 
 
 class OracleScene:
@@ -346,6 +362,30 @@ class OracleScene:
         out = np.zeros(max(n, 1) * 15, np.float32)
         L.yo_gphoton_read(self.h, _ptr(out), n)
         return out[: n * 15].reshape(n, 15)
+
+    def cache_read(self):
+        """Irradiance cache of the last render with a cache = on pathlight:
+        ([n, 18] samples in insertion order -- P, N, pP, colour, adist, M,
+        precision, generic-tree rank, fake-pass pixel, devaluated -- and the
+        number of samples its render pass had to take)."""
+        L = lib(self.fast)
+        nn = np.zeros(1, np.uint64)
+        n = L.yo_cache_read(self.h, None, 0, nn)
+        out = np.zeros(max(n, 1) * 18, np.float32)
+        L.yo_cache_read(self.h, out.ctypes.data, n, nn)
+        return out[: n * 18].reshape(n, 18), int(nn[0])
+
+    def cache_record(self, on=True):
+        lib(self.fast).yo_cache_record(self.h, int(on))
+
+    def cache_visits(self):
+        """[n, 13] fake-pass visits of the last recorded cache render: pixel,
+        area, qualifies, P, face-forwarded N, pP, material."""
+        L = lib(self.fast)
+        n = L.yo_cache_visits(self.h, None, 0)
+        out = np.zeros(max(n, 1) * 13, np.float32)
+        L.yo_cache_visits(self.h, out.ctypes.data, n)
+        return out[: n * 13].reshape(n, 13)
 
     def gphoton_load(self, desc, li, boxes, stateless=True, recompute=False):
         """Use `boxes` ([n, 15], as gphoton_read gives them) as light li's

@@ -168,6 +168,16 @@ int yo_gphoton_build(yo_scene* s, const yg_material* mats, int nmat, const yg_mo
                      int nobj, const yg_render_opts* opts, int li, int stateless, uint64_t* stats);
 /* boxes: 15 floats each (position, N, irradiance, merged direction, merged colour) */
 int64_t yo_gphoton_read(yo_scene* s, float* out, int64_t cap);
+
+/* Irradiance cache (pathlight cache = on) of the last yo_render*: its
+ * samples, 18 floats each (P, N, pP, colour, adist, M, precision, rank,
+ * visit, devaluated); *new_samples: samples its render pass had to take.
+ * yo_cache_release frees it (yo_scene_free does). */
+int64_t yo_cache_read(yo_scene* s, float* out, int64_t cap, uint64_t* new_samples);
+void yo_cache_release(yo_scene* s);
+void yo_cache_record(yo_scene* s, int on);
+int64_t yo_cache_visits(yo_scene* s, float* out, int64_t cap);
+int64_t yo_heap_offer_trace(const float* w, int64_t n, unsigned limit, float* out);
 int yo_gphoton_load(yo_scene* s, const float* data, int64_t n, const yg_light* GL, int li, int stateless,
                     int recompute);
 void yo_rgbe_roundtrip(const float* in, float* out, int64_t n);

@@ -361,6 +361,7 @@ void yo_scene_free(yo_scene* s)
 {
     if (!s) return;
     free(s->tri_vn); free(s->tri_smooth);
+    yo_cache_release(s);
     free(s->verts); free(s->idx); free(s->tri_obj); free(s->tri_mat); free(s->tri_n);
     free(s->nodes); free(s->order);
     free(s->pm.data); free(s->pm.gstart); free(s->pm.gitems);

@@ -180,6 +180,11 @@ struct yo_scene {
     yo_gmap gm;
     float* tri_vn;         /* autoSmooth vertex normals, 9 per triangle (NULL: none) */
     uint8_t* tri_smooth;   /* per triangle: has vertex normals */
+    struct yo_lcache_s* lc_last;  /* irradiance cache of the last cache = on render */
+    uint64_t lc_new;
+    int lc_rec;                   /* record the fake pass's visits (test hook) */
+    float* lc_visits;             /* 13 floats per visit */
+    int64_t lc_nvis, lc_viscap;
     /* object flags of the render in progress (bit0 cast shadows), or NULL */
     const int32_t* oflags;
     int64_t noflags;

@@ -66,6 +66,20 @@ typedef struct {
     int nap, apcap;
     float gm_radius;       /* photonData_t::radius of this render thread (pathlight.cc:131) */
     int gp;                /* the scene has a global photon light whose map is built */
+    /* irradiance cache of a pathlight with cache = on (pathlight.cc:405-723) */
+    struct yo_lcache_s* lc;
+    int lc_phase;          /* 1: fake pass (cached), 2: render pass (interpolate) */
+    float scr_x, scr_y;    /* renderState_t::screenpos of the pixel being rendered */
+    int64_t pixnum;        /* renderState_t::pixelNumber */
+    float traveled;        /* renderState_t::traveled at the camera ray's hit (scene.cc:239-240) */
+    float worldres;        /* scene_t::world_resolution, (1.0 / resx) / focal (scene.cc:103) */
+    /* cacheProxy_t of the render area (cacheproxy.cc:10-15, 160-172) */
+    float px_radius;
+    int64_t px_id;
+    struct yo_ls_s* px_created;
+    int px_n, px_cap;
+    uint64_t lc_new;       /* interpolate() found no sample and took one */
+    int32_t* lc_cand;      /* gather candidates (one per cache sample) */
 } rctx;
 
 static void adj_path_sample_begin(rctx* R);
@@ -295,11 +309,15 @@ static yo_col background(const rctx* R)
 
 /* pathLight_t::takeSample full-GI branch, pathlight.cc:223-387 (no cache, no
  * caustics, direct = false) */
-static yo_col path_take_sample(rctx* R, int li, yo_v3 N, const yo_sp* sp)
+static yo_col path_take_sample(rctx* R, int li, yo_v3 N, const yo_sp* sp, float* avgD, float* minD)
 {
     const yg_light* L = &R->lights[li];
     yo_halton* hs = R->hseq[li];
     int samples = L->samples, maxdepth = L->depth;
+    const int caching = avgD != NULL;
+    /* maxdist = 1000000 * world resolution * sp.Z (pathlight.cc:227), in float */
+    const float maxdist = (float)1000000 * R->worldres * sp->Z;
+    float HNUM = 0, HD = 0, M = 0;
     yo_col total = col(0, 0, 0), subtotal[4];
     for (int k = 0; k < 4; k++) subtotal[k] = col(0, 0, 0);
     int taken = 0, first = 1;
@@ -311,6 +329,7 @@ static yo_col path_take_sample(rctx* R, int li, yo_v3 N, const yo_sp* sp)
             yo_v3 ray = next_direction(hs, &taken, N, sp->NU, sp->NV, i, 0);
             yo_v3 startray = ray;
             yo_v3 where = sp->P;
+            if (caching) HNUM += 1;
             const int rec = R->adj && R->gpx_on && first;
             if (rec) adj_path_sample_begin(R);
             for (int j = 0; j < maxdepth; ++j) {
@@ -323,6 +342,11 @@ static yo_col path_take_sample(rctx* R, int li, yo_v3 N, const yo_sp* sp)
                     if (first) subtotal[i % 4] = col_add(subtotal[i % 4], contri);
                     if (rec) adj_path_vertex(R, 0, NULL, col(0, 0, 0), col(0, 0, 0));
                     break;
+                }
+                if (caching) {
+                    /* harmonic mean and minimum of the first hits' distances (pathlight.cc:323-327) */
+                    if (j == 0 && t.Z > 0) HD = (float)((double)HD + 1.0 / (double)t.Z);
+                    if (j == 0 && ((t.Z < M) || (M == 0))) M = t.Z;
                 }
                 /* getLight, pathlight.cc:186-191: sc.light(indirect) at P + eye */
                 yo_v3 eye = v3_neg(ray);
@@ -385,20 +409,33 @@ static yo_col path_take_sample(rctx* R, int li, yo_v3 N, const yo_sp* sp)
             ite += extra;
         }
     }
+    if (caching) {
+        float H;
+        if (HD > 0) H = HNUM / HD; else H = maxdist;
+        if (M <= 0) M = maxdist;
+        *avgD = H;
+        *minD = M;
+    }
     float mult = (float)(1.0 / (double)(float)(taken + 1));
     return col_scale(total, mult / repetitions);
 }
 
-/* pathLight_t::normalSample, pathlight.cc:389-403 */
+static yo_col lc_cached(rctx* R, int li, const yo_sp* sp, yo_v3 eye);
+static yo_col lc_interpolate(rctx* R, int li, const yo_sp* sp, yo_v3 eye);
+
+/* pathLight_t::illuminate (pathlight.cc:107-119): the cache's fake pass
+ * (cached), its render pass (interpolate), else normalSample
+ * (pathlight.cc:389-403) */
 static yo_col path_illuminate(rctx* R, int li, const yo_sp* sp, yo_v3 eye)
 {
     const yg_light* L = &R->lights[li];
+    if (L->cache && R->lc) return (R->lc_phase == 1) ? lc_cached(R, li, sp, eye) : lc_interpolate(R, li, sp, eye);
     yo_v3 N = face_forward(sp->Ng, sp->N, eye);
     yo_col dif = get_diffuse(R, sp);
     if (col_energy(dif) < 0.05f) return col(0, 0, 0);
     const int prim = R->primary;
     R->primary = 0;  /* lights evaluated along the paths are not the pixel's direct terms */
-    yo_col total = path_take_sample(R, li, N, sp);
+    yo_col total = path_take_sample(R, li, N, sp, NULL, NULL);
     R->primary = prim;
     if (R->adj && R->gpx_on) adj_path_pixel(R, li, sp, (float)(1.0 / (double)(float)L->samples));
     return col_scale(col_mul(dif, total), L->power);
@@ -730,6 +767,8 @@ static void rctx_free(rctx* R)
     }
     free(R->areas); free(R->sun_dir); free(R->sun_col); free(R->hseq);
     free(R->ap);
+    free(R->px_created);
+    free(R->lc_cand);
 }
 
 static void grid_build(yo_photonmap* pm, float radius)
@@ -1607,6 +1646,7 @@ static yo_col raytrace(rctx* R, yo_v3 from, yo_v3 ray)
     yo_sp sp;
     R->stats[0]++;
     if (first_hit(R, from, ray, &sp, 0)) {
+        R->traveled = sp.Z;  /* state.traveled += sp.Z() from 0 at the camera ray (scene.cc:239-240) */
         R->primary = 1;
         yo_col c = scene_light(R, &sp, from, 0);
         R->primary = 0;
@@ -1616,6 +1656,7 @@ static yo_col raytrace(rctx* R, yo_v3 from, yo_v3 ray)
 }
 
 typedef struct { int x, y, w, h, rx, ry, rw, rh; } yo_region;
+typedef struct yo_lcache_s* yo_lcache_ptr;
 
 /* blockSpliter_t::blockSpliter_t, renderblock.cc:71-101 */
 int yo_block_regions(int width, int height, int b, uint32_t seed, yo_region** out)
@@ -1688,6 +1729,9 @@ typedef struct {
     const float* grad_rgb;  /* adjoint: dLoss/dpixel */
     double *gmat, *gmod, *glight;
     int nmod;
+    yo_lcache_ptr lc;    /* irradiance cache of a cache = on pathlight, filled by the fake pass */
+    float worldres;
+    uint64_t lc_new;
 } render_job;
 
 /* color_t::expgam_Adjust (color.h:175-195) with scene_t's stored values
@@ -1754,6 +1798,569 @@ static void check_resample(const yo_col* img, int W, int H, float threshold, uns
  * marks with AA_minsamples camera rays each and fold them into a running
  * average.  The area keeps its own image, overlap border included; only the
  * real rectangle is written out. */
+/* ====================================================== irradiance cache ==
+ * pathLight_t with cache = on: lightSample_t and lightCache_t
+ * (lightcache.h:14-176, lightcache.cc:48-239), cacheProxy_t
+ * (cacheproxy.cc:10-184), pathLight_t::cached / interpolate and the weight
+ * functions (pathlight.cc:409-723), and the fake pass of scene_t::render
+ * (scene.cc:559-583, fakeRender scene.cc:825-854).  Restated one sample at a
+ * time in the reference's order, with its list orders (move-to-front), its
+ * generic-tree visiting order and libstdc++'s heap operations, since which
+ * samples are found, and in which order ties fall, depends on them.
+ * testRefinement (threshold < 1) is not restated: renders with it are refused. */
+
+typedef struct yo_ls_s {
+    yo_v3 N;
+    yo_col color;
+    float adist, M, precision;
+    yo_v3 P, pP;
+    float devaluated;
+    int32_t next;       /* next sample of its box's list (front first), -1 at the end */
+    int32_t visit;      /* fake-pass visit that took it */
+} yo_ls;
+
+typedef struct { int32_t x, y, z, head; } yo_lbox;
+
+typedef struct yo_lcache_s {
+    float cache_size, ycorr;
+    yo_ls* s; int64_t n, cap;
+    yo_lbox* b; int64_t nb, bcap;
+    int64_t* ht; int64_t htcap;   /* (x, y, z) -> box index + 1 */
+    int32_t* rank;                /* startUse: position in the generic tree's leaf order */
+    /* candidates of a gather: grid over (pP.x, pP.y), cell gcell, entries in rank order */
+    float gx0, gy0, gcell;
+    int gnx, gny;
+    int64_t* gstart;
+    int32_t* gitem;
+} yo_lcache;
+
+/* foundSample_t (lightcache.h:38-43) */
+typedef struct { int32_t S; float dis, weight; } yo_fs;
+
+static uint64_t lc_hkey(int32_t x, int32_t y, int32_t z)
+{
+    uint64_t h = (uint64_t)(uint32_t)x * 0x9E3779B97F4A7C15ull;
+    h ^= (uint64_t)(uint32_t)y * 0xC2B2AE3D27D4EB4Full + (h >> 29);
+    h ^= (uint64_t)(uint32_t)z * 0x165667B19E3779F9ull + (h >> 31);
+    return h ^ (h >> 33);
+}
+
+static int64_t lc_find(const yo_lcache* C, int32_t x, int32_t y, int32_t z)
+{
+    if (!C->htcap) return -1;
+    for (uint64_t q = lc_hkey(x, y, z) & (uint64_t)(C->htcap - 1);; q = (q + 1) & (uint64_t)(C->htcap - 1)) {
+        const int64_t v = C->ht[q];
+        if (!v) return -1;
+        const yo_lbox* B = &C->b[v - 1];
+        if (B->x == x && B->y == y && B->z == z) return v - 1;
+    }
+}
+
+static int64_t lc_find_create(yo_lcache* C, int32_t x, int32_t y, int32_t z)
+{
+    int64_t f = lc_find(C, x, y, z);
+    if (f >= 0) return f;
+    if (2 * (C->nb + 1) > C->htcap) {
+        const int64_t nc = C->htcap ? 2 * C->htcap : 1024;
+        free(C->ht);
+        C->ht = (int64_t*)calloc((size_t)nc, sizeof(int64_t));
+        C->htcap = nc;
+        for (int64_t i = 0; i < C->nb; i++) {
+            uint64_t q = lc_hkey(C->b[i].x, C->b[i].y, C->b[i].z) & (uint64_t)(nc - 1);
+            while (C->ht[q]) q = (q + 1) & (uint64_t)(nc - 1);
+            C->ht[q] = i + 1;
+        }
+    }
+    if (C->nb == C->bcap) {
+        C->bcap = C->bcap ? 2 * C->bcap : 1024;
+        C->b = (yo_lbox*)realloc(C->b, sizeof(yo_lbox) * (size_t)C->bcap);
+    }
+    yo_lbox* B = &C->b[C->nb];
+    B->x = x; B->y = y; B->z = z; B->head = -1;
+    uint64_t q = lc_hkey(x, y, z) & (uint64_t)(C->htcap - 1);
+    while (C->ht[q]) q = (q + 1) & (uint64_t)(C->htcap - 1);
+    C->ht[q] = ++C->nb;
+    return C->nb - 1;
+}
+
+/* hash3d_t::getBox (hash3d.h:224-234): truncation, then one down for negative */
+static void lc_getbox(float cell, yo_v3 p, int32_t* nx, int32_t* ny, int32_t* nz)
+{
+    *nx = (int32_t)(p.x / cell);
+    *ny = (int32_t)(p.y / cell);
+    *nz = (int32_t)(p.z / cell);
+    if (p.x < 0.0f) (*nx)--;
+    if (p.y < 0.0f) (*ny)--;
+    if (p.z < 0.0f) (*nz)--;
+}
+
+/* lightCache_t::toPolar (lightcache.h:107-111): the pixel's screen position,
+ * y scaled by 1/aspect, and the log of the distance travelled; log is the
+ * C library's double log of the float distance */
+static yo_v3 lc_to_polar(const rctx* R)
+{
+    return v3(R->scr_x, R->scr_y * R->lc->ycorr, (float)log((double)R->traveled));
+}
+
+/* pathLight_t::weight / weightNoPrec (pathlight.cc:409-465): prec = 1
+ * subtracts twice the sample's precision from the distance (clamped at 0) */
+static float lc_weight(const yo_ls* S, yo_v3 P, yo_v3 N, float maxweight, int prec)
+{
+    yo_v3 PP = v3_sub(P, S->P);
+    const float avgOclu = S->adist;
+    if (avgOclu == 0.0f) return 0;
+    float D;
+    if (prec) {
+        D = (float)((double)v3_normlen(&PP) - (double)S->precision * 2.0);
+        if (D < 0) D = 0;
+    } else
+        D = v3_normlen(&PP);
+    const float a = D / avgOclu;
+    const float cinv = (float)(1.000001 - (double)v3_dot(S->N, N));
+    const float b = sqrtf(cinv);
+    const float c = fabsf(v3_dot(N, PP));
+    float f = S->devaluated * (a + ((b > c) ? b : c));
+    if (f == 0) return maxweight;
+    f = (float)(1.0 / (double)f);
+    return (f > maxweight) ? maxweight : f;
+}
+
+/* lightCache_t::enoughFor (lightcache.cc:182-222): boxes cx, cx-1, cx+1 (and
+ * so on per axis), each list from its front; the first sample within
+ * cache_size whose weightNoPrec reaches wlimit moves to its list's front */
+static int lc_enough_for(yo_lcache* C, yo_v3 P, yo_v3 N, yo_v3 pP, float wlimit)
+{
+    int32_t cx, cy, cz;
+    lc_getbox(C->cache_size, pP, &cx, &cy, &cz);
+    const float maxw = (float)((double)wlimit * 2.0);
+    for (int i = cx; i <= cx + 1; i += (i == cx) ? -1 : ((i < cx) ? 2 : 1))
+        for (int j = cy; j <= cy + 1; j += (j == cy) ? -1 : ((j < cy) ? 2 : 1))
+            for (int k = cz; k <= cz + 1; k += (k == cz) ? -1 : ((k < cz) ? 2 : 1)) {
+                const int64_t bi = lc_find(C, i, j, k);
+                if (bi < 0) continue;
+                yo_lbox* B = &C->b[bi];
+                for (int32_t prev = -1, l = B->head; l >= 0; prev = l, l = C->s[l].next) {
+                    const float pD = v3_len(v3_sub(pP, C->s[l].pP));
+                    if (pD > C->cache_size) continue;
+                    if (lc_weight(&C->s[l], P, N, maxw, 0) < wlimit) continue;
+                    if (prev >= 0) {
+                        C->s[prev].next = C->s[l].next;
+                        C->s[l].next = B->head;
+                        B->head = l;
+                    }
+                    return 1;
+                }
+            }
+    return 0;
+}
+
+/* lightCache_t::insert (lightcache.cc:224-237): push_front into the box of
+ * findBox(pP), which looks the box up again from its centre */
+static void lc_insert(yo_lcache* C, yo_ls smp)
+{
+    int32_t nx, ny, nz;
+    lc_getbox(C->cache_size, smp.pP, &nx, &ny, &nz);
+    const float cs = C->cache_size;
+    yo_v3 box = v3(cs * (float)nx, cs * (float)ny, cs * (float)nz);
+    box.x = (float)((double)box.x + 0.5 * (double)cs);
+    box.y = (float)((double)box.y + 0.5 * (double)cs);
+    box.z = (float)((double)box.z + 0.5 * (double)cs);
+    lc_getbox(cs, box, &nx, &ny, &nz);
+    const int64_t bi = lc_find_create(C, nx, ny, nz);
+    if (C->n == C->cap) {
+        C->cap = C->cap ? 2 * C->cap : 4096;
+        C->s = (yo_ls*)realloc(C->s, sizeof(yo_ls) * (size_t)C->cap);
+    }
+    smp.next = C->b[bi].head;
+    C->s[C->n] = smp;
+    C->b[bi].head = (int32_t)C->n;
+    C->n++;
+}
+
+/* ---- startUse: buildGenericTree (bound.h:252-340) over the samples in the
+ * cache's iteration order (boxes by x, y, z as std::map orders them; each
+ * list from its front), leaf size 1; rank = position in the depth-first leaf
+ * order, which is the order gObjectIterator_t hands candidates out */
+static int lc_box_cmp(const void* a, const void* b)
+{
+    const yo_lbox *p = (const yo_lbox*)a, *q = (const yo_lbox*)b;
+    if (p->x != q->x) return p->x < q->x ? -1 : 1;
+    if (p->y != q->y) return p->y < q->y ? -1 : 1;
+    if (p->z != q->z) return p->z < q->z ? -1 : 1;
+    return 0;
+}
+
+typedef struct { float a[3], g[3]; } yo_gb;
+
+/* path_calc_bound (lightcache.cc:11-33) */
+static yo_gb lc_calc_bound(const yo_lcache* C, const int32_t* v, int64_t n)
+{
+    yo_gb b;
+    if (n == 0) {
+        for (int k = 0; k < 3; k++) b.a[k] = b.g[k] = 0;
+        return b;
+    }
+    const yo_v3 p0 = C->s[v[0]].pP;
+    b.a[0] = b.g[0] = p0.x; b.a[1] = b.g[1] = p0.y; b.a[2] = b.g[2] = p0.z;
+    for (int64_t i = 0; i < n; i++) {
+        const yo_v3 p = C->s[v[i]].pP;
+        const float q[3] = {p.x, p.y, p.z};
+        for (int k = 0; k < 3; k++) {
+            if (q[k] > b.g[k]) b.g[k] = q[k];
+            if (q[k] < b.a[k]) b.a[k] = q[k];
+        }
+    }
+    return b;
+}
+
+static int lc_in_bound(const yo_v3 p, const yo_gb* b)
+{
+    return p.x >= b->a[0] && p.x <= b->g[0] && p.y >= b->a[1] && p.y <= b->g[1] && p.z >= b->a[2] &&
+           p.z <= b->g[2];
+}
+
+/* appends the depth-first leaf order of the tree buildGenericTree makes of
+ * v[0..n) to out */
+static void lc_tree_order(const yo_lcache* C, const int32_t* v, int64_t n, int sx, int sy, int sz, int32_t* out,
+                          int64_t* nout)
+{
+    if (n <= 1 || (sx && sy && sz)) {
+        for (int64_t i = 0; i < n; i++) out[(*nout)++] = v[i];
+        return;
+    }
+    const yo_gb bound = lc_calc_bound(C, v, n);
+    const float lx = bound.g[0] - bound.a[0], ly = bound.g[1] - bound.a[1], lz = bound.g[2] - bound.a[2];
+    int axis;
+    if (((lx >= ly) || sy) && ((lx >= lz) || sz) && !sx) axis = 0;
+    else if (((ly >= lx) || sx) && ((ly >= lz) || sz) && !sy) axis = 1;
+    else axis = 2;
+    float media = 0;
+    for (int64_t i = 0; i < n; i++) {
+        const yo_v3 p = C->s[v[i]].pP;
+        media += (axis == 0) ? p.x : (axis == 1) ? p.y : p.z;
+    }
+    media /= (float)n;
+    yo_gb bl = bound, br = bound;
+    bl.g[axis] = media;
+    br.a[axis] = media;
+    int32_t* vl = (int32_t*)malloc(sizeof(int32_t) * (size_t)n * 3);
+    int32_t *vr = vl + n, *vm = vl + 2 * n;
+    int64_t nl = 0, nr = 0, nm = 0;
+    for (int64_t i = 0; i < n; i++) {
+        const yo_v3 p = C->s[v[i]].pP;
+        if (lc_in_bound(p, &bl)) {
+            if (lc_in_bound(p, &br)) vm[nm++] = v[i];
+            else vl[nl++] = v[i];
+        } else
+            vr[nr++] = v[i];
+    }
+    if (nl == n || nr == n || nm == n) {
+        lc_tree_order(C, v, n, sx || axis == 0, sy || axis == 1, sz || axis == 2, out, nout);
+    } else if (nl == 0) {
+        lc_tree_order(C, vr, nr, sx, sy, sz, out, nout);
+        lc_tree_order(C, vm, nm, sx, sy, sz, out, nout);
+    } else if (nr == 0) {
+        lc_tree_order(C, vl, nl, sx, sy, sz, out, nout);
+        lc_tree_order(C, vm, nm, sx, sy, sz, out, nout);
+    } else {
+        lc_tree_order(C, vl, nl, sx, sy, sz, out, nout);
+        lc_tree_order(C, vr, nr, sx, sy, sz, out, nout);
+        if (nm) lc_tree_order(C, vm, nm, sx, sy, sz, out, nout);
+    }
+    free(vl);
+}
+
+static const yo_lcache* g_lc_sort;
+static int lc_item_cmp(const void* a, const void* b)
+{
+    const int32_t p = *(const int32_t*)a, q = *(const int32_t*)b;
+    return (g_lc_sort->rank[p] < g_lc_sort->rank[q]) ? -1 : (g_lc_sort->rank[p] > g_lc_sort->rank[q]);
+}
+
+/* lightCache_t::startUse (lightcache.cc:48-57), plus the candidate grid */
+static void lc_start_use(yo_lcache* C, float maxradius)
+{
+    yo_lbox* boxes = (yo_lbox*)malloc(sizeof(yo_lbox) * (size_t)(C->nb ? C->nb : 1));
+    memcpy(boxes, C->b, sizeof(yo_lbox) * (size_t)C->nb);
+    qsort(boxes, (size_t)C->nb, sizeof(yo_lbox), lc_box_cmp);
+    int32_t* v = (int32_t*)malloc(sizeof(int32_t) * (size_t)(C->n ? C->n : 1));
+    int64_t nv = 0;
+    for (int64_t i = 0; i < C->nb; i++)
+        for (int32_t l = boxes[i].head; l >= 0; l = C->s[l].next) v[nv++] = l;
+    int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (size_t)(C->n ? C->n : 1));
+    int64_t no = 0;
+    lc_tree_order(C, v, nv, 0, 0, 0, order, &no);
+    free(C->rank);
+    C->rank = (int32_t*)malloc(sizeof(int32_t) * (size_t)(C->n ? C->n : 1));
+    for (int64_t i = 0; i < no; i++) C->rank[order[i]] = (int32_t)i;
+    free(order);
+    free(v);
+    free(boxes);
+    /* candidate grid */
+    float lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+    for (int64_t i = 0; i < C->n; i++) {
+        const float q[2] = {C->s[i].pP.x, C->s[i].pP.y};
+        for (int k = 0; k < 2; k++) {
+            if (q[k] < lo[k]) lo[k] = q[k];
+            if (q[k] > hi[k]) hi[k] = q[k];
+        }
+    }
+    if (C->n == 0) lo[0] = lo[1] = hi[0] = hi[1] = 0;
+    C->gcell = maxradius;
+    C->gx0 = lo[0];
+    C->gy0 = lo[1];
+    C->gnx = (int)((hi[0] - lo[0]) / maxradius) + 1;
+    C->gny = (int)((hi[1] - lo[1]) / maxradius) + 1;
+    free(C->gstart);
+    free(C->gitem);
+    C->gstart = (int64_t*)calloc((size_t)C->gnx * C->gny + 1, sizeof(int64_t));
+    C->gitem = (int32_t*)malloc(sizeof(int32_t) * (size_t)(C->n ? C->n : 1));
+    int32_t* cell = (int32_t*)malloc(sizeof(int32_t) * (size_t)(C->n ? C->n : 1));
+    for (int64_t i = 0; i < C->n; i++) {
+        int cx = (int)((C->s[i].pP.x - C->gx0) / C->gcell), cy = (int)((C->s[i].pP.y - C->gy0) / C->gcell);
+        if (cx >= C->gnx) cx = C->gnx - 1;
+        if (cy >= C->gny) cy = C->gny - 1;
+        cell[i] = cy * C->gnx + cx;
+        C->gstart[cell[i] + 1]++;
+    }
+    for (int64_t c = 0; c < (int64_t)C->gnx * C->gny; c++) C->gstart[c + 1] += C->gstart[c];
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * ((size_t)C->gnx * C->gny + 1));
+    memcpy(fill, C->gstart, sizeof(int64_t) * ((size_t)C->gnx * C->gny + 1));
+    for (int64_t i = 0; i < C->n; i++) C->gitem[fill[cell[i]]++] = (int32_t)i;
+    g_lc_sort = C;
+    for (int64_t c = 0; c < (int64_t)C->gnx * C->gny; c++)
+        qsort(C->gitem + C->gstart[c], (size_t)(C->gstart[c + 1] - C->gstart[c]), sizeof(int32_t), lc_item_cmp);
+    free(fill);
+    free(cell);
+}
+
+/* std::push_heap / std::pop_heap of libstdc++ (the reference is built with
+ * g++) with compareFound_f (a.weight > b.weight): the front is the smallest
+ * weight; which of equal weights stays depends on these exact steps */
+static int fs_comp(const yo_fs* a, const yo_fs* b) { return a->weight > b->weight; }
+
+static void fs_sift_up(yo_fs* f, int64_t hole, int64_t top, yo_fs value)
+{
+    int64_t parent = (hole - 1) / 2;
+    while (hole > top && fs_comp(&f[parent], &value)) {
+        f[hole] = f[parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    f[hole] = value;
+}
+
+static void fs_push_heap(yo_fs* f, int64_t len) { fs_sift_up(f, len - 1, 0, f[len - 1]); }
+
+static void fs_pop_heap(yo_fs* f, int64_t len)
+{
+    if (len <= 1) return;
+    const int64_t last = len - 1;
+    yo_fs value = f[last];
+    f[last] = f[0];
+    /* __adjust_heap(first, 0, last, value) */
+    const int64_t n = last;
+    int64_t hole = 0, child = 0;
+    while (child < (n - 1) / 2) {
+        child = 2 * (child + 1);
+        if (fs_comp(&f[child], &f[child - 1])) child--;
+        f[hole] = f[child];
+        hole = child;
+    }
+    if ((n & 1) == 0 && child == (n - 2) / 2) {
+        child = 2 * (child + 1);
+        f[hole] = f[child - 1];
+        hole = child - 1;
+    }
+    fs_sift_up(f, hole, 0, value);
+}
+
+/* one candidate into the found heap (lightcache.cc:147-157, cacheproxy.cc:70-85) */
+static void fs_offer(yo_fs* found, int64_t* nf, yo_fs t, unsigned limit)
+{
+    if (((uint64_t)*nf >= limit) && (t.weight < found[0].weight)) return;
+    found[(*nf)++] = t;
+    fs_push_heap(found, *nf);
+    if ((uint64_t)*nf > limit) {
+        fs_pop_heap(found, *nf);
+        (*nf)--;
+    }
+}
+
+/* lightCache_t::gatherSamples (lightcache.cc:111-180) with weight(); K
+ * candidates at most in found (capacity K + 1) */
+static float lc_gather(const yo_lcache* C, yo_v3 P, yo_v3 pP, yo_v3 N, yo_fs* found, int64_t* nf, unsigned K,
+                       float* radius, float maxradius, unsigned minimun, float wlimit, int32_t* cand)
+{
+    float best = 0, adist = 0;
+    float maxdist = 0, lastradius = 0;
+    const float maxweight = (float)((double)wlimit * 2.5), minweight = (float)((double)wlimit * 0.6);
+    unsigned reached = 0;
+    int repeat = 1;
+    *nf = 0;
+    while (repeat) {
+        /* the tree's candidates within the circle, in its visiting order */
+        int64_t nc = 0;
+        const int cx = (int)floorf((pP.x - C->gx0) / C->gcell), cy = (int)floorf((pP.y - C->gy0) / C->gcell);
+        for (int y = cy - 1; y <= cy + 1; y++)
+            for (int x = cx - 1; x <= cx + 1; x++) {
+                if (x < 0 || y < 0 || x >= C->gnx || y >= C->gny) continue;
+                const int64_t c = (int64_t)y * C->gnx + x;
+                for (int64_t q = C->gstart[c]; q < C->gstart[c + 1]; q++) cand[nc++] = C->gitem[q];
+            }
+        g_lc_sort = C;
+        qsort(cand, (size_t)nc, sizeof(int32_t), lc_item_cmp);
+        for (int64_t q = 0; q < nc; q++) {
+            const yo_ls* S = &C->s[cand[q]];
+            const float pD = v3_len(v3_sub(pP, S->pP));
+            if (pD >= *radius) continue;
+            if (pD < lastradius) continue;
+            reached++;
+            yo_fs t;
+            t.S = cand[q];
+            t.dis = pD;
+            t.weight = lc_weight(S, P, N, maxweight, 1);
+            if (t.weight > best) {
+                best = t.weight;
+                adist = S->adist;
+            }
+            unsigned limit;
+            if (t.weight <= wlimit) limit = minimun;
+            else {
+                if (pD > maxdist) maxdist = pD;
+                limit = K;
+            }
+            if (!limit) continue;
+            fs_offer(found, nf, t, limit);
+        }
+        float rrad = (*nf == 0 || found[0].dis == 0)
+                         ? (float)(0.0001 * (double)adist)
+                         : *radius * v3_len(v3_sub(C->s[found[0].S].P, P)) / found[0].dis;
+        if (rrad == 0.0f) rrad = 1.0f;
+        repeat = ((*nf == 0) || (((adist / rrad) > wlimit) && (reached < K)) || (best <= minweight)) &&
+                 (*radius < maxradius);
+        if (repeat) {
+            lastradius = *radius;
+            *radius *= 2;
+        }
+        if (*radius > maxradius) *radius = maxradius;
+    }
+    if ((maxdist > 0) && ((double)(maxdist / *radius) < (1.0 / sqrt(2.0)))) *radius = (float)((double)*radius * 0.9);
+    return *nf ? found[0].weight : 0.0f;
+}
+
+static void lc_free(yo_lcache* C)
+{
+    if (!C) return;
+    free(C->s); free(C->b); free(C->ht); free(C->rank); free(C->gstart); free(C->gitem);
+    free(C);
+}
+
+/* pathLight_t::cached (pathlight.cc:676-700) at a camera ray's hit */
+static yo_col lc_cached(rctx* R, int li, const yo_sp* sp, yo_v3 eye)
+{
+    const yg_light* L = &R->lights[li];
+    if (col_energy(get_diffuse(R, sp)) < 0.05f) return col(0, 0, 0);
+    const yo_v3 N = face_forward(sp->Ng, sp->N, eye);
+    const float desired = (float)(1.0 / (double)L->shadow_threshold);
+    const float rq = (float)(1.0 / (double)(0 + 1));
+    yo_col total = col(0, 0, 0);
+    const yo_v3 pP = lc_to_polar(R);
+    if (!lc_enough_for(R->lc, sp->P, N, pP, desired * rq)) {
+        float H, M;
+        const int prim = R->primary;
+        R->primary = 0;
+        total = path_take_sample(R, li, N, sp, &H, &M);
+        R->primary = prim;
+        yo_ls smp;
+        memset(&smp, 0, sizeof(smp));
+        smp.N = N; smp.color = total; smp.adist = H; smp.M = M;
+        smp.precision = R->traveled * R->worldres;
+        smp.P = sp->P; smp.pP = pP; smp.devaluated = 1.0f;
+        smp.visit = (int32_t)R->pixnum;
+        lc_insert(R->lc, smp);
+        total = col(1, 1, 1);
+    }
+    return total;
+}
+
+/* pathLight_t::interpolate (pathlight.cc:543-599) with the area's
+ * cacheProxy_t (cacheproxy.cc:17-87) */
+static yo_col lc_interpolate(rctx* R, int li, const yo_sp* sp, yo_v3 eye)
+{
+    const yg_light* L = &R->lights[li];
+    const yo_lcache* C = R->lc;
+    const yo_col dif = get_diffuse(R, sp);
+    if (col_energy(dif) < 0.05f) return col(0, 0, 0);
+    const yo_v3 N = face_forward(sp->Ng, sp->N, eye);
+    const yo_v3 pP = lc_to_polar(R);
+    const float desired = (float)(1.0 / (double)L->shadow_threshold);
+    const float wlimit = (float)(0.8 * (double)desired);
+    const float search_radius = (float)((double)L->cache_size * 2.0);
+    const unsigned K = (unsigned)L->search;
+    /* cacheProxy_t::gatherSamples: a new pixel resets the proxy */
+    if (R->pixnum != R->px_id) {
+        if (R->px_n > 600) R->px_n = 0;
+        R->px_id = R->pixnum;
+    }
+    yo_fs* found = (yo_fs*)malloc(sizeof(yo_fs) * (K + 2));
+    int64_t nf = 0;
+    lc_gather(C, sp->P, pP, N, found, &nf, K, &R->px_radius, search_radius, 0, wlimit, R->lc_cand);
+    const yo_ls* pool = C->s;
+    if (nf == 0) {
+        /* cacheProxy_t::newSearch: the proxy's own new samples */
+        const float maxweight = (float)((double)wlimit * 2.5);
+        for (int i = 0; i < R->px_n; i++) {
+            yo_fs t;
+            t.dis = v3_len(v3_sub(pP, R->px_created[i].pP));
+            if (t.dis > search_radius) continue;
+            t.S = i;
+            t.weight = lc_weight(&R->px_created[i], sp->P, N, maxweight, 1);
+            const unsigned limit = (t.weight <= wlimit) ? 0u : K;
+            if (!limit) continue;
+            fs_offer(found, &nf, t, limit);
+        }
+        pool = R->px_created;
+    }
+    float farest = 0;
+    if (nf) farest = found[0].weight;
+    if (nf == 1) farest = 0;
+    else if (farest > wlimit) farest = wlimit;
+    for (int64_t i = 0; i < nf; i++)
+        found[i].weight = (float)((double)(found[i].weight - farest) * (1.0 - (double)(found[i].dis / search_radius)));
+    yo_col total = col(0, 0, 0);
+    float amount = 0;
+    for (int64_t i = 0; i < nf; i++) {
+        total = col_add(total, col_scale(pool[found[i].S].color, found[i].weight));
+        amount += found[i].weight;
+    }
+    free(found);
+    if (amount != 0) {
+        amount = (float)(1.0 / (double)amount);
+        total = col_scale(total, amount);
+        return col_scale(col_mul(dif, total), L->power);
+    }
+    /* no sample: take one (pathlight.cc:585-597) */
+    float H, M;
+    const int prim = R->primary;
+    R->primary = 0;
+    const yo_col ncol = path_take_sample(R, li, N, sp, &H, &M);
+    R->primary = prim;
+    R->lc_new++;
+    if (R->px_n == R->px_cap) {
+        R->px_cap = R->px_cap ? 2 * R->px_cap : 64;
+        R->px_created = (yo_ls*)realloc(R->px_created, sizeof(yo_ls) * (size_t)R->px_cap);
+    }
+    yo_ls smp;
+    memset(&smp, 0, sizeof(smp));
+    smp.N = N; smp.color = ncol; smp.adist = H; smp.M = M;
+    smp.precision = R->traveled * R->worldres;
+    smp.P = sp->P; smp.pP = pP; smp.devaluated = 1.0f;
+    smp.next = -1; smp.visit = -1;
+    R->px_created[R->px_n++] = smp;
+    return col_scale(col_mul(dif, ncol), L->power);
+}
+
 static void render_area(rctx* R, const render_job* J, int ri)
 {
     const yo_region* a = &J->regions[ri];
@@ -1766,6 +2373,12 @@ static void render_area(rctx* R, const render_job* J, int ri)
         }
     const int W = a->w, H = a->h, resx = J->cam->resx;
     const int passes = o->aa_passes > 0 ? o->aa_passes : 0;
+    /* a fresh cacheProxy_t per area (cacheproxy.cc:10-15): radius = maxradius */
+    for (int li = 0; li < J->nl; li++)
+        if (J->lights[li].type == YG_LIGHT_PATH && J->lights[li].cache)
+            R->px_radius = (float)((double)J->lights[li].cache_size * 2.0);
+    R->px_id = -1;
+    R->px_n = 0;
     yo_col* img = (yo_col*)malloc(sizeof(yo_col) * (size_t)W * H);
     unsigned int sc = 0;
     for (int i = a->y; i < a->y + a->h; ++i)
@@ -1777,6 +2390,10 @@ static void render_area(rctx* R, const render_job* J, int ri)
                 fy = yo_ri_s(sc, 0);
             }
             yo_v3 ray = cam_ray(J->k, (float)j + fx, (float)i + fy);
+            /* state.screenpos and pixelNumber (scene.cc:749-756) */
+            R->scr_x = (float)(2.0 * (double)(((float)j + fx) / (float)resx) - 1.0);
+            R->scr_y = (float)(1.0 - 2.0 * (double)(((float)i + fy) / (float)J->cam->resy));
+            R->pixnum = (int64_t)j + (int64_t)i * resx;
             const int out = j >= a->rx && j < a->rx + a->rw && i >= a->ry && i < a->ry + a->rh;
             R->gpx_on = 0;
             if (R->adj && out) {
@@ -1837,6 +2454,12 @@ static void* render_worker(void* arg)
     render_job* J = (render_job*)arg;
     rctx R;
     rctx_init(&R, J->s, J->mats, J->nmat, J->mods, J->texs, J->lights, J->nl, J->obj_flags, J->nobj, J->opts);
+    if (J->lc) {
+        R.lc = J->lc;
+        R.lc_phase = 2;
+        R.worldres = J->worldres;
+        R.lc_cand = (int32_t*)malloc(sizeof(int32_t) * (size_t)(J->lc->n + 1));
+    }
     double* priv = NULL;
     const size_t ng = 3 * (size_t)J->nmat + 2 * (size_t)J->nmod + (size_t)J->nl;
     if (J->grad_rgb) {
@@ -1856,6 +2479,7 @@ static void* render_worker(void* arg)
     }
     pthread_mutex_lock(&J->mu);
     for (int k = 0; k < 5; k++) J->stats[k] += R.stats[k];
+    J->lc_new += R.lc_new;
     if (priv) {
         for (int q = 0; q < 3 * J->nmat; q++) J->gmat[q] += priv[q];
         if (J->gmod)
@@ -1866,6 +2490,61 @@ static void* render_worker(void* arg)
     free(priv);
     rctx_free(&R);
     return NULL;
+}
+
+/* The cache's fake pass (scene.cc:559-583, fakeRender scene.cc:825-854):
+ * every area in processing order, a fresh renderState_t (samplers) per area,
+ * every pixel (overlap border included) through its centre; the pathlight's
+ * cached() at the camera ray's hit.  The other lights' terms of the fake
+ * pass are discarded by the reference and have no side effect (area lights,
+ * whose draws would, are refused), so they are not evaluated; the fake
+ * pass's camera rays are not counted (the render pass traces the same ones). */
+static void lc_fake_pass(rctx* R, const render_job* J, int li)
+{
+    const int resx = J->cam->resx, resy = J->cam->resy;
+    R->lc_phase = 1;
+    for (int q = 0; q < J->nreg; q++) {
+        const yo_region* a = &J->regions[J->nreg - 1 - q];
+        for (int l = 0; l < J->nl; l++)
+            if (J->lights[l].type == YG_LIGHT_PATH) {
+                int nd = (J->lights[l].depth + 1) * 2, base = 2;
+                for (int d = 0; d < nd; d++) { halton_init(&R->hseq[l][d], base); base = yo_next_prime(base); }
+            }
+        for (int i = a->y; i < a->y + a->h; ++i)
+            for (int j = a->x; j < a->x + a->w; ++j) {
+                /* fakeRender forms the screen position with double literals */
+                R->scr_x = (float)(2.0 * (((double)j + 0.5) / (double)(float)resx) - 1.0);
+                R->scr_y = (float)(1.0 - 2.0 * (((double)i + 0.5) / (double)(float)resy));
+                R->pixnum = (int64_t)j + (int64_t)i * resx;
+                const yo_v3 ray = cam_ray(J->k, (float)j + 0.5f, (float)i + 0.5f);
+                yo_sp sp;
+                const int hit = first_hit(R, J->k->pos, ray, &sp, 0);
+                if (hit) R->traveled = sp.Z;
+                yo_scene* S = (yo_scene*)R->s;
+                if (S->lc_rec) {
+                    if (S->lc_nvis == S->lc_viscap) {
+                        S->lc_viscap = S->lc_viscap ? 2 * S->lc_viscap : 65536;
+                        S->lc_visits = (float*)realloc(S->lc_visits, sizeof(float) * 13 * (size_t)S->lc_viscap);
+                    }
+                    float* o = S->lc_visits + 13 * S->lc_nvis++;
+                    memset(o, 0, sizeof(float) * 13);
+                    o[0] = (float)R->pixnum;
+                    o[1] = (float)q;
+                    if (hit) {
+                        const yo_v3 N = face_forward(sp.Ng, sp.N, v3_sub(J->k->pos, sp.P));
+                        const yo_v3 pP = lc_to_polar(R);
+                        o[2] = col_energy(get_diffuse(R, &sp)) < 0.05f ? 0.0f : 1.0f;
+                        o[3] = sp.P.x; o[4] = sp.P.y; o[5] = sp.P.z;
+                        o[6] = N.x; o[7] = N.y; o[8] = N.z;
+                        o[9] = pP.x; o[10] = pP.y; o[11] = pP.z;
+                        o[12] = (float)sp.mat;
+                    }
+                }
+                if (!hit) continue;
+                (void)lc_cached(R, li, &sp, v3_sub(J->k->pos, sp.P));
+            }
+    }
+    R->lc_phase = 0;
 }
 
 /* scene_t::render(colorOutput_t&) mono loop, scene.cc:538-600; per area
@@ -1888,6 +2567,22 @@ int yo_render_mt(yo_scene* s, const yg_camera* cam, const yg_material* mats, int
     int sequential = 0;
     for (int li = 0; li < nl; li++)
         if (lights[li].type == YG_LIGHT_AREA) sequential = 1;
+    /* the irradiance cache: no refinement (threshold >= 1), no AA passes, no
+     * lights whose fake-pass terms would draw from ourRandom(), no global map */
+    int cli = -1;
+    for (int li = 0; li < nl; li++)
+        if (lights[li].type == YG_LIGHT_PATH && lights[li].cache) {
+            if (cli >= 0) return -7;  /* "Several cached pathlights" (pathlight.cc:41-45) */
+            cli = li;
+        }
+    if (cli >= 0) {
+        if (lights[cli].threshold < 1.0f || opts->aa_passes > 0) return -7;
+        for (int li = 0; li < nl; li++)
+            if ((lights[li].type == YG_LIGHT_AREA && lights[li].use_in_render && !lights[li].dummy) ||
+                lights[li].type == YG_LIGHT_GLOBALPHOTON)
+                return -7;
+        if (!(lights[cli].cache_size > 0) || !(lights[cli].shadow_threshold > 0) || lights[cli].search < 3) return -2;
+    }
     if (nthreads < 1 || sequential) nthreads = 1;
     if (nthreads > 64) nthreads = 64;
     yo_cam k;
@@ -1901,6 +2596,23 @@ int yo_render_mt(yo_scene* s, const yg_camera* cam, const yg_material* mats, int
     J.lights = lights; J.nl = nl; J.obj_flags = obj_flags; J.nobj = nobj; J.opts = opts;
     J.k = &k; J.regions = regions; J.nreg = nreg; J.rgb = rgb; J.next = 0;
     pthread_mutex_init(&J.mu, NULL);
+    if (cli >= 0) {
+        /* pathLight_t::init and the fake pass, then postInit's startUse */
+        yo_lcache* C = (yo_lcache*)calloc(1, sizeof(yo_lcache));
+        C->cache_size = lights[cli].cache_size;
+        const float aspect = (float)cam->resx / (float)cam->resy;
+        C->ycorr = (float)(1.0 / (double)aspect);
+        J.lc = C;
+        J.worldres = (float)((1.0 / (double)(float)cam->resx) / (double)cam->focal);
+        rctx F;
+        rctx_init(&F, s, mats, nmat, mods, texs, lights, nl, obj_flags, nobj, opts);
+        F.lc = C;
+        F.worldres = J.worldres;
+        lc_fake_pass(&F, &J, cli);
+        for (int q = 0; q < 5; q++) J.stats[q] += F.stats[q];
+        rctx_free(&F);
+        lc_start_use(C, (float)((double)lights[cli].cache_size * 2.0));
+    }
     pthread_t th[64];
     if (nthreads == 1) render_worker(&J);
     else {
@@ -1910,7 +2622,80 @@ int yo_render_mt(yo_scene* s, const yg_camera* cam, const yg_material* mats, int
     pthread_mutex_destroy(&J.mu);
     if (stats) for (int q = 0; q < 5; q++) stats[q] = J.stats[q];
     free(regions);
+    if (J.lc) {
+        lc_free(s->lc_last);
+        s->lc_last = J.lc;
+        s->lc_new = J.lc_new;
+    }
     return 0;
+}
+
+void yo_cache_release(yo_scene* s)
+{
+    lc_free(s->lc_last);
+    s->lc_last = NULL;
+    free(s->lc_visits);
+    s->lc_visits = NULL;
+    s->lc_nvis = s->lc_viscap = 0;
+}
+
+/* Test hook: record the next cache render's fake-pass visits, 13 floats
+ * each: pixel number, area (processing order), qualifies (hit, diffuse
+ * energy >= 0.05), P, face-forwarded N, pP, material */
+void yo_cache_record(yo_scene* s, int on)
+{
+    s->lc_rec = on;
+    s->lc_nvis = 0;
+}
+
+int64_t yo_cache_visits(yo_scene* s, float* out, int64_t cap)
+{
+    if (out) memcpy(out, s->lc_visits, sizeof(float) * 13 * (size_t)(cap < s->lc_nvis ? cap : s->lc_nvis));
+    return s->lc_nvis;
+}
+
+/* Test hook: gatherSamples' offer of candidates with weights w[0..n) (ids
+ * 0..n-1, in that order) into a found heap of at most `limit`; out: the
+ * final heap array (id, weight) in its array order; returns its size */
+int64_t yo_heap_offer_trace(const float* w, int64_t n, unsigned limit, float* out)
+{
+    yo_fs* f = (yo_fs*)malloc(sizeof(yo_fs) * ((size_t)limit + 2));
+    int64_t nf = 0;
+    for (int64_t i = 0; i < n; i++) {
+        yo_fs t;
+        t.S = (int32_t)i;
+        t.dis = 0;
+        t.weight = w[i];
+        fs_offer(f, &nf, t, limit);
+    }
+    for (int64_t i = 0; i < nf; i++) {
+        out[2 * i] = (float)f[i].S;
+        out[2 * i + 1] = f[i].weight;
+    }
+    free(f);
+    return nf;
+}
+
+/* The samples of the last cache render: 18 floats each, in insertion order:
+ * P, N, pP, colour, adist, M, precision, generic-tree rank, fake-pass visit
+ * (pixel number), devaluated.  Returns the count; *new_samples: samples the
+ * render pass took (interpolate found none). */
+int64_t yo_cache_read(yo_scene* s, float* out, int64_t cap, uint64_t* new_samples)
+{
+    const yo_lcache* C = s->lc_last;
+    if (new_samples) *new_samples = s->lc_new;
+    if (!C) return 0;
+    for (int64_t i = 0; i < C->n && i < cap; i++) {
+        const yo_ls* S = &C->s[i];
+        float* o = out + 18 * i;
+        o[0] = S->P.x; o[1] = S->P.y; o[2] = S->P.z;
+        o[3] = S->N.x; o[4] = S->N.y; o[5] = S->N.z;
+        o[6] = S->pP.x; o[7] = S->pP.y; o[8] = S->pP.z;
+        o[9] = S->color.r; o[10] = S->color.g; o[11] = S->color.b;
+        o[12] = S->adist; o[13] = S->M; o[14] = S->precision;
+        o[15] = (float)C->rank[i]; o[16] = (float)S->visit; o[17] = S->devaluated;
+    }
+    return C->n;
 }
 
 /* scene_t::raytrace of camera rays through camera-plane points (px[q], py[q])

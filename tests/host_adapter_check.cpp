@@ -124,7 +124,20 @@ int main()
     // unsupported features are refused with a status, not an exit
     paramMap_t cache;
     cache.setBool("cache", true);
+    cache.setBool("show_samples", true);
     printf("refused %d\n", I.addLight("cached", "pathlight", cache));
+    // the irradiance cache's parameters (pathlight.cc:785-797): search at least 3
+    {
+        paramMap_t pc;
+        pc.setBool("cache", true);
+        pc.setBool("use_QMC", true);
+        pc.setFloat("cache_size", 0.02f);
+        pc.setFloat("shadow_threshold", 0.25f);
+        pc.setInt("search", 1);
+        yg_light L{};
+        const int rc = yafgpu::lightFactory("pathlight", pc, L);
+        printf("pcache %d %d %g %g %d\n", rc, L.cache, L.cache_size, L.shadow_threshold, L.search);
+    }
     // warnings of a finished render: adaptive repetitions counted but not run
     yg_render_stats st{};
     printf("warn0 %d\n", (int)yafgpu::gpuInterface_t::renderWarnings(st).size());

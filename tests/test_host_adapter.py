@@ -60,6 +60,13 @@ def test_unsupported_is_refused(adapter_output):
     assert [l for l in adapter_output if l[0] == "refused"] == [["refused", str(D_ERR_UNSUPPORTED)]]
 
 
+def test_cache_parameters(adapter_output):
+    """pathLight_t::factory with cache on reads cache_size, shadow_threshold
+    and search, raising search to 3 (pathlight.cc:785-797)."""
+    row = [l for l in adapter_output if l[0] == "pcache"][0]
+    assert row[1:] == ["0", "1", "0.02", "0.25", "3"]
+
+
 D_ERR_UNSUPPORTED = -4
 
 

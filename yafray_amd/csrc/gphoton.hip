@@ -515,6 +515,20 @@ extern "C" int yg_gphoton_build(yg_scene* s, uint32_t li, const yg_render_opts* 
         if (L.type == YG_LIGHT_AREA && L.dummy) em.emplace_back();
     GPhotonDev& G = s->gp;
     gphoton_free(G);
+    // YG_RENDER_KTIMING: events around the tracing and pre-gathering launches
+    const bool ktime = (opts->flags & YG_RENDER_KTIMING) != 0;
+    struct Events {
+        hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+        ~Events()
+        {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } evs;
+    hipEvent_t* ev = evs.e;
+    if (ktime)
+        for (int i = 0; i < 4; i++) YG_HIP(hipEventCreate(&ev[i]));
+    bool irr_timed = false;
     uint64_t nph = 0, nrec = 0, nbox = 0;
     unsigned long long nrays = 0;
     GFreer F;
@@ -563,11 +577,13 @@ extern "C" int yg_gphoton_build(yg_scene* s, uint32_t li, const yg_render_opts* 
         YG_HIP(hipMemcpyAsync(d_em, em.data(), sizeof(GpEmit) * em.size(), hipMemcpyHostToDevice, st));
         YG_HIP(hipMemsetAsync(valid, 0, sizeof(uint32_t) * nslots, st));
         YG_HIP(hipMemsetAsync(d_rays, 0, sizeof(unsigned long long), st));
+        if (ktime) YG_HIP(hipEventRecord(ev[0], st));
         YG_LAUNCH_W(s->width, (k_gp_trace<8>), (k_gp_trace<4>), (k_gp_trace<2>), dim3(grid_for(nph, GB)), dim3(GB),
                     stack_lds_bytes(s->stack_entries, GB), st, d_em, ppl, (uint32_t)nph, GL.depth, opts->lcg_seed,
                     budget, s->view(), s->d_mats, s->d_mods, s->d_texs, s->d_obj_flags, opts->min_raydis, G.cell,
                     slots, valid, d_rays);
         YG_HIP(hipGetLastError());
+        if (ktime) YG_HIP(hipEventRecord(ev[1], st));
         YG_HIP(hipMemcpyAsync(&nrays, d_rays, sizeof(nrays), hipMemcpyDeviceToHost, st));
         if ((rc = gp_scan(F, valid, vpos, nslots, st))) return rc;
         nrec = gp_last_sum(valid, vpos, nslots, st);
@@ -672,10 +688,13 @@ extern "C" int yg_gphoton_build(yg_scene* s, uint32_t li, const yg_render_opts* 
             int* hidx;
             if ((rc = F.alloc(&hd, (size_t)GL.search * nbox)) || (rc = F.alloc(&hidx, (size_t)GL.search * nbox)))
                 return rc;
+            if (ktime) YG_HIP(hipEventRecord(ev[2], st));
             hipLaunchKernelGGL(k_gp_irr, dim3(grid_for(nbox, 64)), dim3(64), 0, st, (uint32_t)nbox, GL.search, G.radius,
                                G.d_pos, bcode, brgbe, G.d_N, G.d_gstart, G.d_gitems, G.gmin[0], G.gmin[1], G.gmin[2],
                                G.gdim[0], G.gdim[1], G.gdim[2], tab, hd, hidx, G.d_irr);
             YG_HIP(hipGetLastError());
+            if (ktime) YG_HIP(hipEventRecord(ev[3], st));
+            irr_timed = ktime;
             hipLaunchKernelGGL(k_gp_finish, dim3(grid_for(nbox, 256)), dim3(256), 0, st, (uint32_t)nbox, G.d_pos, bcode,
                                brgbe, G.d_N, G.d_irr, tab, G.d_qN, G.d_qirr, G.d_read);
             YG_HIP(hipGetLastError());
@@ -690,8 +709,12 @@ extern "C" int yg_gphoton_build(yg_scene* s, uint32_t li, const yg_render_opts* 
         stats->stored = nrec;
         stats->clustered = nbox;
         stats->ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        stats->rays = nrays;
+        stats->ms_trace = 0.f;
+        stats->ms_irr = 0.f;
+        if (ktime && nph > 0) YG_HIP(hipEventElapsedTime(&stats->ms_trace, ev[0], ev[1]));
+        if (irr_timed) YG_HIP(hipEventElapsedTime(&stats->ms_irr, ev[2], ev[3]));
     }
-    (void)nrays;
     return YG_OK;
 }
 

@@ -63,7 +63,8 @@ class yg_light(C.Structure):
                 ("threshold", C.c_float), ("from_", F3), ("to", F3), ("angle", C.c_float),
                 ("photons", C.c_int32), ("search", C.c_int32), ("mindepth", C.c_int32),
                 ("bias", C.c_float), ("fixedradius", C.c_float), ("cluster", C.c_float),
-                ("dummy", C.c_int32)]
+                ("dummy", C.c_int32), ("cache", C.c_int32), ("cache_size", C.c_float),
+                ("shadow_threshold", C.c_float)]
 
 
 class yg_render_opts(C.Structure):
@@ -113,7 +114,7 @@ class yg_tune_result(C.Structure):
 
 class yg_photon_stats(C.Structure):
     _fields_ = [("emitted", C.c_uint64), ("stored", C.c_uint64), ("clustered", C.c_uint64),
-                ("ms", C.c_float)]
+                ("ms", C.c_float), ("rays", C.c_uint64), ("ms_trace", C.c_float), ("ms_irr", C.c_float)]
 
 
 YG_TEX_CLOUDS, YG_TEX_MARBLE, YG_TEX_WOOD, YG_TEX_MUSGRAVE = 1, 2, 3, 4
@@ -173,7 +174,10 @@ def globalphotonlight(radius=1.0, depth=2, photons=50000, search=200):
     return L
 
 
-def pathlight(samples=16, power=1.0, depth=3, use_qmc=False, threshold=0.1):
+def pathlight(samples=16, power=1.0, depth=3, use_qmc=False, threshold=0.1, cache=False, cache_size=0.01,
+              shadow_threshold=0.3, search=50):
+    """pathLight_t::factory (pathlight.cc:756-812): cache parameters are read
+    only with cache on, search at least 3."""
     L = yg_light()
     L.type = YG_LIGHT_PATH
     L.use_in_render = 1
@@ -183,6 +187,10 @@ def pathlight(samples=16, power=1.0, depth=3, use_qmc=False, threshold=0.1):
     L.depth = depth
     L.use_qmc = int(use_qmc)
     L.threshold = threshold
+    L.cache = int(cache)
+    L.cache_size = cache_size
+    L.shadow_threshold = shadow_threshold
+    L.search = max(3, search)
     return L
 
 

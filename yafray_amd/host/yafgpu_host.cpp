@@ -119,7 +119,23 @@ int lightFactory(const std::string& type, const paramMap_t& p, yg_light& L)
             std::cerr << "[Warning] Samples value too low, minimum is one\n";
             L.samples = 1;
         }
-        if (cache) return fail(YG_ERR_UNSUPPORTED, "pathlight: cache=on is not on the GPU path");
+        L.cache = cache ? 1 : 0;
+        L.cache_size = 0.01f;
+        L.shadow_threshold = 0.3f;
+        L.search = 50;
+        if (cache) {
+            bool show = false, grad = false, ignorms = false;
+            p.getParam("cache_size", L.cache_size);
+            p.getParam("shadow_threshold", L.shadow_threshold);
+            p.getParam("search", L.search);
+            p.getParam("show_samples", show);
+            p.getParam("gradient", grad);
+            p.getParam("ignore_bumpnormals", ignorms);
+            if (L.search < 3) L.search = 3;
+            if (show) return fail(YG_ERR_UNSUPPORTED, "pathlight: show_samples is not on the GPU path");
+            if (ignorms) return fail(YG_ERR_UNSUPPORTED, "pathlight: ignore_bumpnormals is not on the GPU path");
+            (void)grad;  // read and unused by the reference (pathlight.cc:790)
+        }
         if (direct) return fail(YG_ERR_UNSUPPORTED, "pathlight: direct=on is not on the GPU path");
         if (mode == "occlusion") return fail(YG_ERR_UNSUPPORTED, "pathlight: occlusion mode is not on the GPU path");
         if (!qmc) return fail(YG_ERR_UNSUPPORTED, "pathlight: use_QMC=off (LCG sampler) is not on the GPU path");
