@@ -734,6 +734,89 @@ def _gphoton_cpu_legs(d, img, rst, sc, torch):
     }
 
 
+def bench_cache(args):
+    """Pathlight with the irradiance cache (cache = on) on the 1,045,506-
+    triangle fBm terrain at 1280x720: 64 samples per cache sample, 3
+    bounces, cache_size 0.01, search 50, shadow_threshold 0.3 (the
+    factory's defaults), threshold 1e3 (no refinement passes).  One step =
+    the whole frame: camera rays, the fake pass that places the samples,
+    their takeSample paths, the cache's tree order and the render pass's
+    interpolation plus the sun.
+
+    Also: device time of the GI levels' extension and shadow launches (HIP
+    events), the frame against the oracle's restatement of the reference
+    (image and every sample), and the oracle's frame on the host cores as
+    the CPU baseline (its fake pass is sequential, as the reference's)."""
+    import numpy as np
+    import torch
+
+    from yafray_amd import descriptors as D
+    from yafray_amd import lib as Y
+    from yafray_amd import scenes as S
+
+    d0 = S.terrain_gi(Y.device_heights)
+    cached = D.pathlight(samples=64, depth=3, use_qmc=True, threshold=1e3, cache=True, cache_size=0.01, search=50,
+                         shadow_threshold=0.3)
+    lights = [cached if L.type == D.YG_LIGHT_PATH else L for L in d0.lights]
+    d = D.SceneDesc(d0.verts, d0.idx, d0.tri_obj, d0.tri_mat, d0.obj_flags, d0.cam, list(d0.mats), list(d0.mods),
+                    list(d0.texs), lights, d0.opts)
+    sc = Y.Scene.from_desc(d)
+    out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+    dt, st = _timed(lambda: sc.render(out), args.steps, args.warmup)
+    kt = D.yg_render_opts.from_buffer_copy(d.opts)
+    kt.flags = 3
+    ks = sc.render(out, kt)
+    torch.cuda.synchronize()
+    img = out.cpu().numpy().reshape(d.cam.resy, d.cam.resx, 3)
+    if not np.all(np.isfinite(img)):
+        raise RuntimeError("cache: non-finite pixels in the GPU image")
+    gs = sc.cache_read()
+    rays = int(st.rays_camera + st.rays_extension + st.rays_shadow)
+    bm = json.load(open(os.path.join(ROOT, "tests", "golden", "byte_model.json")))
+    ext_b, sh_b = bm["extension"]["B_ray"], bm["shadow"]["B_ray"]
+    bshadow = int(ks.rays_bounce - ks.rays_extension)
+    kern = {"k_ext": {"ms": float(ks.ms_ext), "bytes": int(ks.rays_extension) * ext_b, "rays": int(ks.rays_extension)},
+            "k_shadow": {"ms": float(ks.ms_shadow), "bytes": bshadow * sh_b, "rays": bshadow}}
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    launches = max(int(ks.bounce_launches), 1)
+    achieved = kern[dom]["bytes"] / (kern[dom]["ms"] * 1e-3) / 1e9 if kern[dom]["ms"] > 0 else 0.0
+    line = {
+        "metric": "cached GI ms/frame", "value": round(dt * 1e3, 3), "unit": "ms/frame",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "pathlight irradiance cache on the 1,045,506-triangle fBm terrain, 1280x720, "
+                               "64 samples per cache sample, 3 bounces, cache_size 0.01, search 50, "
+                               "shadow_threshold 0.3, sun + sky", "triangles": int(d.n_tris)},
+        "cache_samples": int(len(gs)), "rays_per_frame": rays, "mrays_per_s": round(rays / dt / 1e6, 3),
+        "kernels_ms": {k: round(v["ms"], 3) for k, v in kern.items()},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "ms_per_launch": round(kern[dom]["ms"] / launches, 4),
+                     "alg_bytes_per_launch": int(kern[dom]["bytes"] / launches),
+                     "model": "tests/golden/byte_model.json (terrain GI rays, canonical BVH2)"},
+    }
+    if not args.no_cpu:
+        from oracle import pyoracle as po
+        threads = cpu_cores()
+        oc = po.OracleScene.from_desc(d)
+        t0 = time.perf_counter()
+        ref, rs = oc.render(d, threads=threads)
+        cpu_dt = time.perf_counter() - t0
+        os_, nnew = oc.cache_read()
+        rel = float(np.sqrt(np.mean((img.astype(np.float64) - ref) ** 2)) / np.sqrt(np.mean(ref.astype(np.float64) ** 2)))
+        same = len(os_) == len(gs) and bool(np.array_equal(os_[:, 16], gs[:, 16]))
+        line["cpu_baseline"] = {"value": round(cpu_dt * 1e3, 1), "unit": "ms/frame", "cores": threads, "kind": "port",
+                                "sample": "the whole frame by the oracle: fake pass and its samples on one thread "
+                                          "(sequential in the reference), render pass on %d threads" % threads}
+        line["rmse_vs_oracle"] = {"rel_rmse": rel, "exact_px": float(np.mean(np.all(img == ref, axis=2))),
+                                  "same_samples": same, "oracle_samples": int(len(os_)),
+                                  "render_pass_new_samples": int(nnew)}
+        if not (rel <= 1e-3 and same):
+            raise RuntimeError("cache parity failed: rel rmse %.3g, same samples %s" % (rel, same))
+    return line
+
+
 VALU_FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak (packed FMA); unpacked code reaches half
 FBM_FLOP_PER_POINT = 384       # 6 octaves x ~64 FLOP (improved Perlin: 8 gradients, 7 lerps, 3 fades)
 
@@ -1012,7 +1095,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="gi",
                     choices=["gi", "batched", "adjoint", "photon", "direct", "autotune", "aa", "rays",
-                             "noise", "room_gi", "gphoton"])
+                             "noise", "room_gi", "gphoton", "cache"])
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--batch-mode", default="both", choices=["both", "sequential", "batched"],
                     help="batched workload: time both modes (default) or one of them alone")
@@ -1031,7 +1114,7 @@ def main():
         fn = {"batched": bench_batched, "adjoint": bench_adjoint, "photon": bench_photon,
               "direct": bench_direct, "autotune": bench_autotune, "aa": bench_aa,
               "rays": bench_rays, "noise": bench_noise, "room_gi": bench_room_gi,
-              "gphoton": bench_gphoton}[args.workload]
+              "gphoton": bench_gphoton, "cache": bench_cache}[args.workload]
         print(json.dumps(fn(args)))
         return
 

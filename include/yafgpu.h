@@ -497,6 +497,14 @@ int yg_photon_gather(yg_scene* s, const float* d_p, const float* d_n, const floa
 int yg_gphoton_build(yg_scene* s, uint32_t li, const yg_render_opts* opts, yg_photon_stats* stats,
                      void* stream);
 
+/* The irradiance cache of the scene's last yg_render whose pathlight has
+ * cache on (pathlight.cc:405-723; no counterpart call in the reference, which
+ * keeps it inside the light): its samples in insertion order, 18 floats
+ * each -- P, N, polar position, colour, harmonic-mean distance, minimum
+ * distance, precision, rank in the cache's tree order, pixel number, 1 --
+ * into d_out (capacity in samples); *count = the number of samples. */
+int yg_cache_read(yg_scene* s, float* d_out, uint64_t capacity, uint64_t* count, void* stream);
+
 /* Copy the irradiance boxes (15 floats each: position, normal N, irradiance,
  * merged photon direction, merged photon colour; in box order x, y, z) to
  * d_out (capacity in boxes). */

@@ -955,7 +955,8 @@ __global__ __launch_bounds__(RB) void k_finalize(const RenderScene* __restrict__
                                                  const c3* __restrict__ lightc, int nl_max,
                                                  const c3* __restrict__ contri,
                                                  const uint8_t* __restrict__ nseg,
-                                                 Counters* __restrict__ cnt, c3* __restrict__ item_col)
+                                                 Counters* __restrict__ cnt, c3* __restrict__ item_col,
+                                                 const c3* __restrict__ cache_pathc)
 {
     const uint32_t item = blockIdx.x * RB + threadIdx.x;
     uint32_t extra_n = 0;
@@ -969,7 +970,10 @@ __global__ __launch_bounds__(RB) void k_finalize(const RenderScene* __restrict__
         if (ti < 0) col = R.bg;
         else {
             c3 pathc = mkc(0, 0, 0);
-            if (R.path_light >= 0 && qual[item]) {
+            if (cache_pathc) {
+                // the irradiance cache's interpolation (k_cache_interp)
+                if (R.path_light >= 0 && qual[item]) pathc = cache_pathc[item];
+            } else if (R.path_light >= 0 && qual[item]) {
                 c3 total = mkc(0, 0, 0), sub[4];
                 for (int q = 0; q < 4; q++) sub[q] = mkc(0, 0, 0);
                 for (int smp = 0; smp < S; smp++) {
@@ -1015,6 +1019,8 @@ __global__ __launch_bounds__(RB) void k_finalize(const RenderScene* __restrict__
     }
     block_add(&cnt->extra, extra_n);
 }
+
+#include "lcache_dev.h"
 
 // ------------------------------------------------------------------ host --
 
@@ -1228,6 +1234,16 @@ struct RenderWS {
     int aa_pass = -1;             // pass of the launch being enqueued (-1: first pass)
     bool hbase_on = false;        // per-area Halton counters carried across passes
     c3* finalize_to = nullptr;    // per-item colours instead of image pixels
+    // irradiance cache (pathlight cache = on): samples in item slots, the fake
+    // pass's area levels and candidate lists, the render pass's grid
+    DevBuf<f3> lc_pP, lc_P, lc_N;
+    DevBuf<float> lc_adist, lc_M, lc_prec;
+    DevBuf<c3> lc_col, lc_pathc;
+    DevBuf<int32_t> lc_rank, lc_row, lc_lvl, lc_adjs, lc_adj, lc_gstart;
+    DevBuf<uint32_t> lc_in, lc_list, lc_cnt, lc_nb, lc_qual0, lc_gitem;
+    DevBuf<unsigned long long> lc_err;
+    std::vector<uint32_t> lc_ins, lc_pix;  // the last cache frame's samples (items, pixels) in insertion order
+    uint64_t lc_uid = 0;                    // its scene
     // the single-scene render the workspace holds (for YG_RENDER_REUSE)
     uint64_t last_uid = 0;  // yg_scene::uid, 0 = none
     uint64_t last_gen = 0;
@@ -1284,6 +1300,10 @@ struct ScenePlan {
     int aa_passes = 0;
     float aa_threshold = 0;
     bool area_lcg = false;  // antialiasing with area lights: areas in turn
+    // irradiance cache of the pathlight (cache = on)
+    bool cache = false;
+    float cache_size = 0, shadow_threshold = 0;
+    int search = 0;
 };
 
 static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, ScenePlan& P)
@@ -1356,6 +1376,21 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
             R.depth = Lg.depth;
             R.path_power = Lg.power;
             R.threshold = Lg.threshold;
+            if (Lg.cache) {
+                // pathLight_t with cache (pathlight.cc:405-723): the GPU path
+                // renders it without refinement passes and antialiasing
+                if (Lg.threshold < 1.0f)
+                    return yg_fail(YG_ERR_UNSUPPORTED,
+                                   "pathlight cache with threshold < 1 (refinement passes) is not on the GPU path");
+                if (!(Lg.cache_size > 0) || !(Lg.shadow_threshold > 0))
+                    return yg_fail(YG_ERR_ARG, "pathlight cache_size and shadow_threshold must be > 0");
+                if (Lg.search < 3 || Lg.search > LC_KMAX)
+                    return yg_fail(YG_ERR_UNSUPPORTED, "pathlight cache search must be 3..1000");
+                P.cache = true;
+                P.cache_size = Lg.cache_size;
+                P.shadow_threshold = Lg.shadow_threshold;
+                P.search = Lg.search;
+            }
         } else if (Lg.type == YG_LIGHT_PHOTON) {
             if (R.photon_light >= 0) return yg_fail(YG_ERR_UNSUPPORTED, "more than one photonlight");
             if (s->pm.light != li) return yg_fail(YG_ERR_STATE, "photon map not built (yg_photon_build)");
@@ -1375,6 +1410,16 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
         // samples by photonSampler_t (pathlight.cc:731), which is not implemented
         if (R.path_light >= 0 && R.samples > 96)
             return yg_fail(YG_ERR_UNSUPPORTED, "pathlight samples > 96 with a global photon map (photonSampler_t)");
+    }
+    if (P.cache) {
+        if (o->aa_passes > 0)
+            return yg_fail(YG_ERR_UNSUPPORTED, "pathlight cache with antialiasing passes is not on the GPU path");
+        if (R.gp_on)
+            return yg_fail(YG_ERR_UNSUPPORTED, "pathlight cache with a global photon map is not on the GPU path");
+        for (const auto& Lg : s->lights)
+            if (Lg.type == YG_LIGHT_AREA && !Lg.dummy && Lg.use_in_render)
+                return yg_fail(YG_ERR_UNSUPPORTED,
+                               "pathlight cache with area lights (their fake-pass draws) is not on the GPU path");
     }
     if (R.path_light >= 0) {
         if (R.depth > 255) return yg_fail(YG_ERR_UNSUPPORTED, "pathlight depth");
@@ -1663,7 +1708,7 @@ static int launch_frame(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
         if (before_final) YG_HIP(hipEventRecord(before_final, st));
         hipLaunchKernelGGL(k_finalize, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
                            nitems, W.S, W.D, W.item_ti.p, W.item_t.p, W.qual.p, W.lightc.p, nl_max, W.contri.p,
-                           W.nseg.p, W.cnt.p, W.finalize_to);
+                           W.nseg.p, W.cnt.p, W.finalize_to, (const c3*)nullptr);
         YG_HIP(hipGetLastError());
         return YG_OK;
     }
@@ -1696,8 +1741,311 @@ static int launch_frame(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
     if (before_final) YG_HIP(hipEventRecord(before_final, st));
     hipLaunchKernelGGL(k_finalize, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
                        nitems, W.S, W.D, W.item_ti.p, W.item_t.p, W.qual.p, W.lightc.p, nl_max, W.contri.p,
-                       W.nseg.p, W.cnt.p, W.finalize_to);
+                       W.nseg.p, W.cnt.p, W.finalize_to, (const c3*)nullptr);
     YG_HIP(hipGetLastError());
+    return YG_OK;
+}
+
+// ------------------------------------------------- irradiance cache (host) --
+
+// hash3d_t::getBox (hash3d.h:224-234)
+static void lc_getbox(float cell, f3 p, int32_t& nx, int32_t& ny, int32_t& nz)
+{
+    nx = (int32_t)(p.x / cell);
+    ny = (int32_t)(p.y / cell);
+    nz = (int32_t)(p.z / cell);
+    if (p.x < 0.0f) nx--;
+    if (p.y < 0.0f) ny--;
+    if (p.z < 0.0f) nz--;
+}
+
+// Depth-first leaf order of buildGenericTree (bound.h:252-340) over v (leaf
+// size 1): split at the mean of the longest axis of the bound, the points on
+// the cut in both halves' bounds going to a third part; an axis whose split
+// separates nothing is skipped.  Appends to out.
+static void lc_tree_order(const std::vector<f3>& pp, std::vector<uint32_t>& v, bool sx, bool sy, bool sz,
+                          std::vector<uint32_t>& out)
+{
+    const size_t n = v.size();
+    if (n <= 1 || (sx && sy && sz)) {
+        out.insert(out.end(), v.begin(), v.end());
+        return;
+    }
+    float lo[3], hi[3];
+    {
+        const f3 p0 = pp[v[0]];
+        lo[0] = hi[0] = p0.x; lo[1] = hi[1] = p0.y; lo[2] = hi[2] = p0.z;
+        for (uint32_t id : v) {
+            const float q[3] = {pp[id].x, pp[id].y, pp[id].z};
+            for (int k = 0; k < 3; k++) {
+                if (q[k] > hi[k]) hi[k] = q[k];
+                if (q[k] < lo[k]) lo[k] = q[k];
+            }
+        }
+    }
+    const float lx = hi[0] - lo[0], ly = hi[1] - lo[1], lz = hi[2] - lo[2];
+    int axis;
+    if (((lx >= ly) || sy) && ((lx >= lz) || sz) && !sx) axis = 0;
+    else if (((ly >= lx) || sx) && ((ly >= lz) || sz) && !sy) axis = 1;
+    else axis = 2;
+    auto coord = [&](uint32_t id) { return axis == 0 ? pp[id].x : (axis == 1 ? pp[id].y : pp[id].z); };
+    float media = 0;
+    for (uint32_t id : v) media += coord(id);
+    media /= (float)n;
+    auto inside = [&](uint32_t id, const float* a, const float* g) {
+        const float q[3] = {pp[id].x, pp[id].y, pp[id].z};
+        return q[0] >= a[0] && q[0] <= g[0] && q[1] >= a[1] && q[1] <= g[1] && q[2] >= a[2] && q[2] <= g[2];
+    };
+    float la[3] = {lo[0], lo[1], lo[2]}, lg[3] = {hi[0], hi[1], hi[2]};
+    float ra[3] = {lo[0], lo[1], lo[2]}, rg[3] = {hi[0], hi[1], hi[2]};
+    lg[axis] = media;
+    ra[axis] = media;
+    std::vector<uint32_t> vl, vr, vm;
+    for (uint32_t id : v) {
+        if (inside(id, la, lg)) {
+            if (inside(id, ra, rg)) vm.push_back(id);
+            else vl.push_back(id);
+        } else
+            vr.push_back(id);
+    }
+    if (vl.size() == n || vr.size() == n || vm.size() == n) {
+        lc_tree_order(pp, v, sx || axis == 0, sy || axis == 1, sz || axis == 2, out);
+        return;
+    }
+    std::vector<uint32_t>().swap(v);
+    if (vl.empty()) {
+        lc_tree_order(pp, vr, sx, sy, sz, out);
+        lc_tree_order(pp, vm, sx, sy, sz, out);
+    } else if (vr.empty()) {
+        lc_tree_order(pp, vl, sx, sy, sz, out);
+        lc_tree_order(pp, vm, sx, sy, sz, out);
+    } else {
+        lc_tree_order(pp, vl, sx, sy, sz, out);
+        lc_tree_order(pp, vr, sx, sy, sz, out);
+        if (!vm.empty()) lc_tree_order(pp, vm, sx, sy, sz, out);
+    }
+}
+
+// One frame of a pathlight with the irradiance cache, single scene
+// (scene_t::render with repeatFirst, scene.cc:559-603): camera rays; the
+// fake pass, area level by area level (k_cache_fill); the samples'
+// estimates through the GI levels; startUse's tree order and the candidate
+// grid (host); the render pass: the other lights, the interpolation
+// (k_cache_interp) and the pixel sums.
+static int cache_frame(const ScenePlan& P, hipStream_t st, bool ktiming, std::vector<hipEvent_t>& kev,
+                       hipEvent_t before_final)
+{
+    RenderWS& W = g_ws;
+    const uint32_t nitems = W.nitems;
+    const int na = W.na, nl_max = W.nl_max;
+    const RenderScene& R = P.R;
+    int rc;
+    YG_LAUNCH_W(W.width, (k_primary<8>), (k_primary<4>), (k_primary<2>), dim3(grid_for(nitems, RB)), dim3(RB),
+                W.stk_bytes_lvl, st, W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.cnt.p, 1,
+                W.lds_cap, W.stk_ovf.p);
+    YG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_prep, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na, nitems,
+                       W.item_ti.p, W.item_t.p, W.draws.p, W.qual.p, (uint32_t*)nullptr);
+    YG_HIP(hipGetLastError());
+    if ((rc = W.lc_pP.ensure(nitems)) || (rc = W.lc_P.ensure(nitems)) || (rc = W.lc_N.ensure(nitems)) ||
+        (rc = W.lc_adist.ensure(nitems)) || (rc = W.lc_M.ensure(nitems)) || (rc = W.lc_prec.ensure(nitems)) ||
+        (rc = W.lc_col.ensure(nitems)) || (rc = W.lc_pathc.ensure(nitems)) || (rc = W.lc_rank.ensure(nitems)) ||
+        (rc = W.lc_row.ensure(nitems)) || (rc = W.lc_in.ensure(nitems)) || (rc = W.lc_list.ensure(nitems)) ||
+        (rc = W.lc_cnt.ensure((size_t)na)) || (rc = W.lc_nb.ensure(nitems)) || (rc = W.lc_qual0.ensure(nitems)) ||
+        (rc = W.lc_err.ensure(2)))
+        return rc;
+    YG_HIP(hipMemsetAsync(W.lc_in.p, 0, sizeof(uint32_t) * nitems, st));
+    YG_HIP(hipMemsetAsync(W.lc_cnt.p, 0, sizeof(uint32_t) * na, st));
+    YG_HIP(hipMemsetAsync(W.lc_err.p, 0, sizeof(unsigned long long) * 2, st));
+    CacheView cv;
+    cv.pP = W.lc_pP.p; cv.P = W.lc_P.p; cv.N = W.lc_N.p; cv.adist = W.lc_adist.p; cv.M = W.lc_M.p;
+    cv.prec = W.lc_prec.p; cv.col = W.lc_col.p; cv.rank = W.lc_rank.p; cv.in = W.lc_in.p; cv.list = W.lc_list.p;
+    cv.cnt = W.lc_cnt.p; cv.row = W.lc_row.p; cv.nb = W.lc_nb.p; cv.err = W.lc_err.p;
+    CacheParams cp{};
+    cp.cs = P.cache_size;
+    cp.ycorr = (float)(1.0 / (double)((float)R.resx / (float)R.resy));
+    cp.worldres = (float)((1.0 / (double)(float)R.resx) / (double)P.s->cam.focal);
+    const float desired = (float)(1.0 / (double)P.shadow_threshold);
+    cp.wfill = desired * (float)(1.0 / (double)(0 + 1));
+    cp.wlimit = (float)(0.8 * (double)desired);
+    cp.sradius = (float)((double)P.cache_size * 2.0);
+    cp.K = P.search;
+    // pixels within cache_size of a polar position: one pixel is 2 / resx in
+    // both screen coordinates (y is scaled by 1 / aspect)
+    cp.reach = (int)std::ceil((double)P.cache_size * (double)R.resx * 0.5) + 1;
+    // area levels of the fake pass: an area waits for the earlier areas
+    // whose samples can reach its pixels
+    const std::vector<Area>& A = P.areas;
+    std::vector<int> level(na, 0);
+    std::vector<int32_t> adjs(na + 1, 0), adj;
+    for (int k = 0; k < na; k++) {
+        for (int j = 0; j < k; j++) {
+            const int gx = std::max(A[j].x - (A[k].x + A[k].w - 1), A[k].x - (A[j].x + A[j].w - 1));
+            const int gy = std::max(A[j].y - (A[k].y + A[k].h - 1), A[k].y - (A[j].y + A[j].h - 1));
+            if (gx <= cp.reach && gy <= cp.reach) {
+                adj.push_back(j);
+                level[k] = std::max(level[k], level[j] + 1);
+            }
+        }
+        adjs[k + 1] = (int32_t)adj.size();
+    }
+    const int nlev = na ? *std::max_element(level.begin(), level.end()) + 1 : 0;
+    std::vector<int32_t> lvl;
+    std::vector<size_t> loff(nlev + 1, 0);
+    for (int L = 0; L < nlev; L++) {
+        for (int k = 0; k < na; k++)
+            if (level[k] == L) lvl.push_back(k);
+        loff[L + 1] = lvl.size();
+    }
+    if ((rc = W.lc_lvl.ensure(std::max<size_t>(lvl.size(), 1))) || (rc = W.lc_adjs.ensure(adjs.size())) ||
+        (rc = W.lc_adj.ensure(std::max<size_t>(adj.size(), 1))))
+        return rc;
+    YG_HIP(hipMemcpyAsync(W.lc_lvl.p, lvl.data(), sizeof(int32_t) * lvl.size(), hipMemcpyHostToDevice, st));
+    YG_HIP(hipMemcpyAsync(W.lc_adjs.p, adjs.data(), sizeof(int32_t) * adjs.size(), hipMemcpyHostToDevice, st));
+    if (!adj.empty())
+        YG_HIP(hipMemcpyAsync(W.lc_adj.p, adj.data(), sizeof(int32_t) * adj.size(), hipMemcpyHostToDevice, st));
+    const size_t fill_lds = stack_lds_bytes(W.stk_entries, LC_WAVE);
+    for (int L = 0; L < nlev; L++) {
+        const unsigned nb = (unsigned)(loff[L + 1] - loff[L]);
+        YG_LAUNCH_WS(W.width, W.nocast, (k_cache_fill<8>), (k_cache_fill<4>), (k_cache_fill<2>), (k_cache_fill<24>),
+                     (k_cache_fill<20>), (k_cache_fill<18>), dim3(nb), dim3(LC_WAVE), fill_lds, st, W.scenes.p,
+                     W.areas.p, na, W.lc_lvl.p + loff[L], W.lc_adjs.p, W.lc_adj.p, W.item_ti.p, W.item_t.p, W.qual.p,
+                     cv, cp);
+        YG_HIP(hipGetLastError());
+    }
+    // the samples' estimates: takeSample over the sample items, through the GI levels
+    YG_HIP(hipMemcpyAsync(W.lc_qual0.p, W.qual.p, sizeof(uint32_t) * nitems, hipMemcpyDeviceToDevice, st));
+    YG_HIP(hipMemcpyAsync(W.qual.p, W.lc_in.p, sizeof(uint32_t) * nitems, hipMemcpyDeviceToDevice, st));
+    if ((rc = exclusive_scan(W.qual.p, W.qoff.p, nitems, st))) return rc;
+    if ((rc = trace_levels<false>(st, ktiming, kev, AdjAcc{}, nullptr, nullptr))) return rc;
+    hipLaunchKernelGGL(k_cache_color, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na, nitems,
+                       W.S, W.D, W.contri.p, W.nseg.p, cv, W.cnt.p);
+    YG_HIP(hipGetLastError());
+    YG_HIP(hipMemcpyAsync(W.qual.p, W.lc_qual0.p, sizeof(uint32_t) * nitems, hipMemcpyDeviceToDevice, st));
+    // startUse (lightcache.cc:48-57): the samples in the cache's iteration
+    // order -- boxes by (x, y, z) as std::map orders them, each box's list
+    // newest first -- give the generic tree, whose leaf order ranks them
+    std::vector<uint32_t> cnt(na), list(nitems);
+    std::vector<f3> pP(nitems);
+    unsigned long long err[2];
+    YG_HIP(hipMemcpyAsync(cnt.data(), W.lc_cnt.p, sizeof(uint32_t) * na, hipMemcpyDeviceToHost, st));
+    YG_HIP(hipMemcpyAsync(list.data(), W.lc_list.p, sizeof(uint32_t) * nitems, hipMemcpyDeviceToHost, st));
+    YG_HIP(hipMemcpyAsync(pP.data(), W.lc_pP.p, sizeof(f3) * nitems, hipMemcpyDeviceToHost, st));
+    YG_HIP(hipMemcpyAsync(err, W.lc_err.p, sizeof(err), hipMemcpyDeviceToHost, st));
+    YG_HIP(hipStreamSynchronize(st));
+    if (err[0]) return yg_fail(YG_ERR_UNSUPPORTED, "pathlight cache: more samples within reach of a render area than it has pixels");
+    std::vector<uint32_t> ins;  // insertion order: areas in processing order, each area's visits in order
+    for (int k = 0; k < na; k++)
+        for (uint32_t m = 0; m < cnt[k]; m++) ins.push_back(list[A[k].item_start + m]);
+    struct BoxKey {
+        int32_t x, y, z;
+        uint32_t seq;  // insertion position
+    };
+    std::vector<BoxKey> keys(ins.size());
+    const float cs = P.cache_size;
+    for (size_t q = 0; q < ins.size(); q++) {
+        int32_t nx, ny, nz;
+        lc_getbox(cs, pP[ins[q]], nx, ny, nz);
+        // findBox looks the box up again from its centre (hash3d.h:41-42, 195-205)
+        f3 c = mk3(cs * (float)nx, cs * (float)ny, cs * (float)nz);
+        c.x = (float)((double)c.x + 0.5 * (double)cs);
+        c.y = (float)((double)c.y + 0.5 * (double)cs);
+        c.z = (float)((double)c.z + 0.5 * (double)cs);
+        lc_getbox(cs, c, nx, ny, nz);
+        keys[q] = BoxKey{nx, ny, nz, (uint32_t)q};
+    }
+    std::sort(keys.begin(), keys.end(), [](const BoxKey& a, const BoxKey& b) {
+        if (a.x != b.x) return a.x < b.x;
+        if (a.y != b.y) return a.y < b.y;
+        if (a.z != b.z) return a.z < b.z;
+        return a.seq > b.seq;  // push_front: newest first
+    });
+    std::vector<uint32_t> v(keys.size()), order;
+    for (size_t q = 0; q < keys.size(); q++) v[q] = ins[keys[q].seq];
+    order.reserve(v.size());
+    lc_tree_order(pP, v, false, false, false, order);
+    std::vector<int32_t> rank(nitems, 0);
+    for (size_t q = 0; q < order.size(); q++) rank[order[q]] = (int32_t)q;
+    // candidate grid over (pP.x, pP.y): cells a little over the search
+    // radius, entries in rank order
+    float lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+    for (uint32_t id : ins) {
+        lo[0] = std::min(lo[0], pP[id].x); hi[0] = std::max(hi[0], pP[id].x);
+        lo[1] = std::min(lo[1], pP[id].y); hi[1] = std::max(hi[1], pP[id].y);
+    }
+    if (ins.empty()) lo[0] = lo[1] = hi[0] = hi[1] = 0;
+    cp.gcell = cp.sradius * (1.0f + 1.0f / 1024.0f);
+    cp.gx0 = lo[0];
+    cp.gy0 = lo[1];
+    cp.gnx = std::min(4096, (int)((hi[0] - lo[0]) / cp.gcell) + 1);
+    cp.gny = std::min(4096, (int)((hi[1] - lo[1]) / cp.gcell) + 1);
+    const size_t ncell = (size_t)cp.gnx * cp.gny;
+    std::vector<int32_t> gstart(ncell + 1, 0);
+    std::vector<uint32_t> gitem(order.size());
+    auto cell_of = [&](uint32_t id) {
+        int cx = (int)((pP[id].x - cp.gx0) / cp.gcell), cy = (int)((pP[id].y - cp.gy0) / cp.gcell);
+        cx = std::min(std::max(cx, 0), cp.gnx - 1);
+        cy = std::min(std::max(cy, 0), cp.gny - 1);
+        return (size_t)cy * cp.gnx + cx;
+    };
+    for (uint32_t id : order) gstart[cell_of(id) + 1]++;
+    for (size_t c = 0; c < ncell; c++) gstart[c + 1] += gstart[c];
+    {
+        std::vector<int32_t> fillp(gstart.begin(), gstart.end() - 1);
+        for (uint32_t id : order) gitem[fillp[cell_of(id)]++] = id;  // order: ascending rank per cell
+    }
+    if ((rc = W.lc_gstart.ensure(gstart.size())) || (rc = W.lc_gitem.ensure(std::max<size_t>(gitem.size(), 1))))
+        return rc;
+    YG_HIP(hipMemcpyAsync(W.lc_gstart.p, gstart.data(), sizeof(int32_t) * gstart.size(), hipMemcpyHostToDevice, st));
+    if (!gitem.empty())
+        YG_HIP(hipMemcpyAsync(W.lc_gitem.p, gitem.data(), sizeof(uint32_t) * gitem.size(), hipMemcpyHostToDevice, st));
+    YG_HIP(hipMemcpyAsync(W.lc_rank.p, rank.data(), sizeof(int32_t) * nitems, hipMemcpyHostToDevice, st));
+    cp.gstart = W.lc_gstart.p;
+    cp.gitem = W.lc_gitem.p;
+    // render pass: the other lights at the camera hits, then the interpolation
+    if (nl_max > 32) {
+        YG_LAUNCH_WS(W.width, W.nocast, (k_direct<8, true>), (k_direct<4, true>), (k_direct<2, true>),
+                     (k_direct<24, true>), (k_direct<20, true>), (k_direct<18, true>), dim3(grid_for(nitems, RB)),
+                     dim3(RB), W.stk_bytes, st, W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p,
+                     W.lcg_off.p, W.lightc.p, nl_max, W.cnt.p, 0, W.stk_ovf.p, (const uint32_t*)nullptr);
+    } else {
+        YG_LAUNCH_WS(W.width, W.nocast, (k_sun_vis<8>), (k_sun_vis<4>), (k_sun_vis<2>), (k_sun_vis<24>),
+                     (k_sun_vis<20>), (k_sun_vis<18>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes_lvl, st,
+                     W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.sun_vis.p, W.cnt.p, W.lds_cap,
+                     W.stk_ovf.p);
+        YG_HIP(hipGetLastError());
+        YG_LAUNCH_W(W.width, (k_direct<8, false>), (k_direct<4, false>), (k_direct<2, false>),
+                    dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na, nitems, W.item_ti.p,
+                    W.item_t.p, W.lcg_off.p, W.lightc.p, nl_max, W.cnt.p, 0, W.stk_ovf.p,
+                    (const uint32_t*)W.sun_vis.p);
+    }
+    YG_HIP(hipGetLastError());
+    if (W.any_photon) {
+        hipLaunchKernelGGL(k_photon_direct, dim3(grid_for(nitems, 64)), dim3(64), knn_lds_bytes(W.photon_kmax), st,
+                           W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lightc.p, nl_max);
+        YG_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_cache_interp, dim3(na), dim3(LC_WAVE), 0, st, W.scenes.p, W.areas.p, na, W.item_ti.p,
+                       W.item_t.p, W.qual.p, cv, cp, W.lc_pathc.p);
+    YG_HIP(hipGetLastError());
+    if (before_final) YG_HIP(hipEventRecord(before_final, st));
+    hipLaunchKernelGGL(k_finalize, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na, nitems,
+                       W.S, W.D, W.item_ti.p, W.item_t.p, W.qual.p, W.lightc.p, nl_max, W.contri.p, W.nseg.p, W.cnt.p,
+                       W.finalize_to, (const c3*)W.lc_pathc.p);
+    YG_HIP(hipGetLastError());
+    YG_HIP(hipMemcpyAsync(err, W.lc_err.p, sizeof(err), hipMemcpyDeviceToHost, st));
+    YG_HIP(hipStreamSynchronize(st));
+    if (err[0]) return yg_fail(YG_ERR_UNSUPPORTED, "pathlight cache: more than 1024 samples within a gather radius");
+    if (err[1]) return yg_fail(YG_ERR_STATE, "pathlight cache: a render-pass gather found no sample");
+    W.lc_ins = ins;
+    W.lc_pix.resize(ins.size());
+    for (int k = 0, q = 0; k < na; k++)
+        for (uint32_t m = 0; m < cnt[k]; m++, q++) {
+            const uint32_t li = list[A[k].item_start + m] - A[k].item_start;
+            const int i = A[k].y + (int)(li / (uint32_t)A[k].w), j = A[k].x + (int)(li % (uint32_t)A[k].w);
+            W.lc_pix[q] = (uint32_t)(j + i * R.resx);
+        }
+    W.lc_uid = P.s->uid;
     return YG_OK;
 }
 
@@ -1760,6 +2108,10 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         YG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(YG_LCG_POW), pw, sizeof(pw)));
         g_ws.pow_ready = true;
     }
+    bool cache = false;
+    for (auto& P : plans) cache = cache || P.cache;
+    if (cache && plans.size() > 1)
+        return yg_fail(YG_ERR_UNSUPPORTED, "a pathlight cache in a batched render is not on the GPU path");
     // concatenate scenes
     std::vector<RenderScene> scenes;
     std::vector<Area> areas;
@@ -1937,7 +2289,7 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
         YG_HIP(hipMemsetAsync(W.hbase.p, 0, sizeof(uint32_t) * na_real * D, st));
         W.finalize_to = W.item_col.p;
     }
-    if (use_graph && !timing && !W.graphs_off && aa_passes == 0) {
+    if (use_graph && !timing && !W.graphs_off && aa_passes == 0 && !cache) {
         // capture on the second consecutive frame of a shape, replay from then on
         bool captured = false;
         if (!(W.frame_exec && W.frame_key == key) && W.seen_valid && W.seen_key == key) {
@@ -1951,7 +2303,9 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     }
     W.seen_key = key;
     W.seen_valid = true;
-    if (!graph && (rc = launch_frame(st, ktiming, kev, timing ? e1 : nullptr))) return rc;
+    if (!graph && (rc = cache ? cache_frame(plans[0], st, ktiming, kev, timing ? e1 : nullptr)
+                              : launch_frame(st, ktiming, kev, timing ? e1 : nullptr)))
+        return rc;
     uint64_t aa_rays = 0;
     if (aa_passes > 0) {
         // scene.cc:777-816: each pass resamples the pixels the contrast mask
@@ -2107,6 +2461,49 @@ extern "C" int yg_render(yg_scene* s, const yg_render_opts* opts, float* d_rgb, 
         g_ws.last_opts = *opts;
     }
     return rc;
+}
+
+// The samples of the scene's last cache frame, 18 floats each in insertion
+// order (the fake pass's visit order): P, N, pP, colour, adist, M,
+// precision, rank, pixel number, devaluated (1).
+extern "C" int yg_cache_read(yg_scene* s, float* d_out, uint64_t capacity, uint64_t* count, void* stream)
+{
+    if (!s) return yg_fail(YG_ERR_ARG, "null scene");
+    RenderWS& W = g_ws;
+    if (W.lc_uid != s->uid) return yg_fail(YG_ERR_STATE, "no cache frame of this scene (yg_render with cache on)");
+    const size_t n = W.lc_ins.size();
+    if (count) *count = n;
+    if (!d_out || !n) return YG_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t ni = W.nitems;
+    std::vector<f3> P(ni), N(ni), pP(ni);
+    std::vector<c3> col(ni);
+    std::vector<float> ad(ni), M(ni), pr(ni);
+    std::vector<int32_t> rk(ni);
+    YG_HIP(hipMemcpyAsync(P.data(), W.lc_P.p, sizeof(f3) * ni, hipMemcpyDeviceToHost, st));
+    YG_HIP(hipMemcpyAsync(N.data(), W.lc_N.p, sizeof(f3) * ni, hipMemcpyDeviceToHost, st));
+    YG_HIP(hipMemcpyAsync(pP.data(), W.lc_pP.p, sizeof(f3) * ni, hipMemcpyDeviceToHost, st));
+    YG_HIP(hipMemcpyAsync(col.data(), W.lc_col.p, sizeof(c3) * ni, hipMemcpyDeviceToHost, st));
+    YG_HIP(hipMemcpyAsync(ad.data(), W.lc_adist.p, sizeof(float) * ni, hipMemcpyDeviceToHost, st));
+    YG_HIP(hipMemcpyAsync(M.data(), W.lc_M.p, sizeof(float) * ni, hipMemcpyDeviceToHost, st));
+    YG_HIP(hipMemcpyAsync(pr.data(), W.lc_prec.p, sizeof(float) * ni, hipMemcpyDeviceToHost, st));
+    YG_HIP(hipMemcpyAsync(rk.data(), W.lc_rank.p, sizeof(int32_t) * ni, hipMemcpyDeviceToHost, st));
+    YG_HIP(hipStreamSynchronize(st));
+    const size_t m = std::min<size_t>(n, capacity);
+    std::vector<float> out(18 * m);
+    for (size_t q = 0; q < m; q++) {
+        const uint32_t id = W.lc_ins[q];
+        float* o = &out[18 * q];
+        o[0] = P[id].x; o[1] = P[id].y; o[2] = P[id].z;
+        o[3] = N[id].x; o[4] = N[id].y; o[5] = N[id].z;
+        o[6] = pP[id].x; o[7] = pP[id].y; o[8] = pP[id].z;
+        o[9] = col[id].r; o[10] = col[id].g; o[11] = col[id].b;
+        o[12] = ad[id]; o[13] = M[id]; o[14] = pr[id];
+        o[15] = (float)rk[id]; o[16] = (float)W.lc_pix[q]; o[17] = 1.0f;
+    }
+    YG_HIP(hipMemcpyAsync(d_out, out.data(), sizeof(float) * out.size(), hipMemcpyHostToDevice, st));
+    YG_HIP(hipStreamSynchronize(st));
+    return YG_OK;
 }
 
 // Batched render: several scenes in one set of launches (yg_render per scene
@@ -2404,6 +2801,7 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
     int rc = plan_scene(s, opts, d_rgb, plans[0]);
     if (rc) return rc;
     if (plans[0].R.depth > ADJ_MAXD) return yg_fail(YG_ERR_UNSUPPORTED, "adjoint supports depth <= 8");
+    if (plans[0].cache) return yg_fail(YG_ERR_UNSUPPORTED, "adjoint of a pathlight with the irradiance cache");
     const bool timing = opts->flags & YG_RENDER_TIMING;
     RenderWS& W = g_ws;
     const bool reuse = (opts->flags & YG_RENDER_REUSE) && W.last_uid == s->uid && W.last_gen == s->gen &&

@@ -61,6 +61,7 @@ def lib():
         "yg_photon_gather": (C.c_int, [vp, vp, vp, vp, vp, vp, u32, vp, vp]),
         "yg_gphoton_build": (C.c_int, [vp, u32, vp, vp, vp]),
         "yg_gphoton_read": (C.c_int, [vp, vp, C.c_uint64, C.POINTER(C.c_uint64), vp]),
+        "yg_cache_read": (C.c_int, [vp, vp, C.c_uint64, C.POINTER(C.c_uint64), vp]),
     }
     sigs["yg_render_batch"] = (C.c_int, [C.POINTER(vp), u32, vp, C.POINTER(vp), vp, vp])
     sigs["yg_render_adjoint"] = (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp])
@@ -300,6 +301,19 @@ class Scene:
                                     out.numel() // 15 if out is not None else 0, C.byref(cnt), _stream()),
               "yg_gphoton_read")
         return cnt.value
+
+    def cache_read(self):
+        """The irradiance cache of this scene's last cache render: [n, 18]
+        numpy array (P, N, polar position, colour, adist, M, precision,
+        rank, pixel, devaluated) in insertion order."""
+        import numpy as np
+        import torch
+        cnt = C.c_uint64()
+        check(lib().yg_cache_read(self.h, None, 0, C.byref(cnt), _stream()), "yg_cache_read")
+        n = cnt.value
+        out = torch.zeros(max(n, 1) * 18, dtype=torch.float32, device="cuda")
+        check(lib().yg_cache_read(self.h, _ptr(out), n, C.byref(cnt), _stream()), "yg_cache_read")
+        return out.cpu().numpy()[: n * 18].reshape(n, 18).astype(np.float32)
 
     def photon_read(self, out):
         cnt = C.c_uint64()
