@@ -411,6 +411,123 @@ __global__ void k_gp_irr(uint32_t nbox, int K, float radius, const f3* __restric
     irr[b] = cscale(total, (float)(4.0 * M_PI / (double)area));
 }
 
+// The same pre-gathering with one wave per box: the lanes scan the box's
+// grid cells and collect the box photons within the radius that face N into
+// LDS; whenever the buffer fills, it is sorted by (dis, box) and cut to the
+// K nearest; the final K are summed by lane 0 in that order.  (dis, box) is
+// a total order, so the K nearest and their order are the heap's.
+#define GP_IRR_BUF 2048
+__global__ __launch_bounds__(64) void k_gp_irr_wave(uint32_t nbox, int K, float radius, const f3* __restrict__ bpos,
+                                                    const uint32_t* __restrict__ bcode,
+                                                    const uint32_t* __restrict__ brgbe, const f3* __restrict__ bN,
+                                                    const int32_t* __restrict__ gstart,
+                                                    const int32_t* __restrict__ gitems, float gx, float gy, float gz,
+                                                    int dx, int dy, int dz, const float* __restrict__ tab,
+                                                    c3* __restrict__ irr)
+{
+    __shared__ float s_d[GP_IRR_BUF];
+    __shared__ int s_i[GP_IRR_BUF];
+    const uint32_t b = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (b >= nbox) return;
+    const f3 P = bpos[b], N = bN[b];
+    const float gmin[3] = {gx, gy, gz};
+    const int gdim[3] = {dx, dy, dz};
+    const float pp[3] = {P.x, P.y, P.z};
+    int g0[3], g1[3];
+    for (int a = 0; a < 3; a++) {
+        long lo = (long)floorf((pp[a] - radius - gmin[a]) / radius);
+        long hh = (long)floorf((pp[a] + radius - gmin[a]) / radius);
+        if (lo < 0) lo = 0;
+        if (hh >= gdim[a]) hh = gdim[a] - 1;
+        g0[a] = (int)lo;
+        g1[a] = (int)hh;
+    }
+    // sort s_d/s_i[0..n) by (dis, box), padding to a power of two
+    auto sort_buf = [&](int n) {
+        int n2 = 1;
+        while (n2 < n) n2 <<= 1;
+        for (int k = n + lane; k < n2; k += 64) {
+            s_d[k] = INFINITY;
+            s_i[k] = 0x7fffffff;
+        }
+        __syncthreads();
+        for (int size = 2; size <= n2; size <<= 1)
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int k = lane; k < n2; k += 64) {
+                    const int o = k ^ stride;
+                    if (o > k) {
+                        const bool up = (k & size) == 0;
+                        const bool gt = knn_less(s_d[o], s_i[o], s_d[k], s_i[k]);
+                        if (gt == up) {
+                            const float td = s_d[k]; s_d[k] = s_d[o]; s_d[o] = td;
+                            const int ti = s_i[k]; s_i[k] = s_i[o]; s_i[o] = ti;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+    };
+    int n = 0;
+    for (int z = g0[2]; z <= g1[2]; z++)
+        for (int y = g0[1]; y <= g1[1]; y++)
+            for (int x = g0[0]; x <= g1[0]; x++) {
+                const int c = (z * gdim[1] + y) * gdim[0] + x;
+                const int q0 = gstart[c], q1 = gstart[c + 1];
+                for (int q00 = q0; q00 < q1; q00 += 64) {
+                    const int q = q00 + lane;
+                    bool keep = false;
+                    float dis = 0.f;
+                    int idx = 0;
+                    if (q < q1) {
+                        idx = gitems[q];
+                        const uint32_t code = bcode[idx];
+                        if ((code & 255u) != 255u) {
+                            dis = len3(bpos[idx] - P);
+                            keep = !(dis > radius) && dot(dir_dec(code, tab), N) > 0.0f;
+                        }
+                    }
+                    const uint64_t m = __ballot(keep);
+                    if (!m) continue;
+                    if (n + 64 > GP_IRR_BUF) {
+                        sort_buf(n);
+                        if (n > K) n = K;
+                    }
+                    if (keep) {
+                        const int pos = n + (int)__popcll(m & ((1ull << lane) - 1ull));
+                        s_d[pos] = dis;
+                        s_i[pos] = idx;
+                    }
+                    n += (int)__popcll(m);
+                    __syncthreads();
+                }
+            }
+    if (n == 0) {
+        if (lane == 0) irr[b] = mkc(0, 0, 0);
+        return;
+    }
+    sort_buf(n);
+    if (n > K) n = K;
+    if (lane != 0) return;
+    const float farest = s_d[n - 1];
+    if (n == 1 || farest == 0.0f) {
+        const int f0 = s_i[n - 1];
+        float factor = dot(dir_dec(bcode[f0], tab), N);
+        if (factor < 0.0f) factor = 0.0f;
+        irr[b] = cscale(rgbe_dec(brgbe[f0]), factor);
+        return;
+    }
+    c3 total = mkc(0, 0, 0);
+    for (int k = 0; k < n; k++) {
+        const int idx = s_i[k];
+        const float factor = (1.0f - s_d[k] / farest) * dot(dir_dec(bcode[idx], tab), N);
+        if (factor > 0) total = cadd(total, cscale(rgbe_dec(brgbe[idx]), factor));
+    }
+    float area = (float)(M_PI * (double)farest * (double)farest);
+    if (area < 0.00005f) area = 0.00005f;
+    irr[b] = cscale(total, (float)(4.0 * M_PI / (double)area));
+}
+
 // irradiance photons (direction N quantised, colour irr through RGBE) and
 // the 15-float read-out records
 __global__ void k_gp_finish(uint32_t nbox, const f3* __restrict__ bpos, const uint32_t* __restrict__ bcode,
@@ -689,9 +806,14 @@ extern "C" int yg_gphoton_build(yg_scene* s, uint32_t li, const yg_render_opts* 
             if ((rc = F.alloc(&hd, (size_t)GL.search * nbox)) || (rc = F.alloc(&hidx, (size_t)GL.search * nbox)))
                 return rc;
             if (ktime) YG_HIP(hipEventRecord(ev[2], st));
-            hipLaunchKernelGGL(k_gp_irr, dim3(grid_for(nbox, 64)), dim3(64), 0, st, (uint32_t)nbox, GL.search, G.radius,
-                               G.d_pos, bcode, brgbe, G.d_N, G.d_gstart, G.d_gitems, G.gmin[0], G.gmin[1], G.gmin[2],
-                               G.gdim[0], G.gdim[1], G.gdim[2], tab, hd, hidx, G.d_irr);
+            if (GL.search <= GP_IRR_BUF - 64)
+                hipLaunchKernelGGL(k_gp_irr_wave, dim3((unsigned)nbox), dim3(64), 0, st, (uint32_t)nbox, GL.search,
+                                   G.radius, G.d_pos, bcode, brgbe, G.d_N, G.d_gstart, G.d_gitems, G.gmin[0], G.gmin[1],
+                                   G.gmin[2], G.gdim[0], G.gdim[1], G.gdim[2], tab, G.d_irr);
+            else
+                hipLaunchKernelGGL(k_gp_irr, dim3(grid_for(nbox, 64)), dim3(64), 0, st, (uint32_t)nbox, GL.search,
+                                   G.radius, G.d_pos, bcode, brgbe, G.d_N, G.d_gstart, G.d_gitems, G.gmin[0], G.gmin[1],
+                                   G.gmin[2], G.gdim[0], G.gdim[1], G.gdim[2], tab, hd, hidx, G.d_irr);
             YG_HIP(hipGetLastError());
             if (ktime) YG_HIP(hipEventRecord(ev[3], st));
             irr_timed = ktime;

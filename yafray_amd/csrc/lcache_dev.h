@@ -93,24 +93,70 @@ __device__ __forceinline__ f3 lc_polar(int i, int j, int resx, int resy, float y
 }
 
 #define LC_WAVE 64
+#define LC_OWN 1024  // ring of an area's most recent samples kept in LDS by the fake pass
+#define LC_NBL 1024  // candidates from earlier areas kept in LDS
+
+// Per-item surface data of the camera rays' hits, computed once for all
+// items (k_cache_prep) so that the area walks read it in coalesced chunks.
+struct VisitView {
+    f3* P;
+    f3* N;      // face-forwarded shading normal
+    f3* U;      // sp.NU, sp.NV
+    f3* V;
+    f3* pP;     // the fake pass's polar position
+    float* Z;
+    int32_t* mat;
+};
+
+__global__ __launch_bounds__(RB) void k_cache_prep(const RenderScene* __restrict__ scenes,
+                                                   const Area* __restrict__ areas, int nareas, uint32_t nitems,
+                                                   const int32_t* __restrict__ item_ti,
+                                                   const float* __restrict__ item_t,
+                                                   const uint32_t* __restrict__ qual, VisitView vv, CacheParams cp)
+{
+    const uint32_t item = blockIdx.x * RB + threadIdx.x;
+    if (item >= nitems || !qual[item]) return;
+    const Area a = areas[find_area(areas, nareas, item)];
+    const RenderScene& R = scenes[a.scene];
+    const float Z = item_t[item];
+    f3 eye;
+    const Surf sp = primary_surf(R, a, item, item_ti[item], Z, eye);
+    int i, j;
+    item_pixel(a, item, i, j);
+    vv.P[item] = sp.P;
+    vv.N[item] = face_forward(sp.Ng, sp.N, eye);
+    vv.U[item] = sp.NU;
+    vv.V[item] = sp.NV;
+    vv.pP[item] = lc_polar(i, j, R.resx, R.resy, cp.ycorr, Z, true);
+    vv.Z[item] = Z;
+    vv.mat[item] = sp.mat;
+}
 
 // Fake pass of the areas of one level: one wave per area.  Candidates of a
-// visit: the area's own samples so far and the samples of earlier areas
-// that lie within reach of it (gathered into cv.nb at the start), both
-// ordered by pixel row, so that only rows within reach are scanned.
+// visit: the area's own samples so far (the most recent LC_OWN in an LDS
+// ring) and the samples of earlier areas within reach of the area (gathered
+// into cv.nb at the start, sorted by row, the first LC_NBL copied to LDS),
+// both scanned only over the rows within reach; own samples newest first.
 template <int W>
 __global__ __launch_bounds__(LC_WAVE) void k_cache_fill(const RenderScene* __restrict__ scenes,
                                                         const Area* __restrict__ areas, int nareas,
                                                         const int32_t* __restrict__ level_areas,
                                                         const int32_t* __restrict__ adj_start,
                                                         const int32_t* __restrict__ adj,
-                                                        const int32_t* __restrict__ item_ti,
-                                                        const float* __restrict__ item_t,
-                                                        const uint32_t* __restrict__ qual, CacheView cv,
-                                                        CacheParams cp)
+                                                        const uint32_t* __restrict__ qual, VisitView vv,
+                                                        CacheView cv, CacheParams cp)
 {
     extern __shared__ int s_stk[];
     __shared__ float s_z[LC_WAVE];
+    __shared__ f3 o_pP[LC_OWN], o_P[LC_OWN], o_N[LC_OWN];
+    __shared__ float o_H[LC_OWN];
+    __shared__ int o_row[LC_OWN];
+    __shared__ f3 n_pP[LC_NBL], n_P[LC_NBL], n_N[LC_NBL];
+    __shared__ float n_H[LC_NBL];
+    __shared__ int n_row[LC_NBL];
+    __shared__ f3 v_P[LC_WAVE], v_N[LC_WAVE], v_U[LC_WAVE], v_V[LC_WAVE], v_pP[LC_WAVE];
+    __shared__ float v_Z[LC_WAVE];
+    __shared__ uint32_t v_q[LC_WAVE];
     const int lane = threadIdx.x;
     const int ai = level_areas[blockIdx.x];
     const Area a = areas[ai];
@@ -119,9 +165,9 @@ __global__ __launch_bounds__(LC_WAVE) void k_cache_fill(const RenderScene* __res
     st.init(s_stk, lane, LC_WAVE);
     const int S = R.samples;
     const uint32_t base = a.item_start;
+    const uint32_t n_area = (uint32_t)a.w * (uint32_t)a.h;
     // candidates of the earlier areas in reach: their samples within `reach`
-    // pixels of this area's rectangle, ordered by pixel row (each area's list
-    // is in row order; the merge is a stable insertion by row)
+    // pixels of this area's rectangle
     uint32_t nnb = 0;
     const int rlo = a.y - cp.reach, rhi = a.y + a.h - 1 + cp.reach;
     const int clo = a.x - cp.reach, chi = a.x + a.w - 1 + cp.reach;
@@ -142,17 +188,16 @@ __global__ __launch_bounds__(LC_WAVE) void k_cache_fill(const RenderScene* __res
             const uint64_t m = __ballot(keep);
             if (keep) {
                 const uint32_t pos = nnb + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                if (pos < a.w * (uint32_t)a.h) cv.nb[base + pos] = id;
+                if (pos < n_area) cv.nb[base + pos] = id;
             }
             nnb += (uint32_t)__popcll(m);
         }
     }
-    if (nnb > (uint32_t)a.w * (uint32_t)a.h) {
+    if (nnb > n_area) {
         if (lane == 0) atomicAdd(&cv.err[0], 1ull);
         return;
     }
-    // insertion sort of the neighbour candidates by image row (lane 0; a few
-    // hundred entries along the area's border)
+    // by image row (lane 0: a few hundred entries along the area's border)
     if (lane == 0) {
         for (uint32_t k = 1; k < nnb; k++) {
             const uint32_t id = cv.nb[base + k];
@@ -166,93 +211,146 @@ __global__ __launch_bounds__(LC_WAVE) void k_cache_fill(const RenderScene* __res
         }
     }
     __syncthreads();
+    for (uint32_t k = lane; k < nnb && k < LC_NBL; k += LC_WAVE) {
+        const uint32_t id = cv.nb[base + k];
+        n_pP[k] = cv.pP[id];
+        n_P[k] = cv.P[id];
+        n_N[k] = cv.N[id];
+        n_H[k] = cv.adist[id];
+        n_row[k] = cv.row[id];
+    }
+    __syncthreads();
     uint32_t nin = 0, own_lo = 0, nb_lo = 0, nb_hi = 0;
-    const uint32_t nitems_area = (uint32_t)a.w * (uint32_t)a.h;
     const float maxw = (float)((double)cp.wfill * 2.0);
-    for (uint32_t li = 0; li < nitems_area; li++) {
-        const uint32_t item = base + li;
-        if (!qual[item]) continue;
-        const int ti = item_ti[item];
-        const float Z = item_t[item];
-        f3 eye;
-        const Surf sp = primary_surf(R, a, item, ti, Z, eye);
-        const f3 N = face_forward(sp.Ng, sp.N, eye);
-        const int pi = a.y + (int)(li / (uint32_t)a.w), pj = a.x + (int)(li % (uint32_t)a.w);
-        const f3 pP = lc_polar(pi, pj, R.resx, R.resy, cp.ycorr, Z, true);
-        // enoughFor: a sample within cache_size whose weightNoPrec reaches the limit
-        while (own_lo < nin && cv.row[cv.list[base + own_lo]] < pi - cp.reach) own_lo++;
-        while (nb_lo < nnb && cv.row[cv.nb[base + nb_lo]] < pi - cp.reach) nb_lo++;
-        if (nb_hi < nb_lo) nb_hi = nb_lo;
-        while (nb_hi < nnb && cv.row[cv.nb[base + nb_hi]] <= pi + cp.reach) nb_hi++;
-        bool cov = false;
-        const uint32_t n_own = nin - own_lo;
-        for (uint32_t k0 = 0; k0 < n_own && !cov; k0 += LC_WAVE) {
-            const uint32_t k = own_lo + k0 + lane;
-            bool c = false;
-            if (k < nin) {
-                const uint32_t id = cv.list[base + k];
-                const f3 q = cv.pP[id];
-                if (lc_pdist(pP, q) <= cp.cs)
-                    c = lc_weight(cv.P[id], cv.N[id], cv.adist[id], 0.f, sp.P, N, maxw, false) >= cp.wfill;
+    for (uint32_t c0 = 0; c0 < n_area; c0 += LC_WAVE) {
+        const uint32_t nc = (n_area - c0 < LC_WAVE) ? n_area - c0 : LC_WAVE;
+        {
+            const uint32_t item = base + c0 + lane;
+            const uint32_t q = (lane < (int)nc) ? qual[item] : 0u;
+            v_q[lane] = q;
+            if (q) {
+                v_P[lane] = vv.P[item];
+                v_N[lane] = vv.N[item];
+                v_U[lane] = vv.U[item];
+                v_V[lane] = vv.V[item];
+                v_pP[lane] = vv.pP[item];
+                v_Z[lane] = vv.Z[item];
             }
-            cov = __ballot(c) != 0;
-        }
-        for (uint32_t k0 = nb_lo; k0 < nb_hi && !cov; k0 += LC_WAVE) {
-            const uint32_t k = k0 + lane;
-            bool c = false;
-            if (k < nb_hi) {
-                const uint32_t id = cv.nb[base + k];
-                const f3 q = cv.pP[id];
-                if (lc_pdist(pP, q) <= cp.cs)
-                    c = lc_weight(cv.P[id], cv.N[id], cv.adist[id], 0.f, sp.P, N, maxw, false) >= cp.wfill;
-            }
-            cov = __ballot(c) != 0;
-        }
-        if (cov) continue;
-        // takeSample's first hits (pathlight.cc:305-327): the level-0 rays of
-        // this sample's paths, Halton calls nin*S + r of the area's sampler
-        float HD = 0.f, Mv = 0.f;
-        for (int r0 = 0; r0 < S; r0 += LC_WAVE) {
-            const int r = r0 + lane;
-            float z = -1.f;
-            if (r < S) {
-                const uint32_t k = nin * (uint32_t)S + (uint32_t)r;
-                const float z1 = *(const YG_GLOBAL float*)(R.halton + k);
-                const float z2 = *(const YG_GLOBAL float*)(R.halton + (size_t)R.hstride + k);
-                const f3 ray = hemi_direction(z1, z2, N, sp.NU, sp.NV);
-                float t;
-                if (first_hit<W>(R.bvh, sp.P, ray, R.min_raydis, t, st, true) >= 0) z = t;
-            }
-            s_z[lane] = z;
-            __syncthreads();
-            if (lane == 0) {
-                const int n = (S - r0 < LC_WAVE) ? S - r0 : LC_WAVE;
-                for (int q = 0; q < n; q++) {
-                    const float t = s_z[q];
-                    if (t < 0.f) continue;
-                    if (t > 0) HD = (float)((double)HD + 1.0 / (double)t);
-                    if ((t < Mv) || (Mv == 0)) Mv = t;
-                }
-            }
-            __syncthreads();
-        }
-        if (lane == 0) {
-            const float maxdist = (float)1000000 * cp.worldres * Z;
-            const float HNUM = (float)S;
-            const float H = (HD > 0) ? HNUM / HD : maxdist;
-            if (Mv <= 0) Mv = maxdist;
-            cv.pP[item] = pP;
-            cv.P[item] = sp.P;
-            cv.N[item] = N;
-            cv.adist[item] = H;
-            cv.M[item] = Mv;
-            cv.prec[item] = Z * cp.worldres;
-            cv.in[item] = 1u;
-            cv.row[item] = pi;
-            cv.list[base + nin] = item;
         }
         __syncthreads();
-        nin++;
+        for (uint32_t u = 0; u < nc; u++) {
+            if (!v_q[u]) continue;
+            const uint32_t li = c0 + u, item = base + li;
+            const int pi = a.y + (int)(li / (uint32_t)a.w);
+            const f3 P = v_P[u], N = v_N[u], pP = v_pP[u];
+            // rows within reach: advance the windows
+            while (own_lo < nin) {
+                const int r = (nin - own_lo <= LC_OWN) ? o_row[own_lo % LC_OWN] : cv.row[cv.list[base + own_lo]];
+                if (r >= pi - cp.reach) break;
+                own_lo++;
+            }
+            while (nb_lo < nnb) {
+                const int r = (nb_lo < LC_NBL) ? n_row[nb_lo] : cv.row[cv.nb[base + nb_lo]];
+                if (r >= pi - cp.reach) break;
+                nb_lo++;
+            }
+            if (nb_hi < nb_lo) nb_hi = nb_lo;
+            while (nb_hi < nnb) {
+                const int r = (nb_hi < LC_NBL) ? n_row[nb_hi] : cv.row[cv.nb[base + nb_hi]];
+                if (r > pi + cp.reach) break;
+                nb_hi++;
+            }
+            // enoughFor: a sample within cache_size whose weightNoPrec reaches the limit
+            bool cov = false;
+            const uint32_t n_own = nin - own_lo;
+            for (uint32_t k0 = 0; k0 < n_own && !cov; k0 += LC_WAVE) {
+                bool c = false;
+                if (k0 + lane < n_own) {
+                    const uint32_t k = nin - 1 - (k0 + lane);  // newest first
+                    f3 q, sP, sN;
+                    float H;
+                    if (nin - k <= LC_OWN) {
+                        const uint32_t r = k % LC_OWN;
+                        q = o_pP[r]; sP = o_P[r]; sN = o_N[r]; H = o_H[r];
+                    } else {
+                        const uint32_t id = cv.list[base + k];
+                        q = cv.pP[id]; sP = cv.P[id]; sN = cv.N[id]; H = cv.adist[id];
+                    }
+                    if (lc_pdist(pP, q) <= cp.cs) c = lc_weight(sP, sN, H, 0.f, P, N, maxw, false) >= cp.wfill;
+                }
+                cov = __ballot(c) != 0;
+            }
+            for (uint32_t k0 = nb_lo; k0 < nb_hi && !cov; k0 += LC_WAVE) {
+                const uint32_t k = k0 + lane;
+                bool c = false;
+                if (k < nb_hi) {
+                    f3 q, sP, sN;
+                    float H;
+                    if (k < LC_NBL) {
+                        q = n_pP[k]; sP = n_P[k]; sN = n_N[k]; H = n_H[k];
+                    } else {
+                        const uint32_t id = cv.nb[base + k];
+                        q = cv.pP[id]; sP = cv.P[id]; sN = cv.N[id]; H = cv.adist[id];
+                    }
+                    if (lc_pdist(pP, q) <= cp.cs) c = lc_weight(sP, sN, H, 0.f, P, N, maxw, false) >= cp.wfill;
+                }
+                cov = __ballot(c) != 0;
+            }
+            if (cov) continue;
+            // takeSample's first hits (pathlight.cc:305-327): the level-0 rays of
+            // this sample's paths, Halton calls nin*S + r of the area's sampler
+            const f3 U = v_U[u], V = v_V[u];
+            const float Z = v_Z[u];
+            float HD = 0.f, Mv = 0.f;
+            for (int r0 = 0; r0 < S; r0 += LC_WAVE) {
+                const int r = r0 + lane;
+                float z = -1.f;
+                if (r < S) {
+                    const uint32_t k = nin * (uint32_t)S + (uint32_t)r;
+                    const float z1 = *(const YG_GLOBAL float*)(R.halton + k);
+                    const float z2 = *(const YG_GLOBAL float*)(R.halton + (size_t)R.hstride + k);
+                    const f3 ray = hemi_direction(z1, z2, N, U, V);
+                    float t;
+                    if (first_hit<W>(R.bvh, P, ray, R.min_raydis, t, st, true) >= 0) z = t;
+                }
+                s_z[lane] = z;
+                __syncthreads();
+                if (lane == 0) {
+                    const int n = (S - r0 < LC_WAVE) ? S - r0 : LC_WAVE;
+                    for (int q = 0; q < n; q++) {
+                        const float t = s_z[q];
+                        if (t < 0.f) continue;
+                        if (t > 0) HD = (float)((double)HD + 1.0 / (double)t);
+                        if ((t < Mv) || (Mv == 0)) Mv = t;
+                    }
+                }
+                __syncthreads();
+            }
+            if (lane == 0) {
+                const float maxdist = (float)1000000 * cp.worldres * Z;
+                const float HNUM = (float)S;
+                const float H = (HD > 0) ? HNUM / HD : maxdist;
+                if (Mv <= 0) Mv = maxdist;
+                cv.pP[item] = pP;
+                cv.P[item] = P;
+                cv.N[item] = N;
+                cv.adist[item] = H;
+                cv.M[item] = Mv;
+                cv.prec[item] = Z * cp.worldres;
+                cv.in[item] = 1u;
+                cv.row[item] = pi;
+                cv.list[base + nin] = item;
+                const uint32_t r = nin % LC_OWN;
+                o_pP[r] = pP;
+                o_P[r] = P;
+                o_N[r] = N;
+                o_H[r] = H;
+                o_row[r] = pi;
+            }
+            __syncthreads();
+            nin++;
+        }
+        __syncthreads();
     }
     if (lane == 0) cv.cnt[ai] = nin;
 }
@@ -350,15 +448,15 @@ __device__ __forceinline__ void lc_pop_heap(LcFound* f, int len)
 // by lane 0.  Writes the pathlight's term dif x interpolation x power.
 __global__ __launch_bounds__(LC_WAVE) void k_cache_interp(const RenderScene* __restrict__ scenes,
                                                           const Area* __restrict__ areas, int nareas,
-                                                          const int32_t* __restrict__ item_ti,
-                                                          const float* __restrict__ item_t,
-                                                          const uint32_t* __restrict__ qual, CacheView cv,
-                                                          CacheParams cp, c3* __restrict__ pathc)
+                                                          const uint32_t* __restrict__ qual, VisitView vv,
+                                                          CacheView cv, CacheParams cp, c3* __restrict__ pathc)
 {
     __shared__ int32_t c_rank[LC_CAND];
     __shared__ uint32_t c_id[LC_CAND];
-    __shared__ float c_dis[LC_CAND], c_w[LC_CAND];
+    __shared__ float c_dis[LC_CAND], c_w[LC_CAND], c_ad[LC_CAND];
     __shared__ LcFound found[LC_KMAX + 1];
+    __shared__ c3 f_col[LC_KMAX + 1];
+    __shared__ int s_nf;
     __shared__ uint32_t s_nc;
     __shared__ int s_repeat;
     __shared__ float s_radius, s_last;
@@ -376,11 +474,8 @@ __global__ __launch_bounds__(LC_WAVE) void k_cache_interp(const RenderScene* __r
     for (uint32_t li = 0; li < n_area; li++) {
         const uint32_t item = base + li;
         if (!qual[item]) continue;
-        const int ti = item_ti[item];
-        const float Z = item_t[item];
-        f3 eye;
-        const Surf sp = primary_surf(R, a, item, ti, Z, eye);
-        const f3 N = face_forward(sp.Ng, sp.N, eye);
+        const float Z = vv.Z[item];
+        const f3 P = vv.P[item], N = vv.N[item];
         const int pi = a.y + (int)(li / (uint32_t)a.w), pj = a.x + (int)(li % (uint32_t)a.w);
         const f3 pP = lc_polar(pi, pj, R.resx, R.resy, cp.ycorr, Z, false);
         const int gx = (int)floorf((pP.x - cp.gx0) / cp.gcell), gy = (int)floorf((pP.y - cp.gy0) / cp.gcell);
@@ -412,7 +507,8 @@ __global__ __launch_bounds__(LC_WAVE) void k_cache_interp(const RenderScene* __r
                             c_rank[pos] = cv.rank[id];
                             c_id[pos] = id;
                             c_dis[pos] = pD;
-                            c_w[pos] = lc_weight(cv.P[id], cv.N[id], cv.adist[id], cv.prec[id], sp.P, N, maxweight,
+                            c_ad[pos] = cv.adist[id];
+                            c_w[pos] = lc_weight(cv.P[id], cv.N[id], cv.adist[id], cv.prec[id], P, N, maxweight,
                                                  true);
                         }
                     }
@@ -433,7 +529,7 @@ __global__ __launch_bounds__(LC_WAVE) void k_cache_interp(const RenderScene* __r
             while (n2 < nc) n2 <<= 1;
             for (uint32_t k = nc + lane; k < n2; k += LC_WAVE) c_rank[k] = 0x7fffffff;
             __syncthreads();
-            for (uint32_t size = 2; size <= n2; size <<= 1)
+            for (uint32_t size = 2; size <= n2 && nc > 1; size <<= 1)
                 for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
                     for (uint32_t k = lane; k < n2; k += LC_WAVE) {
                         const uint32_t o = k ^ stride;
@@ -444,6 +540,7 @@ __global__ __launch_bounds__(LC_WAVE) void k_cache_interp(const RenderScene* __r
                                 uint32_t ti2 = c_id[k]; c_id[k] = c_id[o]; c_id[o] = ti2;
                                 float td = c_dis[k]; c_dis[k] = c_dis[o]; c_dis[o] = td;
                                 float tw = c_w[k]; c_w[k] = c_w[o]; c_w[o] = tw;
+                                float ta = c_ad[k]; c_ad[k] = c_ad[o]; c_ad[o] = ta;
                             }
                         }
                     }
@@ -456,7 +553,7 @@ __global__ __launch_bounds__(LC_WAVE) void k_cache_interp(const RenderScene* __r
                     const uint32_t id = c_id[k];
                     if (w > best) {
                         best = w;
-                        adist = cv.adist[id];
+                        adist = c_ad[k];
                     }
                     unsigned limit;
                     if (w <= cp.wlimit) limit = 0u;
@@ -479,7 +576,7 @@ __global__ __launch_bounds__(LC_WAVE) void k_cache_interp(const RenderScene* __r
                 }
                 float rrad = (nf == 0 || found[0].dis == 0)
                                  ? (float)(0.0001 * (double)adist)
-                                 : radius * len3(cv.P[found[0].id] - sp.P) / found[0].dis;
+                                 : radius * len3(cv.P[found[0].id] - P) / found[0].dis;
                 if (rrad == 0.0f) rrad = 1.0f;
                 const int rep = ((nf == 0) || (((adist / rrad) > cp.wlimit) && (reached < K)) || (best <= minweight)) &&
                                 (radius < cp.sradius);
@@ -494,6 +591,11 @@ __global__ __launch_bounds__(LC_WAVE) void k_cache_interp(const RenderScene* __r
             }
             __syncthreads();
         }
+        // the found samples' colours, loaded by all lanes
+        if (lane == 0) s_nf = nf;
+        __syncthreads();
+        for (int k = lane; k < s_nf; k += LC_WAVE) f_col[k] = cv.col[found[k].id];
+        __syncthreads();
         if (lane == 0) {
             float radius = s_radius;
             if ((maxdist > 0) && ((double)(maxdist / radius) < (1.0 / sqrt(2.0))))
@@ -509,14 +611,14 @@ __global__ __launch_bounds__(LC_WAVE) void k_cache_interp(const RenderScene* __r
             for (int k = 0; k < nf; k++) {
                 const float w = (float)((double)(found[k].w - farest) *
                                         (1.0 - (double)(found[k].dis / cp.sradius)));
-                total = cadd(total, cscale(cv.col[found[k].id], w));
+                total = cadd(total, cscale(f_col[k], w));
                 amount += w;
             }
             c3 out = mkc(0, 0, 0);
             if (amount != 0) {
                 amount = (float)(1.0 / (double)amount);
                 total = cscale(total, amount);
-                const c3 dif = get_diffuse(R.mats, R.mods, R.texs, sp.mat, sp.P);
+                const c3 dif = get_diffuse(R.mats, R.mods, R.texs, vv.mat[item], P);
                 out = cscale(cmul(dif, total), R.path_power);
             } else
                 atomicAdd(&cv.err[1], 1ull);

@@ -1242,6 +1242,9 @@ struct RenderWS {
     DevBuf<int32_t> lc_rank, lc_row, lc_lvl, lc_adjs, lc_adj, lc_gstart;
     DevBuf<uint32_t> lc_in, lc_list, lc_cnt, lc_nb, lc_qual0, lc_gitem;
     DevBuf<unsigned long long> lc_err;
+    DevBuf<f3> lv_P, lv_N, lv_U, lv_V, lv_pP;
+    DevBuf<float> lv_Z;
+    DevBuf<int32_t> lv_mat;
     std::vector<uint32_t> lc_ins, lc_pix;  // the last cache frame's samples (items, pixels) in insertion order
     uint64_t lc_uid = 0;                    // its scene
     // the single-scene render the workspace holds (for YG_RENDER_REUSE)
@@ -1852,7 +1855,9 @@ static int cache_frame(const ScenePlan& P, hipStream_t st, bool ktiming, std::ve
         (rc = W.lc_col.ensure(nitems)) || (rc = W.lc_pathc.ensure(nitems)) || (rc = W.lc_rank.ensure(nitems)) ||
         (rc = W.lc_row.ensure(nitems)) || (rc = W.lc_in.ensure(nitems)) || (rc = W.lc_list.ensure(nitems)) ||
         (rc = W.lc_cnt.ensure((size_t)na)) || (rc = W.lc_nb.ensure(nitems)) || (rc = W.lc_qual0.ensure(nitems)) ||
-        (rc = W.lc_err.ensure(2)))
+        (rc = W.lc_err.ensure(2)) || (rc = W.lv_P.ensure(nitems)) || (rc = W.lv_N.ensure(nitems)) ||
+        (rc = W.lv_U.ensure(nitems)) || (rc = W.lv_V.ensure(nitems)) || (rc = W.lv_pP.ensure(nitems)) ||
+        (rc = W.lv_Z.ensure(nitems)) || (rc = W.lv_mat.ensure(nitems)))
         return rc;
     YG_HIP(hipMemsetAsync(W.lc_in.p, 0, sizeof(uint32_t) * nitems, st));
     YG_HIP(hipMemsetAsync(W.lc_cnt.p, 0, sizeof(uint32_t) * na, st));
@@ -1904,13 +1909,16 @@ static int cache_frame(const ScenePlan& P, hipStream_t st, bool ktiming, std::ve
     YG_HIP(hipMemcpyAsync(W.lc_adjs.p, adjs.data(), sizeof(int32_t) * adjs.size(), hipMemcpyHostToDevice, st));
     if (!adj.empty())
         YG_HIP(hipMemcpyAsync(W.lc_adj.p, adj.data(), sizeof(int32_t) * adj.size(), hipMemcpyHostToDevice, st));
+    VisitView vv{W.lv_P.p, W.lv_N.p, W.lv_U.p, W.lv_V.p, W.lv_pP.p, W.lv_Z.p, W.lv_mat.p};
+    hipLaunchKernelGGL(k_cache_prep, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na, nitems,
+                       W.item_ti.p, W.item_t.p, W.qual.p, vv, cp);
+    YG_HIP(hipGetLastError());
     const size_t fill_lds = stack_lds_bytes(W.stk_entries, LC_WAVE);
     for (int L = 0; L < nlev; L++) {
         const unsigned nb = (unsigned)(loff[L + 1] - loff[L]);
         YG_LAUNCH_WS(W.width, W.nocast, (k_cache_fill<8>), (k_cache_fill<4>), (k_cache_fill<2>), (k_cache_fill<24>),
                      (k_cache_fill<20>), (k_cache_fill<18>), dim3(nb), dim3(LC_WAVE), fill_lds, st, W.scenes.p,
-                     W.areas.p, na, W.lc_lvl.p + loff[L], W.lc_adjs.p, W.lc_adj.p, W.item_ti.p, W.item_t.p, W.qual.p,
-                     cv, cp);
+                     W.areas.p, na, W.lc_lvl.p + loff[L], W.lc_adjs.p, W.lc_adj.p, W.qual.p, vv, cv, cp);
         YG_HIP(hipGetLastError());
     }
     // the samples' estimates: takeSample over the sample items, through the GI levels
@@ -2025,8 +2033,8 @@ static int cache_frame(const ScenePlan& P, hipStream_t st, bool ktiming, std::ve
                            W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lightc.p, nl_max);
         YG_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_cache_interp, dim3(na), dim3(LC_WAVE), 0, st, W.scenes.p, W.areas.p, na, W.item_ti.p,
-                       W.item_t.p, W.qual.p, cv, cp, W.lc_pathc.p);
+    hipLaunchKernelGGL(k_cache_interp, dim3(na), dim3(LC_WAVE), 0, st, W.scenes.p, W.areas.p, na, W.qual.p, vv,
+                       cv, cp, W.lc_pathc.p);
     YG_HIP(hipGetLastError());
     if (before_final) YG_HIP(hipEventRecord(before_final, st));
     hipLaunchKernelGGL(k_finalize, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na, nitems,
