@@ -161,7 +161,9 @@ typedef struct yg_light {
     /* area */
     float a[3], b[3], c[3], d[3];
     int32_t samples;         /* area "samples", path "samples" */
-    int32_t psamples;        /* area "psamples" (only 0 supported) */
+    int32_t psamples;        /* area "psamples": guessLight's test draws of glibc rand() (0: every
+                                lit point samples the penumbra; > 0 needs the light among the
+                                first 16, outside GI paths, no antialiasing, no batching) */
     /* path */
     int32_t depth;           /* path "depth" / photon "depth" */
     int32_t use_qmc;         /* path/photon "use_QMC" */
@@ -388,8 +390,10 @@ int yg_set_lights(yg_scene* s, const yg_light* lights, uint32_t n);
  * stream runs through each area's passes in turn, the areas are then launched
  * one after another): d_rgb receives resx*resy*3 floats (row-major, top row
  * first).
- * Lights supported: sun, area (psamples 0), path (use_QMC on, cache off),
- * photon (diffuse mode; yg_photon_build must have run). */
+ * Lights supported: sun, area (psamples >= 0), path (use_QMC on; with cache on:
+ * the fake pass, the samples and the interpolating pass in this one call),
+ * photon (diffuse mode; yg_photon_build must have run), global photon
+ * (yg_gphoton_build must have run). */
 int yg_render(yg_scene* s, const yg_render_opts* opts, float* d_rgb,
               yg_render_stats* stats, void* stream);
 

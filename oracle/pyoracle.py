@@ -128,6 +128,7 @@ def _load(path):
         L.yo_cache_record.argtypes = [vp, C.c_int]
         L.yo_cache_visits.restype = C.c_int64
         L.yo_cache_visits.argtypes = [vp, vp, C.c_int64]
+        L.yo_grand_stream.argtypes = [C.c_uint32, C.c_int64, C.c_int64, I32P]
         L.yo_heap_offer_trace.restype = C.c_int64
         L.yo_heap_offer_trace.argtypes = [F32P, C.c_int64, C.c_uint, F32P]
         L.yo_rgbe_roundtrip.argtypes = [F32P, F32P, C.c_int64]

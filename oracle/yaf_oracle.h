@@ -178,6 +178,7 @@ void yo_cache_release(yo_scene* s);
 void yo_cache_record(yo_scene* s, int on);
 int64_t yo_cache_visits(yo_scene* s, float* out, int64_t cap);
 int64_t yo_heap_offer_trace(const float* w, int64_t n, unsigned limit, float* out);
+void yo_grand_stream(uint32_t seed, int64_t skip, int64_t n, int32_t* out);
 int yo_gphoton_load(yo_scene* s, const float* data, int64_t n, const yg_light* GL, int li, int stateless,
                     int recompute);
 void yo_rgbe_roundtrip(const float* in, float* out, int64_t n);

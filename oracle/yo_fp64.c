@@ -415,7 +415,8 @@ static dc area_illuminate(const fctx* F, fsamp* S, int li, const dsp* sp, d3 eye
     for (int i = 0; i < A->samples; ++i) {
         d3 sampleP;
         if (A->samples == 1) {
-            const double r1 = uniform(F, S), r2 = uniform(F, S);
+            /* toY takes the first draw (g++ evaluates + right operand first) */
+            const double r2 = uniform(F, S), r1 = uniform(F, S);
             sampleP = dadd(dadd(A->corner, dmul(A->toX, r1)), dmul(A->toY, r2));
         } else {
             const double dish = uniform(F, S) - 0.5, disv = uniform(F, S) - 0.5;
@@ -677,6 +678,10 @@ int yo_render_fp64(yo_scene* s, const yg_camera* cam, const yg_material* mats, i
             npath++;
         }
         if (L->type == YG_LIGHT_AREA) sequential = 1;
+        /* guessLight's rand() stream is not restated here: the converged mode
+         * samples the penumbra everywhere, which is the integral guessLight
+         * approximates */
+        if (L->type == YG_LIGHT_AREA && L->psamples > 0 && mode == YO_FP64_REFERENCE) return -6;
     }
     if (npath > 1) return -7;
 

@@ -80,6 +80,7 @@ typedef struct {
     int px_n, px_cap;
     uint64_t lc_new;       /* interpolate() found no sample and took one */
     int32_t* lc_cand;      /* gather candidates (one per cache sample) */
+    struct yo_grand_s* grand;  /* glibc rand() state of the render (after the block shuffle's draws) */
 } rctx;
 
 static void adj_path_sample_begin(rctx* R);
@@ -217,7 +218,42 @@ static void area_setup(const yg_light* L, yo_area* A)
     A->toY = v3_sub(d, a);
 }
 
-/* areaLight_t::illuminate, arealight.cc:93-174, psamples = 0 (always PENUMBRA) */
+/* glibc random_r TYPE_3 one output at a time (the recurrence of
+ * yo_glibc_rand: r[i] = r[i-31] + r[i-3], outputs r[i] >> 1 from i = 344) */
+typedef struct yo_grand_s { uint32_t r[34]; int64_t i; } yo_grand;
+
+static int32_t grand_next(yo_grand* g)
+{
+    const uint32_t v = g->r[(g->i - 31) % 34] + g->r[(g->i - 3) % 34];
+    g->r[g->i % 34] = v;
+    g->i++;
+    return (int32_t)(v >> 1);
+}
+
+/* the generator after srand(seed) and `skip` rand() calls */
+static void grand_init(yo_grand* g, uint32_t seed, int64_t skip)
+{
+    if (seed == 0) seed = 1;
+    g->r[0] = seed;
+    for (int i = 1; i < 31; i++) {
+        int64_t v = (16807LL * (int32_t)g->r[i - 1]) % 2147483647LL;
+        if (v < 0) v += 2147483647LL;
+        g->r[i] = (uint32_t)v;
+    }
+    for (int i = 31; i < 34; i++) g->r[i] = g->r[i - 31];
+    g->i = 34;
+    for (int64_t k = 34; k < 344 + skip; k++) (void)grand_next(g);
+}
+
+/* Test hook: n outputs of rand() after srand(seed) and `skip` calls */
+void yo_grand_stream(uint32_t seed, int64_t skip, int64_t n, int32_t* out)
+{
+    yo_grand g;
+    grand_init(&g, seed, skip);
+    for (int64_t k = 0; k < n; k++) out[k] = grand_next(&g);
+}
+
+/* areaLight_t::illuminate, arealight.cc:93-200 */
 static yo_col area_illuminate(rctx* R, int li, const yo_sp* sp, yo_v3 eye)
 {
     const yo_area* A = &R->areas[li];
@@ -228,8 +264,10 @@ static yo_col area_illuminate(rctx* R, int li, const yo_sp* sp, yo_v3 eye)
     if (plane_at <= 0) return col(0, 0, 0); /* fromLight of a zero colour */
     yo_col pc = col_scale(A->color, A->pow);
     if (A->samples == 1) {
-        float r1 = our_random(&R->lcg);
+        /* corner + toX*ourRandom() + toY*ourRandom(): g++ evaluates the right
+         * operand of + first, so toY takes the first draw (tests/evalorder_check.cpp) */
         float r2 = our_random(&R->lcg);
+        float r1 = our_random(&R->lcg);
         R->stats[4] += 2;
         yo_v3 sampleP = v3_add(v3_add(A->corner, v3_mul(A->toX, r1)), v3_mul(A->toY, r2));
         yo_v3 L = v3_sub(sampleP, sp->P);
@@ -241,6 +279,36 @@ static yo_col area_illuminate(rctx* R, int li, const yo_sp* sp, yo_v3 eye)
             resul = col_add(resul, from_light(R, sp, dir, col_scale(pc, LD2), eye));
         }
         return col_scale(resul, plane_at);
+    }
+    /* guessLight (arealight.cc:176-200): psamples draws of glibc rand() pick
+     * sample points; lit and shadowed ones both seen: PENUMBRA, only lit:
+     * LIGHT (every point, unjittered, no shadow ray), none lit: SHADOW */
+    const int fsam = R->lights[li].psamples;
+    if (fsam > 0) {
+        int luz = 0, sombra = 0, status = 0;  /* 0 PENUMBRA, 1 LIGHT, 2 SHADOW */
+        for (int i = 0; i < fsam; ++i) {
+            if (luz && sombra) break;
+            const int rs = grand_next(R->grand) % A->samples;
+            const yo_v3 L = v3_sub(A->points[rs], sp->P);
+            if (v3_dot(L, N) < 0) sombra = 1;
+            else if (is_shadowed(R, sp, A->points[rs], 1)) sombra = 1;
+            else luz = 1;
+        }
+        if (luz && sombra) status = 0;
+        else if (luz) status = 1;
+        else status = 2;
+        if (status == 2) return col(0, 0, 0);  /* fromLight of a zero colour */
+        if (status == 1) {
+            for (int i = 0; i < A->samples; ++i) {
+                yo_v3 dir = v3_sub(A->points[i], sp->P);
+                float LD2 = v3_normlensqr(&dir);
+                LD2 = (LD2 != 0) ? (float)(1.0 / (double)LD2) : 1.0f;
+                resul = col_add(resul, from_light(R, sp, dir, col_scale(pc, LD2), eye));
+            }
+            yo_col r = col_scale(resul, plane_at);
+            const float fsm = (float)A->samples;
+            return col(r.r / fsm, r.g / fsm, r.b / fsm);
+        }
     }
     for (int i = 0; i < A->samples; ++i) {
         float dish = (float)((double)our_random(&R->lcg) - 0.5);
@@ -1732,6 +1800,7 @@ typedef struct {
     yo_lcache_ptr lc;    /* irradiance cache of a cache = on pathlight, filled by the fake pass */
     float worldres;
     uint64_t lc_new;
+    struct yo_grand_s* grand;  /* glibc rand() state shared by the render (areas in turn with area lights) */
 } render_job;
 
 /* color_t::expgam_Adjust (color.h:175-195) with scene_t's stored values
@@ -2454,6 +2523,7 @@ static void* render_worker(void* arg)
     render_job* J = (render_job*)arg;
     rctx R;
     rctx_init(&R, J->s, J->mats, J->nmat, J->mods, J->texs, J->lights, J->nl, J->obj_flags, J->nobj, J->opts);
+    R.grand = J->grand;
     if (J->lc) {
         R.lc = J->lc;
         R.lc_phase = 2;
@@ -2596,6 +2666,11 @@ int yo_render_mt(yo_scene* s, const yg_camera* cam, const yg_material* mats, int
     J.lights = lights; J.nl = nl; J.obj_flags = obj_flags; J.nobj = nobj; J.opts = opts;
     J.k = &k; J.regions = regions; J.nreg = nreg; J.rgb = rgb; J.next = 0;
     pthread_mutex_init(&J.mu, NULL);
+    /* glibc rand() continues after the block shuffle's draws: guessLight
+     * (area lights with psamples > 0) takes its draws from there */
+    yo_grand grand;
+    grand_init(&grand, opts->rand_seed, nreg);
+    J.grand = &grand;
     if (cli >= 0) {
         /* pathLight_t::init and the fake pass, then postInit's startUse */
         yo_lcache* C = (yo_lcache*)calloc(1, sizeof(yo_lcache));

@@ -442,3 +442,56 @@ def wall_bounce_expected(d, sun, samples=4, power=1.5, sun_power=2.0, halton=Non
                     tot += rho * c0 * rw * a * E
             exp[i, j] = rc * power * tot / samples
     return exp, mask
+
+
+# ------------------------------------------------- area light over a floor --
+
+def area_over_floor(samples=16, psamples=4, slab=False, resx=64, resy=48, h=2.0, power=3.0, rho=(0.7, 0.6, 0.5)):
+    """A 1 x 1 area light at height h facing down over a floor (z = 0), the
+    camera looking down at the floor from below the light; with `slab`, an
+    opaque plane at h / 2 larger than the view hides the light from every
+    visible floor point.  No other light, no background."""
+    F = 50.0
+    vf, ff = _quad(0.0, F)
+    verts, idx, tri_obj, tri_mat, flags = [vf], [ff], [0, 0], [0, 0], [7]
+    if slab:
+        vs, fs = _quad(h / 2, F)
+        verts.append(vs)
+        idx.append(fs + 4)
+        tri_obj += [1, 1]
+        tri_mat += [1, 1]
+        flags.append(7)
+    v = np.vstack(verts)
+    f = np.vstack(idx)
+    a, b, c, d = (-0.5, -0.5, h), (-0.5, 0.5, h), (0.5, 0.5, h), (0.5, -0.5, h)  # (b-a)x(d-a) = -z
+    L = D.arealight(a, b, c, d, samples=samples, color=(1.0, 0.9, 0.8), power=power, psamples=psamples)
+    L.use_in_indirect = 0
+    cam = D.camera((0.3, -0.6, 0.25 * h), (0.3, -0.55, 0.0), (0.3, 0.4, 0.25 * h), resx, resy, 1.0, 1.0)
+    mats = [D.material(rho), D.material((0.5, 0.5, 0.5))]
+    return D.SceneDesc(v, f, tri_obj, tri_mat, flags, cam, mats, [], [], [L], D.render_opts(bias=1e-3))
+
+
+def area_grid_points(L):
+    """Sample points of a square area light of n*n samples: the cell centres
+    a + (j + 1/2)/n (b - a) + (i + 1/2)/n (d - a) (fillQuad's grid for equal
+    sides, arealight.cc:31-73)."""
+    n = int(round(math.sqrt(L.samples)))
+    a, b, d = (np.array(L.a[:], np.float64), np.array(L.b[:], np.float64), np.array(L.d[:], np.float64))
+    pts = [a + (j + 0.5) / n * (b - a) + (i + 0.5) / n * (d - a) for i in range(n) for j in range(n)]
+    return np.array(pts)
+
+
+def lambert_from_points(P, pts, L, rho, weights=None):
+    """sum over points of generic diffuse fromLight (basicshaders.cc:30-70, no
+    1/pi) of pow colour / |L|^2 arriving from each point at floor points P
+    (normal +z), times plane_at (the light's height above P along its normal),
+    divided by the number of points: the area light's LIGHT verdict."""
+    col = np.array(L.color[:], np.float64) * float(L.power)
+    out = np.zeros((len(P), 3))
+    for q in pts:
+        Lv = q[None, :] - P
+        d2 = np.sum(Lv * Lv, axis=1)
+        ndl = Lv[:, 2] / np.sqrt(d2)
+        out += np.maximum(ndl, 0)[:, None] / d2[:, None] * col[None, :]
+    plane_at = float(L.a[2]) - P[:, 2]
+    return out * np.array(rho)[None, :] * plane_at[:, None] / len(pts)

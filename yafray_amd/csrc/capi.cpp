@@ -356,8 +356,7 @@ extern "C" int yg_set_lights(yg_scene* s, const yg_light* lights, uint32_t n)
     for (uint32_t i = 0; i < n; i++) {
         const yg_light& L = lights[i];
         if (L.type < YG_LIGHT_SUN || L.type > YG_LIGHT_GLOBALPHOTON) return yg_fail(YG_ERR_ARG, "bad light type");
-        if (L.type == YG_LIGHT_AREA && L.psamples != 0)
-            return yg_fail(YG_ERR_UNSUPPORTED, "area light psamples > 0 (guessLight's rand()) is not implemented");
+        if (L.type == YG_LIGHT_AREA && L.psamples < 0) return yg_fail(YG_ERR_ARG, "area light psamples must be >= 0");
         if (L.type == YG_LIGHT_PATH && !L.use_qmc)
             return yg_fail(YG_ERR_UNSUPPORTED, "pathlight needs use_QMC=on (the LCG sampler is not implemented)");
         if (L.type == YG_LIGHT_PHOTON && !L.use_qmc)

@@ -158,6 +158,104 @@ __device__ __forceinline__ Surf primary_surf(const RenderScene& R, const Area& a
     return sp;
 }
 
+// guessLight (arealight.cc:176-200) needs, for each area light with
+// psamples > 0 at each camera-ray hit in front of it, whether each of its
+// sample points faces the hit (L.N >= 0) and is unshadowed: two bitsets per
+// light (words woff[li].., facing then lit), traced here for every point so
+// that the sequential draw scan (k_guess_scan) only reads bits.  app: bit li
+// set when light li guesses at this item.
+template <int W>
+__global__ __launch_bounds__(RB) void k_guess_vis(const RenderScene* __restrict__ scenes,
+                                                  const Area* __restrict__ areas, int nareas, uint32_t nitems,
+                                                  const int32_t* __restrict__ item_ti,
+                                                  const float* __restrict__ item_t, const int32_t* __restrict__ woff,
+                                                  int nw, uint32_t* __restrict__ vis, uint32_t* __restrict__ app,
+                                                  int lds_cap, int* __restrict__ stk_ovf)
+{
+    extern __shared__ int s_stk[];
+    const uint32_t item = blockIdx.x * RB + threadIdx.x;
+    if (item >= nitems) return;
+    uint32_t mask = 0;
+    const int ti = item_ti[item];
+    if (ti >= 0) {
+        LaneStack st;
+        st.init(s_stk, threadIdx.x, RB);
+        st.spill_to(lds_cap, stk_ovf + item, gridDim.x * RB);
+        const Area a = areas[find_area(areas, nareas, item)];
+        const RenderScene& R = scenes[a.scene];
+        f3 eye;
+        const Surf sp = primary_surf(R, a, item, ti, item_t[item], eye);
+        const f3 N = face_forward(sp.Ng, sp.N, norm3(eye));
+        for (int li = 0; li < R.nl; li++) {
+            const DevLight& L = R.lights[li];
+            if (!L.use_in_render || L.type != YG_LIGHT_AREA || L.fsamples <= 0 || L.samples <= 1) continue;
+            if (!(dot(sp.P - L.from, L.dir) > 0)) continue;
+            mask |= 1u << li;
+            uint32_t* fw = vis + (size_t)item * nw + woff[li];
+            const int nwl = (L.samples + 31) / 32;
+            for (int w = 0; w < nwl; w++) {
+                uint32_t fb = 0, lb = 0;
+                for (int b = 0; b < 32 && w * 32 + b < L.samples; b++) {
+                    const f3 p = R.apts[L.pts + w * 32 + b];
+                    if (dot(p - sp.P, N) < 0) continue;
+                    fb |= 1u << b;
+                    if (!is_shadowed<W>(R.bvh, sp.P, p, sp.obj, true, R.min_raydis, R.self_bias, st)) lb |= 1u << b;
+                }
+                fw[w] = fb;
+                fw[nwl + w] = lb;
+            }
+        }
+    }
+    app[item] = mask;
+}
+
+// guessLight's draws in render order (one lane: the reference's glibc
+// rand() stream is consumed point after point, psamples draws at most, and
+// how many a point takes depends on what its draws saw).  rnd: the stream
+// after the block shuffle's draws.  guess: 2-bit verdicts per light.  Counts
+// the shadow rays the reference traces for its test points.
+__global__ void k_guess_scan(const RenderScene* __restrict__ scenes, uint32_t nitems, const uint32_t* __restrict__ app,
+                             const int32_t* __restrict__ woff, int nw, const uint32_t* __restrict__ vis,
+                             const int32_t* __restrict__ rnd, uint64_t nrnd, uint32_t* __restrict__ guess,
+                             Counters* __restrict__ cnt, unsigned long long* __restrict__ err)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const RenderScene& R = scenes[0];
+    uint64_t pos = 0;
+    unsigned long long rays = 0;
+    for (uint32_t item = 0; item < nitems; item++) {
+        const uint32_t m = app[item];
+        uint32_t g = 0;
+        for (int li = 0; m >> li; li++) {
+            if (!((m >> li) & 1u)) continue;
+            const DevLight& L = R.lights[li];
+            const uint32_t* fw = vis + (size_t)item * nw + woff[li];
+            const int nwl = (L.samples + 31) / 32;
+            bool luz = false, sombra = false;
+            for (int i = 0; i < L.fsamples; ++i) {
+                if (luz && sombra) break;
+                if (pos >= nrnd) {
+                    *err = 1;
+                    return;
+                }
+                const int rs = rnd[pos++] % L.samples;
+                const bool facing = (fw[rs >> 5] >> (rs & 31)) & 1u;
+                const bool lit = (fw[nwl + (rs >> 5)] >> (rs & 31)) & 1u;
+                if (!facing) sombra = true;
+                else {
+                    rays++;
+                    if (!lit) sombra = true;
+                    else luz = true;
+                }
+            }
+            const uint32_t v = (luz && sombra) ? YG_GUESS_PENUMBRA : (luz ? YG_GUESS_LIGHT : YG_GUESS_SHADOW);
+            g |= v << (2 * li);
+        }
+        guess[item] = g;
+    }
+    atomicAdd(&cnt->shadow, rays);
+}
+
 // Per item: ourRandom() draws of the area lights and whether the pathlight
 // samples at all (pathlight.cc:398: diffuse energy >= 0.05).
 __global__ __launch_bounds__(RB) void k_prep(const RenderScene* __restrict__ scenes,
@@ -165,7 +263,8 @@ __global__ __launch_bounds__(RB) void k_prep(const RenderScene* __restrict__ sce
                                              uint32_t nitems, const int32_t* __restrict__ item_ti,
                                              const float* __restrict__ item_t,
                                              uint32_t* __restrict__ draws, uint32_t* __restrict__ qual,
-                                             uint32_t* __restrict__ pre)
+                                             uint32_t* __restrict__ pre,
+                                             const uint32_t* __restrict__ guess = nullptr)
 {
     const uint32_t item = blockIdx.x * RB + threadIdx.x;
     if (item >= nitems) return;
@@ -179,6 +278,8 @@ __global__ __launch_bounds__(RB) void k_prep(const RenderScene* __restrict__ sce
         for (int li = 0; li < R.nl; li++) {
             const DevLight& L = R.lights[li];
             if (!L.use_in_render || L.type != YG_LIGHT_AREA) continue;
+            // guessLight's LIGHT and SHADOW verdicts draw nothing
+            if (guess && L.samples > 1 && ((guess[item] >> (2 * li)) & 3u) != YG_GUESS_PENUMBRA) continue;
             if (dot(sp.P - L.from, L.dir) > 0) {
                 dr += 2u * (uint32_t)L.samples;
                 if (li < R.path_light) drp += 2u * (uint32_t)L.samples;
@@ -241,7 +342,8 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_DIRECT_WP
                                                c3* __restrict__ lightc, int nl_max,
                                                Counters* __restrict__ cnt, int lds_cap, int* __restrict__ stk_ovf,
                                                const uint32_t* __restrict__ sun_vis,
-                                               const uint32_t* __restrict__ pathd = nullptr)
+                                               const uint32_t* __restrict__ pathd = nullptr,
+                                               const uint32_t* __restrict__ guess = nullptr)
 {
     extern __shared__ int s_stk[];
     const uint32_t item = blockIdx.x * RB + threadIdx.x;
@@ -280,8 +382,11 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_DIRECT_WP
                                             YG_LCG_POW);
                             seeded = true;
                         }
-                        if (dot(sp.P - L.from, L.dir) > 0) ndraws += 2u * (uint32_t)L.samples;
-                        c = area_illuminate<W>(R, L, sp, eye, seed, st, nshadow);
+                        const uint32_t gs =
+                            (guess && L.samples > 1) ? ((guess[item] >> (2 * li)) & 3u) : YG_GUESS_PENUMBRA;
+                        if (gs == YG_GUESS_PENUMBRA && dot(sp.P - L.from, L.dir) > 0)
+                            ndraws += 2u * (uint32_t)L.samples;
+                        c = area_illuminate<W>(R, L, sp, eye, seed, st, nshadow, gs);
                     }
                 }
                 lightc[(size_t)item * nl_max + li] = c;
@@ -1124,6 +1229,10 @@ static void area_setup(const yg_light& Lg, DevLight& L, std::vector<f3>& pts, st
     L.dir = norm3(cross(b - a, d - a));
     L.col = cscale(mkc(Lg.color[0], Lg.color[1], Lg.color[2]), Lg.power);
     L.from = (((a + b) + c) + d) * 0.25f;
+    L.fsamples = Lg.psamples;
+    L.corner = a;
+    L.toX = b - a;
+    L.toY = d - a;
 }
 
 // Counts render-buffer allocations: a changed value means some buffer may
@@ -1242,6 +1351,17 @@ struct RenderWS {
     DevBuf<int32_t> lc_rank, lc_row, lc_lvl, lc_adjs, lc_adj, lc_gstart;
     DevBuf<uint32_t> lc_in, lc_list, lc_cnt, lc_nb, lc_qual0, lc_gitem;
     DevBuf<unsigned long long> lc_err;
+    // guessLight (area lights with psamples > 0): verdicts, test bits, rand() stream
+    bool any_guess = false;
+    int g_nw = 0;
+    DevBuf<uint32_t> g_status, g_app, g_vis;
+    DevBuf<int32_t> g_woff, g_rnd;
+    uint64_t g_nrnd = 0;
+    DevBuf<unsigned long long> g_err;
+    UploadCache up_grnd, up_gwoff;
+    std::vector<int32_t> g_host_rnd;
+    uint32_t g_rnd_seed = 0;
+    uint64_t g_rnd_skip = 0;
     DevBuf<f3> lv_P, lv_N, lv_U, lv_V, lv_pP;
     DevBuf<float> lv_Z;
     DevBuf<int32_t> lv_mat;
@@ -1303,6 +1423,8 @@ struct ScenePlan {
     int aa_passes = 0;
     float aa_threshold = 0;
     bool area_lcg = false;  // antialiasing with area lights: areas in turn
+    bool guess = false;     // an area light with psamples > 0 (guessLight)
+    uint32_t rand_seed = 1; // srand() seed: the block shuffle's and guessLight's rand() stream
     // irradiance cache of the pathlight (cache = on)
     bool cache = false;
     float cache_size = 0, shadow_threshold = 0;
@@ -1365,6 +1487,16 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
             L.type = 0;
         } else if (Lg.type == YG_LIGHT_AREA) {
             area_setup(Lg, L, P.apts, P.ajh, P.ajv);
+            if (Lg.psamples < 0) return yg_fail(YG_ERR_ARG, "area light psamples must be >= 0");
+            if (Lg.psamples > 0 && L.samples > 1 && Lg.use_in_render) {
+                if (li >= 16) return yg_fail(YG_ERR_UNSUPPORTED, "area lights with psamples > 0 must be among the first 16 lights");
+                if (Lg.use_in_indirect)
+                    for (const auto& Lp : s->lights)
+                        if (Lp.type == YG_LIGHT_PATH)
+                            return yg_fail(YG_ERR_UNSUPPORTED,
+                                           "area light psamples > 0 inside GI paths is not on the GPU path");
+                P.guess = true;
+            }
         } else if (Lg.type == YG_LIGHT_GLOBALPHOTON) {
             if (R.gp_on) return yg_fail(YG_ERR_UNSUPPORTED, "more than one global photon light");
             if (s->gp.light != li || s->gp.gen != s->geo_gen)
@@ -1429,6 +1561,8 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
         if (R.nl > 32) return yg_fail(YG_ERR_UNSUPPORTED, "more than 32 lights with a pathlight");
     }
     if (o->aa_passes < 0) return yg_fail(YG_ERR_ARG, "aa_passes must be >= 0");
+    if (P.guess && o->aa_passes > 0)
+        return yg_fail(YG_ERR_UNSUPPORTED, "area light psamples > 0 with antialiasing passes is not on the GPU path");
     if (o->aa_passes > 0) {
         if (o->aa_minsamples < 1) return yg_fail(YG_ERR_ARG, "aa_minsamples must be >= 1");
         if (!(o->aa_pixelwidth >= 1.0f && o->aa_pixelwidth <= 2.0f))
@@ -1452,6 +1586,7 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
     P.aa_passes = o->aa_passes;
     P.aa_threshold = o->aa_threshold;
     block_regions(R.resx, R.resy, o->block > 0 ? o->block : 64, o->rand_seed, P.areas);
+    P.rand_seed = o->rand_seed;
     uint32_t acc = 0;
     for (auto& a : P.areas) {
         a.item_start = a.src_start = acc;
@@ -1673,8 +1808,19 @@ static int launch_frame(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
                 W.lds_cap, W.stk_ovf.p);
     YG_HIP(hipGetLastError());
     const bool area_path = W.area_path && W.any_path;
+    if (W.any_guess) {
+        YG_LAUNCH_WS(W.width, W.nocast, (k_guess_vis<8>), (k_guess_vis<4>), (k_guess_vis<2>), (k_guess_vis<24>),
+                     (k_guess_vis<20>), (k_guess_vis<18>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes_lvl, st,
+                     W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.g_woff.p, W.g_nw, W.g_vis.p,
+                     W.g_app.p, W.lds_cap, W.stk_ovf.p);
+        YG_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_guess_scan, dim3(1), dim3(64), 0, st, W.scenes.p, nitems, W.g_app.p, W.g_woff.p, W.g_nw,
+                           W.g_vis.p, W.g_rnd.p, W.g_nrnd, W.g_status.p, W.cnt.p, W.g_err.p);
+        YG_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_prep, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na,
-                       nitems, W.item_ti.p, W.item_t.p, W.draws.p, W.qual.p, area_path ? W.pre.p : nullptr);
+                       nitems, W.item_ti.p, W.item_t.p, W.draws.p, W.qual.p, area_path ? W.pre.p : nullptr,
+                       W.any_guess ? (const uint32_t*)W.g_status.p : nullptr);
     YG_HIP(hipGetLastError());
     if (area_path) {
         // the GI levels first: the items' ourRandom() positions depend on their paths' vertices
@@ -1721,7 +1867,8 @@ static int launch_frame(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
         // the whole stack
         YG_LAUNCH_WS(W.width, W.nocast, (k_direct<8, true>), (k_direct<4, true>), (k_direct<2, true>), (k_direct<24, true>), (k_direct<20, true>), (k_direct<18, true>), dim3(grid_for(nitems, RB)), dim3(RB),
                     W.stk_bytes, st, W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.lcg_off.p,
-                    W.lightc.p, nl_max, W.cnt.p, 0, W.stk_ovf.p, (const uint32_t*)nullptr);
+                    W.lightc.p, nl_max, W.cnt.p, 0, W.stk_ovf.p, (const uint32_t*)nullptr, (const uint32_t*)nullptr,
+                    W.any_guess ? (const uint32_t*)W.g_status.p : nullptr);
     } else {
         YG_LAUNCH_WS(W.width, W.nocast, (k_sun_vis<8>), (k_sun_vis<4>), (k_sun_vis<2>), (k_sun_vis<24>), (k_sun_vis<20>), (k_sun_vis<18>), dim3(grid_for(nitems, RB)), dim3(RB), W.stk_bytes_lvl,
                     st, W.scenes.p, W.areas.p, na, nitems, W.item_ti.p, W.item_t.p, W.sun_vis.p, W.cnt.p,
@@ -2263,6 +2410,42 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     W.any_area = any_area;
     W.any_photon = any_photon;
     W.photon_kmax = photon_kmax;
+    // guessLight: test bits per light, and glibc rand() after the block
+    // shuffle's draws (one per area), psamples draws per light at most per item
+    W.any_guess = false;
+    for (auto& P : plans) W.any_guess = W.any_guess || P.guess;
+    if (W.any_guess) {
+        if (plans.size() > 1)
+            return yg_fail(YG_ERR_UNSUPPORTED, "area light psamples > 0 in a batched render is not on the GPU path");
+        std::vector<int32_t> woff(std::max<size_t>(plans[0].lights.size(), 1), 0);
+        int nw = 0;
+        uint64_t per = 0;
+        for (size_t li = 0; li < plans[0].lights.size(); li++) {
+            const DevLight& L = plans[0].lights[li];
+            if (L.type == YG_LIGHT_AREA && L.use_in_render && L.fsamples > 0 && L.samples > 1) {
+                woff[li] = nw;
+                nw += 2 * ((L.samples + 31) / 32);
+                per += (uint64_t)L.fsamples;
+            }
+        }
+        W.g_nw = nw;
+        const uint64_t nrnd = (uint64_t)nitems * per;
+        if ((rc = W.g_status.ensure(nitems)) || (rc = W.g_app.ensure(nitems)) ||
+            (rc = W.g_vis.ensure((size_t)nitems * nw)) || (rc = W.g_woff.ensure(woff.size())) ||
+            (rc = W.g_rnd.ensure(nrnd)) || (rc = W.g_err.ensure(1)))
+            return rc;
+        const uint32_t seed = plans[0].rand_seed;
+        if (W.g_host_rnd.size() != nrnd || W.g_rnd_seed != seed || W.g_rnd_skip != (uint64_t)na_real) {
+            std::vector<int32_t> all;
+            glibc_rand(seed, (int64_t)(na_real + nrnd), all);
+            W.g_host_rnd.assign(all.begin() + na_real, all.end());
+            W.g_rnd_seed = seed;
+            W.g_rnd_skip = (uint64_t)na_real;
+        }
+        W.g_nrnd = nrnd;
+        if ((rc = upload_cached(W.g_rnd, W.g_host_rnd, W.up_grnd, st)) || (rc = upload_cached(W.g_woff, woff, W.up_gwoff, st)))
+            return rc;
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
     std::vector<hipEvent_t> kev;
     if (ktiming) timing = true;
@@ -2283,7 +2466,7 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
                                           W.stk_bytes,
                                           W.scan_tmp.cap, any_area, any_photon,
                                           (uint64_t)any_path | ((uint64_t)W.nocast << 1) |
-                                              ((uint64_t)W.area_path << 2)};
+                                              ((uint64_t)W.area_path << 2) | ((uint64_t)W.any_guess << 3)};
     int graph = 0;
     W.aa_pass = -1;
     W.hbase_on = aa_passes > 0 && any_path;

@@ -19,7 +19,15 @@ struct DevLight {
     float fixedradius;
     float power;  // the light's "power" parameter
     c3 color;     // the light's "color" parameter
+    int fsamples; // area "psamples": guessLight's test draws (0: always the penumbra loop)
+    f3 corner, toX, toY;  // area: a, b - a, d - a (the one-sample light)
 };
+
+// guessLight's verdict for one area light at one shading point
+// (arealight.cc:176-200): 2 bits per light index in an item's status word
+#define YG_GUESS_PENUMBRA 0u
+#define YG_GUESS_LIGHT 1u
+#define YG_GUESS_SHADOW 2u
 
 // Photon map view for the K-nearest gather.
 struct PhotonView {
@@ -231,17 +239,49 @@ __device__ __forceinline__ int32_t lcg_jump(uint32_t seed, uint64_t n, const uin
     return (int32_t)acc;
 }
 
-// areaLight_t::illuminate, PENUMBRA branch (arealight.cc:93-174), draws taken
-// from the ourRandom() stream starting at *seed
+// areaLight_t::illuminate (arealight.cc:93-174), draws taken from the
+// ourRandom() stream starting at *seed: the one-sample light (a random point
+// of the quad), else by guessLight's verdict `status` (YG_GUESS_*; the
+// penumbra loop when the light has no psamples)
 template <int W>
 __device__ __forceinline__ c3 area_illuminate(const RenderScene& R, const DevLight& L, const Surf& sp,
-                                              f3 eye, int32_t& seed, LaneStack& st, uint32_t& nshadow)
+                                              f3 eye, int32_t& seed, LaneStack& st, uint32_t& nshadow,
+                                              uint32_t status = YG_GUESS_PENUMBRA)
 {
     c3 resul = mkc(0, 0, 0);
     const f3 edir = norm3(eye);
     const f3 N = face_forward(sp.Ng, sp.N, edir);
     const float plane_at = dot(sp.P - L.from, L.dir);
     if (plane_at <= 0) return mkc(0, 0, 0);
+    if (L.samples == 1) {
+        // corner + toX*ourRandom() + toY*ourRandom(): g++ evaluates the right
+        // operand of + first, so toY takes the first draw
+        const float r2 = our_random(seed), r1 = our_random(seed);
+        const f3 sampleP = (L.corner + L.toX * r1) + L.toY * r2;
+        const f3 Lv = sampleP - sp.P;
+        if (dot(Lv, N) < 0) return mkc(0, 0, 0);
+        nshadow++;
+        if (!is_shadowed<W>(R.bvh, sp.P, sampleP, sp.obj, true, R.min_raydis, R.self_bias, st)) {
+            f3 dir = Lv;
+            float LD2 = normlensqr3(dir);
+            LD2 = (LD2 != 0) ? (1.0f / LD2) : 1.0f;
+            resul = cadd(resul, from_light(R, sp, dir, cscale(L.col, LD2), eye));
+        }
+        return cscale(resul, plane_at);
+    }
+    if (status == YG_GUESS_SHADOW) return mkc(0, 0, 0);  // fromLight of a zero colour
+    if (status == YG_GUESS_LIGHT) {
+        // every point unjittered, no shadow ray
+        for (int i = 0; i < L.samples; ++i) {
+            f3 dir = R.apts[L.pts + i] - sp.P;
+            float LD2 = normlensqr3(dir);
+            LD2 = (LD2 != 0) ? (1.0f / LD2) : 1.0f;
+            resul = cadd(resul, from_light(R, sp, dir, cscale(L.col, LD2), eye));
+        }
+        const c3 r = cscale(resul, plane_at);
+        const float fs = (float)L.samples;
+        return mkc(r.r / fs, r.g / fs, r.b / fs);
+    }
     for (int i = 0; i < L.samples; ++i) {
         const float dish = (float)((double)our_random(seed) - 0.5);
         const float disv = (float)((double)our_random(seed) - 0.5);

@@ -94,8 +94,24 @@ int lightFactory(const std::string& type, const paramMap_t& p, yg_light& L)
         p.getParam("samples", L.samples);
         p.getParam("psamples", L.psamples);
         p.getParam("dummy", dummy);
-        if (dummy) return fail(YG_ERR_UNSUPPORTED, "arealight: dummy lights are not on the GPU path");
-        if (L.psamples != 0) return fail(YG_ERR_UNSUPPORTED, "arealight: psamples > 0 is not on the GPU path");
+        L.dummy = dummy ? 1 : 0;  // emits global photons only (arealight.cc:98)
+        if (L.psamples < 0) L.psamples = 0;
+        return YG_OK;
+    }
+    if (type == "globalphotonlight") {
+        // globalPhotonLight_t::factory (globalphotonlight.cc:280-294)
+        L.type = YG_LIGHT_GLOBALPHOTON;
+        L.fixedradius = 1.0f;
+        L.depth = 2;
+        L.photons = 50000;
+        L.search = 200;
+        p.getParam("radius", L.fixedradius);
+        p.getParam("depth", L.depth);
+        p.getParam("photons", L.photons);
+        p.getParam("search", L.search);
+        int cdepth = 4;
+        p.getParam("caus_depth", cdepth);  // caustic objects are refused by the loader
+        L.use_in_indirect = 0;
         return YG_OK;
     }
     if (type == "pathlight") {
@@ -558,8 +574,12 @@ int gpuInterface_t::render(const paramMap_t& params, std::vector<float>& rgb, yg
         o.has_background = 1;
         for (int c = 0; c < 3; c++) o.background[c] = bit->second[c];
     }
-    for (uint32_t li = 0; li < lights.size(); li++)
+    // the photon lights' init (scene_t::setupLights, scene.cc:279-295) before the render
+    for (uint32_t li = 0; li < lights.size(); li++) {
         if (lights[li].type == YG_LIGHT_PHOTON && (rc = yg_photon_build(s, li, &o, nullptr, nullptr))) return rc;
+        if (lights[li].type == YG_LIGHT_GLOBALPHOTON && (rc = yg_gphoton_build(s, li, &o, nullptr, nullptr)))
+            return rc;
+    }
     const yg_camera& cam = cit->second;
     const size_t n = (size_t)cam.resx * cam.resy * 3;
     if (hipMalloc((void**)&guard.d, n * sizeof(float)) != hipSuccess) return fail(YG_ERR_NOMEM, "output buffer");
