@@ -33,7 +33,7 @@ extern "C" {
 #define YG_ERR_UNSUPPORTED (-4)
 #define YG_ERR_NOMEM (-5)
 
-#define YG_ABI_VERSION 12
+#define YG_ABI_VERSION 13
 
 /* ---------------------------------------------------------------- rays -- */
 
@@ -112,12 +112,25 @@ enum {
 
 enum { YG_MOD_MIX = 0, YG_MOD_MUL = 1, YG_MOD_ADD = 2, YG_MOD_SUB = 3 };
 
+/* Texture coordinates of a modulator (modulator_t::doMapping,
+ * texture.cc:125-165; names as string2texcotype, texture.h:135-149). */
+enum {
+    YG_TEXCO_GLOBAL = 0, /* "global": the hit point P */
+    YG_TEXCO_ORCO = 1,   /* "orco" (the factory default): the mesh's interpolated
+                            orco points, or the object's orco matrix times P */
+    YG_TEXCO_UV = 2,     /* "uv" (any other name): (2u - 1, 1 - 2v, 0) */
+    YG_TEXCO_WIN = 3,    /* "window": the camera sample's screen position */
+    YG_TEXCO_NOR = 4,    /* "normal": the shading normal */
+    YG_TEXCO_REFL = 5    /* "reflect": the eye vector reflected about the
+                            face-forwarded shading normal */
+};
+
 /* modulator_t on a procedural texture (modulator_t::doMapping,
- * texture.cc:122-317, procedural branch): the hit point P (texco "global",
- * or "orco" of an object that was never transformed, where orco = P),
- * times the texture matrix when there is one (texpt = M * P), then per
- * output axis a: texpt[a] = size[a] * (T[proj[a]] + ofs[a]); colour blend by
- * mode and factor "color" (texture.cc:499-534). */
+ * texture.cc:122-317, procedural branch): the texture coordinates T of
+ * `texco`, times the texture matrix when there is one (texpt = M * T; no
+ * translation for "normal" and "reflect"), then per output axis a:
+ * texpt[a] = size[a] * (T[proj[a]] + ofs[a]); colour blend by mode and
+ * factor "color" (texture.cc:499-534). */
 typedef struct yg_modulator {
     int32_t tex;         /* index into the texture table */
     int32_t mode;        /* YG_MOD_* */
@@ -127,6 +140,7 @@ typedef struct yg_modulator {
     int32_t proj[3];     /* proj_x/y/z: 0 = the same axis, 1..3 = x, y, z, -1 = none ("n") */
     int32_t has_mtx;     /* 0: no texture matrix (identity) */
     float mtx[12];       /* rows 0..2 of the texture matrix m00..m23, row-major */
+    int32_t texco;       /* YG_TEXCO_* */
 } yg_modulator;
 
 /* genericShader_t in its diffuse form (basicshaders.cc:30-70,191-204):
@@ -300,6 +314,36 @@ int yg_mesh_add(yg_scene* s, const float* xyz, uint32_t nv, const uint32_t* idx,
  * shading normal of triangle_t::getSurface (triangle.cc:142-160) and the
  * tangent frame around it.  Call after yg_mesh_add, before yg_build. */
 int yg_mesh_autosmooth(yg_scene* s, int32_t obj_id, float angle);
+
+/* Texture-coordinate data of mesh obj_id (faces as join_face / join_lface read
+ * them, msin_object.cc:116-230, and render.cc:743-764; addObject_trimesh,
+ * interface_impl.cc:229-238), read by modulators with texco "uv" and "orco":
+ *   face_uv     6 floats per triangle of the object (u_a v_a u_b v_b u_c v_c),
+ *               or NULL;
+ *   face_flags  per triangle: bit0 = the face has uv, bit1 = it has vertex
+ *               colours (NULL: every face has uv when face_uv is given, none
+ *               has vertex colours).  A triangle with neither, no autoSmooth
+ *               normals and no orco points has u = v = -1 (triangle.cc:140),
+ *               else u = v = 0 unless it has uv;
+ *   orco        3 floats per vertex of the object (has_orco="on": the orco
+ *               point stored after each vertex), or NULL;
+ *   orco_mtx    rows 0..2 of meshObject_t's backOrco (toObjectOrco, used when
+ *               orco is NULL), from yg_orco_matrix; NULL: the matrix of an
+ *               object with the identity transform, computed from its
+ *               vertices.
+ * Vertex colours are not stored: the generic shader does not read them.
+ * Call after yg_mesh_add (and yg_mesh_autosmooth), before yg_build. */
+int yg_mesh_texcoords(yg_scene* s, int32_t obj_id, const float* face_uv, const uint8_t* face_flags,
+                      const float* orco, const float* orco_mtx);
+
+/* meshObject_t::transform's orco matrix (mesh.cc:422-493): M (4x4 row-major
+ * object transform, render_t::M; NULL = identity) scaled by half the extent of
+ * the untransformed bound of the triangles' vertices xyz (recalcBound with
+ * BOX_MIN, mesh.cc:343-380), its translation replaced by the centre of the
+ * transformed bound, inverted (matrix4x4_t::inverse, matrix4.cc:78-112); rows
+ * 0..2 into out12.  Vertices in float as the reference: M * p. */
+int yg_orco_matrix(const float* m16, const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                   float* out12);
 
 /* kdTree_t::kdTree_t (kdtree.cc:57-138) + buildObjectTree (object3d.cc:30-53):
  * builds the SAH BVH over all meshes and uploads it. */

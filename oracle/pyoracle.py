@@ -46,6 +46,10 @@ def _load(path):
         L.yo_scene_free.argtypes = [vp]
         L.yo_mesh_autosmooth.restype = C.c_int
         L.yo_mesh_autosmooth.argtypes = [vp, C.c_int, C.c_float]
+        L.yo_mesh_texcoords.restype = C.c_int
+        L.yo_mesh_texcoords.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+        L.yo_orco_matrix.restype = C.c_int
+        L.yo_orco_matrix.argtypes = [vp, vp, C.c_int64, vp, C.c_int64, vp]
         L.yo_shading_normals.argtypes = [vp, I32P, F32P, C.c_int64, F32P]
         L.yo_scene_nodes.restype = C.c_int64
         L.yo_scene_nodes.argtypes = [vp]
@@ -150,6 +154,22 @@ def heap_offer_trace(weights, limit):
     return [(int(out[2 * i]), float(out[2 * i + 1])) for i in range(n)]
 
 
+def orco_matrix(verts, idx, m16=None, fast=False):
+    """meshObject_t::transform's orco matrix (rows 0..2) of a mesh with
+    untransformed vertices verts [nv, 3], triangles idx [nt, 3] and object
+    matrix m16 (4x4, None = identity)."""
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1)
+    f = np.ascontiguousarray(idx, np.uint32).reshape(-1)
+    m = None if m16 is None else np.ascontiguousarray(m16, np.float32).reshape(-1)
+    out = np.zeros(12, np.float32)
+    vp = C.c_void_p
+    rc = lib(fast).yo_orco_matrix(None if m is None else m.ctypes.data_as(vp), v.ctypes.data_as(vp), len(v) // 3,
+                                  f.ctypes.data_as(vp), len(f) // 3, out.ctypes.data_as(vp))
+    if rc:
+        raise RuntimeError("yo_orco_matrix failed (%d)" % rc)
+    return out.reshape(3, 4)
+
+
 class OracleScene:
     """Triangle soup with per-triangle object and material ids."""
 
@@ -159,6 +179,10 @@ class OracleScene:
         oc = cls(desc.verts, desc.idx, desc.tri_obj, desc.tri_mat, fast=fast)
         for o, a in getattr(desc, "smooth", {}).items():
             oc.autosmooth(o, a)
+        for o, t in getattr(desc, "tex", {}).items():
+            sel = np.nonzero(desc.tri_obj == o)[0]
+            corner = desc.orco[desc.idx[sel]].reshape(-1, 9) if t.get("orco") else None
+            oc.texcoords(o, t.get("uv"), t.get("flags"), corner, t.get("mtx"))
         return oc
 
     def __init__(self, verts, idx, tri_obj=None, tri_mat=None, fast=False):
@@ -174,6 +198,20 @@ class OracleScene:
     def autosmooth(self, obj, angle):
         """meshObject_t::autoSmooth of object obj (vertex normals)."""
         lib(self.fast).yo_mesh_autosmooth(self.h, int(obj), float(angle))
+
+    def texcoords(self, obj, face_uv=None, face_flags=None, corner_orco=None, orco_mtx=None):
+        """Texture-coordinate data of object obj: face uv [nt_obj, 6], face
+        flags [nt_obj], corner orco points [nt_obj, 9], orco matrix (12)."""
+        def arr(a, dt):
+            return None if a is None else np.ascontiguousarray(a, dt).reshape(-1)
+        self._tex_keep = getattr(self, "_tex_keep", [])
+        a = [arr(face_uv, np.float32), arr(face_flags, np.uint8), arr(corner_orco, np.float32),
+             arr(orco_mtx, np.float32)]
+        self._tex_keep.append(a)
+        p = [None if x is None else x.ctypes.data_as(C.c_void_p) for x in a]
+        rc = lib(self.fast).yo_mesh_texcoords(self.h, int(obj), *p)
+        if rc:
+            raise RuntimeError("yo_mesh_texcoords failed (%d)" % rc)
 
     def shading_normals(self, prim, h):
         """getSurface's shading normals at hit points h [n, 3] of triangles prim."""

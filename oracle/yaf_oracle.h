@@ -39,6 +39,18 @@ yo_scene* yo_scene_new(const float* verts, int64_t nv, const uint32_t* idx, int6
 void yo_scene_free(yo_scene* s);
 /* meshObject_t::autoSmooth (mesh.cc:199-282) of object obj's triangles */
 int yo_mesh_autosmooth(yo_scene* s, int obj, float angle);
+/* meshObject_t::transform's orco matrix (mesh.cc:422-493): M (4x4 row-major,
+ * NULL = identity) scaled by half the untransformed triangle bound's extent,
+ * translation = centre of the transformed bound, inverted; rows 0..2 */
+int yo_orco_matrix(const float* m16, const float* xyz, int64_t nv, const uint32_t* idx, int64_t nt,
+                   float* out12);
+/* texture-coordinate data of object obj's triangles (in scene order):
+ * face_uv 6 per triangle or NULL, face_flags per triangle (bit0 uv, bit1
+ * vertex colours; NULL: all have uv when face_uv is given), corner_orco 9
+ * per triangle (the orco points of a, b, c) or NULL, orco_mtx 12 or NULL
+ * (keep the identity-transform matrix yo_scene_new computed) */
+int yo_mesh_texcoords(yo_scene* s, int obj, const float* face_uv, const uint8_t* face_flags,
+                      const float* corner_orco, const float* orco_mtx);
 /* getSurface's shading normal of n hits (prim, hit point) */
 void yo_shading_normals(yo_scene* s, const int32_t* prim, const float* h, int64_t n, float* out);
 /* canonical BVH2 node count (binned SAH, 32 bins, leaf <= 4, FP64 boxes) */

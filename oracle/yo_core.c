@@ -260,7 +260,124 @@ yo_scene* yo_scene_new(const float* verts, int64_t nv, const uint32_t* idx, int6
         s->nodes[0].left = s->nodes[0].right = -1; s->nodes[0].start = 0; s->nodes[0].count = 0;
     }
     free(bc.tb); free(bc.cen);
+    /* every object's orco matrix for the identity transform */
+    s->nobj = 0;
+    for (int64_t i = 0; i < nt; i++)
+        if (s->tri_obj[i] + 1 > s->nobj) s->nobj = s->tri_obj[i] + 1;
+    s->obj_orco = (float*)calloc((size_t)(12 * (s->nobj ? s->nobj : 1)), sizeof(float));
+    for (int64_t t0 = 0; t0 < nt;) {
+        int64_t t1 = t0;
+        while (t1 < nt && s->tri_obj[t1] == s->tri_obj[t0]) t1++;
+        yo_orco_matrix(NULL, s->verts, nv, s->idx + 3 * t0, t1 - t0, s->obj_orco + 12 * (int64_t)s->tri_obj[t0]);
+        t0 = t1;
+    }
     return s;
+}
+
+/* matrix4x4_t::inverse (matrix4.cc:78-112): Gauss-Jordan on the rows with the
+ * largest-magnitude pivot, float arithmetic; 0 when singular */
+static int inv4(float m[4][4])
+{
+    float e[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+    for (int c = 0; c < 4; c++) {
+        int piv = 0;
+        float best = 0;
+        for (int r = c; r < 4; r++) {
+            const float a = fabsf(m[r][c]);
+            if (a > best) { best = a; piv = r; }
+        }
+        if (best == 0) return 0;
+        for (int k = 0; k < 4; k++) {
+            float t = m[c][k]; m[c][k] = m[piv][k]; m[piv][k] = t;
+            t = e[c][k]; e[c][k] = e[piv][k]; e[piv][k] = t;
+        }
+        const float d = m[c][c];
+        for (int k = 0; k < 4; k++) m[c][k] = m[c][k] / d;
+        for (int k = 0; k < 4; k++) e[c][k] = e[c][k] / d;
+        for (int r = 0; r < 4; r++) {
+            if (r == c) continue;
+            const float f = m[r][c];
+            for (int k = 0; k < 4; k++) m[r][k] = m[r][k] - m[c][k] * f;
+            for (int k = 0; k < 4; k++) e[r][k] = e[r][k] - e[c][k] * f;
+        }
+    }
+    memcpy(m, e, sizeof(e));
+    return 1;
+}
+
+/* recalcBound (mesh.cc:343-380) over the triangles' corners, +-BOX_MIN (1e-5) */
+static void corner_bound(const float* xyz, const uint32_t* idx, int64_t nt, float lo[3], float hi[3])
+{
+    for (int a = 0; a < 3; a++) lo[a] = hi[a] = xyz[3 * (int64_t)idx[0] + a];
+    for (int64_t k = 0; k < 3 * nt; k++)
+        for (int a = 0; a < 3; a++) {
+            const float v = xyz[3 * (int64_t)idx[k] + a];
+            if (v < lo[a]) lo[a] = v;
+            if (v > hi[a]) hi[a] = v;
+        }
+    for (int a = 0; a < 3; a++) {
+        lo[a] = (float)((double)lo[a] - 1e-5);
+        hi[a] = (float)((double)hi[a] + 1e-5);
+    }
+}
+
+int yo_orco_matrix(const float* m16, const float* xyz, int64_t nv, const uint32_t* idx, int64_t nt,
+                   float* out12)
+{
+    if (nt <= 0) return -1;
+    float M[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+    if (m16) memcpy(M, m16, sizeof(M));
+    float lo[3], hi[3];
+    corner_bound(xyz, idx, nt, lo, hi);
+    float B[4][4];
+    memcpy(B, M, sizeof(B));
+    /* matrix4x4_t::scale (matrix4.cc:180-185): columns 0..2 of rows 0..2 */
+    for (int c = 0; c < 3; c++) {
+        const float ext = hi[c] - lo[c];
+        const float half = (float)(0.5 * (double)ext);
+        for (int r = 0; r < 3; r++) B[r][c] = B[r][c] * half;
+    }
+    /* the vertices transformed by M, each row ((m0 x + m1 y) + m2 z) + m3 */
+    float* w = (float*)malloc(sizeof(float) * 3 * (size_t)(nv ? nv : 1));
+    for (int64_t v = 0; v < nv; v++)
+        for (int r = 0; r < 3; r++)
+            w[3 * v + r] = M[r][0] * xyz[3 * v] + M[r][1] * xyz[3 * v + 1] + M[r][2] * xyz[3 * v + 2] + M[r][3];
+    corner_bound(w, idx, nt, lo, hi);
+    free(w);
+    for (int r = 0; r < 3; r++) {
+        const float sum = lo[r] + hi[r];
+        B[r][3] = 0.5f * sum;
+    }
+    if (!inv4(B)) return -2;
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 4; c++) out12[4 * r + c] = B[r][c];
+    return 0;
+}
+
+int yo_mesh_texcoords(yo_scene* s, int obj, const float* face_uv, const uint8_t* face_flags,
+                      const float* corner_orco, const float* orco_mtx)
+{
+    if (obj < 0 || obj >= s->nobj) return -1;
+    int64_t t0 = 0;
+    while (t0 < s->nt && s->tri_obj[t0] != obj) t0++;
+    int64_t t1 = t0;
+    while (t1 < s->nt && s->tri_obj[t1] == obj) t1++;
+    if (orco_mtx) memcpy(s->obj_orco + 12 * (int64_t)obj, orco_mtx, 12 * sizeof(float));
+    if (!(face_uv || face_flags || corner_orco)) return 0;
+    if (!s->tri_tex) s->tri_tex = (float*)calloc((size_t)(16 * (s->nt ? s->nt : 1)), sizeof(float));
+    for (int64_t t = t0; t < t1; t++) {
+        float* r = s->tri_tex + 16 * t;
+        const int64_t k = t - t0;
+        int fl = face_flags ? (face_flags[k] & 3) : (face_uv ? 1 : 0);
+        if (!face_uv) fl &= ~1;
+        for (int c = 0; c < 6; c++) r[c] = (fl & 1) ? face_uv[6 * k + c] : 0.0f;
+        if (corner_orco) {
+            for (int c = 0; c < 9; c++) r[6 + c] = corner_orco[9 * k + c];
+            fl |= 4;
+        }
+        r[15] = (float)fl;
+    }
+    return 0;
 }
 
 /* meshObject_t::autoSmooth (mesh.cc:199-282) for the triangles of object
@@ -361,6 +478,7 @@ void yo_scene_free(yo_scene* s)
 {
     if (!s) return;
     free(s->tri_vn); free(s->tri_smooth);
+    free(s->tri_tex); free(s->obj_orco);
     yo_cache_release(s);
     free(s->verts); free(s->idx); free(s->tri_obj); free(s->tri_mat); free(s->tri_n);
     free(s->nodes); free(s->order);

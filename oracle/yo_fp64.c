@@ -684,6 +684,11 @@ int yo_render_fp64(yo_scene* s, const yg_camera* cam, const yg_material* mats, i
         if (L->type == YG_LIGHT_AREA && L->psamples > 0 && mode == YO_FP64_REFERENCE) return -6;
     }
     if (npath > 1) return -7;
+    /* texture coordinates other than the hit point are restated only in the
+     * float oracle */
+    for (int m = 0; m < nmat; m++)
+        for (int k = 0; k < mats[m].mod_count; k++)
+            if (mods[mats[m].mod_first + k].texco != YG_TEXCO_GLOBAL) return -6;
 
     fctx F;
     memset(&F, 0, sizeof(F));

@@ -180,6 +180,13 @@ struct yo_scene {
     yo_gmap gm;
     float* tri_vn;         /* autoSmooth vertex normals, 9 per triangle (NULL: none) */
     uint8_t* tri_smooth;   /* per triangle: has vertex normals */
+    /* texture coordinates: per triangle u_a v_a u_b v_b u_c v_c, the orco
+     * points of corners a, b, c, flags (bit0 uv, bit1 vertex colours, bit2
+     * orco points) = 16 floats (NULL: no triangle has any); per object the
+     * orco matrix, rows 0..2 of backOrco (mesh.cc:488-491) */
+    float* tri_tex;
+    int32_t nobj;
+    float* obj_orco;
     struct yo_lcache_s* lc_last;  /* irradiance cache of the last cache = on render */
     uint64_t lc_new;
     int lc_rec;                   /* record the fake pass's visits (test hook) */

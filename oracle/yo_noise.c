@@ -473,18 +473,25 @@ static yo_col tex_color(const yg_texture* t, yo_v3 p)
     return col_add(c1, col_scale(col_sub(c2, c1), f));
 }
 
-/* modulator_t::doMapping, procedural branch (texture.cc:122-180, 311-317):
- * global coordinates (or orco of an untransformed mesh, which is P), the
- * texture matrix (matrix4x4_t * point3d_t, matrix4.h:86-91) when given, then
+/* modulator_t::doMapping, procedural branch (texture.cc:167-180, 311-317)
+ * from the texco's coordinates P: the texture matrix when given
+ * (matrix4x4_t * point3d_t, matrix4.h:86-91; for a direction, vec != 0, the
+ * vector form without translation, matrix4.h:79-84), then
  * size * (T[projected axis] + ofs) per axis, or size * ofs for axis "n". */
-void yo_mod_texpt(const yg_modulator* m, yo_v3 P, float out[3])
+void yo_mod_texpt_v(const yg_modulator* m, yo_v3 P, int vec, float out[3])
 {
     float T[3] = {P.x, P.y, P.z};
     if (m->has_mtx) {
         const float* M = m->mtx;
-        T[0] = M[0] * P.x + M[1] * P.y + M[2] * P.z + M[3];
-        T[1] = M[4] * P.x + M[5] * P.y + M[6] * P.z + M[7];
-        T[2] = M[8] * P.x + M[9] * P.y + M[10] * P.z + M[11];
+        const float tx = vec ? 0.0f : M[3], ty = vec ? 0.0f : M[7], tz = vec ? 0.0f : M[11];
+        T[0] = M[0] * P.x + M[1] * P.y + M[2] * P.z;
+        T[1] = M[4] * P.x + M[5] * P.y + M[6] * P.z;
+        T[2] = M[8] * P.x + M[9] * P.y + M[10] * P.z;
+        if (!vec) {
+            T[0] = T[0] + tx;
+            T[1] = T[1] + ty;
+            T[2] = T[2] + tz;
+        }
     }
     for (int a = 0; a < 3; a++) {
         const int pj = m->proj[a];
@@ -492,12 +499,15 @@ void yo_mod_texpt(const yg_modulator* m, yo_v3 P, float out[3])
     }
 }
 
-/* modulator_t::modulate colour part (texture.cc:499-534) */
-yo_col yo_modulate(const yg_modulator* m, const yg_texture* texs, yo_col C, yo_v3 P)
+void yo_mod_texpt(const yg_modulator* m, yo_v3 P, float out[3]) { yo_mod_texpt_v(m, P, 0, out); }
+
+/* modulator_t::modulate colour part (texture.cc:499-534) at the texco's
+ * coordinates P (vec: a direction) */
+yo_col yo_modulate_v(const yg_modulator* m, const yg_texture* texs, yo_col C, yo_v3 P, int vec)
 {
     const yg_texture* t = &texs[m->tex];
     float q[3];
-    yo_mod_texpt(m, P, q);
+    yo_mod_texpt_v(m, P, vec, q);
     yo_v3 tp = v3(q[0], q[1], q[2]);
     yo_col tc = tex_color(t, tp);
     switch (m->mode) {
@@ -515,6 +525,11 @@ yo_col yo_modulate(const yg_modulator* m, const yg_texture* texs, yo_col C, yo_v
         break;
     }
     return C;
+}
+
+yo_col yo_modulate(const yg_modulator* m, const yg_texture* texs, yo_col C, yo_v3 P)
+{
+    return yo_modulate_v(m, texs, C, P, 0);
 }
 
 void yo_diffuse_at(const yg_material* mats, const yg_modulator* mods, const yg_texture* texs, int mat,

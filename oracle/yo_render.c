@@ -9,6 +9,7 @@
 #include "yo_internal.h"
 
 yo_col yo_modulate(const yg_modulator* m, const yg_texture* texs, yo_col C, yo_v3 P);
+yo_col yo_modulate_v(const yg_modulator* m, const yg_texture* texs, yo_col C, yo_v3 P, int vec);
 void yo_mod_texpt(const yg_modulator* m, yo_v3 P, float out[3]);
 yo_v3 yo_shading_normal(const yo_scene* s, int32_t prim, yo_v3 h);
 int yo_first_hit_one(const yo_scene* s, yo_v3 from, yo_v3 ray, float min_raydis, int32_t* prim,
@@ -24,6 +25,7 @@ typedef struct {
     yo_v3 P, N, Ng, NU, NV;
     float Z;
     int32_t prim, obj, mat;
+    float sx, sy;  /* screenpos: set at camera-ray hits (scene.cc:234), else 0 */
 } yo_sp;
 
 /* area light precomputation, arealight.cc:31-91 */
@@ -108,6 +110,7 @@ static void make_sp(const yo_scene* s, int32_t prim, yo_v3 f, yo_v3 ray, float Z
     create_cs(sp->N, &sp->NU, &sp->NV);
     sp->obj = s->tri_obj[prim];
     sp->mat = s->tri_mat[prim];
+    sp->sx = sp->sy = 0.0f;
 }
 
 /* scene_t::firstHit (scene.cc:688-721); shadow: its shadow flag, which
@@ -121,13 +124,79 @@ static int first_hit(rctx* R, yo_v3 from, yo_v3 ray, yo_sp* sp, int shadow)
     return 1;
 }
 
-/* genericShader_t::getDiffuse, basicshaders.cc:191-204 */
-static yo_col get_diffuse(const rctx* R, const yo_sp* sp)
+/* modulator_t::doMapping's coordinates (texture.cc:125-165) for texco at sp,
+ * eye the shader call's eye vector; *vec = 1 for a direction.  uv and orco
+ * come from triangle_t::getSurface (triangle.cc:138-167): a triangle with no
+ * uv, vertex colours, vertex normals or orco points has u = v = -1 and
+ * hasOrco() false; else getInterpolation's weights (fa = 1 - fb - fc) apply,
+ * with u = v = 0 on a face without uv. */
+static yo_v3 tex_coords(const rctx* R, int texco, const yo_sp* sp, yo_v3 eye, int* vec)
+{
+    const yo_scene* S = R->s;
+    *vec = 0;
+    switch (texco) {
+    case YG_TEXCO_UV:
+    case YG_TEXCO_ORCO: {
+        const float* r = S->tri_tex ? S->tri_tex + 16 * (int64_t)sp->prim : NULL;
+        const int fl = r ? (int)r[15] : 0;
+        if (texco == YG_TEXCO_ORCO && !(fl & 4)) {
+            /* toObjectOrco: backOrco * P (matrix4.h:86-91) */
+            const float* M = S->obj_orco + 12 * (int64_t)sp->obj;
+            const yo_v3 P = sp->P;
+            return v3(M[0] * P.x + M[1] * P.y + M[2] * P.z + M[3], M[4] * P.x + M[5] * P.y + M[6] * P.z + M[7],
+                      M[8] * P.x + M[9] * P.y + M[10] * P.z + M[11]);
+        }
+        const int smooth = S->tri_smooth && S->tri_smooth[sp->prim];
+        float u = -1.0f, v = -1.0f;
+        if (fl || smooth) {
+            float fb, fc;
+            const yo_v3 A = tri_vert(S, sp->prim, 0), B = tri_vert(S, sp->prim, 1), C = tri_vert(S, sp->prim, 2);
+            const float a[3] = {A.x, A.y, A.z}, b[3] = {B.x, B.y, B.z}, c[3] = {C.x, C.y, C.z};
+            const float n[3] = {sp->Ng.x, sp->Ng.y, sp->Ng.z}, h[3] = {sp->P.x, sp->P.y, sp->P.z};
+            yo_tri_bary(a, b, c, n, h, &fb, &fc);
+            const float fa = 1.f - fb - fc;
+            if (texco == YG_TEXCO_ORCO)
+                return v3_add(v3_add(v3_mul(v3(r[6], r[7], r[8]), fa), v3_mul(v3(r[9], r[10], r[11]), fb)),
+                              v3_mul(v3(r[12], r[13], r[14]), fc));
+            if (fl & 1) {
+                u = r[0] * fa + r[2] * fb + r[4] * fc;
+                v = r[1] * fa + r[3] * fb + r[5] * fc;
+            } else
+                u = v = 0.0f;
+        }
+        /* texpt.x = u*2.0 - 1.0, texpt.y = 1.0 - v*2.0 (double literals) */
+        return v3((float)((double)u * 2.0 - 1.0), (float)(1.0 - (double)v * 2.0), 0.0f);
+    }
+    case YG_TEXCO_WIN:
+        return v3(sp->sx, sp->sy, 0.0f);
+    case YG_TEXCO_NOR:
+        *vec = 1;
+        return sp->N;
+    case YG_TEXCO_REFL: {
+        /* N = FACE_FORWARD(Ng, N, neye); reflect(N, neye) (vector3d.h:221-226) */
+        *vec = 1;
+        const yo_v3 ne = v3_norm(eye);
+        const yo_v3 n = face_forward(sp->Ng, sp->N, ne);
+        const float vn = v3_dot(ne, n);
+        if (vn < 0) return v3_neg(ne);
+        return v3_sub(v3_mul(n, 2.0f * vn), ne);
+    }
+    default:
+        return sp->P;
+    }
+}
+
+/* genericShader_t::getDiffuse, basicshaders.cc:191-204, with the call's eye */
+static yo_col get_diffuse(const rctx* R, const yo_sp* sp, yo_v3 eye)
 {
     const yg_material* m = &R->mats[sp->mat];
     yo_col c = col(m->color[0], m->color[1], m->color[2]);
-    for (int k = 0; k < m->mod_count; k++)
-        c = yo_modulate(&R->mods[m->mod_first + k], R->texs, c, sp->P);
+    for (int k = 0; k < m->mod_count; k++) {
+        const yg_modulator* md = &R->mods[m->mod_first + k];
+        int vec;
+        const yo_v3 T = tex_coords(R, md->texco, sp, eye, &vec);
+        c = yo_modulate_v(md, R->texs, c, T, vec);
+    }
     return c;
 }
 
@@ -138,7 +207,7 @@ static yo_col from_light(const rctx* R, const yo_sp* sp, yo_v3 edir_dir, yo_col 
     yo_v3 N = face_forward(sp->Ng, sp->N, edir);
     float inte = v3_dot(N, edir_dir);
     if (inte < 0) return col(0, 0, 0);
-    yo_col lkc = get_diffuse(R, sp);
+    yo_col lkc = get_diffuse(R, sp, eye);
     return col_mul(col_scale(ecol, inte), lkc);
 }
 
@@ -146,7 +215,7 @@ static yo_col from_light(const rctx* R, const yo_sp* sp, yo_v3 edir_dir, yo_col 
 static yo_col from_radiosity(const rctx* R, const yo_sp* sp, yo_v3 edir_dir, yo_col ecol, yo_v3 eye)
 {
     if (v3_dot(face_forward(sp->Ng, sp->N, eye), edir_dir) < 0) return col(0, 0, 0);
-    yo_col lkc = get_diffuse(R, sp);
+    yo_col lkc = get_diffuse(R, sp, eye);
     return col_mul(ecol, lkc);
 }
 
@@ -441,13 +510,13 @@ static yo_col path_take_sample(rctx* R, int li, yo_v3 N, const yo_sp* sp, float*
                     /* getLight with the irradiance map (pathlight.cc:140-185):
                      * irradiance x diffuse, then the lights */
                     const yo_v3 Nv = face_forward(t.Ng, t.N, eye);
-                    light = col_add(col_mul(gm_irradiance(R, &t, Nv), get_diffuse(R, &t)), light);
+                    light = col_add(col_mul(gm_irradiance(R, &t, Nv), get_diffuse(R, &t, Nv)), light);
                 }
                 yo_col contri = col_mul(col_scale(raycolor, v3_dot(startray, N)), light);
                 total = col_add(total, contri);
                 if (first) subtotal[i % 4] = col_add(subtotal[i % 4], contri);
                 yo_v3 HN = face_forward(t.Ng, t.N, v3_neg(ray));
-                raycolor = col_mul(raycolor, get_diffuse(R, &t));
+                raycolor = col_mul(raycolor, get_diffuse(R, &t, v3_neg(ray)));
                 ray = next_direction(hs, &taken, HN, t.NU, t.NV, i, j + 1);
                 where = t.P;
             }
@@ -499,7 +568,7 @@ static yo_col path_illuminate(rctx* R, int li, const yo_sp* sp, yo_v3 eye)
     const yg_light* L = &R->lights[li];
     if (L->cache && R->lc) return (R->lc_phase == 1) ? lc_cached(R, li, sp, eye) : lc_interpolate(R, li, sp, eye);
     yo_v3 N = face_forward(sp->Ng, sp->N, eye);
-    yo_col dif = get_diffuse(R, sp);
+    yo_col dif = get_diffuse(R, sp, eye);
     if (col_energy(dif) < 0.05f) return col(0, 0, 0);
     const int prim = R->primary;
     R->primary = 0;  /* lights evaluated along the paths are not the pixel's direct terms */
@@ -1291,7 +1360,7 @@ static void gshoot(gpctx* G, yo_photon* ph, yo_v3 dir, int depth)
     float diffuse = 0.0f;
     yo_col diffcolor = col(0, 0, 0);
     if (obj_flag(R, sp.obj, 2)) {
-        diffcolor = get_diffuse(R, &sp);
+        diffcolor = get_diffuse(R, &sp, edir);
         diffuse = col_energy(diffcolor);
     }
     float sum = diffuse;  /* trans = 0 */
@@ -1714,6 +1783,8 @@ static yo_col raytrace(rctx* R, yo_v3 from, yo_v3 ray)
     yo_sp sp;
     R->stats[0]++;
     if (first_hit(R, from, ray, &sp, 0)) {
+        sp.sx = R->scr_x;  /* sp.setScreenPos(state.screenpos) (scene.cc:234) */
+        sp.sy = R->scr_y;
         R->traveled = sp.Z;  /* state.traveled += sp.Z() from 0 at the camera ray (scene.cc:239-240) */
         R->primary = 1;
         yo_col c = scene_light(R, &sp, from, 0);
@@ -2329,7 +2400,7 @@ static void lc_free(yo_lcache* C)
 static yo_col lc_cached(rctx* R, int li, const yo_sp* sp, yo_v3 eye)
 {
     const yg_light* L = &R->lights[li];
-    if (col_energy(get_diffuse(R, sp)) < 0.05f) return col(0, 0, 0);
+    if (col_energy(get_diffuse(R, sp, eye)) < 0.05f) return col(0, 0, 0);
     const yo_v3 N = face_forward(sp->Ng, sp->N, eye);
     const float desired = (float)(1.0 / (double)L->shadow_threshold);
     const float rq = (float)(1.0 / (double)(0 + 1));
@@ -2359,7 +2430,7 @@ static yo_col lc_interpolate(rctx* R, int li, const yo_sp* sp, yo_v3 eye)
 {
     const yg_light* L = &R->lights[li];
     const yo_lcache* C = R->lc;
-    const yo_col dif = get_diffuse(R, sp);
+    const yo_col dif = get_diffuse(R, sp, eye);
     if (col_energy(dif) < 0.05f) return col(0, 0, 0);
     const yo_v3 N = face_forward(sp->Ng, sp->N, eye);
     const yo_v3 pP = lc_to_polar(R);
@@ -2493,6 +2564,9 @@ static void render_area(rctx* R, const render_job* J, int ri)
                                                            ((double)yo_ri_lp(cursam + pixnum, 0) - 0.5));
                         const float fy = (float)(0.5 + (double)o->aa_pixelwidth *
                                                            ((double)((float)cursam * totsamdiv) - 0.5));
+                        /* state.screenpos of the sample (scene.cc:798-799) */
+                        R->scr_x = (float)(2.0 * (double)(((float)j + fx) / (float)resx) - 1.0);
+                        R->scr_y = (float)(1.0 - 2.0 * (double)(((float)i + fy) / (float)J->cam->resy));
                         const yo_col c =
                             tonemap(raytrace(R, J->k->pos, cam_ray(J->k, (float)j + fx, (float)i + fy)), o);
                         tot = col_add(tot, c);
@@ -2603,7 +2677,7 @@ static void lc_fake_pass(rctx* R, const render_job* J, int li)
                     if (hit) {
                         const yo_v3 N = face_forward(sp.Ng, sp.N, v3_sub(J->k->pos, sp.P));
                         const yo_v3 pP = lc_to_polar(R);
-                        o[2] = col_energy(get_diffuse(R, &sp)) < 0.05f ? 0.0f : 1.0f;
+                        o[2] = col_energy(get_diffuse(R, &sp, v3_sub(J->k->pos, sp.P))) < 0.05f ? 0.0f : 1.0f;
                         o[3] = sp.P.x; o[4] = sp.P.y; o[5] = sp.P.z;
                         o[6] = N.x; o[7] = N.y; o[8] = N.z;
                         o[9] = pP.x; o[10] = pP.y; o[11] = pP.z;
@@ -2652,6 +2726,11 @@ int yo_render_mt(yo_scene* s, const yg_camera* cam, const yg_material* mats, int
                 lights[li].type == YG_LIGHT_GLOBALPHOTON)
                 return -7;
         if (!(lights[cli].cache_size > 0) || !(lights[cli].shadow_threshold > 0) || lights[cli].search < 3) return -2;
+        /* the fake pass's screen position is formed in double, the render
+         * pass's in float: "window" textures stay off with the cache */
+        for (int m = 0; m < nmat; m++)
+            for (int k = 0; k < mats[m].mod_count; k++)
+                if (mods[mats[m].mod_first + k].texco == YG_TEXCO_WIN) return -7;
     }
     if (nthreads < 1 || sequential) nthreads = 1;
     if (nthreads > 64) nthreads = 64;
@@ -2821,6 +2900,8 @@ int yo_render_adjoint(yo_scene* s, const yg_camera* cam, const yg_material* mats
                       double* gmod, double* glight, int nthreads)
 {
     if (opts->aa_passes > 0) return -5; /* antialiasing passes are not differentiated */
+    for (int k = 0; k < nmod; k++)
+        if (mods[k].texco != YG_TEXCO_GLOBAL) return -7; /* hit-point coordinates only */
     for (int k = 0; k < ntex; k++)
         if (texs[k].type == YG_TEX_MUSGRAVE && texs[k].mtype != YG_MUSGRAVE_FBM) return -7; /* fBm only */
     for (int li = 0; li < nl; li++)

@@ -495,3 +495,106 @@ def lambert_from_points(P, pts, L, rho, weights=None):
         out += np.maximum(ndl, 0)[:, None] / d2[:, None] * col[None, :]
     plane_at = float(L.a[2]) - P[:, 2]
     return out * np.array(rho)[None, :] * plane_at[:, None] / len(pts)
+
+
+# ------------------------------------------------- texture coordinates --
+
+TEXCO_C1, TEXCO_C2 = (0.1, 0.2, 0.3), (0.9, 0.6, 0.5)
+# per texco kind: (YG_TEXCO_*, projection of the modulator)
+TEXCO_KINDS = {
+    "global": (D.YG_TEXCO_GLOBAL, (0, 0, 0)),
+    "uv": (D.YG_TEXCO_UV, (1, -1, -1)),
+    "orco_matrix": (D.YG_TEXCO_ORCO, (1, -1, 3)),
+    "orco_points": (D.YG_TEXCO_ORCO, (1, -1, 3)),
+    "window": (D.YG_TEXCO_WIN, (0, 0, 0)),
+    "normal": (D.YG_TEXCO_NOR, (0, 0, 0)),
+    "reflect": (D.YG_TEXCO_REFL, (0, 0, 0)),
+}
+
+
+def texco_wall(kind, G=8, resx=96, resy=60, aspect=1.3, focal=1.7, D_=4.0, X=3.0, Z=2.0):
+    """camera_grid's wall with one material: marble without turbulence,
+    colours TEXCO_C1 -> TEXCO_C2, blended in fully (MIX, colour 1) through one
+    modulator with texture coordinates `kind`, so that each pixel is
+    c1 + (c2 - c1) (1/2 + 1/2 sin(5 s)) with s the sum of the modulator's
+    texpt.  For "uv" a third of the cells have face uv u = (x + X) / 2X,
+    v = (z + Z) / 2Z, a third vertex colours only and a third neither; for
+    "orco_points" the vertices carry orco points (0.3 x + 0.1, 7, 0.2 z -
+    0.05)."""
+    xs = np.linspace(-X, X, G + 1)
+    zs = np.linspace(-Z, Z, G + 1)
+    verts, faces, uv, flags = [], [], [], []
+    for gz in range(G):
+        for gx in range(G):
+            b = len(verts)
+            q = [[xs[gx], D_, zs[gz]], [xs[gx + 1], D_, zs[gz]], [xs[gx + 1], D_, zs[gz + 1]], [xs[gx], D_, zs[gz + 1]]]
+            verts += q
+            faces += [[b, b + 1, b + 2], [b, b + 2, b + 3]]
+            quv = [((p[0] + X) / (2 * X), (p[2] + Z) / (2 * Z)) for p in q]
+            kind_c = (gx + gz) % 3
+            fl = {0: 0, 1: 1, 2: 2}[kind_c]
+            for tri in ((0, 1, 2), (0, 2, 3)):
+                uv.append([c for t in tri for c in quv[t]])
+                flags.append(fl)
+    verts = np.array(verts, np.float32)
+    texco, proj = TEXCO_KINDS[kind]
+    tex = [D.marble(color1=TEXCO_C1, color2=TEXCO_C2, turbulence=0.0)]
+    mods = [D.modulator(0, D.YG_MOD_MIX, color=1.0, proj=proj, texco=texco)]
+    mats = [D.material((0.4, 0.5, 0.6), 0, 1)]
+    cam = D.camera((0, 0, 0), (0, 1, 0), (0, 0, 1), resx, resy, aspect, focal)
+    texd, orco = {}, None
+    if kind == "uv":
+        texd = {0: {"uv": np.array(uv, np.float32), "flags": np.array(flags, np.uint8)}}
+    elif kind == "orco_points":
+        orco = np.stack([0.3 * verts[:, 0] + 0.1, np.full(len(verts), 7.0), 0.2 * verts[:, 2] - 0.05], 1)
+        texd = {0: {"orco": True}}
+    return D.SceneDesc(verts, np.array(faces, np.uint32), [0] * len(faces), [0] * len(faces), [7], cam, mats,
+                       mods, tex, [D.sunlight((0, -1, 0), power=1.0)], D.render_opts(bias=1e-3), tex=texd,
+                       orco=orco)
+
+
+def texco_wall_expected(kind, G=8, resx=96, resy=60, aspect=1.3, focal=1.7, D_=4.0, X=3.0, Z=2.0):
+    """The expected colour per pixel from camera_grid_expected's hit points
+    (x, D_, z) and doMapping's coordinates (texture.cc:125-165):
+      global   s = x + D_ + z
+      uv       (2u - 1 projected to x): 2u - 1 on cells with uv, -3 on cells
+               with nothing (u = -1), -1 on cells with vertex colours only
+      orco     the orco matrix maps the bound [-X, X] x [-Z, Z] (widened by
+               BOX_MIN = 1e-5) to [-1, 1]: s = x / (X + 1e-5) + z / (Z + 1e-5)
+      orco points  s = (0.3 x + 0.1) + (0.2 z - 0.05)
+      window   the pixel centre's screen position: s = (2 (j + 1/2) / resx - 1)
+               + (1 - 2 (i + 1/2) / resy)
+      normal   N = (0, -1, 0): s = -1
+      reflect  eye = -P, N faces it: reflect(N, eye/|eye|) = (x, -D_, z) / |P|
+    Returns the colours and a mask of pixels 1e-3 cell or more from every
+    cell edge."""
+    j = np.arange(resx) + 0.5
+    i = np.arange(resy) + 0.5
+    x = (D_ * (j / resx - 0.5) / focal)[None, :] * np.ones((resy, 1))
+    z = (-D_ * aspect * (resy / resx) * (i / resy - 0.5) / focal)[:, None] * np.ones((1, resx))
+    fx = (x + X) / (2 * X) * G
+    fz = (z + Z) / (2 * Z) * G
+    gx = np.floor(fx).astype(int)
+    gz = np.floor(fz).astype(int)
+    mask = ((gx >= 0) & (gx < G) & (np.abs(fx - np.round(fx)) > 1e-3) & (gz >= 0) & (gz < G) &
+            (np.abs(fz - np.round(fz)) > 1e-3))
+    if kind == "global":
+        s = x + D_ + z
+    elif kind == "uv":
+        u = (x + X) / (2 * X)
+        c = (gx + gz) % 3
+        s = np.where(c == 1, 2 * u - 1, np.where(c == 0, -3.0, -1.0))
+    elif kind == "orco_matrix":
+        s = x / (X + 1e-5) + z / (Z + 1e-5)
+    elif kind == "orco_points":
+        s = (0.3 * x + 0.1) + (0.2 * z - 0.05)
+    elif kind == "window":
+        s = (2 * j / resx - 1)[None, :] + (1 - 2 * i / resy)[:, None]
+    elif kind == "normal":
+        s = np.full((resy, resx), -1.0)
+    else:
+        r = np.sqrt(x * x + D_ * D_ + z * z)
+        s = (x - D_ + z) / r
+    f = 0.5 + 0.5 * np.sin(5 * s)
+    c1, c2 = np.array(TEXCO_C1), np.array(TEXCO_C2)
+    return c1 + (c2 - c1) * f[..., None], mask

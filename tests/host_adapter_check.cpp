@@ -1,6 +1,7 @@
 // Drives the C++ host adapter (factories by plugin name) without a GPU and
 // prints the descriptors it would hand to libyafgpu.so, one per line.
 #include <cstdio>
+#include <cstring>
 
 #include "yafgpu_host.h"
 
@@ -120,6 +121,16 @@ int main()
         out.clear();
         const int rc3 = yafgpu::shaderFactory("generic", sh2, {m3}, tix, M, out);
         printf("moddef %d %d %d %d %d\n", rc3, out[0].proj[0], out[0].proj[1], out[0].proj[2], out[0].has_mtx);
+        // texco names (string2texcotype, texture.h:135-149; factory default "orco")
+        const char* names[] = {"global", "orco", "uv", "window", "normal", "reflect", "sticky", "-"};
+        for (const char* nm : names) {
+            paramMap_t m4;
+            m4.setString("texname", "mus");
+            if (strcmp(nm, "-") != 0) m4.setString("texco", nm);
+            out.clear();
+            const int rc4 = yafgpu::shaderFactory("generic", sh2, {m4}, tix, M, out);
+            printf("texco %s %d %d\n", nm, rc4, rc4 ? -1 : out[0].texco);
+        }
     }
     // unsupported features are refused with a status, not an exit
     paramMap_t cache;

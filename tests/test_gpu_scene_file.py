@@ -81,3 +81,149 @@ def test_cli_autosmooth_equals_api_render(torch_cuda, tmp_path):
     assert np.array_equal(img, rgbe(smooth))
     d.smooth = {}
     assert not np.array_equal(img, rgbe(gpu_render(d, torch_cuda)))
+
+
+
+def _g(x):
+    """A float32 value written so that it reads back exactly."""
+    return "%.9g" % np.float32(x)
+
+
+TEXCO_XML_T = np.array([[0.9848077, -0.17364818, 0.0, 0.25], [0.17364818, 0.9848077, 0.0, 0.5],
+                        [0.0, 0.0, 1.0, -0.125], [0.0, 0.0, 0.0, 1.0]], np.float32)
+
+
+def texco_geometry():
+    """Float32 geometry of the texture-coordinate scene: a 3 x 3-cell wall
+    (cells with face uv u = (x + 3) / 6, v = (z + 2) / 4, with vertex colours
+    only, or with neither, by (gx + gz) mod 3), a cube, and the cube moved
+    with an orco point after each vertex."""
+    X, Z, D_, G = 3.0, 2.0, 4.0, 3
+    xs = np.linspace(-X, X, G + 1).astype(np.float32)
+    zs = np.linspace(-Z, Z, G + 1).astype(np.float32)
+    pts, wf, uv, flags = [], [], [], []
+    for gz in range(G):
+        for gx in range(G):
+            b = len(pts)
+            q = [[xs[gx], D_, zs[gz]], [xs[gx + 1], D_, zs[gz]], [xs[gx + 1], D_, zs[gz + 1]],
+                 [xs[gx], D_, zs[gz + 1]]]
+            pts += q
+            uvq = [(np.float32((p[0] + X) / (2 * X)), np.float32((p[2] + Z) / (2 * Z))) for p in q]
+            c = (gx + gz) % 3
+            for tri in ((0, 1, 2), (0, 2, 3)):
+                wf.append([b + t for t in tri])
+                uv.append([x for t in tri for x in uvq[t]] if c == 1 else [0.0] * 6)
+                flags.append(c)
+    cv, cf = S.cube()
+    cv = (cv * 0.8 + np.array([1.2, 2.5, -0.6])).astype(np.float32)
+    ov = (cv + np.array([-2.6, 0.2, 1.0])).astype(np.float32)
+    inter = np.empty((2 * len(ov), 3), np.float32)
+    inter[0::2] = ov
+    inter[1::2] = ov * np.float32([0.5, 0.25, -1.0]) + np.float32(0.3)
+    return dict(wall=np.array(pts, np.float32), wf=np.array(wf, np.uint32), uv=np.array(uv, np.float32),
+                flags=np.array(flags, np.uint8), box=cv, bf=cf.astype(np.uint32), inter=inter)
+
+
+def _pts_xml(p):
+    return "".join('<p x="%s" y="%s" z="%s"/>' % tuple(_g(c) for c in q) for q in p)
+
+
+def texco_scene_xml(g, outfile, opts):
+    """The scene file: the wall and the cube inside a transform, the orco
+    mesh (has_orco) outside it, a sphere (which the loader makes nothing
+    of), modulators with texco uv, the factory default (orco, here from the
+    cube's transform) and orco."""
+    faces = []
+    for t, (f, fl) in enumerate(zip(g["wf"], g["flags"])):
+        attrs = ""
+        if fl == 1:
+            attrs = " ".join('%s_%s="%s"' % (c, k, _g(g["uv"][t][2 * i + ci]))
+                             for i, k in enumerate("abc") for ci, c in enumerate("uv"))
+        elif fl == 2:
+            attrs = 'vcol_a_r="1" vcol_a_g="0.5" vcol_a_b="0.25"'
+        faces.append('<f a="%d" b="%d" c="%d" %s/>' % (f[0], f[1], f[2], attrs))
+    wall = ('<object name="A_wall" shader_name="Mwall"><attributes></attributes><mesh autosmooth="0"><points>%s'
+            '</points><faces>%s</faces></mesh></object>' % (_pts_xml(g["wall"]), "".join(faces)))
+    box = ('<object name="B_box" shader_name="Mbox"><attributes></attributes><mesh><points>%s</points><faces>%s'
+           '</faces></mesh></object>' % (_pts_xml(g["box"]),
+                                         "".join('<f a="%d" b="%d" c="%d"/>' % tuple(f) for f in g["bf"])))
+    orco = ('<object name="C_orco" shader_name="Mob"><attributes></attributes><mesh has_orco="on"><points>%s'
+            '</points><faces>%s</faces></mesh></object>'
+            % (_pts_xml(g["inter"]), "".join('<f a="%d" b="%d" c="%d"/>' % tuple(2 * f) for f in g["bf"])))
+    sphere = ('<object name="D_sphere" shader_name="Mwall"><attributes></attributes><sphere radius="1">'
+              '<center x="0" y="3" z="0"/></sphere></object>')
+    T = " ".join('m%d%d="%s"' % (r, c, _g(TEXCO_XML_T[r][c])) for r in range(4) for c in range(4))
+    col = lambda n, c: '<%s r="%s" g="%s" b="%s"/>' % (n, _g(c[0]), _g(c[1]), _g(c[2]))
+    attrs = dict(raydepth=opts.raydepth, bias=_g(opts.self_bias), AA_passes=opts.aa_passes,
+                 AA_minsamples=opts.aa_minsamples, AA_pixelwidth=_g(opts.aa_pixelwidth),
+                 AA_threshold=_g(opts.aa_threshold), AA_jitterfirst="off", exposure=_g(opts.exposure),
+                 gamma=_g(opts.gamma), clamp_rgb="off")
+    return "\n".join([
+        '<?xml version="1.0"?>', "<scene>",
+        '<texture name="mt" type="marble" turbulence="0">%s%s</texture>' % (col("color1", (0.1, 0.2, 0.3)),
+                                                                          col("color2", (0.9, 0.6, 0.5))),
+        '<shader type="generic" name="Mbox"><attributes>%s</attributes>'
+        '<modulator texname="mt" mode="mix" color="0.8" size="0.5"/></shader>' % col("color", (0.6, 0.5, 0.4)),
+        '<shader type="generic" name="Mob"><attributes>%s</attributes>'
+        '<modulator texname="mt" mode="mix" color="0.9" texco="orco"/></shader>' % col("color", (0.5, 0.6, 0.5)),
+        '<shader type="generic" name="Mwall"><attributes>%s</attributes>'
+        '<modulator texname="mt" mode="mix" color="1" texco="uv" proj_y="n" proj_z="n"/></shader>'
+        % col("color", (0.4, 0.5, 0.6)),
+        '<transform %s>%s%s</transform>' % (T, wall, box), orco, sphere,
+        '<light type="sunlight" name="sun" power="1" cast_shadows="on"><from x="0" y="-1" z="0"/>%s</light>'
+        % col("color", (1, 1, 1)),
+        '<camera name="cam" resx="96" resy="60" focal="1.7" aspect_ratio="1.3"><from x="0" y="0" z="0"/>'
+        '<to x="0" y="1" z="0"/><up x="0" y="0" z="1"/></camera>',
+        '<render camera_name="cam" %s><outfile value="%s"/><output_type value="hdr"/></render>'
+        % (" ".join('%s="%s"' % kv for kv in attrs.items()), outfile),
+        "</scene>"]) + "\n"
+
+
+def texco_scene_desc(g, opts):
+    """The same scene through the Python API: world vertices M p in float
+    (meshObject_t::transform), the transformed objects' orco matrices from
+    yg_orco_matrix."""
+    from yafray_amd import lib as Y
+    M = TEXCO_XML_T
+    def world(p):
+        out = np.empty_like(p)
+        for r in range(3):
+            out[:, r] = ((M[r, 0] * p[:, 0] + M[r, 1] * p[:, 1]) + M[r, 2] * p[:, 2]) + M[r, 3]
+        return out
+    wall, wf, box, bf, inter = g["wall"], g["wf"], g["box"], g["bf"], g["inter"]
+    nw, nb = len(wall), len(box)
+    verts = np.concatenate([world(wall), world(box), inter])
+    idx = np.concatenate([wf, bf + nw, 2 * bf + nw + nb]).astype(np.uint32)
+    tri_obj = np.repeat(np.arange(3), [len(wf), len(bf), len(bf)]).astype(np.int32)
+    # materials in shader-name order: Mbox 0, Mob 1, Mwall 2
+    tri_mat = np.repeat([2, 0, 1], [len(wf), len(bf), len(bf)]).astype(np.int32)
+    orco = np.zeros_like(verts)
+    orco[nw + nb:nw + nb + len(inter) - 1] = inter[1:]
+    texd = {0: {"uv": g["uv"], "flags": g["flags"], "mtx": Y.orco_matrix(wall, wf, M)},
+            1: {"mtx": Y.orco_matrix(box, bf, M)},
+            2: {"orco": True}}
+    tex = [D.marble(color1=(0.1, 0.2, 0.3), color2=(0.9, 0.6, 0.5), turbulence=0.0)]
+    mods = [D.modulator(0, D.YG_MOD_MIX, color=0.8, size=0.5, texco="orco"),
+            D.modulator(0, D.YG_MOD_MIX, color=0.9, texco="orco"),
+            D.modulator(0, D.YG_MOD_MIX, color=1.0, proj=(0, -1, -1), texco="uv")]
+    mats = [D.material((0.6, 0.5, 0.4), 0, 1), D.material((0.5, 0.6, 0.5), 1, 1), D.material((0.4, 0.5, 0.6), 2, 1)]
+    cam = D.camera((0, 0, 0), (0, 1, 0), (0, 0, 1), 96, 60, 1.3, 1.7)
+    return D.SceneDesc(verts, idx, tri_obj, tri_mat, [7, 7, 7], cam, mats, mods, tex,
+                       [D.sunlight((0, -1, 0), power=1.0)], opts, tex=texd, orco=orco)
+
+
+def test_cli_texture_coordinates_equal_api_render(torch_cuda, tmp_path):
+    """Face uv and vertex colours, has_orco points, a transform (float M p,
+    orco matrix), a sphere object and modulator texco names read from the
+    scene file: the RGBE output equals the API render of the same scene,
+    which equals the oracle's."""
+    g = texco_geometry()
+    opts = D.render_opts(bias=1e-3, raydepth=3)
+    cli(tmp_path, texco_scene_xml(g, "out.hdr", opts))
+    _, img = read_hdr(str(tmp_path / "out.hdr"))
+    d = texco_scene_desc(g, opts)
+    api = gpu_render(d, torch_cuda)
+    assert np.array_equal(img, rgbe(api))
+    ref, _ = po.OracleScene.from_desc(d).render(d)
+    assert np.mean(np.all(api == ref, axis=2)) > 0.999
+    assert np.abs(api - ref).max() < 1e-5

@@ -70,6 +70,16 @@ def test_cache_parameters(adapter_output):
 D_ERR_UNSUPPORTED = -4
 
 
+def test_texco_names(adapter_output):
+    """The modulator's texco name maps as modulator_t::string2texcotype
+    (texture.h:135-149): unknown names are uv; without the parameter the
+    factory's default "orco" (basicshaders.cc:270)."""
+    got = {l[1]: (int(l[2]), int(l[3])) for l in adapter_output if l[0] == "texco"}
+    assert got == {"global": (0, D.YG_TEXCO_GLOBAL), "orco": (0, D.YG_TEXCO_ORCO), "uv": (0, D.YG_TEXCO_UV),
+                   "window": (0, D.YG_TEXCO_WIN), "normal": (0, D.YG_TEXCO_NOR), "reflect": (0, D.YG_TEXCO_REFL),
+                   "sticky": (0, D.YG_TEXCO_UV), "-": (0, D.YG_TEXCO_ORCO)}
+
+
 def test_render_options_match_python_spec(adapter_output):
     rows = [l for l in adapter_output if l[0] == "ropts"]
     specs = [D.render_opts(),

@@ -91,9 +91,11 @@ def test_unsupported_object_settings_are_refused(checker, tmp_path):
     """Object settings that change the reference's image and that the GPU
     path does not implement make the load fail with YG_ERR_UNSUPPORTED
     instead of being dropped: caustic settings (msin_object.cc:448-476 turn
-    on obj->caustics), face uv / vertex colours and has_orco (uv tangents,
-    mesh.cc:284-340).  autosmooth below 1 degree is a no-op in the reference
-    (mesh.cc:204) and loads."""
+    on obj->caustics).  Face uv, vertex colours and has_orco load (their
+    texture coordinates are read by modulators), and so does a sphere object,
+    which the reference's loader makes nothing of (render.cc:768-776).
+    autosmooth below 1 degree is a no-op in the reference (mesh.cc:204) and
+    loads."""
     d = S.terrain_gi(po.heightfield, nx=8, ny=6, resx=16, resy=12, samples=1, depth=1, nmat=2)
     base = scene_xml(d, "t.tga")
     assert '<mesh autosmooth="0">' in base and "<f a=" in base
@@ -103,9 +105,6 @@ def test_unsupported_object_settings_are_refused(checker, tmp_path):
                                     '<attributes><caus_rcolor r="1" g="1" b="1"/></attributes><mesh', 1),
         "caus_tcolor": base.replace("<attributes></attributes><mesh",
                                     '<attributes><caus_tcolor r="1" g="1" b="1"/></attributes><mesh', 1),
-        "uv": base.replace("<f a=", '<f u_a="0" v_a="0" u_b="1" v_b="0" u_c="0" v_c="1" a=', 1),
-        "vcol": base.replace("<f a=", '<f vcol_a_r="1" vcol_a_g="1" vcol_a_b="1" a=', 1),
-        "orco": base.replace('<mesh autosmooth="0">', '<mesh autosmooth="0" has_orco="on">', 1),
     }
     for name, text in cases.items():
         assert text != base, name
@@ -113,6 +112,20 @@ def test_unsupported_object_settings_are_refused(checker, tmp_path):
         path.write_text(text)
         out = run(checker, "describe", str(path))
         assert out[0] == ["load", str(D_ERR_UNSUPPORTED)], name
+    loads = {
+        "uv": base.replace("<f a=", '<f u_a="0" v_a="0" u_b="1" v_b="0" u_c="0" v_c="1" a=', 1),
+        "vcol": base.replace("<f a=", '<f vcol_a_r="1" vcol_a_g="1" vcol_a_b="1" a=', 1),
+        "orco": base.replace('<mesh autosmooth="0">', '<mesh autosmooth="0" has_orco="on">', 1),
+        "sphere": base.replace('<object name="O0000"',
+                               '<object name="A_sphere" shader_name="M000"><attributes></attributes>'
+                               '<sphere radius="1"><center x="0" y="0" z="0"/></sphere></object>'
+                               '<object name="O0000"', 1),
+    }
+    for name, text in loads.items():
+        assert text != base, name
+        path = tmp_path / ("%s.xml" % name)
+        path.write_text(text)
+        assert run(checker, "describe", str(path))[0] == ["load", "0"], name
     for angle in ("0", "0.5"):
         path = tmp_path / "smooth.xml"
         path.write_text(base.replace('<mesh autosmooth="0">', '<mesh autosmooth="%s">' % angle))

@@ -49,12 +49,13 @@ extern "C" int yg_scene_create(yg_scene** out)
 
 static void free_dev(yg_scene* s)
 {
-    void* ptrs[] = {s->d_nodes, s->d_nodes4, s->d_nodes8, s->d_tvn, s->d_tris, s->d_tprim, s->d_tobj, s->d_tmat, s->d_obj_flags,
+    void* ptrs[] = {s->d_nodes, s->d_nodes4, s->d_nodes8, s->d_tvn, s->d_ptex, s->d_oorco, s->d_tris, s->d_tprim, s->d_tobj, s->d_tmat, s->d_obj_flags,
                     s->d_mats, s->d_mods, s->d_texs, s->pm.d_photons, s->pm.d_gstart,
                     s->pm.d_gitems, s->d_halton, s->d_any_ovf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     s->d_nodes = nullptr; s->d_nodes4 = nullptr; s->d_nodes8 = nullptr; s->d_tvn = nullptr; s->d_tris = nullptr;
+    s->d_ptex = nullptr; s->d_oorco = nullptr;
     s->d_tprim = nullptr; s->d_tobj = nullptr; s->d_tmat = nullptr; s->d_obj_flags = nullptr; s->d_mats = nullptr;
     s->d_mods = nullptr;
     s->d_texs = nullptr; s->pm = PhotonMapDev();
@@ -93,10 +94,15 @@ extern "C" int yg_mesh_add(yg_scene* s, const float* xyz, uint32_t nv, const uin
     if (mat) s->tri_mat.insert(s->tri_mat.end(), mat, mat + nt);
     else s->tri_mat.insert(s->tri_mat.end(), nt, 0);
     s->obj_flags.push_back(flags);
+    s->obj_vbase.push_back(vbase);
+    s->obj_nv.push_back(nv);
+    s->obj_orco.insert(s->obj_orco.end(), 12, 0.0f);
+    s->obj_orco_set.push_back(0);
     if (!s->tri_smooth.empty()) {
         s->tri_smooth.resize(s->tri_mat.size(), 0);
         s->tri_vn.resize(9 * s->tri_mat.size(), 0.0f);
     }
+    if (!s->tri_tex.empty()) s->tri_tex.resize(16 * s->tri_mat.size(), 0.0f);
     s->built = false;
     s->gen++;
     s->geo_gen++;
@@ -171,6 +177,177 @@ extern "C" int yg_mesh_autosmooth(yg_scene* s, int32_t obj_id, float angle)
     return YG_OK;
 }
 
+// ------------------------------------------------- texture coordinates --
+
+// matrix4x4_t::inverse (matrix4.cc:78-112): Gauss-Jordan elimination with
+// partial pivoting on the rows, in float, the identity receiving the same row
+// operations.  False for a singular matrix (where the reference exits).
+static bool inverse4(float m[4][4])
+{
+    float id[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+    for (int i = 0; i < 4; ++i) {
+        float mx = 0;
+        int ci = 0;
+        for (int k = i; k < 4; ++k)
+            if (std::fabs(m[k][i]) > mx) {
+                mx = std::fabs(m[k][i]);
+                ci = k;
+            }
+        if (mx == 0) return false;
+        for (int j = 0; j < 4; ++j) {
+            std::swap(m[i][j], m[ci][j]);
+            std::swap(id[i][j], id[ci][j]);
+        }
+        float f = m[i][i];
+        for (int j = 0; j < 4; ++j) m[i][j] /= f;
+        for (int j = 0; j < 4; ++j) id[i][j] /= f;
+        for (int k = 0; k < 4; ++k) {
+            if (k == i) continue;
+            f = m[k][i];
+            for (int j = 0; j < 4; ++j) {
+                const float t = m[i][j] * f;
+                m[k][j] -= t;
+            }
+            for (int j = 0; j < 4; ++j) {
+                const float t = id[i][j] * f;
+                id[k][j] -= t;
+            }
+        }
+    }
+    memcpy(m, id, sizeof(id));
+    return true;
+}
+
+// meshObject_t::recalcBound (mesh.cc:343-380): the corners of the triangles,
+// widened by BOX_MIN (1e-5, a double, so each bound is rounded once)
+static void tri_bound(const float* xyz, const uint32_t* idx, uint32_t nt, float lo[3], float hi[3])
+{
+    for (int a = 0; a < 3; a++) lo[a] = hi[a] = xyz[3 * (size_t)idx[0] + a];
+    for (uint64_t k = 0; k < 3ull * nt; k++)
+        for (int a = 0; a < 3; a++) {
+            const float v = xyz[3 * (size_t)idx[k] + a];
+            if (v > hi[a]) hi[a] = v;
+            if (v < lo[a]) lo[a] = v;
+        }
+    for (int a = 0; a < 3; a++) {
+        lo[a] = (float)((double)lo[a] - 0.00001);
+        hi[a] = (float)((double)hi[a] + 0.00001);
+    }
+}
+
+extern "C" int yg_orco_matrix(const float* m16, const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                              float* out12)
+{
+    if (!xyz || !idx || !out12 || nt == 0) return yg_fail(YG_ERR_ARG, "null or empty mesh");
+    for (uint64_t i = 0; i < 3ull * nt; i++)
+        if (idx[i] >= nv) return yg_fail(YG_ERR_ARG, "vertex index out of range");
+    float M[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+    if (m16) memcpy(M, m16, sizeof(M));
+    // backOrco = M scaled by half the untransformed bound's extent (columns 0..2)
+    float lo[3], hi[3];
+    tri_bound(xyz, idx, nt, lo, hi);
+    float B[4][4];
+    memcpy(B, M, sizeof(B));
+    for (int c = 0; c < 3; c++) {
+        const float sc = (float)(0.5 * (double)(hi[c] - lo[c]));
+        for (int r = 0; r < 3; r++) B[r][c] *= sc;
+    }
+    // the transformed vertices (matrix4x4_t * point3d_t, matrix4.h:86-91)
+    std::vector<float> w(3 * (size_t)nv);
+    for (uint32_t v = 0; v < nv; v++) {
+        const float x = xyz[3 * (size_t)v], y = xyz[3 * (size_t)v + 1], z = xyz[3 * (size_t)v + 2];
+        for (int r = 0; r < 3; r++) {
+            float acc = M[r][0] * x;
+            const float ty = M[r][1] * y;
+            acc += ty;
+            const float tz = M[r][2] * z;
+            acc += tz;
+            acc += M[r][3];
+            w[3 * (size_t)v + r] = acc;
+        }
+    }
+    // translation = centre of the transformed bound, then the inverse
+    tri_bound(w.data(), idx, nt, lo, hi);
+    for (int r = 0; r < 3; r++) {
+        const float sum = lo[r] + hi[r];
+        B[r][3] = 0.5f * sum;
+    }
+    if (!inverse4(B)) return yg_fail(YG_ERR_ARG, "singular orco matrix");
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 4; c++) out12[4 * r + c] = B[r][c];
+    return YG_OK;
+}
+
+extern "C" int yg_mesh_texcoords(yg_scene* s, int32_t obj_id, const float* face_uv, const uint8_t* face_flags,
+                                 const float* orco, const float* orco_mtx)
+{
+    if (!s) return yg_fail(YG_ERR_ARG, "null scene");
+    if (obj_id < 0 || obj_id >= s->nobj) return yg_fail(YG_ERR_ARG, "object id out of range");
+    const size_t nt = s->tri_mat.size();
+    size_t t0 = 0;
+    while (t0 < nt && s->tri_obj[t0] != obj_id) t0++;
+    size_t t1 = t0;
+    while (t1 < nt && s->tri_obj[t1] == obj_id) t1++;
+    if (orco_mtx) {
+        memcpy(&s->obj_orco[12 * (size_t)obj_id], orco_mtx, 12 * sizeof(float));
+        s->obj_orco_set[obj_id] = 1;
+    }
+    if ((face_uv || face_flags || orco) && t1 > t0) {
+        if (s->tri_tex.empty()) s->tri_tex.assign(16 * nt, 0.0f);
+        const uint32_t vb = s->obj_vbase[obj_id];
+        for (size_t t = t0; t < t1; t++) {
+            float* r = &s->tri_tex[16 * t];
+            const size_t k = t - t0;
+            int fl = face_flags ? (face_flags[k] & 3) : (face_uv ? 1 : 0);
+            if (face_uv && (fl & 1))
+                for (int c = 0; c < 6; c++) r[c] = face_uv[6 * k + c];
+            else
+                for (int c = 0; c < 6; c++) r[c] = 0.0f;
+            if (!face_uv) fl &= ~1;
+            if (orco) {
+                for (int c = 0; c < 3; c++) {
+                    const size_t v = s->idx[3 * t + c] - vb;
+                    for (int a = 0; a < 3; a++) r[6 + 3 * c + a] = orco[3 * v + a];
+                }
+                fl |= 4;
+            }
+            r[15] = (float)fl;
+        }
+    }
+    s->built = false;
+    s->gen++;
+    s->geo_gen++;
+    return YG_OK;
+}
+
+// Device copies of the texture-coordinate data; the orco matrix of every
+// object the caller did not give one is its identity-transform matrix.
+static int upload_texdata(yg_scene* s)
+{
+    int rc;
+    const size_t nt = s->tri_mat.size();
+    for (int32_t o = 0; o < s->nobj; o++) {
+        if (s->obj_orco_set[o]) continue;
+        size_t t0 = 0;
+        while (t0 < nt && s->tri_obj[t0] != o) t0++;
+        size_t t1 = t0;
+        while (t1 < nt && s->tri_obj[t1] == o) t1++;
+        float* out = &s->obj_orco[12 * (size_t)o];
+        if (t1 == t0) continue;
+        // BOX_MIN keeps the scaled matrix invertible for any mesh
+        if ((rc = yg_orco_matrix(nullptr, s->verts.data(), (uint32_t)(s->verts.size() / 3), &s->idx[3 * t0],
+                                 (uint32_t)(t1 - t0), out)))
+            return rc;
+    }
+    if (s->nobj > 0 && (rc = dev_upload(&s->d_oorco, s->obj_orco.data(), s->obj_orco.size(), &s->dev_bytes)))
+        return rc;
+    if (!s->tri_tex.empty())
+        if ((rc = dev_upload(&s->d_ptex, reinterpret_cast<const float4*>(s->tri_tex.data()), s->tri_tex.size() / 4,
+                             &s->dev_bytes)))
+            return rc;
+    return YG_OK;
+}
+
 extern "C" int yg_build(yg_scene* s, const yg_build_opts* opts)
 {
     if (!s) return yg_fail(YG_ERR_ARG, "null scene");
@@ -194,9 +371,10 @@ extern "C" int yg_build(yg_scene* s, const yg_build_opts* opts)
     // the device copy replaces the previous one
     auto free_old = [s]() {
         void* old[] = {s->d_nodes, s->d_nodes4, s->d_nodes8, s->d_tvn, s->d_tris, s->d_tprim, s->d_tobj,
-                       s->d_tmat, s->d_obj_flags};
+                       s->d_tmat, s->d_obj_flags, s->d_ptex, s->d_oorco};
         for (void* p : old)
             if (p) (void)hipFree(p);
+        s->d_ptex = nullptr; s->d_oorco = nullptr;
         s->d_nodes = nullptr; s->d_nodes4 = nullptr; s->d_nodes8 = nullptr; s->d_tvn = nullptr;
         s->d_tris = nullptr; s->d_tprim = nullptr; s->d_tobj = nullptr; s->d_tmat = nullptr;
         s->d_obj_flags = nullptr;
@@ -218,6 +396,7 @@ extern "C" int yg_build(yg_scene* s, const yg_build_opts* opts)
             return yg_fail(YG_ERR_STATE, "BVH deeper than the traversal stack");
         }
         if ((rc = dev_upload(&s->d_obj_flags, s->obj_flags.data(), s->obj_flags.size(), &s->dev_bytes))) return rc;
+        if ((rc = upload_texdata(s))) return rc;
         s->n_nodes = di.n_nodes;
         s->max_depth = di.max_depth;
         s->width = width;
@@ -243,6 +422,7 @@ extern "C" int yg_build(yg_scene* s, const yg_build_opts* opts)
     if ((rc = dev_upload(&s->d_tobj, h.tobj.data(), h.tobj.size(), &s->dev_bytes))) return rc;
     if ((rc = dev_upload(&s->d_tmat, h.tmat.data(), h.tmat.size(), &s->dev_bytes))) return rc;
     if ((rc = dev_upload(&s->d_obj_flags, s->obj_flags.data(), s->obj_flags.size(), &s->dev_bytes))) return rc;
+    if ((rc = upload_texdata(s))) return rc;
     if (!s->tri_smooth.empty()) {
         // vertex normals in leaf order, with a has-normals flag per triangle
         std::vector<float4> tvn(3 * (size_t)nt);
@@ -314,6 +494,8 @@ extern "C" int yg_set_shading(yg_scene* s, const yg_material* mats, uint32_t nma
             return yg_fail(YG_ERR_ARG, "modulator texture index out of bounds");
         for (int a = 0; a < 3; a++)
             if (mods[k].proj[a] < -1 || mods[k].proj[a] > 3) return yg_fail(YG_ERR_ARG, "modulator projection out of range");
+        if (mods[k].texco < YG_TEXCO_GLOBAL || mods[k].texco > YG_TEXCO_REFL)
+            return yg_fail(YG_ERR_ARG, "modulator texco out of range");
     }
     for (uint32_t k = 0; k < ntex; k++) {
         if (texs[k].noise < YG_NOISE_NEWPERLIN || texs[k].noise > YG_NOISE_BLENDER)

@@ -183,7 +183,15 @@ __global__ __launch_bounds__(GB) void k_gp_trace(const GpEmit* __restrict__ em, 
             valid[slot] = 1;
         }
         if (!(fl & 4)) break;
-        c3 dif = get_diffuse(mats, mods, texs, bvh.tmat[ti], pos);
+        // getDiffuse(nullstate, sp, edir) (globalphotonlight.cc:54,93)
+        Surf hs;
+        hs.P = pos;
+        hs.N = Nsh;
+        hs.Ng = Ng;
+        hs.ti = ti;
+        hs.obj = bvh.tobj[ti];
+        hs.mat = bvh.tmat[ti];
+        c3 dif = get_diffuse(bvh, mats, mods, texs, hs, edir);
         float diffuse = energy(dif);
         float sum = diffuse;
         if (sum > 0.0f) sum = (float)(1.0 / (double)sum);

@@ -105,7 +105,7 @@ struct VisitView {
     f3* V;
     f3* pP;     // the fake pass's polar position
     float* Z;
-    int32_t* mat;
+    c3* dif;    // getDiffuse(state, sp, eye) at the camera hit (pathlight.cc:536,582)
 };
 
 __global__ __launch_bounds__(RB) void k_cache_prep(const RenderScene* __restrict__ scenes,
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(RB) void k_cache_prep(const RenderScene* __restrict
     vv.V[item] = sp.NV;
     vv.pP[item] = lc_polar(i, j, R.resx, R.resy, cp.ycorr, Z, true);
     vv.Z[item] = Z;
-    vv.mat[item] = sp.mat;
+    vv.dif[item] = get_diffuse(R, sp, eye);
 }
 
 // Fake pass of the areas of one level: one wave per area.  Candidates of a
@@ -618,8 +618,7 @@ __global__ __launch_bounds__(LC_WAVE) void k_cache_interp(const RenderScene* __r
             if (amount != 0) {
                 amount = (float)(1.0 / (double)amount);
                 total = cscale(total, amount);
-                const c3 dif = get_diffuse(R.mats, R.mods, R.texs, vv.mat[item], P);
-                out = cscale(cmul(dif, total), R.path_power);
+                out = cscale(cmul(vv.dif[item], total), R.path_power);
             } else
                 atomicAdd(&cv.err[1], 1ull);
             pathc[item] = out;

@@ -369,17 +369,26 @@ __device__ __forceinline__ c3 tex_color(const yg_texture& t, f3 p, const unsigne
     return cadd(c1, cscale(csub(c2, c1), f));
 }
 
-// modulator_t::doMapping for a procedural texture (texture.cc:122-180 and
-// 311-317): the texture matrix (matrix4x4_t * point3d_t, matrix4.h:86-91),
-// then per axis size * (T[projected axis] + ofs), or size * ofs for "n"
-__device__ __forceinline__ f3 mod_texpt(const yg_modulator& m, f3 P)
+// modulator_t::doMapping for a procedural texture (texture.cc:167-180 and
+// 311-317) from the texture coordinates P of the modulator's texco: the
+// texture matrix (matrix4x4_t * point3d_t, matrix4.h:86-91; for a direction,
+// texco "normal" or "reflect", matrix4x4_t * vector3d_t without the
+// translation, matrix4.h:79-84), then per axis size * (T[projected axis] +
+// ofs), or size * ofs for "n"
+__device__ __forceinline__ f3 mod_texpt(const yg_modulator& m, f3 P, bool vec = false)
 {
     float T[3] = {P.x, P.y, P.z};
     if (m.has_mtx) {
         const float* M = m.mtx;
-        T[0] = M[0] * P.x + M[1] * P.y + M[2] * P.z + M[3];
-        T[1] = M[4] * P.x + M[5] * P.y + M[6] * P.z + M[7];
-        T[2] = M[8] * P.x + M[9] * P.y + M[10] * P.z + M[11];
+        if (vec) {
+            T[0] = M[0] * P.x + M[1] * P.y + M[2] * P.z;
+            T[1] = M[4] * P.x + M[5] * P.y + M[6] * P.z;
+            T[2] = M[8] * P.x + M[9] * P.y + M[10] * P.z;
+        } else {
+            T[0] = M[0] * P.x + M[1] * P.y + M[2] * P.z + M[3];
+            T[1] = M[4] * P.x + M[5] * P.y + M[6] * P.z + M[7];
+            T[2] = M[8] * P.x + M[9] * P.y + M[10] * P.z + M[11];
+        }
     }
     float o[3];
     for (int a = 0; a < 3; a++) {
@@ -389,11 +398,12 @@ __device__ __forceinline__ f3 mod_texpt(const yg_modulator& m, f3 P)
     return mk3(o[0], o[1], o[2]);
 }
 
-// modulator_t::modulate colour part (texture.cc:499-534)
-__device__ __forceinline__ c3 modulate(const yg_modulator& m, const yg_texture* texs, c3 C, f3 P)
+// modulator_t::modulate colour part (texture.cc:499-534) at texture
+// coordinates P (vec: a direction)
+__device__ __forceinline__ c3 modulate(const yg_modulator& m, const yg_texture* texs, c3 C, f3 P, bool vec = false)
 {
     const yg_texture& t = texs[m.tex];
-    const f3 tp = mod_texpt(m, P);
+    const f3 tp = mod_texpt(m, P, vec);
     c3 tc = tex_color(t, tp);
     switch (m.mode) {
     case YG_MOD_MIX:
@@ -568,12 +578,3 @@ __device__ __forceinline__ RhoD diffuse_d(const yg_material* mats, const yg_modu
     return r;
 }
 
-// genericShader_t::getDiffuse (basicshaders.cc:191-204)
-__device__ __forceinline__ c3 get_diffuse(const yg_material* mats, const yg_modulator* mods,
-                                          const yg_texture* texs, int mat, f3 P)
-{
-    const yg_material& m = mats[mat];
-    c3 c = mkc(m.color[0], m.color[1], m.color[2]);
-    for (int k = 0; k < m.mod_count; k++) c = modulate(mods[m.mod_first + k], texs, c, P);
-    return c;
-}

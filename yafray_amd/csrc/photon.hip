@@ -119,7 +119,17 @@ __global__ __launch_bounds__(PB) void k_photon_seg(PhotonSrc src, BVHView bvh, c
         // fromRadiosity (basicshaders.cc:30-44) at the previous hit
         const f3 prevP = s.pos;
         if (dot(face_forward(nrm, nsh, dir), s.edir) < 0) col = mkc(0, 0, 0);
-        else col = cmul(s.col, get_diffuse(mats, mods, texs, bvh.tmat[s.ti], prevP));
+        else {
+            // the modulators see eye = refDir (photonlight.cc:237)
+            Surf hs;
+            hs.P = prevP;
+            hs.N = nsh;
+            hs.Ng = nrm;
+            hs.ti = s.ti;
+            hs.obj = bvh.tobj[s.ti];
+            hs.mat = bvh.tmat[s.ti];
+            col = cmul(s.col, get_diffuse(bvh, mats, mods, texs, hs, dir));
+        }
         pos = s.pos;
         lastpos = s.lastpos;
     }

@@ -155,6 +155,9 @@ __device__ __forceinline__ Surf primary_surf(const RenderScene& R, const Area& a
     const f3 f = R.cam_pos + ray * R.min_raydis;
     Surf sp = make_surf(R.bvh, ti, f, ray, t);
     eye = R.cam_pos - sp.P;
+    // state.screenpos (scene.cc:749-750)
+    sp.sx = (float)(2.0 * (double)(px / (float)R.resx) - 1.0);
+    sp.sy = (float)(1.0 - 2.0 * (double)(py / (float)R.resy));
     return sp;
 }
 
@@ -286,7 +289,7 @@ __global__ __launch_bounds__(RB) void k_prep(const RenderScene* __restrict__ sce
             }
         }
         if (R.path_light >= 0 && R.lights[R.path_light].use_in_render) {
-            const c3 dif = get_diffuse(R.mats, R.mods, R.texs, sp.mat, sp.P);
+            const c3 dif = get_diffuse(R, sp, eye);
             q = (cenergy(dif) < 0.05f) ? 0u : 1u;
         }
     }
@@ -703,12 +706,14 @@ __global__ __launch_bounds__(SB) void k_shade(const RenderScene* __restrict__ sc
                 if (L.cast_shadows && (mask >> li & 1u)) fl = cadd(fl, mkc(0, 0, 0));
                 else fl = cadd(fl, from_light(R, sp, L.dir, L.col, eye));
             }
-            rho = get_diffuse(R.mats, R.mods, R.texs, sp.mat, sp.P);
+            // raycolor's factor: getDiffuse(state, tempsp, -ray) (pathlight.cc:339)
+            rho = get_diffuse(R, sp, -ray);
             if (R.gp_on) {
                 // getLight with the irradiance map (pathlight.cc:140-185):
-                // irradiance x diffuse, then the lights
+                // irradiance x getDiffuse(state, sp, N), then the lights
                 const f3 Nv = face_forward(sp.Ng, sp.N, -ray);
-                fl = cadd(cmul(gp_irradiance(R.gp, sp.P, Nv), rho), fl);
+                const c3 rn = diffuse_reads_eye(R.mats, R.mods, sp.mat) ? get_diffuse(R, sp, Nv) : rho;
+                fl = cadd(cmul(gp_irradiance(R.gp, sp.P, Nv), rn), fl);
             }
             contri[(size_t)p * D + j] = cmul(cscale(raycolor, s), fl);
             nseg[p] = (uint8_t)(j + 1);
@@ -1106,7 +1111,7 @@ __global__ __launch_bounds__(RB) void k_finalize(const RenderScene* __restrict__
                 total = cscale(total, mult / 1.0f);
                 f3 eye;
                 const Surf sp = primary_surf(R, a, item, ti, item_t[item], eye);
-                const c3 dif = get_diffuse(R.mats, R.mods, R.texs, sp.mat, sp.P);
+                const c3 dif = get_diffuse(R, sp, eye);
                 pathc = cscale(cmul(dif, total), R.path_power);
             }
             col = mkc(0, 0, 0);
@@ -1364,7 +1369,7 @@ struct RenderWS {
     uint64_t g_rnd_skip = 0;
     DevBuf<f3> lv_P, lv_N, lv_U, lv_V, lv_pP;
     DevBuf<float> lv_Z;
-    DevBuf<int32_t> lv_mat;
+    DevBuf<c3> lv_dif;
     std::vector<uint32_t> lc_ins, lc_pix;  // the last cache frame's samples (items, pixels) in insertion order
     uint64_t lc_uid = 0;                    // its scene
     // the single-scene render the workspace holds (for YG_RENDER_REUSE)
@@ -1521,6 +1526,13 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
                     return yg_fail(YG_ERR_ARG, "pathlight cache_size and shadow_threshold must be > 0");
                 if (Lg.search < 3 || Lg.search > LC_KMAX)
                     return yg_fail(YG_ERR_UNSUPPORTED, "pathlight cache search must be 3..1000");
+                // the fake pass forms the screen position in double
+                // (scene.cc:559-583), the render pass in float: a "window"
+                // texture seen by both passes is not on the GPU path
+                for (const yg_modulator& md : s->mods)
+                    if (md.texco == YG_TEXCO_WIN)
+                        return yg_fail(YG_ERR_UNSUPPORTED,
+                                       "pathlight cache with a texco window modulator is not on the GPU path");
                 P.cache = true;
                 P.cache_size = Lg.cache_size;
                 P.shadow_threshold = Lg.shadow_threshold;
@@ -2004,7 +2016,7 @@ static int cache_frame(const ScenePlan& P, hipStream_t st, bool ktiming, std::ve
         (rc = W.lc_cnt.ensure((size_t)na)) || (rc = W.lc_nb.ensure(nitems)) || (rc = W.lc_qual0.ensure(nitems)) ||
         (rc = W.lc_err.ensure(2)) || (rc = W.lv_P.ensure(nitems)) || (rc = W.lv_N.ensure(nitems)) ||
         (rc = W.lv_U.ensure(nitems)) || (rc = W.lv_V.ensure(nitems)) || (rc = W.lv_pP.ensure(nitems)) ||
-        (rc = W.lv_Z.ensure(nitems)) || (rc = W.lv_mat.ensure(nitems)))
+        (rc = W.lv_Z.ensure(nitems)) || (rc = W.lv_dif.ensure(nitems)))
         return rc;
     YG_HIP(hipMemsetAsync(W.lc_in.p, 0, sizeof(uint32_t) * nitems, st));
     YG_HIP(hipMemsetAsync(W.lc_cnt.p, 0, sizeof(uint32_t) * na, st));
@@ -2056,7 +2068,7 @@ static int cache_frame(const ScenePlan& P, hipStream_t st, bool ktiming, std::ve
     YG_HIP(hipMemcpyAsync(W.lc_adjs.p, adjs.data(), sizeof(int32_t) * adjs.size(), hipMemcpyHostToDevice, st));
     if (!adj.empty())
         YG_HIP(hipMemcpyAsync(W.lc_adj.p, adj.data(), sizeof(int32_t) * adj.size(), hipMemcpyHostToDevice, st));
-    VisitView vv{W.lv_P.p, W.lv_N.p, W.lv_U.p, W.lv_V.p, W.lv_pP.p, W.lv_Z.p, W.lv_mat.p};
+    VisitView vv{W.lv_P.p, W.lv_N.p, W.lv_U.p, W.lv_V.p, W.lv_pP.p, W.lv_Z.p, W.lv_dif.p};
     hipLaunchKernelGGL(k_cache_prep, dim3(grid_for(nitems, RB)), dim3(RB), 0, st, W.scenes.p, W.areas.p, na, nitems,
                        W.item_ti.p, W.item_t.p, W.qual.p, vv, cp);
     YG_HIP(hipGetLastError());
@@ -2987,6 +2999,9 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
             return yg_fail(YG_ERR_UNSUPPORTED, "adjoint differentiates Musgrave fBm only, not the multifractals");
     }
     if (!s->mods.empty() && !d_grad_mod) return yg_fail(YG_ERR_ARG, "d_grad_mod required");
+    for (const yg_modulator& md : s->mods)
+        if (md.texco != YG_TEXCO_GLOBAL)
+            return yg_fail(YG_ERR_UNSUPPORTED, "adjoint differentiates modulators with texco global only");
     hipStream_t st = (hipStream_t)stream;
     std::vector<ScenePlan> plans(1);
     int rc = plan_scene(s, opts, d_rgb, plans[0]);

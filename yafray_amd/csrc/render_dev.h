@@ -160,6 +160,7 @@ __device__ __forceinline__ float ri_lp(uint32_t i)
 struct Surf {
     f3 P, N, Ng, NU, NV;
     int obj, mat, ti;
+    float sx = 0.0f, sy = 0.0f;  // screen position (camera-ray hits; 0 elsewhere, scene.cc:234)
 };
 
 // The shading normal of triangle_t::getSurface (triangle.cc:142-160) at the
@@ -191,7 +192,96 @@ __device__ __forceinline__ Surf make_surf(const BVHView& bvh, int ti, f3 f, f3 r
     create_cs(s.N, s.NU, s.NV);
     s.obj = bvh.tobj[ti];
     s.mat = bvh.tmat[ti];
+    s.sx = 0.0f;
+    s.sy = 0.0f;
     return s;
+}
+
+// reflect (vector3d.h:221-226)
+__device__ __forceinline__ f3 reflect3(f3 n, f3 v)
+{
+    const float vn = dot(v, n);
+    if (vn < 0) return -v;
+    return n * (2.0f * vn) - v;
+}
+
+// The texture coordinates of modulator_t::doMapping (texture.cc:125-165) for
+// texco at hit sp, seen along eye (the eye vector of the shader call); vec is
+// set for a direction ("normal", "reflect").  uv and orco points come from
+// triangle_t::getSurface (triangle.cc:138-167): a triangle with no uv, vertex
+// colours, vertex normals or orco points has u = v = -1 and no orco point;
+// otherwise getInterpolation's weights interpolate them (u = v = 0 without uv).
+__device__ __forceinline__ f3 tex_coords(const BVHView& bvh, int texco, const Surf& sp, f3 eye, bool& vec)
+{
+    vec = false;
+    if (texco == YG_TEXCO_UV || texco == YG_TEXCO_ORCO) {
+        const float4* r = bvh.ptex ? bvh.ptex + 4 * (size_t)bvh.tprim[sp.ti] : nullptr;
+        const float4 r0 = r ? r[0] : make_float4(0, 0, 0, 0), r1 = r ? r[1] : make_float4(0, 0, 0, 0);
+        const float4 r2 = r ? r[2] : make_float4(0, 0, 0, 0), r3 = r ? r[3] : make_float4(0, 0, 0, 0);
+        const int fl = (int)r3.w;
+        if (texco == YG_TEXCO_ORCO && !(fl & 4)) {
+            // meshObject_t::toObjectOrco: backOrco * P
+            const float* M = bvh.oorco + 12 * (size_t)sp.obj;
+            const f3 P = sp.P;
+            return mk3(M[0] * P.x + M[1] * P.y + M[2] * P.z + M[3], M[4] * P.x + M[5] * P.y + M[6] * P.z + M[7],
+                       M[8] * P.x + M[9] * P.y + M[10] * P.z + M[11]);
+        }
+        const bool smooth = bvh.tvn && bvh.tvn[3 * sp.ti].w != 0.0f;
+        float u = -1.0f, v = -1.0f;
+        if (fl || smooth) {
+            const float4 TA = bvh.tris[3 * sp.ti], TB = bvh.tris[3 * sp.ti + 1], TC = bvh.tris[3 * sp.ti + 2];
+            float fb, fc;
+            tri_bary(sp.Ng, mk3(TA.x, TA.y, TA.z), mk3(TB.x, TB.y, TB.z), mk3(TC.x, TC.y, TC.z), sp.P, fb, fc);
+            const float fa = 1.0f - fb - fc;
+            if (texco == YG_TEXCO_ORCO)
+                return mk3(r1.z, r1.w, r2.x) * fa + mk3(r2.y, r2.z, r2.w) * fb + mk3(r3.x, r3.y, r3.z) * fc;
+            u = (fl & 1) ? r0.x * fa + r0.z * fb + r1.x * fc : 0.0f;
+            v = (fl & 1) ? r0.y * fa + r0.w * fb + r1.y * fc : 0.0f;
+        }
+        return mk3((float)((double)u * 2.0 - 1.0), (float)(1.0 - (double)v * 2.0), 0.0f);
+    }
+    if (texco == YG_TEXCO_WIN) return mk3(sp.sx, sp.sy, 0.0f);
+    if (texco == YG_TEXCO_NOR) {
+        vec = true;
+        return sp.N;
+    }
+    if (texco == YG_TEXCO_REFL) {
+        vec = true;
+        const f3 ne = norm3(eye);
+        return reflect3(face_forward(sp.Ng, sp.N, ne), ne);
+    }
+    return sp.P;
+}
+
+// Whether material mat has a modulator whose coordinates depend on the eye
+// vector (texco "reflect")
+__device__ __forceinline__ bool diffuse_reads_eye(const yg_material* mats, const yg_modulator* mods, int mat)
+{
+    const yg_material& m = mats[mat];
+    for (int k = 0; k < m.mod_count; k++)
+        if (mods[m.mod_first + k].texco == YG_TEXCO_REFL) return true;
+    return false;
+}
+
+// genericShader_t::getDiffuse (basicshaders.cc:191-204) at hit sp, eye the
+// call's eye vector
+__device__ __forceinline__ c3 get_diffuse(const BVHView& bvh, const yg_material* mats, const yg_modulator* mods,
+                                          const yg_texture* texs, const Surf& sp, f3 eye)
+{
+    const yg_material& m = mats[sp.mat];
+    c3 c = mkc(m.color[0], m.color[1], m.color[2]);
+    for (int k = 0; k < m.mod_count; k++) {
+        const yg_modulator& md = mods[m.mod_first + k];
+        bool vec;
+        const f3 T = tex_coords(bvh, md.texco, sp, eye, vec);
+        c = modulate(md, texs, c, T, vec);
+    }
+    return c;
+}
+
+__device__ __forceinline__ c3 get_diffuse(const RenderScene& R, const Surf& sp, f3 eye)
+{
+    return get_diffuse(R.bvh, R.mats, R.mods, R.texs, sp, eye);
 }
 
 // genericShader_t::fromLight, diffuse part (basicshaders.cc:46-70)
@@ -201,7 +291,7 @@ __device__ __forceinline__ c3 from_light(const RenderScene& R, const Surf& sp, f
     const f3 N = face_forward(sp.Ng, sp.N, edir);
     const float inte = dot(N, ldir);
     if (inte < 0) return mkc(0, 0, 0);
-    const c3 lkc = get_diffuse(R.mats, R.mods, R.texs, sp.mat, sp.P);
+    const c3 lkc = get_diffuse(R, sp, eye);
     return cmul(cscale(lcol, inte), lkc);
 }
 

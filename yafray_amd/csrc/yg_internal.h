@@ -80,6 +80,8 @@ struct BVHView {
     const int32_t* oflags;  // per object: bit0 cast shadows, bit1 receive, bit2 use for radiosity
     int32_t nocast;         // some object casts no shadows (scene shadow queries skip it)
     const float4* tvn;      // vertex normals per leaf-ordered triangle (3 float4), or null
+    const float4* ptex;     // texture-coordinate data per primitive (4 float4), or null
+    const float* oorco;     // per object: rows 0..2 of the orco matrix (12 floats)
 };
 
 struct HostBVH {
@@ -179,6 +181,17 @@ struct yg_scene {
     std::vector<float> tri_vn;
     std::vector<uint8_t> tri_smooth;
     float4* d_tvn = nullptr;  // leaf order: (na, 1), (nb, 0), (nc, 0); w of the first = has normals
+    // yg_mesh_texcoords: per triangle in add order 16 floats (u_a v_a u_b v_b
+    // | u_c v_c oa.x oa.y | oa.z ob.xyz | oc.xyz flags; flags bit0 uv, bit1
+    // vertex colours, bit2 orco points) once any object has such data (empty:
+    // none has); per object its first vertex, vertex count, orco matrix
+    // (rows 0..2 of backOrco) and whether the caller set that matrix
+    std::vector<float> tri_tex;
+    std::vector<uint32_t> obj_vbase, obj_nv;
+    std::vector<float> obj_orco;
+    std::vector<uint8_t> obj_orco_set;
+    float4* d_ptex = nullptr;  // tri_tex on the device
+    float* d_oorco = nullptr;  // obj_orco on the device (defaults filled in by yg_build)
 
     bool built = false;
     int32_t max_depth = 0;
@@ -234,6 +247,8 @@ struct yg_scene {
         v.nobj = nobj;
         v.oflags = d_obj_flags;
         v.tvn = d_tvn;
+        v.ptex = d_ptex;
+        v.oorco = d_oorco;
         v.nocast = 0;
         for (int32_t f : obj_flags)
             if (!(f & 1)) v.nocast = 1;

@@ -47,7 +47,20 @@ MUSGRAVE_TYPES = {"fBm": YG_MUSGRAVE_FBM, "multifractal": YG_MUSGRAVE_MULTIFRACT
 
 class yg_modulator(C.Structure):
     _fields_ = [("tex", C.c_int32), ("mode", C.c_int32), ("color", C.c_float), ("size", F3),
-                ("ofs", F3), ("proj", C.c_int32 * 3), ("has_mtx", C.c_int32), ("mtx", C.c_float * 12)]
+                ("ofs", F3), ("proj", C.c_int32 * 3), ("has_mtx", C.c_int32), ("mtx", C.c_float * 12),
+                ("texco", C.c_int32)]
+
+
+# modulator texture coordinates (YG_TEXCO_*); TEXCO_NAMES maps the plugin's
+# "texco" strings as modulator_t::string2texcotype (texture.h:135-149): any
+# other name is "uv"
+YG_TEXCO_GLOBAL, YG_TEXCO_ORCO, YG_TEXCO_UV, YG_TEXCO_WIN, YG_TEXCO_NOR, YG_TEXCO_REFL = range(6)
+TEXCO_NAMES = {"global": YG_TEXCO_GLOBAL, "orco": YG_TEXCO_ORCO, "uv": YG_TEXCO_UV, "window": YG_TEXCO_WIN,
+               "normal": YG_TEXCO_NOR, "reflect": YG_TEXCO_REFL}
+
+
+def texco_of(name):
+    return TEXCO_NAMES.get(name, YG_TEXCO_UV)
 
 
 class yg_material(C.Structure):
@@ -260,12 +273,16 @@ def wood(color1=(0, 0, 0), color2=(1, 1, 1), depth=2, turbulence=1.0, size=1.0, 
 
 
 def modulator(tex, mode=YG_MOD_MIX, color=0.0, size=1.0, sizexyz=(1, 1, 1), ofs=(0, 0, 0), proj=(0, 0, 0),
-              mtx=None):
+              mtx=None, texco=YG_TEXCO_GLOBAL):
     """proj: per output axis 0 = same axis, 1..3 = x/y/z, -1 = none ("n");
-    mtx: 3x4 (or 4x4) texture matrix applied to the hit point, or None."""
+    mtx: 3x4 (or 4x4) texture matrix applied to the texture coordinates, or
+    None; texco: YG_TEXCO_* or a plugin name ("global", "orco", "uv",
+    "window", "normal", "reflect").  The plugin's default is "orco"; this
+    constructor's is the hit point (YG_TEXCO_GLOBAL)."""
     M = yg_modulator()
     M.tex = tex
     M.mode = mode
+    M.texco = texco_of(texco) if isinstance(texco, str) else int(texco)
     M.color = color
     s = (size, size, size) if size != 1.0 else sizexyz
     M.size = _f3(s)
@@ -336,7 +353,7 @@ class SceneDesc:
     """Geometry + shading + lights + camera + render options of one scene."""
 
     def __init__(self, verts, idx, tri_obj, tri_mat, obj_flags, cam, mats, mods, texs, lights,
-                 opts, smooth=None):
+                 opts, smooth=None, tex=None, orco=None):
         self.verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
         self.idx = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
         self.tri_obj = np.ascontiguousarray(tri_obj, np.int32)
@@ -350,6 +367,12 @@ class SceneDesc:
         self.opts = opts
         # autosmooth angle (degrees) per object id; objects absent are flat
         self.smooth = dict(smooth or {})
+        # texture-coordinate data per object id: {"uv": [nt_obj, 6] or None,
+        # "flags": [nt_obj] (bit0 uv, bit1 vertex colours) or None, "orco":
+        # bool (the object's vertices have orco points in self.orco), "mtx":
+        # 12 floats (orco matrix) or None}; orco: [nv, 3] orco point per vertex
+        self.tex = dict(tex or {})
+        self.orco = None if orco is None else np.ascontiguousarray(orco, np.float32).reshape(-1, 3)
 
     @property
     def n_tris(self):

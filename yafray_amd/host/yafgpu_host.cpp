@@ -318,8 +318,13 @@ int shaderFactory(const std::string& type, const paramMap_t& p, const std::vecto
         mp.getParam("ofsz", m.ofs[2]);
         mp.getParam("mode", mode);
         mp.getParam("texco", texco);
-        if (texco != "global")
-            return fail(YG_ERR_UNSUPPORTED, "modulator texco '" + texco + "' is not on the GPU path (use global)");
+        // string2texcotype (texture.h:135-149): an unknown name is "uv"
+        m.texco = YG_TEXCO_UV;
+        if (texco == "global") m.texco = YG_TEXCO_GLOBAL;
+        else if (texco == "orco") m.texco = YG_TEXCO_ORCO;
+        else if (texco == "window") m.texco = YG_TEXCO_WIN;
+        else if (texco == "normal") m.texco = YG_TEXCO_NOR;
+        else if (texco == "reflect") m.texco = YG_TEXCO_REFL;
         // string2texprojection (texture.h:166-173): "nxyz".find(axis), 'n' or
         // an unknown name = no axis
         const char* defaults[3] = {"x", "y", "z"};
@@ -451,6 +456,24 @@ int gpuInterface_t::addObject_trimesh(const std::string& name, const std::vector
     return YG_OK;
 }
 
+int gpuInterface_t::setObjectTexcoords(const std::string& name, const std::vector<float>& face_uv,
+                                       const std::vector<uint8_t>& face_flags, const std::vector<float>& orco,
+                                       const std::vector<float>& orco_mtx)
+{
+    auto it = objects_.find(name);
+    if (it == objects_.end()) return fail(YG_ERR_ARG, "setObjectTexcoords: no object " + name);
+    Obj& o = it->second;
+    const size_t nf = o.f.size() / 3;
+    if ((!face_uv.empty() && face_uv.size() != 6 * nf) || (!face_flags.empty() && face_flags.size() != nf) ||
+        (!orco.empty() && orco.size() != o.v.size()) || (!orco_mtx.empty() && orco_mtx.size() != 12))
+        return fail(YG_ERR_ARG, "setObjectTexcoords: bad array sizes");
+    o.uv = face_uv;
+    o.fflags = face_flags;
+    o.orco = orco;
+    o.mtx = orco_mtx;
+    return YG_OK;
+}
+
 int gpuInterface_t::describe(std::vector<yg_light>& lights, std::vector<yg_material>& mats,
                              std::vector<yg_modulator>& mods, std::vector<yg_texture>& texs)
 {
@@ -561,6 +584,12 @@ int gpuInterface_t::render(const paramMap_t& params, std::vector<float>& rgb, yg
             return rc;
         // render.cc:764: obj->autoSmooth(angle) after the mesh is built
         if (o.smooth >= 1.0f && (rc = yg_mesh_autosmooth(s, oid, o.smooth))) return rc;
+        if (!o.uv.empty() || !o.fflags.empty() || !o.orco.empty() || !o.mtx.empty())
+            if ((rc = yg_mesh_texcoords(s, oid, o.uv.empty() ? nullptr : o.uv.data(),
+                                        o.fflags.empty() ? nullptr : o.fflags.data(),
+                                        o.orco.empty() ? nullptr : o.orco.data(),
+                                        o.mtx.empty() ? nullptr : o.mtx.data())))
+                return rc;
     }
     if ((rc = yg_build(s, nullptr))) return rc;
     if ((rc = yg_set_camera(s, &cit->second))) return rc;

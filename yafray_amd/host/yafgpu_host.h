@@ -74,6 +74,14 @@ public:
                           const std::vector<uint32_t>& faces, const std::vector<std::string>& shaders,
                           bool cast_shadows = true, bool recv_rad = true, bool use_rad = true,
                           float smooth_angle = 0.0f);
+    // texture-coordinate data of object name (render.cc:743-764,
+    // interface_impl.cc:229-238): face_uv 6 per face (u_a v_a u_b v_b u_c
+    // v_c) or empty, face_flags per face (bit0 uv, bit1 vertex colours) or
+    // empty, orco 3 per vertex (has_orco) or empty, orco_mtx the 12-float
+    // orco matrix (yg_orco_matrix of the object's transform) or empty
+    int setObjectTexcoords(const std::string& name, const std::vector<float>& face_uv,
+                           const std::vector<uint8_t>& face_flags, const std::vector<float>& orco,
+                           const std::vector<float>& orco_mtx);
     // render params: camera_name, background_name, bias, raydepth, AA_*, exposure,
     // gamma, clamp_rgb, output_type; rgb gets resx*resy*3 floats, tone-mapped
     // unless output_type is hdr or exr.  Photon lights are built first.
@@ -103,6 +111,8 @@ private:
         std::vector<std::string> sh;
         int flags;
         float smooth = 0.0f;
+        std::vector<float> uv, orco, mtx;
+        std::vector<uint8_t> fflags;
     };
     std::map<std::string, yg_texture> textures_;
     std::map<std::string, std::pair<paramMap_t, std::vector<paramMap_t>>> shaders_;

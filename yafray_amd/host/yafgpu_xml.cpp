@@ -233,14 +233,15 @@ paramMap_t generic_params(const xmlNode_t& n)
 struct Loader {
     gpuInterface_t& I;
     std::vector<paramMap_t>& renders;
-    double M[4][4];  // current transform (render_t::M)
+    float M[4][4];  // current transform (render_t::M, a float matrix4x4_t)
 
     int item(const xmlNode_t& n);
 
     int transform(const xmlNode_t& n)
     {
-        // render_t::transform (render.cc:114-140): M = old * T for the enclosed items
-        double T[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+        // render_t::transform (render.cc:114-140): M = old * T for the enclosed
+        // items, the float product of matrix4.h:65-77 (sums from 0 over j)
+        float T[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
         for (int r = 0; r < 4; r++)
             for (int c = 0; c < 4; c++) {
                 const std::string key = "m" + std::to_string(r) + std::to_string(c);
@@ -248,12 +249,15 @@ struct Loader {
                 const std::string* v = attr(n, key.c_str());
                 if (v && as_float(*v, f)) T[r][c] = f;
             }
-        double old[4][4];
+        float old[4][4];
         memcpy(old, M, sizeof(M));
         for (int r = 0; r < 4; r++)
             for (int c = 0; c < 4; c++) {
-                double s = 0;
-                for (int k = 0; k < 4; k++) s += old[r][k] * T[k][c];
+                float s = 0;
+                for (int k = 0; k < 4; k++) {
+                    const float p = old[r][k] * T[k][c];
+                    s += p;
+                }
                 M[r][c] = s;
             }
         int rc = YG_OK;
@@ -271,25 +275,31 @@ struct Loader {
         if (!name || name->empty()) return hostFail(YG_ERR_ARG, "object without a name");
         if (attr(n, "original")) return hostFail(YG_ERR_UNSUPPORTED, "object references (original=) are not on the GPU path");
         const xmlNode_t* mesh = nullptr;
+        bool sphere = false;
         for (const auto& k : n.kids)
             if (k.name == "mesh") mesh = &k;
-            else if (k.name == "sphere") return hostFail(YG_ERR_UNSUPPORTED, "sphere objects are not on the GPU path");
+            else if (k.name == "sphere") sphere = true;
+        if (!mesh && sphere) {
+            // render_t::sphere returns NULL (render.cc:768-776): the reference
+            // adds no object for a sphere
+            std::cerr << "[Warning]: sphere object " << *name << " ignored (the scene loader makes no spheres)\n";
+            return YG_OK;
+        }
         if (!mesh) return hostFail(YG_ERR_UNSUPPORTED, "object " + *name + ": only meshes are on the GPU path");
         // Settings that change what the reference renders for this object and
         // that the GPU path does not implement are refused, not ignored:
         // caustic settings turn on obj->caustics (msin_object.cc:448-476),
-        // which makes the pathlight follow caustic bounces (pathlight.cc:194-222);
-        // face uv and has_orco give the surface uv tangents
-        // (meshObject_t::tangentsFromUV, mesh.cc:284-340, render.cc:765), which
-        // the pathlight samples its directions around.
+        // which makes the pathlight follow caustic bounces (pathlight.cc:194-222).
+        // Face uv, vertex colours and orco points are read: the uv tangents
+        // they give (meshObject_t::tangentsFromUV, mesh.cc:284-340) set only
+        // surfacePoint_t's TU/TV, which the generic shader and the lights do
+        // not read (the pathlight samples around NU/NV, createCS of N), and
+        // the generic shader does not read vertex colours.
         if (attr(n, "caus_IOR") || has_descendant(n, "caus_rcolor") || has_descendant(n, "caus_tcolor"))
             return hostFail(YG_ERR_UNSUPPORTED, "object " + *name + ": caustic settings (caus_IOR, caus_rcolor, "
                                                 "caus_tcolor) are not on the GPU path");
-        {
-            const std::string* orco = attr(*mesh, "has_orco");
-            if (orco && *orco == "on")
-                return hostFail(YG_ERR_UNSUPPORTED, "object " + *name + ": has_orco is not on the GPU path");
-        }
+        bool has_orco = false;
+        if (const std::string* orco = attr(*mesh, "has_orco")) has_orco = (*orco == "on");
         float smooth_angle = 0;
         if (const std::string* as = attr(*mesh, "autosmooth")) {
             // render.cc:764 with msin_object.cc:271-280: a float angle; below 1
@@ -304,9 +314,12 @@ struct Loader {
         op.getParam("shadow", shadow);
         op.getParam("emit_rad", emit);
         op.getParam("recv_rad", recv);
-        std::vector<float> v;
+        std::vector<float> raw, v;
         std::vector<uint32_t> f;
         std::vector<std::string> sh;
+        std::vector<float> uv;
+        std::vector<uint8_t> fflags;
+        bool any_tex = false;
         for (const auto& part : mesh->kids) {
             if (part.name == "points") {
                 for (const auto& pt : part.kids) {
@@ -316,8 +329,7 @@ struct Loader {
                         if (a.first.size() == 1 && strchr("xyz", a.first[0]) && as_float(a.second, val))
                             x[a.first[0] - 'x'] = val;
                     }
-                    for (int r = 0; r < 3; r++)  // meshObject_t::factory transforms by M (mesh.cc:460-462)
-                        v.push_back((float)(M[r][0] * x[0] + M[r][1] * x[1] + M[r][2] * x[2] + M[r][3]));
+                    raw.insert(raw.end(), x, x + 3);
                 }
             } else if (part.name == "faces") {
                 for (const auto& fc : part.kids) {
@@ -328,23 +340,70 @@ struct Loader {
                     f.push_back((uint32_t)a);
                     f.push_back((uint32_t)b);
                     f.push_back((uint32_t)c);
-                    for (const auto& a : fc.attrs)
-                        if (a.first.compare(0, 2, "u_") == 0 || a.first.compare(0, 2, "v_") == 0 ||
-                            a.first.compare(0, 5, "vcol_") == 0)
-                            return hostFail(YG_ERR_UNSUPPORTED, "object " + *name + ": face " + a.first +
-                                                                " (uv or vertex colours) is not on the GPU path");
+                    // join_face (msin_object.cc:116-213): u_a..v_c into uv[0..5]
+                    // (u at even, v at odd places), vcol_* set has_vcol; a face
+                    // that names some of them keeps 0 for the rest
+                    float fuv[6] = {0, 0, 0, 0, 0, 0};
+                    uint8_t fl = 0;
+                    static const char* uvk[6] = {"u_a", "v_a", "u_b", "v_b", "u_c", "v_c"};
+                    for (int k = 0; k < 6; k++)
+                        if (const std::string* t = attr(fc, uvk[k])) {
+                            float val;
+                            if (as_float(*t, val)) {
+                                fuv[k] = val;
+                                fl |= 1;
+                            }
+                        }
+                    for (const auto& at : fc.attrs)
+                        if (at.first.compare(0, 5, "vcol_") == 0) fl |= 2;
+                    uv.insert(uv.end(), fuv, fuv + 6);
+                    fflags.push_back(fl);
+                    if (fl) any_tex = true;
                     const std::string* fs = attr(fc, "shader_name");
                     sh.push_back(fs ? *fs : (shader ? *shader : std::string()));
                 }
             }
         }
-        const uint32_t np = (uint32_t)(v.size() / 3);
+        const uint32_t np = (uint32_t)(raw.size() / 3);
         for (uint32_t& k : f)
             if (k >= np) {
                 std::cerr << "[Warning]: Point " << k << " out of bounds in object\n";  // render.cc:719-731
                 k = 0;
             }
-        return I.addObject_trimesh(*name, v, f, sh, shadow, recv, emit, smooth_angle);
+        // meshObject_t::transform (mesh.cc:422-469): vertices = M * p in float;
+        // with has_orco every second point is the orco point of the one
+        // before it and keeps its untransformed value
+        v.resize(raw.size());
+        std::vector<float> orco;
+        if (has_orco) orco.assign(raw.size(), 0.0f);
+        for (uint32_t i = 0; i < np; i++) {
+            const float* x = &raw[3 * (size_t)i];
+            const bool is_orco = has_orco && (i & 1);
+            for (int r = 0; r < 3; r++) {
+                float acc = M[r][0] * x[0];
+                const float t1 = M[r][1] * x[1];
+                acc += t1;
+                const float t2 = M[r][2] * x[2];
+                acc += t2;
+                acc += M[r][3];
+                v[3 * (size_t)i + r] = is_orco ? x[r] : acc;
+            }
+            if (has_orco && i + 1 < np)
+                for (int r = 0; r < 3; r++) orco[3 * (size_t)i + r] = raw[3 * (size_t)(i + 1) + r];
+        }
+        int rc = I.addObject_trimesh(*name, v, f, sh, shadow, recv, emit, smooth_angle);
+        if (rc || f.empty()) return rc;
+        std::vector<float> mtx(12);
+        if ((rc = yg_orco_matrix(&M[0][0], raw.data(), np, f.data(), (uint32_t)(f.size() / 3), mtx.data()))) {
+            char msg[256];
+            yg_last_error(msg, sizeof(msg));
+            return hostFail(rc, "object " + *name + ": " + msg);
+        }
+        if (!any_tex) {
+            uv.clear();
+            fflags.clear();
+        }
+        return I.setObjectTexcoords(*name, uv, fflags, orco, mtx);
     }
 
     int shader(const xmlNode_t& n)

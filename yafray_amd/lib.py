@@ -40,6 +40,8 @@ def lib():
         "yg_scene_destroy": (C.c_int, [vp]),
         "yg_mesh_add": (C.c_int, [vp, vp, u32, vp, u32, vp, i32, C.POINTER(i32)]),
         "yg_mesh_autosmooth": (C.c_int, [vp, i32, f32]),
+        "yg_mesh_texcoords": (C.c_int, [vp, i32, vp, vp, vp, vp]),
+        "yg_orco_matrix": (C.c_int, [vp, vp, u32, vp, u32, vp]),
         "yg_build": (C.c_int, [vp, vp]),
         "yg_scene_info": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_uint64), C.POINTER(i32)]),
@@ -124,6 +126,16 @@ class Scene:
         """meshObject_t::autoSmooth of object obj (vertex normals; before build)."""
         check(lib().yg_mesh_autosmooth(self.h, int(obj), float(angle)), "yg_mesh_autosmooth")
 
+    def texcoords(self, obj, face_uv=None, face_flags=None, orco=None, orco_mtx=None):
+        """Face uv [nt_obj, 6], face flags [nt_obj] (bit0 uv, bit1 vertex
+        colours), orco points [nv_obj, 3] and the orco matrix (12 floats) of
+        object obj (before build)."""
+        def arr(a, dt):
+            return None if a is None else np.ascontiguousarray(a, dt).reshape(-1)
+        uv, fl, oc, mt = arr(face_uv, np.float32), arr(face_flags, np.uint8), arr(orco, np.float32), arr(orco_mtx, np.float32)
+        ptr = lambda a: None if a is None else a.ctypes.data_as(vp)
+        check(lib().yg_mesh_texcoords(self.h, int(obj), ptr(uv), ptr(fl), ptr(oc), ptr(mt)), "yg_mesh_texcoords")
+
     def add_mesh(self, verts, idx, mat=None, flags=7):
         v = np.ascontiguousarray(verts, np.float32).reshape(-1)
         f = np.ascontiguousarray(idx, np.uint32).reshape(-1)
@@ -172,6 +184,12 @@ class Scene:
             oid = s.add_mesh(v, f, desc.tri_mat[sel], int(desc.obj_flags[o]))
             if desc.smooth.get(o, 0) >= 1:
                 s.autosmooth(oid, desc.smooth[o])
+            t = desc.tex.get(o)
+            if t:
+                oc = None
+                if t.get("orco"):
+                    oc = desc.orco[lo:hi] if (contiguous and len(tri)) else desc.orco[used]
+                s.texcoords(oid, t.get("uv"), t.get("flags"), oc, t.get("mtx"))
         if build:
             s.build(**kw)
         s.set_desc(desc)
@@ -324,6 +342,19 @@ class Scene:
     def photon_gather(self, p, n, ng, eye, mat, out):
         check(lib().yg_photon_gather(self.h, _ptr(p), _ptr(n), _ptr(ng), _ptr(eye), _ptr(mat),
                                      mat.shape[0], _ptr(out), _stream()), "yg_photon_gather")
+
+
+def orco_matrix(verts, idx, m16=None):
+    """meshObject_t::transform's orco matrix, rows 0..2 (yg_orco_matrix): the
+    mesh's untransformed vertices [nv, 3], triangles [nt, 3], object matrix
+    m16 (4x4 row-major, None = identity)."""
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1)
+    f = np.ascontiguousarray(idx, np.uint32).reshape(-1)
+    m = None if m16 is None else np.ascontiguousarray(m16, np.float32).reshape(-1)
+    out = np.zeros(12, np.float32)
+    check(lib().yg_orco_matrix(None if m is None else m.ctypes.data_as(vp), v.ctypes.data_as(vp), len(v) // 3,
+                               f.ctypes.data_as(vp), len(f) // 3, out.ctypes.data_as(vp)), "yg_orco_matrix")
+    return out.reshape(3, 4)
 
 
 def render_batch(scenes, outs, opts=None, timing=False):
