@@ -33,7 +33,7 @@ extern "C" {
 #define YG_ERR_UNSUPPORTED (-4)
 #define YG_ERR_NOMEM (-5)
 
-#define YG_ABI_VERSION 13
+#define YG_ABI_VERSION 14
 
 /* ---------------------------------------------------------------- rays -- */
 
@@ -557,6 +557,14 @@ int yg_cache_read(yg_scene* s, float* d_out, uint64_t capacity, uint64_t* count,
  * merged photon direction, merged photon colour; in box order x, y, z) to
  * d_out (capacity in boxes). */
 int yg_gphoton_read(yg_scene* s, float* d_out, uint64_t capacity, uint64_t* count, void* stream);
+
+/* The raw photon map (globalPhotonMap_t photonMap, globalphotonlight.cc:38,
+ * the map photonSampler_t gathers from, pathtools.cc:172-174) in store
+ * order: 5 words per photon -- position (3 float bit patterns), direction
+ * code theta | phi << 8 (dirConverter_t bytes, photon.h:14-37; theta 255:
+ * none), RGBE colour r | g << 8 | b << 16 | e << 24 (color.cc:140-155).
+ * d_out: device memory of capacity records, or NULL to query *count. */
+int yg_gphoton_read_photons(yg_scene* s, uint32_t* d_out, uint64_t capacity, uint64_t* count, void* stream);
 
 #ifdef __cplusplus
 }

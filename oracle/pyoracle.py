@@ -127,6 +127,12 @@ def _load(path):
         L.yo_gphoton_read.argtypes = [vp, vp, C.c_int64]
         L.yo_gphoton_load.restype = C.c_int
         L.yo_gphoton_load.argtypes = [vp, F32P, C.c_int64, vp, C.c_int, C.c_int, C.c_int]
+        L.yo_gphoton_read_photons.restype = C.c_int64
+        L.yo_gphoton_read_photons.argtypes = [vp, vp, C.c_int64]
+        L.yo_gphoton_load_photons.restype = C.c_int
+        L.yo_gphoton_load_photons.argtypes = [vp, vp, C.c_int64]
+        L.yo_psamp_plan.restype = C.c_int
+        L.yo_psamp_plan.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp]
         L.yo_cache_read.restype = C.c_int64
         L.yo_cache_read.argtypes = [vp, vp, C.c_int64, U64P]
         L.yo_cache_record.argtypes = [vp, C.c_int]
@@ -401,6 +407,35 @@ class OracleScene:
         out = np.zeros(max(n, 1) * 15, np.float32)
         L.yo_gphoton_read(self.h, _ptr(out), n)
         return out[: n * 15].reshape(n, 15)
+
+    def gphoton_read_photons(self):
+        """[n, 5] uint32: the raw photon map in store order (position as float
+        bits, direction code theta | phi << 8, RGBE r | g << 8 | b << 16 | e << 24)."""
+        L = lib(self.fast)
+        n = L.yo_gphoton_read_photons(self.h, None, 0)
+        out = np.zeros(max(n, 1) * 5, np.uint32)
+        L.yo_gphoton_read_photons(self.h, out.ctypes.data, n)
+        return out[: n * 5].reshape(n, 5)
+
+    def gphoton_load_photons(self, recs):
+        """Replace the raw photon map by [n, 5] uint32 records (after gphoton_load)."""
+        r = np.ascontiguousarray(recs, np.uint32)
+        if lib(self.fast).yo_gphoton_load_photons(self.h, r.ctypes.data, len(r)) != 0:
+            raise RuntimeError("yo_gphoton_load_photons: no map loaded")
+
+    def psamp_plan(self, P, N, Ru, Rv, samples, grid=36):
+        """photonSampler_t::samplingFrom at one point (stateless gather):
+        (hits[sectors], weight[sectors], multi, photons found)."""
+        v = [np.ascontiguousarray(x, np.float32) for x in (P, N, Ru, Rv)]
+        hits = np.zeros(64, np.int32)
+        w = np.zeros(64, np.float32)
+        m = np.zeros(1, np.float32)
+        nf = np.zeros(1, np.int32)
+        ns = lib(self.fast).yo_psamp_plan(self.h, *[x.ctypes.data for x in v], int(samples), int(grid),
+                                          hits.ctypes.data, w.ctypes.data, m.ctypes.data, nf.ctypes.data)
+        if ns < 0:
+            raise RuntimeError("yo_psamp_plan failed: %d" % ns)
+        return hits[:ns], w[:ns], float(m[0]), int(nf[0])
 
     def cache_read(self):
         """Irradiance cache of the last render with a cache = on pathlight:

@@ -180,6 +180,16 @@ int yo_gphoton_build(yo_scene* s, const yg_material* mats, int nmat, const yg_mo
                      int nobj, const yg_render_opts* opts, int li, int stateless, uint64_t* stats);
 /* boxes: 15 floats each (position, N, irradiance, merged direction, merged colour) */
 int64_t yo_gphoton_read(yo_scene* s, float* out, int64_t cap);
+/* the raw photon map in store order, 5 words per photon: position (3 floats),
+ * direction code (theta | phi << 8; theta 255: none), RGBE (r | g << 8 | b << 16 | e << 24) */
+int64_t yo_gphoton_read_photons(yo_scene* s, uint32_t* out, int64_t cap);
+/* replace the raw photon map by another implementation's (same 5-word format) */
+int yo_gphoton_load_photons(yo_scene* s, const uint32_t* data, int64_t n);
+/* photonSampler_t's sector plan (pathtools.cc:172-236) for one shading point:
+ * hits[sectors], weight[sectors], multi; the gather is stateless (radius = maxradius).
+ * Returns the number of sectors, or -1 without a raw map. */
+int yo_psamp_plan(yo_scene* s, const float* P, const float* N, const float* Ru, const float* Rv, int samples,
+                  int grid, int32_t* hits, float* weight, float* multi, int32_t* nfound);
 
 /* Irradiance cache (pathlight cache = on) of the last yo_render*: its
  * samples, 18 floats each (P, N, pP, colour, adist, M, precision, rank,

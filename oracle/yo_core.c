@@ -484,6 +484,7 @@ void yo_scene_free(yo_scene* s)
     free(s->nodes); free(s->order);
     free(s->pm.data); free(s->pm.gstart); free(s->pm.gitems);
     free(s->gm.box); free(s->gm.gstart); free(s->gm.gitems);
+    free(s->gm.rec); free(s->gm.rgstart); free(s->gm.rgitems);
     free(s);
 }
 

@@ -164,7 +164,18 @@ typedef struct {
     int64_t* gstart;
     int64_t* gitems;
     float irr_radius;      /* pathLight_t's photonData_t radius state (pathlight.cc:131) */
+    /* the raw photon map (globalPhotonMap_t photonMap, globalphotonlight.cc:38) in
+     * store order, with a uniform grid (cell = radius) for photonSampler_t's gathers */
+    int64_t nrec;
+    struct yo_gphot_s* rec;
+    float rgmin[3];
+    int rgdim[3];
+    int64_t* rgstart;
+    int64_t* rgitems;
 } yo_gmap;
+
+/* storedPhoton_t of the raw map: position, 8-bit direction, RGBE colour */
+typedef struct yo_gphot_s { float pos[3]; uint8_t th, ph; uint8_t rgbe[4]; } yo_gphot;
 
 struct yo_scene {
     int64_t nv, nt;

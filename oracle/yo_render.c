@@ -39,6 +39,19 @@ typedef struct {
     float pow;
 } yo_area;
 
+/* photonSampler_t (pathtools.h:64-126) of a render area: the hemigrid of
+ * paralels x meridians sectors, the current shading point's plan, the walk */
+#define YO_PS_MAXSEC 64
+typedef struct {
+    int made;              /* created by this area's first takeSample */
+    float radius;          /* its gather radius state */
+    int par, mer, sectors, search;
+    float pdiv, mdiv, multi;
+    int hits[YO_PS_MAXSEC];
+    float weight[YO_PS_MAXSEC];
+    int cur[3];
+} yo_psamp;
+
 typedef struct {
     const yo_scene* s;
     const yg_material* mats;
@@ -78,6 +91,7 @@ typedef struct {
     /* cacheProxy_t of the render area (cacheproxy.cc:10-15, 160-172) */
     float px_radius;
     int64_t px_id;
+    yo_psamp ps;           /* photonSampler_t of the current area (pathlight samples > 96 with a global map) */
     struct yo_ls_s* px_created;
     int px_n, px_cap;
     uint64_t lc_new;       /* interpolate() found no sample and took one */
@@ -444,8 +458,14 @@ static yo_col background(const rctx* R)
     return col(R->opts.background[0], R->opts.background[1], R->opts.background[2]);
 }
 
+static void ps_create(rctx* R, int li);
+static void ps_sampling_from(rctx* R, yo_psamp* S, yo_v3 P, yo_v3 N, yo_v3 Ru, yo_v3 Rv, int samples);
+static yo_v3 ps_next_direction(yo_psamp* S, yo_halton* hs, yo_v3 N, yo_v3 Ru, yo_v3 Rv, int curlev,
+                               yo_col* raycolor);
+
 /* pathLight_t::takeSample full-GI branch, pathlight.cc:223-387 (no cache, no
- * caustics, direct = false) */
+ * caustics, direct = false); getSampler (pathlight.cc:725-738) gives the
+ * photonSampler_t when a global photon map is published and samples > 96 */
 static yo_col path_take_sample(rctx* R, int li, yo_v3 N, const yo_sp* sp, float* avgD, float* minD)
 {
     const yg_light* L = &R->lights[li];
@@ -459,11 +479,19 @@ static yo_col path_take_sample(rctx* R, int li, yo_v3 N, const yo_sp* sp, float*
     for (int k = 0; k < 4; k++) subtotal[k] = col(0, 0, 0);
     int taken = 0, first = 1;
     float repetitions = 0.0f;
+    yo_psamp* PS = NULL;
+    if (R->gp && R->s->gm.rec && samples > 96) {
+        PS = &R->ps;
+        if (!PS->made) ps_create(R, li);
+        ps_sampling_from(R, PS, sp->P, N, sp->NU, sp->NV, samples);
+    }
     for (int ite = 1; ite > 0; --ite, repetitions += 1.0f) {
         taken = 0;
+        if (PS) PS->cur[0] = PS->cur[1] = PS->cur[2] = 0;  /* photonSampler_t::reset */
         for (int i = 0; i < samples; ++i) {
             yo_col raycolor = col(1, 1, 1);
-            yo_v3 ray = next_direction(hs, &taken, N, sp->NU, sp->NV, i, 0);
+            yo_v3 ray = PS ? ps_next_direction(PS, hs, N, sp->NU, sp->NV, 0, &raycolor)
+                           : next_direction(hs, &taken, N, sp->NU, sp->NV, i, 0);
             yo_v3 startray = ray;
             yo_v3 where = sp->P;
             if (caching) HNUM += 1;
@@ -517,14 +545,15 @@ static yo_col path_take_sample(rctx* R, int li, yo_v3 N, const yo_sp* sp, float*
                 if (first) subtotal[i % 4] = col_add(subtotal[i % 4], contri);
                 yo_v3 HN = face_forward(t.Ng, t.N, v3_neg(ray));
                 raycolor = col_mul(raycolor, get_diffuse(R, &t, v3_neg(ray)));
-                ray = next_direction(hs, &taken, HN, t.NU, t.NV, i, j + 1);
+                ray = PS ? ps_next_direction(PS, hs, HN, t.NU, t.NV, j + 1, &raycolor)
+                         : next_direction(hs, &taken, HN, t.NU, t.NV, i, j + 1);
                 where = t.P;
             }
             if (rec) adj_path_sample_end(R, v3_dot(startray, N));
         }
         if (first) {
             first = 0;
-            float mult = (float)(1.0 / (double)(float)(taken + 1));
+            float mult = PS ? PS->multi : (float)(1.0 / (double)(float)(taken + 1));
             float minR = 1000, minG = 1000, minB = 1000, maxR = 0, maxG = 0, maxB = 0;
             for (int i = 0; i < 4; ++i) {
                 subtotal[i] = col_scale(subtotal[i], mult);
@@ -553,7 +582,7 @@ static yo_col path_take_sample(rctx* R, int li, yo_v3 N, const yo_sp* sp, float*
         *avgD = H;
         *minD = M;
     }
-    float mult = (float)(1.0 / (double)(float)(taken + 1));
+    float mult = PS ? PS->multi : (float)(1.0 / (double)(float)(taken + 1));
     return col_scale(total, mult / repetitions);
 }
 
@@ -1467,6 +1496,328 @@ static void gm_grid(yo_gmap* M)
     free(fill); free(cellof);
 }
 
+/* uniform grid (cell = radius) over the raw map's photons */
+static void gr_grid(yo_gmap* M)
+{
+    free(M->rgstart); free(M->rgitems);
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = 0; i < M->nrec; i++)
+        for (int a = 0; a < 3; a++) {
+            lo[a] = fminf(lo[a], M->rec[i].pos[a]);
+            hi[a] = fmaxf(hi[a], M->rec[i].pos[a]);
+        }
+    int64_t nc = 1;
+    for (int a = 0; a < 3; a++) {
+        M->rgmin[a] = M->nrec ? lo[a] : 0.0f;
+        int d = M->nrec ? (int)floorf((hi[a] - lo[a]) / M->radius) + 1 : 1;
+        if (d < 1) d = 1;
+        if (d > 256) d = 256;
+        M->rgdim[a] = d;
+        nc *= d;
+    }
+    M->rgstart = (int64_t*)calloc((size_t)nc + 1, sizeof(int64_t));
+    M->rgitems = (int64_t*)malloc(sizeof(int64_t) * (size_t)(M->nrec > 0 ? M->nrec : 1));
+    int64_t* cof = (int64_t*)malloc(sizeof(int64_t) * (size_t)(M->nrec > 0 ? M->nrec : 1));
+    for (int64_t i = 0; i < M->nrec; i++) {
+        int c[3];
+        for (int a = 0; a < 3; a++) {
+            int k = (int)floorf((M->rec[i].pos[a] - M->rgmin[a]) / M->radius);
+            if (k < 0) k = 0;
+            if (k >= M->rgdim[a]) k = M->rgdim[a] - 1;
+            c[a] = k;
+        }
+        cof[i] = ((int64_t)c[2] * M->rgdim[1] + c[1]) * M->rgdim[0] + c[0];
+        M->rgstart[cof[i] + 1]++;
+    }
+    for (int64_t c = 0; c < nc; c++) M->rgstart[c + 1] += M->rgstart[c];
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nc + 1));
+    memcpy(fill, M->rgstart, sizeof(int64_t) * (size_t)nc);
+    for (int64_t i = 0; i < M->nrec; i++) M->rgitems[fill[cof[i]]++] = i;
+    free(fill); free(cof);
+}
+
+/* globalPhotonMap_t::gather (photon.cc:167-212) over the raw map, mincos 0:
+ * the K nearest photons within *radius whose direction faces N, sorted by
+ * (distance, store index); the radius doubles while fewer than K are reached
+ * and shrinks by 0.95 when K is under 0.49 of those reached.  (The reference
+ * sums its found photons in heap order; the sorted order here changes only
+ * float rounding.) */
+static int gr_gather(const yo_gmap* M, yo_v3 P, yo_v3 N, int K, float* radius, yo_gfound* found)
+{
+    int reached = 0, nf = 0;
+    while ((reached < K) && (*radius <= M->radius)) {
+        reached = 0;
+        nf = 0;
+        const float r = *radius;
+        const float pp[3] = {P.x, P.y, P.z};
+        int g0[3], g1[3];
+        for (int a = 0; a < 3; a++) {
+            long lo = (long)floorf((pp[a] - r - M->rgmin[a]) / M->radius);
+            long hi = (long)floorf((pp[a] + r - M->rgmin[a]) / M->radius);
+            if (lo < 0) lo = 0;
+            if (hi >= M->rgdim[a]) hi = M->rgdim[a] - 1;
+            g0[a] = (int)lo; g1[a] = (int)hi;
+        }
+        for (int z = g0[2]; z <= g1[2]; z++)
+            for (int y = g0[1]; y <= g1[1]; y++)
+                for (int x = g0[0]; x <= g1[0]; x++) {
+                    const int64_t c = ((int64_t)z * M->rgdim[1] + y) * M->rgdim[0] + x;
+                    for (int64_t q = M->rgstart[c]; q < M->rgstart[c + 1]; q++) {
+                        const int64_t i = M->rgitems[q];
+                        const yo_gphot* g = &M->rec[i];
+                        const float D = v3_len(v3_sub(v3_ld(g->pos), P));
+                        if ((D > r) || (v3_dot(dir_dec(g->th, g->ph), N) <= 0.0f)) continue;
+                        reached++;
+                        yo_gfound f = {D, i};
+                        if (nf < K) { found[nf++] = f; continue; }
+                        int far = 0;
+                        for (int k = 1; k < nf; k++)
+                            if (gfound_cmp(&found[k], &found[far]) > 0) far = k;
+                        if (gfound_cmp(&f, &found[far]) < 0) found[far] = f;
+                    }
+                }
+        if (reached < K) *radius *= 2;
+    }
+    if (reached > K) {
+        const float f = (float)K / (float)reached;
+        if (f < (float)(0.7 * 0.7)) *radius *= 0.95f;
+    }
+    if (*radius > M->radius) *radius = M->radius;
+    qsort(found, (size_t)nf, sizeof(yo_gfound), gfound_cmp);
+    return nf;
+}
+
+/* ------------------------------------------------ photonSampler_t -- */
+
+/* ourRandomI (vector3d.h:264-274): the next Park-Miller state itself */
+static int32_t our_random_i(int32_t* seed)
+{
+    const int a = 16807, m = 2147483647, q = m / a, r = m % a;
+    int32_t v = *seed;
+    v = a * (v % q) - r * (v / q);
+    if (v < 0) v += m;
+    *seed = v;
+    return v;
+}
+
+/* Halton::setStart (mcqmc.h:24-33) */
+static void halton_start(yo_halton* h, unsigned int i)
+{
+    double value = 0.0, f, factor;
+    f = factor = h->inv_base;
+    while (i > 0) {
+        value += (double)(i % h->base) * factor;
+        i /= h->base;
+        factor *= f;
+    }
+    h->value = value;
+}
+
+static void ps_grid(yo_psamp* S, int grid)
+{
+    const float base = (float)sqrt((double)(float)grid / 2.0);
+    S->par = (int)(base + 0.5);
+    S->mer = S->par * 2;
+    S->pdiv = (float)(1.0 / (double)(float)S->par);
+    S->mdiv = (float)(2.0 * M_PI / (double)(float)S->mer);
+    S->sectors = S->par * S->mer;
+    S->search = 3 * S->sectors;
+}
+
+/* photonSampler_t ctor (pathtools.cc:80-115) at the area's first takeSample:
+ * every Halton dimension starts at an ourRandomI() draw, grid 36 */
+static void ps_create(rctx* R, int li)
+{
+    const yg_light* L = &R->lights[li];
+    const int nd = (L->depth + 1) * 2;
+    for (int d = 0; d < nd; d++) halton_start(&R->hseq[li][d], (unsigned int)our_random_i(&R->lcg));
+    ps_grid(&R->ps, 36);
+    R->ps.radius = R->s->gm.radius;
+    R->ps.made = 1;
+}
+
+/* photonSampler_t::getCoords (pathtools.cc:118-145).  sqrt(1 - z z) is NaN
+ * when |z| exceeds 1 by rounding (undefined in the reference's int cast):
+ * taken as 0 here. */
+static void ps_coords(const yo_psamp* S, yo_v3 v, yo_v3 N, yo_v3 Ru, yo_v3 Rv, int* c0, int* c1)
+{
+    float x = v3_dot(v, Ru), y = v3_dot(v, Rv), z = v3_dot(v, N);
+    float fix = (float)sqrt(1.0 - (double)(z * z));
+    if (fix != fix) fix = 0.0f;
+    int first = (int)(fix / S->pdiv);
+    if (first >= S->par) first--;
+    if (fix > 1.0f) fix = 1.0f;
+    if (fix != 0.0f) { x /= fix; y /= fix; }
+    if (x > 1.0f) x = 1.0f;
+    if (x < -1.0f) x = -1.0f;
+    float an = (float)acos((double)x);
+    if (y < 0) an = (float)(2.0 * M_PI - (double)an);
+    int second = (int)(an / S->mdiv);
+    if (second >= S->mer) second--;
+    *c0 = first;
+    *c1 = second;
+}
+
+static float max_abs_diff(yo_col a, yo_col b)
+{
+    return fmaxf(fabsf(a.r - b.r), fmaxf(fabsf(a.g - b.g), fabsf(a.b - b.b)));
+}
+
+/* photonSampler_t::samplingFrom (pathtools.cc:172-236) with giveMaxDiff
+ * (:147-170): energy of the gathered photons per sector, hits spread by the
+ * energy differences to the 8 neighbours, weight = max hits / hits */
+static void ps_plan(const yo_gmap* M, yo_psamp* S, float* radius, yo_v3 P, yo_v3 N, yo_v3 Ru, yo_v3 Rv,
+                    int samples, int* nfound)
+{
+    yo_gfound* found = (yo_gfound*)malloc(sizeof(yo_gfound) * (size_t)(S->search + 1));
+    const int nf = gr_gather(M, P, N, S->search, radius, found);
+    if (nfound) *nfound = nf;
+    yo_col energy[YO_PS_MAXSEC];
+    const int par = S->par, mer = S->mer;
+    for (int k = 0; k < S->sectors; k++) { energy[k] = col(0, 0, 0); S->hits[k] = 0; }
+    for (int k = 0; k < nf; k++) {
+        const yo_gphot* g = &M->rec[found[k].idx];
+        const yo_v3 d = dir_dec(g->th, g->ph);
+        if (v3_dot(d, N) > 0.0f) {
+            int c0, c1;
+            ps_coords(S, d, N, Ru, Rv, &c0, &c1);
+            energy[c0 * mer + c1] = col_add(energy[c0 * mer + c1], rgbe_dec(g->rgbe));
+        }
+    }
+    free(found);
+    float totaldiff = 0;
+    for (int i = 0; i < par; ++i)
+        for (int j = 0; j < mer; ++j) {
+            const yo_col e = energy[i * mer + j];
+            float mx = 0, t;
+            const int il = i - 1, ih = i + 1, jl = j - 1, jh = j + 1;
+            if (il >= 0) {
+                if (jl >= 0) { t = max_abs_diff(energy[il * mer + jl], e); if (t > mx) mx = t; }
+                t = max_abs_diff(energy[il * mer + j], e); if (t > mx) mx = t;
+                if (jh < mer) { t = max_abs_diff(energy[il * mer + jh], e); if (t > mx) mx = t; }
+            }
+            if (jl >= 0) { t = max_abs_diff(energy[i * mer + jl], e); if (t > mx) mx = t; }
+            if (jh < mer) { t = max_abs_diff(energy[i * mer + jh], e); if (t > mx) mx = t; }
+            if (ih < par) {
+                if (jl >= 0) { t = max_abs_diff(energy[ih * mer + jl], e); if (t > mx) mx = t; }
+                t = max_abs_diff(energy[ih * mer + j], e); if (t > mx) mx = t;
+                if (jh < mer) { t = max_abs_diff(energy[ih * mer + jh], e); if (t > mx) mx = t; }
+            }
+            S->weight[i * mer + j] = mx;
+            totaldiff += mx;
+        }
+    int assign = samples - S->sectors;
+    const float sph = (totaldiff != 0.0f) ? (float)assign / totaldiff : 0;
+    for (int k = 0; k < S->sectors; k++) {
+        int h = (int)((double)(sph * S->weight[k]) + 0.5);
+        if (h > assign) h = assign;
+        if (h < 0) h = 0;  /* only for a non-finite energy (the reference's cast is undefined there) */
+        assign -= h;
+        S->hits[k] = h + 1;
+    }
+    int noway = 0;
+    while (assign) {
+        for (int k = 0; k < S->sectors && assign; k++)
+            if ((S->hits[k] > 1) || noway) { S->hits[k]++; assign--; }
+        noway = 1;
+    }
+    int mx = 1;
+    for (int k = 0; k < S->sectors; k++)
+        if (S->hits[k] > mx) mx = S->hits[k];
+    const float maxf = (float)mx;
+    for (int k = 0; k < S->sectors; k++) S->weight[k] = maxf / (float)S->hits[k];
+    S->multi = (float)(1.0 / (double)(maxf * (float)S->sectors));
+    S->cur[0] = S->cur[1] = S->cur[2] = 0;
+}
+
+static void ps_sampling_from(rctx* R, yo_psamp* S, yo_v3 P, yo_v3 N, yo_v3 Ru, yo_v3 Rv, int samples)
+{
+    const yo_gmap* M = &R->s->gm;
+    if (M->stateless) S->radius = M->radius;
+    ps_plan(M, S, &S->radius, P, N, Ru, Rv, samples, NULL);
+}
+
+/* photonSampler_t::nextDirection (pathtools.cc:238-261) and nextSample
+ * (pathtools.h:92-106): level 0 draws inside the walk's current sector and
+ * weighs the ray by weight x 2 z1; deeper levels use z1 as the sine */
+static yo_v3 ps_next_direction(yo_psamp* S, yo_halton* hs, yo_v3 N, yo_v3 Ru, yo_v3 Rv, int curlev,
+                               yo_col* raycolor)
+{
+    float z1, z2;
+    if (curlev == 0) {
+        z1 = ((float)S->cur[0] + halton_next(&hs[0])) * S->pdiv;
+        z2 = ((float)S->cur[1] + halton_next(&hs[1])) * S->mdiv;
+        *raycolor = col_scale(*raycolor, S->weight[S->cur[0] * S->mer + S->cur[1]] * 2.0f * z1);
+        if (++S->cur[2] == S->hits[S->cur[0] * S->mer + S->cur[1]]) {
+            S->cur[2] = 0;
+            if (++S->cur[1] == S->mer) {
+                S->cur[1] = 0;
+                if (++S->cur[0] == S->par) S->cur[0] = 0;
+            }
+        }
+    } else {
+        curlev <<= 1;
+        z1 = halton_next(&hs[curlev]);
+        z2 = (float)((double)halton_next(&hs[curlev + 1]) * 2.0 * M_PI);
+    }
+    if (z1 > 1.0f) z1 = 1.0f;
+    yo_v3 a = v3_add(v3_mul(Ru, dcos(z2)), v3_mul(Rv, dsin(z2)));
+    return v3_add(v3_mul(a, z1), v3_mul(N, (float)sqrt(1.0 - (double)(z1 * z1))));
+}
+
+int yo_psamp_plan(yo_scene* s, const float* P, const float* N, const float* Ru, const float* Rv, int samples,
+                  int grid, int32_t* hits, float* weight, float* multi, int32_t* nfound)
+{
+    const yo_gmap* M = &s->gm;
+    if (!M->rec) return -1;
+    yo_psamp S;
+    memset(&S, 0, sizeof(S));
+    ps_grid(&S, grid);
+    if (S.sectors > YO_PS_MAXSEC || samples < S.sectors) return -2;
+    float radius = M->radius;
+    int nf = 0;
+    ps_plan(M, &S, &radius, v3_ld(P), v3_ld(N), v3_ld(Ru), v3_ld(Rv), samples, &nf);
+    memcpy(hits, S.hits, sizeof(int32_t) * (size_t)S.sectors);
+    memcpy(weight, S.weight, sizeof(float) * (size_t)S.sectors);
+    *multi = S.multi;
+    if (nfound) *nfound = nf;
+    return S.sectors;
+}
+
+int64_t yo_gphoton_read_photons(yo_scene* s, uint32_t* out, int64_t cap)
+{
+    const yo_gmap* M = &s->gm;
+    if (!M->rec) return -1;
+    for (int64_t i = 0; i < M->nrec && i < cap; i++) {
+        const yo_gphot* g = &M->rec[i];
+        uint32_t* o = out + 5 * i;
+        memcpy(o, g->pos, 12);
+        o[3] = (uint32_t)g->th | ((uint32_t)g->ph << 8);
+        o[4] = (uint32_t)g->rgbe[0] | ((uint32_t)g->rgbe[1] << 8) | ((uint32_t)g->rgbe[2] << 16) |
+               ((uint32_t)g->rgbe[3] << 24);
+    }
+    return M->nrec;
+}
+
+int yo_gphoton_load_photons(yo_scene* s, const uint32_t* data, int64_t n)
+{
+    yo_gmap* M = &s->gm;
+    if (!M->built) return -1;
+    free(M->rec);
+    M->rec = (yo_gphot*)calloc((size_t)(n > 0 ? n : 1), sizeof(yo_gphot));
+    for (int64_t i = 0; i < n; i++) {
+        const uint32_t* o = data + 5 * i;
+        yo_gphot* g = &M->rec[i];
+        memcpy(g->pos, o, 12);
+        g->th = (uint8_t)(o[3] & 255u);
+        g->ph = (uint8_t)((o[3] >> 8) & 255u);
+        for (int k = 0; k < 4; k++) g->rgbe[k] = (uint8_t)(o[4] >> (8 * k));
+    }
+    M->nrec = n;
+    gr_grid(M);
+    return 0;
+}
+
 /* globalPhotonMap_t::gather (photon.cc:167-210) over the boxes' photons:
  * direction and colour of box i as `which` says (0: the merged photon, only
  * boxes with a direction; 1: the irradiance photon, direction N, colour
@@ -1634,6 +1985,7 @@ int yo_gphoton_build(yo_scene* s, const yg_material* mats, int nmat, const yg_mo
     R.lcg = (int32_t)opts->lcg_seed;
     yo_gmap* M = &s->gm;
     free(M->box); free(M->gstart); free(M->gitems);
+    free(M->rec); free(M->rgstart); free(M->rgitems);
     memset(M, 0, sizeof(*M));
     M->light = li;
     M->radius = GL->fixedradius;
@@ -1668,6 +2020,16 @@ int yo_gphoton_build(yo_scene* s, const yg_material* mats, int nmat, const yg_mo
             }
         }
     }
+    /* the raw map keeps the records in store order (photonMap->store) */
+    M->nrec = recs.n;
+    M->rec = (yo_gphot*)calloc((size_t)(recs.n > 0 ? recs.n : 1), sizeof(yo_gphot));
+    for (int64_t i = 0; i < recs.n; i++) {
+        yo_gphot* g = &M->rec[i];
+        g->pos[0] = recs.v[i].pos.x; g->pos[1] = recs.v[i].pos.y; g->pos[2] = recs.v[i].pos.z;
+        g->th = recs.v[i].th; g->ph = recs.v[i].ph;
+        memcpy(g->rgbe, recs.v[i].rgbe, 4);
+    }
+    gr_grid(M);
     /* the hash boxes in (x, y, z) order, records of a box in store order */
     qsort(recs.v, (size_t)recs.n, sizeof(yo_grec), grec_cmp);
     int64_t nbox = 0;
@@ -1728,6 +2090,7 @@ int yo_gphoton_load(yo_scene* s, const float* data, int64_t n, const yg_light* G
 {
     yo_gmap* M = &s->gm;
     free(M->box); free(M->gstart); free(M->gitems);
+    free(M->rec); free(M->rgstart); free(M->rgitems);
     memset(M, 0, sizeof(*M));
     M->light = li;
     M->radius = GL->fixedradius;
@@ -2506,6 +2869,7 @@ static void render_area(rctx* R, const render_job* J, int ri)
     const yo_region* a = &J->regions[ri];
     const yg_render_opts* o = J->opts;
     /* a fresh renderState_t (and sampler context) per area, scene.cc:726 */
+    R->ps.made = 0;
     for (int li = 0; li < J->nl; li++)
         if (J->lights[li].type == YG_LIGHT_PATH) {
             int nd = (J->lights[li].depth + 1) * 2, base = 2;
@@ -2704,9 +3068,10 @@ int yo_render_mt(yo_scene* s, const yg_camera* cam, const yg_material* mats, int
     for (int li = 0; li < nl; li++)
         if (lights[li].type == YG_LIGHT_GLOBALPHOTON) {
             if (!s->gm.built || s->gm.light != li) return -3;  /* map not built */
-            for (int lj = 0; lj < nl; lj++)
-                if (lights[lj].type == YG_LIGHT_PATH && lights[lj].samples > 96) return -6;  /* photonSampler_t */
-            nthreads = 1;  /* the gathers carry a radius from call to call */
+            /* one thread: the gathers carry a radius from call to call, and
+             * photonSampler_t (pathlight samples > 96) draws its Halton starts
+             * from ourRandomI() in area order */
+            nthreads = 1;
         }
     int sequential = 0;
     for (int li = 0; li < nl; li++)

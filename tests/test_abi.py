@@ -30,7 +30,7 @@ def test_error_path_without_gpu():
     """Argument errors are reported through the status code and yg_last_error."""
     from yafray_amd import lib as Y
     L = Y.lib()
-    assert L.yg_abi_version() == 13
+    assert L.yg_abi_version() == 14
     assert L.yg_scene_create(None) == -1
     assert "null" in Y.last_error()
     h = ctypes.c_void_p()

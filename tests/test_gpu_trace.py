@@ -52,7 +52,7 @@ def _check(prim_g, t_g, u_g, v_g, prim_c, t_c, u_c, v_c):
 
 def test_library_loads(torch_cuda):
     Y = _lib(torch_cuda)
-    assert Y.lib().yg_abi_version() == 13
+    assert Y.lib().yg_abi_version() == 14
 
 
 @pytest.mark.parametrize("builder", [D.YG_BUILDER_HOST, D.YG_BUILDER_DEVICE, D.YG_BUILDER_DEVICE_PLOC])

@@ -242,6 +242,24 @@ __global__ void k_gp_collect(uint32_t nslots, const uint32_t* __restrict__ valid
         }
 }
 
+// the raw photon map: records in store order decoded for the gathers of
+// photonSampler_t, and as 5-word records (position bits, direction code, RGBE)
+__global__ void k_gp_raw(uint32_t n, const GRec* __restrict__ rec, const float* __restrict__ tab, f3* __restrict__ rpos,
+                         f3* __restrict__ rdir, c3* __restrict__ rcol, uint32_t* __restrict__ raw)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const GRec r = rec[i];
+    rpos[i] = r.pos;
+    rdir[i] = dir_dec(r.code, tab);
+    rcol[i] = rgbe_dec(r.rgbe);
+    raw[5 * (size_t)i + 0] = __float_as_uint(r.pos.x);
+    raw[5 * (size_t)i + 1] = __float_as_uint(r.pos.y);
+    raw[5 * (size_t)i + 2] = __float_as_uint(r.pos.z);
+    raw[5 * (size_t)i + 3] = r.code;
+    raw[5 * (size_t)i + 4] = r.rgbe;
+}
+
 __global__ void k_gp_keys(uint32_t n, const GRec* __restrict__ rec, int bx0, int by0, int bz0, int sy, int sz,
                           unsigned long long* __restrict__ keys, uint32_t* __restrict__ vals)
 {
@@ -602,11 +620,54 @@ uint32_t gp_last_sum(const uint32_t* flag, const uint32_t* ex, size_t n, hipStre
     return a + b;
 }
 
+// uniform grid (cell edge `cell`, at most 256 cells per axis; points beyond
+// the last cell stay in it) over points, built on the host
+static void host_grid(const std::vector<f3>& pts, float cell, float* gmin, int* gdim, std::vector<int32_t>& gs,
+                      std::vector<int32_t>& gi)
+{
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (const f3& p : pts) {
+        const float v[3] = {p.x, p.y, p.z};
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], v[a]);
+            hi[a] = std::max(hi[a], v[a]);
+        }
+    }
+    int64_t ncell = 1;
+    for (int a = 0; a < 3; a++) {
+        gmin[a] = pts.empty() ? 0.f : lo[a];
+        int dd = pts.empty() ? 1 : (int)floorf((hi[a] - lo[a]) / cell) + 1;
+        if (dd < 1) dd = 1;
+        if (dd > 256) dd = 256;
+        gdim[a] = dd;
+        ncell *= dd;
+    }
+    gs.assign((size_t)ncell + 1, 0);
+    gi.assign(pts.size(), 0);
+    std::vector<int32_t> cof(pts.size());
+    for (size_t i = 0; i < pts.size(); i++) {
+        const float v[3] = {pts[i].x, pts[i].y, pts[i].z};
+        int c[3];
+        for (int a = 0; a < 3; a++) {
+            int kk = (int)floorf((v[a] - gmin[a]) / cell);
+            if (kk < 0) kk = 0;
+            if (kk >= gdim[a]) kk = gdim[a] - 1;
+            c[a] = kk;
+        }
+        cof[i] = (c[2] * gdim[1] + c[1]) * gdim[0] + c[0];
+        gs[(size_t)cof[i] + 1]++;
+    }
+    for (int64_t c = 0; c < ncell; c++) gs[c + 1] += gs[c];
+    std::vector<int32_t> fill(gs.begin(), gs.end() - 1);
+    for (size_t i = 0; i < pts.size(); i++) gi[fill[cof[i]]++] = (int32_t)i;
+}
+
 }  // namespace
 
 void gphoton_free(GPhotonDev& g)
 {
-    void* ptrs[] = {g.d_key, g.d_N, g.d_irr, g.d_pos, g.d_qN, g.d_qirr, g.d_gstart, g.d_gitems, g.d_read};
+    void* ptrs[] = {g.d_key,  g.d_N,    g.d_irr,  g.d_pos,     g.d_qN,     g.d_qirr,   g.d_gstart,
+                    g.d_gitems, g.d_read, g.d_rpos, g.d_rdir, g.d_rcol, g.d_rraw, g.d_rgstart, g.d_rgitems};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     g = GPhotonDev();
@@ -722,6 +783,22 @@ extern "C" int yg_gphoton_build(yg_scene* s, uint32_t li, const yg_render_opts* 
         YG_HIP(hipMemcpyAsync(hb, bounds, sizeof(hb), hipMemcpyDeviceToHost, st));
         YG_HIP(hipStreamSynchronize(st));
         if (nrec > 0) {
+            // the raw map for photonSampler_t
+            if ((rc = dmalloc_keep(&G.d_rpos, nrec)) || (rc = dmalloc_keep(&G.d_rdir, nrec)) ||
+                (rc = dmalloc_keep(&G.d_rcol, nrec)) || (rc = dmalloc_keep(&G.d_rraw, 5 * nrec)))
+                return rc;
+            hipLaunchKernelGGL(k_gp_raw, dim3(grid_for(nrec, 256)), dim3(256), 0, st, (uint32_t)nrec, recs, tab,
+                               G.d_rpos, G.d_rdir, G.d_rcol, G.d_rraw);
+            YG_HIP(hipGetLastError());
+            std::vector<f3> rp(nrec);
+            YG_HIP(hipMemcpyAsync(rp.data(), G.d_rpos, sizeof(f3) * nrec, hipMemcpyDeviceToHost, st));
+            YG_HIP(hipStreamSynchronize(st));
+            std::vector<int32_t> rgs, rgi;
+            host_grid(rp, G.radius, G.rgmin, G.rgdim, rgs, rgi);
+            if ((rc = dev_upload(&G.d_rgstart, rgs.data(), rgs.size())) ||
+                (rc = dev_upload(&G.d_rgitems, rgi.data(), rgi.size())))
+                return rc;
+            G.nrec = (int64_t)nrec;
             auto bits = [](long span) {
                 int b = 0;
                 while (b < 62 && (1L << b) <= span) b++;
@@ -857,6 +934,22 @@ extern "C" int yg_gphoton_read(yg_scene* s, float* d_out, uint64_t capacity, uin
     if (d_out && n) {
         const uint64_t m = std::min(n, capacity);
         YG_HIP(hipMemcpyAsync(d_out, s->gp.d_read, sizeof(float) * 15 * m, hipMemcpyDeviceToDevice,
+                              (hipStream_t)stream));
+        YG_HIP(hipStreamSynchronize((hipStream_t)stream));
+    }
+    return YG_OK;
+}
+
+extern "C" int yg_gphoton_read_photons(yg_scene* s, uint32_t* d_out, uint64_t capacity, uint64_t* count,
+                                       void* stream)
+{
+    if (!s) return yg_fail(YG_ERR_ARG, "null scene");
+    if (s->gp.light < 0) return yg_fail(YG_ERR_STATE, "global photon map not built (yg_gphoton_build)");
+    const uint64_t n = (uint64_t)s->gp.nrec;
+    if (count) *count = n;
+    if (d_out && n) {
+        const uint64_t m = std::min(n, capacity);
+        YG_HIP(hipMemcpyAsync(d_out, s->gp.d_rraw, sizeof(uint32_t) * 5 * m, hipMemcpyDeviceToDevice,
                               (hipStream_t)stream));
         YG_HIP(hipStreamSynchronize((hipStream_t)stream));
     }

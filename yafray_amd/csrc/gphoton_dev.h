@@ -22,6 +22,16 @@ struct GPhotonView {
     int bmin[3], bmax[3];           // box index bounds
     int sy, sz;                     // key shifts
     int64_t n;
+    // the raw photon map (photonMap, globalphotonlight.cc:38) in store order,
+    // decoded, with a uniform grid (cell = radius): photonSampler_t's gathers
+    const f3* rpos;
+    const f3* rdir;
+    const c3* rcol;
+    const int32_t* rgstart;
+    const int32_t* rgitems;
+    float rgmin[3];
+    int rgdim[3];
+    int64_t nrec;
 };
 
 // hash3d_t::getBox (hash3d.h:218-228)

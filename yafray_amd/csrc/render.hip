@@ -502,7 +502,7 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WP
     const uint32_t* __restrict__ active, const uint32_t* __restrict__ nact, uint32_t nactive_cap, int S, int D,
     const int32_t* __restrict__ item_ti, const float* __restrict__ item_t, const uint32_t* __restrict__ qoff,
     const uint32_t* __restrict__ hidx, const f3* __restrict__ st_where, const f3* __restrict__ st_hn,
-    const int32_t* __restrict__ st_ti, const c3* __restrict__ st_col, float* __restrict__ st_s,
+    const int32_t* __restrict__ st_ti, c3* __restrict__ st_col, float* __restrict__ st_s,
     c3* __restrict__ contri, uint8_t* __restrict__ nseg, f3* __restrict__ h_p, f3* __restrict__ h_ray,
     int32_t* __restrict__ h_ti, uint32_t* __restrict__ qcount, uint32_t* __restrict__ queue,
     uint32_t* __restrict__ flag, const uint32_t* __restrict__ hbase, Counters* __restrict__ cnt,
@@ -557,9 +557,34 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WP
             raycolor = st_col[p];
             k = hidx[(size_t)j * nactive_cap + p];
         }
-        const float z1 = *(const YG_GLOBAL float*)(R.halton + (size_t)(2 * j) * R.hstride + k);
-        const float z2 = *(const YG_GLOBAL float*)(R.halton + (size_t)(2 * j + 1) * R.hstride + k);
-        const f3 ray = hemi_direction(z1, z2, N, Ru, Rv);
+        f3 ray;
+        if (R.ps_on) {
+            // photonSampler_t::nextDirection (pathtools.cc:238-261): the area's
+            // own Halton values; level 0 inside the walk's sector (the item's
+            // plan), the ray weighed by weight x 2 z1
+            const size_t hb = (size_t)(ai - (int)R.area_base) * R.ps_nd;
+            if (j == 0) {
+                const int nsec = R.ps_par * R.ps_mer;
+                const int32_t* hits = R.ps_hits + (size_t)item * nsec;
+                int sec = 0, acc = hits[0];
+                while (acc <= smp && sec + 1 < nsec) acc += hits[++sec];
+                const int c0 = sec / R.ps_mer, c1 = sec % R.ps_mer;
+                const float z1 = ((float)c0 + R.ps_h[hb * R.ps_hstride + k]) * R.ps_pdiv;
+                const float z2 = ((float)c1 + R.ps_h[(hb + 1) * R.ps_hstride + k]) * R.ps_mdiv;
+                const float wv = R.ps_w[(size_t)item * nsec + sec] * 2.0f * z1;
+                raycolor = mkc(wv, wv, wv);
+                if (!ADJ) st_col[p] = raycolor;
+                ray = ps_direction(z1, z2, N, Ru, Rv);
+            } else {
+                const float z1 = R.ps_h[(hb + 2 * j) * R.ps_hstride + k];
+                const float z2 = (float)((double)R.ps_h[(hb + 2 * j + 1) * R.ps_hstride + k] * 2.0 * M_PI);
+                ray = ps_direction(z1, z2, N, Ru, Rv);
+            }
+        } else {
+            const float z1 = *(const YG_GLOBAL float*)(R.halton + (size_t)(2 * j) * R.hstride + k);
+            const float z2 = *(const YG_GLOBAL float*)(R.halton + (size_t)(2 * j + 1) * R.hstride + k);
+            ray = hemi_direction(z1, z2, N, Ru, Rv);
+        }
         float s;
         if (j == 0) {
             s = dot(ray, N);
@@ -695,7 +720,7 @@ __global__ __launch_bounds__(SB) void k_shade(const RenderScene* __restrict__ sc
         // scene_t::light with indirect = true (sun lights only inside GI paths)
         const uint32_t mask = vis[p];
         const f3 eye = (sp.P + (-ray)) - sp.P;
-        const c3 raycolor = (j == 0) ? mkc(1, 1, 1) : st_col[p];
+        const c3 raycolor = (j == 0 && !R.ps_on) ? mkc(1, 1, 1) : st_col[p];
         const float s = st_s[p];
         c3 rho;
         if (!ADJ) {
@@ -947,6 +972,270 @@ __global__ void k_area_draws(const Area* __restrict__ areas, int nareas, uint32_
 
 // Level-0 active list (every sample of every qualifying item) and its
 // length nact[0]; continuation flags and per-level queue lengths start at 0.
+// ------------------------------------------------ photonSampler_t --
+//
+// The sampler of a render area is made at its first takeSample
+// (getSampler, pathlight.cc:725-738); its constructor starts every Halton
+// dimension at an ourRandomI() draw (pathtools.cc:80-92), so the areas that
+// make one draw 2 (depth + 1) values each, in area order.
+
+// per area: does a pixel reach takeSample (one wave per area)
+__global__ __launch_bounds__(64) void k_ps_flag(const RenderScene* __restrict__ scenes,
+                                                const Area* __restrict__ areas, int nareas,
+                                                const uint32_t* __restrict__ qual)
+{
+    const int a = blockIdx.x;
+    if (a >= nareas) return;
+    const Area A = areas[a];
+    const uint32_t n = (uint32_t)(A.w * A.h);
+    int any = 0;
+    for (uint32_t k = threadIdx.x; k < n; k += 64) any |= qual[A.item_start + k] ? 1 : 0;
+    any = __any(any);
+    if (threadIdx.x == 0) scenes[A.scene].ps_flag[a - (int)scenes[A.scene].area_base] = any ? 1u : 0u;
+}
+
+// nextPrime (mcqmc.h:84-97)
+__device__ __forceinline__ int next_prime_dev(int last)
+{
+    int np = last + (last & 1) + 1;
+    for (;;) {
+        int dv = 3;
+        bool ispr = true;
+        while (ispr && dv * dv <= np) {
+            ispr = (np % dv) != 0;
+            dv += 2;
+        }
+        if (ispr) return np;
+        np += 2;
+    }
+}
+
+// one thread per (area, dimension): Halton::setStart of its ourRandomI()
+// draw (mcqmc.h:24-33), then getNext (mcqmc.h:35-49) for every draw of the
+// area's qualifying samples
+__global__ void k_ps_halton(const RenderScene* __restrict__ scenes, const Area* __restrict__ areas, int nareas,
+                            const uint32_t* __restrict__ qual, const uint32_t* __restrict__ qoff, int S)
+{
+    const RenderScene& R = scenes[0];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = t / R.ps_nd, d = t % R.ps_nd;
+    if (a >= nareas || !R.ps_flag[a]) return;
+    uint32_t rank = 0;
+    for (int b = 0; b < a; b++) rank += R.ps_flag[b];
+    uint32_t i = (uint32_t)lcg_jump(R.lcg_seed, (uint64_t)rank * (uint64_t)R.ps_nd + (uint64_t)d + 1, YG_LCG_POW);
+    int base = 2;
+    for (int k = 0; k < d; k++) base = next_prime_dev(base);
+    const double inv = 1.0 / (double)base;
+    double value = 0.0, f = inv, factor = inv;
+    while (i > 0) {
+        value += (double)(i % (uint32_t)base) * factor;
+        i /= (uint32_t)base;
+        factor *= f;
+    }
+    const Area A = areas[a];
+    const uint32_t last = A.item_start + (uint32_t)(A.w * A.h) - 1u;
+    uint32_t cnt = (qoff[last] + qual[last] - qoff[A.item_start]) * (uint32_t)S;
+    if (cnt > R.ps_hstride) cnt = R.ps_hstride;
+    float* out = R.ps_h + ((size_t)a * R.ps_nd + d) * R.ps_hstride;
+    for (uint32_t k = 0; k < cnt; k++) {
+        const double r = 1.0 - value - 0.0000000001;
+        if (inv < r) value += inv;
+        else {
+            double hh, h = inv;
+            do {
+                hh = h;
+                h *= inv;
+            } while (h >= r);
+            value += hh + h - 1.0;
+        }
+        out[k] = (float)value;
+    }
+}
+
+// photonSampler_t::getCoords (pathtools.cc:118-145); a NaN sine (|z| above 1
+// by rounding, undefined in the reference's int cast) is taken as 0
+__device__ __forceinline__ int ps_sector(const RenderScene& R, f3 v, f3 N, f3 Ru, f3 Rv)
+{
+    float x = dot(v, Ru), y = dot(v, Rv), z = dot(v, N);
+    float fix = (float)sqrt(1.0 - (double)(z * z));
+    if (fix != fix) fix = 0.0f;
+    int first = (int)(fix / R.ps_pdiv);
+    if (first >= R.ps_par) first--;
+    if (fix > 1.0f) fix = 1.0f;
+    if (fix != 0.0f) {
+        x /= fix;
+        y /= fix;
+    }
+    if (x > 1.0f) x = 1.0f;
+    if (x < -1.0f) x = -1.0f;
+    float an = (float)acos((double)x);
+    if (y < 0) an = (float)(2.0 * M_PI - (double)an);
+    int second = (int)(an / R.ps_mdiv);
+    if (second >= R.ps_mer) second--;
+    return first * R.ps_mer + second;
+}
+
+__device__ __forceinline__ float max_abs_diff(c3 a, c3 b)
+{
+    return fmaxf(fabsf(a.r - b.r), fmaxf(fabsf(a.g - b.g), fabsf(a.b - b.b)));
+}
+
+// photonSampler_t::samplingFrom (pathtools.cc:172-236) at every qualifying
+// item, one wave per item: the lanes gather the 3 x sectors nearest raw
+// photons within the map's radius that face N (sorted by distance, then
+// store index, with a bitonic network in LDS, cut whenever the buffer
+// fills); lane 0 sums their colours per sector, spreads the samples by the
+// sectors' energy differences (giveMaxDiff) and writes the plan
+#define PS_BUF 1024
+__global__ __launch_bounds__(64) void k_ps_plan(const RenderScene* __restrict__ scenes,
+                                                const Area* __restrict__ areas, int nareas,
+                                                const int32_t* __restrict__ item_ti,
+                                                const float* __restrict__ item_t,
+                                                const uint32_t* __restrict__ qual)
+{
+    __shared__ float s_d[PS_BUF];
+    __shared__ int s_i[PS_BUF];
+    __shared__ c3 s_e[64];
+    __shared__ float s_w[64];
+    __shared__ int s_h[64];
+    const uint32_t item = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (!qual[item]) return;
+    const Area a = areas[find_area(areas, nareas, item)];
+    const RenderScene& R = scenes[a.scene];
+    f3 eye;
+    const Surf sp = primary_surf(R, a, item, item_ti[item], item_t[item], eye);
+    const f3 P = sp.P, N = face_forward(sp.Ng, sp.N, eye), Ru = sp.NU, Rv = sp.NV;
+    const GPhotonView& G = R.gp;
+    const int nsec = R.ps_par * R.ps_mer, K = 3 * nsec;
+    const float radius = G.radius;
+    auto sort_buf = [&](int n) {
+        int n2 = 1;
+        while (n2 < n) n2 <<= 1;
+        for (int k = n + lane; k < n2; k += 64) {
+            s_d[k] = INFINITY;
+            s_i[k] = 0x7fffffff;
+        }
+        __syncthreads();
+        for (int size = 2; size <= n2; size <<= 1)
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int k = lane; k < n2; k += 64) {
+                    const int o = k ^ stride;
+                    if (o > k) {
+                        const bool up = (k & size) == 0;
+                        const bool gt = (s_d[o] < s_d[k]) || (s_d[o] == s_d[k] && s_i[o] < s_i[k]);
+                        if (gt == up) {
+                            const float td = s_d[k]; s_d[k] = s_d[o]; s_d[o] = td;
+                            const int ti = s_i[k]; s_i[k] = s_i[o]; s_i[o] = ti;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+    };
+    int n = 0;
+    if (G.nrec > 0) {
+        const float pp[3] = {P.x, P.y, P.z};
+        int g0[3], g1[3];
+        for (int q = 0; q < 3; q++) {
+            long lo = (long)floorf((pp[q] - radius - G.rgmin[q]) / radius);
+            long hh = (long)floorf((pp[q] + radius - G.rgmin[q]) / radius);
+            if (lo < 0) lo = 0;
+            if (hh >= G.rgdim[q]) hh = G.rgdim[q] - 1;
+            g0[q] = (int)lo;
+            g1[q] = (int)hh;
+        }
+        for (int z = g0[2]; z <= g1[2]; z++)
+            for (int y = g0[1]; y <= g1[1]; y++)
+                for (int x = g0[0]; x <= g1[0]; x++) {
+                    const int c = (z * G.rgdim[1] + y) * G.rgdim[0] + x;
+                    const int q0 = G.rgstart[c], q1 = G.rgstart[c + 1];
+                    for (int q00 = q0; q00 < q1; q00 += 64) {
+                        const int q = q00 + lane;
+                        bool keep = false;
+                        float dis = 0.f;
+                        int idx = 0;
+                        if (q < q1) {
+                            idx = G.rgitems[q];
+                            dis = len3(G.rpos[idx] - P);
+                            keep = !(dis > radius) && dot(G.rdir[idx], N) > 0.0f;
+                        }
+                        const uint64_t m = __ballot(keep);
+                        if (!m) continue;
+                        if (n + 64 > PS_BUF) {
+                            sort_buf(n);
+                            if (n > K) n = K;
+                        }
+                        if (keep) s_d[n + (int)__popcll(m & ((1ull << lane) - 1ull))] = dis;
+                        if (keep) s_i[n + (int)__popcll(m & ((1ull << lane) - 1ull))] = idx;
+                        n += (int)__popcll(m);
+                        __syncthreads();
+                    }
+                }
+        if (n > 0) sort_buf(n);
+        if (n > K) n = K;
+    }
+    if (lane != 0) return;
+    const int par = R.ps_par, mer = R.ps_mer;
+    for (int k = 0; k < nsec; k++) s_e[k] = mkc(0, 0, 0);
+    for (int k = 0; k < n; k++) {
+        const int idx = s_i[k];
+        const f3 d = G.rdir[idx];
+        if (dot(d, N) > 0.0f) {
+            const int sec = ps_sector(R, d, N, Ru, Rv);
+            s_e[sec] = cadd(s_e[sec], G.rcol[idx]);
+        }
+    }
+    float totaldiff = 0;
+    for (int i = 0; i < par; ++i)
+        for (int j = 0; j < mer; ++j) {
+            const c3 e = s_e[i * mer + j];
+            float mx = 0, t;
+            const int il = i - 1, ih = i + 1, jl = j - 1, jh = j + 1;
+            if (il >= 0) {
+                if (jl >= 0) { t = max_abs_diff(s_e[il * mer + jl], e); if (t > mx) mx = t; }
+                t = max_abs_diff(s_e[il * mer + j], e); if (t > mx) mx = t;
+                if (jh < mer) { t = max_abs_diff(s_e[il * mer + jh], e); if (t > mx) mx = t; }
+            }
+            if (jl >= 0) { t = max_abs_diff(s_e[i * mer + jl], e); if (t > mx) mx = t; }
+            if (jh < mer) { t = max_abs_diff(s_e[i * mer + jh], e); if (t > mx) mx = t; }
+            if (ih < par) {
+                if (jl >= 0) { t = max_abs_diff(s_e[ih * mer + jl], e); if (t > mx) mx = t; }
+                t = max_abs_diff(s_e[ih * mer + j], e); if (t > mx) mx = t;
+                if (jh < mer) { t = max_abs_diff(s_e[ih * mer + jh], e); if (t > mx) mx = t; }
+            }
+            s_w[i * mer + j] = mx;
+            totaldiff += mx;
+        }
+    int assign = R.samples - nsec;
+    const float sph = (totaldiff != 0.0f) ? (float)assign / totaldiff : 0;
+    for (int k = 0; k < nsec; k++) {
+        int h = (int)((double)(sph * s_w[k]) + 0.5);
+        if (h > assign) h = assign;
+        if (h < 0) h = 0;  // only for a non-finite energy (the reference's cast is undefined there)
+        assign -= h;
+        s_h[k] = h + 1;
+    }
+    bool noway = false;
+    while (assign > 0) {
+        for (int k = 0; k < nsec && assign; k++)
+            if ((s_h[k] > 1) || noway) {
+                s_h[k]++;
+                assign--;
+            }
+        noway = true;
+    }
+    int mx = 1;
+    for (int k = 0; k < nsec; k++)
+        if (s_h[k] > mx) mx = s_h[k];
+    const float maxf = (float)mx;
+    for (int k = 0; k < nsec; k++) {
+        R.ps_hits[(size_t)item * nsec + k] = s_h[k];
+        R.ps_w[(size_t)item * nsec + k] = maxf / (float)s_h[k];
+    }
+    R.ps_multi[item] = (float)(1.0 / (double)(maxf * (float)nsec));
+}
+
 __global__ void k_path_init(uint32_t nitems, int S, int D, const uint32_t* __restrict__ qual,
                             const uint32_t* __restrict__ qoff, uint32_t* __restrict__ active,
                             uint32_t* __restrict__ flag, uint32_t* __restrict__ nact,
@@ -1095,7 +1384,7 @@ __global__ __launch_bounds__(RB) void k_finalize(const RenderScene* __restrict__
                         sub[smp % 4] = cadd(sub[smp % 4], c);
                     }
                 }
-                const float mult = (float)(1.0 / (double)(float)S);
+                const float mult = R.ps_on ? R.ps_multi[item] : (float)(1.0 / (double)(float)S);
                 // adaptive repetition test (pathlight.cc:349-374): only counted
                 float minR = 1000, minG = 1000, minB = 1000, maxR = 0, maxG = 0, maxB = 0;
                 for (int q = 0; q < 4; q++) {
@@ -1358,6 +1647,8 @@ struct RenderWS {
     DevBuf<unsigned long long> lc_err;
     // guessLight (area lights with psamples > 0): verdicts, test bits, rand() stream
     bool any_guess = false;
+    bool ps_on = false;  // photonSampler_t plans and Halton tables before the levels
+    int ps_nd = 0;
     int g_nw = 0;
     DevBuf<uint32_t> g_status, g_app, g_vis;
     DevBuf<int32_t> g_woff, g_rnd;
@@ -1436,6 +1727,14 @@ struct ScenePlan {
     int search = 0;
 };
 
+// photonSampler_t buffers of the frame being rendered (one scene at a time)
+static struct {
+    DevBuf<float> h;
+    DevBuf<uint32_t> flag;
+    DevBuf<int32_t> hits;
+    DevBuf<float> w, multi;
+} g_ps;
+
 static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, ScenePlan& P)
 {
     if (!s) return yg_fail(YG_ERR_ARG, "null scene");
@@ -1471,6 +1770,12 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
     R.path_light = -1;
     R.photon_light = -1;
     R.gp_on = 0;
+    R.ps_on = 0;
+    R.ps_h = nullptr;
+    R.ps_flag = nullptr;
+    R.ps_hits = nullptr;
+    R.ps_w = nullptr;
+    R.ps_multi = nullptr;
     R.nl = (int)s->lights.size();
     for (int li = 0; li < R.nl; li++) {
         const yg_light& Lg = s->lights[li];
@@ -1554,9 +1859,17 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
                 return yg_fail(YG_ERR_UNSUPPORTED,
                                "area lights inside GI paths together with a global photon map are not on the GPU path");
         // with the global map published, a pathlight of more than 96 samples
-        // samples by photonSampler_t (pathlight.cc:731), which is not implemented
-        if (R.path_light >= 0 && R.samples > 96)
-            return yg_fail(YG_ERR_UNSUPPORTED, "pathlight samples > 96 with a global photon map (photonSampler_t)");
+        // samples by photonSampler_t (pathlight.cc:731): its Halton starts are
+        // ourRandomI() draws in area order, so no other light may draw
+        if (R.path_light >= 0 && R.samples > 96) {
+            if (o->aa_passes > 0)
+                return yg_fail(YG_ERR_UNSUPPORTED, "photonSampler_t (pathlight samples > 96) with antialiasing passes");
+            for (const auto& Lg : s->lights)
+                if (Lg.type == YG_LIGHT_AREA && !Lg.dummy && Lg.use_in_render)
+                    return yg_fail(YG_ERR_UNSUPPORTED,
+                                   "photonSampler_t (pathlight samples > 96) with area lights drawing from ourRandom()");
+            R.ps_on = 1;
+        }
     }
     if (P.cache) {
         if (o->aa_passes > 0)
@@ -1628,6 +1941,30 @@ static int plan_scene(yg_scene* s, const yg_render_opts* o, float* d_rgb, SceneP
             s->halton_stride = R.hstride;
         }
         R.halton = s->d_halton;
+        if (R.ps_on) {
+            // photonSampler_t ctor, grid 36 (pathtools.cc:100-106)
+            const float base = (float)sqrt((double)(float)36 / 2.0);
+            R.ps_par = (int)(base + 0.5);
+            R.ps_mer = 2 * R.ps_par;
+            R.ps_nd = ndim;
+            R.ps_pdiv = (float)(1.0 / (double)(float)R.ps_par);
+            R.ps_mdiv = (float)(2.0 * M_PI / (double)(float)R.ps_mer);
+            const int nsec = R.ps_par * R.ps_mer;
+            if (R.samples < nsec) return yg_fail(YG_ERR_UNSUPPORTED, "photonSampler_t needs samples >= sectors");
+            const uint64_t hs = (uint64_t)P.max_area_items * (uint64_t)R.samples;
+            if (hs > UINT32_MAX) return yg_fail(YG_ERR_UNSUPPORTED, "too many sampler draws per render area");
+            R.ps_hstride = (uint32_t)hs;
+            int rc;
+            if ((rc = g_ps.flag.ensure(P.areas.size())) || (rc = g_ps.h.ensure(P.areas.size() * (size_t)ndim * hs)) ||
+                (rc = g_ps.hits.ensure((size_t)P.nitems * nsec)) || (rc = g_ps.w.ensure((size_t)P.nitems * nsec)) ||
+                (rc = g_ps.multi.ensure(P.nitems)))
+                return rc;
+            R.ps_h = g_ps.h.p;
+            R.ps_flag = g_ps.flag.p;
+            R.ps_hits = g_ps.hits.p;
+            R.ps_w = g_ps.w.p;
+            R.ps_multi = g_ps.multi.p;
+        }
     }
     if (R.photon_light >= 0) {
         R.pm.photons = s->pm.d_photons;
@@ -1663,6 +2000,16 @@ static int trace_levels(hipStream_t st, bool ktiming, std::vector<hipEvent_t>& k
     const int S = W.S, D = W.D, na = W.na;
     const uint32_t npaths = W.npaths, nitems = W.nitems;
     int rc;
+    if (W.ps_on) {
+        hipLaunchKernelGGL(k_ps_flag, dim3(na), dim3(64), 0, st, W.scenes.p, W.areas.p, na, W.qual.p);
+        YG_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_ps_halton, dim3(grid_for((size_t)na * W.ps_nd, 64)), dim3(64), 0, st, W.scenes.p,
+                           W.areas.p, na, W.qual.p, W.qoff.p, S);
+        YG_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_ps_plan, dim3(nitems), dim3(64), 0, st, W.scenes.p, W.areas.p, na, W.item_ti.p,
+                           W.item_t.p, W.qual.p);
+        YG_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_path_init, dim3(grid_for(nitems, 256)), dim3(256), 0, st, nitems, S, D, W.qual.p,
                        W.qoff.p, W.active0.p, W.flag.p, W.nact.p, W.qcnt.p, W.dense.p);
     YG_HIP(hipGetLastError());
@@ -2426,6 +2773,8 @@ static int render_plans(std::vector<ScenePlan>& plans, hipStream_t st, yg_render
     // shuffle's draws (one per area), psamples draws per light at most per item
     W.any_guess = false;
     for (auto& P : plans) W.any_guess = W.any_guess || P.guess;
+    W.ps_on = plans.size() == 1 && plans[0].R.ps_on;
+    W.ps_nd = plans[0].R.ps_nd;
     if (W.any_guess) {
         if (plans.size() > 1)
             return yg_fail(YG_ERR_UNSUPPORTED, "area light psamples > 0 in a batched render is not on the GPU path");
@@ -2719,6 +3068,8 @@ extern "C" int yg_render_batch(yg_scene** scenes, uint32_t n, const yg_render_op
     for (uint32_t k = 0; k < n; k++) {
         int rc = plan_scene(scenes[k], &opts[k], d_rgb[k], plans[k]);
         if (rc) return rc;
+        if (plans[k].R.ps_on && n > 1)
+            return yg_fail(YG_ERR_UNSUPPORTED, "photonSampler_t (pathlight samples > 96) in a batch");
     }
     bool timing = false, ktiming = false, graph = true;
     for (uint32_t k = 0; k < n; k++) {
@@ -3008,6 +3359,7 @@ extern "C" int yg_render_adjoint(yg_scene* s, const yg_render_opts* opts, const 
     if (rc) return rc;
     if (plans[0].R.depth > ADJ_MAXD) return yg_fail(YG_ERR_UNSUPPORTED, "adjoint supports depth <= 8");
     if (plans[0].cache) return yg_fail(YG_ERR_UNSUPPORTED, "adjoint of a pathlight with the irradiance cache");
+    if (plans[0].R.ps_on) return yg_fail(YG_ERR_UNSUPPORTED, "adjoint with photonSampler_t (pathlight samples > 96)");
     const bool timing = opts->flags & YG_RENDER_TIMING;
     RenderWS& W = g_ws;
     const bool reuse = (opts->flags & YG_RENDER_REUSE) && W.last_uid == s->uid && W.last_gen == s->gen &&

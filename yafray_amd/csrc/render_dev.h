@@ -81,6 +81,17 @@ struct RenderScene {
     // Russian roulette (YG_RENDER_ROULETTE): first GI level it decides, 0 = off
     int rr_from;
     uint32_t rr_seed;
+    // photonSampler_t (pathtools.cc:80-266): the pathlight's sampler when the
+    // global photon map is published and samples > 96 (pathlight.cc:731)
+    int ps_on;
+    int ps_par, ps_mer, ps_nd;   // hemigrid paralels x meridians, Halton dimensions
+    float ps_pdiv, ps_mdiv;
+    float* ps_h;                 // [area][dim][ps_hstride] Halton values from the area's ourRandomI() starts
+    uint32_t ps_hstride;
+    uint32_t* ps_flag;           // [area] the area's renderState_t creates the sampler
+    int32_t* ps_hits;            // [item][sector] samples of each sector (samplingFrom)
+    float* ps_w;                 // [item][sector] weight = max hits / hits
+    float* ps_multi;             // [item] multiplier() = 1 / (max hits x sectors)
 };
 
 // Uniform [0, 1) of a path's roulette at one level: an integer hash of
@@ -399,6 +410,15 @@ __device__ __forceinline__ f3 hemi_direction(float z1, float z2, f3 N, f3 Ru, f3
     z2 = (float)((double)z2 * (2.0 * M_PI));
     const f3 a = Ru * dcos(z2) + Rv * dsin(z2);
     return a * (float)sqrt(1.0 - (double)z1) + N * sqrtf(z1);
+}
+
+// photonSampler_t::nextDirection's vector (pathtools.cc:257-260): z1 is the
+// sine of the angle to N
+__device__ __forceinline__ f3 ps_direction(float z1, float z2, f3 N, f3 Ru, f3 Rv)
+{
+    if (z1 > 1.0f) z1 = 1.0f;
+    const f3 a = Ru * dcos(z2) + Rv * dsin(z2);
+    return a * z1 + N * (float)sqrt(1.0 - (double)(z1 * z1));
 }
 
 // scene_t::light with indirect = true restricted to sun lights (the only

@@ -156,6 +156,17 @@ struct GPhotonDev {
     int32_t* d_gstart = nullptr;
     int32_t* d_gitems = nullptr;
     float* d_read = nullptr;        // 15 floats per box (yg_gphoton_read)
+    // raw photons in store order: decoded position / direction / colour,
+    // the 5-word records of yg_gphoton_read_photons, grid (cell = radius)
+    int64_t nrec = 0;
+    f3* d_rpos = nullptr;
+    f3* d_rdir = nullptr;
+    c3* d_rcol = nullptr;
+    uint32_t* d_rraw = nullptr;
+    int32_t* d_rgstart = nullptr;
+    int32_t* d_rgitems = nullptr;
+    float rgmin[3] = {0, 0, 0};
+    int rgdim[3] = {1, 1, 1};
     GPhotonView view() const
     {
         GPhotonView v;
@@ -165,6 +176,9 @@ struct GPhotonDev {
             v.gmin[a] = gmin[a]; v.gdim[a] = gdim[a]; v.bmin[a] = bmin[a]; v.bmax[a] = bmax[a];
         }
         v.cell = cell; v.radius = radius; v.sy = sy; v.sz = sz; v.n = n;
+        v.rpos = d_rpos; v.rdir = d_rdir; v.rcol = d_rcol; v.rgstart = d_rgstart; v.rgitems = d_rgitems;
+        for (int a = 0; a < 3; a++) { v.rgmin[a] = rgmin[a]; v.rgdim[a] = rgdim[a]; }
+        v.nrec = nrec;
         return v;
     }
 };

@@ -63,6 +63,7 @@ def lib():
         "yg_photon_gather": (C.c_int, [vp, vp, vp, vp, vp, vp, u32, vp, vp]),
         "yg_gphoton_build": (C.c_int, [vp, u32, vp, vp, vp]),
         "yg_gphoton_read": (C.c_int, [vp, vp, C.c_uint64, C.POINTER(C.c_uint64), vp]),
+        "yg_gphoton_read_photons": (C.c_int, [vp, vp, C.c_uint64, C.POINTER(C.c_uint64), vp]),
         "yg_cache_read": (C.c_int, [vp, vp, C.c_uint64, C.POINTER(C.c_uint64), vp]),
     }
     sigs["yg_render_batch"] = (C.c_int, [C.POINTER(vp), u32, vp, C.POINTER(vp), vp, vp])
@@ -318,6 +319,16 @@ class Scene:
         check(lib().yg_gphoton_read(self.h, _ptr(out) if out is not None else None,
                                     out.numel() // 15 if out is not None else 0, C.byref(cnt), _stream()),
               "yg_gphoton_read")
+        return cnt.value
+
+    def gphoton_read_photons(self, out=None):
+        """The raw photon map in store order into an int32 CUDA tensor of 5
+        words per photon (position bits, direction code, RGBE); returns the
+        photon count (out=None: count only)."""
+        cnt = C.c_uint64()
+        check(lib().yg_gphoton_read_photons(self.h, _ptr(out) if out is not None else None,
+                                            out.numel() // 5 if out is not None else 0, C.byref(cnt), _stream()),
+              "yg_gphoton_read_photons")
         return cnt.value
 
     def cache_read(self):
