@@ -682,6 +682,85 @@ def bench_gphoton(args):
     return line
 
 
+def bench_psamp(args):
+    """photonSampler_t: the gphoton room (1,000,000 photons, depth 2,
+    search 100, radius 1, sun) with a 128-sample one-bounce pathlight, which
+    samples by photonSampler_t (pathlight samples > 96 with the map
+    published) at 640x480.  One step = the frame (the map is built once
+    before the timed region).  Parity leg: the same scene at 96x72 rendered
+    by the oracle from the GPU's map (boxes and raw photons); its time is
+    the CPU baseline (one thread: the reference's Halton starts are drawn in
+    area order)."""
+    import numpy as np
+    import torch
+
+    from oracle import pyoracle as po
+    from yafray_amd import lib as Y
+    from yafray_amd import scenes as S
+
+    def scene(rx, ry):
+        return S.gphoton_room(rx, ry, subdiv=5, photons=1000000, depth=2, search=100, radius=1.0, samples=128,
+                              path_depth=1, sun=(0.3, -0.2, 1.0))
+
+    d = scene(640, 480)
+    sc = Y.Scene.from_desc(d)
+    out = torch.zeros(d.cam.resx * d.cam.resy * 3, dtype=torch.float32, device="cuda")
+    sc.gphoton_build(1)
+    dt, st = _timed(lambda: sc.render(out), args.steps, args.warmup)
+    torch.cuda.synchronize()
+    img = out.cpu().numpy()
+    if not np.all(np.isfinite(img)):
+        raise RuntimeError("psamp: non-finite pixels in the GPU image")
+    rays = int(st.rays_camera + st.rays_extension + st.rays_shadow)
+    line = {
+        "metric": "photonSampler_t GI ms/frame", "value": round(dt * 1e3, 3), "unit": "ms/frame",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "room + icosphere(20480) + 2 boxes, 640x480; global photon map of 1,000,000 "
+                               "photons, depth 2, search 100, radius 1; sun; pathlight 128 spp (photonSampler_t: "
+                               "96-photon gather and 4 x 8 sector plan per pixel), 1 bounce",
+                   "triangles": int(d.n_tris)},
+        "rays_per_frame": rays, "mrays_per_s": round(rays / dt / 1e6, 3), "image_finite": True,
+    }
+    if not args.no_cpu:
+        ds = scene(96, 72)
+        scs = Y.Scene.from_desc(ds)
+        outs = torch.zeros(96 * 72 * 3, dtype=torch.float32, device="cuda")
+        scs.gphoton_build(1)
+        sst = scs.render(outs)
+        n = scs.gphoton_read()
+        boxes = torch.zeros(max(n, 1) * 15, dtype=torch.float32, device="cuda")
+        scs.gphoton_read(boxes)
+        nr = scs.gphoton_read_photons()
+        recs = torch.zeros(max(nr, 1) * 5, dtype=torch.int32, device="cuda")
+        scs.gphoton_read_photons(recs)
+        torch.cuda.synchronize()
+        gimg = outs.cpu().numpy().reshape(72, 96, 3)
+        print("psamp: oracle frame on the GPU's map (96x72)", file=sys.stderr, flush=True)
+        oc = po.OracleScene.from_desc(ds, fast=True)
+        oc.gphoton_load(ds, 1, boxes.cpu().numpy()[: n * 15].reshape(n, 15), stateless=True)
+        oc.gphoton_load_photons(recs.cpu().numpy().view(np.uint32)[: nr * 5].reshape(nr, 5))
+        t0 = time.perf_counter()
+        ref, rs = oc.render(ds)
+        cpu_s = time.perf_counter() - t0
+        oc2 = po.OracleScene.from_desc(ds)
+        oc2.gphoton_load(ds, 1, boxes.cpu().numpy()[: n * 15].reshape(n, 15), stateless=True)
+        oc2.gphoton_load_photons(recs.cpu().numpy().view(np.uint32)[: nr * 5].reshape(nr, 5))
+        ref2, rs2 = oc2.render(ds)
+        rel = float(np.sqrt(np.mean((gimg.astype(np.float64) - ref2) ** 2)) / np.sqrt(np.mean(ref2.astype(np.float64) ** 2)))
+        equal = (int(sst.rays_camera), int(sst.rays_extension), int(sst.rays_shadow)) == tuple(int(x) for x in rs2[:3])
+        if rel > 1e-4 or not equal:
+            raise RuntimeError("psamp parity failed: rel rmse %.3g, ray counts equal %s" % (rel, equal))
+        crays = int(rs[0] + rs[1] + rs[2])
+        line["rmse_vs_oracle"] = {"rel_rmse": rel, "bit_exact_pixels": float(np.mean(gimg == ref2)),
+                                  "ray_counts_equal": equal, "sample": "96x72 frame of the same scene, oracle "
+                                  "rendering from the GPU's map"}
+        line["cpu_baseline"] = {"value": round(crays / cpu_s / 1e6, 4), "unit": "Mrays/s", "cores": 1,
+                                "kind": "port", "sample": "oracle (-O3 -ffast-math build) rendering the 96x72 frame "
+                                "(%d rays in %.1f s, one thread)" % (crays, cpu_s)}
+    return line
+
+
 def _gphoton_cpu_legs(d, img, rst, sc, torch):
     """Oracle legs of the global photon workload, on the host: the frame
     rendered with the GPU's boxes loaded (equal arithmetic), the oracle's own
@@ -1152,7 +1231,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="gi",
                     choices=["gi", "batched", "adjoint", "photon", "direct", "autotune", "aa", "rays",
-                             "noise", "room_gi", "gphoton", "cache"])
+                             "noise", "room_gi", "gphoton", "cache", "psamp"])
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--batch-mode", default="both", choices=["both", "sequential", "batched"],
                     help="batched workload: time both modes (default) or one of them alone")
@@ -1171,7 +1250,7 @@ def main():
         fn = {"batched": bench_batched, "adjoint": bench_adjoint, "photon": bench_photon,
               "direct": bench_direct, "autotune": bench_autotune, "aa": bench_aa,
               "rays": bench_rays, "noise": bench_noise, "room_gi": bench_room_gi,
-              "gphoton": bench_gphoton, "cache": bench_cache}[args.workload]
+              "gphoton": bench_gphoton, "cache": bench_cache, "psamp": bench_psamp}[args.workload]
         print(json.dumps(fn(args)))
         return
 

@@ -562,22 +562,22 @@ __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(YG_BOUNCE_WP
             // photonSampler_t::nextDirection (pathtools.cc:238-261): the area's
             // own Halton values; level 0 inside the walk's sector (the item's
             // plan), the ray weighed by weight x 2 z1
-            const size_t hb = (size_t)(ai - (int)R.area_base) * R.ps_nd;
+            const float* hv = R.ps_h + ((size_t)(ai - (int)R.area_base) * R.ps_hstride + k) * R.ps_nd;
             if (j == 0) {
                 const int nsec = R.ps_par * R.ps_mer;
                 const int32_t* hits = R.ps_hits + (size_t)item * nsec;
                 int sec = 0, acc = hits[0];
                 while (acc <= smp && sec + 1 < nsec) acc += hits[++sec];
                 const int c0 = sec / R.ps_mer, c1 = sec % R.ps_mer;
-                const float z1 = ((float)c0 + R.ps_h[hb * R.ps_hstride + k]) * R.ps_pdiv;
-                const float z2 = ((float)c1 + R.ps_h[(hb + 1) * R.ps_hstride + k]) * R.ps_mdiv;
+                const float z1 = ((float)c0 + hv[0]) * R.ps_pdiv;
+                const float z2 = ((float)c1 + hv[1]) * R.ps_mdiv;
                 const float wv = R.ps_w[(size_t)item * nsec + sec] * 2.0f * z1;
                 raycolor = mkc(wv, wv, wv);
                 if (!ADJ) st_col[p] = raycolor;
                 ray = ps_direction(z1, z2, N, Ru, Rv);
             } else {
-                const float z1 = R.ps_h[(hb + 2 * j) * R.ps_hstride + k];
-                const float z2 = (float)((double)R.ps_h[(hb + 2 * j + 1) * R.ps_hstride + k] * 2.0 * M_PI);
+                const float z1 = hv[2 * j];
+                const float z2 = (float)((double)hv[2 * j + 1] * 2.0 * M_PI);
                 ray = ps_direction(z1, z2, N, Ru, Rv);
             }
         } else {
@@ -1036,7 +1036,8 @@ __global__ void k_ps_halton(const RenderScene* __restrict__ scenes, const Area* 
     const uint32_t last = A.item_start + (uint32_t)(A.w * A.h) - 1u;
     uint32_t cnt = (qoff[last] + qual[last] - qoff[A.item_start]) * (uint32_t)S;
     if (cnt > R.ps_hstride) cnt = R.ps_hstride;
-    float* out = R.ps_h + ((size_t)a * R.ps_nd + d) * R.ps_hstride;
+    // [area][draw][dimension]: a wave's stores fill a few lines
+    float* out = R.ps_h + (size_t)a * R.ps_hstride * R.ps_nd + d;
     for (uint32_t k = 0; k < cnt; k++) {
         const double r = 1.0 - value - 0.0000000001;
         if (inv < r) value += inv;
@@ -1048,7 +1049,7 @@ __global__ void k_ps_halton(const RenderScene* __restrict__ scenes, const Area* 
             } while (h >= r);
             value += hh + h - 1.0;
         }
-        out[k] = (float)value;
+        out[(size_t)k * R.ps_nd] = (float)value;
     }
 }
 
@@ -1134,6 +1135,10 @@ __global__ __launch_bounds__(64) void k_ps_plan(const RenderScene* __restrict__ 
             }
     };
     int n = 0;
+    // after the first cut only candidates below the K-th kept (distance, then
+    // index) can still be among the K nearest
+    float thr_d = INFINITY;
+    int thr_i = 0x7fffffff;
     if (G.nrec > 0) {
         const float pp[3] = {P.x, P.y, P.z};
         int g0[3], g1[3];
@@ -1158,13 +1163,18 @@ __global__ __launch_bounds__(64) void k_ps_plan(const RenderScene* __restrict__ 
                         if (q < q1) {
                             idx = G.rgitems[q];
                             dis = len3(G.rpos[idx] - P);
-                            keep = !(dis > radius) && dot(G.rdir[idx], N) > 0.0f;
+                            keep = !(dis > radius) && ((dis < thr_d) || (dis == thr_d && idx < thr_i)) &&
+                                   dot(G.rdir[idx], N) > 0.0f;
                         }
                         const uint64_t m = __ballot(keep);
                         if (!m) continue;
-                        if (n + 64 > PS_BUF) {
+                        if (n + 64 > PS_BUF || (n >= 2 * K && thr_i == 0x7fffffff)) {
                             sort_buf(n);
                             if (n > K) n = K;
+                            if (n == K) {
+                                thr_d = s_d[K - 1];
+                                thr_i = s_i[K - 1];
+                            }
                         }
                         if (keep) s_d[n + (int)__popcll(m & ((1ull << lane) - 1ull))] = dis;
                         if (keep) s_i[n + (int)__popcll(m & ((1ull << lane) - 1ull))] = idx;

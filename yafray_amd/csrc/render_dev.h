@@ -86,7 +86,7 @@ struct RenderScene {
     int ps_on;
     int ps_par, ps_mer, ps_nd;   // hemigrid paralels x meridians, Halton dimensions
     float ps_pdiv, ps_mdiv;
-    float* ps_h;                 // [area][dim][ps_hstride] Halton values from the area's ourRandomI() starts
+    float* ps_h;                 // [area][ps_hstride][dim] Halton values from the area's ourRandomI() starts
     uint32_t ps_hstride;
     uint32_t* ps_flag;           // [area] the area's renderState_t creates the sampler
     int32_t* ps_hits;            // [item][sector] samples of each sector (samplingFrom)
