@@ -152,6 +152,11 @@ int lightFactory(const std::string& type, const paramMap_t& p, yg_light& L)
             if (ignorms) return fail(YG_ERR_UNSUPPORTED, "pathlight: ignore_bumpnormals is not on the GPU path");
             (void)grad;  // read and unused by the reference (pathlight.cc:790)
         }
+        // photonSampler_t's hemigrid (pathlight.cc:774, pathtools.cc:100-106):
+        // the device plans 4 x 8 sectors, the grid of the default 36
+        int grid = 36;
+        p.getParam("grid", grid);
+        if (grid != 36) return fail(YG_ERR_UNSUPPORTED, "pathlight: grid other than 36 is not on the GPU path");
         if (direct) return fail(YG_ERR_UNSUPPORTED, "pathlight: direct=on is not on the GPU path");
         if (mode == "occlusion") return fail(YG_ERR_UNSUPPORTED, "pathlight: occlusion mode is not on the GPU path");
         if (!qmc) return fail(YG_ERR_UNSUPPORTED, "pathlight: use_QMC=off (LCG sampler) is not on the GPU path");
